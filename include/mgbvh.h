@@ -1,0 +1,182 @@
+/*
+ * mgbvh -- MI355X-native SAH BVH builder with the node format and builder API
+ * of marigold's `bvh` crate.  Plain C ABI: plain pointers and sizes, integer
+ * status codes, no exceptions or aborts across the boundary.
+ *
+ * Reference interfaces replaced (marigold, file:line):
+ *   mgbvh_build_tris    BoundingVolumeHierarchy::<1,1>::new(&mut [MeshTriangle],
+ *                       &[MeshVertex], BvhSettings)          bvh/src/lib.rs:698-878
+ *                       (T = MeshTriangle, S = MeshVertex,   mesh_interface/src/lib.rs:19-30;
+ *                       caller marigold/src/app/data/scene/model.rs:93)
+ *   mgbvh_build_aabbs   BoundingVolumeHierarchy::<1,1>::new(&mut [T], &[()], ..)
+ *                       for T: AsBoundingVolume, e.g. the TLAS over UploadedMesh
+ *                       (mesh_interface/src/lib.rs:87-91; caller
+ *                       marigold/src/app/data/scene/geometry/mod.rs:196)
+ *   mgbvh_node32        AsGpuBytes for BvhNode, MAX_LEAF_OBJECTS == 1 branch
+ *                       bvh/src/lib.rs:157-169 == scene.slang:261-266 (32 bytes)
+ *   mgbvh_settings      BvhSettings                          bvh/src/lib.rs:668-674
+ *   mgbvh_stats         BvhProfilingInfo                     bvh/src/lib.rs:676-689
+ *   *_device variants   the same builds over device-resident buffers (no host
+ *                       copies inside the timed region)
+ */
+#ifndef MGBVH_H
+#define MGBVH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGBVH_ABI_VERSION 1
+
+/* status codes (0 = ok) */
+#define MGBVH_OK 0
+#define MGBVH_ERR_INVALID_ARG -1   /* null pointer, bad stride, capacity too small */
+#define MGBVH_ERR_UNSUPPORTED -2   /* e.g. MIN/MAX leaf objects other than 1/1 */
+#define MGBVH_ERR_NONFINITE -3     /* an input coordinate is NaN/inf or |x| >= 2^60 */
+#define MGBVH_ERR_INDEX -4         /* triangle index >= vertex count */
+#define MGBVH_ERR_DEVICE -5        /* HIP runtime error (message in mgbvh_last_error) */
+#define MGBVH_ERR_NO_DEVICE -6     /* no usable gfx950 device */
+#define MGBVH_ERR_INTERNAL -7      /* builder invariant violated */
+
+/* 32-byte GPU node, bvh/src/lib.rs:157-169 (MAX_LEAF_OBJECTS == 1):
+ * a node is a leaf iff child_node == 0; a leaf's start_index indexes the
+ * permuted object list; internal node children are child_node (gt) and
+ * child_node + 1 (lt). */
+typedef struct mgbvh_node32 {
+  float min[3];
+  uint32_t start_index;
+  float max[3];
+  uint32_t child_node;
+} mgbvh_node32;
+
+/* An object's axis-aligned box (BoundingVolume, bvh/src/lib.rs:41-46). */
+typedef struct mgbvh_aabb {
+  float min[3];
+  float max[3];
+} mgbvh_aabb;
+
+/* Split decision of one internal node (indexed by node index):
+ * axis_method = axis | method << 8, method 0 = binned (lib.rs:331-389),
+ * 1 = adaptive threshold (lib.rs:391-442), 2 = median (lib.rs:444-491);
+ * value = bin cut i (binned) | threshold bits (adaptive) | m (median);
+ * cost_bits = the fp32 SAH cost bits of the chosen split. */
+typedef struct mgbvh_split_rec {
+  uint32_t axis_method;
+  uint32_t value;
+  uint32_t count_lt;
+  uint32_t cost_bits;
+} mgbvh_split_rec;
+
+#define MGBVH_METHOD_PLAIN 0   /* the reference algorithm, bit-exact */
+#define MGBVH_METHOD_LEARNED 1 /* learned split-window proposer + exact guard */
+
+/* BvhSettings (bvh/src/lib.rs:668-674) plus the build method. */
+typedef struct mgbvh_settings {
+  const char* name;                     /* BvhSettings.name (may be NULL) */
+  int32_t has_bounds;                   /* BvhSettings.bounds is Some(..) */
+  float bounds_min[3];
+  float bounds_max[3];
+  uint32_t max_depth;                   /* BvhSettings.max_depth */
+  int32_t profiling_info;               /* BvhSettings.profiling_info */
+  const char* profiling_info_directory; /* BvhSettings.profiling_info_directory (NULL = None) */
+  uint32_t min_leaf_objects;            /* const generic MIN_LEAF_OBJECTS (must be 1) */
+  uint32_t max_leaf_objects;            /* const generic MAX_LEAF_OBJECTS (must be 1) */
+  uint32_t method;                      /* MGBVH_METHOD_* */
+  uint32_t reserved[7];
+} mgbvh_settings;
+
+/* BvhProfilingInfo (bvh/src/lib.rs:676-689) plus builder counters. */
+typedef struct mgbvh_stats {
+  double construction_time;   /* seconds, device time of the build */
+  uint32_t node_count;
+  uint32_t leaf_node_count;
+  uint32_t min_leaf_object_count;
+  uint32_t max_leaf_object_count;
+  double avg_leaf_object_count;
+  double stddev_leaf_object_count;
+  uint32_t total_object_count;
+  uint32_t height;
+  uint32_t max_depth;
+  uint32_t n_large_levels;    /* breadth-first levels run over nodes with > 32 objects */
+  uint32_t n_median;          /* median fallback splits */
+  uint32_t n_cutoff_leaves;   /* leaves forced by max_depth holding > 1 object */
+  uint32_t n_guard_fallbacks; /* learned mode: windows rejected by the exact guard */
+  uint32_t reserved[5];
+} mgbvh_stats;
+
+typedef struct mgbvh_ctx mgbvh_ctx;
+
+/* Fills *s with BvhSettings defaults of the marigold app (max_depth 32, MIN=MAX=1). */
+void mgbvh_settings_default(mgbvh_settings* s);
+
+/* Creates a builder context on HIP device `device` (owns a stream and scratch). */
+int mgbvh_ctx_create(int device, mgbvh_ctx** out);
+int mgbvh_ctx_destroy(mgbvh_ctx* ctx);
+/* Thread-local message of the last failing call on this thread. */
+const char* mgbvh_last_error(void);
+/* Stream the context enqueues on (hipStream_t as void*). */
+void* mgbvh_ctx_stream(mgbvh_ctx* ctx);
+int mgbvh_ctx_synchronize(mgbvh_ctx* ctx);
+
+/* ---- host-buffer entry points (drop-in for BoundingVolumeHierarchy::new) ----
+ * positions: vertex positions, xyz floats at `positions + k * vstride_bytes`.
+ * tris: n x 3 u32 vertex indices, PERMUTED IN PLACE like `list: &mut [T]`.
+ * out_nodes: capacity 2n-1; *out_nnodes receives the node count.
+ * out_lens (optional): BvhNode.len per node.  out_perm (optional): original
+ * object index at each position of the permuted list.  out_splits (optional,
+ * capacity 2n-1): split record per internal node.  out_stats optional. */
+int mgbvh_build_tris(mgbvh_ctx* ctx, const float* positions, uint32_t vstride_bytes,
+                     uint64_t nverts, uint32_t* tris, uint64_t ntris,
+                     const mgbvh_settings* settings, mgbvh_node32* out_nodes,
+                     uint32_t* out_lens, uint64_t* out_nnodes, uint32_t* out_perm,
+                     mgbvh_split_rec* out_splits, mgbvh_stats* out_stats);
+
+/* Objects given by their boxes (T: AsBoundingVolume).  The caller applies
+ * out_perm (required) to its own object list, as the crate permutes `list`. */
+int mgbvh_build_aabbs(mgbvh_ctx* ctx, const mgbvh_aabb* aabbs, uint64_t n,
+                      const mgbvh_settings* settings, mgbvh_node32* out_nodes,
+                      uint32_t* out_lens, uint64_t* out_nnodes, uint32_t* out_perm,
+                      mgbvh_split_rec* out_splits, mgbvh_stats* out_stats);
+
+/* ---- device-buffer entry points (all pointers are device pointers) ----
+ * Outputs: d_nodes (2n-1), d_perm (n); d_tris_out (optional, n x 3) receives the
+ * permuted triangle list; d_lens and d_splits optional.  Enqueued on the
+ * context stream; synchronizes only to read the node count. */
+int mgbvh_build_tris_device(mgbvh_ctx* ctx, const float* d_positions, uint32_t vstride_bytes,
+                            uint64_t nverts, const uint32_t* d_tris, uint64_t ntris,
+                            const mgbvh_settings* settings, mgbvh_node32* d_nodes,
+                            uint32_t* d_lens, uint32_t* d_perm, uint32_t* d_tris_out,
+                            mgbvh_split_rec* d_splits, uint64_t* out_nnodes,
+                            mgbvh_stats* out_stats);
+
+int mgbvh_build_aabbs_device(mgbvh_ctx* ctx, const mgbvh_aabb* d_aabbs, uint64_t n,
+                             const mgbvh_settings* settings, mgbvh_node32* d_nodes,
+                             uint32_t* d_lens, uint32_t* d_perm, mgbvh_split_rec* d_splits,
+                             uint64_t* out_nnodes, mgbvh_stats* out_stats);
+
+/* ---- device memory helpers (so callers need no other GPU runtime) ---- */
+int mgbvh_device_alloc(mgbvh_ctx* ctx, uint64_t bytes, void** out);
+int mgbvh_device_free(mgbvh_ctx* ctx, void* p);
+int mgbvh_copy_h2d(mgbvh_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int mgbvh_copy_d2h(mgbvh_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* ---- per-kernel timing: HIP event pairs around each launch of the kernel
+ * classes selected by `kernel_mask` (bit k = class k), accumulated per class.
+ * Classes: 0 setup, 1 root, 2 bin, 3 sweep, 4 partition, 5 small-subtree,
+ * 6 compact, 7 permute, 8 trace. ---- */
+#define MGBVH_KERNEL_KINDS 16
+int mgbvh_ctx_set_profiling(mgbvh_ctx* ctx, uint32_t kernel_mask);
+int mgbvh_ctx_kernel_times(mgbvh_ctx* ctx, double* ms /* [16] */, uint64_t* launches /* [16] */,
+                           int reset);
+
+/* ---- timing helpers: HIP events recorded on the context stream ---- */
+int mgbvh_event_record(mgbvh_ctx* ctx, int slot);           /* slot 0..59 */
+int mgbvh_event_elapsed_ms(mgbvh_ctx* ctx, int a, int b, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MGBVH_H */

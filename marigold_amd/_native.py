@@ -1,0 +1,315 @@
+"""ctypes binding of the mgbvh C ABI (include/mgbvh.h).
+
+Loads ``libmgbvh.so`` built in-tree next to this file and fails loudly when it
+is missing: there is no CPU fallback for the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmgbvh.so")
+
+OK = 0
+ERR_INVALID_ARG = -1
+ERR_UNSUPPORTED = -2
+ERR_NONFINITE = -3
+ERR_INDEX = -4
+ERR_DEVICE = -5
+ERR_NO_DEVICE = -6
+ERR_INTERNAL = -7
+
+METHOD_PLAIN = 0
+METHOD_LEARNED = 1
+
+NODE_DTYPE = np.dtype([("min", "<f4", (3,)), ("start_index", "<u4"), ("max", "<f4", (3,)),
+                       ("child_node", "<u4")])
+SPLIT_DTYPE = np.dtype([("axis_method", "<u4"), ("value", "<u4"), ("count_lt", "<u4"),
+                        ("cost_bits", "<u4")])
+assert NODE_DTYPE.itemsize == 32 and SPLIT_DTYPE.itemsize == 16
+
+
+class Settings(C.Structure):
+    _fields_ = [
+        ("name", C.c_char_p),
+        ("has_bounds", C.c_int32),
+        ("bounds_min", C.c_float * 3),
+        ("bounds_max", C.c_float * 3),
+        ("max_depth", C.c_uint32),
+        ("profiling_info", C.c_int32),
+        ("profiling_info_directory", C.c_char_p),
+        ("min_leaf_objects", C.c_uint32),
+        ("max_leaf_objects", C.c_uint32),
+        ("method", C.c_uint32),
+        ("reserved", C.c_uint32 * 7),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("construction_time", C.c_double),
+        ("node_count", C.c_uint32),
+        ("leaf_node_count", C.c_uint32),
+        ("min_leaf_object_count", C.c_uint32),
+        ("max_leaf_object_count", C.c_uint32),
+        ("avg_leaf_object_count", C.c_double),
+        ("stddev_leaf_object_count", C.c_double),
+        ("total_object_count", C.c_uint32),
+        ("height", C.c_uint32),
+        ("max_depth", C.c_uint32),
+        ("n_large_levels", C.c_uint32),
+        ("n_median", C.c_uint32),
+        ("n_cutoff_leaves", C.c_uint32),
+        ("n_guard_fallbacks", C.c_uint32),
+        ("reserved", C.c_uint32 * 5),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class MgbvhError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"mgbvh error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_U64 = C.c_uint64
+_I = C.c_int
+
+
+def _sig(lib):
+    S = C.POINTER(Settings)
+    St = C.POINTER(Stats)
+    U64P = C.POINTER(C.c_uint64)
+    lib.mgbvh_settings_default.argtypes = [S]
+    lib.mgbvh_settings_default.restype = None
+    lib.mgbvh_last_error.argtypes = []
+    lib.mgbvh_last_error.restype = C.c_char_p
+    lib.mgbvh_ctx_create.argtypes = [_I, C.POINTER(_P)]
+    lib.mgbvh_ctx_destroy.argtypes = [_P]
+    lib.mgbvh_ctx_stream.argtypes = [_P]
+    lib.mgbvh_ctx_stream.restype = _P
+    lib.mgbvh_ctx_synchronize.argtypes = [_P]
+    lib.mgbvh_build_tris.argtypes = [_P, _P, _U32, _U64, _P, _U64, S, _P, _P, U64P, _P, _P, St]
+    lib.mgbvh_build_aabbs.argtypes = [_P, _P, _U64, S, _P, _P, U64P, _P, _P, St]
+    lib.mgbvh_build_tris_device.argtypes = [_P, _P, _U32, _U64, _P, _U64, S, _P, _P, _P, _P, _P,
+                                            U64P, St]
+    lib.mgbvh_build_aabbs_device.argtypes = [_P, _P, _U64, S, _P, _P, _P, _P, U64P, St]
+    lib.mgbvh_device_alloc.argtypes = [_P, _U64, C.POINTER(_P)]
+    lib.mgbvh_device_free.argtypes = [_P, _P]
+    lib.mgbvh_copy_h2d.argtypes = [_P, _P, _P, _U64]
+    lib.mgbvh_copy_d2h.argtypes = [_P, _P, _P, _U64]
+    lib.mgbvh_event_record.argtypes = [_P, _I]
+    lib.mgbvh_event_elapsed_ms.argtypes = [_P, _I, _I, C.POINTER(C.c_float)]
+    lib.mgbvh_ctx_set_profiling.argtypes = [_P, _U32]
+    lib.mgbvh_ctx_kernel_times.argtypes = [_P, _P, _P, _I]
+    for fn in ("mgbvh_ctx_set_profiling", "mgbvh_ctx_kernel_times", "mgbvh_ctx_create", "mgbvh_ctx_destroy", "mgbvh_ctx_synchronize", "mgbvh_build_tris",
+               "mgbvh_build_aabbs", "mgbvh_build_tris_device", "mgbvh_build_aabbs_device",
+               "mgbvh_device_alloc", "mgbvh_device_free", "mgbvh_copy_h2d", "mgbvh_copy_d2h",
+               "mgbvh_event_record", "mgbvh_event_elapsed_ms"):
+        getattr(lib, fn).restype = _I
+    _sig_extra(lib)
+
+
+def _sig_extra(lib):
+    """Signatures of entry points added beside the builder (traversal, batching)."""
+    for name, (argtypes, restype) in _EXTRA.items():
+        if hasattr(lib, name):
+            f = getattr(lib, name)
+            f.argtypes = argtypes
+            f.restype = restype
+
+
+_EXTRA: dict = {}
+
+
+def lib():
+    """The loaded libmgbvh.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+                " (there is no CPU fallback)")
+        _lib = C.CDLL(LIB_PATH)
+        _sig(_lib)
+    return _lib
+
+
+def last_error() -> str:
+    return lib().mgbvh_last_error().decode(errors="replace")
+
+
+def check(rc: int):
+    if rc != OK:
+        raise MgbvhError(rc, last_error())
+
+
+def ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def default_settings(max_depth: int = 32, bounds=None, method: int = METHOD_PLAIN, name: str = "",
+                     profiling_info: bool = False, profiling_info_directory: str | None = None):
+    s = Settings()
+    lib().mgbvh_settings_default(C.byref(s))
+    s.max_depth = max_depth
+    s.method = method
+    s._name = name.encode()  # keep the bytes alive
+    s.name = s._name
+    s.profiling_info = 1 if profiling_info else 0
+    if profiling_info_directory is not None:
+        s._dir = str(profiling_info_directory).encode()
+        s.profiling_info_directory = s._dir
+    if bounds is not None:
+        lo, hi = bounds
+        s.has_bounds = 1
+        for a in range(3):
+            s.bounds_min[a] = float(lo[a])
+            s.bounds_max[a] = float(hi[a])
+    return s
+
+
+class DeviceBuffer:
+    """Device allocation owned by a Context (freed with it or by `free`)."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib().mgbvh_device_alloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    @property
+    def value(self):
+        return self.ptr.value
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().mgbvh_copy_h2d(self.ctx.handle, self.ptr, ptr(a), a.nbytes))
+
+    def download(self, dtype, count) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib().mgbvh_copy_d2h(self.ctx.handle, ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().mgbvh_device_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().mgbvh_ctx_create(device, C.byref(h)))
+        self.handle = h
+        self._bufs = []
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        b = DeviceBuffer(self, nbytes)
+        self._bufs.append(b)
+        return b
+
+    def to_device(self, a: np.ndarray) -> DeviceBuffer:
+        a = np.ascontiguousarray(a)
+        b = self.alloc(max(a.nbytes, 16))
+        b.upload(a)
+        return b
+
+    def synchronize(self):
+        check(lib().mgbvh_ctx_synchronize(self.handle))
+
+    def record(self, slot: int):
+        check(lib().mgbvh_event_record(self.handle, slot))
+
+    def elapsed_ms(self, a: int, b: int) -> float:
+        ms = C.c_float()
+        check(lib().mgbvh_event_elapsed_ms(self.handle, a, b, C.byref(ms)))
+        return float(ms.value)
+
+    KERNEL_KINDS = ["setup", "root", "bin", "sweep", "partition", "small", "compact", "permute",
+                    "trace"]
+
+    def set_profiling(self, kinds):
+        mask = 0
+        for k in kinds:
+            mask |= 1 << self.KERNEL_KINDS.index(k)
+        check(lib().mgbvh_ctx_set_profiling(self.handle, mask))
+
+    def kernel_times(self, reset: bool = True):
+        ms = np.zeros(16, dtype=np.float64)
+        n = np.zeros(16, dtype=np.uint64)
+        check(lib().mgbvh_ctx_kernel_times(self.handle, ptr(ms), ptr(n), 1 if reset else 0))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.KERNEL_KINDS) if n[i]}
+
+    def close(self):
+        if self.handle is not None:
+            for b in self._bufs:
+                b.free()
+            self._bufs = []
+            lib().mgbvh_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- host-buffer builds -------------------------------------------------
+    def build_tris(self, positions: np.ndarray, tris: np.ndarray, settings: Settings,
+                   want_splits: bool = False, want_stats: bool = True):
+        """Builds over host arrays.  `tris` (N,3) uint32 is permuted in place.
+        Returns (nodes, lens, perm, splits, stats)."""
+        assert positions.dtype == np.float32 and positions.flags["C_CONTIGUOUS"]
+        assert tris.dtype == np.uint32 and tris.flags["C_CONTIGUOUS"]
+        n = tris.shape[0]
+        vstride = positions.strides[0] if positions.ndim == 2 else 12
+        nslots = max(2 * n - 1, 1)
+        nodes = np.zeros(nslots, dtype=NODE_DTYPE)
+        lens = np.zeros(nslots, dtype=np.uint32)
+        perm = np.zeros(max(n, 1), dtype=np.uint32)
+        splits = np.zeros(nslots, dtype=SPLIT_DTYPE) if want_splits else None
+        nn = C.c_uint64()
+        st = Stats()
+        check(lib().mgbvh_build_tris(self.handle, ptr(positions), vstride, positions.shape[0],
+                                     ptr(tris), n, C.byref(settings), ptr(nodes), ptr(lens),
+                                     C.byref(nn), ptr(perm), ptr(splits),
+                                     C.byref(st) if want_stats else None))
+        k = nn.value
+        return (nodes[:k], lens[:k], perm[:n], None if splits is None else splits[:k],
+                st.as_dict() if want_stats else None)
+
+    def build_aabbs(self, boxes: np.ndarray, settings: Settings, want_splits: bool = False,
+                    want_stats: bool = True):
+        """Builds over (N,6) float32 boxes [min xyz, max xyz].
+        Returns (nodes, lens, perm, splits, stats)."""
+        boxes = np.ascontiguousarray(boxes, dtype=np.float32)
+        n = boxes.shape[0]
+        nslots = max(2 * n - 1, 1)
+        nodes = np.zeros(nslots, dtype=NODE_DTYPE)
+        lens = np.zeros(nslots, dtype=np.uint32)
+        perm = np.zeros(max(n, 1), dtype=np.uint32)
+        splits = np.zeros(nslots, dtype=SPLIT_DTYPE) if want_splits else None
+        nn = C.c_uint64()
+        st = Stats()
+        check(lib().mgbvh_build_aabbs(self.handle, ptr(boxes), n, C.byref(settings), ptr(nodes),
+                                      ptr(lens), C.byref(nn), ptr(perm), ptr(splits),
+                                      C.byref(st) if want_stats else None))
+        k = nn.value
+        return (nodes[:k], lens[:k], perm[:n], None if splits is None else splits[:k],
+                st.as_dict() if want_stats else None)

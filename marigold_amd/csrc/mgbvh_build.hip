@@ -1,0 +1,1017 @@
+// mgbvh plain builder: breadth-first SAH BVH construction on gfx950 that
+// reproduces marigold's recursive builder (bvh/src/lib.rs:536-665) bit for bit.
+//
+// Pipeline for one object list of n boxes:
+//   k_setup_*     object AABBs (mesh_interface/src/lib.rs:19-30) into 32-byte
+//                 records, root bounds and root centroid bounds (lib.rs:220-228, 499-509)
+//   level loop    for every node with > 32 objects (binned sweep, lib.rs:331-389):
+//     k_bin       per chunk of 2048 objects: LDS-staged 3-axis x 32-bin histograms
+//                 (count + bin box), merged into a per-node histogram
+//     k_sweep     one wavefront per node: 3 x 31 prefix-SAH candidates, first
+//                 minimum under f32::total_cmp (lib.rs:373-388, 515-524), median
+//                 fallback (lib.rs:527-531), node record, child allocation
+//     k_partition stable coalesced partition by bin (select_nth_unstable at
+//                 lib.rs:601-607 restated as a stable split), children's centroid bounds
+//   k_small       one wavefront per subtree of <= 32 objects: adaptive sweep
+//                 (lib.rs:391-442) down to the leaves, entirely in LDS
+//   k_compact     only when max_depth cut a subtree short: squeeze the slots
+//                 reserved for the missing nodes
+//
+// Node numbering: the reference pushes each split's (gt, lt) pair at
+// nodes.len() in depth-first, gt-first order (lib.rs:648-663).  With one object
+// per leaf a subtree of n objects owns exactly 2(n-1) descendant slots, so a
+// node's pair base P gives its children P(gt) = P + 2 and
+// P(lt) = P + 2 + 2(n_gt - 1) without any global order.
+#include "mgbvh_common.h"
+#include "mgbvh_internal.h"
+
+namespace mg {
+
+// --------------------------------------------------------------- records
+__device__ inline void rec_load(const Rec* r, Box& b, uint32_t& id) {
+  const float4* p = reinterpret_cast<const float4*>(r);
+  float4 a = p[0];
+  float4 c = p[1];
+  b.mn[0] = a.x; b.mn[1] = a.y; b.mn[2] = a.z;
+  b.mx[0] = a.w; b.mx[1] = c.x; b.mx[2] = c.y;
+  id = __float_as_uint(c.z);
+}
+__device__ inline void rec_store(Rec* r, const Box& b, uint32_t id) {
+  float4* p = reinterpret_cast<float4*>(r);
+  p[0] = make_float4(b.mn[0], b.mn[1], b.mn[2], b.mx[0]);
+  p[1] = make_float4(b.mx[1], b.mx[2], __uint_as_float(id), 0.0f);
+}
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline bool coord_ok(float x) { return fabsf(x) < 1.152921504606846976e18f; }  // 2^60
+
+// Block-wide max of 12 accumulator words, then one atomicMax each.
+__device__ inline void block_acc12_flush(uint32_t (&acc)[12], uint32_t* dst) {
+  __shared__ uint32_t red[4][12];
+  const uint32_t w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = wave_max_u32(acc[k]);
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) red[w][k] = acc[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 12) {
+    uint32_t v = red[0][threadIdx.x];
+    for (uint32_t k = 1; k < (blockDim.x >> 6); ++k) v = max(v, red[k][threadIdx.x]);
+    if (v) atomicMax(&dst[threadIdx.x], v);
+  }
+}
+
+__device__ inline void acc12_add(uint32_t (&acc)[12], const Box& b) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    acc[a] = max(acc[a], acc_min_enc(b.mn[a]));
+    acc[3 + a] = max(acc[3 + a], acc_max_enc(b.mx[a]));
+    float c = center(b.mn[a], b.mx[a]);
+    acc[6 + a] = max(acc[6 + a], acc_min_enc(c));
+    acc[9 + a] = max(acc[9 + a], acc_max_enc(c));
+  }
+}
+
+// ------------------------------------------------------------------ setup
+// MeshTriangle::bounding_volume (mesh_interface/src/lib.rs:19-30):
+// min = v1.min(v2).min(v3), max likewise; -0.0 is stored as +0.0.
+__global__ __launch_bounds__(256) void k_setup_tris(const float* __restrict__ pos, uint32_t vs,
+                                                    uint64_t nverts,
+                                                    const uint32_t* __restrict__ tris,
+                                                    uint32_t n, Rec* __restrict__ recs,
+                                                    Counters* ctr) {
+  uint32_t acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = 0;
+  uint32_t err = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint32_t i0 = tris[3 * (size_t)i + 0];
+    uint32_t i1 = tris[3 * (size_t)i + 1];
+    uint32_t i2 = tris[3 * (size_t)i + 2];
+    if (i0 >= nverts || i1 >= nverts || i2 >= nverts) {
+      err |= ERR_INDEX;
+      i0 = i1 = i2 = 0;
+    }
+    const float* v1 = pos + (size_t)i0 * vs;
+    const float* v2 = pos + (size_t)i1 * vs;
+    const float* v3 = pos + (size_t)i2 * vs;
+    Box b;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float x1 = v1[a], x2 = v2[a], x3 = v3[a];
+      if (!coord_ok(x1) || !coord_ok(x2) || !coord_ok(x3)) err |= ERR_NONFINITE;
+      b.mn[a] = canon_zero(fmin_x86(fmin_x86(x1, x2), x3));
+      b.mx[a] = canon_zero(fmax_x86(fmax_x86(x1, x2), x3));
+    }
+    rec_store(&recs[i], b, i);
+    acc12_add(acc, b);
+  }
+  if (err) atomicOr(&ctr->err, err);
+  block_acc12_flush(acc, ctr->root_acc);
+}
+
+// Objects given as boxes (AsBoundingVolume, e.g. UploadedMesh at
+// mesh_interface/src/lib.rs:87-91).
+__global__ __launch_bounds__(256) void k_setup_aabbs(const float* __restrict__ boxes, uint32_t n,
+                                                     Rec* __restrict__ recs, Counters* ctr) {
+  uint32_t acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = 0;
+  uint32_t err = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    Box b;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float lo = boxes[6 * (size_t)i + a], hi = boxes[6 * (size_t)i + 3 + a];
+      if (!coord_ok(lo) || !coord_ok(hi)) err |= ERR_NONFINITE;
+      b.mn[a] = canon_zero(lo);
+      b.mx[a] = canon_zero(hi);
+    }
+    rec_store(&recs[i], b, i);
+    acc12_add(acc, b);
+  }
+  if (err) atomicOr(&ctr->err, err);
+  block_acc12_flush(acc, ctr->root_acc);
+}
+
+// Root node, lib.rs:220-243 and 731-735.
+__global__ void k_root(Counters* ctr, LNode* L, uint32_t* cnode, uint32_t* ck, SNode* small,
+                       uint32_t n, int has_bounds, float bx0, float by0, float bz0, float bx1,
+                       float by1, float bz1, int large) {
+  __shared__ uint32_t nch_s;
+  if (threadIdx.x == 0) {
+    float bmn[3], bmx[3];
+    if (has_bounds) {
+      bmn[0] = bx0; bmn[1] = by0; bmn[2] = bz0;
+      bmx[0] = bx1; bmx[1] = by1; bmx[2] = bz1;
+    } else {
+      for (int a = 0; a < 3; ++a) {
+        bmn[a] = acc_min_dec(ctr->root_acc[a]);
+        bmx[a] = acc_max_dec(ctr->root_acc[3 + a]);
+      }
+    }
+    if (large) {
+      LNode r = {};
+      r.start = 0; r.count = n; r.idx = 0; r.P = 1; r.depth = 0;
+      r.chunk0 = 0; r.nchunk = (n + CH - 1) / CH;
+      for (int a = 0; a < 3; ++a) {
+        r.bmn[a] = bmn[a]; r.bmx[a] = bmx[a];
+        r.cacc[a] = ctr->root_acc[6 + a];
+        r.cacc[3 + a] = ctr->root_acc[9 + a];
+      }
+      r.child_slot[0] = r.child_slot[1] = -1;
+      L[0] = r;
+      nch_s = r.nchunk;
+    } else {
+      SNode s = {};
+      s.start = 0; s.count = n; s.idx = 0; s.P = 1; s.depth = 0; s.parity = 0;
+      for (int a = 0; a < 3; ++a) { s.bmn[a] = bmn[a]; s.bmx[a] = bmx[a]; }
+      small[0] = s;
+      ctr->n_small = 1;
+      nch_s = 0;
+    }
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < nch_s; k += blockDim.x) {
+    cnode[k] = 0;
+    ck[k] = k;
+  }
+}
+
+// ------------------------------------------------------------------ binning
+// One block per chunk of <= CH objects of one node: 3 x 32 bins of
+// {count, bin box} in LDS, flushed into the node histogram; per-chunk counts
+// are kept for the stable partition.
+__global__ __launch_bounds__(256) void k_bin(const LNode* __restrict__ L,
+                                             const uint32_t* __restrict__ cnode,
+                                             const uint32_t* __restrict__ ck,
+                                             const Rec* __restrict__ recs,
+                                             uint32_t* __restrict__ hist,
+                                             uint32_t* __restrict__ blk) {
+  __shared__ uint32_t h[HIST_WORDS];
+  const uint32_t c = blockIdx.x;
+  const uint32_t slot = cnode[c];
+  const uint32_t k = ck[c];
+  const LNode& nd = L[slot];
+  for (uint32_t t = threadIdx.x; t < HIST_WORDS; t += blockDim.x) h[t] = 0;
+  float cmin[3], cext[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    cmin[a] = acc_min_dec(nd.cacc[a]);
+    float cmax = acc_max_dec(nd.cacc[3 + a]);
+    cext[a] = __fsub_rn(cmax, cmin[a]);
+  }
+  __syncthreads();
+  const uint32_t lo = nd.start + k * CH;
+  const uint32_t hi = min(lo + CH, nd.start + nd.count);
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    Box b;
+    uint32_t id;
+    rec_load(&recs[i], b, id);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float cc = center(b.mn[a], b.mx[a]);
+      uint32_t bi = bin_index(cc, cmin[a], cext[a]);
+      uint32_t* hb = &h[(a * MG_NBINS + bi) * BIN_WORDS];
+      atomicAdd(&hb[0], 1u);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        atomicMax(&hb[1 + j], acc_min_enc(b.mn[j]));
+        atomicMax(&hb[4 + j], acc_max_enc(b.mx[j]));
+      }
+    }
+  }
+  __syncthreads();
+  uint32_t* gh = hist + (size_t)slot * HIST_WORDS;
+  for (uint32_t t = threadIdx.x; t < HIST_WORDS; t += blockDim.x) {
+    uint32_t v = h[t];
+    uint32_t f = t % BIN_WORDS;
+    if (f == 0) {
+      blk[(size_t)c * 96 + t / BIN_WORDS] = v;
+      if (v) atomicAdd(&gh[t], v);
+    } else if (v) {
+      atomicMax(&gh[t], v);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ sweep
+__device__ inline Box hist_box(const uint32_t* hb) {
+  Box b;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    b.mn[j] = acc_min_dec(hb[1 + j]);
+    b.mx[j] = acc_max_dec(hb[4 + j]);
+  }
+  return b;
+}
+
+// evaluate_binned_split (lib.rs:255-290) for cut i on axis a.
+__device__ inline void eval_cut(const uint32_t* h, int a, uint32_t i, Box& lt, Box& gt,
+                                uint32_t& nl, uint32_t& ng) {
+  lt = box_empty();
+  gt = box_empty();
+  nl = 0;
+  ng = 0;
+  for (uint32_t b = 0; b < MG_NBINS; ++b) {
+    const uint32_t* hb = &h[(a * MG_NBINS + b) * BIN_WORDS];
+    uint32_t cnt = hb[0];
+    if (!cnt) continue;
+    Box bb = hist_box(hb);
+    if (b < i) { nl += cnt; box_grow(lt, bb); }
+    else { ng += cnt; box_grow(gt, bb); }
+  }
+}
+
+__device__ inline void emit_child(uint32_t lane, uint32_t start, uint32_t count, const Box& b,
+                                  uint32_t idx, uint32_t P, uint32_t depth, uint32_t max_depth,
+                                  uint32_t parity_next, LNode* Ln, uint32_t* cn_next,
+                                  uint32_t* ck_next, SNode* small, Counters* ctr,
+                                  int32_t& slot_out) {
+  const bool large = count > MG_SMALL && depth < max_depth;
+  uint32_t s = 0, c0 = 0, nch = 0;
+  if (lane == 0) {
+    if (large) {
+      s = atomicAdd(&ctr->n_large_next, 1u);
+      nch = (count + CH - 1) / CH;
+      c0 = atomicAdd(&ctr->n_chunks_next, nch);
+      LNode r = {};
+      r.start = start; r.count = count; r.idx = idx; r.P = P; r.depth = depth;
+      r.chunk0 = c0; r.nchunk = nch;
+      for (int a = 0; a < 3; ++a) { r.bmn[a] = b.mn[a]; r.bmx[a] = b.mx[a]; }
+      r.child_slot[0] = r.child_slot[1] = -1;
+      Ln[s] = r;
+    } else {
+      s = atomicAdd(&ctr->n_small, 1u);
+      SNode sn = {};
+      sn.start = start; sn.count = count; sn.idx = idx; sn.P = P; sn.depth = depth;
+      sn.parity = parity_next;
+      for (int a = 0; a < 3; ++a) { sn.bmn[a] = b.mn[a]; sn.bmx[a] = b.mx[a]; }
+      small[s] = sn;
+    }
+  }
+  s = __shfl(s, 0, 64);
+  c0 = __shfl(c0, 0, 64);
+  nch = __shfl(nch, 0, 64);
+  if (large) {
+    for (uint32_t k = lane; k < nch; k += 64) {
+      cn_next[c0 + k] = s;
+      ck_next[c0 + k] = k;
+    }
+    slot_out = (int32_t)s;
+  } else {
+    slot_out = -1;
+  }
+}
+
+// One wavefront per node with > 32 objects: select_split (lib.rs:493-534) over
+// the binned histograms, node record, children.
+__global__ __launch_bounds__(256) void k_sweep(LNode* __restrict__ L, uint32_t nL,
+                                               const uint32_t* __restrict__ hist,
+                                               const Rec* __restrict__ recs, LNode* Ln,
+                                               uint32_t* cn_next, uint32_t* ck_next,
+                                               SNode* small, Counters* ctr,
+                                               NodeOut out, uint32_t max_depth,
+                                               uint32_t parity_next) {
+  __shared__ uint32_t sh[4][HIST_WORDS];
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t lane = lane_id();
+  const uint32_t slot = blockIdx.x * 4 + w;
+  if (slot >= nL) return;  // only wave-local synchronisation below
+  uint32_t* h = sh[w];
+  const uint32_t* gh = hist + (size_t)slot * HIST_WORDS;
+  for (uint32_t t = lane; t < HIST_WORDS; t += 64) h[t] = gh[t];
+  wave_sync();
+  LNode nd = L[slot];
+  Box nb;
+  float cmin[3], cext[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    nb.mn[a] = nd.bmn[a];
+    nb.mx[a] = nd.bmx[a];
+    cmin[a] = acc_min_dec(nd.cacc[a]);
+    cext[a] = __fsub_rn(acc_max_dec(nd.cacc[3 + a]), cmin[a]);
+  }
+  const float sa_p = box_sa(nb);
+  uint64_t best = ~0ull;
+  for (uint32_t q = lane; q < 3 * MG_NBINS; q += 64) {
+    const int a = (int)(q / MG_NBINS);
+    const uint32_t i = q % MG_NBINS;
+    if (i == 0 || cext[a] == 0.0f) continue;  // lib.rs:339-341
+    Box lt, gt;
+    uint32_t nl, ng;
+    eval_cut(h, a, i, lt, gt, nl, ng);
+    if (nl < 1 || ng < 1) continue;           // lib.rs:379-381
+    float cost = split_cost(sa_p, box_sa(lt), box_sa(gt), nl, ng);
+    uint64_t key = ((uint64_t)tc_ukey(cost) << 32) | q;
+    best = key < best ? key : best;
+  }
+  best = wave_min_u64(best);
+
+  Box lt, gt;
+  uint32_t nl, ng, axis, value, kind;
+  float cost;
+  if (best != ~0ull) {
+    const uint32_t q = (uint32_t)best;
+    axis = q / MG_NBINS;
+    value = q % MG_NBINS;
+    kind = 0;
+    eval_cut(h, (int)axis, value, lt, gt, nl, ng);
+    cost = split_cost(sa_p, box_sa(lt), box_sa(gt), nl, ng);
+  } else {
+    // Median fallback (lib.rs:444-491, 527-531).  With > 32 objects it is only
+    // reached when every centroid coincides, so each axis' select keeps the
+    // list order and axis 0 wins: lt = first n/2 objects.
+    const uint32_t m = nd.count / 2;
+    uint32_t acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = 0;
+    for (uint32_t j = lane; j < nd.count; j += 64) {
+      Box b;
+      uint32_t id;
+      rec_load(&recs[nd.start + j], b, id);
+      const int o = j < m ? 0 : 6;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        acc[o + a] = max(acc[o + a], acc_min_enc(b.mn[a]));
+        acc[o + 3 + a] = max(acc[o + 3 + a], acc_max_enc(b.mx[a]));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = wave_max_u32(acc[k]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lt.mn[a] = acc_min_dec(acc[a]);
+      lt.mx[a] = acc_max_dec(acc[3 + a]);
+      gt.mn[a] = acc_min_dec(acc[6 + a]);
+      gt.mx[a] = acc_max_dec(acc[9 + a]);
+    }
+    nl = m;
+    ng = nd.count - m;
+    axis = 0;
+    value = m;
+    kind = 2;
+    cost = split_cost(sa_p, box_sa(lt), box_sa(gt), nl, ng);
+  }
+
+  if (lane == 0) {
+    out.write_node(nd.idx, nb, nd.start, nd.P, nd.count);
+    out.write_split(nd.idx, axis | (kind << 8), value, nl, fbits(cost));
+    atomicMax(&ctr->height, nd.depth);
+    if (kind == 2) atomicAdd(&ctr->n_median, 1u);
+    L[slot].axis = axis;
+    L[slot].cut = value;
+    L[slot].n_lt = nl;
+    L[slot].kind = kind;
+  }
+  const uint32_t cd = nd.depth + 1;
+  int32_t sg, sl;
+  emit_child(lane, nd.start + nl, ng, gt, nd.P, nd.P + 2, cd, max_depth, parity_next, Ln,
+             cn_next, ck_next, small, ctr, sg);
+  emit_child(lane, nd.start, nl, lt, nd.P + 1, nd.P + 2 + 2 * (ng - 1), cd, max_depth,
+             parity_next, Ln, cn_next, ck_next, small, ctr, sl);
+  if (lane == 0) {
+    L[slot].child_slot[0] = sg;
+    L[slot].child_slot[1] = sl;
+  }
+}
+
+// ------------------------------------------------------------------ partition
+// Stable partition of one chunk: lt objects keep their order at the front of
+// the node range, gt objects after n_lt.  Offsets of earlier chunks come from
+// their per-bin counts.  Children's centroid bounds are accumulated here.
+__global__ __launch_bounds__(256) void k_partition(const LNode* __restrict__ L,
+                                                   const uint32_t* __restrict__ cnode,
+                                                   const uint32_t* __restrict__ ck,
+                                                   const uint32_t* __restrict__ blk,
+                                                   const Rec* __restrict__ src,
+                                                   Rec* __restrict__ dst, LNode* Ln) {
+  __shared__ uint32_t s_red[4];
+  __shared__ uint32_t s_lt[4], s_gt[4];
+  const uint32_t c = blockIdx.x;
+  const uint32_t slot = cnode[c];
+  const uint32_t k = ck[c];
+  const LNode& nd = L[slot];
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t lane = lane_id();
+  const uint32_t kind = nd.kind, axis = nd.axis, cut = nd.cut, n_lt = nd.n_lt;
+  const uint32_t lo = nd.start + k * CH;
+  const uint32_t hi = min(lo + CH, nd.start + nd.count);
+
+  uint32_t lt_before;
+  if (kind == 0) {
+    uint32_t part = 0;
+    const uint32_t tot = k * cut;
+    for (uint32_t t = threadIdx.x; t < tot; t += blockDim.x) {
+      uint32_t kk = t / cut, b = t % cut;
+      part += blk[(size_t)(nd.chunk0 + kk) * 96 + axis * MG_NBINS + b];
+    }
+    part = wave_sum_u32(part);
+    if (lane == 0) s_red[w] = part;
+    __syncthreads();
+    lt_before = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+  } else {
+    lt_before = min(k * CH, n_lt);
+  }
+  const uint32_t gt_before = k * CH - lt_before;
+  const float cmin = acc_min_dec(nd.cacc[axis]);
+  const float cext = __fsub_rn(acc_max_dec(nd.cacc[3 + axis]), cmin);
+
+  uint32_t acc_lt[6], acc_gt[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) { acc_lt[j] = 0; acc_gt[j] = 0; }
+  uint32_t run_lt = 0, run_gt = 0;
+  for (uint32_t base = lo; base < hi; base += blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    const bool valid = i < hi;
+    Box b;
+    uint32_t id = 0;
+    bool pred = false;
+    if (valid) {
+      rec_load(&src[i], b, id);
+      if (kind == 0) pred = bin_index(center(b.mn[axis], b.mx[axis]), cmin, cext) < cut;
+      else pred = (i - nd.start) < n_lt;
+    }
+    const uint64_t ml = __ballot(valid && pred);
+    const uint64_t mg = __ballot(valid && !pred);
+    if (lane == 0) { s_lt[w] = (uint32_t)__popcll(ml); s_gt[w] = (uint32_t)__popcll(mg); }
+    __syncthreads();
+    uint32_t pl = 0, pg = 0, tl = 0, tg = 0;
+    for (uint32_t q = 0; q < 4; ++q) {
+      if (q < w) { pl += s_lt[q]; pg += s_gt[q]; }
+      tl += s_lt[q];
+      tg += s_gt[q];
+    }
+    if (valid) {
+      uint32_t p;
+      if (pred) p = nd.start + lt_before + run_lt + pl + lanemask_lt_popc(ml);
+      else p = nd.start + n_lt + gt_before + run_gt + pg + lanemask_lt_popc(mg);
+      rec_store(&dst[p], b, id);
+      uint32_t* acc = pred ? acc_lt : acc_gt;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        float cc = center(b.mn[a], b.mx[a]);
+        acc[a] = max(acc[a], acc_min_enc(cc));
+        acc[3 + a] = max(acc[3 + a], acc_max_enc(cc));
+      }
+    }
+    run_lt += tl;
+    run_gt += tg;
+    __syncthreads();
+  }
+  // children centroid bounds
+  __shared__ uint32_t red[4][12];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    acc_lt[j] = wave_max_u32(acc_lt[j]);
+    acc_gt[j] = wave_max_u32(acc_gt[j]);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) { red[w][j] = acc_gt[j]; red[w][6 + j] = acc_lt[j]; }
+  }
+  __syncthreads();
+  if (threadIdx.x < 12) {
+    const int which = threadIdx.x < 6 ? 0 : 1;  // 0 = gt child, 1 = lt child
+    const int32_t cs = nd.child_slot[which];
+    uint32_t v = max(max(red[0][threadIdx.x], red[1][threadIdx.x]),
+                     max(red[2][threadIdx.x], red[3][threadIdx.x]));
+    if (cs >= 0 && v) atomicMax(&Ln[cs].cacc[threadIdx.x % 6], v);
+  }
+}
+
+// ------------------------------------------------------------------ small subtrees
+struct SmallStackEnt {
+  uint32_t ls_lc;  // lstart | lcount << 8
+  uint32_t idx, P, depth;
+  float bmn[3], bmx[3];
+};
+struct SmallLDS {
+  float ct[MG_SMALL][3];
+  float mn[MG_SMALL][3];
+  float mx[MG_SMALL][3];
+  uint32_t id[MG_SMALL];
+  uint32_t key[MG_SMALL];
+  uint8_t list[MG_SMALL];
+  uint8_t tmp[MG_SMALL];
+  SmallStackEnt st[MG_SMALL + 2];
+};
+
+__device__ inline void wave_box_reduce(Box& b) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    b.mn[a] = acc_min_dec(wave_max_u32(acc_min_enc(b.mn[a])));
+    b.mx[a] = acc_max_dec(wave_max_u32(acc_max_enc(b.mx[a])));
+  }
+}
+
+// Stable partition of list[ls, ls+lc) by a per-lane predicate (lanes < lc).
+__device__ inline void lds_stable_partition(SmallLDS& S, uint32_t lane, uint32_t ls, uint32_t lc,
+                                            bool pred, uint32_t nl) {
+  const bool in = lane < lc;
+  const uint64_t ml = __ballot(in && pred);
+  const uint64_t mg = __ballot(in && !pred);
+  uint8_t o = in ? S.list[ls + lane] : 0;
+  if (in) {
+    uint32_t p = pred ? lanemask_lt_popc(ml) : nl + lanemask_lt_popc(mg);
+    S.tmp[ls + p] = o;
+  }
+  wave_sync();
+  if (in) S.list[ls + lane] = S.tmp[ls + lane];
+  wave_sync();
+}
+
+// One wavefront per node of <= 32 objects (or a leaf forced by max_depth):
+// split (lib.rs:536-665) with the adaptive sweep (lib.rs:391-442) down to the
+// leaves, all in LDS.
+__global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small, uint32_t nS,
+                                               const Rec* __restrict__ rec0,
+                                               const Rec* __restrict__ rec1, NodeOut out,
+                                               uint32_t* __restrict__ perm, Counters* ctr,
+                                               uint32_t* gaps, uint32_t max_depth) {
+  __shared__ SmallLDS lds[4];
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t lane = lane_id();
+  const uint32_t e = blockIdx.x * 4 + w;
+  if (e >= nS) return;  // only wave-local synchronisation below
+  SmallLDS& S = lds[w];
+  const SNode sn = small[e];
+  const Rec* src = sn.parity ? rec1 : rec0;
+  Box rb;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) { rb.mn[a] = sn.bmn[a]; rb.mx[a] = sn.bmx[a]; }
+
+  if (sn.count == 1 || sn.depth == max_depth || sn.count > MG_SMALL) {
+    if (lane == 0) {
+      out.write_node(sn.idx, rb, sn.start, 0, sn.count);
+      if (sn.count > 1) {
+        uint32_t g = atomicAdd(&ctr->n_gaps, 1u);
+        gaps[2 * g] = sn.P;
+        gaps[2 * g + 1] = 2 * (sn.count - 1);
+        atomicAdd(&ctr->n_cutoff, 1u);
+      }
+    }
+    for (uint32_t j = lane; j < sn.count; j += 64) {
+      Box b;
+      uint32_t id;
+      rec_load(&src[sn.start + j], b, id);
+      perm[sn.start + j] = id;
+    }
+    return;
+  }
+
+  const uint32_t n = sn.count;
+  if (lane < n) {
+    Box b;
+    uint32_t id;
+    rec_load(&src[sn.start + lane], b, id);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      S.mn[lane][a] = b.mn[a];
+      S.mx[lane][a] = b.mx[a];
+      S.ct[lane][a] = center(b.mn[a], b.mx[a]);
+    }
+    S.id[lane] = id;
+    S.list[lane] = (uint8_t)lane;
+  }
+  if (lane == 0) {
+    SmallStackEnt r;
+    r.ls_lc = 0u | (n << 8);
+    r.idx = sn.idx; r.P = sn.P; r.depth = sn.depth;
+    for (int a = 0; a < 3; ++a) { r.bmn[a] = rb.mn[a]; r.bmx[a] = rb.mx[a]; }
+    S.st[0] = r;
+  }
+  wave_sync();
+  int sp = 1;
+  uint32_t hmax = 0;
+  bool any_split = false;
+  while (sp > 0) {
+    --sp;
+    const SmallStackEnt t = S.st[sp];
+    wave_sync();
+    const uint32_t ls = t.ls_lc & 0xff, lc = t.ls_lc >> 8;
+    Box nb;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { nb.mn[a] = t.bmn[a]; nb.mx[a] = t.bmx[a]; }
+    if (lc == 1 || t.depth == max_depth) {
+      if (lane == 0) {
+        out.write_node(t.idx, nb, sn.start + ls, 0, lc);
+        if (lc > 1) {
+          uint32_t g = atomicAdd(&ctr->n_gaps, 1u);
+          gaps[2 * g] = t.P;
+          gaps[2 * g + 1] = 2 * (lc - 1);
+          atomicAdd(&ctr->n_cutoff, 1u);
+        }
+      }
+      if (lane < lc) perm[sn.start + ls + lane] = S.id[S.list[ls + lane]];
+      continue;
+    }
+    // centroid bounds (lib.rs:499-509)
+    Box cb = box_empty();
+    if (lane < lc) {
+      const uint32_t o = S.list[ls + lane];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) { cb.mn[a] = S.ct[o][a]; cb.mx[a] = S.ct[o][a]; }
+    }
+    wave_box_reduce(cb);
+    const float sa_p = box_sa(nb);
+    uint64_t best = ~0ull;
+    for (uint32_t q = lane; q < 3 * lc; q += 64) {
+      const uint32_t a = q / lc, j = q % lc;
+      const float th = lc < 8 ? S.ct[S.list[ls + j]][a]
+                              : adaptive_threshold(cb.mn[a], cb.mx[a], lc, j);
+      Box lt = box_empty(), gt = box_empty();
+      uint32_t nl = 0, ng = 0;
+      for (uint32_t kk = 0; kk < lc; ++kk) {
+        const uint32_t o = S.list[ls + kk];
+        Box ob;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) { ob.mn[x] = S.mn[o][x]; ob.mx[x] = S.mx[o][x]; }
+        if (S.ct[o][a] < th) { box_grow(lt, ob); ++nl; }
+        else { box_grow(gt, ob); ++ng; }
+      }
+      if (nl < 1 || ng < 1) continue;
+      float cost = split_cost(sa_p, box_sa(lt), box_sa(gt), nl, ng);
+      uint64_t key = ((uint64_t)tc_ukey(cost) << 32) | (a * 32 + j);
+      best = key < best ? key : best;
+    }
+    best = wave_min_u64(best);
+
+    Box lt, gt;
+    uint32_t nl, ng, axis, value, kind;
+    float cost;
+    if (best != ~0ull) {
+      const uint32_t q = (uint32_t)best;
+      axis = q / 32;
+      const uint32_t j = q % 32;
+      const float th = lc < 8 ? S.ct[S.list[ls + j]][axis]
+                              : adaptive_threshold(cb.mn[axis], cb.mx[axis], lc, j);
+      lt = box_empty();
+      gt = box_empty();
+      bool pred = false;
+      if (lane < lc) {
+        const uint32_t o = S.list[ls + lane];
+        pred = S.ct[o][axis] < th;
+        Box ob;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) { ob.mn[x] = S.mn[o][x]; ob.mx[x] = S.mx[o][x]; }
+        if (pred) box_grow(lt, ob);
+        else box_grow(gt, ob);
+      }
+      wave_box_reduce(lt);
+      wave_box_reduce(gt);
+      nl = (uint32_t)__popcll(__ballot(lane < lc && pred));
+      ng = lc - nl;
+      cost = split_cost(sa_p, box_sa(lt), box_sa(gt), nl, ng);
+      value = fbits(th);
+      kind = 1;
+      lds_stable_partition(S, lane, ls, lc, pred, nl);  // partition_in_place, lib.rs:598-600
+    } else {
+      // median fallback on axes 0,1,2, each reordering the list (lib.rs:527-531);
+      // the children take the winning axis' boxes, the list keeps the axis-2 order.
+      const uint32_t m = lc / 2;
+      bool have = false;
+      uint32_t bkey = 0;
+      axis = 0;
+      for (uint32_t a = 0; a < 3; ++a) {
+        if (lane < lc) S.key[lane] = tc_ukey(S.ct[S.list[ls + lane]][a]);
+        wave_sync();
+        bool sel = false;
+        if (lane < lc) {
+          const uint32_t kj = S.key[lane];
+          uint32_t rank = 0;
+          for (uint32_t i2 = 0; i2 < lc; ++i2) {
+            const uint32_t ki = S.key[i2];
+            rank += (ki < kj) || (ki == kj && i2 < lane);
+          }
+          sel = rank < m;
+        }
+        wave_sync();
+        lds_stable_partition(S, lane, ls, lc, sel, m);
+        Box l2 = box_empty(), g2 = box_empty();
+        if (lane < lc) {
+          const uint32_t o = S.list[ls + lane];
+          Box ob;
+#pragma unroll
+          for (int x = 0; x < 3; ++x) { ob.mn[x] = S.mn[o][x]; ob.mx[x] = S.mx[o][x]; }
+          if (lane < m) box_grow(l2, ob);
+          else box_grow(g2, ob);
+        }
+        wave_box_reduce(l2);
+        wave_box_reduce(g2);
+        const float c2 = split_cost(sa_p, box_sa(l2), box_sa(g2), m, lc - m);
+        const uint32_t k2 = tc_ukey(c2);
+        if (!have || k2 < bkey) {
+          have = true;
+          bkey = k2;
+          axis = a;
+          lt = l2;
+          gt = g2;
+          cost = c2;
+        }
+      }
+      nl = m;
+      ng = lc - m;
+      value = m;
+      kind = 2;
+    }
+    if (lane == 0) {
+      out.write_node(t.idx, nb, sn.start + ls, t.P, lc);
+      out.write_split(t.idx, axis | (kind << 8), value, nl, fbits(cost));
+      if (kind == 2) atomicAdd(&ctr->n_median, 1u);
+    }
+    hmax = max(hmax, t.depth);
+    any_split = true;
+    if (lane == 0) {
+      SmallStackEnt c;
+      // lt child first on the stack, gt popped first (order does not matter)
+      c.ls_lc = ls | (nl << 8);
+      c.idx = t.P + 1; c.P = t.P + 2 + 2 * (ng - 1); c.depth = t.depth + 1;
+      for (int a = 0; a < 3; ++a) { c.bmn[a] = lt.mn[a]; c.bmx[a] = lt.mx[a]; }
+      S.st[sp] = c;
+      c.ls_lc = (ls + nl) | (ng << 8);
+      c.idx = t.P; c.P = t.P + 2; c.depth = t.depth + 1;
+      for (int a = 0; a < 3; ++a) { c.bmn[a] = gt.mn[a]; c.bmx[a] = gt.mx[a]; }
+      S.st[sp + 1] = c;
+    }
+    sp += 2;
+    wave_sync();
+  }
+  if (lane == 0 && any_split) atomicMax(&ctr->height, hmax);
+}
+
+// ------------------------------------------------------------------ compaction
+// Only after max_depth cut subtrees short: slot i -> i - (reserved slots before i).
+__device__ inline uint32_t gap_shift(const uint32_t* gstart, const uint32_t* gcum, uint32_t ng,
+                                     uint32_t i, bool& inside, const uint32_t* glen) {
+  // number of gaps with start <= i
+  uint32_t lo = 0, hi = ng;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (gstart[mid] <= i) lo = mid + 1;
+    else hi = mid;
+  }
+  inside = lo > 0 && i < gstart[lo - 1] + glen[lo - 1];
+  return lo == 0 ? 0 : gcum[lo - 1];
+}
+
+__global__ void k_compact(const mgbvh_node32* __restrict__ nin, const uint32_t* __restrict__ lin,
+                          const mgbvh_split_rec* __restrict__ sin, mgbvh_node32* nout,
+                          uint32_t* lout, mgbvh_split_rec* sout, uint32_t nslots,
+                          const uint32_t* gstart, const uint32_t* glen, const uint32_t* gcum,
+                          uint32_t ng) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  bool inside;
+  const uint32_t sh = gap_shift(gstart, gcum, ng, i, inside, glen);
+  if (inside) return;
+  const uint32_t ni = i - sh;
+  mgbvh_node32 nd = nin[i];
+  if (nd.child_node != 0) {
+    bool in2;
+    nd.child_node -= gap_shift(gstart, gcum, ng, nd.child_node, in2, glen);
+  }
+  nout[ni] = nd;
+  if (lout) lout[ni] = lin[i];
+  if (sout) sout[ni] = sin[i];
+}
+
+__global__ void k_permute_tris(const uint32_t* __restrict__ tin, const uint32_t* __restrict__ perm,
+                               uint32_t* __restrict__ tout, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t s = perm[i];
+  tout[3 * (size_t)i + 0] = tin[3 * (size_t)s + 0];
+  tout[3 * (size_t)i + 1] = tin[3 * (size_t)s + 1];
+  tout[3 * (size_t)i + 2] = tin[3 * (size_t)s + 2];
+}
+
+}  // namespace mg
+
+// ====================================================================== host
+#include <algorithm>
+#include <vector>
+
+namespace mg {
+
+#define MG_HIP(call)                                   \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);  \
+  } while (0)
+
+#define MG_PROF(kind, end)                                                   \
+  do {                                                                       \
+    if (c->prof_mask & (1u << (kind))) {                                     \
+      int r_ = (end) ? prof_end(c, (kind)) : prof_begin(c, (kind));         \
+      if (r_) return r_;                                                     \
+    }                                                                        \
+  } while (0)
+
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+int build_device(mgbvh_ctx* c, const BuildInput& in, uint32_t n, const mgbvh_settings* s,
+                 const BuildOutput& o, BuildResult* res) {
+  *res = BuildResult{};
+  if (n == 0) return MGBVH_OK;  // lib.rs:703-708: empty BVH
+  hipStream_t st = c->stream;
+  const uint32_t capL = n / (MG_SMALL + 1) + 2;
+  const uint32_t capC = cdiv(n, CH) + capL + 2;
+  const uint64_t nslots = 2ull * n - 1;
+  int rc;
+  if ((rc = ensure(c->recs[0], (size_t)n * sizeof(Rec)))) return rc;
+  if ((rc = ensure(c->recs[1], (size_t)n * sizeof(Rec)))) return rc;
+  for (int p = 0; p < 2; ++p) {
+    if ((rc = ensure(c->L[p], (size_t)capL * sizeof(LNode)))) return rc;
+    if ((rc = ensure(c->cn[p], (size_t)capC * 4))) return rc;
+    if ((rc = ensure(c->ck[p], (size_t)capC * 4))) return rc;
+  }
+  if ((rc = ensure(c->hist, (size_t)capL * HIST_WORDS * 4))) return rc;
+  if ((rc = ensure(c->blk, (size_t)capC * 96 * 4))) return rc;
+  if ((rc = ensure(c->small, (size_t)n * sizeof(SNode)))) return rc;
+  if ((rc = ensure(c->ctr, sizeof(Counters)))) return rc;
+  if ((rc = ensure(c->gaps, ((size_t)n / 2 + 1) * 8))) return rc;
+  uint32_t* lens = o.lens;
+  if (!lens) {
+    if ((rc = ensure(c->lens, nslots * 4))) return rc;
+    lens = (uint32_t*)c->lens.p;
+  }
+  Rec* recs[2] = {(Rec*)c->recs[0].p, (Rec*)c->recs[1].p};
+  LNode* L[2] = {(LNode*)c->L[0].p, (LNode*)c->L[1].p};
+  uint32_t* cn[2] = {(uint32_t*)c->cn[0].p, (uint32_t*)c->cn[1].p};
+  uint32_t* ck[2] = {(uint32_t*)c->ck[0].p, (uint32_t*)c->ck[1].p};
+  Counters* ctr = (Counters*)c->ctr.p;
+  SNode* small = (SNode*)c->small.p;
+  uint32_t* hist = (uint32_t*)c->hist.p;
+  uint32_t* blk = (uint32_t*)c->blk.p;
+  uint32_t* gaps = (uint32_t*)c->gaps.p;
+  NodeOut out{o.nodes, lens, o.splits};
+
+  MG_HIP(hipEventRecord(c->ev[62], st));
+  MG_HIP(hipMemsetAsync(ctr, 0, sizeof(Counters), st));
+  if (o.splits) MG_HIP(hipMemsetAsync(o.splits, 0xff, nslots * sizeof(mgbvh_split_rec), st));
+
+  const uint32_t gsetup = std::min<uint32_t>(cdiv(n, 256), 1024);
+  MG_PROF(KK_SETUP, 0);
+  if (in.kind == 0)
+    k_setup_tris<<<gsetup, 256, 0, st>>>(in.pos, in.vstride_floats, in.nverts, in.tris, n,
+                                         recs[0], ctr);
+  else
+    k_setup_aabbs<<<gsetup, 256, 0, st>>>(in.boxes, n, recs[0], ctr);
+  MG_HIP(hipGetLastError());
+  MG_PROF(KK_SETUP, 1);
+  const int large = n > MG_SMALL && s->max_depth > 0;
+  k_root<<<1, 256, 0, st>>>(ctr, L[0], cn[0], ck[0], small, n, s->has_bounds, s->bounds_min[0],
+                            s->bounds_min[1], s->bounds_min[2], s->bounds_max[0],
+                            s->bounds_max[1], s->bounds_max[2], large);
+  MG_HIP(hipGetLastError());
+
+  uint32_t nL = large ? 1 : 0, nC = large ? cdiv(n, CH) : 0;
+  int cur = 0;
+  uint32_t levels = 0;
+  while (nL > 0) {
+    MG_HIP(hipMemsetAsync(hist, 0, (size_t)nL * HIST_WORDS * 4, st));
+    MG_HIP(hipMemsetAsync(ctr, 0, 8, st));  // n_large_next, n_chunks_next
+    MG_PROF(KK_BIN, 0);
+    k_bin<<<nC, 256, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk);
+    MG_PROF(KK_BIN, 1);
+    MG_PROF(KK_SWEEP, 0);
+    k_sweep<<<cdiv(nL, 4), 256, 0, st>>>(L[cur], nL, hist, recs[cur], L[cur ^ 1], cn[cur ^ 1],
+                                         ck[cur ^ 1], small, ctr, out, s->max_depth,
+                                         (uint32_t)(cur ^ 1));
+    MG_PROF(KK_SWEEP, 1);
+    MG_PROF(KK_PARTITION, 0);
+    k_partition<<<nC, 256, 0, st>>>(L[cur], cn[cur], ck[cur], blk, recs[cur], recs[cur ^ 1],
+                                    L[cur ^ 1]);
+    MG_PROF(KK_PARTITION, 1);
+    MG_HIP(hipGetLastError());
+    MG_HIP(hipMemcpyAsync(c->h_ctr, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
+    MG_HIP(hipStreamSynchronize(st));
+    if (c->h_ctr->err) break;
+    nL = c->h_ctr->n_large_next;
+    nC = c->h_ctr->n_chunks_next;
+    cur ^= 1;
+    ++levels;
+    if (levels > 4096) { set_error("level loop did not terminate"); return MGBVH_ERR_INTERNAL; }
+  }
+  MG_HIP(hipMemcpyAsync(c->h_ctr, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
+  MG_HIP(hipStreamSynchronize(st));
+  if (c->h_ctr->err & ERR_INDEX) { set_error("triangle index out of range"); return MGBVH_ERR_INDEX; }
+  if (c->h_ctr->err & ERR_NONFINITE) {
+    set_error("non-finite or out-of-range (|x| >= 2^60) input coordinate");
+    return MGBVH_ERR_NONFINITE;
+  }
+  const uint32_t nS = c->h_ctr->n_small;
+  if (nS) {
+    MG_PROF(KK_SMALL, 0);
+    k_small<<<cdiv(nS, 4), 256, 0, st>>>(small, nS, recs[0], recs[1], out, o.perm, ctr, gaps,
+                                         s->max_depth);
+    MG_PROF(KK_SMALL, 1);
+    MG_HIP(hipGetLastError());
+  }
+  MG_HIP(hipMemcpyAsync(c->h_ctr, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
+  MG_HIP(hipStreamSynchronize(st));
+  const Counters hc = *c->h_ctr;
+  uint64_t removed = 0;
+  if (hc.n_gaps) {
+    std::vector<uint32_t> g(2 * (size_t)hc.n_gaps);
+    MG_HIP(hipMemcpy(g.data(), gaps, g.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<std::pair<uint32_t, uint32_t>> gp(hc.n_gaps);
+    for (uint32_t i = 0; i < hc.n_gaps; ++i) gp[i] = {g[2 * i], g[2 * i + 1]};
+    std::sort(gp.begin(), gp.end());
+    std::vector<uint32_t> tab(3 * (size_t)hc.n_gaps);
+    uint64_t cum = 0;
+    for (uint32_t i = 0; i < hc.n_gaps; ++i) {
+      cum += gp[i].second;
+      tab[i] = gp[i].first;
+      tab[hc.n_gaps + i] = gp[i].second;
+      tab[2 * hc.n_gaps + i] = (uint32_t)cum;
+    }
+    removed = cum;
+    if ((rc = ensure(c->gtab, tab.size() * 4))) return rc;
+    if ((rc = ensure(c->tmpn, nslots * sizeof(mgbvh_node32)))) return rc;
+    if ((rc = ensure(c->tmpl, nslots * 4))) return rc;
+    if (o.splits && (rc = ensure(c->tmps, nslots * sizeof(mgbvh_split_rec)))) return rc;
+    MG_HIP(hipMemcpyAsync(c->gtab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, st));
+    const uint32_t* gt = (const uint32_t*)c->gtab.p;
+    k_compact<<<cdiv(nslots, 256), 256, 0, st>>>(
+        o.nodes, lens, o.splits, (mgbvh_node32*)c->tmpn.p, (uint32_t*)c->tmpl.p,
+        o.splits ? (mgbvh_split_rec*)c->tmps.p : nullptr, (uint32_t)nslots, gt, gt + hc.n_gaps,
+        gt + 2 * hc.n_gaps, hc.n_gaps);
+    MG_HIP(hipGetLastError());
+    const uint64_t nn = nslots - removed;
+    MG_HIP(hipMemcpyAsync(o.nodes, c->tmpn.p, nn * sizeof(mgbvh_node32), hipMemcpyDeviceToDevice,
+                          st));
+    MG_HIP(hipMemcpyAsync(lens, c->tmpl.p, nn * 4, hipMemcpyDeviceToDevice, st));
+    if (o.splits)
+      MG_HIP(hipMemcpyAsync(o.splits, c->tmps.p, nn * sizeof(mgbvh_split_rec),
+                            hipMemcpyDeviceToDevice, st));
+  }
+  if (o.tris_out && in.kind == 0) {
+    MG_PROF(KK_PERMUTE, 0);
+    k_permute_tris<<<cdiv(n, 256), 256, 0, st>>>(in.tris, o.perm, o.tris_out, n);
+    MG_PROF(KK_PERMUTE, 1);
+    MG_HIP(hipGetLastError());
+  }
+  MG_HIP(hipEventRecord(c->ev[63], st));
+  MG_HIP(hipEventSynchronize(c->ev[63]));
+  float ms = 0.0f;
+  MG_HIP(hipEventElapsedTime(&ms, c->ev[62], c->ev[63]));
+  if ((rc = prof_collect(c))) return rc;
+  res->nnodes = nslots - removed;
+  res->height = hc.height;
+  res->n_levels = levels;
+  res->n_median = hc.n_median;
+  res->n_cutoff = hc.n_cutoff;
+  res->ms = ms;
+  return MGBVH_OK;
+}
+
+}  // namespace mg

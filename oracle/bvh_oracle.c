@@ -1,0 +1,607 @@
+/*
+ * CPU oracle for the marigold `bvh` crate -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Op-for-op C restatement of `bvh/src/lib.rs` (the reference is Rust and cannot
+ * be compiled in this image: no cargo/rustc, crates not vendored).  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load this file's
+ * library, and only as the checker / CPU baseline, never as the product.
+ *
+ * Semantics pinned to the reference (file:line in bvh/src/lib.rs unless noted):
+ *   AABB algebra .................. :61-111
+ *   binned sweep (32 bins) ........ :331-389, cost :255-290
+ *   adaptive threshold sweep ...... :391-442, cost :292-329
+ *   median fallback ............... :444-491
+ *   select_split .................. :493-534
+ *   split / node order ............ :536-665
+ *   new() ......................... :698-878
+ *
+ * Deliberate, documented choices where Rust's behaviour is not pinned:
+ *   - `select_nth_unstable_by` and `partition_in_place` leave an unspecified
+ *     order inside each side.  The oracle uses the canonical STABLE variant
+ *     (ties by current list position).  Choices that depend on that order are
+ *     counted in or_stats.n_tie_events so tests can assert they never happen.
+ *   - glam min/max on +-0 is operand-order dependent; triangle AABBs map -0.0
+ *     to +0.0 so that every min/max fold is order independent.
+ *   - NaN costs follow x86-64 SSE: an invalid operation yields the default
+ *     NaN 0xFFC00000, which f32::total_cmp orders below -inf.
+ */
+#include "bvh_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define DEPTH_COST 1.0f  /* lib.rs:217 */
+#define OBJECT_COST 8.0f /* lib.rs:218 */
+
+static inline float f_min(float a, float b) { return a < b ? a : b; } /* minps */
+static inline float f_max(float a, float b) { return a > b ? a : b; } /* maxps */
+
+static inline uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float bitsf(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+/* f32::total_cmp key: i ^ (((i >> 31) as u32) >> 1) as i32, compared signed. */
+int32_t or_total_cmp_key(float f) {
+  int32_t i = (int32_t)fbits(f);
+  return i ^ (int32_t)(((uint32_t)(i >> 31)) >> 1);
+}
+
+static inline float x86_nan_canon(float f) { return f != f ? bitsf(0xFFC00000u) : f; }
+
+static const or_bv BV_EMPTY = {{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}};
+
+/* lib.rs:71-73 */
+static inline float bv_center(const or_bv* b, int a) { return (b->mn[a] + b->mx[a]) * 0.5f; }
+/* lib.rs:75-77 */
+static inline float bv_extent(const or_bv* b, int a) { return b->mx[a] - b->mn[a]; }
+/* lib.rs:103-106 */
+static inline int bv_is_empty(const or_bv* b) {
+  return (b->mx[0] < b->mn[0]) | (b->mx[1] < b->mn[1]) | (b->mx[2] < b->mn[2]);
+}
+/* lib.rs:79-91: 2 * ((w*h + w*d) + h*d), no FMA. */
+static float bv_sa(const or_bv* b) {
+  float w = b->mx[0] - b->mn[0];
+  float h = b->mx[1] - b->mn[1];
+  float d = b->mx[2] - b->mn[2];
+  if (!bv_is_empty(b)) {
+    float wh = w * h;
+    float wd = w * d;
+    float hd = h * d;
+    float s = wh + wd;
+    s = s + hd;
+    return 2.0f * s;
+  }
+  return 0.0f;
+}
+/* lib.rs:98-101: self.min = self.min.min(other.min) ... */
+static inline void bv_grow(or_bv* acc, const or_bv* o) {
+  for (int a = 0; a < 3; ++a) {
+    acc->mn[a] = f_min(acc->mn[a], o->mn[a]);
+    acc->mx[a] = f_max(acc->mx[a], o->mx[a]);
+  }
+}
+/* lib.rs:108-110 */
+static int bv_contains(const or_bv* b, const float* p) {
+  if (bv_is_empty(b)) return 0;
+  for (int a = 0; a < 3; ++a)
+    if (!(p[a] >= b->mn[a]) || !(p[a] <= b->mx[a])) return 0;
+  return 1;
+}
+
+float or_surface_area(const float* bv6) {
+  or_bv b;
+  memcpy(&b, bv6, sizeof b);
+  return bv_sa(&b);
+}
+
+/* ((SA_lt / SA_p) * 8) * n_lt ; cost = (1 + lt) + gt   (lib.rs:278-288) */
+static float split_cost(const or_bv* parent, const or_bv* lt, const or_bv* gt, uint32_t n_lt,
+                        uint32_t n_gt) {
+  float sp = bv_sa(parent);
+  float lt_cost = bv_sa(lt) / sp;
+  lt_cost = lt_cost * OBJECT_COST;
+  lt_cost = lt_cost * (float)n_lt;
+  float gt_cost = bv_sa(gt) / sp;
+  gt_cost = gt_cost * OBJECT_COST;
+  gt_cost = gt_cost * (float)n_gt;
+  float c = DEPTH_COST + lt_cost;
+  c = c + gt_cost;
+  return x86_nan_canon(c);
+}
+
+float or_binned_cost(const float* p6, const float* l6, const float* g6, uint32_t n_lt,
+                     uint32_t n_gt) {
+  or_bv p, l, g;
+  memcpy(&p, p6, sizeof p);
+  memcpy(&l, l6, sizeof l);
+  memcpy(&g, g6, sizeof g);
+  return split_cost(&p, &l, &g, n_lt, n_gt);
+}
+
+/* `(x).floor() as usize` saturating cast, then .min(B-1)   (lib.rs:358-362) */
+uint32_t or_bin_index(float c, float cmin, float cext, uint32_t nbins) {
+  float pct = (c - cmin) / cext;
+  float f = floorf(pct * (float)nbins);
+  uint64_t bi;
+  if (!(f > 0.0f)) bi = 0; /* NaN and negatives saturate to 0 */
+  else if (f >= 18446744073709551616.0f) bi = UINT64_MAX;
+  else bi = (uint64_t)f;
+  if (bi > nbins - 1) bi = nbins - 1;
+  return (uint32_t)bi;
+}
+
+/* ---------------------------------------------------------------- AABBs */
+
+static inline float canon_zero(float x) { return x == 0.0f ? 0.0f : x; }
+
+void or_tri_aabbs(const float* pos, uint32_t vs, const uint32_t* tris, uint64_t n, float* out) {
+  for (uint64_t t = 0; t < n; ++t) {
+    const float* v1 = pos + (uint64_t)tris[3 * t + 0] * vs;
+    const float* v2 = pos + (uint64_t)tris[3 * t + 1] * vs;
+    const float* v3 = pos + (uint64_t)tris[3 * t + 2] * vs;
+    for (int a = 0; a < 3; ++a) {
+      /* v1.min(v2).min(v3), mesh_interface/src/lib.rs:25-26 */
+      float mn = f_min(f_min(v1[a], v2[a]), v3[a]);
+      float mx = f_max(f_max(v1[a], v2[a]), v3[a]);
+      out[6 * t + a] = canon_zero(mn);
+      out[6 * t + 3 + a] = canon_zero(mx);
+    }
+  }
+}
+
+/* ---------------------------------------------------------------- builder */
+
+typedef struct {
+  const or_bv* aabb; /* per object id */
+  uint32_t* list;    /* object ids */
+  or_node* nodes;
+  uint32_t nnodes;
+  or_rec* recs;
+  or_stats* st;
+  uint32_t min_leaf, max_leaf, max_depth;
+  uint32_t* tmp;     /* scratch, n entries */
+  uint64_t* sortkey; /* scratch, n entries */
+} ctx_t;
+
+typedef struct {
+  or_bv blt, bgt;
+  uint32_t n_lt, n_gt;
+  float cost;
+  int kind; /* 0 = Index (binned), 1 = Threshold (adaptive), 2 = Exact (median) */
+  int axis;
+  uint32_t value; /* bin cut i / threshold bits / median index */
+  float threshold;
+  uint32_t index;
+} split_t;
+
+static inline void obj_center(const ctx_t* c, uint32_t id, float* ctr) {
+  const or_bv* b = &c->aabb[id];
+  for (int a = 0; a < 3; ++a) ctr[a] = bv_center(b, a);
+}
+
+static inline int better(const split_t* cand, const split_t* best, int have_best) {
+  /* Iterator::min_by keeps the first of equal minima (strictly less replaces). */
+  return !have_best || or_total_cmp_key(cand->cost) < or_total_cmp_key(best->cost);
+}
+
+/* evaluate_binned_split, lib.rs:255-290 */
+static void eval_binned(const or_bv* parent, const uint32_t* cnt, const or_bv* bb, uint32_t B,
+                        uint32_t i, split_t* s) {
+  s->blt = BV_EMPTY;
+  s->bgt = BV_EMPTY;
+  uint32_t nl = 0, ng = 0;
+  for (uint32_t k = 0; k < i; ++k) { nl += cnt[k]; bv_grow(&s->blt, &bb[k]); }
+  for (uint32_t k = i; k < B; ++k) { ng += cnt[k]; bv_grow(&s->bgt, &bb[k]); }
+  s->n_lt = nl;
+  s->n_gt = ng;
+  s->cost = split_cost(parent, &s->blt, &s->bgt, nl, ng);
+}
+
+/* binned_sweep, lib.rs:331-389 */
+static int binned_sweep(ctx_t* c, const or_bv* parent, const or_bv* cb, const uint32_t* list,
+                        uint32_t n, int axis, split_t* out) {
+  if (bv_extent(cb, axis) == 0.0f) return 0;
+  uint32_t B = n < 16 ? 16 : (n > 32 ? 32 : n);
+  uint32_t cnt[32];
+  or_bv bb[32];
+  for (uint32_t k = 0; k < B; ++k) { cnt[k] = 0; bb[k] = BV_EMPTY; }
+  for (uint32_t k = 0; k < n; ++k) {
+    const or_bv* ob = &c->aabb[list[k]];
+    float ctr = bv_center(ob, axis);
+    uint32_t bi = or_bin_index(ctr, cb->mn[axis], bv_extent(cb, axis), B);
+    cnt[bi] += 1;
+    bv_grow(&bb[bi], ob);
+  }
+  int have = 0;
+  for (uint32_t i = 1; i < B; ++i) {
+    split_t s;
+    eval_binned(parent, cnt, bb, B, i, &s);
+    if (s.n_lt < c->min_leaf || s.n_gt < c->min_leaf) continue;
+    s.kind = 0;
+    s.axis = axis;
+    s.value = i;
+    s.index = s.n_lt;
+    if (better(&s, out, have)) { *out = s; have = 1; }
+  }
+  return have;
+}
+
+/* evaluate_threshold_split, lib.rs:292-329 */
+static void eval_threshold(const ctx_t* c, const or_bv* parent, const uint32_t* list, uint32_t n,
+                           int axis, float t, split_t* s) {
+  s->blt = BV_EMPTY;
+  s->bgt = BV_EMPTY;
+  uint32_t nl = 0, ng = 0;
+  for (uint32_t k = 0; k < n; ++k) {
+    const or_bv* ob = &c->aabb[list[k]];
+    if (bv_center(ob, axis) < t) { bv_grow(&s->blt, ob); nl++; }
+    else { bv_grow(&s->bgt, ob); ng++; }
+  }
+  s->n_lt = nl;
+  s->n_gt = ng;
+  s->cost = split_cost(parent, &s->blt, &s->bgt, nl, ng);
+}
+
+/* adaptive_sweep, lib.rs:391-442.  *tie is set when the first-minimum choice
+ * depended on list order (an equal-cost candidate with a different partition). */
+static int adaptive_sweep(ctx_t* c, const or_bv* parent, const or_bv* cb, const uint32_t* list,
+                          uint32_t n, int axis, split_t* out, int* tie) {
+  const uint32_t accurate = 8;
+  int have = 0;
+  split_t cands[32];
+  uint32_t nc = 0;
+  if (n < accurate) {
+    for (uint32_t k = 0; k < n; ++k) {
+      float t = bv_center(&c->aabb[list[k]], axis);
+      split_t s;
+      eval_threshold(c, parent, list, n, axis, t, &s);
+      if (s.n_lt < c->min_leaf || s.n_gt < c->min_leaf) continue;
+      s.kind = 1; s.axis = axis; s.threshold = t; s.value = fbits(t);
+      cands[nc++] = s;
+      if (better(&s, out, have)) { *out = s; have = 1; }
+    }
+    if (have) {
+      for (uint32_t k = 0; k < nc; ++k)
+        if (fbits(cands[k].cost) == fbits(out->cost) && cands[k].n_lt != out->n_lt) *tie = 1;
+    }
+  } else {
+    uint32_t steps = n < accurate ? accurate : (n > 32 ? 32 : n);
+    float step = bv_extent(cb, axis) / (float)steps;
+    for (uint32_t i = 0; i < steps; ++i) {
+      float fi = (float)i + 0.5f;
+      float off = step * fi;
+      float t = cb->mn[axis] + off;
+      split_t s;
+      eval_threshold(c, parent, list, n, axis, t, &s);
+      if (s.n_lt < c->min_leaf || s.n_gt < c->min_leaf) continue;
+      s.kind = 1; s.axis = axis; s.threshold = t; s.value = fbits(t);
+      if (better(&s, out, have)) { *out = s; have = 1; }
+    }
+  }
+  return have;
+}
+
+static int cmp_u64(const void* a, const void* b) {
+  uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* Canonical stable select_nth_unstable_by(m, total_cmp(center[axis])):
+ * the m smallest (ties by current position) go first, both sides keep their
+ * relative order.  Returns 1 when keys tie across the m boundary. */
+static int stable_select(ctx_t* c, uint32_t* list, uint32_t n, uint32_t m, int axis) {
+  if (n == 0) return 0;
+  for (uint32_t k = 0; k < n; ++k) {
+    float ctr = bv_center(&c->aabb[list[k]], axis);
+    uint32_t key = (uint32_t)or_total_cmp_key(ctr) ^ 0x80000000u;
+    c->sortkey[k] = ((uint64_t)key << 32) | k;
+  }
+  qsort(c->sortkey, n, sizeof(uint64_t), cmp_u64);
+  int tie = 0;
+  if (m > 0 && m < n && (c->sortkey[m - 1] >> 32) == (c->sortkey[m] >> 32)) {
+    /* A tie across the boundary matters only if the tied objects' boxes differ;
+     * identical boxes give the same children whichever goes left. */
+    uint32_t kb = (uint32_t)(c->sortkey[m] >> 32);
+    const or_bv* ref = NULL;
+    tie = 2; /* benign unless proven otherwise */
+    for (uint32_t k = 0; k < n; ++k) {
+      if ((uint32_t)(c->sortkey[k] >> 32) != kb) continue;
+      const or_bv* b = &c->aabb[list[(uint32_t)(c->sortkey[k] & 0xffffffffu)]];
+      if (!ref) ref = b;
+      else if (memcmp(ref, b, sizeof *b) != 0) { tie = 1; break; }
+    }
+  }
+  /* mark: tmp[pos] = 1 if selected */
+  for (uint32_t k = 0; k < n; ++k) c->tmp[k] = 0;
+  for (uint32_t k = 0; k < m; ++k) c->tmp[(uint32_t)(c->sortkey[k] & 0xffffffffu)] = 1;
+  /* stable partition into sortkey (reuse as id buffer) */
+  uint32_t w = 0;
+  uint32_t* ids = (uint32_t*)c->sortkey; /* n u64 entries hold >= n u32 ids */
+  uint32_t* flags = c->tmp;
+  uint32_t* out2 = ids + n;              /* second half of the u64 scratch */
+  for (uint32_t k = 0; k < n; ++k) out2[k] = flags[k];
+  for (uint32_t k = 0; k < n; ++k) if (out2[k]) ids[w++] = list[k];
+  for (uint32_t k = 0; k < n; ++k) if (!out2[k]) ids[w++] = list[k];
+  memcpy(list, ids, n * sizeof(uint32_t));
+  return tie;
+}
+
+/* median_split, lib.rs:444-491 (mutates the list) */
+static void median_split(ctx_t* c, const or_bv* parent, uint32_t* list, uint32_t n, int axis,
+                         split_t* s, int* tie) {
+  uint32_t m = n / 2;
+  int t = stable_select(c, list, n, m, axis);
+  if (t == 1) *tie = 1;
+  else if (t == 2 && *tie == 0) *tie = 2;
+  s->blt = BV_EMPTY;
+  s->bgt = BV_EMPTY;
+  for (uint32_t k = 0; k < m; ++k) bv_grow(&s->blt, &c->aabb[list[k]]);
+  for (uint32_t k = m; k < n; ++k) bv_grow(&s->bgt, &c->aabb[list[k]]);
+  s->n_lt = m;
+  s->n_gt = n - m;
+  /* lib.rs:470-482: the lengths come from the slices */
+  s->cost = split_cost(parent, &s->blt, &s->bgt, m, n - m);
+  s->kind = 2;
+  s->axis = axis;
+  s->value = m;
+  s->index = m;
+}
+
+/* select_split, lib.rs:493-534 */
+static int select_split(ctx_t* c, const or_bv* parent, uint32_t* list, uint32_t n, split_t* best,
+                        int* median_quirk) {
+  or_bv cb;
+  for (int a = 0; a < 3; ++a) { cb.mn[a] = INFINITY; cb.mx[a] = -INFINITY; }
+  for (uint32_t k = 0; k < n; ++k) {
+    float ctr[3];
+    obj_center(c, list[k], ctr);
+    for (int a = 0; a < 3; ++a) {
+      cb.mn[a] = f_min(cb.mn[a], ctr[a]);
+      cb.mx[a] = f_max(cb.mx[a], ctr[a]);
+    }
+  }
+  int have = 0;
+  int tie_axis[3] = {0, 0, 0};
+  for (int axis = 0; axis < 3; ++axis) {
+    split_t s;
+    int ok;
+    if (n <= 32) ok = adaptive_sweep(c, parent, &cb, list, n, axis, &s, &tie_axis[axis]);
+    else ok = binned_sweep(c, parent, &cb, list, n, axis, &s);
+    if (ok && better(&s, best, have)) { *best = s; have = 1; }
+  }
+  if (have && best->kind == 1 && tie_axis[best->axis]) c->st->n_tie_events++;
+  *median_quirk = 0;
+  if (!have && n > c->max_leaf) {
+    /* each call reorders the list; the final list is the axis-2 one (R11) */
+    uint32_t* memb[3];
+    uint32_t* snap = (uint32_t*)malloc(sizeof(uint32_t) * 3 * (size_t)n);
+    int tie = 0;
+    for (int axis = 0; axis < 3; ++axis) {
+      split_t s;
+      median_split(c, parent, list, n, axis, &s, &tie);
+      memb[axis] = snap + (size_t)axis * n;
+      memcpy(memb[axis], list, sizeof(uint32_t) * n);
+      if (better(&s, best, have)) { *best = s; have = 1; }
+    }
+    if (tie == 1) c->st->n_tie_events++;
+    else if (tie == 2) c->st->n_tie_benign++;
+    /* Does the winning axis' lt set differ from the final (axis 2) one? */
+    if (best->axis != 2) {
+      uint32_t m = n / 2;
+      /* sets differ iff some element of memb[best][0..m) is not in memb[2][0..m) */
+      for (uint32_t k = 0; k < n; ++k) c->tmp[k] = 0;
+      /* map ids -> mark via sort of copies */
+      uint32_t* a = (uint32_t*)malloc(sizeof(uint32_t) * 2 * (size_t)m + 1);
+      uint32_t* b2 = a + m;
+      memcpy(a, memb[best->axis], m * sizeof(uint32_t));
+      memcpy(b2, memb[2], m * sizeof(uint32_t));
+      for (uint32_t k = 0; k < m; ++k) { c->sortkey[k] = a[k]; }
+      qsort(c->sortkey, m, sizeof(uint64_t), cmp_u64);
+      for (uint32_t k = 0; k < m; ++k) a[k] = (uint32_t)c->sortkey[k];
+      for (uint32_t k = 0; k < m; ++k) { c->sortkey[k] = b2[k]; }
+      qsort(c->sortkey, m, sizeof(uint64_t), cmp_u64);
+      for (uint32_t k = 0; k < m; ++k)
+        if (a[k] != (uint32_t)c->sortkey[k]) { *median_quirk = 1; break; }
+      free(a);
+    }
+    free(snap);
+  }
+  return have;
+}
+
+static inline int u32_lt_threshold(const ctx_t* c, uint32_t id, int axis, float t) {
+  return bv_center(&c->aabb[id], axis) < t;
+}
+
+/* split, lib.rs:536-665.  `self` lives at nodes[self_idx]. */
+static void split_node(ctx_t* c, uint32_t self_idx, uint32_t depth) {
+  or_node self = c->nodes[self_idx];
+  if (self.len <= 1) return;
+  if (depth == c->max_depth || (self.len <= c->max_leaf && self.len < c->min_leaf * 2)) {
+    if (depth == c->max_depth && self.len > 1) c->st->n_cutoff_leaves++;
+    return;
+  }
+  or_node child_gt = {BV_EMPTY, self.start, 0, 0};
+  or_node child_lt = {BV_EMPTY, self.start, 0, 0};
+  uint32_t* list = c->list + self.start;
+  uint32_t n = self.len;
+
+  split_t sp;
+  int quirk = 0;
+  if (!select_split(c, &self.b, list, n, &sp, &quirk)) { c->st->n_refused++; return; }
+  if (self.len <= c->max_leaf && sp.cost >= OBJECT_COST * (float)self.len) {
+    c->st->n_cost_leaf++;
+    return;
+  }
+  if (quirk) c->st->n_median_quirk++;
+
+  child_lt.b = sp.blt;
+  child_gt.b = sp.bgt;
+  child_lt.len = sp.n_lt;
+  child_gt.len = sp.n_gt;
+
+  uint32_t split_index;
+  if (sp.kind == 1) {
+    /* partition_in_place(center[axis] < t): canonical stable partition */
+    uint32_t w = 0;
+    for (uint32_t k = 0; k < n; ++k)
+      if (u32_lt_threshold(c, list[k], sp.axis, sp.threshold)) c->tmp[w++] = list[k];
+    uint32_t nl = w;
+    for (uint32_t k = 0; k < n; ++k)
+      if (!u32_lt_threshold(c, list[k], sp.axis, sp.threshold)) c->tmp[w++] = list[k];
+    memcpy(list, c->tmp, n * sizeof(uint32_t));
+    split_index = nl;
+  } else if (sp.kind == 0) {
+    int t = stable_select(c, list, n, sp.index, sp.axis);
+    if (t == 1) c->st->n_tie_events++;
+    else if (t == 2) c->st->n_tie_benign++;
+    split_index = sp.index;
+  } else {
+    split_index = sp.index;
+  }
+
+  /* asserts lib.rs:618-646 */
+  if (split_index == 0 || split_index == n) c->st->n_assert_fail++;
+  if (child_lt.len != split_index || child_gt.len != n - split_index) c->st->n_assert_fail++;
+  for (uint32_t k = 0; k < n; ++k) {
+    float ctr[3];
+    obj_center(c, list[k], ctr);
+    const or_bv* cb = k < split_index ? &child_lt.b : &child_gt.b;
+    if (!bv_contains(cb, ctr)) { c->st->n_assert_fail++; break; }
+  }
+
+  child_gt.start = self.start + child_lt.len;
+
+  if (child_gt.len > 0 && child_lt.len > 0) {
+    self.child = c->nnodes;
+    uint32_t gi = c->nnodes, li = c->nnodes + 1;
+    c->nodes[gi] = child_gt;
+    c->nodes[li] = child_lt;
+    c->nnodes += 2;
+    c->nodes[self_idx] = self;
+    if (depth > c->st->height) c->st->height = depth;
+    if (c->recs) {
+      or_rec* r = &c->recs[self_idx];
+      r->axis_method = (uint32_t)sp.axis | ((uint32_t)sp.kind << 8);
+      r->value = sp.value;
+      r->count_lt = sp.n_lt;
+      r->cost_bits = fbits(sp.cost);
+    }
+    if (sp.kind == 0) c->st->n_binned++;
+    else if (sp.kind == 1) c->st->n_adaptive++;
+    else c->st->n_median++;
+    split_node(c, gi, depth + 1);
+    split_node(c, li, depth + 1);
+  }
+}
+
+int or_build(const float* aabbs, uint32_t n, const or_settings* s, uint32_t* list,
+             or_node* nodes, or_rec* recs, or_stats* st) {
+  memset(st, 0, sizeof *st);
+  if (n == 0) return 0; /* lib.rs:703-708: empty BVH */
+  ctx_t c;
+  c.aabb = (const or_bv*)aabbs;
+  c.list = list;
+  c.nodes = nodes;
+  c.nnodes = 0;
+  c.recs = recs;
+  c.st = st;
+  c.min_leaf = s->min_leaf;
+  c.max_leaf = s->max_leaf;
+  c.max_depth = s->max_depth;
+  c.tmp = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
+  c.sortkey = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)n + 8);
+  if (!c.tmp || !c.sortkey) { free(c.tmp); free(c.sortkey); return -1; }
+  if (recs) memset(recs, 0xff, sizeof(or_rec) * (2 * (size_t)n - 1));
+
+  or_node root;
+  if (s->has_bounds) {
+    root.b = s->bounds; /* root_with_bounds, lib.rs:230-243 */
+  } else {
+    root.b = BV_EMPTY; /* root, lib.rs:220-228 */
+    for (uint32_t k = 0; k < n; ++k) bv_grow(&root.b, &c.aabb[list[k]]);
+  }
+  root.start = 0;
+  root.len = n;
+  root.child = 0;
+  nodes[0] = root;
+  c.nnodes = 1;
+  split_node(&c, 0, 0);
+  st->nnodes = c.nnodes;
+  free(c.tmp);
+  free(c.sortkey);
+  return 0;
+}
+
+void or_encode_nodes(const or_node* nodes, uint32_t nn, uint32_t* out8) {
+  for (uint32_t i = 0; i < nn; ++i) {
+    uint32_t* o = out8 + 8 * (size_t)i;
+    for (int a = 0; a < 3; ++a) o[a] = fbits(nodes[i].b.mn[a]);
+    o[3] = nodes[i].start;
+    for (int a = 0; a < 3; ++a) o[4 + a] = fbits(nodes[i].b.mx[a]);
+    o[7] = nodes[i].child;
+  }
+}
+
+double or_tree_sah(const or_node* nodes, uint32_t nn) {
+  if (nn == 0) return 0.0;
+  double sr = (double)bv_sa(&nodes[0].b);
+  if (sr <= 0.0) return 0.0;
+  double acc = 0.0;
+  for (uint32_t i = 0; i < nn; ++i) {
+    double sa = (double)bv_sa(&nodes[i].b) / sr;
+    if (nodes[i].child != 0) acc += sa * 1.0;
+    else acc += sa * 8.0 * (double)nodes[i].len;
+  }
+  return acc;
+}
+
+/* ---------------------------------------------------------------- closest hit */
+
+/* Moller-Trumbore as in scene.slang:19-89 (float3 cross/dot expanded left to
+ * right, no FMA).  Returns t, or +inf on a miss. */
+float or_ray_tri(const float* r, const float* v1, const float* v2, const float* v3) {
+  float e1[3], e2[3], h[3], s[3], q[3];
+  for (int a = 0; a < 3; ++a) { e1[a] = v2[a] - v1[a]; e2[a] = v3[a] - v1[a]; }
+  const float* o = r;
+  const float* d = r + 3;
+  h[0] = d[1] * e2[2] - d[2] * e2[1];
+  h[1] = d[2] * e2[0] - d[0] * e2[2];
+  h[2] = d[0] * e2[1] - d[1] * e2[0];
+  float det = h[0] * e1[0] + h[1] * e1[1];
+  det = det + h[2] * e1[2];
+  if (fabsf(det) < 1e-6f) return INFINITY;
+  float f = 1.0f / det;
+  for (int a = 0; a < 3; ++a) s[a] = o[a] - v1[a];
+  float du = s[0] * h[0] + s[1] * h[1];
+  du = du + s[2] * h[2];
+  float u = f * du;
+  if (u < 0.0f || u > 1.0f) return INFINITY;
+  q[0] = s[1] * e1[2] - s[2] * e1[1];
+  q[1] = s[2] * e1[0] - s[0] * e1[2];
+  q[2] = s[0] * e1[1] - s[1] * e1[0];
+  float dv = d[0] * q[0] + d[1] * q[1];
+  dv = dv + d[2] * q[2];
+  float v = f * dv;
+  if (v < 0.0f || u + v > 1.0f) return INFINITY;
+  float dt = e2[0] * q[0] + e2[1] * q[1];
+  dt = dt + e2[2] * q[2];
+  float t = f * dt;
+  if (t < 1e-6f) return INFINITY;
+  return t;
+}
+
+void or_closest_hit_brute(const float* pos, uint32_t vs, const uint32_t* tris, uint32_t ntris,
+                          const float* rays, uint32_t nrays, float* t_out, uint32_t* prim_out) {
+  for (uint32_t r = 0; r < nrays; ++r) {
+    float best = INFINITY;
+    uint32_t bp = 0xffffffffu;
+    for (uint32_t k = 0; k < ntris; ++k) {
+      float t = or_ray_tri(rays + 6 * (size_t)r, pos + (size_t)tris[3 * k] * vs,
+                           pos + (size_t)tris[3 * k + 1] * vs, pos + (size_t)tris[3 * k + 2] * vs);
+      if (t < best) { best = t; bp = k; } /* ascending k: strict < keeps lowest id */
+    }
+    t_out[r] = best;
+    prim_out[r] = bp;
+  }
+}

@@ -1,0 +1,83 @@
+/*
+ * CPU oracle for the marigold `bvh` crate (TEST INFRASTRUCTURE ONLY).
+ *
+ * A plain-C, op-for-op restatement of `bvh/src/lib.rs` used by tests/, the
+ * smoke check and bench.py's cpu_baseline leg as the checker.  The product
+ * (marigold_amd / libmgbvh.so) never links or calls this code.
+ *
+ * Everything is fp32 with strict left-to-right operation order and no FMA
+ * contraction (build with -ffp-contract=off, no -ffast-math).
+ */
+#ifndef MGBVH_ORACLE_H
+#define MGBVH_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* BoundingVolume, bvh/src/lib.rs:41-46 (x,y,z lanes of two Vec3A). */
+typedef struct { float mn[3]; float mx[3]; } or_bv;
+
+/* BvhNode<MIN,MAX>, bvh/src/lib.rs:149-155. */
+typedef struct { or_bv b; uint32_t start, len, child; } or_node;
+
+/* Split decision record: one per internal node, indexed by node index.
+ * method: 0 = binned (lib.rs:331), 1 = adaptive threshold (lib.rs:391),
+ * 2 = median fallback (lib.rs:444).  value: bin cut i / threshold bits / m. */
+typedef struct { uint32_t axis_method, value, count_lt, cost_bits; } or_rec;
+
+typedef struct {
+  uint32_t min_leaf, max_leaf;  /* const generics MIN_LEAF_OBJECTS / MAX_LEAF_OBJECTS */
+  uint32_t max_depth;           /* BvhSettings.max_depth */
+  int32_t has_bounds;           /* BvhSettings.bounds is Some */
+  or_bv bounds;
+} or_settings;
+
+typedef struct {
+  uint32_t nnodes, height;
+  uint32_t n_binned, n_adaptive, n_median;
+  uint32_t n_cutoff_leaves;     /* leaves forced by depth == max_depth holding > 1 object */
+  uint32_t n_tie_events;        /* list-order-dependent choices that can change the node array */
+  uint32_t n_median_quirk;      /* Exact split whose membership != winning axis' membership */
+  uint32_t n_assert_fail;       /* a reference assert! would have panicked */
+  uint32_t n_cost_leaf;         /* leaf because split.cost >= leaf_cost (MAX > 1 only) */
+  uint32_t n_refused;           /* leaf because select_split returned None */
+  uint32_t n_tie_benign;        /* order-dependent choices between objects with identical boxes */
+} or_stats;
+
+/* MeshTriangle::bounding_volume (mesh_interface/src/lib.rs:19-30); -0.0 -> +0.0. */
+void or_tri_aabbs(const float* pos, uint32_t vstride_floats, const uint32_t* tris,
+                  uint64_t n, float* aabbs_out);
+
+/* BoundingVolumeHierarchy::new (lib.rs:698-878) over objects given by AABBs.
+ * list: in/out object ids (permuted in place like `&mut [T]`).
+ * nodes: capacity 2n-1.  recs: capacity 2n-1 (may be NULL).
+ * Returns 0 on success. */
+int or_build(const float* aabbs, uint32_t n, const or_settings* s, uint32_t* list,
+             or_node* nodes, or_rec* recs, or_stats* st);
+
+/* Pack nodes into the 32-byte GPU record (lib.rs:157-212, MAX == 1 branch). */
+void or_encode_nodes(const or_node* nodes, uint32_t nnodes, uint32_t* out8);
+
+/* Tree SAH cost with the reference's constants (lib.rs:217-218), in double. */
+double or_tree_sah(const or_node* nodes, uint32_t nnodes);
+
+/* Single helpers exported for known-answer tests. */
+float or_surface_area(const float* bv6);
+int32_t or_total_cmp_key(float f);
+uint32_t or_bin_index(float c, float cmin, float cext, uint32_t nbins);
+float or_binned_cost(const float* parent6, const float* lt6, const float* gt6,
+                     uint32_t n_lt, uint32_t n_gt);
+
+/* Closest hit (t, prim) by brute force: Moller-Trumbore per scene.slang:19-89,
+ * lowest primitive id wins ties.  rays: n x {ox,oy,oz,dx,dy,dz}. */
+void or_closest_hit_brute(const float* pos, uint32_t vstride_floats, const uint32_t* tris,
+                          uint32_t ntris, const float* rays, uint32_t nrays,
+                          float* t_out, uint32_t* prim_out);
+float or_ray_tri(const float* ray6, const float* v1, const float* v2, const float* v3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

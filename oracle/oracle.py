@@ -1,0 +1,160 @@
+"""ctypes binding of the CPU oracle (oracle/bvh_oracle.c) -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the checker or the CPU baseline.  It restates marigold's
+`bvh/src/lib.rs` in C; see the header of bvh_oracle.c for the file:line map.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(HERE, "liboracle.so")
+SRC = [os.path.join(HERE, "bvh_oracle.c")]
+
+NODE_DTYPE = np.dtype([("min", "<f4", (3,)), ("max", "<f4", (3,)), ("start", "<u4"), ("len", "<u4"),
+                       ("child", "<u4")])
+REC_DTYPE = np.dtype([("axis_method", "<u4"), ("value", "<u4"), ("count_lt", "<u4"),
+                      ("cost_bits", "<u4")])
+
+
+class Settings(C.Structure):
+    _fields_ = [("min_leaf", C.c_uint32), ("max_leaf", C.c_uint32), ("max_depth", C.c_uint32),
+                ("has_bounds", C.c_int32), ("bounds", C.c_float * 6)]
+
+
+class Stats(C.Structure):
+    _fields_ = [(k, C.c_uint32) for k in (
+        "nnodes", "height", "n_binned", "n_adaptive", "n_median", "n_cutoff_leaves", "n_tie_events",
+        "n_median_quirk", "n_assert_fail", "n_cost_leaf", "n_refused", "n_tie_benign")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def build_lib(force: bool = False) -> str:
+    """Compiles the oracle with gcc (-O3, no FMA contraction, no fast-math)."""
+    if force or not os.path.exists(SO) or any(os.path.getmtime(s) > os.path.getmtime(SO) for s in SRC):
+        cmd = ["gcc", "-O3", "-fPIC", "-shared", "-std=c11", "-ffp-contract=off", "-fno-fast-math",
+               "-pthread", "-o", SO] + SRC + ["-lm"]
+        subprocess.run(cmd, check=True)
+    return SO
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build_lib()
+        L = C.CDLL(SO)
+        P = C.c_void_p
+        L.or_tri_aabbs.argtypes = [P, C.c_uint32, P, C.c_uint64, P]
+        L.or_build.argtypes = [P, C.c_uint32, C.POINTER(Settings), P, P, P, C.POINTER(Stats)]
+        L.or_build.restype = C.c_int
+        L.or_encode_nodes.argtypes = [P, C.c_uint32, P]
+        L.or_tree_sah.argtypes = [P, C.c_uint32]
+        L.or_tree_sah.restype = C.c_double
+        L.or_surface_area.argtypes = [P]
+        L.or_surface_area.restype = C.c_float
+        L.or_total_cmp_key.argtypes = [C.c_float]
+        L.or_total_cmp_key.restype = C.c_int32
+        L.or_bin_index.argtypes = [C.c_float, C.c_float, C.c_float, C.c_uint32]
+        L.or_bin_index.restype = C.c_uint32
+        L.or_binned_cost.argtypes = [P, P, P, C.c_uint32, C.c_uint32]
+        L.or_binned_cost.restype = C.c_float
+        L.or_closest_hit_brute.argtypes = [P, C.c_uint32, P, C.c_uint32, P, C.c_uint32, P, P]
+        L.or_ray_tri.argtypes = [P, P, P, P]
+        L.or_ray_tri.restype = C.c_float
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def tri_aabbs(pos: np.ndarray, tris: np.ndarray) -> np.ndarray:
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    tris = np.ascontiguousarray(tris, dtype=np.uint32)
+    out = np.empty((tris.shape[0], 6), dtype=np.float32)
+    lib().or_tri_aabbs(_p(pos), pos.shape[1], _p(tris), tris.shape[0], _p(out))
+    return out
+
+
+def build(aabbs: np.ndarray, max_depth: int = 32, bounds=None, min_leaf: int = 1, max_leaf: int = 1,
+          list_in: np.ndarray | None = None):
+    """Runs BoundingVolumeHierarchy::new over objects with the given boxes.
+    Returns dict(nodes, list, recs, stats)."""
+    aabbs = np.ascontiguousarray(aabbs, dtype=np.float32)
+    n = aabbs.shape[0]
+    s = Settings(min_leaf, max_leaf, max_depth, 0)
+    if bounds is not None:
+        s.has_bounds = 1
+        for a in range(3):
+            s.bounds[a] = float(bounds[0][a])
+            s.bounds[3 + a] = float(bounds[1][a])
+    lst = np.arange(n, dtype=np.uint32) if list_in is None else np.array(list_in, dtype=np.uint32)
+    cap = max(2 * n - 1, 1)
+    nodes = np.zeros(cap, dtype=NODE_DTYPE)
+    recs = np.zeros(cap, dtype=REC_DTYPE)
+    st = Stats()
+    rc = lib().or_build(_p(aabbs), n, C.byref(s), _p(lst), _p(nodes), _p(recs), C.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"oracle build failed: {rc}")
+    k = st.nnodes
+    return {"nodes": nodes[:k], "list": lst, "recs": recs[:k], "stats": st.as_dict()}
+
+
+def encode(nodes: np.ndarray) -> np.ndarray:
+    """32-byte GPU records as (k, 8) uint32."""
+    nodes = np.ascontiguousarray(nodes)
+    out = np.empty((nodes.shape[0], 8), dtype=np.uint32)
+    lib().or_encode_nodes(_p(nodes), nodes.shape[0], _p(out))
+    return out
+
+
+def tree_sah(nodes: np.ndarray) -> float:
+    nodes = np.ascontiguousarray(nodes)
+    return float(lib().or_tree_sah(_p(nodes), nodes.shape[0]))
+
+
+def surface_area(b6) -> float:
+    a = np.ascontiguousarray(b6, dtype=np.float32)
+    return float(lib().or_surface_area(_p(a)))
+
+
+def total_cmp_key(f: float) -> int:
+    return int(lib().or_total_cmp_key(C.c_float(f)))
+
+
+def bin_index(c, cmin, cext, nbins=32) -> int:
+    return int(lib().or_bin_index(C.c_float(c), C.c_float(cmin), C.c_float(cext), nbins))
+
+
+def binned_cost(parent6, lt6, gt6, n_lt, n_gt) -> float:
+    p = np.ascontiguousarray(parent6, dtype=np.float32)
+    l = np.ascontiguousarray(lt6, dtype=np.float32)
+    g = np.ascontiguousarray(gt6, dtype=np.float32)
+    return float(lib().or_binned_cost(_p(p), _p(l), _p(g), n_lt, n_gt))
+
+
+def ray_tri(ray6, v1, v2, v3) -> float:
+    arrs = [np.ascontiguousarray(x, dtype=np.float32) for x in (ray6, v1, v2, v3)]
+    return float(lib().or_ray_tri(*[_p(a) for a in arrs]))
+
+
+def closest_hit_brute(pos, tris, rays):
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    tris = np.ascontiguousarray(tris, dtype=np.uint32)
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    t = np.empty(rays.shape[0], dtype=np.float32)
+    prim = np.empty(rays.shape[0], dtype=np.uint32)
+    lib().or_closest_hit_brute(_p(pos), pos.shape[1], _p(tris), tris.shape[0], _p(rays),
+                               rays.shape[0], _p(t), _p(prim))
+    return t, prim
