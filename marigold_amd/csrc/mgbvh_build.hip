@@ -48,6 +48,11 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Value of lane k (wave-uniform k) broadcast to the whole wavefront.
+__device__ inline float rl(float v, uint32_t k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)k));
+}
+
 __device__ inline bool coord_ok(float x) { return fabsf(x) < 1.152921504606846976e18f; }  // 2^60
 
 // Block-wide max of 12 accumulator words, then one atomicMax each.
@@ -340,18 +345,54 @@ __global__ __launch_bounds__(256) void k_sweep(LNode* __restrict__ L, uint32_t n
     cext[a] = __fsub_rn(acc_max_dec(nd.cacc[3 + a]), cmin[a]);
   }
   const float sa_p = box_sa(nb);
+  // Candidates (axis a, cut i), lane = a * 32 + bin: lt = bins [0, i), gt = bins
+  // [i, 32) from 32-lane inclusive prefix / suffix scans (exact min/max unions).
   uint64_t best = ~0ull;
-  for (uint32_t q = lane; q < 3 * MG_NBINS; q += 64) {
-    const int a = (int)(q / MG_NBINS);
-    const uint32_t i = q % MG_NBINS;
-    if (i == 0 || cext[a] == 0.0f) continue;  // lib.rs:339-341
-    Box lt, gt;
-    uint32_t nl, ng;
-    eval_cut(h, a, i, lt, gt, nl, ng);
-    if (nl < 1 || ng < 1) continue;           // lib.rs:379-381
-    float cost = split_cost(sa_p, box_sa(lt), box_sa(gt), nl, ng);
-    uint64_t key = ((uint64_t)tc_ukey(cost) << 32) | q;
-    best = key < best ? key : best;
+#pragma unroll
+  for (uint32_t pass = 0; pass < 2; ++pass) {
+    const uint32_t q = lane + 64 * pass;
+    const bool active = q < 3 * MG_NBINS;
+    const uint32_t b = q % MG_NBINS;
+    const int a = active ? (int)(q / MG_NBINS) : 0;
+    uint32_t cnt = 0;
+    Box bb = box_empty();
+    if (active) {
+      const uint32_t* hb = &h[q * BIN_WORDS];
+      cnt = hb[0];
+      if (cnt) bb = hist_box(hb);
+    }
+    uint32_t pc = cnt, sc = cnt;
+    Box pb = bb, sb = bb;
+#pragma unroll
+    for (uint32_t d = 1; d < MG_NBINS; d <<= 1) {
+      const uint32_t oc = __shfl_up(pc, d, MG_NBINS);
+      const uint32_t uc = __shfl_down(sc, d, MG_NBINS);
+      Box ob, ub;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        ob.mn[x] = __shfl_up(pb.mn[x], d, MG_NBINS);
+        ob.mx[x] = __shfl_up(pb.mx[x], d, MG_NBINS);
+        ub.mn[x] = __shfl_down(sb.mn[x], d, MG_NBINS);
+        ub.mx[x] = __shfl_down(sb.mx[x], d, MG_NBINS);
+      }
+      if (b >= d) { pc += oc; box_grow(pb, ob); }
+      if (b + d < MG_NBINS) { sc += uc; box_grow(sb, ub); }
+    }
+    // lt of cut b is the prefix ending at bin b - 1
+    const uint32_t nl = __shfl_up(pc, 1, MG_NBINS);
+    Box lt;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      lt.mn[x] = __shfl_up(pb.mn[x], 1, MG_NBINS);
+      lt.mx[x] = __shfl_up(pb.mx[x], 1, MG_NBINS);
+    }
+    const uint32_t ng = sc;
+    // lib.rs:339-341 (zero extent refused), 379-381 (MIN objects per side)
+    if (active && b >= 1 && cext[a] != 0.0f && nl >= 1 && ng >= 1) {
+      const float cost = split_cost(sa_p, box_sa(lt), box_sa(sb), nl, ng);
+      const uint64_t key = ((uint64_t)tc_ukey(cost) << 32) | q;
+      best = key < best ? key : best;
+    }
   }
   best = wave_min_u64(best);
 
@@ -494,12 +535,17 @@ __global__ __launch_bounds__(256) void k_partition(const LNode* __restrict__ L,
       if (pred) p = nd.start + lt_before + run_lt + pl + lanemask_lt_popc(ml);
       else p = nd.start + n_lt + gt_before + run_gt + pg + lanemask_lt_popc(mg);
       rec_store(&dst[p], b, id);
-      uint32_t* acc = pred ? acc_lt : acc_gt;
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        float cc = center(b.mn[a], b.mx[a]);
-        acc[a] = max(acc[a], acc_min_enc(cc));
-        acc[3 + a] = max(acc[3 + a], acc_max_enc(cc));
+        const float cc = center(b.mn[a], b.mx[a]);
+        const uint32_t emn = acc_min_enc(cc), emx = acc_max_enc(cc);
+        if (pred) {
+          acc_lt[a] = max(acc_lt[a], emn);
+          acc_lt[3 + a] = max(acc_lt[3 + a], emx);
+        } else {
+          acc_gt[a] = max(acc_gt[a], emn);
+          acc_gt[3 + a] = max(acc_gt[3 + a], emx);
+        }
       }
     }
     run_lt += tl;
@@ -528,263 +574,294 @@ __global__ __launch_bounds__(256) void k_partition(const LNode* __restrict__ L,
 }
 
 // ------------------------------------------------------------------ small subtrees
-struct SmallStackEnt {
-  uint32_t ls_lc;  // lstart | lcount << 8
-  uint32_t idx, P, depth;
-  float bmn[3], bmx[3];
-};
-struct SmallLDS {
-  float ct[MG_SMALL][3];
-  float mn[MG_SMALL][3];
-  float mx[MG_SMALL][3];
-  uint32_t id[MG_SMALL];
-  uint32_t key[MG_SMALL];
-  uint8_t list[MG_SMALL];
-  uint8_t tmp[MG_SMALL];
-  SmallStackEnt st[MG_SMALL + 2];
+// One wavefront builds two subtrees of <= 32 objects (lanes 0-31 and 32-63),
+// level by level.  Every node of a level is a segment of consecutive lanes
+// (lane = position in the node's list), so all nodes of a level are split at
+// once: segmented scans give centroid bounds and the first-minimum split, each
+// lane evaluates the 3 candidates (axis a, threshold r) of its list position
+// (adaptive sweep, lib.rs:391-442), and a stable in-segment partition moves the
+// objects for the next level.
+struct __align__(16) SmallObj {
+  float4 a;  // c0 c1 c2 mn0
+  float4 b;  // mn1 mn2 mx0 mx1
+  float4 d;  // mx2 id - -
 };
 
-__device__ inline void wave_box_reduce(Box& b) {
+__device__ inline uint32_t seg_max_u32(uint32_t v, uint32_t lane, uint32_t ls, uint32_t last) {
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    b.mn[a] = acc_min_dec(wave_max_u32(acc_min_enc(b.mn[a])));
-    b.mx[a] = acc_max_dec(wave_max_u32(acc_max_enc(b.mx[a])));
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(v, d, 64);
+    if (lane >= ls + d) v = max(v, o);
   }
+  return __shfl(v, (int)last, 64);
 }
 
-// Stable partition of list[ls, ls+lc) by a per-lane predicate (lanes < lc).
-__device__ inline void lds_stable_partition(SmallLDS& S, uint32_t lane, uint32_t ls, uint32_t lc,
-                                            bool pred, uint32_t nl) {
-  const bool in = lane < lc;
-  const uint64_t ml = __ballot(in && pred);
-  const uint64_t mg = __ballot(in && !pred);
-  uint8_t o = in ? S.list[ls + lane] : 0;
-  if (in) {
-    uint32_t p = pred ? lanemask_lt_popc(ml) : nl + lanemask_lt_popc(mg);
-    S.tmp[ls + p] = o;
+__device__ inline uint64_t seg_min_u64(uint64_t v, uint32_t lane, uint32_t ls, uint32_t last) {
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint64_t o = __shfl_up(v, d, 64);
+    if (lane >= ls + d) v = o < v ? o : v;
   }
+  return __shfl(v, (int)last, 64);
+}
+
+// Union of the boxes of lanes [ls, last] that have `take`.
+__device__ inline Box seg_box(bool take, const float* mn, const float* mx, uint32_t lane,
+                              uint32_t ls, uint32_t last) {
+  Box r;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    r.mn[x] = acc_min_dec(seg_max_u32(take ? acc_min_enc(mn[x]) : 0u, lane, ls, last));
+    r.mx[x] = acc_max_dec(seg_max_u32(take ? acc_max_enc(mx[x]) : 0u, lane, ls, last));
+  }
+  return r;
+}
+
+struct SmallLane {
+  float c[3], mn[3], mx[3];
+  uint32_t id;
+};
+
+__device__ inline void obj_put(SmallObj* S, uint32_t pos, const SmallLane& o) {
+  S[pos].a = make_float4(o.c[0], o.c[1], o.c[2], o.mn[0]);
+  S[pos].b = make_float4(o.mn[1], o.mn[2], o.mx[0], o.mx[1]);
+  S[pos].d = make_float4(o.mx[2], __uint_as_float(o.id), 0.0f, 0.0f);
+}
+__device__ inline void obj_get(const SmallObj* S, uint32_t pos, SmallLane& o) {
+  const float4 a = S[pos].a, b = S[pos].b, d = S[pos].d;
+  o.c[0] = a.x; o.c[1] = a.y; o.c[2] = a.z; o.mn[0] = a.w;
+  o.mn[1] = b.x; o.mn[2] = b.y; o.mx[0] = b.z; o.mx[1] = b.w;
+  o.mx[2] = d.x; o.id = __float_as_uint(d.y);
+}
+
+// Moves each lane's object to `newpos` (a permutation of the lanes).
+__device__ inline void lane_move(SmallObj* S, uint32_t lane, uint32_t newpos, SmallLane& o) {
+  obj_put(S, newpos, o);
   wave_sync();
-  if (in) S.list[ls + lane] = S.tmp[ls + lane];
+  obj_get(S, lane, o);
   wave_sync();
 }
 
-// One wavefront per node of <= 32 objects (or a leaf forced by max_depth):
-// split (lib.rs:536-665) with the adaptive sweep (lib.rs:391-442) down to the
-// leaves, all in LDS.
 __global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small, uint32_t nS,
                                                const Rec* __restrict__ rec0,
                                                const Rec* __restrict__ rec1, NodeOut out,
                                                uint32_t* __restrict__ perm, Counters* ctr,
                                                uint32_t* gaps, uint32_t max_depth) {
-  __shared__ SmallLDS lds[4];
+  __shared__ SmallObj lds_obj[4][64];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t lane = lane_id();
-  const uint32_t e = blockIdx.x * 4 + w;
-  if (e >= nS) return;  // only wave-local synchronisation below
-  SmallLDS& S = lds[w];
-  const SNode sn = small[e];
-  const Rec* src = sn.parity ? rec1 : rec0;
-  Box rb;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) { rb.mn[a] = sn.bmn[a]; rb.mx[a] = sn.bmx[a]; }
+  SmallObj* S = lds_obj[w];
+  const uint32_t wave = blockIdx.x * 4 + w;
+  if (2 * wave >= nS) return;  // only wave-local synchronisation below
+  const uint32_t h = lane >> 5, hl = lane & 31, off = 32 * h;
+  const uint32_t e = 2 * wave + h;
 
-  if (sn.count == 1 || sn.depth == max_depth || sn.count > MG_SMALL) {
-    if (lane == 0) {
-      out.write_node(sn.idx, rb, sn.start, 0, sn.count);
-      if (sn.count > 1) {
-        uint32_t g = atomicAdd(&ctr->n_gaps, 1u);
+  bool has = false;
+  SmallLane o;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) { o.c[x] = 0.0f; o.mn[x] = 0.0f; o.mx[x] = 0.0f; }
+  o.id = 0;
+  uint32_t ls = lane, lc = 1, idx = 0, P = 0, depth = 0, base = 0;
+  Box nb = box_empty();
+  if (e < nS) {
+    const SNode sn = small[e];
+    const Rec* src = sn.parity ? rec1 : rec0;
+    if (sn.count > MG_SMALL) {
+      // leaf forced by max_depth (lib.rs:556) holding more than 32 objects
+      if (hl == 0) {
+        Box b;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) { b.mn[x] = sn.bmn[x]; b.mx[x] = sn.bmx[x]; }
+        out.write_node(sn.idx, b, sn.start, 0, sn.count);
+        const uint32_t g = atomicAdd(&ctr->n_gaps, 1u);
         gaps[2 * g] = sn.P;
         gaps[2 * g + 1] = 2 * (sn.count - 1);
         atomicAdd(&ctr->n_cutoff, 1u);
       }
-    }
-    for (uint32_t j = lane; j < sn.count; j += 64) {
+      for (uint32_t j = hl; j < sn.count; j += 32) perm[sn.start + j] = src[sn.start + j].id;
+    } else if (hl < sn.count) {
       Box b;
-      uint32_t id;
-      rec_load(&src[sn.start + j], b, id);
-      perm[sn.start + j] = id;
-    }
-    return;
-  }
-
-  const uint32_t n = sn.count;
-  if (lane < n) {
-    Box b;
-    uint32_t id;
-    rec_load(&src[sn.start + lane], b, id);
+      rec_load(&src[sn.start + hl], b, o.id);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      S.mn[lane][a] = b.mn[a];
-      S.mx[lane][a] = b.mx[a];
-      S.ct[lane][a] = center(b.mn[a], b.mx[a]);
+      for (int x = 0; x < 3; ++x) {
+        o.mn[x] = b.mn[x];
+        o.mx[x] = b.mx[x];
+        o.c[x] = center(b.mn[x], b.mx[x]);
+        nb.mn[x] = sn.bmn[x];
+        nb.mx[x] = sn.bmx[x];
+      }
+      has = true;
+      ls = off; lc = sn.count; idx = sn.idx; P = sn.P; depth = sn.depth; base = sn.start;
     }
-    S.id[lane] = id;
-    S.list[lane] = (uint8_t)lane;
   }
-  if (lane == 0) {
-    SmallStackEnt r;
-    r.ls_lc = 0u | (n << 8);
-    r.idx = sn.idx; r.P = sn.P; r.depth = sn.depth;
-    for (int a = 0; a < 3; ++a) { r.bmn[a] = rb.mn[a]; r.bmx[a] = rb.mx[a]; }
-    S.st[0] = r;
-  }
+  obj_put(S, lane, o);
   wave_sync();
-  int sp = 1;
+
   uint32_t hmax = 0;
-  bool any_split = false;
-  while (sp > 0) {
-    --sp;
-    const SmallStackEnt t = S.st[sp];
-    wave_sync();
-    const uint32_t ls = t.ls_lc & 0xff, lc = t.ls_lc >> 8;
-    Box nb;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) { nb.mn[a] = t.bmn[a]; nb.mx[a] = t.bmx[a]; }
-    if (lc == 1 || t.depth == max_depth) {
-      if (lane == 0) {
-        out.write_node(t.idx, nb, sn.start + ls, 0, lc);
+  bool split_any = false;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  while (__ballot(has)) {
+    // leaves: one object, or depth == max_depth (lib.rs:548-558)
+    if (has && (lc == 1 || depth == max_depth)) {
+      if (lane == ls) {
+        out.write_node(idx, nb, base + ls - off, 0, lc);
         if (lc > 1) {
-          uint32_t g = atomicAdd(&ctr->n_gaps, 1u);
-          gaps[2 * g] = t.P;
+          const uint32_t g = atomicAdd(&ctr->n_gaps, 1u);
+          gaps[2 * g] = P;
           gaps[2 * g + 1] = 2 * (lc - 1);
           atomicAdd(&ctr->n_cutoff, 1u);
         }
       }
-      if (lane < lc) perm[sn.start + ls + lane] = S.id[S.list[ls + lane]];
-      continue;
+      perm[base + lane - off] = o.id;
+      has = false;
     }
-    // centroid bounds (lib.rs:499-509)
-    Box cb = box_empty();
-    if (lane < lc) {
-      const uint32_t o = S.list[ls + lane];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) { cb.mn[a] = S.ct[o][a]; cb.mx[a] = S.ct[o][a]; }
-    }
-    wave_box_reduce(cb);
+    const bool act = has;
+    if (!__ballot(act)) break;
+    const uint32_t last = ls + lc - 1;
+    const uint32_t r = lane - ls;
+    // centroid bounds of each node (lib.rs:499-509)
+    Box cb = seg_box(act, o.c, o.c, lane, ls, last);
     const float sa_p = box_sa(nb);
-    uint64_t best = ~0ull;
-    for (uint32_t q = lane; q < 3 * lc; q += 64) {
-      const uint32_t a = q / lc, j = q % lc;
-      const float th = lc < 8 ? S.ct[S.list[ls + j]][a]
-                              : adaptive_threshold(cb.mn[a], cb.mx[a], lc, j);
-      Box lt = box_empty(), gt = box_empty();
-      uint32_t nl = 0, ng = 0;
-      for (uint32_t kk = 0; kk < lc; ++kk) {
-        const uint32_t o = S.list[ls + kk];
-        Box ob;
+    float th[3];
 #pragma unroll
-        for (int x = 0; x < 3; ++x) { ob.mn[x] = S.mn[o][x]; ob.mx[x] = S.mx[o][x]; }
-        if (S.ct[o][a] < th) { box_grow(lt, ob); ++nl; }
-        else { box_grow(gt, ob); ++ng; }
+    for (int a = 0; a < 3; ++a)
+      th[a] = lc < 8 ? o.c[a] : adaptive_threshold(cb.mn[a], cb.mx[a], lc, r);
+    // the three candidates (a, r) of this lane over the node's objects
+    Box lt[3], gt[3];
+    uint32_t nl[3], ng[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { lt[a] = box_empty(); gt[a] = box_empty(); nl[a] = 0; ng[a] = 0; }
+    const uint32_t maxlc = wave_max_u32(act ? lc : 0u);
+    for (uint32_t k = 0; k < maxlc; ++k) {
+      const bool v = act && k < lc;
+      SmallLane q;
+      obj_get(S, v ? ls + k : lane, q);
+      if (v) {
+        Box qb;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) { qb.mn[x] = q.mn[x]; qb.mx[x] = q.mx[x]; }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          if (q.c[a] < th[a]) { box_grow(lt[a], qb); ++nl[a]; }
+          else { box_grow(gt[a], qb); ++ng[a]; }
+        }
       }
-      if (nl < 1 || ng < 1) continue;
-      float cost = split_cost(sa_p, box_sa(lt), box_sa(gt), nl, ng);
-      uint64_t key = ((uint64_t)tc_ukey(cost) << 32) | (a * 32 + j);
-      best = key < best ? key : best;
     }
-    best = wave_min_u64(best);
-
-    Box lt, gt;
-    uint32_t nl, ng, axis, value, kind;
-    float cost;
-    if (best != ~0ull) {
-      const uint32_t q = (uint32_t)best;
-      axis = q / 32;
-      const uint32_t j = q % 32;
-      const float th = lc < 8 ? S.ct[S.list[ls + j]][axis]
-                              : adaptive_threshold(cb.mn[axis], cb.mx[axis], lc, j);
-      lt = box_empty();
-      gt = box_empty();
-      bool pred = false;
-      if (lane < lc) {
-        const uint32_t o = S.list[ls + lane];
-        pred = S.ct[o][axis] < th;
-        Box ob;
+    uint64_t key = ~0ull;
+    int ba = 0;
 #pragma unroll
-        for (int x = 0; x < 3; ++x) { ob.mn[x] = S.mn[o][x]; ob.mx[x] = S.mx[o][x]; }
-        if (pred) box_grow(lt, ob);
-        else box_grow(gt, ob);
+    for (int a = 0; a < 3; ++a) {
+      if (act && nl[a] >= 1 && ng[a] >= 1) {
+        const float cst = split_cost(sa_p, box_sa(lt[a]), box_sa(gt[a]), nl[a], ng[a]);
+        const uint64_t k2 = ((uint64_t)tc_ukey(cst) << 32) | (uint32_t)(a * 32 + r);
+        if (k2 < key) { key = k2; ba = a; }
       }
-      wave_box_reduce(lt);
-      wave_box_reduce(gt);
-      nl = (uint32_t)__popcll(__ballot(lane < lc && pred));
-      ng = lc - nl;
-      cost = split_cost(sa_p, box_sa(lt), box_sa(gt), nl, ng);
-      value = fbits(th);
-      kind = 1;
-      lds_stable_partition(S, lane, ls, lc, pred, nl);  // partition_in_place, lib.rs:598-600
-    } else {
-      // median fallback on axes 0,1,2, each reordering the list (lib.rs:527-531);
-      // the children take the winning axis' boxes, the list keeps the axis-2 order.
-      const uint32_t m = lc / 2;
-      bool have = false;
-      uint32_t bkey = 0;
-      axis = 0;
-      for (uint32_t a = 0; a < 3; ++a) {
-        if (lane < lc) S.key[lane] = tc_ukey(S.ct[S.list[ls + lane]][a]);
-        wave_sync();
-        bool sel = false;
-        if (lane < lc) {
-          const uint32_t kj = S.key[lane];
-          uint32_t rank = 0;
-          for (uint32_t i2 = 0; i2 < lc; ++i2) {
-            const uint32_t ki = S.key[i2];
-            rank += (ki < kj) || (ki == kj && i2 < lane);
-          }
-          sel = rank < m;
-        }
-        wave_sync();
-        lds_stable_partition(S, lane, ls, lc, sel, m);
-        Box l2 = box_empty(), g2 = box_empty();
-        if (lane < lc) {
-          const uint32_t o = S.list[ls + lane];
-          Box ob;
+    }
+    // first minimum over (cost, axis, candidate) of the node (lib.rs:515-524)
+    const uint64_t skey = seg_min_u64(act ? key : ~0ull, lane, ls, last);
+    const bool med = act && skey == ~0ull;
+    // the winner lane's candidate
+    Box mylt, mygt;
 #pragma unroll
-          for (int x = 0; x < 3; ++x) { ob.mn[x] = S.mn[o][x]; ob.mx[x] = S.mx[o][x]; }
-          if (lane < m) box_grow(l2, ob);
-          else box_grow(g2, ob);
+    for (int x = 0; x < 3; ++x) {
+      mylt.mn[x] = ba == 0 ? lt[0].mn[x] : (ba == 1 ? lt[1].mn[x] : lt[2].mn[x]);
+      mylt.mx[x] = ba == 0 ? lt[0].mx[x] : (ba == 1 ? lt[1].mx[x] : lt[2].mx[x]);
+      mygt.mn[x] = ba == 0 ? gt[0].mn[x] : (ba == 1 ? gt[1].mn[x] : gt[2].mn[x]);
+      mygt.mx[x] = ba == 0 ? gt[0].mx[x] : (ba == 1 ? gt[1].mx[x] : gt[2].mx[x]);
+    }
+    const uint32_t mynl = ba == 0 ? nl[0] : (ba == 1 ? nl[1] : nl[2]);
+    const float myth = ba == 0 ? th[0] : (ba == 1 ? th[1] : th[2]);
+    const uint32_t q32 = (uint32_t)skey;
+    const uint32_t wl = (act && !med) ? ls + (q32 & 31u) : lane;
+    uint32_t axis = (act && !med) ? (q32 >> 5) : 0;
+    Box clt, cgt;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      clt.mn[x] = __shfl(mylt.mn[x], (int)wl, 64);
+      clt.mx[x] = __shfl(mylt.mx[x], (int)wl, 64);
+      cgt.mn[x] = __shfl(mygt.mn[x], (int)wl, 64);
+      cgt.mx[x] = __shfl(mygt.mx[x], (int)wl, 64);
+    }
+    uint32_t n_lt = __shfl(mynl, (int)wl, 64);
+    const float tw = __shfl(myth, (int)wl, 64);
+    uint32_t kind = 1, value = fbits(tw);
+    float cost = 0.0f;
+    bool pred = act && (axis == 0 ? o.c[0] : (axis == 1 ? o.c[1] : o.c[2])) < tw;
+
+    if (__ballot(med)) {
+      // median fallback on axes 0,1,2, each reordering the list (lib.rs:444-491,
+      // 527-531); the children take the winning axis' boxes and the axis-2 order.
+      const uint32_t m = lc / 2;
+      const uint64_t segm = (lc >= 64 ? ~0ull : ((1ull << lc) - 1)) << ls;
+      bool have = false;
+      uint32_t bk = 0, maxis = 0;
+      Box mlt = box_empty(), mgt = box_empty();
+      float mcost = 0.0f;
+      for (uint32_t a = 0; a < 3; ++a) {
+        const float ca = a == 0 ? o.c[0] : (a == 1 ? o.c[1] : o.c[2]);
+        const uint32_t kme = tc_ukey(ca);
+        uint32_t rank = 0;
+        for (uint32_t k = 0; k < maxlc; ++k) {
+          const uint32_t kk = __shfl(kme, (int)(med && k < lc ? ls + k : lane), 64);
+          if (med && k < lc) rank += (kk < kme) || (kk == kme && k < r);
         }
-        wave_box_reduce(l2);
-        wave_box_reduce(g2);
+        const bool sel = med && rank < m;
+        const uint64_t sm = __ballot(sel);
+        const uint32_t np = !med ? lane
+                                 : (sel ? ls + (uint32_t)__popcll(sm & segm & below)
+                                        : ls + m + (uint32_t)__popcll(~sm & segm & below));
+        lane_move(S, lane, np, o);
+        const bool inl = med && r < m;
+        const Box l2 = seg_box(inl, o.mn, o.mx, lane, ls, last);
+        const Box g2 = seg_box(med && !inl, o.mn, o.mx, lane, ls, last);
         const float c2 = split_cost(sa_p, box_sa(l2), box_sa(g2), m, lc - m);
         const uint32_t k2 = tc_ukey(c2);
-        if (!have || k2 < bkey) {
-          have = true;
-          bkey = k2;
-          axis = a;
-          lt = l2;
-          gt = g2;
-          cost = c2;
-        }
+        if (!have || k2 < bk) { have = true; bk = k2; maxis = a; mlt = l2; mgt = g2; mcost = c2; }
       }
-      nl = m;
-      ng = lc - m;
-      value = m;
-      kind = 2;
+      if (med) {
+        axis = maxis; kind = 2; value = m; n_lt = m; clt = mlt; cgt = mgt; cost = mcost;
+        pred = r < m;
+      }
     }
-    if (lane == 0) {
-      out.write_node(t.idx, nb, sn.start + ls, t.P, lc);
-      out.write_split(t.idx, axis | (kind << 8), value, nl, fbits(cost));
-      if (kind == 2) atomicAdd(&ctr->n_median, 1u);
+    if (act && !med) cost = split_cost(sa_p, box_sa(clt), box_sa(cgt), n_lt, lc - n_lt);
+
+    // stable partition inside each node (partition_in_place, lib.rs:598-600)
+    {
+      const uint64_t segm = (lc >= 64 ? ~0ull : ((1ull << lc) - 1)) << ls;
+      const uint64_t pm = __ballot(pred);
+      uint32_t np = lane;
+      if (act && !med)
+        np = pred ? ls + (uint32_t)__popcll(pm & segm & below)
+                  : ls + n_lt + (uint32_t)__popcll(~pm & segm & below);
+      if (__ballot(act && !med)) lane_move(S, lane, np, o);
     }
-    hmax = max(hmax, t.depth);
-    any_split = true;
-    if (lane == 0) {
-      SmallStackEnt c;
-      // lt child first on the stack, gt popped first (order does not matter)
-      c.ls_lc = ls | (nl << 8);
-      c.idx = t.P + 1; c.P = t.P + 2 + 2 * (ng - 1); c.depth = t.depth + 1;
-      for (int a = 0; a < 3; ++a) { c.bmn[a] = lt.mn[a]; c.bmx[a] = lt.mx[a]; }
-      S.st[sp] = c;
-      c.ls_lc = (ls + nl) | (ng << 8);
-      c.idx = t.P; c.P = t.P + 2; c.depth = t.depth + 1;
-      for (int a = 0; a < 3; ++a) { c.bmn[a] = gt.mn[a]; c.bmx[a] = gt.mx[a]; }
-      S.st[sp + 1] = c;
+    if (act) {
+      if (lane == ls) {
+        out.write_node(idx, nb, base + ls - off, P, lc);
+        out.write_split(idx, axis | (kind << 8), value, n_lt, fbits(cost));
+        if (kind == 2) atomicAdd(&ctr->n_median, 1u);
+      }
+      hmax = max(hmax, depth);
+      split_any = true;
+      // children (lib.rs:648-663): gt at P, lt at P + 1
+      const uint32_t ng2 = lc - n_lt;
+      if (lane < ls + n_lt) {
+        lc = n_lt;
+        idx = P + 1;
+        P = P + 2 + 2 * (ng2 - 1);
+        nb = clt;
+      } else {
+        ls = ls + n_lt;
+        lc = ng2;
+        idx = P;
+        P = P + 2;
+        nb = cgt;
+      }
+      ++depth;
     }
-    sp += 2;
-    wave_sync();
   }
-  if (lane == 0 && any_split) atomicMax(&ctr->height, hmax);
+  hmax = wave_max_u32(split_any ? hmax + 1 : 0u);
+  if (lane == 0 && hmax) atomicMax(&ctr->height, hmax - 1);
 }
 
 // ------------------------------------------------------------------ compaction
@@ -951,7 +1028,7 @@ int build_device(mgbvh_ctx* c, const BuildInput& in, uint32_t n, const mgbvh_set
   const uint32_t nS = c->h_ctr->n_small;
   if (nS) {
     MG_PROF(KK_SMALL, 0);
-    k_small<<<cdiv(nS, 4), 256, 0, st>>>(small, nS, recs[0], recs[1], out, o.perm, ctr, gaps,
+    k_small<<<cdiv(nS, 8), 256, 0, st>>>(small, nS, recs[0], recs[1], out, o.perm, ctr, gaps,
                                          s->max_depth);
     MG_PROF(KK_SMALL, 1);
     MG_HIP(hipGetLastError());

@@ -1,0 +1,265 @@
+"""Pins for the CPU oracle (oracle/bvh_oracle.c) independent of its own code.
+
+The reference crate ships no tests, fixtures or golden vectors (it is Rust and
+cannot be built here), so every check below comes from closed forms, exact
+rational arithmetic, brute force over all candidate splits, or structural
+invariants of bvh/src/lib.rs:536-665 -- each chosen so that a dropped term, a
+wrong constant, sign or index, or a wrong tie-break fails it.
+"""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from marigold_amd import scenes
+from oracle import oracle as O
+
+
+def f32(x):
+    return float(np.float32(x))
+
+
+# ---------------------------------------------------------------- closed forms
+def test_surface_area_closed_forms():
+    assert O.surface_area([0, 0, 0, 1, 1, 1]) == 6.0
+    assert O.surface_area([0, 0, 0, 2, 3, 4]) == 52.0          # 2 * (6 + 8 + 12)
+    assert O.surface_area([0, 0, 0, 1, 1, 0]) == 2.0           # flat box
+    assert O.surface_area([1, 0, 0, 0, 1, 1]) == 0.0           # empty (max < min)
+    inf = float("inf")
+    assert O.surface_area([inf, inf, inf, -inf, -inf, -inf]) == 0.0  # BoundingVolume::EMPTY
+
+
+def test_total_cmp_order():
+    vals = [np.float32(-np.nan) * -1, -np.inf, -1.0, -0.0, 0.0, 1e-45, 1.0, np.inf, np.nan]
+    neg_nan = np.array([0xFFC00000], dtype=np.uint32).view(np.float32)[0]
+    vals[0] = neg_nan
+    keys = [O.total_cmp_key(float(v)) for v in vals]
+    assert keys == sorted(keys) and len(set(keys)) == len(keys)
+
+
+def test_bin_index_known_answers():
+    assert O.bin_index(0.0, 0.0, 1.0) == 0
+    assert O.bin_index(1.0, 0.0, 1.0) == 31                    # 32 clamps to B - 1
+    assert O.bin_index(0.5, 0.0, 1.0) == 16
+    assert O.bin_index(0.03125, 0.0, 1.0) == 1
+    assert O.bin_index(0.03124, 0.0, 1.0) == 0
+    assert O.bin_index(0.999, 0.0, 1.0) == 31
+    assert O.bin_index(2.0, 2.0, 0.0) == 0                     # 0/0 = NaN saturates to 0
+    assert O.bin_index(5.5, 5.0, 1.0) == 16
+    # monotone in the centre: a sweep over a range never decreases
+    cs = np.linspace(3.0, 7.0, 2001, dtype=np.float32)
+    b = [O.bin_index(float(c), 3.0, 4.0) for c in cs]
+    assert all(x <= y for x, y in zip(b, b[1:])) and b[0] == 0 and b[-1] == 31
+
+
+def test_split_cost_exact_rational():
+    # parent [0,1]^3 (SA 6); halves [0,.5]x[0,1]^2 (SA 4); 2 objects each:
+    # cost = 1 + (4/6*8)*2 + (4/6*8)*2 = 1 + 64/3
+    c = O.binned_cost([0, 0, 0, 1, 1, 1], [0, 0, 0, .5, 1, 1], [.5, 0, 0, 1, 1, 1], 2, 2)
+    exact = 1 + Fraction(64, 3)
+    assert abs(Fraction(c) - exact) <= Fraction(4) * Fraction(2) ** -20
+    # asymmetric: lt [0,.25]x[0,1]x[0,2] SA = 2(.25+.5+2)=5.5, gt SA 2(.75*1+.75*2+2)=8.5, parent SA 2(1+2+2)=10
+    c = O.binned_cost([0, 0, 0, 1, 1, 2], [0, 0, 0, .25, 1, 2], [.25, 0, 0, 1, 1, 2], 3, 5)
+    exact = 1 + Fraction(55, 100) * 8 * 3 + Fraction(85, 100) * 8 * 5
+    assert abs(Fraction(c) - exact) <= Fraction(2) ** -17
+    # SA_parent == 0: 0/0 is the x86 default NaN (sign bit set)
+    c = O.binned_cost([0, 0, 0, 0, 0, 0], [0, 0, 0, 0, 0, 0], [0, 0, 0, 0, 0, 0], 1, 1)
+    assert np.float32(c).view(np.uint32) == 0xFFC00000
+
+
+def test_tri_aabbs_match_numpy():
+    pos, tris = scenes.soup(1000, seed=3)
+    ab = O.tri_aabbs(pos, tris)
+    v = pos[tris]
+    assert np.array_equal(ab[:, :3], v.min(axis=1)) and np.array_equal(ab[:, 3:], v.max(axis=1))
+
+
+# ---------------------------------------------------------------- hand-derived trees
+def test_two_objects_known_tree():
+    # A = [0,1]x[0,1]x[0,1], B = [2,3]x[0,1]x[0,1].  n = 2 < 8: thresholds are the
+    # centres in list order, 0.5 (lt empty, refused) then 2.5 (lt = {A}).
+    boxes = np.array([[0, 0, 0, 1, 1, 1], [2, 0, 0, 3, 1, 1]], np.float32)
+    r = O.build(boxes)
+    nd = r["nodes"]
+    assert len(nd) == 3
+    assert nd["child"][0] == 1 and nd["len"][0] == 2
+    assert (nd["start"][1], nd["len"][1], nd["child"][1]) == (1, 1, 0)   # gt = B
+    assert (nd["start"][2], nd["len"][2], nd["child"][2]) == (0, 1, 0)   # lt = A
+    assert list(nd["min"][1]) == [2, 0, 0] and list(nd["max"][2]) == [1, 1, 1]
+    assert list(r["list"]) == [0, 1]
+    rec = r["recs"][0]
+    assert rec["axis_method"] == (0 | 1 << 8)
+    assert np.array([rec["value"]], np.uint32).view(np.float32)[0] == 2.5
+    # cost: parent [0,3]x[0,1]^2 SA 14; children SA 6 each: 1 + 6/14*8 + 6/14*8
+    exact = 1 + Fraction(6, 14) * 16
+    cost = np.array([rec["cost_bits"]], np.uint32).view(np.float32)[0]
+    assert abs(Fraction(float(cost)) - exact) <= Fraction(2) ** -19
+
+
+def test_identical_objects_all_median():
+    n = 37
+    boxes = np.tile(np.array([[0, 0, 0, 1, 2, 3]], np.float32), (n, 1))
+    r = O.build(boxes)
+    st = r["stats"]
+    assert st["n_median"] == n - 1 and st["n_binned"] == 0 and st["n_adaptive"] == 0
+    assert st["n_tie_events"] == 0  # identical boxes: ties never change the node array
+    # median of n objects: lt = first n//2 in list order
+    assert r["recs"][0]["count_lt"] == n // 2 and r["recs"][0]["axis_method"] == (0 | 2 << 8)
+
+
+# ---------------------------------------------------------------- brute force
+def _sah(parent, lt, gt, nl, ng):
+    def sa(b):
+        w, h, d = (Fraction(float(b[3 + a])) - Fraction(float(b[a])) for a in range(3))
+        return 2 * (w * h + w * d + h * d) if min(w, h, d) >= 0 else Fraction(0)
+    return 1 + sa(lt) / sa(parent) * 8 * nl + sa(gt) / sa(parent) * 8 * ng
+
+
+def _union(b):
+    return np.concatenate([b[:, :3].min(axis=0), b[:, 3:].max(axis=0)])
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_root_split_is_minimum_small(seed):
+    """n <= 7: the chosen root split has the minimum exact SAH over every split
+    {c_axis < c_k} of every axis (all the adaptive sweep's candidates)."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 8))
+    c = rng.random((n, 3)).astype(np.float32)
+    e = (rng.random((n, 3)) * 0.2).astype(np.float32)
+    boxes = np.concatenate([c - e, c + e], axis=1).astype(np.float32)
+    r = O.build(boxes)
+    parent = _union(boxes)
+    cen = (boxes[:, :3] + boxes[:, 3:]) * np.float32(0.5)
+    best = None
+    for a in range(3):
+        for k in range(n):
+            m = cen[:, a] < cen[k, a]
+            if 0 < m.sum() < n:
+                v = _sah(parent, _union(boxes[m]), _union(boxes[~m]), int(m.sum()), int((~m).sum()))
+                best = v if best is None else min(best, v)
+    rec = r["recs"][0]
+    got = Fraction(float(np.array([rec["cost_bits"]], np.uint32).view(np.float32)[0]))
+    assert best is not None
+    assert abs(got - best) <= best * Fraction(1, 10 ** 5)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_root_split_is_minimum_binned(seed):
+    """n > 32: the chosen cost is the minimum over the 3 x 31 bin cuts, with the
+    bins computed independently here in float64 and checked for agreement."""
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(33, 400))
+    c = rng.random((n, 3)).astype(np.float32)
+    e = (rng.random((n, 3)) * 0.05).astype(np.float32)
+    boxes = np.concatenate([c - e, c + e], axis=1).astype(np.float32)
+    r = O.build(boxes)
+    parent = _union(boxes)
+    cen = ((boxes[:, :3] + boxes[:, 3:]) * np.float32(0.5)).astype(np.float32)
+    best = None
+    for a in range(3):
+        lo, hi = cen[:, a].min(), cen[:, a].max()
+        ext = np.float32(hi - lo)
+        bins = np.minimum(np.floor(((cen[:, a] - lo) / ext).astype(np.float32) * np.float32(32)), 31)
+        for i in range(1, 32):
+            m = bins < i
+            if 0 < m.sum() < n:
+                v = _sah(parent, _union(boxes[m]), _union(boxes[~m]), int(m.sum()), int((~m).sum()))
+                best = v if best is None else min(best, v)
+    rec = r["recs"][0]
+    assert rec["axis_method"] >> 8 == 0
+    got = Fraction(float(np.array([rec["cost_bits"]], np.uint32).view(np.float32)[0]))
+    assert abs(got - best) <= best * Fraction(1, 10 ** 5)
+
+
+# ---------------------------------------------------------------- invariants
+def _check_tree(boxes, r, max_depth=32):
+    nd = r["nodes"]
+    lst = r["list"]
+    n = boxes.shape[0]
+    child = nd["child"].astype(np.int64)
+    internal = np.nonzero(child)[0]
+    # full binary tree, gt at child, lt at child + 1, exact child bounds
+    assert len(nd) == 2 * np.count_nonzero(child == 0) - 1
+    for v in internal:
+        g, l = child[v], child[v] + 1
+        assert nd["start"][l] == nd["start"][v]
+        assert nd["start"][g] == nd["start"][v] + nd["len"][l]
+        assert nd["len"][g] + nd["len"][l] == nd["len"][v]
+    # leaves cover the permuted list exactly once
+    leaves = np.nonzero(child == 0)[0]
+    cover = np.zeros(n, np.int64)
+    for v in leaves:
+        cover[nd["start"][v]:nd["start"][v] + nd["len"][v]] += 1
+    assert np.all(cover == 1)
+    assert np.array_equal(np.sort(lst), np.arange(n))
+    # node bounds are the exact union of their objects (root: union when no bounds given)
+    pb = boxes[lst]
+    for v in range(len(nd)):
+        s, k = nd["start"][v], nd["len"][v]
+        u = _union(pb[s:s + k])
+        assert np.array_equal(nd["min"][v], u[:3]) and np.array_equal(nd["max"][v], u[3:])
+    # depth-first gt-first numbering (pair base of a subtree = parent pair + 2 ...)
+    order = [0]
+    stack = [0]
+    nxt = 1
+    while stack:
+        v = stack.pop()
+        if child[v]:
+            assert child[v] == nxt
+            nxt += 2
+            stack.append(child[v] + 1)
+            stack.append(child[v])
+    return nd
+
+
+def test_soup_invariants_and_counts():
+    pos, tris = scenes.soup(4000, seed=8)
+    boxes = O.tri_aabbs(pos, tris)
+    r = O.build(boxes)
+    nd = _check_tree(boxes, r)
+    assert len(nd) == 2 * 4000 - 1
+    assert np.all(nd["len"][nd["child"] == 0] == 1)
+    st = r["stats"]
+    assert st["n_assert_fail"] == 0 and st["n_tie_events"] == 0
+
+
+def test_depth_cutoff_invariants():
+    pos, tris = scenes.soup(3000, seed=5)
+    boxes = O.tri_aabbs(pos, tris)
+    for d in (0, 1, 3, 6):
+        r = O.build(boxes, max_depth=d)
+        nd = _check_tree(boxes, r, max_depth=d)
+        assert r["stats"]["height"] <= max(d - 1, 0)
+        if d == 0:
+            assert len(nd) == 1 and nd["len"][0] == 3000
+
+
+def test_node_array_independent_of_input_order():
+    pos, tris = scenes.heightfield(40, 24, seed=6)
+    boxes = O.tri_aabbs(pos, tris)
+    a = O.build(boxes)
+    perm = np.random.default_rng(1).permutation(len(boxes)).astype(np.uint32)
+    b = O.build(boxes, list_in=perm)
+    assert np.array_equal(O.encode(a["nodes"]), O.encode(b["nodes"]))
+    assert a["stats"]["n_tie_events"] == 0
+
+
+# ---------------------------------------------------------------- closest hit
+def test_ray_triangle_known_answers():
+    v1, v2, v3 = [0, 0, 0], [1, 0, 0], [0, 1, 0]
+    assert O.ray_tri([0.25, 0.25, 1, 0, 0, -1], v1, v2, v3) == 1.0
+    assert O.ray_tri([0.25, 0.25, 3, 0, 0, -1], v1, v2, v3) == 3.0
+    assert O.ray_tri([0.75, 0.75, 1, 0, 0, -1], v1, v2, v3) == float("inf")   # u + v > 1
+    assert O.ray_tri([0.25, 0.25, 1, 0, 0, 1], v1, v2, v3) == float("inf")    # behind
+    assert O.ray_tri([0.25, 0.25, 1, 1, 0, 0], v1, v2, v3) == float("inf")    # parallel
+
+
+def test_closest_hit_lowest_index_tie():
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 0], [1, 0, 0], [0, 1, 0],
+                    [0, 0, -1], [1, 0, -1], [0, 1, -1]], np.float32)
+    tris = np.array([[6, 7, 8], [3, 4, 5], [0, 1, 2]], np.uint32)
+    rays = np.array([[0.2, 0.2, 1, 0, 0, -1], [5, 5, 1, 0, 0, -1]], np.float32)
+    t, p = O.closest_hit_brute(pos, tris, rays)
+    assert t[0] == 1.0 and p[0] == 1          # triangles 1 and 2 tie at t = 1
+    assert t[1] == np.inf and p[1] == 0xFFFFFFFF
