@@ -28,23 +28,23 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 def tree_terms(nodes: np.ndarray, lens: np.ndarray):
     """Sums over the built tree used by the algorithmic byte counts:
-    s_large = sum of object counts of internal nodes with > 32 objects,
-    s_small = sum of object counts of roots of <= 32-object subtrees,
-    n_small_nodes = nodes inside those subtrees (written by k_small),
-    n_large = internal nodes with > 32 objects."""
+    s_large = object counts summed over internal nodes with > 32 objects (each
+    such level reads and writes every record once), n_large = those nodes,
+    s_small = objects in the roots of <= 32-object subtrees, n_small_nodes =
+    nodes written by the small-subtree kernel."""
     child = nodes["child_node"].astype(np.int64)
     nn = len(nodes)
     parent_len = np.full(nn, np.iinfo(np.int64).max, dtype=np.int64)
     internal = np.nonzero(child)[0]
     parent_len[child[internal]] = lens[internal]
     parent_len[child[internal] + 1] = lens[internal]
-    big = (lens > 32) & (child != 0)
+    large = (lens > 32) & (child != 0)
     small_root = (lens <= 32) & (parent_len > 32)
     return {
-        "s_large": int(lens[big].astype(np.int64).sum()),
-        "n_large": int(big.sum()),
+        "s_large": int(lens[large].astype(np.int64).sum()),
+        "n_large": int(large.sum()),
         "s_small": int(lens[small_root].astype(np.int64).sum()),
-        "n_small_nodes": int(nn - big.sum()),
+        "n_small_nodes": int(nn - large.sum()),
     }
 
 

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <random>
 #include <string>
 #include <sys/stat.h>
@@ -135,7 +136,22 @@ static void fill_stats(const std::vector<mgbvh_node32>& nodes, const std::vector
   st->avg_leaf_object_count = avg;
   st->stddev_leaf_object_count = std::sqrt(var / (double)cnt);
   st->total_object_count = (uint32_t)nobj;
-  st->height = r.height;
+  // height: deepest node that was split (the fetch_max at lib.rs:656)
+  uint32_t height = 0;
+  if (nn > 1) {
+    std::vector<std::pair<uint32_t, uint32_t>> stack;
+    stack.push_back({0u, 0u});
+    while (!stack.empty()) {
+      const std::pair<uint32_t, uint32_t> top = stack.back();
+      stack.pop_back();
+      const uint32_t ch = nodes[top.first].child_node;
+      if (ch == 0 || (uint64_t)ch + 1 >= nn) continue;
+      height = std::max(height, top.second);
+      stack.push_back({ch, top.second + 1});
+      stack.push_back({ch + 1, top.second + 1});
+    }
+  }
+  st->height = height;
   st->max_depth = s->max_depth;
   st->n_large_levels = r.n_levels;
   st->n_median = r.n_median;

@@ -275,67 +275,57 @@ __device__ inline void eval_cut(const uint32_t* h, int a, uint32_t i, Box& lt, B
   }
 }
 
-__device__ inline void emit_child(uint32_t lane, uint32_t start, uint32_t count, const Box& b,
-                                  uint32_t idx, uint32_t P, uint32_t depth, uint32_t max_depth,
-                                  uint32_t parity_next, LNode* Ln, uint32_t* cn_next,
-                                  uint32_t* ck_next, SNode* small, Counters* ctr,
-                                  int32_t& slot_out) {
-  const bool large = count > MG_SMALL && depth < max_depth;
-  uint32_t s = 0, c0 = 0, nch = 0;
-  if (lane == 0) {
-    if (large) {
-      s = atomicAdd(&ctr->n_large_next, 1u);
-      nch = (count + CH - 1) / CH;
-      c0 = atomicAdd(&ctr->n_chunks_next, nch);
-      LNode r = {};
-      r.start = start; r.count = count; r.idx = idx; r.P = P; r.depth = depth;
-      r.chunk0 = c0; r.nchunk = nch;
-      for (int a = 0; a < 3; ++a) { r.bmn[a] = b.mn[a]; r.bmx[a] = b.mx[a]; }
-      r.child_slot[0] = r.child_slot[1] = -1;
-      Ln[s] = r;
-    } else {
-      s = atomicAdd(&ctr->n_small, 1u);
-      SNode sn = {};
-      sn.start = start; sn.count = count; sn.idx = idx; sn.P = P; sn.depth = depth;
-      sn.parity = parity_next;
-      for (int a = 0; a < 3; ++a) { sn.bmn[a] = b.mn[a]; sn.bmx[a] = b.mx[a]; }
-      small[s] = sn;
-    }
-  }
-  s = __shfl(s, 0, 64);
-  c0 = __shfl(c0, 0, 64);
-  nch = __shfl(nch, 0, 64);
+// Child record of a split: a node with > 32 objects for the next level (slot s,
+// chunks [c0, c0 + nch)), or a subtree root of <= 32 objects / a depth-cut leaf.
+__device__ inline void put_child(bool large, uint32_t s, uint32_t c0, uint32_t start,
+                                 uint32_t count, const Box& b, uint32_t idx, uint32_t P,
+                                 uint32_t depth, uint32_t parity_next, LNode* Ln, SNode* small) {
   if (large) {
-    for (uint32_t k = lane; k < nch; k += 64) {
-      cn_next[c0 + k] = s;
-      ck_next[c0 + k] = k;
-    }
-    slot_out = (int32_t)s;
+    LNode r = {};
+    r.start = start; r.count = count; r.idx = idx; r.P = P; r.depth = depth;
+    r.chunk0 = c0; r.nchunk = (count + CH - 1) / CH;
+    for (int a = 0; a < 3; ++a) { r.bmn[a] = b.mn[a]; r.bmx[a] = b.mx[a]; }
+    r.child_slot[0] = r.child_slot[1] = -1;
+    Ln[s] = r;
   } else {
-    slot_out = -1;
+    SNode sn = {};
+    sn.start = start; sn.count = count; sn.idx = idx; sn.P = P; sn.depth = depth;
+    sn.parity = parity_next;
+    for (int a = 0; a < 3; ++a) { sn.bmn[a] = b.mn[a]; sn.bmx[a] = b.mx[a]; }
+    small[s] = sn;
   }
 }
 
+constexpr uint32_t SW_WAVES = 16;  // nodes per k_sweep block (one wavefront each)
+
 // One wavefront per node with > 32 objects: select_split (lib.rs:493-534) over
-// the binned histograms, node record, children.
-__global__ __launch_bounds__(256) void k_sweep(LNode* __restrict__ L, uint32_t nL,
-                                               const uint32_t* __restrict__ hist,
-                                               const Rec* __restrict__ recs, LNode* Ln,
-                                               uint32_t* cn_next, uint32_t* ck_next,
-                                               SNode* small, Counters* ctr,
-                                               NodeOut out, uint32_t max_depth,
-                                               uint32_t parity_next) {
-  __shared__ uint32_t sh[4][HIST_WORDS];
+// the binned histograms, node record, children.  Children are appended to the
+// next level's lists with one atomic per list per block (a same-address atomic
+// costs ~11 ns, serialised, so per-node atomics would dominate).
+__global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t nL,
+                                                const uint32_t* __restrict__ hist,
+                                                const Rec* __restrict__ recs, LNode* Ln,
+                                                uint32_t* cn_next, uint32_t* ck_next,
+                                                SNode* small, Counters* ctr,
+                                                NodeOut out, uint32_t max_depth,
+                                                uint32_t parity_next) {
+  __shared__ uint32_t sh[SW_WAVES][HIST_WORDS];
+  __shared__ uint32_t s_need[SW_WAVES][3];  // large slots, chunks, small slots
+  __shared__ uint32_t s_base[SW_WAVES][3];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t lane = lane_id();
-  const uint32_t slot = blockIdx.x * 4 + w;
-  if (slot >= nL) return;  // only wave-local synchronisation below
+  const uint32_t slot = blockIdx.x * SW_WAVES + w;
+  const bool active = slot < nL;  // wave-uniform; every wave reaches the barriers
   uint32_t* h = sh[w];
+  Box lt = box_empty(), gt = box_empty(), nb = box_empty();
+  uint32_t nl = 0, ng = 0, axis = 0, value = 0, kind = 0;
+  float cost = 0.0f;
+  LNode nd = {};
+  if (active) {
   const uint32_t* gh = hist + (size_t)slot * HIST_WORDS;
   for (uint32_t t = lane; t < HIST_WORDS; t += 64) h[t] = gh[t];
   wave_sync();
-  LNode nd = L[slot];
-  Box nb;
+  nd = L[slot];
   float cmin[3], cext[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -351,12 +341,12 @@ __global__ __launch_bounds__(256) void k_sweep(LNode* __restrict__ L, uint32_t n
 #pragma unroll
   for (uint32_t pass = 0; pass < 2; ++pass) {
     const uint32_t q = lane + 64 * pass;
-    const bool active = q < 3 * MG_NBINS;
+    const bool qact = q < 3 * MG_NBINS;
     const uint32_t b = q % MG_NBINS;
-    const int a = active ? (int)(q / MG_NBINS) : 0;
+    const int a = qact ? (int)(q / MG_NBINS) : 0;
     uint32_t cnt = 0;
     Box bb = box_empty();
-    if (active) {
+    if (qact) {
       const uint32_t* hb = &h[q * BIN_WORDS];
       cnt = hb[0];
       if (cnt) bb = hist_box(hb);
@@ -379,26 +369,23 @@ __global__ __launch_bounds__(256) void k_sweep(LNode* __restrict__ L, uint32_t n
       if (b + d < MG_NBINS) { sc += uc; box_grow(sb, ub); }
     }
     // lt of cut b is the prefix ending at bin b - 1
-    const uint32_t nl = __shfl_up(pc, 1, MG_NBINS);
-    Box lt;
+    const uint32_t cnl = __shfl_up(pc, 1, MG_NBINS);
+    Box clt;
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-      lt.mn[x] = __shfl_up(pb.mn[x], 1, MG_NBINS);
-      lt.mx[x] = __shfl_up(pb.mx[x], 1, MG_NBINS);
+      clt.mn[x] = __shfl_up(pb.mn[x], 1, MG_NBINS);
+      clt.mx[x] = __shfl_up(pb.mx[x], 1, MG_NBINS);
     }
-    const uint32_t ng = sc;
+    const uint32_t cng = sc;
     // lib.rs:339-341 (zero extent refused), 379-381 (MIN objects per side)
-    if (active && b >= 1 && cext[a] != 0.0f && nl >= 1 && ng >= 1) {
-      const float cost = split_cost(sa_p, box_sa(lt), box_sa(sb), nl, ng);
-      const uint64_t key = ((uint64_t)tc_ukey(cost) << 32) | q;
+    if (qact && b >= 1 && cext[a] != 0.0f && cnl >= 1 && cng >= 1) {
+      const float ccost = split_cost(sa_p, box_sa(clt), box_sa(sb), cnl, cng);
+      const uint64_t key = ((uint64_t)tc_ukey(ccost) << 32) | q;
       best = key < best ? key : best;
     }
   }
   best = wave_min_u64(best);
 
-  Box lt, gt;
-  uint32_t nl, ng, axis, value, kind;
-  float cost;
   if (best != ~0ull) {
     const uint32_t q = (uint32_t)best;
     axis = q / MG_NBINS;
@@ -442,26 +429,52 @@ __global__ __launch_bounds__(256) void k_sweep(LNode* __restrict__ L, uint32_t n
     cost = split_cost(sa_p, box_sa(lt), box_sa(gt), nl, ng);
   }
 
+  }  // active
+
+  // children (lib.rs:648-663): gt at P, lt at P + 1
+  const uint32_t cd = nd.depth + 1;
+  const bool lg = active && ng > MG_SMALL && cd < max_depth;
+  const bool ll = active && nl > MG_SMALL && cd < max_depth;
+  const uint32_t chg = lg ? (ng + CH - 1) / CH : 0, chl = ll ? (nl + CH - 1) / CH : 0;
+  if (lane == 0) {
+    s_need[w][0] = (uint32_t)lg + (uint32_t)ll;
+    s_need[w][1] = chg + chl;
+    s_need[w][2] = active ? 2 - ((uint32_t)lg + (uint32_t)ll) : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const uint32_t k = threadIdx.x;
+    uint32_t tot = 0;
+    for (uint32_t q = 0; q < SW_WAVES; ++q) tot += s_need[q][k];
+    uint32_t* counter = k == 0 ? &ctr->n_large_next : (k == 1 ? &ctr->n_chunks_next : &ctr->n_small);
+    uint32_t run = tot ? atomicAdd(counter, tot) : 0;
+    for (uint32_t q = 0; q < SW_WAVES; ++q) {
+      s_base[q][k] = run;
+      run += s_need[q][k];
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const uint32_t bl = s_base[w][0], bc = s_base[w][1], bs = s_base[w][2];
+  const uint32_t sgs = lg ? bl : bs;                          // gt child slot
+  const uint32_t sls = ll ? bl + (uint32_t)lg : bs + (uint32_t)!lg;  // lt child slot
+  const uint32_t c0g = bc, c0l = bc + chg;
   if (lane == 0) {
     out.write_node(nd.idx, nb, nd.start, nd.P, nd.count);
     out.write_split(nd.idx, axis | (kind << 8), value, nl, fbits(cost));
-    atomicMax(&ctr->height, nd.depth);
     if (kind == 2) atomicAdd(&ctr->n_median, 1u);
+    put_child(lg, sgs, c0g, nd.start + nl, ng, gt, nd.P, nd.P + 2, cd, parity_next, Ln, small);
+    put_child(ll, sls, c0l, nd.start, nl, lt, nd.P + 1, nd.P + 2 + 2 * (ng - 1), cd, parity_next,
+              Ln, small);
     L[slot].axis = axis;
     L[slot].cut = value;
     L[slot].n_lt = nl;
     L[slot].kind = kind;
+    L[slot].child_slot[0] = lg ? (int32_t)sgs : -1;
+    L[slot].child_slot[1] = ll ? (int32_t)sls : -1;
   }
-  const uint32_t cd = nd.depth + 1;
-  int32_t sg, sl;
-  emit_child(lane, nd.start + nl, ng, gt, nd.P, nd.P + 2, cd, max_depth, parity_next, Ln,
-             cn_next, ck_next, small, ctr, sg);
-  emit_child(lane, nd.start, nl, lt, nd.P + 1, nd.P + 2 + 2 * (ng - 1), cd, max_depth,
-             parity_next, Ln, cn_next, ck_next, small, ctr, sl);
-  if (lane == 0) {
-    L[slot].child_slot[0] = sg;
-    L[slot].child_slot[1] = sl;
-  }
+  for (uint32_t k = lane; k < chg; k += 64) { cn_next[c0g + k] = sgs; ck_next[c0g + k] = k; }
+  for (uint32_t k = lane; k < chl; k += 64) { cn_next[c0l + k] = sls; ck_next[c0l + k] = k; }
 }
 
 // ------------------------------------------------------------------ partition
@@ -697,8 +710,6 @@ __global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small, 
   obj_put(S, lane, o);
   wave_sync();
 
-  uint32_t hmax = 0;
-  bool split_any = false;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   while (__ballot(has)) {
     // leaves: one object, or depth == max_depth (lib.rs:548-558)
@@ -841,8 +852,6 @@ __global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small, 
         out.write_split(idx, axis | (kind << 8), value, n_lt, fbits(cost));
         if (kind == 2) atomicAdd(&ctr->n_median, 1u);
       }
-      hmax = max(hmax, depth);
-      split_any = true;
       // children (lib.rs:648-663): gt at P, lt at P + 1
       const uint32_t ng2 = lc - n_lt;
       if (lane < ls + n_lt) {
@@ -860,8 +869,6 @@ __global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small, 
       ++depth;
     }
   }
-  hmax = wave_max_u32(split_any ? hmax + 1 : 0u);
-  if (lane == 0 && hmax) atomicMax(&ctr->height, hmax - 1);
 }
 
 // ------------------------------------------------------------------ compaction
@@ -1000,7 +1007,7 @@ int build_device(mgbvh_ctx* c, const BuildInput& in, uint32_t n, const mgbvh_set
     k_bin<<<nC, 256, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk);
     MG_PROF(KK_BIN, 1);
     MG_PROF(KK_SWEEP, 0);
-    k_sweep<<<cdiv(nL, 4), 256, 0, st>>>(L[cur], nL, hist, recs[cur], L[cur ^ 1], cn[cur ^ 1],
+    k_sweep<<<cdiv(nL, SW_WAVES), 1024, 0, st>>>(L[cur], nL, hist, recs[cur], L[cur ^ 1], cn[cur ^ 1],
                                          ck[cur ^ 1], small, ctr, out, s->max_depth,
                                          (uint32_t)(cur ^ 1));
     MG_PROF(KK_SWEEP, 1);
