@@ -605,3 +605,66 @@ void or_closest_hit_brute(const float* pos, uint32_t vs, const uint32_t* tris, u
     prim_out[r] = bp;
   }
 }
+
+/* Slab test of the traversal (not the reference's box test, which may cull a
+ * box holding the closest hit by rounding): the far distance is widened by
+ * (1 + 2^-20) and NaNs from 0 * inf are ignored, the near distance must not
+ * exceed the current closest t widened by (1 + 2^-10). */
+#define OR_TF_WIDEN 1.00000095367431640625f /* 1 + 2^-20 */
+#define OR_TN_SLACK 1.0009765625f           /* 1 + 2^-10 */
+
+static inline float nanmin(float a, float b) { return a != a ? b : (b != b ? a : (a < b ? a : b)); }
+static inline float nanmax(float a, float b) { return a != a ? b : (b != b ? a : (a > b ? a : b)); }
+
+static int slab(const float* o, const float* inv, const float* bmn, const float* bmx, float best,
+                float* tn_out) {
+  float tn = -INFINITY, tf = INFINITY;
+  for (int a = 0; a < 3; ++a) {
+    float t0 = (bmn[a] - o[a]) * inv[a];
+    float t1 = (bmx[a] - o[a]) * inv[a];
+    tn = nanmax(tn, nanmin(t0, t1));
+    tf = nanmin(tf, nanmax(t0, t1));
+  }
+  tf = tf * OR_TF_WIDEN;
+  *tn_out = tn;
+  return tn <= tf && tf >= 0.0f && tn <= best * OR_TN_SLACK;
+}
+
+void or_closest_hit_bvh(const uint32_t* nodes8, const uint32_t* lens, const float* pos,
+                        uint32_t vs, const uint32_t* tris, const uint32_t* perm, const float* rays,
+                        uint32_t nrays, float* t_out, uint32_t* prim_out) {
+  for (uint32_t r = 0; r < nrays; ++r) {
+    const float* ray = rays + 6 * (size_t)r;
+    float inv[3];
+    for (int a = 0; a < 3; ++a) inv[a] = 1.0f / ray[3 + a];
+    float best = INFINITY;
+    uint32_t bp = 0xffffffffu;
+    uint32_t stack[256];
+    int sp = 0;
+    float tn;
+    const float* b0 = (const float*)nodes8;
+    if (slab(ray, inv, b0, b0 + 4, best, &tn)) stack[sp++] = 0;
+    while (sp > 0) {
+      uint32_t v = stack[--sp];
+      const uint32_t* nd = nodes8 + 8 * (size_t)v;
+      const float* bf = (const float*)nd;
+      if (!slab(ray, inv, bf, bf + 4, best, &tn)) continue;
+      if (nd[7] == 0) {
+        uint32_t len = lens ? lens[v] : 1;
+        for (uint32_t k = nd[3]; k < nd[3] + len; ++k) {
+          const uint32_t* tr = tris + 3 * (size_t)k;
+          float t = or_ray_tri(ray, pos + (size_t)tr[0] * vs, pos + (size_t)tr[1] * vs,
+                               pos + (size_t)tr[2] * vs);
+          uint32_t id = perm[k];
+          if (t < best || (t == best && t != INFINITY && id < bp)) { best = t; bp = id; }
+        }
+      } else {
+        if (sp + 2 > 256) continue;
+        stack[sp++] = nd[7] + 1;
+        stack[sp++] = nd[7];
+      }
+    }
+    t_out[r] = best;
+    prim_out[r] = bp;
+  }
+}

@@ -77,6 +77,14 @@ void or_closest_hit_brute(const float* pos, uint32_t vstride_floats, const uint3
                           float* t_out, uint32_t* prim_out);
 float or_ray_tri(const float* ray6, const float* v1, const float* v2, const float* v3);
 
+/* Closest hit by traversing a built tree (32-byte nodes as (k, 8) u32, lens
+ * optional for depth-cut leaves): conservative slab test, so the result equals
+ * the brute force.  tris_perm: the permuted triangle list, perm: original
+ * index of each position.  rays: n x {ox,oy,oz,dx,dy,dz}. */
+void or_closest_hit_bvh(const uint32_t* nodes8, const uint32_t* lens, const float* pos,
+                        uint32_t vstride_floats, const uint32_t* tris_perm, const uint32_t* perm,
+                        const float* rays, uint32_t nrays, float* t_out, uint32_t* prim_out);
+
 #ifdef __cplusplus
 }
 #endif

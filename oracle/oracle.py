@@ -69,6 +69,7 @@ def lib():
         L.or_binned_cost.argtypes = [P, P, P, C.c_uint32, C.c_uint32]
         L.or_binned_cost.restype = C.c_float
         L.or_closest_hit_brute.argtypes = [P, C.c_uint32, P, C.c_uint32, P, C.c_uint32, P, P]
+        L.or_closest_hit_bvh.argtypes = [P, P, P, C.c_uint32, P, P, P, C.c_uint32, P, P]
         L.or_ray_tri.argtypes = [P, P, P, P]
         L.or_ray_tri.restype = C.c_float
         _lib = L
@@ -157,4 +158,19 @@ def closest_hit_brute(pos, tris, rays):
     prim = np.empty(rays.shape[0], dtype=np.uint32)
     lib().or_closest_hit_brute(_p(pos), pos.shape[1], _p(tris), tris.shape[0], _p(rays),
                                rays.shape[0], _p(t), _p(prim))
+    return t, prim
+
+
+def closest_hit_bvh(nodes8, lens, pos, tris_perm, perm, rays):
+    """Closest hit through a built tree: nodes8 = (k, 8) u32 node records."""
+    nodes8 = np.ascontiguousarray(nodes8, dtype=np.uint32)
+    lens = None if lens is None else np.ascontiguousarray(lens, dtype=np.uint32)
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    tris_perm = np.ascontiguousarray(tris_perm, dtype=np.uint32)
+    perm = np.ascontiguousarray(perm, dtype=np.uint32)
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    t = np.empty(rays.shape[0], dtype=np.float32)
+    prim = np.empty(rays.shape[0], dtype=np.uint32)
+    lib().or_closest_hit_bvh(_p(nodes8), _p(lens), _p(pos), pos.shape[1], _p(tris_perm), _p(perm),
+                             _p(rays), rays.shape[0], _p(t), _p(prim))
     return t, prim
