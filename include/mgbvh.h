@@ -172,6 +172,96 @@ int mgbvh_ctx_set_profiling(mgbvh_ctx* ctx, uint32_t kernel_mask);
 int mgbvh_ctx_kernel_times(mgbvh_ctx* ctx, double* ms /* [16] */, uint64_t* launches /* [16] */,
                            int reset);
 
+/* ---- closest-hit traversal --------------------------------------------------
+ * Replaces the GPU traversal of the path tracer, scene.slang:90-221
+ * (raytraceTlas / raytraceBlas) with the triangle test of scene.slang:19-89.
+ * Hit rules (tree independent, so any valid tree of the same objects gives the
+ * same hits): Moller-Trumbore in the reference's operation order; closest t;
+ * equal t goes to the lowest (instance, triangle) original index; the box test
+ * never culls a box that holds the closest hit.  A leaf holds lens[v] objects
+ * (d_lens NULL: one, as the 32-byte node format stores no length). */
+typedef struct mgbvh_ray {
+  float org[3];
+  uint32_t reserved;   /* must be 0 */
+  float dir[3];
+  float tmax;          /* hits with t > tmax are ignored (+inf: none) */
+} mgbvh_ray;
+
+typedef struct mgbvh_hit {
+  float t;             /* tmax when there is no hit */
+  uint32_t prim;       /* original triangle index in its mesh, 0xffffffff = miss */
+  uint32_t inst;       /* original instance index (0 for a single BLAS), 0xffffffff = miss */
+  uint32_t tri_slot;   /* position of the hit triangle in the permuted (concatenated)
+                          triangle list, the index the reference's shader uses */
+} mgbvh_hit;
+
+/* One BLAS inside concatenated arrays (the fields of Mesh, scene.slang:245-254). */
+typedef struct mgbvh_blas_desc {
+  uint32_t node_offset;    /* first node (Mesh.blasRoot); child indices are relative */
+  uint32_t tri_offset;     /* first triangle of the permuted list (Mesh.triangleOffset) */
+  uint32_t vertex_offset;  /* added to triangle vertex indices (Mesh.vertexOffset) */
+  uint32_t reserved;
+} mgbvh_blas_desc;
+
+/* One TLAS object: a BLAS placed by an affine transform (Mesh.transform).
+ * world_to_object = Mesh.inverseTransform as 3 rows x 4 floats; the ray is
+ * taken to object space unnormalised, so t is the same parameter in both. */
+typedef struct mgbvh_instance {
+  float world_to_object[12];
+  uint32_t blas;
+  uint32_t reserved[3];
+} mgbvh_instance;
+
+/* Pinhole camera of the primary rays (screen.slang:48-72 without jitter). */
+typedef struct mgbvh_camera {
+  float pos[3];
+  float fwd[3];
+  float right[3];
+  float up[3];
+  float fov_y;   /* radians */
+  float aspect;  /* width / height */
+} mgbvh_camera;
+
+/* Closest hit of each ray in one BLAS.  d_tris_perm / d_perm: the permuted
+ * triangle list and the original index at each position (build outputs). */
+int mgbvh_trace_closest_device(mgbvh_ctx* ctx, const mgbvh_node32* d_nodes, const uint32_t* d_lens,
+                               const float* d_positions, uint32_t vstride_bytes,
+                               const uint32_t* d_tris_perm, const uint32_t* d_perm,
+                               const mgbvh_ray* d_rays, uint64_t nrays, mgbvh_hit* d_hits);
+
+/* Closest hit of each ray in a two-level scene.  d_instances is in the
+ * caller's original order; d_tlas_perm (the TLAS build's perm output) maps a
+ * TLAS leaf's start_index to an instance, so the instance list is never
+ * reordered.  d_tlas_lens optional.  BLASes are concatenated as d_blas says:
+ * node, lens, permuted-triangle and perm arrays share the offsets. */
+int mgbvh_trace_scene_device(mgbvh_ctx* ctx, const mgbvh_node32* d_tlas_nodes,
+                             const uint32_t* d_tlas_lens, const uint32_t* d_tlas_perm,
+                             const mgbvh_instance* d_instances, uint32_t ninstances,
+                             const mgbvh_blas_desc* d_blas, uint32_t nblas,
+                             const mgbvh_node32* d_blas_nodes, const uint32_t* d_blas_lens,
+                             const float* d_positions, uint32_t vstride_bytes,
+                             const uint32_t* d_tris_perm, const uint32_t* d_blas_perm,
+                             const mgbvh_ray* d_rays, uint64_t nrays, mgbvh_hit* d_hits);
+
+/* Primary rays of a width x height image, row-major from the top-left pixel. */
+int mgbvh_gen_primary_rays_device(mgbvh_ctx* ctx, const mgbvh_camera* cam, uint32_t width,
+                                  uint32_t height, mgbvh_ray* d_rays);
+
+/* One cosine-weighted bounce per ray (pathtrace.slang:49, random.slang:5-41,
+ * intersect.slang:49-51): from the hit point offset 0.001 along the world
+ * normal facing the ray, direction normalize(normal + random unit vector); a
+ * missed ray is re-cast from its origin in a random direction.  The triangle
+ * is d_tris_perm[hit.tri_slot]; for a scene pass the instances and BLAS table
+ * (vertex offset, normal taken to world space), else NULL and 0.  seed: the
+ * frame offset added to the ray index to seed its PCG stream. */
+int mgbvh_gen_bounce_rays_device(mgbvh_ctx* ctx, const mgbvh_ray* d_rays_in,
+                                 const mgbvh_hit* d_hits, uint64_t nrays,
+                                 const mgbvh_instance* d_instances, uint32_t ninstances,
+                                 const mgbvh_blas_desc* d_blas, uint32_t nblas,
+                                 const float* d_positions, uint32_t vstride_bytes,
+                                 const uint32_t* d_tris_perm, uint32_t seed,
+                                 mgbvh_ray* d_rays_out);
+
 /* ---- timing helpers: HIP events recorded on the context stream ---- */
 int mgbvh_event_record(mgbvh_ctx* ctx, int slot);           /* slot 0..59 */
 int mgbvh_event_elapsed_ms(mgbvh_ctx* ctx, int a, int b, float* ms);

@@ -30,6 +30,31 @@ NODE_DTYPE = np.dtype([("min", "<f4", (3,)), ("start_index", "<u4"), ("max", "<f
 SPLIT_DTYPE = np.dtype([("axis_method", "<u4"), ("value", "<u4"), ("count_lt", "<u4"),
                         ("cost_bits", "<u4")])
 assert NODE_DTYPE.itemsize == 32 and SPLIT_DTYPE.itemsize == 16
+RAY_DTYPE = np.dtype([("org", "<f4", (3,)), ("reserved", "<u4"), ("dir", "<f4", (3,)),
+                      ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<u4"), ("inst", "<u4"), ("tri_slot", "<u4")])
+BLAS_DTYPE = np.dtype([("node_offset", "<u4"), ("tri_offset", "<u4"), ("vertex_offset", "<u4"),
+                       ("reserved", "<u4")])
+INSTANCE_DTYPE = np.dtype([("world_to_object", "<f4", (12,)), ("blas", "<u4"),
+                           ("reserved", "<u4", (3,))])
+assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16
+assert BLAS_DTYPE.itemsize == 16 and INSTANCE_DTYPE.itemsize == 64
+MISS = 0xFFFFFFFF
+
+
+class Camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("fwd", C.c_float * 3), ("right", C.c_float * 3),
+                ("up", C.c_float * 3), ("fov_y", C.c_float), ("aspect", C.c_float)]
+
+
+def make_rays(org, dirs, tmax=np.inf) -> np.ndarray:
+    """(N,3) origins and directions -> RAY_DTYPE array."""
+    dirs = np.asarray(dirs, dtype=np.float32)
+    r = np.zeros(dirs.shape[0], dtype=RAY_DTYPE)
+    r["org"] = org
+    r["dir"] = dirs
+    r["tmax"] = tmax
+    return r
 
 
 class Settings(C.Structure):
@@ -128,7 +153,14 @@ def _sig_extra(lib):
             f.restype = restype
 
 
-_EXTRA: dict = {}
+_EXTRA: dict = {
+    "mgbvh_trace_closest_device": ([_P, _P, _P, _P, _U32, _P, _P, _P, _U64, _P], _I),
+    "mgbvh_trace_scene_device": ([_P, _P, _P, _P, _P, _U32, _P, _U32, _P, _P, _P, _U32, _P, _P,
+                                  _P, _U64, _P], _I),
+    "mgbvh_gen_primary_rays_device": ([_P, C.POINTER(Camera), _U32, _U32, _P], _I),
+    "mgbvh_gen_bounce_rays_device": ([_P, _P, _P, _U64, _P, _U32, _P, _U32, _P, _U32, _P, _U32,
+                                      _P], _I),
+}
 
 
 def lib():
@@ -212,6 +244,11 @@ class DeviceBuffer:
         self.ptr = None
 
 
+def _dv(b):
+    """Device pointer of a DeviceBuffer (None -> NULL)."""
+    return None if b is None else b.ptr
+
+
 class Context:
     def __init__(self, device: int = 0):
         h = C.c_void_p()
@@ -269,6 +306,29 @@ class Context:
 
     def __exit__(self, *exc):
         self.close()
+
+    # ---- traversal (device buffers: DeviceBuffer or None) ----------------------
+    def trace_closest(self, nodes, lens, pos, vstride, tris_perm, perm, rays, nrays, hits):
+        check(lib().mgbvh_trace_closest_device(self.handle, _dv(nodes), _dv(lens), _dv(pos),
+                                               vstride, _dv(tris_perm), _dv(perm), _dv(rays),
+                                               nrays, _dv(hits)))
+
+    def trace_scene(self, tlas_nodes, tlas_lens, tlas_perm, instances, ninstances, blas, nblas,
+                    blas_nodes, blas_lens, pos, vstride, tris_perm, blas_perm, rays, nrays, hits):
+        check(lib().mgbvh_trace_scene_device(
+            self.handle, _dv(tlas_nodes), _dv(tlas_lens), _dv(tlas_perm), _dv(instances),
+            ninstances, _dv(blas), nblas, _dv(blas_nodes), _dv(blas_lens), _dv(pos), vstride,
+            _dv(tris_perm), _dv(blas_perm), _dv(rays), nrays, _dv(hits)))
+
+    def gen_primary(self, cam: "Camera", width: int, height: int, rays):
+        check(lib().mgbvh_gen_primary_rays_device(self.handle, C.byref(cam), width, height,
+                                                  _dv(rays)))
+
+    def gen_bounce(self, rays_in, hits, nrays, instances, ninstances, blas, nblas, pos, vstride,
+                   tris_perm, seed, rays_out):
+        check(lib().mgbvh_gen_bounce_rays_device(
+            self.handle, _dv(rays_in), _dv(hits), nrays, _dv(instances), ninstances, _dv(blas),
+            nblas, _dv(pos), vstride, _dv(tris_perm), seed, _dv(rays_out)))
 
     # ---- host-buffer builds -------------------------------------------------
     def build_tris(self, positions: np.ndarray, tris: np.ndarray, settings: Settings,

@@ -608,13 +608,12 @@ void or_closest_hit_brute(const float* pos, uint32_t vs, const uint32_t* tris, u
 
 /* Slab test of the traversal (not the reference's box test, which may cull a
  * box holding the closest hit by rounding): the far distance is widened by
- * (1 + 2^-20) and NaNs from 0 * inf are ignored, the near distance must not
- * exceed the current closest t widened by (1 + 2^-10). */
+ * (1 + 2^-20), an axis whose plane distance is 0 * inf (the ray runs inside a
+ * face plane) does not bound t, and the near distance must not exceed the
+ * current closest t widened by (1 + 2^-10). */
 #define OR_TF_WIDEN 1.00000095367431640625f /* 1 + 2^-20 */
 #define OR_TN_SLACK 1.0009765625f           /* 1 + 2^-10 */
 
-static inline float nanmin(float a, float b) { return a != a ? b : (b != b ? a : (a < b ? a : b)); }
-static inline float nanmax(float a, float b) { return a != a ? b : (b != b ? a : (a > b ? a : b)); }
 
 static int slab(const float* o, const float* inv, const float* bmn, const float* bmx, float best,
                 float* tn_out) {
@@ -622,8 +621,10 @@ static int slab(const float* o, const float* inv, const float* bmn, const float*
   for (int a = 0; a < 3; ++a) {
     float t0 = (bmn[a] - o[a]) * inv[a];
     float t1 = (bmx[a] - o[a]) * inv[a];
-    tn = nanmax(tn, nanmin(t0, t1));
-    tf = nanmin(tf, nanmax(t0, t1));
+    if (t0 != t0 || t1 != t1) continue; /* 0 * inf: the ray runs inside a face plane */
+    float lo = t0 < t1 ? t0 : t1, hi = t0 < t1 ? t1 : t0;
+    tn = tn > lo ? tn : lo;
+    tf = tf < hi ? tf : hi;
   }
   tf = tf * OR_TF_WIDEN;
   *tn_out = tn;
@@ -666,5 +667,46 @@ void or_closest_hit_bvh(const uint32_t* nodes8, const uint32_t* lens, const floa
     }
     t_out[r] = best;
     prim_out[r] = bp;
+  }
+}
+
+/* World ray to object space, as the traversal does (row dot products summed
+ * left to right, no FMA). */
+static void to_object(const float* m, const float* w, float* r) {
+  for (int i = 0; i < 3; ++i) {
+    const float* row = m + 4 * i;
+    float o = row[0] * w[0] + row[1] * w[1];
+    o = o + row[2] * w[2];
+    r[i] = o + row[3];
+    float d = row[0] * w[3] + row[1] * w[4];
+    r[3 + i] = d + row[2] * w[5];
+  }
+}
+
+void or_closest_hit_scene_brute(const float* pos, uint32_t vs, const uint32_t* tris,
+                                const uint32_t* mesh3, const uint32_t* inst16, uint32_t ninst,
+                                const float* rays, uint32_t nrays, float* t_out,
+                                uint32_t* prim_out, uint32_t* inst_out) {
+  for (uint32_t r = 0; r < nrays; ++r) {
+    float best = INFINITY;
+    uint32_t bp = 0xffffffffu, bi = 0xffffffffu;
+    for (uint32_t i = 0; i < ninst; ++i) {
+      const uint32_t* rec = inst16 + 16 * (size_t)i;
+      float m[12];
+      memcpy(m, rec, sizeof m);
+      const uint32_t* mesh = mesh3 + 3 * (size_t)rec[12];
+      float lr[6];
+      to_object(m, rays + 6 * (size_t)r, lr);
+      for (uint32_t k = 0; k < mesh[1]; ++k) {
+        const uint32_t* tr = tris + 3 * ((size_t)mesh[0] + k);
+        float t = or_ray_tri(lr, pos + (size_t)(tr[0] + mesh[2]) * vs,
+                             pos + (size_t)(tr[1] + mesh[2]) * vs,
+                             pos + (size_t)(tr[2] + mesh[2]) * vs);
+        if (t < best) { best = t; bp = k; bi = i; } /* ascending (i, k): strict < keeps lowest */
+      }
+    }
+    t_out[r] = best;
+    prim_out[r] = bp;
+    inst_out[r] = bi;
   }
 }

@@ -85,6 +85,18 @@ void or_closest_hit_bvh(const uint32_t* nodes8, const uint32_t* lens, const floa
                         uint32_t vstride_floats, const uint32_t* tris_perm, const uint32_t* perm,
                         const float* rays, uint32_t nrays, float* t_out, uint32_t* prim_out);
 
+/* Closest hit in a two-level scene by brute force: every instance (ascending
+ * index) takes the ray to object space with its 3x4 world_to_object rows
+ * (((m0*x + m1*y) + m2*z) + m3, direction without m3) and tests every triangle
+ * of its mesh (ascending index); (t, inst, prim) lexicographic minimum.
+ * inst16: ninst records of 16 words (12 floats, BLAS index at word 12).
+ * mesh3: per mesh {first triangle, triangle count, vertex offset} into the
+ * concatenated ORIGINAL-order triangle list. */
+void or_closest_hit_scene_brute(const float* pos, uint32_t vstride_floats, const uint32_t* tris,
+                                const uint32_t* mesh3, const uint32_t* inst16, uint32_t ninst,
+                                const float* rays, uint32_t nrays, float* t_out,
+                                uint32_t* prim_out, uint32_t* inst_out);
+
 #ifdef __cplusplus
 }
 #endif

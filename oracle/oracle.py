@@ -70,6 +70,8 @@ def lib():
         L.or_binned_cost.restype = C.c_float
         L.or_closest_hit_brute.argtypes = [P, C.c_uint32, P, C.c_uint32, P, C.c_uint32, P, P]
         L.or_closest_hit_bvh.argtypes = [P, P, P, C.c_uint32, P, P, P, C.c_uint32, P, P]
+        L.or_closest_hit_scene_brute.argtypes = [P, C.c_uint32, P, P, P, C.c_uint32, P,
+                                                 C.c_uint32, P, P, P]
         L.or_ray_tri.argtypes = [P, P, P, P]
         L.or_ray_tri.restype = C.c_float
         _lib = L
@@ -174,3 +176,22 @@ def closest_hit_bvh(nodes8, lens, pos, tris_perm, perm, rays):
     lib().or_closest_hit_bvh(_p(nodes8), _p(lens), _p(pos), pos.shape[1], _p(tris_perm), _p(perm),
                              _p(rays), rays.shape[0], _p(t), _p(prim))
     return t, prim
+
+
+def closest_hit_scene_brute(pos, tris, mesh3, inst16, rays):
+    """Two-level closest hit by brute force: tris = concatenated original-order
+    triangles, mesh3 = (M, 3) {first triangle, count, vertex offset},
+    inst16 = (I, 16) u32 instance records (12 floats, mesh at word 12),
+    rays = (N, 6).  Returns (t, prim, inst)."""
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    tris = np.ascontiguousarray(tris, dtype=np.uint32)
+    mesh3 = np.ascontiguousarray(mesh3, dtype=np.uint32)
+    inst16 = np.ascontiguousarray(inst16, dtype=np.uint32)
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    n = rays.shape[0]
+    t = np.empty(n, dtype=np.float32)
+    prim = np.empty(n, dtype=np.uint32)
+    inst = np.empty(n, dtype=np.uint32)
+    lib().or_closest_hit_scene_brute(_p(pos), pos.shape[1], _p(tris), _p(mesh3), _p(inst16),
+                                     inst16.shape[0], _p(rays), n, _p(t), _p(prim), _p(inst))
+    return t, prim, inst

@@ -82,3 +82,40 @@ def test_no_device_error():
     with pytest.raises(N.MgbvhError) as e:
         N.Context(0)
     assert e.value.code == N.ERR_NO_DEVICE
+
+
+def test_trace_struct_layouts_match_header(tmp_path):
+    prog = tmp_path / "layout2.c"
+    prog.write_text(r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "mgbvh.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu\n", sizeof(mgbvh_ray), sizeof(mgbvh_hit), sizeof(mgbvh_blas_desc),
+         sizeof(mgbvh_instance), sizeof(mgbvh_camera));
+  printf("%zu %zu %zu %zu %zu %zu\n", offsetof(mgbvh_ray, dir), offsetof(mgbvh_ray, tmax),
+         offsetof(mgbvh_hit, tri_slot), offsetof(mgbvh_blas_desc, vertex_offset),
+         offsetof(mgbvh_instance, blas), offsetof(mgbvh_camera, aspect));
+  return 0;
+}
+''')
+    exe = tmp_path / "layout2"
+    subprocess.run(["gcc", "-std=c11", "-I" + os.path.join(ROOT, "include"), str(prog), "-o", str(exe)],
+                   check=True)
+    a, b = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")[:2]
+    assert [int(x) for x in a.split()] == [N.RAY_DTYPE.itemsize, N.HIT_DTYPE.itemsize,
+                                           N.BLAS_DTYPE.itemsize, N.INSTANCE_DTYPE.itemsize,
+                                           C.sizeof(N.Camera)]
+    assert [int(x) for x in b.split()] == [N.RAY_DTYPE.fields["dir"][1], N.RAY_DTYPE.fields["tmax"][1],
+                                           N.HIT_DTYPE.fields["tri_slot"][1],
+                                           N.BLAS_DTYPE.fields["vertex_offset"][1],
+                                           N.INSTANCE_DTYPE.fields["blas"][1], N.Camera.aspect.offset]
+
+
+def test_trace_null_arguments_fail_cleanly():
+    L = N.lib()
+    assert L.mgbvh_trace_closest_device(None, None, None, None, 12, None, None, None, 1, None) \
+        == N.ERR_INVALID_ARG
+    assert L.mgbvh_trace_scene_device(None, None, None, None, None, 0, None, 0, None, None, None,
+                                      12, None, None, None, 1, None) == N.ERR_INVALID_ARG
+    assert L.mgbvh_gen_primary_rays_device(None, None, 4, 4, None) == N.ERR_INVALID_ARG

@@ -263,3 +263,127 @@ def test_closest_hit_lowest_index_tie():
     t, p = O.closest_hit_brute(pos, tris, rays)
     assert t[0] == 1.0 and p[0] == 1          # triangles 1 and 2 tie at t = 1
     assert t[1] == np.inf and p[1] == 0xFFFFFFFF
+
+
+def _oracle_tree(pos, tris, max_depth=32):
+    ref = O.build(O.tri_aabbs(pos, tris), max_depth=max_depth)
+    perm = ref["list"]
+    return O.encode(ref["nodes"]), ref["nodes"]["len"].astype(np.uint32), tris[perm], perm
+
+
+@pytest.mark.parametrize("max_depth", [32, 4, 0])
+def test_closest_hit_bvh_equals_brute(max_depth):
+    # the conservative slab test culls nothing that holds the closest hit, and
+    # multi-object leaves (depth cut) are tested completely
+    pos, tris = scenes.soup(1500, seed=12, extent=0.05)
+    tris = np.ascontiguousarray(np.concatenate([tris, tris[:100]]))
+    rng = np.random.default_rng(3)
+    o = rng.standard_normal((1500, 3))
+    o = 0.5 + 2.0 * o / np.linalg.norm(o, axis=1, keepdims=True)
+    tgt = pos[rng.integers(0, pos.shape[0], 1500)]
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d[:40, 1] = 0.0  # zero components: infinite inverse, 0 * inf in the slab
+    rays = np.concatenate([o, d], axis=1).astype(np.float32)
+    n8, lens, tp, perm = _oracle_tree(pos, tris, max_depth)
+    t0, p0 = O.closest_hit_brute(pos, tris, rays)
+    t1, p1 = O.closest_hit_bvh(n8, lens, pos, tp, perm, rays)
+    assert (p0 != 0xFFFFFFFF).sum() > 500
+    assert np.any(p0 < 100)  # duplicated triangles: the lower index wins
+    assert np.array_equal(t0.view(np.uint32), t1.view(np.uint32))
+    assert np.array_equal(p0, p1)
+
+
+def _inst(m12, mesh):
+    rec = np.zeros(16, np.uint32)
+    rec[:12] = np.asarray(m12, np.float32).view(np.uint32)
+    rec[12] = mesh
+    return rec
+
+
+IDENT = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]
+
+
+def test_scene_brute_known_answers():
+    # unit triangle in object space, instance scaled by 2: world triangle (0,0,0)-(2,0,0)-(0,2,0)
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    tris = np.array([[0, 1, 2]], np.uint32)
+    mesh3 = np.array([[0, 1, 0]], np.uint32)
+    w2o = [0.5, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 0.5, 0]
+    rays = np.array([[0.5, 0.5, 5, 0, 0, -1],     # object ray (0.25,0.25,2.5) dir (0,0,-0.5): t = 5
+                     [1.5, 0.3, 5, 0, 0, -1],     # inside the world triangle only (x+y < 2)
+                     [1.5, 0.7, 5, 0, 0, -1]],    # outside: x + y > 2
+                    np.float32)
+    t, p, i = O.closest_hit_scene_brute(pos, tris, mesh3, np.stack([_inst(w2o, 0)]), rays)
+    assert t[0] == 5.0 and p[0] == 0 and i[0] == 0
+    assert t[1] == 5.0 and p[1] == 0
+    assert t[2] == np.inf and p[2] == 0xFFFFFFFF and i[2] == 0xFFFFFFFF
+    # translation by +3 in x: world_to_object subtracts 3
+    w2o_t = [1, 0, 0, -3, 0, 1, 0, 0, 0, 0, 1, 0]
+    t, p, i = O.closest_hit_scene_brute(pos, tris, mesh3, np.stack([_inst(w2o_t, 0)]),
+                                        np.array([[3.25, 0.25, 2, 0, 0, -1]], np.float32))
+    assert t[0] == 2.0 and p[0] == 0
+    # rotation by 90 degrees about y (object x -> world -z): world_to_object maps world z to -x
+    w2o_r = [0, 0, -1, 0, 0, 1, 0, 0, 1, 0, 0, 0]
+    t, p, i = O.closest_hit_scene_brute(pos, tris, mesh3, np.stack([_inst(w2o_r, 0)]),
+                                        np.array([[4, 0.25, -0.25, -1, 0, 0]], np.float32))
+    assert t[0] == 4.0 and p[0] == 0
+
+
+def test_scene_brute_matches_single_mesh_brute():
+    # identity instances reproduce the single-mesh brute force; mesh offsets
+    # (first triangle, vertex offset) select the right mesh; a duplicated
+    # instance loses every tie to the lower instance index
+    pa, ta = scenes.soup(400, seed=13, extent=0.1)
+    pb, tb = scenes.soup(300, seed=14, extent=0.1)
+    pos = np.concatenate([pa, pb])
+    tris = np.concatenate([ta, tb]).astype(np.uint32)
+    mesh3 = np.array([[0, 400, 0], [400, 300, pa.shape[0]]], np.uint32)
+    rng = np.random.default_rng(4)
+    o = (rng.random((800, 3)) * 1.2 - 0.1).astype(np.float32)
+    o[:, 2] = -1.0
+    d = np.tile(np.float32([0.01, -0.02, 1.0]), (800, 1))
+    rays = np.concatenate([o, d], axis=1).astype(np.float32)
+    tA, pA = O.closest_hit_brute(pa, ta, rays)
+    tB, pB = O.closest_hit_brute(pb, tb, rays)
+    for mesh, (tw, pw) in ((0, (tA, pA)), (1, (tB, pB))):
+        t, p, i = O.closest_hit_scene_brute(pos, tris, mesh3, np.stack([_inst(IDENT, mesh)]), rays)
+        assert np.array_equal(t.view(np.uint32), tw.view(np.uint32)) and np.array_equal(p, pw)
+    inst = np.stack([_inst(IDENT, 1), _inst(IDENT, 0), _inst(IDENT, 1)])
+    t, p, i = O.closest_hit_scene_brute(pos, tris, mesh3, inst, rays)
+    want_t = np.minimum(tA, tB)
+    assert np.array_equal(t, want_t)
+    a_wins = (tA < tB)
+    assert np.all(i[a_wins] == 1)
+    assert np.all(i[(tB < tA)] == 0)  # instance 2 duplicates instance 0 and never wins
+    assert (tB < tA).sum() > 50 and a_wins.sum() > 50
+    # a translated instance equals the single-mesh brute force on rays moved by -c
+    c = np.float32([0.75, -0.5, 2.0])
+    inst_t = np.stack([_inst([1, 0, 0, -c[0], 0, 1, 0, -c[1], 0, 0, 1, -c[2]], 0)])
+    rays_w = rays.copy()
+    rays_w[:, :3] += c
+    t, p, i = O.closest_hit_scene_brute(pos, tris, mesh3, inst_t, rays_w)
+    moved = rays.copy()
+    moved[:, :3] = rays_w[:, :3] - c  # the same float32 subtraction the transform performs
+    t2, p2 = O.closest_hit_brute(pa, ta, moved)
+    assert np.array_equal(t.view(np.uint32), t2.view(np.uint32)) and np.array_equal(p, p2)
+
+
+def test_closest_hit_bvh_rays_in_face_planes():
+    # axis-aligned rays whose origin lies exactly in a vertex (hence box face)
+    # plane: the plane distance is 0 * inf and must not cull the box
+    pos, tris = scenes.heightfield(32, 32, seed=3)
+    rng = np.random.default_rng(5)
+    n = 2000
+    o = np.column_stack([rng.random(n), rng.uniform(-0.5, 0.5, n), rng.random(n)]).astype(np.float32)
+    axes = rng.integers(0, 6, n)
+    d = np.zeros((n, 3), np.float32)
+    d[np.arange(n), axes % 3] = np.where(axes < 3, 1.0, -1.0)
+    o[:, 0] = pos[rng.integers(0, pos.shape[0], n), 0]  # every origin in a grid plane x = i/32
+    rays = np.concatenate([o, d], axis=1)
+    n8, lens, tp, perm = _oracle_tree(pos, tris)
+    t0, p0 = O.closest_hit_brute(pos, tris, rays)
+    t1, p1 = O.closest_hit_bvh(n8, lens, pos, tp, perm, rays)
+    assert (p0 != 0xFFFFFFFF).sum() > 300
+    assert np.array_equal(t0.view(np.uint32), t1.view(np.uint32))
+    assert np.array_equal(p0, p1)
