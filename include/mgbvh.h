@@ -262,6 +262,38 @@ int mgbvh_gen_bounce_rays_device(mgbvh_ctx* ctx, const mgbvh_ray* d_rays_in,
                                  const uint32_t* d_tris_perm, uint32_t seed,
                                  mgbvh_ray* d_rays_out);
 
+/* ---- batched BLAS build ------------------------------------------------------
+ * Replaces the per-mesh BLAS loop of the scene upload (model.rs:76-95, one
+ * BoundingVolumeHierarchy::new per mesh on rayon workers): every mesh of the
+ * concatenated triangle list gets the tree mgbvh_build_tris_device would give
+ * it alone, but all meshes share each breadth-first level's launches. */
+typedef struct mgbvh_mesh_desc {
+  uint32_t tri_offset;     /* first triangle = sum of tri_count of the earlier meshes */
+  uint32_t tri_count;      /* 0 = empty mesh (no nodes, lib.rs:703-708) */
+  uint32_t vertex_offset;  /* the mesh's vertex indices address vertex_offset + index */
+  uint32_t vertex_count;   /* every vertex index of the mesh is < vertex_count */
+} mgbvh_mesh_desc;
+
+/* meshes (HOST array, nmeshes entries) describe d_tris (concatenated, indices
+ * local to each mesh).  mesh_bounds (HOST, optional) = BvhSettings.bounds per
+ * mesh (NULL: the union of the mesh's triangles); settings.has_bounds and
+ * settings.profiling_info must be 0.  Mesh m's nodes start at slot
+ * out_blas[m].node_offset = sum over earlier meshes of 2 n - 1 (d_nodes, d_lens
+ * and d_splits hold sum(2 n - 1) slots); its child indices are relative to
+ * that slot (Mesh.blasRoot, scene.slang:182), its leaf start indices and
+ * d_perm entries relative to tri_offset.  out_node_counts (HOST, optional)
+ * receives each mesh's node count (2 n - 1 unless max_depth cut it short).
+ * out_stats (optional): construction_time, node_count, total_object_count,
+ * n_large_levels, n_median, n_cutoff_leaves summed over meshes; others 0. */
+int mgbvh_build_blas_batch_device(mgbvh_ctx* ctx, const float* d_positions,
+                                  uint32_t vstride_bytes, uint64_t nverts, const uint32_t* d_tris,
+                                  const mgbvh_mesh_desc* meshes, uint32_t nmeshes,
+                                  const mgbvh_aabb* mesh_bounds, const mgbvh_settings* settings,
+                                  mgbvh_node32* d_nodes, uint32_t* d_lens, uint32_t* d_perm,
+                                  uint32_t* d_tris_out, mgbvh_split_rec* d_splits,
+                                  mgbvh_blas_desc* out_blas, uint32_t* out_node_counts,
+                                  mgbvh_stats* out_stats);
+
 /* ---- timing helpers: HIP events recorded on the context stream ---- */
 int mgbvh_event_record(mgbvh_ctx* ctx, int slot);           /* slot 0..59 */
 int mgbvh_event_elapsed_ms(mgbvh_ctx* ctx, int a, int b, float* ms);

@@ -40,6 +40,9 @@ INSTANCE_DTYPE = np.dtype([("world_to_object", "<f4", (12,)), ("blas", "<u4"),
 assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16
 assert BLAS_DTYPE.itemsize == 16 and INSTANCE_DTYPE.itemsize == 64
 MISS = 0xFFFFFFFF
+MESH_DTYPE = np.dtype([("tri_offset", "<u4"), ("tri_count", "<u4"), ("vertex_offset", "<u4"),
+                       ("vertex_count", "<u4")])
+AABB_DTYPE = np.dtype([("min", "<f4", (3,)), ("max", "<f4", (3,))])
 
 
 class Camera(C.Structure):
@@ -158,6 +161,8 @@ _EXTRA: dict = {
     "mgbvh_trace_scene_device": ([_P, _P, _P, _P, _P, _U32, _P, _U32, _P, _P, _P, _U32, _P, _P,
                                   _P, _U64, _P], _I),
     "mgbvh_gen_primary_rays_device": ([_P, C.POINTER(Camera), _U32, _U32, _P], _I),
+    "mgbvh_build_blas_batch_device": ([_P, _P, _U32, _U64, _P, _P, _U32, _P, C.POINTER(Settings),
+                                       _P, _P, _P, _P, _P, _P, _P, C.POINTER(Stats)], _I),
     "mgbvh_gen_bounce_rays_device": ([_P, _P, _P, _U64, _P, _U32, _P, _U32, _P, _U32, _P, _U32,
                                       _P], _I),
 }
@@ -329,6 +334,23 @@ class Context:
         check(lib().mgbvh_gen_bounce_rays_device(
             self.handle, _dv(rays_in), _dv(hits), nrays, _dv(instances), ninstances, _dv(blas),
             nblas, _dv(pos), vstride, _dv(tris_perm), seed, _dv(rays_out)))
+
+    # ---- batched BLAS build (device buffers) ---------------------------------
+    def build_blas_batch(self, pos, vstride, nverts, tris, meshes: np.ndarray, settings: Settings,
+                         nodes, lens, perm, tris_out=None, splits=None, mesh_bounds=None):
+        """meshes: MESH_DTYPE array (host).  Returns (blas BLAS_DTYPE array,
+        node counts, stats dict)."""
+        meshes = np.ascontiguousarray(meshes, dtype=MESH_DTYPE)
+        nm = meshes.shape[0]
+        blas = np.zeros(max(nm, 1), dtype=BLAS_DTYPE)
+        counts = np.zeros(max(nm, 1), dtype=np.uint32)
+        mb = None if mesh_bounds is None else np.ascontiguousarray(mesh_bounds, dtype=np.float32)
+        st = Stats()
+        check(lib().mgbvh_build_blas_batch_device(
+            self.handle, _dv(pos), vstride, nverts, _dv(tris), ptr(meshes), nm, ptr(mb),
+            C.byref(settings), _dv(nodes), _dv(lens), _dv(perm), _dv(tris_out), _dv(splits),
+            ptr(blas), ptr(counts), C.byref(st)))
+        return blas[:nm], counts[:nm], st.as_dict()
 
     # ---- host-buffer builds -------------------------------------------------
     def build_tris(self, positions: np.ndarray, tris: np.ndarray, settings: Settings,

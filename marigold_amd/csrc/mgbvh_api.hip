@@ -324,6 +324,84 @@ int mgbvh_build_tris_device(mgbvh_ctx* c, const float* d_pos, uint32_t vstride_b
   return MGBVH_OK;
 }
 
+int mgbvh_build_blas_batch_device(mgbvh_ctx* c, const float* d_pos, uint32_t vstride_bytes,
+                                  uint64_t nverts, const uint32_t* d_tris,
+                                  const mgbvh_mesh_desc* meshes, uint32_t nmeshes,
+                                  const mgbvh_aabb* mesh_bounds, const mgbvh_settings* s,
+                                  mgbvh_node32* d_nodes, uint32_t* d_lens, uint32_t* d_perm,
+                                  uint32_t* d_tris_out, mgbvh_split_rec* d_splits,
+                                  mgbvh_blas_desc* out_blas, uint32_t* out_node_counts,
+                                  mgbvh_stats* out_stats) {
+  if (!c) { set_error("ctx is NULL"); return MGBVH_ERR_INVALID_ARG; }
+  int rc = check_settings(s);
+  if (rc) return rc;
+  if (s->has_bounds || s->profiling_info) {
+    set_error("batched builds take per-mesh bounds (mesh_bounds) and write no profiling JSON");
+    return MGBVH_ERR_UNSUPPORTED;
+  }
+  if (nmeshes && !meshes) { set_error("meshes is NULL"); return MGBVH_ERR_INVALID_ARG; }
+  if (vstride_bytes < 12 || vstride_bytes % 4) {
+    set_error("vstride_bytes must be a multiple of 4 and >= 12");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  std::vector<MeshIn> ms(nmeshes);
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < nmeshes; ++i) {
+    const mgbvh_mesh_desc& d = meshes[i];
+    if (d.tri_offset != total) {
+      set_error("mesh " + std::to_string(i) + ": tri_offset must equal the sum of earlier tri_count");
+      return MGBVH_ERR_INVALID_ARG;
+    }
+    if (d.tri_count && (uint64_t)d.vertex_offset + d.vertex_count > nverts) {
+      set_error("mesh " + std::to_string(i) + ": vertex_offset + vertex_count > nverts");
+      return MGBVH_ERR_INDEX;
+    }
+    MeshIn& m = ms[i];
+    m.tri_offset = d.tri_offset;
+    m.count = d.tri_count;
+    m.vertex_offset = d.vertex_offset;
+    m.vertex_count = d.vertex_count;
+    m.has_bounds = mesh_bounds != nullptr;
+    for (int a = 0; a < 3; ++a) {
+      m.bmn[a] = mesh_bounds ? mesh_bounds[i].min[a] : 0.0f;
+      m.bmx[a] = mesh_bounds ? mesh_bounds[i].max[a] : 0.0f;
+    }
+    total += d.tri_count;
+  }
+  if (total > 0x7fffffffull) { set_error("more than 2^31-1 triangles"); return MGBVH_ERR_INVALID_ARG; }
+  if (total && (!d_pos || !d_tris || !d_nodes || !d_perm)) {
+    set_error("null device buffer");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  BuildInput in{0, d_pos, vstride_bytes / 4, nverts, d_tris, nullptr};
+  BuildOutput o{d_nodes, d_lens, d_perm, d_tris_out, d_splits};
+  BuildResult r;
+  rc = build_multi(c, in, ms, s, o, &r);
+  if (rc) return rc;
+  for (uint32_t i = 0; i < nmeshes; ++i) {
+    if (out_blas) {
+      out_blas[i].node_offset = r.mesh_nbase[i];
+      out_blas[i].tri_offset = meshes[i].tri_offset;
+      out_blas[i].vertex_offset = meshes[i].vertex_offset;
+      out_blas[i].reserved = 0;
+    }
+    if (out_node_counts) out_node_counts[i] = r.mesh_nodes[i];
+  }
+  if (out_stats) {
+    memset(out_stats, 0, sizeof *out_stats);
+    out_stats->construction_time = r.ms * 1e-3;
+    out_stats->node_count = (uint32_t)r.nnodes;
+    out_stats->total_object_count = (uint32_t)total;
+    out_stats->max_depth = s->max_depth;
+    out_stats->n_large_levels = r.n_levels;
+    out_stats->n_median = r.n_median;
+    out_stats->n_cutoff_leaves = r.n_cutoff;
+  }
+  return MGBVH_OK;
+}
+
 int mgbvh_build_aabbs_device(mgbvh_ctx* c, const mgbvh_aabb* d_aabbs, uint64_t n,
                              const mgbvh_settings* s, mgbvh_node32* d_nodes, uint32_t* d_lens,
                              uint32_t* d_perm, mgbvh_split_rec* d_splits, uint64_t* out_nnodes,

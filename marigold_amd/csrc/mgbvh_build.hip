@@ -85,28 +85,49 @@ __device__ inline void acc12_add(uint32_t (&acc)[12], const Box& b) {
 }
 
 // ------------------------------------------------------------------ setup
+// Mesh of setup chunk c: the last mesh whose first chunk is <= c (empty meshes
+// own no chunks and share their first chunk with the next mesh).
+__device__ inline uint32_t chunk_mesh(const MeshRow* rows, uint32_t nm, uint32_t c) {
+  uint32_t lo = 0, hi = nm;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (rows[mid].setup_chunk0 <= c) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // MeshTriangle::bounding_volume (mesh_interface/src/lib.rs:19-30):
-// min = v1.min(v2).min(v3), max likewise; -0.0 is stored as +0.0.
+// min = v1.min(v2).min(v3), max likewise; -0.0 is stored as +0.0.  One block
+// per chunk of CH triangles of one mesh; the mesh's root bounds and root
+// centroid bounds (lib.rs:220-228, 499-509) are accumulated per mesh.
 __global__ __launch_bounds__(256) void k_setup_tris(const float* __restrict__ pos, uint32_t vs,
-                                                    uint64_t nverts,
                                                     const uint32_t* __restrict__ tris,
-                                                    uint32_t n, Rec* __restrict__ recs,
-                                                    Counters* ctr) {
+                                                    const MeshRow* __restrict__ rows,
+                                                    uint32_t nm, Rec* __restrict__ recs,
+                                                    uint32_t* __restrict__ macc, Counters* ctr) {
+  __shared__ uint32_t s_m;
+  if (threadIdx.x == 0) s_m = chunk_mesh(rows, nm, blockIdx.x);
+  __syncthreads();
+  const MeshRow& mr = rows[s_m];
+  const uint32_t vcount = mr.vertex_count, voff = mr.vertex_offset;
+  const uint32_t lo = mr.tri_offset + (blockIdx.x - mr.setup_chunk0) * CH;
+  const uint32_t hi = min(lo + CH, mr.tri_offset + mr.count);
   uint32_t acc[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) acc[k] = 0;
   uint32_t err = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
     uint32_t i0 = tris[3 * (size_t)i + 0];
     uint32_t i1 = tris[3 * (size_t)i + 1];
     uint32_t i2 = tris[3 * (size_t)i + 2];
-    if (i0 >= nverts || i1 >= nverts || i2 >= nverts) {
+    if (i0 >= vcount || i1 >= vcount || i2 >= vcount) {
       err |= ERR_INDEX;
       i0 = i1 = i2 = 0;
     }
-    const float* v1 = pos + (size_t)i0 * vs;
-    const float* v2 = pos + (size_t)i1 * vs;
-    const float* v3 = pos + (size_t)i2 * vs;
+    const float* v1 = pos + (size_t)(voff + i0) * vs;
+    const float* v2 = pos + (size_t)(voff + i1) * vs;
+    const float* v3 = pos + (size_t)(voff + i2) * vs;
     Box b;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -119,74 +140,84 @@ __global__ __launch_bounds__(256) void k_setup_tris(const float* __restrict__ po
     acc12_add(acc, b);
   }
   if (err) atomicOr(&ctr->err, err);
-  block_acc12_flush(acc, ctr->root_acc);
+  block_acc12_flush(acc, macc + 12 * (size_t)s_m);
 }
 
 // Objects given as boxes (AsBoundingVolume, e.g. UploadedMesh at
 // mesh_interface/src/lib.rs:87-91).
-__global__ __launch_bounds__(256) void k_setup_aabbs(const float* __restrict__ boxes, uint32_t n,
-                                                     Rec* __restrict__ recs, Counters* ctr) {
+__global__ __launch_bounds__(256) void k_setup_aabbs(const float* __restrict__ boxes,
+                                                     const MeshRow* __restrict__ rows,
+                                                     uint32_t nm, Rec* __restrict__ recs,
+                                                     uint32_t* __restrict__ macc, Counters* ctr) {
+  __shared__ uint32_t s_m;
+  if (threadIdx.x == 0) s_m = chunk_mesh(rows, nm, blockIdx.x);
+  __syncthreads();
+  const MeshRow& mr = rows[s_m];
+  const uint32_t lo = mr.tri_offset + (blockIdx.x - mr.setup_chunk0) * CH;
+  const uint32_t hi = min(lo + CH, mr.tri_offset + mr.count);
   uint32_t acc[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) acc[k] = 0;
   uint32_t err = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
     Box b;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      float lo = boxes[6 * (size_t)i + a], hi = boxes[6 * (size_t)i + 3 + a];
-      if (!coord_ok(lo) || !coord_ok(hi)) err |= ERR_NONFINITE;
-      b.mn[a] = canon_zero(lo);
-      b.mx[a] = canon_zero(hi);
+      float lo_a = boxes[6 * (size_t)i + a], hi_a = boxes[6 * (size_t)i + 3 + a];
+      if (!coord_ok(lo_a) || !coord_ok(hi_a)) err |= ERR_NONFINITE;
+      b.mn[a] = canon_zero(lo_a);
+      b.mx[a] = canon_zero(hi_a);
     }
     rec_store(&recs[i], b, i);
     acc12_add(acc, b);
   }
   if (err) atomicOr(&ctr->err, err);
-  block_acc12_flush(acc, ctr->root_acc);
+  block_acc12_flush(acc, macc + 12 * (size_t)s_m);
 }
 
-// Root node, lib.rs:220-243 and 731-735.
-__global__ void k_root(Counters* ctr, LNode* L, uint32_t* cnode, uint32_t* ck, SNode* small,
-                       uint32_t n, int has_bounds, float bx0, float by0, float bz0, float bx1,
-                       float by1, float bz1, int large) {
-  __shared__ uint32_t nch_s;
+// Root node of every mesh, lib.rs:220-243 and 731-735: a node with > 32
+// objects starts the breadth-first levels, a smaller one (or any root when
+// max_depth == 0) goes straight to the small-subtree kernel.  One wave per mesh.
+__global__ void k_root(const MeshRow* __restrict__ rows, const uint32_t* __restrict__ macc,
+                       Counters* ctr, LNode* L, uint32_t* cnode, uint32_t* ck, SNode* small,
+                       uint32_t n_small_roots) {
+  const uint32_t m = blockIdx.x;
+  if (m == 0 && threadIdx.x == 0) ctr->n_small = n_small_roots;
+  const MeshRow mr = rows[m];
+  if (mr.root_kind == 0) return;
+  const uint32_t* acc = macc + 12 * (size_t)m;
   if (threadIdx.x == 0) {
     float bmn[3], bmx[3];
-    if (has_bounds) {
-      bmn[0] = bx0; bmn[1] = by0; bmn[2] = bz0;
-      bmx[0] = bx1; bmx[1] = by1; bmx[2] = bz1;
-    } else {
-      for (int a = 0; a < 3; ++a) {
-        bmn[a] = acc_min_dec(ctr->root_acc[a]);
-        bmx[a] = acc_max_dec(ctr->root_acc[3 + a]);
-      }
+    for (int a = 0; a < 3; ++a) {
+      bmn[a] = mr.has_bounds ? mr.bmn[a] : acc_min_dec(acc[a]);
+      bmx[a] = mr.has_bounds ? mr.bmx[a] : acc_max_dec(acc[3 + a]);
     }
-    if (large) {
+    if (mr.root_kind == 1) {
       LNode r = {};
-      r.start = 0; r.count = n; r.idx = 0; r.P = 1; r.depth = 0;
-      r.chunk0 = 0; r.nchunk = (n + CH - 1) / CH;
+      r.start = mr.tri_offset; r.count = mr.count; r.idx = mr.nbase; r.P = mr.nbase + 1;
+      r.depth = 0; r.chunk0 = mr.lchunk0; r.nchunk = (mr.count + CH - 1) / CH;
       for (int a = 0; a < 3; ++a) {
         r.bmn[a] = bmn[a]; r.bmx[a] = bmx[a];
-        r.cacc[a] = ctr->root_acc[6 + a];
-        r.cacc[3 + a] = ctr->root_acc[9 + a];
+        r.cacc[a] = acc[6 + a];
+        r.cacc[3 + a] = acc[9 + a];
       }
       r.child_slot[0] = r.child_slot[1] = -1;
-      L[0] = r;
-      nch_s = r.nchunk;
+      r.nbase = mr.nbase; r.obase = mr.tri_offset;
+      L[mr.root_slot] = r;
     } else {
-      SNode s = {};
-      s.start = 0; s.count = n; s.idx = 0; s.P = 1; s.depth = 0; s.parity = 0;
-      for (int a = 0; a < 3; ++a) { s.bmn[a] = bmn[a]; s.bmx[a] = bmx[a]; }
-      small[0] = s;
-      ctr->n_small = 1;
-      nch_s = 0;
+      SNode sn = {};
+      sn.start = mr.tri_offset; sn.count = mr.count; sn.idx = mr.nbase; sn.P = mr.nbase + 1;
+      sn.depth = 0; sn.parity = 0; sn.nbase = mr.nbase; sn.obase = mr.tri_offset;
+      for (int a = 0; a < 3; ++a) { sn.bmn[a] = bmn[a]; sn.bmx[a] = bmx[a]; }
+      small[mr.root_slot] = sn;
     }
   }
-  __syncthreads();
-  for (uint32_t k = threadIdx.x; k < nch_s; k += blockDim.x) {
-    cnode[k] = 0;
-    ck[k] = k;
+  if (mr.root_kind == 1) {
+    const uint32_t nch = (mr.count + CH - 1) / CH;
+    for (uint32_t k = threadIdx.x; k < nch; k += blockDim.x) {
+      cnode[mr.lchunk0 + k] = mr.root_slot;
+      ck[mr.lchunk0 + k] = k;
+    }
   }
 }
 
@@ -279,18 +310,20 @@ __device__ inline void eval_cut(const uint32_t* h, int a, uint32_t i, Box& lt, B
 // chunks [c0, c0 + nch)), or a subtree root of <= 32 objects / a depth-cut leaf.
 __device__ inline void put_child(bool large, uint32_t s, uint32_t c0, uint32_t start,
                                  uint32_t count, const Box& b, uint32_t idx, uint32_t P,
-                                 uint32_t depth, uint32_t parity_next, LNode* Ln, SNode* small) {
+                                 uint32_t depth, uint32_t parity_next, uint32_t nbase,
+                                 uint32_t obase, LNode* Ln, SNode* small) {
   if (large) {
     LNode r = {};
     r.start = start; r.count = count; r.idx = idx; r.P = P; r.depth = depth;
     r.chunk0 = c0; r.nchunk = (count + CH - 1) / CH;
     for (int a = 0; a < 3; ++a) { r.bmn[a] = b.mn[a]; r.bmx[a] = b.mx[a]; }
     r.child_slot[0] = r.child_slot[1] = -1;
+    r.nbase = nbase; r.obase = obase;
     Ln[s] = r;
   } else {
     SNode sn = {};
     sn.start = start; sn.count = count; sn.idx = idx; sn.P = P; sn.depth = depth;
-    sn.parity = parity_next;
+    sn.parity = parity_next; sn.nbase = nbase; sn.obase = obase;
     for (int a = 0; a < 3; ++a) { sn.bmn[a] = b.mn[a]; sn.bmx[a] = b.mx[a]; }
     small[s] = sn;
   }
@@ -460,12 +493,13 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
   const uint32_t sls = ll ? bl + (uint32_t)lg : bs + (uint32_t)!lg;  // lt child slot
   const uint32_t c0g = bc, c0l = bc + chg;
   if (lane == 0) {
-    out.write_node(nd.idx, nb, nd.start, nd.P, nd.count);
+    out.write_node(nd.idx, nb, nd.start - nd.obase, nd.P - nd.nbase, nd.count);
     out.write_split(nd.idx, axis | (kind << 8), value, nl, fbits(cost));
     if (kind == 2) atomicAdd(&ctr->n_median, 1u);
-    put_child(lg, sgs, c0g, nd.start + nl, ng, gt, nd.P, nd.P + 2, cd, parity_next, Ln, small);
+    put_child(lg, sgs, c0g, nd.start + nl, ng, gt, nd.P, nd.P + 2, cd, parity_next, nd.nbase,
+              nd.obase, Ln, small);
     put_child(ll, sls, c0l, nd.start, nl, lt, nd.P + 1, nd.P + 2 + 2 * (ng - 1), cd, parity_next,
-              Ln, small);
+              nd.nbase, nd.obase, Ln, small);
     L[slot].axis = axis;
     L[slot].cut = value;
     L[slot].n_lt = nl;
@@ -655,10 +689,26 @@ __device__ inline void lane_move(SmallObj* S, uint32_t lane, uint32_t newpos, Sm
   wave_sync();
 }
 
+// Writes the final object order: perm (index inside the mesh) and, for
+// triangles, the permuted triangle list.
+__device__ inline void put_perm(uint32_t pos, uint32_t id, uint32_t obase, uint32_t* perm,
+                                const uint32_t* tris_in, uint32_t* tris_out) {
+  perm[pos] = id - obase;
+  if (tris_out) {
+    const uint32_t a = tris_in[3 * (size_t)id], b = tris_in[3 * (size_t)id + 1],
+                   c = tris_in[3 * (size_t)id + 2];
+    tris_out[3 * (size_t)pos] = a;
+    tris_out[3 * (size_t)pos + 1] = b;
+    tris_out[3 * (size_t)pos + 2] = c;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small, uint32_t nS,
                                                const Rec* __restrict__ rec0,
                                                const Rec* __restrict__ rec1, NodeOut out,
-                                               uint32_t* __restrict__ perm, Counters* ctr,
+                                               uint32_t* __restrict__ perm,
+                                               const uint32_t* __restrict__ tris_in,
+                                               uint32_t* __restrict__ tris_out, Counters* ctr,
                                                uint32_t* gaps, uint32_t max_depth) {
   __shared__ SmallObj lds_obj[4][64];
   const uint32_t w = threadIdx.x >> 6;
@@ -674,7 +724,7 @@ __global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small, 
 #pragma unroll
   for (int x = 0; x < 3; ++x) { o.c[x] = 0.0f; o.mn[x] = 0.0f; o.mx[x] = 0.0f; }
   o.id = 0;
-  uint32_t ls = lane, lc = 1, idx = 0, P = 0, depth = 0, base = 0;
+  uint32_t ls = lane, lc = 1, idx = 0, P = 0, depth = 0, base = 0, nbase = 0, obase = 0;
   Box nb = box_empty();
   if (e < nS) {
     const SNode sn = small[e];
@@ -685,13 +735,14 @@ __global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small, 
         Box b;
 #pragma unroll
         for (int x = 0; x < 3; ++x) { b.mn[x] = sn.bmn[x]; b.mx[x] = sn.bmx[x]; }
-        out.write_node(sn.idx, b, sn.start, 0, sn.count);
+        out.write_node(sn.idx, b, sn.start - sn.obase, 0, sn.count);
         const uint32_t g = atomicAdd(&ctr->n_gaps, 1u);
         gaps[2 * g] = sn.P;
         gaps[2 * g + 1] = 2 * (sn.count - 1);
         atomicAdd(&ctr->n_cutoff, 1u);
       }
-      for (uint32_t j = hl; j < sn.count; j += 32) perm[sn.start + j] = src[sn.start + j].id;
+      for (uint32_t j = hl; j < sn.count; j += 32)
+        put_perm(sn.start + j, src[sn.start + j].id, sn.obase, perm, tris_in, tris_out);
     } else if (hl < sn.count) {
       Box b;
       rec_load(&src[sn.start + hl], b, o.id);
@@ -705,6 +756,7 @@ __global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small, 
       }
       has = true;
       ls = off; lc = sn.count; idx = sn.idx; P = sn.P; depth = sn.depth; base = sn.start;
+      nbase = sn.nbase; obase = sn.obase;
     }
   }
   obj_put(S, lane, o);
@@ -715,7 +767,7 @@ __global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small, 
     // leaves: one object, or depth == max_depth (lib.rs:548-558)
     if (has && (lc == 1 || depth == max_depth)) {
       if (lane == ls) {
-        out.write_node(idx, nb, base + ls - off, 0, lc);
+        out.write_node(idx, nb, base + ls - off - obase, 0, lc);
         if (lc > 1) {
           const uint32_t g = atomicAdd(&ctr->n_gaps, 1u);
           gaps[2 * g] = P;
@@ -723,7 +775,7 @@ __global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small, 
           atomicAdd(&ctr->n_cutoff, 1u);
         }
       }
-      perm[base + lane - off] = o.id;
+      put_perm(base + lane - off, o.id, obase, perm, tris_in, tris_out);
       has = false;
     }
     const bool act = has;
@@ -848,7 +900,7 @@ __global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small, 
     }
     if (act) {
       if (lane == ls) {
-        out.write_node(idx, nb, base + ls - off, P, lc);
+        out.write_node(idx, nb, base + ls - off - obase, P - nbase, lc);
         out.write_split(idx, axis | (kind << 8), value, n_lt, fbits(cost));
         if (kind == 2) atomicAdd(&ctr->n_median, 1u);
       }
@@ -872,17 +924,19 @@ __global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small, 
 }
 
 // ------------------------------------------------------------------ compaction
-// Only after max_depth cut subtrees short: slot i -> i - (reserved slots before i).
-__device__ inline uint32_t gap_shift(const uint32_t* gstart, const uint32_t* gcum, uint32_t ng,
-                                     uint32_t i, bool& inside, const uint32_t* glen) {
-  // number of gaps with start <= i
+// Only after max_depth cut subtrees short.  A cut leaf of n objects at pair
+// base P leaves slots [P, P + 2(n - 1)) unused; inside its mesh, slot i moves
+// to i - (unused slots of that mesh before i) and child indices likewise.
+// upto(x) = total length of the gaps starting at or before slot x.
+__device__ inline uint32_t gap_upto(const uint32_t* gstart, const uint32_t* gcum, uint32_t ng,
+                                    uint32_t x, uint32_t& nle) {
   uint32_t lo = 0, hi = ng;
   while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (gstart[mid] <= i) lo = mid + 1;
+    const uint32_t mid = (lo + hi) >> 1;
+    if (gstart[mid] <= x) lo = mid + 1;
     else hi = mid;
   }
-  inside = lo > 0 && i < gstart[lo - 1] + glen[lo - 1];
+  nle = lo;
   return lo == 0 ? 0 : gcum[lo - 1];
 }
 
@@ -890,31 +944,28 @@ __global__ void k_compact(const mgbvh_node32* __restrict__ nin, const uint32_t* 
                           const mgbvh_split_rec* __restrict__ sin, mgbvh_node32* nout,
                           uint32_t* lout, mgbvh_split_rec* sout, uint32_t nslots,
                           const uint32_t* gstart, const uint32_t* glen, const uint32_t* gcum,
-                          uint32_t ng) {
+                          uint32_t ng, const uint32_t* __restrict__ mnb, uint32_t nm) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nslots) return;
-  bool inside;
-  const uint32_t sh = gap_shift(gstart, gcum, ng, i, inside, glen);
-  if (inside) return;
-  const uint32_t ni = i - sh;
-  mgbvh_node32 nd = nin[i];
-  if (nd.child_node != 0) {
-    bool in2;
-    nd.child_node -= gap_shift(gstart, gcum, ng, nd.child_node, in2, glen);
+  // mesh of slot i: the last mesh whose first slot is <= i
+  uint32_t lo = 0, hi = nm;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (mnb[mid] <= i) lo = mid;
+    else hi = mid;
   }
+  const uint32_t base = mnb[lo];
+  uint32_t nle;
+  const uint32_t before = base == 0 ? 0 : gap_upto(gstart, gcum, ng, base - 1, nle);
+  const uint32_t cum = gap_upto(gstart, gcum, ng, i, nle);
+  if (nle > 0 && i < gstart[nle - 1] + glen[nle - 1]) return;  // an unused slot
+  const uint32_t ni = i - (cum - before);
+  mgbvh_node32 nd = nin[i];
+  if (nd.child_node != 0)
+    nd.child_node -= gap_upto(gstart, gcum, ng, base + nd.child_node, nle) - before;
   nout[ni] = nd;
   if (lout) lout[ni] = lin[i];
   if (sout) sout[ni] = sin[i];
-}
-
-__global__ void k_permute_tris(const uint32_t* __restrict__ tin, const uint32_t* __restrict__ perm,
-                               uint32_t* __restrict__ tout, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t s = perm[i];
-  tout[3 * (size_t)i + 0] = tin[3 * (size_t)s + 0];
-  tout[3 * (size_t)i + 1] = tin[3 * (size_t)s + 1];
-  tout[3 * (size_t)i + 2] = tin[3 * (size_t)s + 2];
 }
 
 }  // namespace mg
@@ -943,12 +994,68 @@ static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b -
 
 int build_device(mgbvh_ctx* c, const BuildInput& in, uint32_t n, const mgbvh_settings* s,
                  const BuildOutput& o, BuildResult* res) {
+  MeshIn m{};
+  m.tri_offset = 0;
+  m.count = n;
+  m.vertex_offset = 0;
+  m.vertex_count = in.nverts > 0xffffffffull ? 0xffffffffu : (uint32_t)in.nverts;
+  m.has_bounds = s->has_bounds;
+  for (int a = 0; a < 3; ++a) { m.bmn[a] = s->bounds_min[a]; m.bmx[a] = s->bounds_max[a]; }
+  return build_multi(c, in, std::vector<MeshIn>{m}, s, o, res);
+}
+
+int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& meshes,
+                const mgbvh_settings* s, const BuildOutput& o, BuildResult* res) {
   *res = BuildResult{};
-  if (n == 0) return MGBVH_OK;  // lib.rs:703-708: empty BVH
+  const uint32_t nm = (uint32_t)meshes.size();
+  res->mesh_nodes.assign(nm, 0);
+  res->mesh_nbase.assign(nm, 0);
+  // plan: chunks, node slots and root slots of every mesh (meshes are contiguous)
+  std::vector<MeshRow> rows(nm);
+  uint64_t n64 = 0, slots64 = 0, setup_chunks = 0, lchunks = 0;
+  uint32_t n_large = 0, n_small = 0;
+  for (uint32_t i = 0; i < nm; ++i) {
+    const MeshIn& m = meshes[i];
+    if (m.tri_offset != n64) {
+      set_error("meshes must be contiguous: tri_offset of each mesh = sum of earlier counts");
+      return MGBVH_ERR_INVALID_ARG;
+    }
+    MeshRow& r = rows[i];
+    r = MeshRow{};
+    r.tri_offset = m.tri_offset;
+    r.count = m.count;
+    r.vertex_offset = m.vertex_offset;
+    r.vertex_count = m.vertex_count;
+    r.setup_chunk0 = (uint32_t)setup_chunks;
+    r.nbase = (uint32_t)slots64;
+    r.has_bounds = m.has_bounds ? 1u : 0u;
+    for (int a = 0; a < 3; ++a) { r.bmn[a] = m.bmn[a]; r.bmx[a] = m.bmx[a]; }
+    res->mesh_nbase[i] = r.nbase;
+    if (m.count == 0) {
+      r.root_kind = 0;  // lib.rs:703-708: empty BVH
+    } else if (m.count > MG_SMALL && s->max_depth > 0) {
+      r.root_kind = 1;
+      r.root_slot = n_large++;
+      r.lchunk0 = (uint32_t)lchunks;
+      lchunks += cdiv(m.count, CH);
+    } else {
+      r.root_kind = 2;
+      r.root_slot = n_small++;
+    }
+    n64 += m.count;
+    slots64 += m.count ? 2ull * m.count - 1 : 0;
+    setup_chunks += cdiv(m.count, CH);
+    if (n64 > 0x7fffffffull || slots64 > 0xffffffffull) {
+      set_error("more than 2^31-1 objects in one build");
+      return MGBVH_ERR_INVALID_ARG;
+    }
+  }
+  const uint32_t n = (uint32_t)n64;
+  if (n == 0) return MGBVH_OK;
   hipStream_t st = c->stream;
   const uint32_t capL = n / (MG_SMALL + 1) + 2;
   const uint32_t capC = cdiv(n, CH) + capL + 2;
-  const uint64_t nslots = 2ull * n - 1;
+  const uint64_t nslots = slots64;
   int rc;
   if ((rc = ensure(c->recs[0], (size_t)n * sizeof(Rec)))) return rc;
   if ((rc = ensure(c->recs[1], (size_t)n * sizeof(Rec)))) return rc;
@@ -962,6 +1069,8 @@ int build_device(mgbvh_ctx* c, const BuildInput& in, uint32_t n, const mgbvh_set
   if ((rc = ensure(c->small, (size_t)n * sizeof(SNode)))) return rc;
   if ((rc = ensure(c->ctr, sizeof(Counters)))) return rc;
   if ((rc = ensure(c->gaps, ((size_t)n / 2 + 1) * 8))) return rc;
+  if ((rc = ensure(c->mrows, (size_t)nm * sizeof(MeshRow)))) return rc;
+  if ((rc = ensure(c->macc, (size_t)nm * 12 * 4))) return rc;
   uint32_t* lens = o.lens;
   if (!lens) {
     if ((rc = ensure(c->lens, nslots * 4))) return rc;
@@ -976,28 +1085,31 @@ int build_device(mgbvh_ctx* c, const BuildInput& in, uint32_t n, const mgbvh_set
   uint32_t* hist = (uint32_t*)c->hist.p;
   uint32_t* blk = (uint32_t*)c->blk.p;
   uint32_t* gaps = (uint32_t*)c->gaps.p;
+  MeshRow* drows = (MeshRow*)c->mrows.p;
+  uint32_t* macc = (uint32_t*)c->macc.p;
   NodeOut out{o.nodes, lens, o.splits};
 
   MG_HIP(hipEventRecord(c->ev[62], st));
+  MG_HIP(hipMemcpyAsync(drows, rows.data(), (size_t)nm * sizeof(MeshRow), hipMemcpyHostToDevice,
+                        st));
   MG_HIP(hipMemsetAsync(ctr, 0, sizeof(Counters), st));
+  MG_HIP(hipMemsetAsync(macc, 0, (size_t)nm * 12 * 4, st));
   if (o.splits) MG_HIP(hipMemsetAsync(o.splits, 0xff, nslots * sizeof(mgbvh_split_rec), st));
 
-  const uint32_t gsetup = std::min<uint32_t>(cdiv(n, 256), 1024);
   MG_PROF(KK_SETUP, 0);
   if (in.kind == 0)
-    k_setup_tris<<<gsetup, 256, 0, st>>>(in.pos, in.vstride_floats, in.nverts, in.tris, n,
-                                         recs[0], ctr);
+    k_setup_tris<<<(uint32_t)setup_chunks, 256, 0, st>>>(in.pos, in.vstride_floats, in.tris, drows,
+                                                         nm, recs[0], macc, ctr);
   else
-    k_setup_aabbs<<<gsetup, 256, 0, st>>>(in.boxes, n, recs[0], ctr);
+    k_setup_aabbs<<<(uint32_t)setup_chunks, 256, 0, st>>>(in.boxes, drows, nm, recs[0], macc, ctr);
   MG_HIP(hipGetLastError());
   MG_PROF(KK_SETUP, 1);
-  const int large = n > MG_SMALL && s->max_depth > 0;
-  k_root<<<1, 256, 0, st>>>(ctr, L[0], cn[0], ck[0], small, n, s->has_bounds, s->bounds_min[0],
-                            s->bounds_min[1], s->bounds_min[2], s->bounds_max[0],
-                            s->bounds_max[1], s->bounds_max[2], large);
+  MG_PROF(KK_ROOT, 0);
+  k_root<<<nm, 64, 0, st>>>(drows, macc, ctr, L[0], cn[0], ck[0], small, n_small);
+  MG_PROF(KK_ROOT, 1);
   MG_HIP(hipGetLastError());
 
-  uint32_t nL = large ? 1 : 0, nC = large ? cdiv(n, CH) : 0;
+  uint32_t nL = n_large, nC = (uint32_t)lchunks;
   int cur = 0;
   uint32_t levels = 0;
   while (nL > 0) {
@@ -1007,9 +1119,9 @@ int build_device(mgbvh_ctx* c, const BuildInput& in, uint32_t n, const mgbvh_set
     k_bin<<<nC, 256, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk);
     MG_PROF(KK_BIN, 1);
     MG_PROF(KK_SWEEP, 0);
-    k_sweep<<<cdiv(nL, SW_WAVES), 1024, 0, st>>>(L[cur], nL, hist, recs[cur], L[cur ^ 1], cn[cur ^ 1],
-                                         ck[cur ^ 1], small, ctr, out, s->max_depth,
-                                         (uint32_t)(cur ^ 1));
+    k_sweep<<<cdiv(nL, SW_WAVES), 1024, 0, st>>>(L[cur], nL, hist, recs[cur], L[cur ^ 1],
+                                                 cn[cur ^ 1], ck[cur ^ 1], small, ctr, out,
+                                                 s->max_depth, (uint32_t)(cur ^ 1));
     MG_PROF(KK_SWEEP, 1);
     MG_PROF(KK_PARTITION, 0);
     k_partition<<<nC, 256, 0, st>>>(L[cur], cn[cur], ck[cur], blk, recs[cur], recs[cur ^ 1],
@@ -1035,7 +1147,9 @@ int build_device(mgbvh_ctx* c, const BuildInput& in, uint32_t n, const mgbvh_set
   const uint32_t nS = c->h_ctr->n_small;
   if (nS) {
     MG_PROF(KK_SMALL, 0);
-    k_small<<<cdiv(nS, 8), 256, 0, st>>>(small, nS, recs[0], recs[1], out, o.perm, ctr, gaps,
+    k_small<<<cdiv(nS, 8), 256, 0, st>>>(small, nS, recs[0], recs[1], out, o.perm,
+                                         in.kind == 0 ? in.tris : nullptr,
+                                         in.kind == 0 ? o.tris_out : nullptr, ctr, gaps,
                                          s->max_depth);
     MG_PROF(KK_SMALL, 1);
     MG_HIP(hipGetLastError());
@@ -1043,6 +1157,8 @@ int build_device(mgbvh_ctx* c, const BuildInput& in, uint32_t n, const mgbvh_set
   MG_HIP(hipMemcpyAsync(c->h_ctr, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
   MG_HIP(hipStreamSynchronize(st));
   const Counters hc = *c->h_ctr;
+  for (uint32_t i = 0; i < nm; ++i)
+    res->mesh_nodes[i] = meshes[i].count ? 2 * meshes[i].count - 1 : 0;
   uint64_t removed = 0;
   if (hc.n_gaps) {
     std::vector<uint32_t> g(2 * (size_t)hc.n_gaps);
@@ -1052,37 +1168,40 @@ int build_device(mgbvh_ctx* c, const BuildInput& in, uint32_t n, const mgbvh_set
     std::sort(gp.begin(), gp.end());
     std::vector<uint32_t> tab(3 * (size_t)hc.n_gaps);
     uint64_t cum = 0;
+    uint32_t mi = 0;
     for (uint32_t i = 0; i < hc.n_gaps; ++i) {
       cum += gp[i].second;
       tab[i] = gp[i].first;
       tab[hc.n_gaps + i] = gp[i].second;
       tab[2 * hc.n_gaps + i] = (uint32_t)cum;
+      while (mi + 1 < nm && rows[mi + 1].nbase <= gp[i].first) ++mi;
+      res->mesh_nodes[mi] -= gp[i].second;
     }
     removed = cum;
+    std::vector<uint32_t> mnb(nm);
+    for (uint32_t i = 0; i < nm; ++i) mnb[i] = rows[i].nbase;
     if ((rc = ensure(c->gtab, tab.size() * 4))) return rc;
+    if ((rc = ensure(c->mnb, (size_t)nm * 4))) return rc;
     if ((rc = ensure(c->tmpn, nslots * sizeof(mgbvh_node32)))) return rc;
     if ((rc = ensure(c->tmpl, nslots * 4))) return rc;
     if (o.splits && (rc = ensure(c->tmps, nslots * sizeof(mgbvh_split_rec)))) return rc;
     MG_HIP(hipMemcpyAsync(c->gtab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, st));
+    MG_HIP(hipMemcpyAsync(c->mnb.p, mnb.data(), mnb.size() * 4, hipMemcpyHostToDevice, st));
     const uint32_t* gt = (const uint32_t*)c->gtab.p;
+    MG_PROF(KK_COMPACT, 0);
     k_compact<<<cdiv(nslots, 256), 256, 0, st>>>(
         o.nodes, lens, o.splits, (mgbvh_node32*)c->tmpn.p, (uint32_t*)c->tmpl.p,
         o.splits ? (mgbvh_split_rec*)c->tmps.p : nullptr, (uint32_t)nslots, gt, gt + hc.n_gaps,
-        gt + 2 * hc.n_gaps, hc.n_gaps);
+        gt + 2 * hc.n_gaps, hc.n_gaps, (const uint32_t*)c->mnb.p, nm);
+    MG_PROF(KK_COMPACT, 1);
     MG_HIP(hipGetLastError());
-    const uint64_t nn = nslots - removed;
-    MG_HIP(hipMemcpyAsync(o.nodes, c->tmpn.p, nn * sizeof(mgbvh_node32), hipMemcpyDeviceToDevice,
-                          st));
-    MG_HIP(hipMemcpyAsync(lens, c->tmpl.p, nn * 4, hipMemcpyDeviceToDevice, st));
+    // every mesh keeps its first slot; its compacted nodes fill the front of its range
+    MG_HIP(hipMemcpyAsync(o.nodes, c->tmpn.p, nslots * sizeof(mgbvh_node32),
+                          hipMemcpyDeviceToDevice, st));
+    MG_HIP(hipMemcpyAsync(lens, c->tmpl.p, nslots * 4, hipMemcpyDeviceToDevice, st));
     if (o.splits)
-      MG_HIP(hipMemcpyAsync(o.splits, c->tmps.p, nn * sizeof(mgbvh_split_rec),
+      MG_HIP(hipMemcpyAsync(o.splits, c->tmps.p, nslots * sizeof(mgbvh_split_rec),
                             hipMemcpyDeviceToDevice, st));
-  }
-  if (o.tris_out && in.kind == 0) {
-    MG_PROF(KK_PERMUTE, 0);
-    k_permute_tris<<<cdiv(n, 256), 256, 0, st>>>(in.tris, o.perm, o.tris_out, n);
-    MG_PROF(KK_PERMUTE, 1);
-    MG_HIP(hipGetLastError());
   }
   MG_HIP(hipEventRecord(c->ev[63], st));
   MG_HIP(hipEventSynchronize(c->ev[63]));

@@ -2,6 +2,9 @@
 #pragma once
 #include <stdint.h>
 
+#include <string>
+#include <vector>
+
 #include "../../include/mgbvh.h"
 #include "mgbvh_common.h"
 
@@ -22,7 +25,10 @@ struct __align__(16) Rec {
   uint32_t pad;
 };
 
-// A node with > 32 objects at the current breadth-first level.
+// A node with > 32 objects at the current breadth-first level.  Object
+// positions, idx and P are absolute (over all meshes of a batched build);
+// nbase / obase are the node slot and object position of the mesh's root, so
+// a mesh's node array holds child = P - nbase and start_index = start - obase.
 struct LNode {
   uint32_t start, count, idx, P;      // object range, final node index, pair base
   uint32_t depth, chunk0, nchunk, kind;
@@ -30,15 +36,27 @@ struct LNode {
   uint32_t cacc[6];                   // centroid bounds: ~okey(min) x3, okey(max) x3
   uint32_t axis, cut, n_lt, pad0;     // chosen split
   int32_t child_slot[2];              // next-level slot of (gt, lt) child, -1 if none
-  uint32_t pad1[2];
+  uint32_t nbase, obase;
 };
 
 // A subtree root with <= 32 objects, or a leaf forced by max_depth.
 struct SNode {
   uint32_t start, count, idx, P;
-  uint32_t depth, parity, pad0, pad1;
+  uint32_t depth, parity, nbase, obase;
   float bmn[3], bmx[3];
   uint32_t pad2[2];
+};
+
+// One mesh of a (batched) build, device copy of the host-side plan.
+struct MeshRow {
+  uint32_t tri_offset, count, vertex_offset, vertex_count;
+  uint32_t setup_chunk0;  // first setup chunk (chunks of CH objects per mesh)
+  uint32_t nbase;         // first node slot: sum of 2 n - 1 over earlier meshes
+  uint32_t root_slot;     // LNode slot (large root) or SNode slot (small root)
+  uint32_t root_kind;     // 0 empty, 1 large root, 2 small root / leaf
+  uint32_t lchunk0;       // first level-0 chunk of a large root
+  uint32_t has_bounds;
+  float bmn[3], bmx[3];
 };
 
 struct Counters {
@@ -82,9 +100,6 @@ __device__ inline void NodeOut::write_node(uint32_t idx, const Box& b, uint32_t 
 // ---------------------------------------------------------------- host side
 #include <hip/hip_runtime.h>
 
-#include <string>
-#include <vector>
-
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
@@ -109,10 +124,20 @@ struct BuildOutput {
   mgbvh_split_rec* splits;
 };
 
+// One mesh of a batched build (host side).  Triangle vertex indices are local
+// to the mesh (< vertex_count) and address vertex_offset + index.
+struct MeshIn {
+  uint32_t tri_offset, count, vertex_offset, vertex_count;
+  int has_bounds;
+  float bmn[3], bmx[3];
+};
+
 struct BuildResult {
-  uint64_t nnodes;
+  uint64_t nnodes;            // total over meshes
   uint32_t height, n_levels, n_median, n_cutoff, n_guard;
   float ms;
+  std::vector<uint32_t> mesh_nodes;   // node count per mesh
+  std::vector<uint32_t> mesh_nbase;   // first node slot per mesh
 };
 
 }  // namespace mg
@@ -131,6 +156,7 @@ struct mgbvh_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[64] = {};
   DevBuf recs[2], L[2], hist, blk, cn[2], ck[2], small, ctr, gaps, gtab, tmpn, tmpl, tmps, lens;
+  DevBuf mrows, macc, mnb;
   DevBuf io_pos, io_tris, io_nodes, io_lens, io_perm, io_tris_out, io_splits, io_boxes;
   mg::Counters* h_ctr = nullptr;  // pinned
 };
@@ -141,6 +167,10 @@ int hip_fail(hipError_t e, const char* what);
 int ensure(DevBuf& b, size_t bytes);
 int build_device(mgbvh_ctx* c, const BuildInput& in, uint32_t n, const mgbvh_settings* s,
                  const BuildOutput& o, BuildResult* res);
+// Many object lists (meshes) built together: each gets the tree build_device
+// would give it alone, in node slots [nbase, nbase + 2n - 1).
+int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& meshes,
+                const mgbvh_settings* s, const BuildOutput& o, BuildResult* res);
 // Event pair around one kernel launch (no-op unless kind is in ctx->prof_mask).
 int prof_begin(mgbvh_ctx* c, int kind);
 int prof_end(mgbvh_ctx* c, int kind);

@@ -1,0 +1,153 @@
+"""Batched BLAS build against one-mesh-at-a-time builds, through the C ABI.
+
+Every mesh of a batch must get exactly the tree, permutation, permuted
+triangle list and split records of its own single-mesh build (which the
+builder tests pin bit for bit to the oracle); a few meshes are also checked
+against the oracle directly.
+"""
+import numpy as np
+import pytest
+
+from marigold_amd import _native as N
+from marigold_amd import scenes
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_meshes():
+    out = []
+    for k, n in enumerate([1, 2, 31, 32, 33, 100, 2047, 2049, 5000, 0, 7, 300, 65, 4097]):
+        out.append(scenes.soup(n, seed=500 + k) if n else
+                   (np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32)))
+    out.append(scenes.heightfield(40, 30, seed=7))
+    out.append(scenes.clusters(3000, k=8, seed=8))
+    # identical triangles: the median fallback inside a batch
+    p = np.tile(np.array([[0.1, 0.2, 0.3], [0.4, 0.2, 0.3], [0.1, 0.6, 0.9]], np.float32), (90, 1))
+    out.append((p, np.arange(270, dtype=np.uint32).reshape(90, 3)))
+    out.append(scenes.heightfield(8, 3, seed=9))
+    return out
+
+
+def _concat(meshes):
+    pos = np.ascontiguousarray(np.concatenate([m[0] for m in meshes]).astype(np.float32))
+    tris = np.ascontiguousarray(np.concatenate([m[1] for m in meshes]).astype(np.uint32))
+    desc = np.zeros(len(meshes), dtype=N.MESH_DTYPE)
+    to = vo = 0
+    for i, (p, t) in enumerate(meshes):
+        desc[i] = (to, t.shape[0], vo, p.shape[0])
+        to += t.shape[0]
+        vo += p.shape[0]
+    return pos, tris, desc
+
+
+def _batch(ctx, meshes, max_depth=32, mesh_bounds=None):
+    pos, tris, desc = _concat(meshes)
+    n = tris.shape[0]
+    slots = int(sum(2 * m[1].shape[0] - 1 for m in meshes if m[1].shape[0]))
+    d_pos, d_tris = ctx.to_device(pos), ctx.to_device(tris)
+    d_nodes, d_lens = ctx.alloc(32 * slots), ctx.alloc(4 * slots)
+    d_splits = ctx.alloc(16 * slots)
+    d_perm, d_tout = ctx.alloc(4 * n), ctx.alloc(12 * n)
+    blas, counts, st = ctx.build_blas_batch(d_pos, 12, pos.shape[0], d_tris, desc,
+                                            N.default_settings(max_depth=max_depth), d_nodes,
+                                            d_lens, d_perm, d_tout, d_splits, mesh_bounds)
+    res = dict(blas=blas, counts=counts, stats=st,
+               nodes=d_nodes.download(N.NODE_DTYPE, slots), lens=d_lens.download(np.uint32, slots),
+               splits=d_splits.download(N.SPLIT_DTYPE, slots),
+               perm=d_perm.download(np.uint32, n), tris=d_tout.download(np.uint32, 3 * n).reshape(-1, 3))
+    for b in (d_pos, d_tris, d_nodes, d_lens, d_splits, d_perm, d_tout):
+        b.free()
+    return res, desc
+
+
+def _check_against_single(ctx, meshes, res, desc, max_depth=32, mesh_bounds=None, tag=""):
+    for i, (p, t) in enumerate(meshes):
+        n = t.shape[0]
+        assert res["blas"][i]["tri_offset"] == desc[i]["tri_offset"]
+        if n == 0:
+            assert res["counts"][i] == 0
+            continue
+        bounds = None if mesh_bounds is None else (mesh_bounds[i, :3], mesh_bounds[i, 3:])
+        tt = t.copy()
+        nodes, lens, perm, splits, _ = ctx.build_tris(
+            p, tt, N.default_settings(max_depth=max_depth, bounds=bounds), want_splits=True)
+        k = nodes.shape[0]
+        nb = int(res["blas"][i]["node_offset"])
+        to = int(desc[i]["tri_offset"])
+        assert res["counts"][i] == k, (tag, i)
+        got = res["nodes"][nb:nb + k].view(np.uint32).reshape(-1, 8)
+        want = nodes.view(np.uint32).reshape(-1, 8)
+        bad = np.nonzero((got != want).any(axis=1))[0]
+        assert bad.size == 0, (tag, "mesh", i, "n", n, "node", bad[:5], got[bad[:2]], want[bad[:2]])
+        assert np.array_equal(res["lens"][nb:nb + k], lens), (tag, i)
+        assert np.array_equal(res["splits"][nb:nb + k].view(np.uint32), splits.view(np.uint32)), (tag, i)
+        assert np.array_equal(res["perm"][to:to + n], perm), (tag, i)
+        assert np.array_equal(res["tris"][to:to + n], tt), (tag, i)
+
+
+def test_batch_equals_single_builds(ctx):
+    print("[case] batch of 18 mixed meshes", flush=True)
+    meshes = _mixed_meshes()
+    res, desc = _batch(ctx, meshes)
+    _check_against_single(ctx, meshes, res, desc, tag="mixed")
+    assert res["stats"]["total_object_count"] == sum(m[1].shape[0] for m in meshes)
+    assert res["stats"]["n_median"] > 0
+
+
+def test_batch_against_oracle(ctx):
+    print("[case] batch vs oracle", flush=True)
+    meshes = [scenes.soup(3000, seed=61), scenes.heightfield(30, 20, seed=62), scenes.soup(40, seed=63)]
+    res, desc = _batch(ctx, meshes)
+    for i, (p, t) in enumerate(meshes):
+        ref = O.build(O.tri_aabbs(p, t))
+        want = O.encode(ref["nodes"])
+        nb, k = int(res["blas"][i]["node_offset"]), want.shape[0]
+        assert np.array_equal(res["nodes"][nb:nb + k].view(np.uint32).reshape(-1, 8), want), i
+        to, n = int(desc[i]["tri_offset"]), t.shape[0]
+        assert np.array_equal(res["perm"][to:to + n], ref["list"]), i
+
+
+@pytest.mark.parametrize("max_depth", [0, 2, 6])
+def test_batch_depth_cut(ctx, max_depth):
+    # cut leaves leave unused slots that are squeezed out inside each mesh
+    print(f"[case] batch max_depth={max_depth}", flush=True)
+    meshes = [scenes.soup(n, seed=700 + n) for n in (500, 3, 40, 0, 2500, 90)]
+    res, desc = _batch(ctx, meshes, max_depth=max_depth)
+    _check_against_single(ctx, meshes, res, desc, max_depth=max_depth, tag=f"cut{max_depth}")
+    if max_depth < 6:
+        assert res["stats"]["n_cutoff_leaves"] > 0
+
+
+def test_batch_mesh_bounds(ctx):
+    print("[case] batch with per-mesh bounds", flush=True)
+    meshes = [scenes.soup(n, seed=800 + n) for n in (1000, 50, 20)]
+    mb = np.array([np.concatenate([p.min(axis=0) - 0.25, p.max(axis=0) + 0.5]) for p, _ in meshes],
+                  np.float32)
+    res, desc = _batch(ctx, meshes, mesh_bounds=mb)
+    _check_against_single(ctx, meshes, res, desc, mesh_bounds=mb, tag="bounds")
+
+
+def test_batch_errors(ctx):
+    meshes = [scenes.soup(10, seed=1), scenes.soup(20, seed=2)]
+    pos, tris, desc = _concat(meshes)
+    d_pos, d_tris = ctx.to_device(pos), ctx.to_device(tris)
+    d_nodes, d_lens, d_perm = ctx.alloc(32 * 60), ctx.alloc(4 * 60), ctx.alloc(4 * 30)
+    s = N.default_settings()
+    bad = desc.copy()
+    bad[1]["tri_offset"] = 11  # not contiguous
+    with pytest.raises(N.MgbvhError) as e:
+        ctx.build_blas_batch(d_pos, 12, pos.shape[0], d_tris, bad, s, d_nodes, d_lens, d_perm)
+    assert e.value.code == N.ERR_INVALID_ARG
+    bad = desc.copy()
+    bad[1]["vertex_count"] = 10_000  # beyond the vertex array
+    with pytest.raises(N.MgbvhError) as e:
+        ctx.build_blas_batch(d_pos, 12, pos.shape[0], d_tris, bad, s, d_nodes, d_lens, d_perm)
+    assert e.value.code == N.ERR_INDEX
+    bad = desc.copy()
+    bad[0]["vertex_count"] = 5  # triangle indices of mesh 0 reach 29
+    with pytest.raises(N.MgbvhError) as e:
+        ctx.build_blas_batch(d_pos, 12, pos.shape[0], d_tris, bad, s, d_nodes, d_lens, d_perm)
+    assert e.value.code == N.ERR_INDEX
+    for b in (d_pos, d_tris, d_nodes, d_lens, d_perm):
+        b.free()
