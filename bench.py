@@ -1,12 +1,21 @@
 #!/usr/bin/env python3
 """BVH build throughput on one MI355X.
 
-A step is one full BVH build of the workload's mesh (device-resident input ->
-device-resident 32-byte nodes + permutation), through the C ABI's device entry
-point.  Prints one JSON line with Mtri/s, the dominant kernel's HBM roofline
-fraction (timed live with HIP events on the build stream) and a CPU baseline
-(the oracle, a single-threaded C restatement of marigold's builder, timed on a
-bounded sample on this host).
+Workloads:
+* ``scene`` (default): the 2,000-mesh scene (log-uniform 100..200,000
+  triangles per mesh, half soups and half heightfields, ~52M triangles) with
+  10,000 instances.  A step is one whole scene build: every BLAS, the instance
+  world boxes and the TLAS, device-resident input -> device-resident nodes.
+  `value` is the batched builder; the per-mesh plain builder (one build call
+  per mesh, as the reference builds one BVH per mesh) is timed beside it.
+* ``heightfield``: one 1024x512-cell heightfield mesh (1,048,576 triangles),
+  a step is one plain build through mgbvh_build_tris_device.
+* ``soup``: one 65,536-triangle soup.
+
+Prints one JSON line with Mtri/s, the dominant kernel's HBM roofline fraction
+(timed live with HIP events on the build stream) and a CPU baseline (the
+oracle, a C restatement of marigold's builder, timed on a bounded sample on
+this host).
 """
 from __future__ import annotations
 
@@ -59,11 +68,15 @@ def kernel_bytes(kind: str, n: int, nverts: int, vstride: int, t: dict) -> float
         return 64.0 * t["s_large"]                            # record read + write
     if kind == "sweep":
         return t["n_large"] * (3 * 32 * 7 * 4 + 32.0)        # histogram read + node write
-    if kind == "small":
-        return 32.0 * t["s_small"] + 36.0 * t["n_small_nodes"] + 4.0 * t["s_small"]
-    if kind == "permute":
-        return 4.0 * n + 24.0 * n
+    if kind == "small":  # records, node + len writes, perm write, permuted triangle read + write
+        return 32.0 * n + 36.0 * t["n_small_nodes"] + 4.0 * n + 24.0 * n
     return 0.0
+
+
+def terms_from_stats(st: dict) -> dict:
+    """The same sums from the builder's own work counters."""
+    return {"s_large": int(st["large_object_visits"]), "n_large": int(st["large_nodes"]),
+            "n_small_nodes": int(st["node_count"]) - int(st["large_nodes"])}
 
 
 def workload(name: str):
@@ -96,18 +109,167 @@ def cpu_baseline(pos, tris, budget_s: float = 12.0):
                       f"of marigold bvh (median {med:.3f} s; AABB setup excluded)"}
 
 
+def cpu_baseline_scene(meshes, budget_s: float = 15.0, threads: int = 0):
+    """Per-mesh oracle builds on a thread pool (one mesh per task, as rayon's
+    par_iter over meshes in model.rs:76-95), over a random sample of meshes
+    until the budget is spent."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O  # CPU baseline only
+    threads = threads or min(16, os.cpu_count() or 1)
+    rng = np.random.default_rng(0)
+    order = rng.permutation(len(meshes))
+    done_tris = 0
+    t0 = time.perf_counter()
+    deadline = t0 + budget_s
+
+    def one(i):
+        p, t = meshes[i]
+        O.build(O.tri_aabbs(p, t), max_depth=32)
+        return t.shape[0]
+
+    n_done = 0
+    with ThreadPoolExecutor(threads) as ex:
+        it = iter(order)
+        pending = set()
+        for _ in range(threads):
+            i = next(it, None)
+            if i is not None:
+                pending.add(ex.submit(one, i))
+        while pending:
+            f = next(iter(pending))
+            done_tris += f.result()
+            n_done += 1
+            pending.remove(f)
+            if time.perf_counter() < deadline:
+                i = next(it, None)
+                if i is not None:
+                    pending.add(ex.submit(one, i))
+    wall = time.perf_counter() - t0
+    return {"value": done_tris / wall / 1e6, "unit": "Mtri/s", "cores": threads, "kind": "port",
+            "sample": f"{n_done} randomly chosen meshes of the scene ({done_tris} tris) built by the C "
+                      f"restatement of marigold bvh, one mesh per task on {threads} threads, "
+                      f"{wall:.1f} s wall; AABB setup included, TLAS excluded"}
+
+
+def scene_main(args):
+    from marigold_amd import _native as N
+    from marigold_amd import scenes
+    from marigold_amd.scene import GpuScene
+
+    t0 = time.perf_counter()
+    sc = scenes.batched_scene()
+    meshes = list(zip(sc["positions"], sc["tris"]))
+    n = int(sum(t.shape[0] for _, t in meshes))
+    nv = int(sum(p.shape[0] for p, _ in meshes))
+    print(f"[bench] scene: {len(meshes)} meshes, {n} tris, {nv} verts, "
+          f"{len(sc['instance_mesh'])} instances (generated in {time.perf_counter() - t0:.1f} s)",
+          file=sys.stderr, flush=True)
+    ctx = N.Context(0)
+    bat = GpuScene(ctx, meshes, sc["transforms"], sc["instance_mesh"], mode="batched", build=False)
+    print("[bench] uploaded", file=sys.stderr, flush=True)
+
+    # untimed warm-up with every kernel class timed, to find the dominant one
+    ctx.set_profiling(N.Context.KERNEL_KINDS)
+    for _ in range(max(args.warmup, 1)):
+        bat.build()
+    prof = ctx.kernel_times(reset=True)
+    dominant = max(prof, key=lambda k: prof[k][0])
+    # the work counters of one batched BLAS build
+    d = bat.d
+    _, _, st = ctx.build_blas_batch(d["pos"], 12, nv, d["tris"], bat.mesh_desc, bat.blas_settings,
+                                    d["blas_nodes"], d["blas_lens"], d["blas_perm"], d["tris_perm"])
+    terms = terms_from_stats(st)
+    print(f"[bench] warm-up done; dominant kernel class {dominant}", file=sys.stderr, flush=True)
+
+    ctx.set_profiling([dominant])
+    ctx.synchronize()
+    ctx.record(0)
+    for k in range(args.steps):
+        bat.build()
+    ctx.record(1)
+    ctx.synchronize()
+    ms_step = ctx.elapsed_ms(0, 1) / args.steps
+    kt = ctx.kernel_times(reset=True)[dominant]
+    dom_ms = kt[0] / args.steps
+    dom_bytes = kernel_bytes(dominant, n, nv, 12, terms)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    build_bytes = sum(kernel_bytes(k, n, nv, 12, terms) for k in ("setup", "bin", "partition",
+                                                                   "sweep", "small"))
+    print(f"[bench] batched {ms_step:.2f} ms/scene", file=sys.stderr, flush=True)
+    bat_nodes_sum = int(bat.download("blas_nodes", np.uint32, 8 * bat.n_slots).astype(np.uint64).sum())
+    bat.free()
+
+    # the plain builder: one device build call per mesh, then the same TLAS path
+    ms_plain, same = None, None
+    if args.plain_steps > 0:
+        ctx.set_profiling([])
+        pla = GpuScene(ctx, meshes, sc["transforms"], sc["instance_mesh"], mode="plain",
+                       build=False)
+        pla.build()
+        ctx.synchronize()
+        ctx.record(2)
+        for _ in range(args.plain_steps):
+            pla.build()
+        ctx.record(3)
+        ctx.synchronize()
+        ms_plain = ctx.elapsed_ms(2, 3) / args.plain_steps
+        same = bat_nodes_sum == int(pla.download("blas_nodes", np.uint32, 8 * pla.n_slots)
+                                    .astype(np.uint64).sum())
+        print(f"[bench] plain {ms_plain:.2f} ms/scene", file=sys.stderr, flush=True)
+        pla.free()
+    out = {
+        "metric": "BVH build Mtri/s, plain vs learned+batched, at SAH <=1.03x plain; 1 MI355X",
+        "value": n / (ms_step * 1e-3) / 1e6,
+        "unit": "Mtri/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded)",
+        "config": {"workload": "2,000-mesh scene (log-uniform 100..200,000 tris, 50/50 soup / "
+                               "heightfield) + TLAS over 10,000 instances; whole scene build",
+                   "tris": n, "verts": nv, "meshes": len(meshes), "instances": len(sc["instance_mesh"]),
+                   "max_depth": 32, "method": "batched (plain split rule)"},
+        "plain_per_mesh": None if ms_plain is None else
+        {"value": n / (ms_plain * 1e-3) / 1e6, "ms_per_step": ms_plain, "steps": args.plain_steps,
+         "node_arrays_identical": same},
+        "speedup_vs_plain": None if ms_plain is None else ms_plain / ms_step,
+        "sah_ratio_vs_plain": 1.0 if same else None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
+                     "kernel_ms_per_build": dom_ms, "alg_bytes_per_build": dom_bytes},
+        "build_alg_gbs": build_bytes / (ms_step * 1e-3) / 1e9,
+        "kernel_ms_per_build": {k: v[0] / max(args.warmup, 1) for k, v in prof.items()},
+        "work": terms,
+    }
+    ctx.close()
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline_scene(meshes)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="heightfield")
+    ap.add_argument("--workload", default="scene", choices=["scene", "heightfield", "soup"])
+    ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the builder shards "
-              f"meshes across GPUs only by design, see the sharding notes)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: multi-GPU sharding "
+              f"of the meshes is a design note only", file=sys.stderr)
         raise SystemExit(2)
+    if args.workload == "scene":
+        return scene_main(args)
 
     from marigold_amd import _native as N
 
@@ -156,7 +318,7 @@ def main():
     dom_bytes = kernel_bytes(dominant, n, nv, 12, terms)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     build_bytes = sum(kernel_bytes(k, n, nv, 12, terms) for k in ("setup", "bin", "partition", "sweep",
-                                                                   "small", "permute"))
+                                                                   "small"))
     out = {
         "metric": "BVH build Mtri/s, plain vs learned+batched, at SAH <=1.03x plain; 1 MI355X",
         "value": n / (ms_step * 1e-3) / 1e6,

@@ -104,7 +104,10 @@ typedef struct mgbvh_stats {
   uint32_t n_median;          /* median fallback splits */
   uint32_t n_cutoff_leaves;   /* leaves forced by max_depth holding > 1 object */
   uint32_t n_guard_fallbacks; /* learned mode: windows rejected by the exact guard */
-  uint32_t reserved[5];
+  uint64_t large_object_visits; /* sum of object counts of the nodes with > 32 objects
+                                   (the binned levels read and move each such object once) */
+  uint32_t large_nodes;       /* nodes with > 32 objects (binned sweeps) */
+  uint32_t reserved[2];
 } mgbvh_stats;
 
 typedef struct mgbvh_ctx mgbvh_ctx;
@@ -293,6 +296,15 @@ int mgbvh_build_blas_batch_device(mgbvh_ctx* ctx, const float* d_positions,
                                   uint32_t* d_tris_out, mgbvh_split_rec* d_splits,
                                   mgbvh_blas_desc* out_blas, uint32_t* out_node_counts,
                                   mgbvh_stats* out_stats);
+
+/* World boxes of instances, the TLAS objects (model.rs:189-221): the 8
+ * corners of the instance's BLAS root box through its object_to_world
+ * transform (3 rows x 4 floats per instance, each row summed left to right as
+ * glam's Mat4 * Vec4), then the component-wise min and max. */
+int mgbvh_instance_boxes_device(mgbvh_ctx* ctx, const mgbvh_node32* d_blas_nodes,
+                                const mgbvh_blas_desc* d_blas, uint32_t nblas,
+                                const float* d_object_to_world, const uint32_t* d_instance_blas,
+                                uint32_t ninstances, mgbvh_aabb* d_boxes);
 
 /* ---- timing helpers: HIP events recorded on the context stream ---- */
 int mgbvh_event_record(mgbvh_ctx* ctx, int slot);           /* slot 0..59 */

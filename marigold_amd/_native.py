@@ -92,7 +92,9 @@ class Stats(C.Structure):
         ("n_median", C.c_uint32),
         ("n_cutoff_leaves", C.c_uint32),
         ("n_guard_fallbacks", C.c_uint32),
-        ("reserved", C.c_uint32 * 5),
+        ("large_object_visits", C.c_uint64),
+        ("large_nodes", C.c_uint32),
+        ("reserved", C.c_uint32 * 2),
     ]
 
     def as_dict(self):
@@ -161,6 +163,7 @@ _EXTRA: dict = {
     "mgbvh_trace_scene_device": ([_P, _P, _P, _P, _P, _U32, _P, _U32, _P, _P, _P, _U32, _P, _P,
                                   _P, _U64, _P], _I),
     "mgbvh_gen_primary_rays_device": ([_P, C.POINTER(Camera), _U32, _U32, _P], _I),
+    "mgbvh_instance_boxes_device": ([_P, _P, _P, _U32, _P, _P, _U32, _P], _I),
     "mgbvh_build_blas_batch_device": ([_P, _P, _U32, _U64, _P, _P, _U32, _P, C.POINTER(Settings),
                                        _P, _P, _P, _P, _P, _P, _P, C.POINTER(Stats)], _I),
     "mgbvh_gen_bounce_rays_device": ([_P, _P, _P, _U64, _P, _U32, _P, _U32, _P, _U32, _P, _U32,
@@ -351,6 +354,34 @@ class Context:
             C.byref(settings), _dv(nodes), _dv(lens), _dv(perm), _dv(tris_out), _dv(splits),
             ptr(blas), ptr(counts), C.byref(st)))
         return blas[:nm], counts[:nm], st.as_dict()
+
+    def instance_boxes(self, blas_nodes, blas, nblas, object_to_world, instance_blas, ninst,
+                       boxes):
+        check(lib().mgbvh_instance_boxes_device(self.handle, _dv(blas_nodes), _dv(blas), nblas,
+                                                _dv(object_to_world), _dv(instance_blas), ninst,
+                                                _dv(boxes)))
+
+    def build_aabbs_device(self, boxes, n, settings: Settings, nodes, lens, perm, splits=None):
+        nn = C.c_uint64()
+        check(lib().mgbvh_build_aabbs_device(self.handle, _dv(boxes), n, C.byref(settings),
+                                             _dv(nodes), _dv(lens), _dv(perm), _dv(splits),
+                                             C.byref(nn), None))
+        return nn.value
+
+    def build_tris_device(self, pos, vstride, nverts, tris, n, settings: Settings, nodes, lens,
+                          perm, tris_out=None, splits=None, pos_offset=0, tri_offset=0,
+                          node_offset=0):
+        """Device build of one mesh whose arrays start at the given element
+        offsets of larger device buffers.  Returns the node count."""
+        nn = C.c_uint64()
+        def at(b, off, size):
+            return None if b is None else C.c_void_p(b.ptr.value + off * size)
+        check(lib().mgbvh_build_tris_device(
+            self.handle, at(pos, pos_offset, vstride), vstride, nverts, at(tris, tri_offset, 12),
+            n, C.byref(settings), at(nodes, node_offset, 32), at(lens, node_offset, 4),
+            at(perm, tri_offset, 4), at(tris_out, tri_offset, 12), at(splits, node_offset, 16),
+            C.byref(nn), None))
+        return nn.value
 
     # ---- host-buffer builds -------------------------------------------------
     def build_tris(self, positions: np.ndarray, tris: np.ndarray, settings: Settings,

@@ -157,6 +157,8 @@ static void fill_stats(const std::vector<mgbvh_node32>& nodes, const std::vector
   st->n_median = r.n_median;
   st->n_cutoff_leaves = r.n_cutoff;
   st->n_guard_fallbacks = r.n_guard;
+  st->large_object_visits = r.large_visits;
+  st->large_nodes = (uint32_t)r.large_nodes;
 }
 
 // bvh/src/lib.rs:851-866: bvh_<name>_<9-digit id>.json
@@ -398,6 +400,8 @@ int mgbvh_build_blas_batch_device(mgbvh_ctx* c, const float* d_pos, uint32_t vst
     out_stats->n_large_levels = r.n_levels;
     out_stats->n_median = r.n_median;
     out_stats->n_cutoff_leaves = r.n_cutoff;
+    out_stats->large_object_visits = r.large_visits;
+    out_stats->large_nodes = (uint32_t)r.large_nodes;
   }
   return MGBVH_OK;
 }

@@ -343,7 +343,7 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
                                                 NodeOut out, uint32_t max_depth,
                                                 uint32_t parity_next) {
   __shared__ uint32_t sh[SW_WAVES][HIST_WORDS];
-  __shared__ uint32_t s_need[SW_WAVES][3];  // large slots, chunks, small slots
+  __shared__ uint32_t s_need[SW_WAVES][4];  // large slots, chunks, small slots, objects
   __shared__ uint32_t s_base[SW_WAVES][3];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t lane = lane_id();
@@ -473,8 +473,14 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
     s_need[w][0] = (uint32_t)lg + (uint32_t)ll;
     s_need[w][1] = chg + chl;
     s_need[w][2] = active ? 2 - ((uint32_t)lg + (uint32_t)ll) : 0;
+    s_need[w][3] = active ? nd.count : 0;
   }
   __syncthreads();
+  if (threadIdx.x == 3) {  // work counter: objects of the nodes split on this level
+    unsigned long long tot = 0;
+    for (uint32_t q = 0; q < SW_WAVES; ++q) tot += s_need[q][3];
+    if (tot) atomicAdd(&ctr->sum_large, tot);
+  }
   if (threadIdx.x < 3) {
     const uint32_t k = threadIdx.x;
     uint32_t tot = 0;
@@ -1112,7 +1118,9 @@ int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& m
   uint32_t nL = n_large, nC = (uint32_t)lchunks;
   int cur = 0;
   uint32_t levels = 0;
+  uint64_t large_nodes = 0;
   while (nL > 0) {
+    large_nodes += nL;
     MG_HIP(hipMemsetAsync(hist, 0, (size_t)nL * HIST_WORDS * 4, st));
     MG_HIP(hipMemsetAsync(ctr, 0, 8, st));  // n_large_next, n_chunks_next
     MG_PROF(KK_BIN, 0);
@@ -1213,6 +1221,8 @@ int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& m
   res->n_levels = levels;
   res->n_median = hc.n_median;
   res->n_cutoff = hc.n_cutoff;
+  res->large_nodes = large_nodes;
+  res->large_visits = hc.sum_large;
   res->ms = ms;
   return MGBVH_OK;
 }

@@ -63,7 +63,8 @@ struct Counters {
   uint32_t n_large_next, n_chunks_next, n_small, height;
   uint32_t err, n_gaps, n_median, n_cutoff;
   uint32_t root_acc[12];  // bounds (~min, max) and centroid bounds (~min, max)
-  uint32_t pad[12];
+  unsigned long long sum_large;  // objects of all nodes split on the level path
+  uint32_t pad[10];
 };
 
 struct NodeOut {
@@ -135,6 +136,7 @@ struct MeshIn {
 struct BuildResult {
   uint64_t nnodes;            // total over meshes
   uint32_t height, n_levels, n_median, n_cutoff, n_guard;
+  uint64_t large_nodes, large_visits;  // nodes with > 32 objects, sum of their object counts
   float ms;
   std::vector<uint32_t> mesh_nodes;   // node count per mesh
   std::vector<uint32_t> mesh_nbase;   // first node slot per mesh

@@ -1,17 +1,22 @@
 """Two-level scene on the device: one BLAS per mesh, a TLAS over instances.
 
 Mirrors the scene upload of marigold (model.rs:137-230: meshes concatenated
-with vertex/triangle offsets and a BLAS root per mesh; instance world boxes
+with vertex/triangle offsets and a BLAS root per mesh, instance world boxes
 from the 8 transformed corners of the mesh box; geometry/mod.rs:179-199: the
 TLAS built over the instances with TLAS_MAX_DEPTH = 32, scene/mod.rs:28-29).
-Every build and trace runs through the C ABI of libmgbvh.so.
+Every build and trace runs on the device through the C ABI of libmgbvh.so:
+
+* ``mode="plain"``   one mgbvh_build_tris_device call per mesh, as the
+                     reference calls BoundingVolumeHierarchy::new per mesh;
+* ``mode="batched"`` one mgbvh_build_blas_batch_device call for all meshes.
+
+Both give identical node arrays (the batched build is bit-identical per mesh).
 """
 from __future__ import annotations
 
 import numpy as np
 
 from . import _native as N
-from . import scenes
 
 BLAS_MAX_DEPTH = 32
 TLAS_MAX_DEPTH = 32
@@ -24,71 +29,110 @@ def world_to_object(transforms: np.ndarray) -> np.ndarray:
 
 
 class GpuScene:
-    """Builds every mesh's BLAS and the TLAS on the device and keeps the
-    concatenated arrays resident for tracing."""
+    """Scene arrays resident on the device; `build()` rebuilds every BLAS,
+    the instance boxes and the TLAS from them without host round trips of
+    the geometry."""
 
     def __init__(self, ctx: N.Context, meshes, transforms: np.ndarray, instance_mesh: np.ndarray,
-                 blas_max_depth: int = BLAS_MAX_DEPTH, tlas_max_depth: int = TLAS_MAX_DEPTH,
-                 method: int = N.METHOD_PLAIN):
+                 mode: str = "batched", blas_max_depth: int = BLAS_MAX_DEPTH,
+                 tlas_max_depth: int = TLAS_MAX_DEPTH, build: bool = True):
+        assert mode in ("plain", "batched")
         self.ctx = ctx
-        nm = len(meshes)
-        self.n_meshes = nm
+        self.mode = mode
+        self.n_meshes = len(meshes)
         self.n_instances = int(len(instance_mesh))
-        vert_off, tri_off, node_off = [], [], []
-        pos_all, tris_orig, tris_perm, perm_all, nodes_all, lens_all = [], [], [], [], [], []
-        nv = nt = nn = 0
-        mesh_min = np.empty((nm, 3), np.float32)
-        mesh_max = np.empty((nm, 3), np.float32)
-        self.blas_stats = []
-        for p, t in meshes:
-            t_work = np.ascontiguousarray(t, dtype=np.uint32).copy()
-            nodes, lens, perm, _, st = ctx.build_tris(
-                np.ascontiguousarray(p, dtype=np.float32), t_work,
-                N.default_settings(max_depth=blas_max_depth, method=method))
-            self.blas_stats.append(st)
-            vert_off.append(nv); tri_off.append(nt); node_off.append(nn)
-            mesh_min[len(vert_off) - 1] = nodes[0]["min"]
-            mesh_max[len(vert_off) - 1] = nodes[0]["max"]
-            pos_all.append(p); tris_orig.append(t); tris_perm.append(t_work)
-            perm_all.append(perm); nodes_all.append(nodes); lens_all.append(lens)
-            nv += p.shape[0]; nt += t.shape[0]; nn += nodes.shape[0]
-        self.positions = np.ascontiguousarray(np.concatenate(pos_all).astype(np.float32))
-        self.tris_orig = np.ascontiguousarray(np.concatenate(tris_orig).astype(np.uint32))
-        self.tris_perm = np.ascontiguousarray(np.concatenate(tris_perm))
-        self.blas_perm = np.ascontiguousarray(np.concatenate(perm_all))
-        self.blas_nodes = np.ascontiguousarray(np.concatenate(nodes_all))
-        self.blas_lens = np.ascontiguousarray(np.concatenate(lens_all))
-        self.blas = np.zeros(nm, dtype=N.BLAS_DTYPE)
-        self.blas["node_offset"] = node_off
-        self.blas["tri_offset"] = tri_off
-        self.blas["vertex_offset"] = vert_off
-        self.mesh3 = np.stack([np.array(tri_off), [m[1].shape[0] for m in meshes],
-                               np.array(vert_off)], axis=1).astype(np.uint32)
+        counts = np.array([t.shape[0] for _, t in meshes], dtype=np.int64)
+        vcounts = np.array([p.shape[0] for p, _ in meshes], dtype=np.int64)
+        self.tri_offsets = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)
+        self.vert_offsets = np.concatenate([[0], np.cumsum(vcounts)[:-1]]).astype(np.int64)
+        slots = np.where(counts > 0, 2 * counts - 1, 0)
+        self.node_offsets = np.concatenate([[0], np.cumsum(slots)[:-1]]).astype(np.int64)
+        self.n_tris = int(counts.sum())
+        self.n_slots = int(slots.sum())
+        self.counts = counts
+        self.positions = np.ascontiguousarray(np.concatenate([p for p, _ in meshes]).astype(np.float32))
+        self.tris_orig = np.ascontiguousarray(np.concatenate([t for _, t in meshes]).astype(np.uint32))
+        self.mesh_desc = np.zeros(self.n_meshes, dtype=N.MESH_DTYPE)
+        self.mesh_desc["tri_offset"] = self.tri_offsets
+        self.mesh_desc["tri_count"] = counts
+        self.mesh_desc["vertex_offset"] = self.vert_offsets
+        self.mesh_desc["vertex_count"] = vcounts
+        self.mesh3 = np.stack([self.tri_offsets, counts, self.vert_offsets], axis=1).astype(np.uint32)
+        self.blas = np.zeros(self.n_meshes, dtype=N.BLAS_DTYPE)
+        self.blas["node_offset"] = self.node_offsets
+        self.blas["tri_offset"] = self.tri_offsets
+        self.blas["vertex_offset"] = self.vert_offsets
         self.transforms = np.ascontiguousarray(transforms, dtype=np.float32)
+        self.o2w = np.ascontiguousarray(self.transforms[:, :3, :].reshape(-1, 12))
+        self.instance_mesh = np.ascontiguousarray(instance_mesh, dtype=np.uint32)
         self.instances = np.zeros(self.n_instances, dtype=N.INSTANCE_DTYPE)
         self.instances["world_to_object"] = world_to_object(self.transforms)
-        self.instances["blas"] = np.asarray(instance_mesh, dtype=np.uint32)
-        which = np.asarray(instance_mesh, dtype=np.int64)
-        self.instance_boxes = scenes.instance_boxes(mesh_min[which], mesh_max[which],
-                                                    self.transforms)
-        tnodes, tlens, tperm, _, tst = ctx.build_aabbs(
-            self.instance_boxes, N.default_settings(max_depth=tlas_max_depth, method=method,
-                                                    name="tlas"))
-        self.tlas_nodes, self.tlas_lens, self.tlas_perm, self.tlas_stats = tnodes, tlens, tperm, tst
-        d = ctx.to_device
+        self.instances["blas"] = self.instance_mesh
+        self.blas_settings = N.default_settings(max_depth=blas_max_depth)
+        self.tlas_settings = N.default_settings(max_depth=tlas_max_depth, name="tlas")
+        a, up = ctx.alloc, ctx.to_device
+        ni = self.n_instances
         self.d = {
-            "pos": d(self.positions), "tris_perm": d(self.tris_perm),
-            "blas_perm": d(self.blas_perm), "blas_nodes": d(self.blas_nodes),
-            "blas_lens": d(self.blas_lens), "blas": d(self.blas),
-            "instances": d(self.instances), "tlas_nodes": d(self.tlas_nodes),
-            "tlas_lens": d(self.tlas_lens), "tlas_perm": d(self.tlas_perm),
+            "pos": up(self.positions), "tris": up(self.tris_orig),
+            "tris_perm": a(12 * self.n_tris), "blas_perm": a(4 * self.n_tris),
+            "blas_nodes": a(32 * self.n_slots), "blas_lens": a(4 * self.n_slots),
+            "blas": up(self.blas), "instances": up(self.instances), "o2w": up(self.o2w),
+            "inst_blas": up(self.instance_mesh), "boxes": a(24 * ni),
+            "tlas_nodes": a(32 * max(2 * ni - 1, 1)), "tlas_lens": a(4 * max(2 * ni - 1, 1)),
+            "tlas_perm": a(4 * max(ni, 1)),
         }
+        self.n_tlas_nodes = 0
+        if build:
+            self.build()
+
+    # ------------------------------------------------------------------ build
+    def build_blas(self):
+        c, d = self.ctx, self.d
+        if self.mode == "batched":
+            blas, counts, st = c.build_blas_batch(
+                d["pos"], 12, self.positions.shape[0], d["tris"], self.mesh_desc,
+                self.blas_settings, d["blas_nodes"], d["blas_lens"], d["blas_perm"],
+                d["tris_perm"])
+            assert np.array_equal(blas["node_offset"], self.blas["node_offset"])
+            self.blas_node_counts = counts
+        else:
+            counts = np.zeros(self.n_meshes, dtype=np.uint32)
+            for m in range(self.n_meshes):
+                n = int(self.counts[m])
+                if n == 0:
+                    continue
+                counts[m] = c.build_tris_device(
+                    d["pos"], 12, int(self.mesh_desc[m]["vertex_count"]), d["tris"], n,
+                    self.blas_settings, d["blas_nodes"], d["blas_lens"], d["blas_perm"],
+                    d["tris_perm"], pos_offset=int(self.vert_offsets[m]),
+                    tri_offset=int(self.tri_offsets[m]), node_offset=int(self.node_offsets[m]))
+            self.blas_node_counts = counts
+
+    def build_tlas(self):
+        c, d = self.ctx, self.d
+        c.instance_boxes(d["blas_nodes"], d["blas"], self.n_meshes, d["o2w"], d["inst_blas"],
+                         self.n_instances, d["boxes"])
+        self.n_tlas_nodes = c.build_aabbs_device(d["boxes"], self.n_instances, self.tlas_settings,
+                                                 d["tlas_nodes"], d["tlas_lens"], d["tlas_perm"])
+
+    def build(self):
+        self.build_blas()
+        self.build_tlas()
+
+    # ------------------------------------------------------------------ readback
+    def download(self, name, dtype, count):
+        return self.d[name].download(dtype, count)
+
+    @property
+    def blas_perm(self) -> np.ndarray:
+        return self.download("blas_perm", np.uint32, self.n_tris)
 
     @property
     def inst16(self) -> np.ndarray:
         """Instance records as (I,16) u32 words (the oracle's format)."""
         return self.instances.view(np.uint32).reshape(-1, 16)
 
+    # ------------------------------------------------------------------ trace
     def trace(self, rays_dev: N.DeviceBuffer, nrays: int, hits_dev: N.DeviceBuffer):
         d = self.d
         self.ctx.trace_scene(d["tlas_nodes"], d["tlas_lens"], d["tlas_perm"], d["instances"],
@@ -100,6 +144,11 @@ class GpuScene:
         d = self.d
         self.ctx.gen_bounce(rays_in, hits, nrays, d["instances"], self.n_instances, d["blas"],
                             self.n_meshes, d["pos"], 12, d["tris_perm"], seed, rays_out)
+
+    def free(self):
+        for b in self.d.values():
+            b.free()
+        self.d = {}
 
 
 def look_at_camera(pos, target, fov_y: float, aspect: float, up=(0.0, 1.0, 0.0)) -> N.Camera:

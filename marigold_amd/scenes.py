@@ -13,16 +13,14 @@ NaN or -0.0.
 * ``batched_scene`` 2,000 meshes with log-uniform triangle counts in
                     [100, 200,000], half soups and half heightfields, plus
                     10,000 instances (random rotation, scale U[0.5, 2],
-                    translation U[-100, 100]^3) and their world boxes computed
-                    from 8 transformed corners as in
-                    marigold/src/app/data/scene/model.rs:190-234.
+                    translation U[-100, 100]^3).
 * ``clusters``      N triangles in k Gaussian clusters (sigma U[0.01, 0.1]).
 """
 from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["soup", "heightfield", "value_noise", "batched_scene", "clusters", "instance_boxes"]
+__all__ = ["soup", "heightfield", "value_noise", "batched_scene", "clusters", "random_transforms"]
 
 
 def _canon(x: np.ndarray) -> np.ndarray:
@@ -120,36 +118,28 @@ def _random_rotations(rng, m):
     return R
 
 
-def instance_boxes(mesh_min: np.ndarray, mesh_max: np.ndarray, transforms: np.ndarray) -> np.ndarray:
-    """World boxes of instances: 8 corners of the local box through the 4x4
-    transform, then component-wise min/max (model.rs:194-221).  float32 (M, 6)."""
-    m = transforms.shape[0]
-    lo = mesh_min.astype(np.float32)
-    hi = mesh_max.astype(np.float32)
-    corners = np.empty((m, 8, 4), dtype=np.float32)
-    for k in range(8):
-        sel = [(k >> 2) & 1, (k >> 1) & 1, k & 1]
-        for a in range(3):
-            corners[:, k, a] = np.where(sel[a], hi[:, a], lo[:, a])
-        corners[:, k, 3] = 1.0
-    T = transforms.astype(np.float32)
-    world = np.einsum("mij,mkj->mki", T, corners)[:, :, :3]
-    out = np.concatenate([world.min(axis=1), world.max(axis=1)], axis=1)
-    return _canon(out.astype(np.float32))
+def random_transforms(rng, m: int, scale=(0.5, 2.0), translate=100.0) -> np.ndarray:
+    """(m,4,4) float32 object-to-world: uniform random rotation (unit
+    quaternion), uniform scale, uniform translation in [-translate, translate]^3."""
+    R = _random_rotations(rng, m)
+    s = rng.uniform(scale[0], scale[1], size=m)
+    t = rng.uniform(-translate, translate, size=(m, 3))
+    T = np.zeros((m, 4, 4))
+    T[:, :3, :3] = R * s[:, None, None]
+    T[:, :3, 3] = t
+    T[:, 3, 3] = 1.0
+    return T.astype(np.float32)
 
 
 def batched_scene(n_meshes: int = 2000, n_instances: int = 10000, lo: int = 100, hi: int = 200_000,
                   seed: int = 3):
-    """Meshes with log-uniform triangle counts (soup or heightfield), instances and
-    their world boxes.  Returns dict with lists `positions`, `tris`, `kinds` and
-    arrays `mesh_min`, `mesh_max` (per mesh), `transforms` (M,4,4 float32),
-    `instance_mesh` (M,), `instance_boxes` (M,6)."""
+    """Meshes with log-uniform triangle counts (soup or heightfield) and
+    instances of them.  Returns dict with lists `positions`, `tris`, `kinds`,
+    `transforms` (M,4,4 float32 object-to-world) and `instance_mesh` (M,)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     counts = np.exp(rng.uniform(np.log(lo), np.log(hi), size=n_meshes)).astype(np.int64)
     kinds = rng.integers(0, 2, size=n_meshes)  # 0 soup, 1 heightfield
     positions, tris = [], []
-    mins = np.empty((n_meshes, 3), dtype=np.float32)
-    maxs = np.empty((n_meshes, 3), dtype=np.float32)
     for i in range(n_meshes):
         n = int(counts[i])
         if kinds[i] == 0:
@@ -161,25 +151,11 @@ def batched_scene(n_meshes: int = 2000, n_instances: int = 10000, lo: int = 100,
             p, t = heightfield(w, h, seed=20_000 + i)
         positions.append(p)
         tris.append(t)
-        mins[i] = p.min(axis=0)
-        maxs[i] = p.max(axis=0)
     which = rng.integers(0, n_meshes, size=n_instances)
-    R = _random_rotations(rng, n_instances)
-    scale = rng.uniform(0.5, 2.0, size=n_instances)
-    trans = rng.uniform(-100.0, 100.0, size=(n_instances, 3))
-    T = np.zeros((n_instances, 4, 4))
-    T[:, :3, :3] = R * scale[:, None, None]
-    T[:, :3, 3] = trans
-    T[:, 3, 3] = 1.0
-    T32 = T.astype(np.float32)
-    boxes = instance_boxes(mins[which], maxs[which], T32)
     return {
         "positions": positions,
         "tris": tris,
         "kinds": kinds,
-        "mesh_min": mins,
-        "mesh_max": maxs,
-        "transforms": T32,
+        "transforms": random_transforms(rng, n_instances),
         "instance_mesh": which.astype(np.uint32),
-        "instance_boxes": boxes,
     }

@@ -195,3 +195,28 @@ def closest_hit_scene_brute(pos, tris, mesh3, inst16, rays):
     lib().or_closest_hit_scene_brute(_p(pos), pos.shape[1], _p(tris), _p(mesh3), _p(inst16),
                                      inst16.shape[0], _p(rays), n, _p(t), _p(prim), _p(inst))
     return t, prim, inst
+
+
+def instance_boxes(mesh_min, mesh_max, o2w12):
+    """World boxes of instances as model.rs:189-221 computes them: the 8
+    corners of the mesh box through the transform (each row
+    ((m0*x + m1*y) + m2*z) + m3 in float32, no FMA), then min and max.
+    o2w12 = (M, 12) rows of the 3x4 object-to-world.  Returns (M, 6) float32."""
+    lo = np.asarray(mesh_min, dtype=np.float32)
+    hi = np.asarray(mesh_max, dtype=np.float32)
+    m = np.asarray(o2w12, dtype=np.float32).reshape(-1, 3, 4)
+    out = np.empty((m.shape[0], 6), dtype=np.float32)
+    out[:, :3] = np.inf
+    out[:, 3:] = -np.inf
+    for k in range(8):
+        x = hi[:, 0] if k & 4 else lo[:, 0]
+        y = hi[:, 1] if k & 2 else lo[:, 1]
+        z = hi[:, 2] if k & 1 else lo[:, 2]
+        for a in range(3):
+            w = m[:, a, 0] * x
+            w = w + m[:, a, 1] * y
+            w = w + m[:, a, 2] * z
+            w = w + m[:, a, 3]
+            out[:, a] = np.minimum(out[:, a], w)
+            out[:, 3 + a] = np.maximum(out[:, 3 + a], w)
+    return out

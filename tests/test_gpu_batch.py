@@ -151,3 +151,35 @@ def test_batch_errors(ctx):
     assert e.value.code == N.ERR_INDEX
     for b in (d_pos, d_tris, d_nodes, d_lens, d_perm):
         b.free()
+
+
+def test_scene_plain_and_batched_identical(ctx):
+    # a 60-mesh scene with 300 instances: the batched build, the GPU instance
+    # boxes and the TLAS equal the per-mesh plain builds and the oracle's boxes
+    print("[case] 60-mesh scene plain vs batched", flush=True)
+    from marigold_amd.scene import GpuScene
+    sc = scenes.batched_scene(n_meshes=60, n_instances=300, lo=20, hi=20_000, seed=11)
+    meshes = list(zip(sc["positions"], sc["tris"]))
+    a = GpuScene(ctx, meshes, sc["transforms"], sc["instance_mesh"], mode="plain")
+    b = GpuScene(ctx, meshes, sc["transforms"], sc["instance_mesh"], mode="batched")
+    for name, dt, cnt in (("blas_nodes", np.uint32, 8 * a.n_slots), ("blas_lens", np.uint32, a.n_slots),
+                          ("blas_perm", np.uint32, a.n_tris), ("tris_perm", np.uint32, 3 * a.n_tris),
+                          ("boxes", np.float32, 6 * a.n_instances),
+                          ("tlas_nodes", np.uint32, 8 * a.n_tlas_nodes),
+                          ("tlas_perm", np.uint32, a.n_instances)):
+        x, y = a.download(name, dt, cnt), b.download(name, dt, cnt)
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), name
+    assert np.array_equal(a.blas_node_counts, b.blas_node_counts)
+    assert a.n_tlas_nodes == b.n_tlas_nodes == 2 * a.n_instances - 1
+    # instance boxes against the oracle (value equality: -0 and +0 may differ)
+    nodes = a.download("blas_nodes", N.NODE_DTYPE, a.n_slots)
+    roots = nodes[a.node_offsets]
+    want = O.instance_boxes(roots["min"][sc["instance_mesh"]], roots["max"][sc["instance_mesh"]], a.o2w)
+    got = a.download("boxes", np.float32, 6 * a.n_instances).reshape(-1, 6)
+    assert np.array_equal(got, want)
+    # the TLAS against the oracle over those boxes
+    ref = O.build(want)
+    tl = a.download("tlas_nodes", np.uint32, 8 * a.n_tlas_nodes).reshape(-1, 8)
+    assert np.array_equal(tl, O.encode(ref["nodes"]))
+    a.free()
+    b.free()

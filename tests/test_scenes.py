@@ -2,6 +2,7 @@
 import numpy as np
 
 from marigold_amd import scenes
+from oracle import oracle as O
 
 
 def _clean(pos):
@@ -33,11 +34,27 @@ def test_heightfield_counts_and_range():
     assert np.array_equal(p, p2)
 
 
+def test_instance_boxes_known_answers():
+    # translation only: the box moves; a quarter turn about y swaps x and z
+    lo = np.array([[0, 0, 0], [1, 2, 3]], np.float32)
+    hi = np.array([[1, 2, 3], [2, 4, 6]], np.float32)
+    T = np.zeros((2, 3, 4), np.float32)
+    T[0] = [[1, 0, 0, 5], [0, 1, 0, -1], [0, 0, 1, 0.5]]
+    T[1] = [[0, 0, 1, 0], [0, 1, 0, 0], [-1, 0, 0, 0]]  # x' = z, z' = -x
+    b = O.instance_boxes(lo, hi, T.reshape(2, 12))
+    assert np.array_equal(b[0], np.float32([5, -1, 0.5, 6, 1, 3.5]))
+    assert np.array_equal(b[1], np.float32([3, 2, -2, 6, 4, -1]))
+
+
 def test_instance_boxes_contain_transformed_corners():
     rng = np.random.default_rng(0)
-    lo = rng.random((5, 3)).astype(np.float32)
-    hi = lo + np.float32(1.0)
-    T = np.tile(np.eye(4, dtype=np.float32), (5, 1, 1))
-    T[:, :3, 3] = rng.random((5, 3)) * 10
-    b = scenes.instance_boxes(lo, hi, T)
-    assert np.allclose(b[:, :3], lo + T[:, :3, 3]) and np.allclose(b[:, 3:], hi + T[:, :3, 3])
+    lo = rng.random((50, 3)).astype(np.float32)
+    hi = lo + rng.random((50, 3)).astype(np.float32)
+    T = scenes.random_transforms(rng, 50)
+    b = O.instance_boxes(lo, hi, T[:, :3, :].reshape(50, 12))
+    for k in range(8):
+        c = np.where([(k >> 2) & 1, (k >> 1) & 1, k & 1], hi, lo).astype(np.float64)
+        w = np.einsum("mij,mj->mi", T[:, :3, :3].astype(np.float64), c) + T[:, :3, 3]
+        assert np.all(w >= b[:, :3] - 1e-4) and np.all(w <= b[:, 3:] + 1e-4)
+        # every box face is touched by some corner
+    assert np.all(b[:, :3] <= b[:, 3:])
