@@ -71,7 +71,8 @@ typedef struct mgbvh_split_rec {
 } mgbvh_split_rec;
 
 #define MGBVH_METHOD_PLAIN 0   /* the reference algorithm, bit-exact */
-#define MGBVH_METHOD_LEARNED 1 /* learned split-window proposer + exact guard */
+#define MGBVH_METHOD_LEARNED 1 /* learned split-window proposer + guard (needs a model,
+                                  mgbvh_ctx_set_model) */
 
 /* BvhSettings (bvh/src/lib.rs:668-674) plus the build method. */
 typedef struct mgbvh_settings {
@@ -85,7 +86,17 @@ typedef struct mgbvh_settings {
   uint32_t min_leaf_objects;            /* const generic MIN_LEAF_OBJECTS (must be 1) */
   uint32_t max_leaf_objects;            /* const generic MAX_LEAF_OBJECTS (must be 1) */
   uint32_t method;                      /* MGBVH_METHOD_* */
-  uint32_t reserved[7];
+  /* MGBVH_METHOD_LEARNED only (0 = default): nodes with at least
+   * learned_min_objects objects take the learned path; the window spans
+   * learned_window cuts either side of the predicted cut; a window whose best
+   * proxy SAH exceeds (1 + learned_guard) x the best proxy SAH over all cuts
+   * falls back to the plain binned sweep. */
+  uint32_t learned_min_objects;
+  uint32_t learned_window;
+  float learned_guard;
+  uint32_t learned_no_network;          /* ablation: centre each window on the best proxy
+                                           cut instead of the network's choice */
+  uint32_t reserved[3];
 } mgbvh_settings;
 
 /* BvhProfilingInfo (bvh/src/lib.rs:676-689) plus builder counters. */
@@ -107,7 +118,8 @@ typedef struct mgbvh_stats {
   uint64_t large_object_visits; /* sum of object counts of the nodes with > 32 objects
                                    (the binned levels read and move each such object once) */
   uint32_t large_nodes;       /* nodes with > 32 objects (binned sweeps) */
-  uint32_t reserved[2];
+  uint32_t n_window_splits;   /* learned mode: nodes split inside their predicted window */
+  uint32_t reserved[1];
 } mgbvh_stats;
 
 typedef struct mgbvh_ctx mgbvh_ctx;
@@ -305,6 +317,64 @@ int mgbvh_instance_boxes_device(mgbvh_ctx* ctx, const mgbvh_node32* d_blas_nodes
                                 const mgbvh_blas_desc* d_blas, uint32_t nblas,
                                 const float* d_object_to_world, const uint32_t* d_instance_blas,
                                 uint32_t ninstances, mgbvh_aabb* d_boxes);
+
+/* ---- learned split proposer ---------------------------------------------------
+ * No counterpart in the reference: the network of MGBVH_METHOD_LEARNED, a
+ * 102 -> 128 -> 128 -> 96 MLP (ReLU) from a node's centroid-bin counts and
+ * extents to split logits (axis * 32 + cut).  Features of a node with n > 32
+ * objects: log1p(count) / log1p(n) of the 3 x 32 centroid bins (axis-major),
+ * node extent / largest node extent (3), centroid extent / node extent (3).
+ * Parameters (floats): W1[128][128] (inputs 102..127 zero), b1[128],
+ * W2[128][128], b2[128], W3[96][128], b3[96].  Training runs in FP32 on the
+ * device (softmax cross-entropy, Adam); inference in BF16 on MFMA with FP32
+ * accumulation. */
+#define MGBVH_MODEL_FEATURES 102
+#define MGBVH_MODEL_CLASSES 96
+#define MGBVH_MODEL_PARAMS 45408
+
+typedef struct mgbvh_model mgbvh_model;
+
+/* Kaiming-uniform weights from `seed`, zero biases. */
+int mgbvh_model_create(mgbvh_ctx* ctx, uint64_t seed, mgbvh_model** out);
+int mgbvh_model_destroy(mgbvh_model* model);
+int mgbvh_model_get_params(mgbvh_ctx* ctx, mgbvh_model* model, float* host_params);
+int mgbvh_model_set_params(mgbvh_ctx* ctx, mgbvh_model* model, const float* host_params);
+
+/* Split samples of a plain build: the features and the chosen binned split
+ * (axis * 32 + cut) of every node with >= min_objects objects split by the
+ * binned sweep.  d_features: max_samples x 102 floats. */
+int mgbvh_collect_samples_device(mgbvh_ctx* ctx, const float* d_positions, uint32_t vstride_bytes,
+                                 uint64_t nverts, const uint32_t* d_tris, uint64_t ntris,
+                                 const mgbvh_settings* settings, uint32_t min_objects,
+                                 float* d_features, uint32_t* d_labels, uint32_t max_samples,
+                                 uint32_t* out_count);
+
+/* Mean cross-entropy over n samples and its gradient (MGBVH_MODEL_PARAMS
+ * floats, parameter layout above) at the current parameters; no update. */
+int mgbvh_model_gradients_device(mgbvh_ctx* ctx, mgbvh_model* model, const float* d_features,
+                                 const uint32_t* d_labels, uint32_t n, float* d_grads,
+                                 float* out_loss);
+
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over shuffled minibatches; returns the
+ * last epoch's mean loss and accuracy. */
+int mgbvh_model_train_device(mgbvh_ctx* ctx, mgbvh_model* model, const float* d_features,
+                             const uint32_t* d_labels, uint32_t n, uint32_t epochs, uint32_t batch,
+                             float lr, uint64_t seed, float* out_loss, float* out_accuracy);
+
+/* The network MGBVH_METHOD_LEARNED builds on this context use (NULL: none). */
+int mgbvh_ctx_set_model(mgbvh_ctx* ctx, mgbvh_model* model);
+
+/* Logits (n x 96) by the BF16 MFMA path the learned builder uses. */
+int mgbvh_model_infer_device(mgbvh_ctx* ctx, mgbvh_model* model, const float* d_features,
+                             uint32_t n, float* d_logits);
+
+/* Surface-area cost of each BLAS with the reference's constants
+ * (bvh/src/lib.rs:217-218): sum over internal nodes of SA(n) / SA(root) * 1
+ * plus sum over leaves of SA(l) / SA(root) * 8 * len(l), in double.  blas and
+ * node_counts are HOST arrays (node_offset of each BLAS and its node count). */
+int mgbvh_tree_sah_device(mgbvh_ctx* ctx, const mgbvh_node32* d_nodes, const uint32_t* d_lens,
+                          const mgbvh_blas_desc* blas, const uint32_t* node_counts, uint32_t nblas,
+                          double* out_sah);
 
 /* ---- timing helpers: HIP events recorded on the context stream ---- */
 int mgbvh_event_record(mgbvh_ctx* ctx, int slot);           /* slot 0..59 */

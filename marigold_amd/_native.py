@@ -72,7 +72,11 @@ class Settings(C.Structure):
         ("min_leaf_objects", C.c_uint32),
         ("max_leaf_objects", C.c_uint32),
         ("method", C.c_uint32),
-        ("reserved", C.c_uint32 * 7),
+        ("learned_min_objects", C.c_uint32),
+        ("learned_window", C.c_uint32),
+        ("learned_guard", C.c_float),
+        ("learned_no_network", C.c_uint32),
+        ("reserved", C.c_uint32 * 3),
     ]
 
 
@@ -94,7 +98,8 @@ class Stats(C.Structure):
         ("n_guard_fallbacks", C.c_uint32),
         ("large_object_visits", C.c_uint64),
         ("large_nodes", C.c_uint32),
-        ("reserved", C.c_uint32 * 2),
+        ("n_window_splits", C.c_uint32),
+        ("reserved", C.c_uint32 * 1),
     ]
 
     def as_dict(self):
@@ -164,6 +169,18 @@ _EXTRA: dict = {
                                   _P, _U64, _P], _I),
     "mgbvh_gen_primary_rays_device": ([_P, C.POINTER(Camera), _U32, _U32, _P], _I),
     "mgbvh_instance_boxes_device": ([_P, _P, _P, _U32, _P, _P, _U32, _P], _I),
+    "mgbvh_model_create": ([_P, _U64, C.POINTER(_P)], _I),
+    "mgbvh_model_destroy": ([_P], _I),
+    "mgbvh_model_get_params": ([_P, _P, _P], _I),
+    "mgbvh_model_set_params": ([_P, _P, _P], _I),
+    "mgbvh_collect_samples_device": ([_P, _P, _U32, _U64, _P, _U64, C.POINTER(Settings), _U32,
+                                      _P, _P, _U32, C.POINTER(_U32)], _I),
+    "mgbvh_model_gradients_device": ([_P, _P, _P, _P, _U32, _P, C.POINTER(C.c_float)], _I),
+    "mgbvh_model_train_device": ([_P, _P, _P, _P, _U32, _U32, _U32, C.c_float, _U64,
+                                  C.POINTER(C.c_float), C.POINTER(C.c_float)], _I),
+    "mgbvh_model_infer_device": ([_P, _P, _P, _U32, _P], _I),
+    "mgbvh_ctx_set_model": ([_P, _P], _I),
+    "mgbvh_tree_sah_device": ([_P, _P, _P, _P, _P, _U32, _P], _I),
     "mgbvh_build_blas_batch_device": ([_P, _P, _U32, _U64, _P, _P, _U32, _P, C.POINTER(Settings),
                                        _P, _P, _P, _P, _P, _P, _P, C.POINTER(Stats)], _I),
     "mgbvh_gen_bounce_rays_device": ([_P, _P, _P, _U64, _P, _U32, _P, _U32, _P, _U32, _P, _U32,
@@ -201,11 +218,17 @@ def ptr(a: np.ndarray | None):
 
 
 def default_settings(max_depth: int = 32, bounds=None, method: int = METHOD_PLAIN, name: str = "",
-                     profiling_info: bool = False, profiling_info_directory: str | None = None):
+                     profiling_info: bool = False, profiling_info_directory: str | None = None,
+                     learned_min_objects: int = 0, learned_window: int = 0,
+                     learned_guard: float = 0.0, learned_no_network: bool = False):
     s = Settings()
     lib().mgbvh_settings_default(C.byref(s))
     s.max_depth = max_depth
     s.method = method
+    s.learned_min_objects = learned_min_objects
+    s.learned_window = learned_window
+    s.learned_guard = learned_guard
+    s.learned_no_network = 1 if learned_no_network else 0
     s._name = name.encode()  # keep the bytes alive
     s.name = s._name
     s.profiling_info = 1 if profiling_info else 0
@@ -252,6 +275,54 @@ class DeviceBuffer:
         self.ptr = None
 
 
+MODEL_FEATURES = 102
+MODEL_CLASSES = 96
+MODEL_PARAMS = 45408
+
+
+class Model:
+    """The learned build method's split proposer (device parameters)."""
+
+    def __init__(self, ctx: "Context", seed: int = 1):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib().mgbvh_model_create(ctx.handle, seed, C.byref(h)))
+        self.handle = h
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(MODEL_PARAMS, dtype=np.float32)
+        check(lib().mgbvh_model_get_params(self.ctx.handle, self.handle, ptr(p)))
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.shape == (MODEL_PARAMS,)
+        check(lib().mgbvh_model_set_params(self.ctx.handle, self.handle, ptr(p)))
+
+    def gradients(self, feats, labels, n: int, grads=None):
+        loss = C.c_float()
+        check(lib().mgbvh_model_gradients_device(self.ctx.handle, self.handle, _dv(feats),
+                                                 _dv(labels), n, _dv(grads), C.byref(loss)))
+        return loss.value
+
+    def train(self, feats, labels, n: int, epochs: int, batch: int = 256, lr: float = 1e-3,
+              seed: int = 1):
+        loss, acc = C.c_float(), C.c_float()
+        check(lib().mgbvh_model_train_device(self.ctx.handle, self.handle, _dv(feats), _dv(labels),
+                                             n, epochs, batch, lr, seed, C.byref(loss),
+                                             C.byref(acc)))
+        return loss.value, acc.value
+
+    def infer(self, feats, n: int, logits):
+        check(lib().mgbvh_model_infer_device(self.ctx.handle, self.handle, _dv(feats), n,
+                                             _dv(logits)))
+
+    def close(self):
+        if self.handle is not None:
+            lib().mgbvh_model_destroy(self.handle)
+            self.handle = None
+
+
 def _dv(b):
     """Device pointer of a DeviceBuffer (None -> NULL)."""
     return None if b is None else b.ptr
@@ -287,7 +358,10 @@ class Context:
         return float(ms.value)
 
     KERNEL_KINDS = ["setup", "root", "bin", "sweep", "partition", "small", "compact", "permute",
-                    "trace"]
+                    "trace", "learn"]
+
+    def set_model(self, model):
+        check(lib().mgbvh_ctx_set_model(self.handle, None if model is None else model.handle))
 
     def set_profiling(self, kinds):
         mask = 0
@@ -382,6 +456,22 @@ class Context:
             at(perm, tri_offset, 4), at(tris_out, tri_offset, 12), at(splits, node_offset, 16),
             C.byref(nn), None))
         return nn.value
+
+    def tree_sah(self, nodes, lens, blas: np.ndarray, node_counts: np.ndarray) -> np.ndarray:
+        blas = np.ascontiguousarray(blas, dtype=BLAS_DTYPE)
+        counts = np.ascontiguousarray(node_counts, dtype=np.uint32)
+        out = np.zeros(max(len(blas), 1), dtype=np.float64)
+        check(lib().mgbvh_tree_sah_device(self.handle, _dv(nodes), _dv(lens), ptr(blas),
+                                          ptr(counts), len(blas), ptr(out)))
+        return out[:len(blas)]
+
+    def collect_samples(self, pos, vstride, nverts, tris, n, settings: Settings, min_objects,
+                        feats, labels, max_samples) -> int:
+        cnt = C.c_uint32()
+        check(lib().mgbvh_collect_samples_device(self.handle, _dv(pos), vstride, nverts, _dv(tris),
+                                                 n, C.byref(settings), min_objects, _dv(feats),
+                                                 _dv(labels), max_samples, C.byref(cnt)))
+        return cnt.value
 
     # ---- host-buffer builds -------------------------------------------------
     def build_tris(self, positions: np.ndarray, tris: np.ndarray, settings: Settings,

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmgbvh.so")
 
-SOURCES = ["mgbvh_build.hip", "mgbvh_api.hip", "mgbvh_trace.hip", "mgbvh_scene.hip"]
+SOURCES = ["mgbvh_build.hip", "mgbvh_api.hip", "mgbvh_trace.hip", "mgbvh_scene.hip", "mgbvh_learned.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -80,14 +80,23 @@ static void free_buf(DevBuf& b) {
   b.cap = 0;
 }
 
-static int check_settings(const mgbvh_settings* s) {
+static int check_settings(const mgbvh_settings* s, const mgbvh_ctx* c) {
   if (!s) { set_error("settings is NULL"); return MGBVH_ERR_INVALID_ARG; }
   if (s->min_leaf_objects != 1 || s->max_leaf_objects != 1) {
     set_error("only MIN_LEAF_OBJECTS = MAX_LEAF_OBJECTS = 1 is built on the GPU");
     return MGBVH_ERR_UNSUPPORTED;
   }
-  if (s->method != MGBVH_METHOD_PLAIN) {
-    set_error("unknown build method for this entry point");
+  if (s->method == MGBVH_METHOD_LEARNED) {
+    if (!c || !c->model) {
+      set_error("the learned method needs a model (mgbvh_ctx_set_model)");
+      return MGBVH_ERR_UNSUPPORTED;
+    }
+    if (!(s->learned_guard >= 0.0f) || s->learned_window > 31) {
+      set_error("learned_guard must be >= 0 and learned_window <= 31");
+      return MGBVH_ERR_INVALID_ARG;
+    }
+  } else if (s->method != MGBVH_METHOD_PLAIN) {
+    set_error("unknown build method");
     return MGBVH_ERR_UNSUPPORTED;
   }
   return MGBVH_OK;
@@ -159,6 +168,7 @@ static void fill_stats(const std::vector<mgbvh_node32>& nodes, const std::vector
   st->n_guard_fallbacks = r.n_guard;
   st->large_object_visits = r.large_visits;
   st->large_nodes = (uint32_t)r.large_nodes;
+  st->n_window_splits = r.n_window;
 }
 
 // bvh/src/lib.rs:851-866: bvh_<name>_<9-digit id>.json
@@ -227,6 +237,12 @@ void mgbvh_settings_default(mgbvh_settings* s) {
 
 const char* mgbvh_last_error(void) { return g_err.c_str(); }
 
+int mgbvh_ctx_set_model(mgbvh_ctx* c, mgbvh_model* model) {
+  if (!c) { set_error("ctx is NULL"); return MGBVH_ERR_INVALID_ARG; }
+  c->model = model;
+  return MGBVH_OK;
+}
+
 int mgbvh_ctx_create(int device, mgbvh_ctx** out) {
   if (!out) { set_error("out is NULL"); return MGBVH_ERR_INVALID_ARG; }
   *out = nullptr;
@@ -292,7 +308,7 @@ int mgbvh_build_tris_device(mgbvh_ctx* c, const float* d_pos, uint32_t vstride_b
                             uint32_t* d_perm, uint32_t* d_tris_out, mgbvh_split_rec* d_splits,
                             uint64_t* out_nnodes, mgbvh_stats* out_stats) {
   if (!c) { set_error("ctx is NULL"); return MGBVH_ERR_INVALID_ARG; }
-  int rc = check_settings(s);
+  int rc = check_settings(s, c);
   if (rc) return rc;
   if (ntris > 0x7fffffffull) { set_error("more than 2^31-1 objects"); return MGBVH_ERR_INVALID_ARG; }
   if (vstride_bytes < 12 || vstride_bytes % 4) {
@@ -335,7 +351,7 @@ int mgbvh_build_blas_batch_device(mgbvh_ctx* c, const float* d_pos, uint32_t vst
                                   mgbvh_blas_desc* out_blas, uint32_t* out_node_counts,
                                   mgbvh_stats* out_stats) {
   if (!c) { set_error("ctx is NULL"); return MGBVH_ERR_INVALID_ARG; }
-  int rc = check_settings(s);
+  int rc = check_settings(s, c);
   if (rc) return rc;
   if (s->has_bounds || s->profiling_info) {
     set_error("batched builds take per-mesh bounds (mesh_bounds) and write no profiling JSON");
@@ -402,6 +418,8 @@ int mgbvh_build_blas_batch_device(mgbvh_ctx* c, const float* d_pos, uint32_t vst
     out_stats->n_cutoff_leaves = r.n_cutoff;
     out_stats->large_object_visits = r.large_visits;
     out_stats->large_nodes = (uint32_t)r.large_nodes;
+    out_stats->n_guard_fallbacks = r.n_guard;
+    out_stats->n_window_splits = r.n_window;
   }
   return MGBVH_OK;
 }
@@ -411,7 +429,7 @@ int mgbvh_build_aabbs_device(mgbvh_ctx* c, const mgbvh_aabb* d_aabbs, uint64_t n
                              uint32_t* d_perm, mgbvh_split_rec* d_splits, uint64_t* out_nnodes,
                              mgbvh_stats* out_stats) {
   if (!c) { set_error("ctx is NULL"); return MGBVH_ERR_INVALID_ARG; }
-  int rc = check_settings(s);
+  int rc = check_settings(s, c);
   if (rc) return rc;
   if (n > 0x7fffffffull) { set_error("more than 2^31-1 objects"); return MGBVH_ERR_INVALID_ARG; }
   if (n && (!d_aabbs || !d_nodes || !d_perm)) {
@@ -447,7 +465,7 @@ int mgbvh_build_tris(mgbvh_ctx* c, const float* positions, uint32_t vstride_byte
                      mgbvh_node32* out_nodes, uint32_t* out_lens, uint64_t* out_nnodes,
                      uint32_t* out_perm, mgbvh_split_rec* out_splits, mgbvh_stats* out_stats) {
   if (!c) { set_error("ctx is NULL"); return MGBVH_ERR_INVALID_ARG; }
-  int rc = check_settings(s);
+  int rc = check_settings(s, c);
   if (rc) return rc;
   if (ntris == 0) {
     if (out_nnodes) *out_nnodes = 0;
@@ -523,7 +541,7 @@ int mgbvh_build_aabbs(mgbvh_ctx* c, const mgbvh_aabb* aabbs, uint64_t n,
                       uint64_t* out_nnodes, uint32_t* out_perm, mgbvh_split_rec* out_splits,
                       mgbvh_stats* out_stats) {
   if (!c) { set_error("ctx is NULL"); return MGBVH_ERR_INVALID_ARG; }
-  int rc = check_settings(s);
+  int rc = check_settings(s, c);
   if (rc) return rc;
   if (n == 0) {
     if (out_nnodes) *out_nnodes = 0;

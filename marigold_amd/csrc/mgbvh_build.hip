@@ -180,7 +180,7 @@ __global__ __launch_bounds__(256) void k_setup_aabbs(const float* __restrict__ b
 // max_depth == 0) goes straight to the small-subtree kernel.  One wave per mesh.
 __global__ void k_root(const MeshRow* __restrict__ rows, const uint32_t* __restrict__ macc,
                        Counters* ctr, LNode* L, uint32_t* cnode, uint32_t* ck, SNode* small,
-                       uint32_t n_small_roots) {
+                       uint32_t n_small_roots, uint32_t learn_min) {
   const uint32_t m = blockIdx.x;
   if (m == 0 && threadIdx.x == 0) ctr->n_small = n_small_roots;
   const MeshRow mr = rows[m];
@@ -203,6 +203,7 @@ __global__ void k_root(const MeshRow* __restrict__ rows, const uint32_t* __restr
       }
       r.child_slot[0] = r.child_slot[1] = -1;
       r.nbase = mr.nbase; r.obase = mr.tri_offset;
+      r.mode = (learn_min && mr.count >= learn_min) ? LM_COUNTED : LM_PLAIN;
       L[mr.root_slot] = r;
     } else {
       SNode sn = {};
@@ -225,17 +226,22 @@ __global__ void k_root(const MeshRow* __restrict__ rows, const uint32_t* __restr
 // One block per chunk of <= CH objects of one node: 3 x 32 bins of
 // {count, bin box} in LDS, flushed into the node histogram; per-chunk counts
 // are kept for the stable partition.
+// Learned method: pass 1 counts all three axes of a counted node (no bin
+// boxes); pass 2 adds the bin boxes of its predicted axis (window) or of all
+// axes (fallback).  Plain nodes are binned completely in pass 1.
 __global__ __launch_bounds__(256) void k_bin(const LNode* __restrict__ L,
                                              const uint32_t* __restrict__ cnode,
                                              const uint32_t* __restrict__ ck,
                                              const Rec* __restrict__ recs,
                                              uint32_t* __restrict__ hist,
-                                             uint32_t* __restrict__ blk) {
+                                             uint32_t* __restrict__ blk, int pass) {
   __shared__ uint32_t h[HIST_WORDS];
   const uint32_t c = blockIdx.x;
   const uint32_t slot = cnode[c];
   const uint32_t k = ck[c];
   const LNode& nd = L[slot];
+  const uint32_t lm = nd.mode, lk = lmode_kind(lm), la = lmode_axis(lm);
+  if (pass == 2 && lk < LM_WINDOW) return;  // block-uniform: nothing left to bin
   for (uint32_t t = threadIdx.x; t < HIST_WORDS; t += blockDim.x) h[t] = 0;
   float cmin[3], cext[3];
 #pragma unroll
@@ -253,14 +259,18 @@ __global__ __launch_bounds__(256) void k_bin(const LNode* __restrict__ L,
     rec_load(&recs[i], b, id);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
+      const bool boxes = pass == 1 ? lk == LM_PLAIN : (lk == LM_FALLBACK || a == (int)la);
+      if (pass == 2 && !boxes) continue;
       float cc = center(b.mn[a], b.mx[a]);
       uint32_t bi = bin_index(cc, cmin[a], cext[a]);
       uint32_t* hb = &h[(a * MG_NBINS + bi) * BIN_WORDS];
-      atomicAdd(&hb[0], 1u);
+      if (pass == 1) atomicAdd(&hb[0], 1u);
+      if (boxes) {
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        atomicMax(&hb[1 + j], acc_min_enc(b.mn[j]));
-        atomicMax(&hb[4 + j], acc_max_enc(b.mx[j]));
+        for (int j = 0; j < 3; ++j) {
+          atomicMax(&hb[1 + j], acc_min_enc(b.mn[j]));
+          atomicMax(&hb[4 + j], acc_max_enc(b.mx[j]));
+        }
       }
     }
   }
@@ -270,8 +280,10 @@ __global__ __launch_bounds__(256) void k_bin(const LNode* __restrict__ L,
     uint32_t v = h[t];
     uint32_t f = t % BIN_WORDS;
     if (f == 0) {
-      blk[(size_t)c * 96 + t / BIN_WORDS] = v;
-      if (v) atomicAdd(&gh[t], v);
+      if (pass == 1) {
+        blk[(size_t)c * 96 + t / BIN_WORDS] = v;
+        if (v) atomicAdd(&gh[t], v);
+      }
     } else if (v) {
       atomicMax(&gh[t], v);
     }
@@ -311,7 +323,8 @@ __device__ inline void eval_cut(const uint32_t* h, int a, uint32_t i, Box& lt, B
 __device__ inline void put_child(bool large, uint32_t s, uint32_t c0, uint32_t start,
                                  uint32_t count, const Box& b, uint32_t idx, uint32_t P,
                                  uint32_t depth, uint32_t parity_next, uint32_t nbase,
-                                 uint32_t obase, LNode* Ln, SNode* small) {
+                                 uint32_t obase, LNode* Ln, SNode* small, uint32_t learn_min,
+                                 Counters* ctr) {
   if (large) {
     LNode r = {};
     r.start = start; r.count = count; r.idx = idx; r.P = P; r.depth = depth;
@@ -319,6 +332,8 @@ __device__ inline void put_child(bool large, uint32_t s, uint32_t c0, uint32_t s
     for (int a = 0; a < 3; ++a) { r.bmn[a] = b.mn[a]; r.bmx[a] = b.mx[a]; }
     r.child_slot[0] = r.child_slot[1] = -1;
     r.nbase = nbase; r.obase = obase;
+    r.mode = (learn_min && count >= learn_min) ? LM_COUNTED : LM_PLAIN;
+    if (r.mode == LM_COUNTED) atomicAdd(&ctr->n_learned_next, 1u);
     Ln[s] = r;
   } else {
     SNode sn = {};
@@ -341,7 +356,7 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
                                                 uint32_t* cn_next, uint32_t* ck_next,
                                                 SNode* small, Counters* ctr,
                                                 NodeOut out, uint32_t max_depth,
-                                                uint32_t parity_next) {
+                                                uint32_t parity_next, uint32_t learn_min) {
   __shared__ uint32_t sh[SW_WAVES][HIST_WORDS];
   __shared__ uint32_t s_need[SW_WAVES][4];  // large slots, chunks, small slots, objects
   __shared__ uint32_t s_base[SW_WAVES][3];
@@ -368,6 +383,11 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
     cext[a] = __fsub_rn(acc_max_dec(nd.cacc[3 + a]), cmin[a]);
   }
   const float sa_p = box_sa(nb);
+  // learned method: a window node only considers its window's cuts
+  const uint32_t lm = nd.mode;
+  const bool win = lmode_kind(lm) == LM_WINDOW;
+  const int wa = (int)lmode_axis(lm);
+  const uint32_t wlo = lmode_lo(lm), whi = lmode_hi(lm);
   // Candidates (axis a, cut i), lane = a * 32 + bin: lt = bins [0, i), gt = bins
   // [i, 32) from 32-lane inclusive prefix / suffix scans (exact min/max unions).
   uint64_t best = ~0ull;
@@ -411,7 +431,8 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
     }
     const uint32_t cng = sc;
     // lib.rs:339-341 (zero extent refused), 379-381 (MIN objects per side)
-    if (qact && b >= 1 && cext[a] != 0.0f && cnl >= 1 && cng >= 1) {
+    const bool inwin = !win || (a == wa && b >= wlo && b <= whi);
+    if (qact && inwin && b >= 1 && cext[a] != 0.0f && cnl >= 1 && cng >= 1) {
       const float ccost = split_cost(sa_p, box_sa(clt), box_sa(sb), cnl, cng);
       const uint64_t key = ((uint64_t)tc_ukey(ccost) << 32) | q;
       best = key < best ? key : best;
@@ -462,6 +483,21 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
     cost = split_cost(sa_p, box_sa(lt), box_sa(gt), nl, ng);
   }
 
+  // training sample of the split proposer: this node's features and choice
+  if (out.samples && kind == 0 && nd.count >= out.sample_min) {
+    uint32_t si = 0;
+    if (lane == 0) si = atomicAdd(&ctr->n_samples, 1u);
+    si = __shfl(si, 0, 64);
+    if (si < out.max_samples) {
+      float* f = out.samples + (size_t)si * MLP_IN;
+      const float inv = 1.0f / log1pf((float)nd.count);
+      for (uint32_t q = lane; q < 3 * MG_NBINS; q += 64) f[q] = log1pf((float)h[q * BIN_WORDS]) * inv;
+      if (lane == 0) {
+        node_extent_features(nb.mn, nb.mx, cext, f + 3 * MG_NBINS);
+        out.labels[si] = axis * MG_NBINS + value;
+      }
+    }
+  }
   }  // active
 
   // children (lib.rs:648-663): gt at P, lt at P + 1
@@ -503,9 +539,9 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
     out.write_split(nd.idx, axis | (kind << 8), value, nl, fbits(cost));
     if (kind == 2) atomicAdd(&ctr->n_median, 1u);
     put_child(lg, sgs, c0g, nd.start + nl, ng, gt, nd.P, nd.P + 2, cd, parity_next, nd.nbase,
-              nd.obase, Ln, small);
+              nd.obase, Ln, small, learn_min, ctr);
     put_child(ll, sls, c0l, nd.start, nl, lt, nd.P + 1, nd.P + 2 + 2 * (ng - 1), cd, parity_next,
-              nd.nbase, nd.obase, Ln, small);
+              nd.nbase, nd.obase, Ln, small, learn_min, ctr);
     L[slot].axis = axis;
     L[slot].cut = value;
     L[slot].n_lt = nl;
@@ -1094,6 +1130,10 @@ int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& m
   MeshRow* drows = (MeshRow*)c->mrows.p;
   uint32_t* macc = (uint32_t*)c->macc.p;
   NodeOut out{o.nodes, lens, o.splits};
+  out.samples = o.samples;
+  out.labels = o.labels;
+  out.max_samples = o.max_samples;
+  out.sample_min = o.sample_min;
 
   MG_HIP(hipEventRecord(c->ev[62], st));
   MG_HIP(hipMemcpyAsync(drows, rows.data(), (size_t)nm * sizeof(MeshRow), hipMemcpyHostToDevice,
@@ -1110,8 +1150,20 @@ int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& m
     k_setup_aabbs<<<(uint32_t)setup_chunks, 256, 0, st>>>(in.boxes, drows, nm, recs[0], macc, ctr);
   MG_HIP(hipGetLastError());
   MG_PROF(KK_SETUP, 1);
+  LearnParams lp{0, 0, 0.0f, 0};
+  uint32_t n_counted = 0;
+  if (s->method == MGBVH_METHOD_LEARNED) {
+    if (!c->model) { set_error("the learned method needs a model"); return MGBVH_ERR_UNSUPPORTED; }
+    lp.min_objects = s->learned_min_objects ? std::max<uint32_t>(s->learned_min_objects, MG_SMALL + 1)
+                                            : 512u;
+    lp.window = s->learned_window ? s->learned_window : 3u;
+    lp.guard = s->learned_guard;
+    lp.no_network = s->learned_no_network;
+    for (uint32_t i = 0; i < nm; ++i)
+      if (rows[i].root_kind == 1 && rows[i].count >= lp.min_objects) ++n_counted;
+  }
   MG_PROF(KK_ROOT, 0);
-  k_root<<<nm, 64, 0, st>>>(drows, macc, ctr, L[0], cn[0], ck[0], small, n_small);
+  k_root<<<nm, 64, 0, st>>>(drows, macc, ctr, L[0], cn[0], ck[0], small, n_small, lp.min_objects);
   MG_PROF(KK_ROOT, 1);
   MG_HIP(hipGetLastError());
 
@@ -1123,13 +1175,23 @@ int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& m
     large_nodes += nL;
     MG_HIP(hipMemsetAsync(hist, 0, (size_t)nL * HIST_WORDS * 4, st));
     MG_HIP(hipMemsetAsync(ctr, 0, 8, st));  // n_large_next, n_chunks_next
+    MG_HIP(hipMemsetAsync(&ctr->n_learned_next, 0, 8, st));  // and n_learned_cur
     MG_PROF(KK_BIN, 0);
-    k_bin<<<nC, 256, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk);
+    k_bin<<<nC, 256, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk, 1);
     MG_PROF(KK_BIN, 1);
+    if (n_counted) {
+      MG_PROF(KK_LEARN, 0);
+      if ((rc = learned_decide(c, L[cur], nL, hist, n_counted, ctr, lp))) return rc;
+      MG_PROF(KK_LEARN, 1);
+      MG_PROF(KK_BIN, 0);
+      k_bin<<<nC, 256, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk, 2);
+      MG_PROF(KK_BIN, 1);
+    }
     MG_PROF(KK_SWEEP, 0);
     k_sweep<<<cdiv(nL, SW_WAVES), 1024, 0, st>>>(L[cur], nL, hist, recs[cur], L[cur ^ 1],
                                                  cn[cur ^ 1], ck[cur ^ 1], small, ctr, out,
-                                                 s->max_depth, (uint32_t)(cur ^ 1));
+                                                 s->max_depth, (uint32_t)(cur ^ 1),
+                                                 lp.min_objects);
     MG_PROF(KK_SWEEP, 1);
     MG_PROF(KK_PARTITION, 0);
     k_partition<<<nC, 256, 0, st>>>(L[cur], cn[cur], ck[cur], blk, recs[cur], recs[cur ^ 1],
@@ -1141,6 +1203,7 @@ int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& m
     if (c->h_ctr->err) break;
     nL = c->h_ctr->n_large_next;
     nC = c->h_ctr->n_chunks_next;
+    n_counted = c->h_ctr->n_learned_next;
     cur ^= 1;
     ++levels;
     if (levels > 4096) { set_error("level loop did not terminate"); return MGBVH_ERR_INTERNAL; }
@@ -1223,6 +1286,9 @@ int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& m
   res->n_cutoff = hc.n_cutoff;
   res->large_nodes = large_nodes;
   res->large_visits = hc.sum_large;
+  res->n_samples = hc.n_samples;
+  res->n_window = hc.n_window;
+  res->n_guard = hc.n_fallback;
   res->ms = ms;
   return MGBVH_OK;
 }

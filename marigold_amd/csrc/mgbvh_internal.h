@@ -34,9 +34,30 @@ struct LNode {
   uint32_t depth, chunk0, nchunk, kind;
   float bmn[3], bmx[3];               // node bounds (SA of the parent in the cost)
   uint32_t cacc[6];                   // centroid bounds: ~okey(min) x3, okey(max) x3
-  uint32_t axis, cut, n_lt, pad0;     // chosen split
+  uint32_t axis, cut, n_lt, mode;     // chosen split; learned-method state (lmode_*)
   int32_t child_slot[2];              // next-level slot of (gt, lt) child, -1 if none
   uint32_t nbase, obase;
+};
+
+// Learned-method state of a node with > 32 objects (LNode::mode):
+// bits 0-1 kind (0 plain, 1 counted, 2 window, 3 fallback), bits 2-3 predicted
+// axis, bits 4-8 first and bits 9-13 last cut of the window.
+constexpr uint32_t LM_PLAIN = 0, LM_COUNTED = 1, LM_WINDOW = 2, LM_FALLBACK = 3;
+__host__ __device__ inline uint32_t lmode_pack(uint32_t kind, uint32_t axis, uint32_t lo,
+                                               uint32_t hi) {
+  return kind | (axis << 2) | (lo << 4) | (hi << 9);
+}
+__host__ __device__ inline uint32_t lmode_kind(uint32_t m) { return m & 3u; }
+__host__ __device__ inline uint32_t lmode_axis(uint32_t m) { return (m >> 2) & 3u; }
+__host__ __device__ inline uint32_t lmode_lo(uint32_t m) { return (m >> 4) & 31u; }
+__host__ __device__ inline uint32_t lmode_hi(uint32_t m) { return (m >> 9) & 31u; }
+
+// Parameters of the learned build method inside one build.
+struct LearnParams {
+  uint32_t min_objects;  // 0: method off
+  uint32_t window;
+  float guard;
+  uint32_t no_network;   // windows centred on the best proxy cut (ablation)
 };
 
 // A subtree root with <= 32 objects, or a leaf forced by max_depth.
@@ -64,13 +85,43 @@ struct Counters {
   uint32_t err, n_gaps, n_median, n_cutoff;
   uint32_t root_acc[12];  // bounds (~min, max) and centroid bounds (~min, max)
   unsigned long long sum_large;  // objects of all nodes split on the level path
-  uint32_t pad[10];
+  uint32_t n_samples;            // split samples written (may exceed the capacity)
+  uint32_t n_learned_next;       // counted (learned) nodes created for the next level
+  uint32_t n_learned_cur;        // rows of this level's network batch
+  uint32_t n_window, n_fallback; // learned nodes swept in a window / by the plain sweep
+  uint32_t pad[5];
 };
+
+// Split proposer network: 102 features -> 128 -> 128 (ReLU) -> 96 logits
+// (axis * 32 + cut).  Features of a node with > 32 objects: log1p(count) /
+// log1p(n) of its 3 x 32 centroid bins, node extents over the largest node
+// extent, centroid extents over node extents.
+constexpr int MLP_IN = 102, MLP_INP = 128, MLP_H = 128, MLP_OUT = 96;
+constexpr int MLP_NPARAM = MLP_H * MLP_INP + MLP_H + MLP_H * MLP_H + MLP_H + MLP_OUT * MLP_H + MLP_OUT;
+
+// Feature 96 + a and 99 + a of a node (the 96 count features come from its bins).
+__device__ inline void node_extent_features(const float* bmn, const float* bmx, const float* cext,
+                                            float* f6) {
+  float ext[3], emax = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    ext[a] = bmx[a] - bmn[a];
+    emax = fmaxf(emax, ext[a]);
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    f6[a] = emax > 0.0f ? ext[a] / emax : 0.0f;
+    f6[3 + a] = ext[a] > 0.0f ? fminf(cext[a] / ext[a], 1.0f) : 0.0f;
+  }
+}
 
 struct NodeOut {
   mgbvh_node32* nodes;
   uint32_t* lens;
   mgbvh_split_rec* splits;
+  float* samples = nullptr;      // split samples: MLP_IN features per binned split
+  uint32_t* labels = nullptr;    // axis * 32 + cut of each sample
+  uint32_t max_samples = 0, sample_min = 0;
   __device__ inline void write_node(uint32_t idx, const Box& b, uint32_t start,
                                     uint32_t child, uint32_t len) const;
   __device__ inline void write_split(uint32_t idx, uint32_t am, uint32_t value, uint32_t nl,
@@ -123,6 +174,9 @@ struct BuildOutput {
   uint32_t* perm;
   uint32_t* tris_out;
   mgbvh_split_rec* splits;
+  float* samples = nullptr;
+  uint32_t* labels = nullptr;
+  uint32_t max_samples = 0, sample_min = 0;
 };
 
 // One mesh of a batched build (host side).  Triangle vertex indices are local
@@ -137,6 +191,8 @@ struct BuildResult {
   uint64_t nnodes;            // total over meshes
   uint32_t height, n_levels, n_median, n_cutoff, n_guard;
   uint64_t large_nodes, large_visits;  // nodes with > 32 objects, sum of their object counts
+  uint32_t n_samples;
+  uint32_t n_window;                   // learned nodes split inside their window
   float ms;
   std::vector<uint32_t> mesh_nodes;   // node count per mesh
   std::vector<uint32_t> mesh_nbase;   // first node slot per mesh
@@ -146,7 +202,7 @@ struct BuildResult {
 
 // Kernel classes timed with HIP events when profiling is enabled.
 enum KernelKind { KK_SETUP = 0, KK_ROOT, KK_BIN, KK_SWEEP, KK_PARTITION, KK_SMALL, KK_COMPACT,
-                  KK_PERMUTE, KK_TRACE, KK_COUNT = 16 };
+                  KK_PERMUTE, KK_TRACE, KK_LEARN, KK_COUNT = 16 };
 
 struct mgbvh_ctx {
   int device = 0;
@@ -161,6 +217,8 @@ struct mgbvh_ctx {
   DevBuf mrows, macc, mnb;
   DevBuf io_pos, io_tris, io_nodes, io_lens, io_perm, io_tris_out, io_splits, io_boxes;
   mg::Counters* h_ctr = nullptr;  // pinned
+  mgbvh_model* model = nullptr;   // network of the learned build method (not owned)
+  DevBuf lfeat, llogit, llist;    // learned method: features, logits, slot of each row
 };
 
 namespace mg {
@@ -177,4 +235,8 @@ int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& m
 int prof_begin(mgbvh_ctx* c, int kind);
 int prof_end(mgbvh_ctx* c, int kind);
 int prof_collect(mgbvh_ctx* c);  // after a synchronize: accumulate and recycle
+// Learned method, one level: features of the counted nodes, network
+// inference, window and guard decision (LNode::mode -> window or fallback).
+int learned_decide(mgbvh_ctx* c, LNode* L, uint32_t nL, const uint32_t* hist,
+                   uint32_t n_counted, Counters* ctr, const LearnParams& lp);
 }  // namespace mg

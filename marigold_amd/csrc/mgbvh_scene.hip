@@ -1,4 +1,7 @@
 // Scene-level pieces around the builder: instance world boxes for the TLAS.
+#include <algorithm>
+#include <vector>
+
 #include "mgbvh_common.h"
 #include "mgbvh_internal.h"
 
@@ -53,9 +56,75 @@ __global__ __launch_bounds__(256) void k_instance_boxes(const mgbvh_node32* __re
   out[i] = r;
 }
 
+// SAH cost terms of every node slot: mesh m owns slots [off[m], off[m] + cnt[m]).
+__global__ __launch_bounds__(256) void k_tree_sah(const mgbvh_node32* __restrict__ nodes,
+                                                  const uint32_t* __restrict__ lens,
+                                                  const uint32_t* __restrict__ off,
+                                                  const uint32_t* __restrict__ cnt, uint32_t nm,
+                                                  uint32_t nslots, double* __restrict__ sums) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  uint32_t lo = 0, hi = nm;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  if (i >= off[lo] + cnt[lo]) return;  // unused tail slot of a depth-cut mesh
+  auto sa = [](const mgbvh_node32& n) {  // BoundingVolume::surface_area in f32, lib.rs:79-91
+    Box b;
+    for (int a = 0; a < 3; ++a) { b.mn[a] = n.min[a]; b.mx[a] = n.max[a]; }
+    return (double)box_sa(b);
+  };
+  const mgbvh_node32 nd = nodes[i];
+  const double root = sa(nodes[off[lo]]);
+  const double w = nd.child_node ? 1.0 : 8.0 * (double)(lens ? lens[i] : 1u);
+  atomicAdd(&sums[lo], root > 0.0 ? sa(nd) / root * w : w);
+}
+
 }  // namespace mg
 
 using namespace mg;
+
+extern "C" int mgbvh_tree_sah_device(mgbvh_ctx* c, const mgbvh_node32* d_nodes,
+                                     const uint32_t* d_lens, const mgbvh_blas_desc* blas,
+                                     const uint32_t* node_counts, uint32_t nblas,
+                                     double* out_sah) {
+  if (!c || !d_nodes || !blas || !node_counts || !out_sah) {
+    set_error("null argument");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  if (nblas == 0) return MGBVH_OK;
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  std::vector<uint32_t> tab(2 * (size_t)nblas);
+  uint64_t nslots = 0;
+  for (uint32_t m = 0; m < nblas; ++m) {
+    tab[m] = blas[m].node_offset;
+    tab[nblas + m] = node_counts[m];
+    if (m && blas[m].node_offset < blas[m - 1].node_offset) {
+      set_error("BLAS node offsets must be ascending");
+      return MGBVH_ERR_INVALID_ARG;
+    }
+    nslots = std::max<uint64_t>(nslots, (uint64_t)blas[m].node_offset + node_counts[m]);
+  }
+  if (nslots > 0xffffffffull) { set_error("too many node slots"); return MGBVH_ERR_INVALID_ARG; }
+  int rc;
+  if ((rc = ensure(c->gtab, tab.size() * 4)) || (rc = ensure(c->tmps, (size_t)nblas * 8))) return rc;
+  hipStream_t st = c->stream;
+  e = hipMemcpyAsync(c->gtab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemsetAsync(c->tmps.p, 0, (size_t)nblas * 8, st);
+  if (e != hipSuccess) return hip_fail(e, "tree sah setup");
+  const uint32_t* t = (const uint32_t*)c->gtab.p;
+  if (nslots)
+    k_tree_sah<<<(uint32_t)((nslots + 255) / 256), 256, 0, st>>>(
+        d_nodes, d_lens, t, t + nblas, nblas, (uint32_t)nslots, (double*)c->tmps.p);
+  e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out_sah, c->tmps.p, (size_t)nblas * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e == hipSuccess ? MGBVH_OK : hip_fail(e, "k_tree_sah");
+}
 
 extern "C" int mgbvh_instance_boxes_device(mgbvh_ctx* c, const mgbvh_node32* d_blas_nodes,
                                            const mgbvh_blas_desc* d_blas, uint32_t nblas,

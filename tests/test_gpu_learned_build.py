@@ -1,0 +1,159 @@
+"""The learned build method (MGBVH_METHOD_LEARNED) on the GPU.
+
+The network is trained on the GPU, by the library's own kernels, on the split
+decisions of plain builds of the 65,536-triangle soup and a 512x256-cell
+heightfield; it is tested on Gaussian-cluster meshes it has not seen.
+
+* trees are valid BVHs of the input (structure, ranges, exact bounds);
+* closest hits equal those of the plain tree bit for bit;
+* tree SAH stays within 1.03x of the plain tree's;
+* ablation: the trained network keeps far more nodes on the cheap window path
+  than an untrained one (whose windows the guard rejects), and gives a lower
+  SAH than windows centred on the proxy's best cut without a network.
+"""
+
+import numpy as np
+import pytest
+
+from marigold_amd import _native as N
+from marigold_amd import scenes
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+@pytest.fixture(scope="module")
+def models(ctx):
+    cap = 60000
+    d_f, d_l = ctx.alloc(cap * 102 * 4), ctx.alloc(cap * 4)
+    feats, labels = [], []
+    for pos, tris in (scenes.soup(65536, seed=1), scenes.heightfield(512, 256, seed=2)):
+        dp, dt = ctx.to_device(pos), ctx.to_device(tris)
+        k = ctx.collect_samples(dp, 12, pos.shape[0], dt, tris.shape[0], N.default_settings(), 33,
+                                d_f, d_l, cap)
+        feats.append(d_f.download(np.float32, k * 102))
+        labels.append(d_l.download(np.uint32, k))
+        dp.free()
+        dt.free()
+    f, l = np.concatenate(feats), np.concatenate(labels)
+    n = l.shape[0]
+    d_f.upload(f)
+    d_l.upload(l)
+    trained = N.Model(ctx, seed=1)
+    trained.train(d_f, d_l, n, epochs=20, batch=512, lr=2e-3, seed=1)
+    loss, acc = trained.train(d_f, d_l, n, epochs=10, batch=512, lr=5e-4, seed=2)
+    print(f"[train] {n} samples, loss {loss:.3f}, accuracy {acc:.3f}", flush=True)
+    untrained = N.Model(ctx, seed=2)
+    yield {"trained": trained, "untrained": untrained, "n": n, "acc": acc}
+    trained.close()
+    untrained.close()
+    d_f.free()
+    d_l.free()
+    ctx.set_model(None)
+
+def _build(ctx, pos, tris, model=None, **kw):
+    ctx.set_model(model)
+    t = tris.copy()
+    method = N.METHOD_PLAIN if model is None else N.METHOD_LEARNED
+    s = N.default_settings(method=method, **kw)
+    nodes, lens, perm, _, st = ctx.build_tris(pos, t, s)
+    return nodes, lens, perm, t, st
+
+def _check_tree(pos, tris, nodes, lens, perm):
+    """Structure and exact bounds of a BVH over `tris` with one object per leaf."""
+    n = tris.shape[0]
+    assert nodes.shape[0] == 2 * n - 1
+    child = nodes["child_node"].astype(np.int64)
+    start = nodes["start_index"].astype(np.int64)
+    internal = np.nonzero(child)[0]
+    leaf = np.nonzero(child == 0)[0]
+    assert np.all(lens[leaf] == 1)
+    assert np.array_equal(np.sort(start[leaf]), np.arange(n))
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    kids = np.concatenate([child[internal], child[internal] + 1])
+    assert np.array_equal(np.sort(kids), np.arange(1, 2 * n - 1))  # every node but the root once
+    g, l = child[internal], child[internal] + 1
+    assert np.all(start[l] == start[internal])
+    assert np.all(lens[l].astype(np.int64) + lens[g] == lens[internal])
+    assert np.all(start[g] == start[internal] + lens[l])
+    ab = O.tri_aabbs(pos, tris)[perm]
+    lo = nodes["min"].copy()
+    hi = nodes["max"].copy()
+    assert np.array_equal(lo[leaf], ab[start[leaf], :3]) and np.array_equal(hi[leaf], ab[start[leaf], 3:])
+    want_lo = np.minimum(lo[g], lo[l])
+    want_hi = np.maximum(hi[g], hi[l])
+    assert np.array_equal(lo[internal], want_lo) and np.array_equal(hi[internal], want_hi)
+
+def _sah(ctx, nodes, lens):
+    return O.tree_sah(_as_oracle_nodes(nodes, lens))
+
+def _as_oracle_nodes(nodes, lens):
+    out = np.zeros(nodes.shape[0], dtype=O.NODE_DTYPE)
+    out["min"] = nodes["min"]
+    out["max"] = nodes["max"]
+    out["start"] = nodes["start_index"]
+    out["len"] = lens
+    out["child"] = nodes["child_node"]
+    return out
+
+def _trace(ctx, pos, nodes, lens, perm, tperm, rays):
+    d = [ctx.to_device(x) for x in (nodes, lens, pos, tperm, perm, rays)]
+    d_h = ctx.alloc(16 * rays.shape[0])
+    ctx.trace_closest(d[0], d[1], d[2], 12, d[3], d[4], d[5], rays.shape[0], d_h)
+    h = d_h.download(N.HIT_DTYPE, rays.shape[0])
+    for b in d + [d_h]:
+        b.free()
+    return h
+
+def test_learned_tree_valid_hits_identical_sah(ctx, models):
+    print("[case] learned build of 400k clustered triangles", flush=True)
+    pos, tris = scenes.clusters(400_000, k=64, seed=4)
+    pl = _build(ctx, pos, tris)
+    le = _build(ctx, pos, tris, models["trained"], learned_min_objects=256, learned_window=3,
+                learned_guard=0.02)
+    st = le[4]
+    assert st["n_window_splits"] > 500 and st["n_guard_fallbacks"] < st["n_window_splits"]
+    _check_tree(pos, tris, *le[:3])
+    s_plain, s_learned = _sah(ctx, pl[0], pl[1]), _sah(ctx, le[0], le[1])
+    print(f"SAH plain {s_plain:.3f} learned {s_learned:.3f} ratio {s_learned / s_plain:.4f}",
+          flush=True)
+    assert s_learned <= 1.03 * s_plain
+    assert not np.array_equal(pl[2], le[2])  # a different tree
+    rng = np.random.default_rng(12)
+    o = rng.random((20000, 3)).astype(np.float32) * 1.4 - 0.2
+    d = rng.standard_normal((20000, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = N.make_rays(o, d)
+    a = _trace(ctx, pos, pl[0], pl[1], pl[2], pl[3], rays)
+    b = _trace(ctx, pos, le[0], le[1], le[2], le[3], rays)
+    assert (a["prim"] != N.MISS).sum() > 5000
+    assert np.array_equal(a["t"].view(np.uint32), b["t"].view(np.uint32))
+    assert np.array_equal(a["prim"], b["prim"])
+
+def test_tree_sah_kernel_matches_oracle(ctx):
+    pos, tris = scenes.soup(20000, seed=3)
+    nodes, lens, perm, t, _ = _build(ctx, pos, tris)
+    d_n, d_l = ctx.to_device(nodes), ctx.to_device(lens)
+    blas = np.zeros(1, N.BLAS_DTYPE)
+    got = ctx.tree_sah(d_n, d_l, blas, np.array([nodes.shape[0]], np.uint32))[0]
+    want = O.tree_sah(O.build(O.tri_aabbs(pos, tris))["nodes"])
+    assert abs(got - want) <= 1e-12 * want
+    d_n.free()
+    d_l.free()
+
+def test_learned_ablation(ctx, models):
+    print("[case] ablation on 400k clustered triangles (seed 5)", flush=True)
+    pos, tris = scenes.clusters(400_000, k=64, seed=5)
+    kw = dict(learned_min_objects=256, learned_window=3, learned_guard=0.02)
+    pl = _build(ctx, pos, tris)
+    tr = _build(ctx, pos, tris, models["trained"], **kw)
+    un = _build(ctx, pos, tris, models["untrained"], **kw)
+    nn = _build(ctx, pos, tris, models["trained"], learned_no_network=True, **kw)
+    sp = _sah(ctx, pl[0], pl[1])
+    r = {k: _sah(ctx, v[0], v[1]) / sp for k, v in (("trained", tr), ("untrained", un),
+                                                    ("no-network", nn))}
+    w = {k: v[4]["n_window_splits"] for k, v in (("trained", tr), ("untrained", un),
+                                                ("no-network", nn))}
+    print(f"SAH ratio {r}, window splits {w}", flush=True)
+    assert all(v <= 1.03 for v in r.values())
+    assert w["trained"] > 5 * w["untrained"]   # the guard rejects most untrained windows
+    assert r["trained"] < r["no-network"]      # the network beats the proxy's own choice
