@@ -73,6 +73,20 @@ def kernel_bytes(kind: str, n: int, nverts: int, vstride: int, t: dict) -> float
     return 0.0
 
 
+def measured_traffic(kernel: str, path: str = os.path.join(ROOT, "profiles",
+                                                           "scene_build_traffic.md")):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated), or None."""
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+                return int(f[2])
+    except OSError:
+        pass
+    return None
+
+
 def terms_from_stats(st: dict) -> dict:
     """The same sums from the builder's own work counters."""
     return {"s_large": int(st["large_object_visits"]), "n_large": int(st["large_nodes"]),
@@ -151,60 +165,154 @@ def cpu_baseline_scene(meshes, budget_s: float = 15.0, threads: int = 0):
                       f"{wall:.1f} s wall; AABB setup included, TLAS excluded"}
 
 
+def _hits_of(ctx, sc_or_trace, rays_dev, n):
+    h = ctx.alloc(16 * n)
+    sc_or_trace(rays_dev, n, h)
+    out = h.download(N_HIT(), n)
+    h.free()
+    return out
+
+
+def N_HIT():
+    from marigold_amd import _native as N
+    return N.HIT_DTYPE
+
+
+def same_hits(a, b) -> bool:
+    return bool(np.array_equal(a["t"].view(np.uint32), b["t"].view(np.uint32))
+                and np.array_equal(a["prim"], b["prim"]) and np.array_equal(a["inst"], b["inst"]))
+
+
+def heightfield_16m_rays(ctx, learned: dict):
+    """The full-form check: the 1024x512-cell heightfield built by the plain
+    and by the learned method, 8,388,608 primary rays (pinhole 4096 x 2048) and
+    8,388,608 cosine bounces from the primary hits, closest hits compared."""
+    from marigold_amd import _native as N
+    from marigold_amd import scenes
+    from marigold_amd.scene import look_at_camera
+    pos, tris = scenes.heightfield(1024, 512, seed=2)
+    n = tris.shape[0]
+    d_pos, d_tris = ctx.to_device(pos), ctx.to_device(tris)
+    trees = []
+    for method, extra in ((N.METHOD_PLAIN, {}), (N.METHOD_LEARNED, learned)):
+        dn, dl = ctx.alloc(32 * (2 * n - 1)), ctx.alloc(4 * (2 * n - 1))
+        dp, dt = ctx.alloc(4 * n), ctx.alloc(12 * n)
+        ctx.build_tris_device(d_pos, 12, pos.shape[0], d_tris, n,
+                              N.default_settings(method=method, **extra), dn, dl, dp, dt)
+        trees.append((dn, dl, dp, dt))
+    w, h = 4096, 2048
+    nr = w * h
+    cam = look_at_camera((0.5, 0.6, -0.2), (0.5, 0.0, 0.5), 0.6, w / h)
+    r0, r1 = ctx.alloc(32 * nr), ctx.alloc(32 * nr)
+    hp0, hl0, hp1, hl1 = (ctx.alloc(16 * nr) for _ in range(4))
+    ctx.gen_primary(cam, w, h, r0)
+    (pn, pl, pp, pt), (ln, ll, lp_, lt) = trees
+    ctx.record(4)
+    ctx.trace_closest(pn, pl, d_pos, 12, pt, pp, r0, nr, hp0)
+    ctx.record(5)
+    ctx.trace_closest(ln, ll, d_pos, 12, lt, lp_, r0, nr, hl0)
+    ctx.gen_bounce(r0, hp0, nr, None, 0, None, 0, d_pos, 12, pt, 7, r1)
+    ctx.record(6)
+    ctx.trace_closest(pn, pl, d_pos, 12, pt, pp, r1, nr, hp1)
+    ctx.record(7)
+    ctx.trace_closest(ln, ll, d_pos, 12, lt, lp_, r1, nr, hl1)
+    ms_primary, ms_bounce = ctx.elapsed_ms(4, 5), ctx.elapsed_ms(6, 7)
+    a0, b0 = hp0.download(N.HIT_DTYPE, nr), hl0.download(N.HIT_DTYPE, nr)
+    a1, b1 = hp1.download(N.HIT_DTYPE, nr), hl1.download(N.HIT_DTYPE, nr)
+    # learned hits point at the learned tree's permuted list: compare by original index
+    same = same_hits(a0, b0) and same_hits(a1, b1)
+    out = {"rays": 2 * nr, "identical": same,
+           "primary_hits": int((a0["prim"] != N.MISS).sum()),
+           "bounce_hits": int((a1["prim"] != N.MISS).sum()),
+           "trace_mrays_per_s": {"primary": nr / (ms_primary * 1e-3) / 1e6,
+                                 "bounce": nr / (ms_bounce * 1e-3) / 1e6}}
+    for b in (d_pos, d_tris, r0, r1, hp0, hl0, hp1, hl1):
+        b.free()
+    for t in trees:
+        for b in t:
+            b.free()
+    return out
+
+
 def scene_main(args):
     from marigold_amd import _native as N
     from marigold_amd import scenes
-    from marigold_amd.scene import GpuScene
+    from marigold_amd.learned import DEFAULT_LEARNED, train_split_model
+    from marigold_amd.scene import GpuScene, look_at_camera
 
     t0 = time.perf_counter()
     sc = scenes.batched_scene()
     meshes = list(zip(sc["positions"], sc["tris"]))
     n = int(sum(t.shape[0] for _, t in meshes))
     nv = int(sum(p.shape[0] for p, _ in meshes))
-    print(f"[bench] scene: {len(meshes)} meshes, {n} tris, {nv} verts, "
-          f"{len(sc['instance_mesh'])} instances (generated in {time.perf_counter() - t0:.1f} s)",
-          file=sys.stderr, flush=True)
+    log(f"scene: {len(meshes)} meshes, {n} tris, {nv} verts, {len(sc['instance_mesh'])} instances "
+        f"(generated in {time.perf_counter() - t0:.1f} s)")
     ctx = N.Context(0)
-    bat = GpuScene(ctx, meshes, sc["transforms"], sc["instance_mesh"], mode="batched", build=False)
-    print("[bench] uploaded", file=sys.stderr, flush=True)
+    model, train_info = train_split_model(ctx, epochs=(4, 2) if args.quick_train else (30, 15))
+    ctx.set_model(model)
+    log(f"split proposer trained on the GPU: {train_info}")
+    learned = dict(DEFAULT_LEARNED)
+    scene_args = (ctx, meshes, sc["transforms"], sc["instance_mesh"])
 
-    # untimed warm-up with every kernel class timed, to find the dominant one
+    # primary rays into the scene for the hit comparison of the two trees
+    w, h = 1024, 1024
+    cam = look_at_camera((0.0, 60.0, -260.0), (0.0, 0.0, 0.0), 0.9, 1.0)
+    d_rays = ctx.alloc(32 * w * h)
+    ctx.gen_primary(cam, w, h, d_rays)
+
+    # ---- learned + batched (the headline)
+    lea = GpuScene(*scene_args, mode="batched", build=False, method=N.METHOD_LEARNED,
+                   learned=learned)
     ctx.set_profiling(N.Context.KERNEL_KINDS)
     for _ in range(max(args.warmup, 1)):
-        bat.build()
+        lea.build()
     prof = ctx.kernel_times(reset=True)
     dominant = max(prof, key=lambda k: prof[k][0])
-    # the work counters of one batched BLAS build
-    d = bat.d
-    _, _, st = ctx.build_blas_batch(d["pos"], 12, nv, d["tris"], bat.mesh_desc, bat.blas_settings,
+    d = lea.d
+    _, _, st = ctx.build_blas_batch(d["pos"], 12, nv, d["tris"], lea.mesh_desc, lea.blas_settings,
                                     d["blas_nodes"], d["blas_lens"], d["blas_perm"], d["tris_perm"])
     terms = terms_from_stats(st)
-    print(f"[bench] warm-up done; dominant kernel class {dominant}", file=sys.stderr, flush=True)
-
+    log(f"warm-up done; dominant kernel class {dominant}")
     ctx.set_profiling([dominant])
     ctx.synchronize()
     ctx.record(0)
-    for k in range(args.steps):
-        bat.build()
+    for _ in range(args.steps):
+        lea.build()
     ctx.record(1)
     ctx.synchronize()
     ms_step = ctx.elapsed_ms(0, 1) / args.steps
     kt = ctx.kernel_times(reset=True)[dominant]
+    ctx.set_profiling([])
     dom_ms = kt[0] / args.steps
     dom_bytes = kernel_bytes(dominant, n, nv, 12, terms)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     build_bytes = sum(kernel_bytes(k, n, nv, 12, terms) for k in ("setup", "bin", "partition",
                                                                    "sweep", "small"))
-    print(f"[bench] batched {ms_step:.2f} ms/scene", file=sys.stderr, flush=True)
-    bat_nodes_sum = int(bat.download("blas_nodes", np.uint32, 8 * bat.n_slots).astype(np.uint64).sum())
+    sah_learned = lea.scene_sah()
+    hits_learned = _hits_of(ctx, lea.trace, d_rays, w * h)
+    log(f"learned+batched {ms_step:.2f} ms/scene, SAH {sah_learned:.4f}")
+    lea.free()
+
+    # ---- batched, plain split rule (bit-identical to the per-mesh plain builds)
+    bat = GpuScene(*scene_args, mode="batched", build=False)
+    bat.build()
+    ctx.synchronize()
+    ctx.record(2)
+    for _ in range(args.steps):
+        bat.build()
+    ctx.record(3)
+    ctx.synchronize()
+    ms_bat = ctx.elapsed_ms(2, 3) / args.steps
+    sah_plain = bat.scene_sah()
+    hits_plain = _hits_of(ctx, bat.trace, d_rays, w * h)
+    bat_sum = int(bat.download("blas_nodes", np.uint32, 8 * bat.n_slots).astype(np.uint64).sum())
+    log(f"batched plain {ms_bat:.2f} ms/scene, SAH {sah_plain:.4f}")
     bat.free()
 
-    # the plain builder: one device build call per mesh, then the same TLAS path
-    ms_plain, same = None, None
+    # ---- plain: one device build call per mesh (the reference's API shape)
+    ms_plain, same_nodes = None, None
     if args.plain_steps > 0:
-        ctx.set_profiling([])
-        pla = GpuScene(ctx, meshes, sc["transforms"], sc["instance_mesh"], mode="plain",
-                       build=False)
+        pla = GpuScene(*scene_args, mode="plain", build=False)
         pla.build()
         ctx.synchronize()
         ctx.record(2)
@@ -213,10 +321,24 @@ def scene_main(args):
         ctx.record(3)
         ctx.synchronize()
         ms_plain = ctx.elapsed_ms(2, 3) / args.plain_steps
-        same = bat_nodes_sum == int(pla.download("blas_nodes", np.uint32, 8 * pla.n_slots)
+        same_nodes = bat_sum == int(pla.download("blas_nodes", np.uint32, 8 * pla.n_slots)
                                     .astype(np.uint64).sum())
-        print(f"[bench] plain {ms_plain:.2f} ms/scene", file=sys.stderr, flush=True)
+        log(f"plain per mesh {ms_plain:.2f} ms/scene")
         pla.free()
+    d_rays.free()
+
+    full = None
+    if not args.no_rays:
+        log("16M-ray hit comparison on the heightfield")
+        full = heightfield_16m_rays(ctx, learned)
+    if args.calibrate_copy:
+        # a known-size device copy for scaling the FETCH_SIZE / WRITE_SIZE counters
+        nbytes = 1 << 30
+        a, b = ctx.alloc(nbytes), ctx.alloc(nbytes)
+        N.check(N.lib().mgbvh_copy_d2d(ctx.handle, b.ptr, a.ptr, nbytes))
+        a.free()
+        b.free()
+    model.close()
     out = {
         "metric": "BVH build Mtri/s, plain vs learned+batched, at SAH <=1.03x plain; 1 MI355X",
         "value": n / (ms_step * 1e-3) / 1e6,
@@ -232,27 +354,39 @@ def scene_main(args):
         "data": "synthetic (seeded)",
         "config": {"workload": "2,000-mesh scene (log-uniform 100..200,000 tris, 50/50 soup / "
                                "heightfield) + TLAS over 10,000 instances; whole scene build",
-                   "tris": n, "verts": nv, "meshes": len(meshes), "instances": len(sc["instance_mesh"]),
-                   "max_depth": 32, "method": "batched (plain split rule)"},
+                   "tris": n, "verts": nv, "meshes": len(meshes),
+                   "instances": len(sc["instance_mesh"]), "max_depth": 32,
+                   "method": "learned + batched", "learned": learned},
         "plain_per_mesh": None if ms_plain is None else
         {"value": n / (ms_plain * 1e-3) / 1e6, "ms_per_step": ms_plain, "steps": args.plain_steps,
-         "node_arrays_identical": same},
+         "node_arrays_equal_batched_plain": same_nodes},
+        "batched_plain": {"value": n / (ms_bat * 1e-3) / 1e6, "ms_per_step": ms_bat},
         "speedup_vs_plain": None if ms_plain is None else ms_plain / ms_step,
-        "sah_ratio_vs_plain": 1.0 if same else None,
+        "sah": {"plain": sah_plain, "learned": sah_learned, "ratio": sah_learned / sah_plain},
+        "learned_stats": {"window_splits": st["n_window_splits"],
+                          "guard_fallbacks": st["n_guard_fallbacks"], "training": train_info},
+        "scene_hits_identical": {"rays": w * h, "identical": same_hits(hits_plain, hits_learned),
+                                 "hits": int((hits_plain["prim"] != N.MISS).sum())},
+        "heightfield_16m_rays": full,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
-                     "kernel_ms_per_build": dom_ms, "alg_bytes_per_build": dom_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
+                     "kernel": dominant, "kernel_ms_per_launch": dom_ms,
+                     "alg_bytes_per_launch": dom_bytes},
         "build_alg_gbs": build_bytes / (ms_step * 1e-3) / 1e9,
         "kernel_ms_per_build": {k: v[0] / max(args.warmup, 1) for k, v in prof.items()},
         "work": terms,
     }
     ctx.close()
     if not args.no_cpu_baseline:
-        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline_scene(meshes)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+
+
+def log(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -263,6 +397,11 @@ def main():
     ap.add_argument("--workload", default="scene", choices=["scene", "heightfield", "soup"])
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-rays", action="store_true", help="skip the 16M-ray hit comparison")
+    ap.add_argument("--quick-train", action="store_true",
+                    help="6 training epochs instead of 45 (profiling runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="end with one 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: multi-GPU sharding "

@@ -177,6 +177,9 @@ int mgbvh_device_alloc(mgbvh_ctx* ctx, uint64_t bytes, void** out);
 int mgbvh_device_free(mgbvh_ctx* ctx, void* p);
 int mgbvh_copy_h2d(mgbvh_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 int mgbvh_copy_d2h(mgbvh_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Device-to-device copy on the context stream (also the known-size transfer
+ * the traffic counters are calibrated with). */
+int mgbvh_copy_d2d(mgbvh_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 
 /* ---- per-kernel timing: HIP event pairs around each launch of the kernel
  * classes selected by `kernel_mask` (bit k = class k), accumulated per class.

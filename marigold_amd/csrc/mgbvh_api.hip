@@ -616,6 +616,14 @@ int mgbvh_copy_h2d(mgbvh_ctx* c, void* dst, const void* src, uint64_t bytes) {
   return e == hipSuccess ? MGBVH_OK : hip_fail(e, "hipMemcpy H2D");
 }
 
+int mgbvh_copy_d2d(mgbvh_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c || (bytes && (!dst || !src))) { set_error("null argument"); return MGBVH_ERR_INVALID_ARG; }
+  if (!bytes) return MGBVH_OK;
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? MGBVH_OK : hip_fail(e, "hipMemcpyAsync(d2d)");
+}
+
 int mgbvh_copy_d2h(mgbvh_ctx* c, void* dst, const void* src, uint64_t bytes) {
   if (!c) { set_error("null argument"); return MGBVH_ERR_INVALID_ARG; }
   if (!bytes) return MGBVH_OK;

@@ -180,7 +180,7 @@ __global__ __launch_bounds__(256) void k_setup_aabbs(const float* __restrict__ b
 // max_depth == 0) goes straight to the small-subtree kernel.  One wave per mesh.
 __global__ void k_root(const MeshRow* __restrict__ rows, const uint32_t* __restrict__ macc,
                        Counters* ctr, LNode* L, uint32_t* cnode, uint32_t* ck, SNode* small,
-                       uint32_t n_small_roots, uint32_t learn_min) {
+                       uint32_t n_small_roots, uint32_t learn_min, uint32_t* lrows) {
   const uint32_t m = blockIdx.x;
   if (m == 0 && threadIdx.x == 0) ctr->n_small = n_small_roots;
   const MeshRow mr = rows[m];
@@ -204,6 +204,7 @@ __global__ void k_root(const MeshRow* __restrict__ rows, const uint32_t* __restr
       r.child_slot[0] = r.child_slot[1] = -1;
       r.nbase = mr.nbase; r.obase = mr.tri_offset;
       r.mode = (learn_min && mr.count >= learn_min) ? LM_COUNTED : LM_PLAIN;
+      if (r.mode == LM_COUNTED) lrows[mr.lrow] = mr.root_slot;
       L[mr.root_slot] = r;
     } else {
       SNode sn = {};
@@ -324,7 +325,7 @@ __device__ inline void put_child(bool large, uint32_t s, uint32_t c0, uint32_t s
                                  uint32_t count, const Box& b, uint32_t idx, uint32_t P,
                                  uint32_t depth, uint32_t parity_next, uint32_t nbase,
                                  uint32_t obase, LNode* Ln, SNode* small, uint32_t learn_min,
-                                 Counters* ctr) {
+                                 Counters* ctr, uint32_t* lrows) {
   if (large) {
     LNode r = {};
     r.start = start; r.count = count; r.idx = idx; r.P = P; r.depth = depth;
@@ -333,7 +334,7 @@ __device__ inline void put_child(bool large, uint32_t s, uint32_t c0, uint32_t s
     r.child_slot[0] = r.child_slot[1] = -1;
     r.nbase = nbase; r.obase = obase;
     r.mode = (learn_min && count >= learn_min) ? LM_COUNTED : LM_PLAIN;
-    if (r.mode == LM_COUNTED) atomicAdd(&ctr->n_learned_next, 1u);
+    if (r.mode == LM_COUNTED) lrows[atomicAdd(&ctr->n_learned_next, 1u)] = s;
     Ln[s] = r;
   } else {
     SNode sn = {};
@@ -356,7 +357,8 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
                                                 uint32_t* cn_next, uint32_t* ck_next,
                                                 SNode* small, Counters* ctr,
                                                 NodeOut out, uint32_t max_depth,
-                                                uint32_t parity_next, uint32_t learn_min) {
+                                                uint32_t parity_next, uint32_t learn_min,
+                                                uint32_t* lrows_next) {
   __shared__ uint32_t sh[SW_WAVES][HIST_WORDS];
   __shared__ uint32_t s_need[SW_WAVES][4];  // large slots, chunks, small slots, objects
   __shared__ uint32_t s_base[SW_WAVES][3];
@@ -539,9 +541,9 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
     out.write_split(nd.idx, axis | (kind << 8), value, nl, fbits(cost));
     if (kind == 2) atomicAdd(&ctr->n_median, 1u);
     put_child(lg, sgs, c0g, nd.start + nl, ng, gt, nd.P, nd.P + 2, cd, parity_next, nd.nbase,
-              nd.obase, Ln, small, learn_min, ctr);
+              nd.obase, Ln, small, learn_min, ctr, lrows_next);
     put_child(ll, sls, c0l, nd.start, nl, lt, nd.P + 1, nd.P + 2 + 2 * (ng - 1), cd, parity_next,
-              nd.nbase, nd.obase, Ln, small, learn_min, ctr);
+              nd.nbase, nd.obase, Ln, small, learn_min, ctr, lrows_next);
     L[slot].axis = axis;
     L[slot].cut = value;
     L[slot].n_lt = nl;
@@ -1135,6 +1137,21 @@ int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& m
   out.max_samples = o.max_samples;
   out.sample_min = o.sample_min;
 
+  LearnParams lp{0, 0, 0.0f, 0};
+  uint32_t n_counted = 0;
+  if (s->method == MGBVH_METHOD_LEARNED) {
+    if (!c->model) { set_error("the learned method needs a model"); return MGBVH_ERR_UNSUPPORTED; }
+    lp.min_objects = s->learned_min_objects ? std::max<uint32_t>(s->learned_min_objects, MG_SMALL + 1)
+                                            : 512u;
+    lp.window = s->learned_window ? s->learned_window : 3u;
+    lp.guard = s->learned_guard;
+    lp.no_network = s->learned_no_network;
+    for (uint32_t i = 0; i < nm; ++i)
+      if (rows[i].root_kind == 1 && rows[i].count >= lp.min_objects) rows[i].lrow = n_counted++;
+    for (int p = 0; p < 2; ++p)
+      if ((rc = ensure(c->lrows[p], (size_t)capL * 4))) return rc;
+  }
+  uint32_t* lrows[2] = {(uint32_t*)c->lrows[0].p, (uint32_t*)c->lrows[1].p};
   MG_HIP(hipEventRecord(c->ev[62], st));
   MG_HIP(hipMemcpyAsync(drows, rows.data(), (size_t)nm * sizeof(MeshRow), hipMemcpyHostToDevice,
                         st));
@@ -1150,20 +1167,10 @@ int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& m
     k_setup_aabbs<<<(uint32_t)setup_chunks, 256, 0, st>>>(in.boxes, drows, nm, recs[0], macc, ctr);
   MG_HIP(hipGetLastError());
   MG_PROF(KK_SETUP, 1);
-  LearnParams lp{0, 0, 0.0f, 0};
-  uint32_t n_counted = 0;
-  if (s->method == MGBVH_METHOD_LEARNED) {
-    if (!c->model) { set_error("the learned method needs a model"); return MGBVH_ERR_UNSUPPORTED; }
-    lp.min_objects = s->learned_min_objects ? std::max<uint32_t>(s->learned_min_objects, MG_SMALL + 1)
-                                            : 512u;
-    lp.window = s->learned_window ? s->learned_window : 3u;
-    lp.guard = s->learned_guard;
-    lp.no_network = s->learned_no_network;
-    for (uint32_t i = 0; i < nm; ++i)
-      if (rows[i].root_kind == 1 && rows[i].count >= lp.min_objects) ++n_counted;
-  }
+
   MG_PROF(KK_ROOT, 0);
-  k_root<<<nm, 64, 0, st>>>(drows, macc, ctr, L[0], cn[0], ck[0], small, n_small, lp.min_objects);
+  k_root<<<nm, 64, 0, st>>>(drows, macc, ctr, L[0], cn[0], ck[0], small, n_small, lp.min_objects,
+                            lrows[0]);
   MG_PROF(KK_ROOT, 1);
   MG_HIP(hipGetLastError());
 
@@ -1175,13 +1182,13 @@ int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& m
     large_nodes += nL;
     MG_HIP(hipMemsetAsync(hist, 0, (size_t)nL * HIST_WORDS * 4, st));
     MG_HIP(hipMemsetAsync(ctr, 0, 8, st));  // n_large_next, n_chunks_next
-    MG_HIP(hipMemsetAsync(&ctr->n_learned_next, 0, 8, st));  // and n_learned_cur
+    MG_HIP(hipMemsetAsync(&ctr->n_learned_next, 0, 4, st));
     MG_PROF(KK_BIN, 0);
     k_bin<<<nC, 256, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk, 1);
     MG_PROF(KK_BIN, 1);
     if (n_counted) {
       MG_PROF(KK_LEARN, 0);
-      if ((rc = learned_decide(c, L[cur], nL, hist, n_counted, ctr, lp))) return rc;
+      if ((rc = learned_decide(c, L[cur], lrows[cur], n_counted, hist, ctr, lp))) return rc;
       MG_PROF(KK_LEARN, 1);
       MG_PROF(KK_BIN, 0);
       k_bin<<<nC, 256, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk, 2);
@@ -1191,7 +1198,7 @@ int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& m
     k_sweep<<<cdiv(nL, SW_WAVES), 1024, 0, st>>>(L[cur], nL, hist, recs[cur], L[cur ^ 1],
                                                  cn[cur ^ 1], ck[cur ^ 1], small, ctr, out,
                                                  s->max_depth, (uint32_t)(cur ^ 1),
-                                                 lp.min_objects);
+                                                 lp.min_objects, lrows[cur ^ 1]);
     MG_PROF(KK_SWEEP, 1);
     MG_PROF(KK_PARTITION, 0);
     k_partition<<<nC, 256, 0, st>>>(L[cur], cn[cur], ck[cur], blk, recs[cur], recs[cur ^ 1],

@@ -78,6 +78,7 @@ struct MeshRow {
   uint32_t lchunk0;       // first level-0 chunk of a large root
   uint32_t has_bounds;
   float bmn[3], bmx[3];
+  uint32_t lrow;          // learned method: network row of a counted root
 };
 
 struct Counters {
@@ -87,7 +88,7 @@ struct Counters {
   unsigned long long sum_large;  // objects of all nodes split on the level path
   uint32_t n_samples;            // split samples written (may exceed the capacity)
   uint32_t n_learned_next;       // counted (learned) nodes created for the next level
-  uint32_t n_learned_cur;        // rows of this level's network batch
+  uint32_t pad_l;
   uint32_t n_window, n_fallback; // learned nodes swept in a window / by the plain sweep
   uint32_t pad[5];
 };
@@ -218,7 +219,7 @@ struct mgbvh_ctx {
   DevBuf io_pos, io_tris, io_nodes, io_lens, io_perm, io_tris_out, io_splits, io_boxes;
   mg::Counters* h_ctr = nullptr;  // pinned
   mgbvh_model* model = nullptr;   // network of the learned build method (not owned)
-  DevBuf lfeat, llogit, llist;    // learned method: features, logits, slot of each row
+  DevBuf lfeat, llogit, lrows[2]; // learned method: features, logits, slot of each row
 };
 
 namespace mg {
@@ -237,6 +238,6 @@ int prof_end(mgbvh_ctx* c, int kind);
 int prof_collect(mgbvh_ctx* c);  // after a synchronize: accumulate and recycle
 // Learned method, one level: features of the counted nodes, network
 // inference, window and guard decision (LNode::mode -> window or fallback).
-int learned_decide(mgbvh_ctx* c, LNode* L, uint32_t nL, const uint32_t* hist,
-                   uint32_t n_counted, Counters* ctr, const LearnParams& lp);
+int learned_decide(mgbvh_ctx* c, LNode* L, const uint32_t* rows, uint32_t n_counted,
+                   const uint32_t* hist, Counters* ctr, const LearnParams& lp);
 }  // namespace mg

@@ -228,24 +228,17 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ fea
 }
 
 // ------------------------------------------------------------ learned build step
-// One wavefront per node of the level: a counted node gets a network row
-// (its features) and its slot in `list`.
-__global__ __launch_bounds__(256) void k_feat(LNode* __restrict__ L, uint32_t nL,
-                                              const uint32_t* __restrict__ hist, Counters* ctr,
-                                              uint32_t cap, float* __restrict__ feat,
-                                              uint32_t* __restrict__ list) {
-  const uint32_t slot = blockIdx.x * 4 + (threadIdx.x >> 6);
+// One wavefront per network row (a counted node, listed when it was created):
+// its features.
+__global__ __launch_bounds__(256) void k_feat(const LNode* __restrict__ L,
+                                              const uint32_t* __restrict__ rows, uint32_t n,
+                                              const uint32_t* __restrict__ hist,
+                                              float* __restrict__ feat) {
+  const uint32_t row = blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t lane = lane_id();
-  if (slot >= nL) return;
+  if (row >= n) return;
+  const uint32_t slot = rows[row];
   const LNode& nd = L[slot];
-  if (lmode_kind(nd.mode) != LM_COUNTED) return;
-  uint32_t row = 0;
-  if (lane == 0) row = atomicAdd(&ctr->n_learned_cur, 1u);
-  row = __shfl(row, 0, 64);
-  if (row >= cap) {  // no network row: the node takes the plain sweep
-    if (lane == 0) L[slot].mode = LM_FALLBACK;
-    return;
-  }
   const uint32_t* h = hist + (size_t)slot * HIST_WORDS;
   float* f = feat + (size_t)row * MLP_IN;
   const float inv = 1.0f / log1pf((float)nd.count);
@@ -254,7 +247,6 @@ __global__ __launch_bounds__(256) void k_feat(LNode* __restrict__ L, uint32_t nL
     float cext[3];
     for (int a = 0; a < 3; ++a) cext[a] = acc_max_dec(nd.cacc[3 + a]) - acc_min_dec(nd.cacc[a]);
     node_extent_features(nd.bmn, nd.bmx, cext, f + 3 * MG_NBINS);
-    list[row] = slot;
   }
 }
 
@@ -439,18 +431,16 @@ int model_infer(mgbvh_ctx* c, mgbvh_model* m, const float* feat, int stride, int
   return e == hipSuccess ? MGBVH_OK : hip_fail(e, "k_mlp_infer");
 }
 
-int learned_decide(mgbvh_ctx* c, LNode* L, uint32_t nL, const uint32_t* hist,
-                   uint32_t n_counted, Counters* ctr, const LearnParams& lp) {
+int learned_decide(mgbvh_ctx* c, LNode* L, const uint32_t* list, uint32_t n_counted,
+                   const uint32_t* hist, Counters* ctr, const LearnParams& lp) {
   int rc;
   if ((rc = ensure(c->lfeat, (size_t)n_counted * MLP_IN * 4)) ||
-      (rc = ensure(c->llogit, (size_t)n_counted * MLP_OUT * 4)) ||
-      (rc = ensure(c->llist, (size_t)n_counted * 4)))
+      (rc = ensure(c->llogit, (size_t)n_counted * MLP_OUT * 4)))
     return rc;
   hipStream_t st = c->stream;
   float* feat = (float*)c->lfeat.p;
   float* logits = (float*)c->llogit.p;
-  uint32_t* list = (uint32_t*)c->llist.p;
-  k_feat<<<cdivu(nL, 4), 256, 0, st>>>(L, nL, hist, ctr, n_counted, feat, list);
+  k_feat<<<cdivu(n_counted, 4), 256, 0, st>>>(L, list, n_counted, hist, feat);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "k_feat");
   if (!lp.no_network && (rc = model_infer(c, c->model, feat, MLP_IN, (int)n_counted, logits)))

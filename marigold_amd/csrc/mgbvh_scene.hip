@@ -79,7 +79,20 @@ __global__ __launch_bounds__(256) void k_tree_sah(const mgbvh_node32* __restrict
   const mgbvh_node32 nd = nodes[i];
   const double root = sa(nodes[off[lo]]);
   const double w = nd.child_node ? 1.0 : 8.0 * (double)(lens ? lens[i] : 1u);
-  atomicAdd(&sums[lo], root > 0.0 ? sa(nd) / root * w : w);
+  double term = root > 0.0 ? sa(nd) / root * w : w;
+  // one atomic per wave when the whole wave lies in one mesh (the common case)
+  const uint32_t m0 = __shfl(lo, 0, 64);
+  const uint64_t active = __ballot(true);
+  if (__ballot(lo == m0) == active) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double v = __shfl_xor(term, o, 64);
+      term += ((active >> (lane_id() ^ o)) & 1ull) ? v : 0.0;
+    }
+    if (lane_id() == (uint32_t)__builtin_ctzll(active)) atomicAdd(&sums[lo], term);
+  } else {
+    atomicAdd(&sums[lo], term);
+  }
 }
 
 }  // namespace mg

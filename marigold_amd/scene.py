@@ -35,7 +35,8 @@ class GpuScene:
 
     def __init__(self, ctx: N.Context, meshes, transforms: np.ndarray, instance_mesh: np.ndarray,
                  mode: str = "batched", blas_max_depth: int = BLAS_MAX_DEPTH,
-                 tlas_max_depth: int = TLAS_MAX_DEPTH, build: bool = True):
+                 tlas_max_depth: int = TLAS_MAX_DEPTH, build: bool = True,
+                 method: int = N.METHOD_PLAIN, learned: dict | None = None):
         assert mode in ("plain", "batched")
         self.ctx = ctx
         self.mode = mode
@@ -68,7 +69,8 @@ class GpuScene:
         self.instances = np.zeros(self.n_instances, dtype=N.INSTANCE_DTYPE)
         self.instances["world_to_object"] = world_to_object(self.transforms)
         self.instances["blas"] = self.instance_mesh
-        self.blas_settings = N.default_settings(max_depth=blas_max_depth)
+        self.blas_settings = N.default_settings(max_depth=blas_max_depth, method=method,
+                                                **(learned or {}))
         self.tlas_settings = N.default_settings(max_depth=tlas_max_depth, name="tlas")
         a, up = ctx.alloc, ctx.to_device
         ni = self.n_instances
@@ -118,6 +120,15 @@ class GpuScene:
     def build(self):
         self.build_blas()
         self.build_tlas()
+
+    def blas_sah(self) -> np.ndarray:
+        """SAH cost of every BLAS (reference constants, relative to its root)."""
+        return self.ctx.tree_sah(self.d["blas_nodes"], self.d["blas_lens"], self.blas,
+                                 self.blas_node_counts)
+
+    def scene_sah(self) -> float:
+        """Triangle-weighted mean of the BLAS SAH costs."""
+        return float((self.blas_sah() * self.counts).sum() / max(self.counts.sum(), 1))
 
     # ------------------------------------------------------------------ readback
     def download(self, name, dtype, count):
