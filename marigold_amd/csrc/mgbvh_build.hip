@@ -180,7 +180,8 @@ __global__ __launch_bounds__(256) void k_setup_aabbs(const float* __restrict__ b
 // max_depth == 0) goes straight to the small-subtree kernel.  One wave per mesh.
 __global__ void k_root(const MeshRow* __restrict__ rows, const uint32_t* __restrict__ macc,
                        Counters* ctr, LNode* L, uint32_t* cnode, uint32_t* ck, SNode* small,
-                       uint32_t n_small_roots, uint32_t learn_min, uint32_t* lrows) {
+                       uint32_t n_small_roots, uint32_t learn_min, uint32_t* lrows,
+                       uint32_t* hist) {
   const uint32_t m = blockIdx.x;
   if (m == 0 && threadIdx.x == 0) ctr->n_small = n_small_roots;
   const MeshRow mr = rows[m];
@@ -220,6 +221,10 @@ __global__ void k_root(const MeshRow* __restrict__ rows, const uint32_t* __restr
       cnode[mr.lchunk0 + k] = mr.root_slot;
       ck[mr.lchunk0 + k] = k;
     }
+    // several chunks merge into this row with atomics: it starts at zero
+    if (nch > 1)
+      for (uint32_t t = threadIdx.x; t < HIST_WORDS; t += blockDim.x)
+        hist[(size_t)mr.root_slot * HIST_WORDS + t] = 0;
   }
 }
 
@@ -242,7 +247,9 @@ __global__ __launch_bounds__(256) void k_bin(const LNode* __restrict__ L,
   const uint32_t k = ck[c];
   const LNode& nd = L[slot];
   const uint32_t lm = nd.mode, lk = lmode_kind(lm), la = lmode_axis(lm);
+  if (nd.flags & LF_PREBINNED) return;       // block-uniform: binned by the parent's partition
   if (pass == 2 && lk < LM_WINDOW) return;  // block-uniform: nothing left to bin
+  const bool single = nd.nchunk == 1;
   for (uint32_t t = threadIdx.x; t < HIST_WORDS; t += blockDim.x) h[t] = 0;
   float cmin[3], cext[3];
 #pragma unroll
@@ -283,8 +290,11 @@ __global__ __launch_bounds__(256) void k_bin(const LNode* __restrict__ L,
     if (f == 0) {
       if (pass == 1) {
         blk[(size_t)c * 96 + t / BIN_WORDS] = v;
-        if (v) atomicAdd(&gh[t], v);
+        if (single) gh[t] = v;
+        else if (v) atomicAdd(&gh[t], v);
       }
+    } else if (single) {
+      gh[t] = v;  // pass 2 only adds boxes: the counts written by pass 1 stay
     } else if (v) {
       atomicMax(&gh[t], v);
     }
@@ -325,7 +335,7 @@ __device__ inline void put_child(bool large, uint32_t s, uint32_t c0, uint32_t s
                                  uint32_t count, const Box& b, uint32_t idx, uint32_t P,
                                  uint32_t depth, uint32_t parity_next, uint32_t nbase,
                                  uint32_t obase, LNode* Ln, SNode* small, uint32_t learn_min,
-                                 Counters* ctr, uint32_t* lrows) {
+                                 Counters* ctr, uint32_t* lrows, bool prebinned) {
   if (large) {
     LNode r = {};
     r.start = start; r.count = count; r.idx = idx; r.P = P; r.depth = depth;
@@ -333,7 +343,8 @@ __device__ inline void put_child(bool large, uint32_t s, uint32_t c0, uint32_t s
     for (int a = 0; a < 3; ++a) { r.bmn[a] = b.mn[a]; r.bmx[a] = b.mx[a]; }
     r.child_slot[0] = r.child_slot[1] = -1;
     r.nbase = nbase; r.obase = obase;
-    r.mode = (learn_min && count >= learn_min) ? LM_COUNTED : LM_PLAIN;
+    r.flags = prebinned ? LF_PREBINNED : 0u;
+    r.mode = (!prebinned && learn_min && count >= learn_min) ? LM_COUNTED : LM_PLAIN;
     if (r.mode == LM_COUNTED) lrows[atomicAdd(&ctr->n_learned_next, 1u)] = s;
     Ln[s] = r;
   } else {
@@ -541,9 +552,9 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
     out.write_split(nd.idx, axis | (kind << 8), value, nl, fbits(cost));
     if (kind == 2) atomicAdd(&ctr->n_median, 1u);
     put_child(lg, sgs, c0g, nd.start + nl, ng, gt, nd.P, nd.P + 2, cd, parity_next, nd.nbase,
-              nd.obase, Ln, small, learn_min, ctr, lrows_next);
+              nd.obase, Ln, small, learn_min, ctr, lrows_next, nd.nchunk == 1);
     put_child(ll, sls, c0l, nd.start, nl, lt, nd.P + 1, nd.P + 2 + 2 * (ng - 1), cd, parity_next,
-              nd.nbase, nd.obase, Ln, small, learn_min, ctr, lrows_next);
+              nd.nbase, nd.obase, Ln, small, learn_min, ctr, lrows_next, nd.nchunk == 1);
     L[slot].axis = axis;
     L[slot].cut = value;
     L[slot].n_lt = nl;
@@ -564,7 +575,8 @@ __global__ __launch_bounds__(256) void k_partition(const LNode* __restrict__ L,
                                                    const uint32_t* __restrict__ ck,
                                                    const uint32_t* __restrict__ blk,
                                                    const Rec* __restrict__ src,
-                                                   Rec* __restrict__ dst, LNode* Ln) {
+                                                   Rec* __restrict__ dst, LNode* Ln,
+                                                   uint32_t* __restrict__ hist) {
   __shared__ uint32_t s_red[4];
   __shared__ uint32_t s_lt[4], s_gt[4];
   const uint32_t c = blockIdx.x;
@@ -655,12 +667,71 @@ __global__ __launch_bounds__(256) void k_partition(const LNode* __restrict__ L,
     for (int j = 0; j < 6; ++j) { red[w][j] = acc_gt[j]; red[w][6 + j] = acc_lt[j]; }
   }
   __syncthreads();
+  const bool single = nd.nchunk == 1;
   if (threadIdx.x < 12) {
     const int which = threadIdx.x < 6 ? 0 : 1;  // 0 = gt child, 1 = lt child
     const int32_t cs = nd.child_slot[which];
     uint32_t v = max(max(red[0][threadIdx.x], red[1][threadIdx.x]),
                      max(red[2][threadIdx.x], red[3][threadIdx.x]));
-    if (cs >= 0 && v) atomicMax(&Ln[cs].cacc[threadIdx.x % 6], v);
+    if (cs >= 0) {
+      if (single) Ln[cs].cacc[threadIdx.x % 6] = v;  // this block saw all of the child
+      else if (v) atomicMax(&Ln[cs].cacc[threadIdx.x % 6], v);
+    }
+  }
+  if (!single) {
+    // a child spanning several chunks merges its histogram atomically: zero its row
+    if (k == 0) {
+      for (int which = 0; which < 2; ++which) {
+        const int32_t cs = nd.child_slot[which];
+        if (cs >= 0 && Ln[cs].nchunk > 1)
+          for (uint32_t t = threadIdx.x; t < HIST_WORDS; t += blockDim.x)
+            hist[(size_t)cs * HIST_WORDS + t] = 0;
+      }
+    }
+    return;
+  }
+  // The whole node was in this chunk: bin each large child now, from the
+  // records just written and the child's exact centroid bounds.
+  __shared__ uint32_t h[HIST_WORDS];
+  for (int which = 0; which < 2; ++which) {
+    const int32_t cs = nd.child_slot[which];
+    if (cs < 0) continue;  // block-uniform
+    const uint32_t cstart = which == 0 ? nd.start + n_lt : nd.start;
+    const uint32_t ccount = which == 0 ? nd.count - n_lt : n_lt;
+    float cmin3[3], cext3[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      uint32_t mnw = 0, mxw = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        mnw = max(mnw, red[q][6 * which + a]);
+        mxw = max(mxw, red[q][6 * which + 3 + a]);
+      }
+      cmin3[a] = acc_min_dec(mnw);
+      cext3[a] = __fsub_rn(acc_max_dec(mxw), cmin3[a]);
+    }
+    for (uint32_t t = threadIdx.x; t < HIST_WORDS; t += blockDim.x) h[t] = 0;
+    __syncthreads();  // also orders this block's record stores before the reads below
+    for (uint32_t i = cstart + threadIdx.x; i < cstart + ccount; i += blockDim.x) {
+      Box b;
+      uint32_t id;
+      rec_load(&dst[i], b, id);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const uint32_t bi = bin_index(center(b.mn[a], b.mx[a]), cmin3[a], cext3[a]);
+        uint32_t* hb = &h[(a * MG_NBINS + bi) * BIN_WORDS];
+        atomicAdd(&hb[0], 1u);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          atomicMax(&hb[1 + j], acc_min_enc(b.mn[j]));
+          atomicMax(&hb[4 + j], acc_max_enc(b.mx[j]));
+        }
+      }
+    }
+    __syncthreads();
+    uint32_t* gh = hist + (size_t)cs * HIST_WORDS;
+    for (uint32_t t = threadIdx.x; t < HIST_WORDS; t += blockDim.x) gh[t] = h[t];
+    __syncthreads();
   }
 }
 
@@ -1170,7 +1241,7 @@ int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& m
 
   MG_PROF(KK_ROOT, 0);
   k_root<<<nm, 64, 0, st>>>(drows, macc, ctr, L[0], cn[0], ck[0], small, n_small, lp.min_objects,
-                            lrows[0]);
+                            lrows[0], hist);
   MG_PROF(KK_ROOT, 1);
   MG_HIP(hipGetLastError());
 
@@ -1180,7 +1251,6 @@ int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& m
   uint64_t large_nodes = 0;
   while (nL > 0) {
     large_nodes += nL;
-    MG_HIP(hipMemsetAsync(hist, 0, (size_t)nL * HIST_WORDS * 4, st));
     MG_HIP(hipMemsetAsync(ctr, 0, 8, st));  // n_large_next, n_chunks_next
     MG_HIP(hipMemsetAsync(&ctr->n_learned_next, 0, 4, st));
     MG_PROF(KK_BIN, 0);
@@ -1202,7 +1272,7 @@ int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& m
     MG_PROF(KK_SWEEP, 1);
     MG_PROF(KK_PARTITION, 0);
     k_partition<<<nC, 256, 0, st>>>(L[cur], cn[cur], ck[cur], blk, recs[cur], recs[cur ^ 1],
-                                    L[cur ^ 1]);
+                                    L[cur ^ 1], hist);
     MG_PROF(KK_PARTITION, 1);
     MG_HIP(hipGetLastError());
     MG_HIP(hipMemcpyAsync(c->h_ctr, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));

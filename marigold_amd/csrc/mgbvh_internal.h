@@ -37,7 +37,13 @@ struct LNode {
   uint32_t axis, cut, n_lt, mode;     // chosen split; learned-method state (lmode_*)
   int32_t child_slot[2];              // next-level slot of (gt, lt) child, -1 if none
   uint32_t nbase, obase;
+  uint32_t flags, pad3[3];            // LF_* (128 bytes in all)
 };
+
+// LNode::flags.  A node whose parent fits one chunk is binned by the parent's
+// partition block (which knows the node's exact centroid bounds first), so
+// its own binning pass is skipped.
+constexpr uint32_t LF_PREBINNED = 1;
 
 // Learned-method state of a node with > 32 objects (LNode::mode):
 // bits 0-1 kind (0 plain, 1 counted, 2 window, 3 fallback), bits 2-3 predicted

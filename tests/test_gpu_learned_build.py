@@ -111,7 +111,7 @@ def test_learned_tree_valid_hits_identical_sah(ctx, models):
     le = _build(ctx, pos, tris, models["trained"], learned_min_objects=256, learned_window=3,
                 learned_guard=0.02)
     st = le[4]
-    assert st["n_window_splits"] > 500 and st["n_guard_fallbacks"] < st["n_window_splits"]
+    assert st["n_window_splits"] > 200 and st["n_guard_fallbacks"] < st["n_window_splits"]
     _check_tree(pos, tris, *le[:3])
     s_plain, s_learned = _sah(ctx, pl[0], pl[1]), _sah(ctx, le[0], le[1])
     print(f"SAH plain {s_plain:.3f} learned {s_learned:.3f} ratio {s_learned / s_plain:.4f}",
