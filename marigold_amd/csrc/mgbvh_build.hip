@@ -312,23 +312,6 @@ __device__ inline Box hist_box(const uint32_t* hb) {
   return b;
 }
 
-// evaluate_binned_split (lib.rs:255-290) for cut i on axis a.
-__device__ inline void eval_cut(const uint32_t* h, int a, uint32_t i, Box& lt, Box& gt,
-                                uint32_t& nl, uint32_t& ng) {
-  lt = box_empty();
-  gt = box_empty();
-  nl = 0;
-  ng = 0;
-  for (uint32_t b = 0; b < MG_NBINS; ++b) {
-    const uint32_t* hb = &h[(a * MG_NBINS + b) * BIN_WORDS];
-    uint32_t cnt = hb[0];
-    if (!cnt) continue;
-    Box bb = hist_box(hb);
-    if (b < i) { nl += cnt; box_grow(lt, bb); }
-    else { ng += cnt; box_grow(gt, bb); }
-  }
-}
-
 // Child record of a split: a node with > 32 objects for the next level (slot s,
 // chunks [c0, c0 + nch)), or a subtree root of <= 32 objects / a depth-cut leaf.
 __device__ inline void put_child(bool large, uint32_t s, uint32_t c0, uint32_t start,
@@ -370,22 +353,18 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
                                                 NodeOut out, uint32_t max_depth,
                                                 uint32_t parity_next, uint32_t learn_min,
                                                 uint32_t* lrows_next) {
-  __shared__ uint32_t sh[SW_WAVES][HIST_WORDS];
   __shared__ uint32_t s_need[SW_WAVES][4];  // large slots, chunks, small slots, objects
   __shared__ uint32_t s_base[SW_WAVES][3];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t lane = lane_id();
   const uint32_t slot = blockIdx.x * SW_WAVES + w;
   const bool active = slot < nL;  // wave-uniform; every wave reaches the barriers
-  uint32_t* h = sh[w];
   Box lt = box_empty(), gt = box_empty(), nb = box_empty();
   uint32_t nl = 0, ng = 0, axis = 0, value = 0, kind = 0;
   float cost = 0.0f;
   LNode nd = {};
   if (active) {
-  const uint32_t* gh = hist + (size_t)slot * HIST_WORDS;
-  for (uint32_t t = lane; t < HIST_WORDS; t += 64) h[t] = gh[t];
-  wave_sync();
+  const uint32_t* h = hist + (size_t)slot * HIST_WORDS;  // each lane reads its own bins
   nd = L[slot];
   float cmin[3], cext[3];
 #pragma unroll
@@ -404,6 +383,8 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
   // Candidates (axis a, cut i), lane = a * 32 + bin: lt = bins [0, i), gt = bins
   // [i, 32) from 32-lane inclusive prefix / suffix scans (exact min/max unions).
   uint64_t best = ~0ull;
+  Box plt[2], pgt[2];       // each pass's candidate boxes (lt = prefix, gt = suffix)
+  uint32_t pnl[2], png[2];
 #pragma unroll
   for (uint32_t pass = 0; pass < 2; ++pass) {
     const uint32_t q = lane + 64 * pass;
@@ -443,6 +424,10 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
       clt.mx[x] = __shfl_up(pb.mx[x], 1, MG_NBINS);
     }
     const uint32_t cng = sc;
+    plt[pass] = clt;
+    pgt[pass] = sb;
+    pnl[pass] = cnl;
+    png[pass] = cng;
     // lib.rs:339-341 (zero extent refused), 379-381 (MIN objects per side)
     const bool inwin = !win || (a == wa && b >= wlo && b <= whi);
     if (qact && inwin && b >= 1 && cext[a] != 0.0f && cnl >= 1 && cng >= 1) {
@@ -458,7 +443,19 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
     axis = q / MG_NBINS;
     value = q % MG_NBINS;
     kind = 0;
-    eval_cut(h, (int)axis, value, lt, gt, nl, ng);
+    // the winning candidate's boxes and counts, from the lane that evaluated it
+    // (prefix / suffix unions of exact min/max: the same values as summing bins)
+    const int src = (int)(q & 63u);
+    const bool p1 = q >= 64;  // wave-uniform
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      lt.mn[x] = __shfl(p1 ? plt[1].mn[x] : plt[0].mn[x], src, 64);
+      lt.mx[x] = __shfl(p1 ? plt[1].mx[x] : plt[0].mx[x], src, 64);
+      gt.mn[x] = __shfl(p1 ? pgt[1].mn[x] : pgt[0].mn[x], src, 64);
+      gt.mx[x] = __shfl(p1 ? pgt[1].mx[x] : pgt[0].mx[x], src, 64);
+    }
+    nl = __shfl(p1 ? pnl[1] : pnl[0], src, 64);
+    ng = __shfl(p1 ? png[1] : png[0], src, 64);
     cost = split_cost(sa_p, box_sa(lt), box_sa(gt), nl, ng);
   } else {
     // Median fallback (lib.rs:444-491, 527-531).  With > 32 objects it is only
