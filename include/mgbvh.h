@@ -379,6 +379,46 @@ int mgbvh_tree_sah_device(mgbvh_ctx* ctx, const mgbvh_node32* d_nodes, const uin
                           const mgbvh_blas_desc* blas, const uint32_t* node_counts, uint32_t nblas,
                           double* out_sah);
 
+/* ---- scene plans: a whole scene build as one HIP graph ----------------------
+ * No counterpart in the reference (it rebuilds with rayon, model.rs:76-95,
+ * geometry/mod.rs:175-199).  mgbvh_scene_plan_create records every BLAS
+ * level of a batched build (persistent level kernels over device-side work
+ * queues, max_depth levels), the instance boxes and the TLAS build as one
+ * graph over buffers the plan owns; mgbvh_scene_plan_launch replays it (new
+ * vertex positions in the same buffers, same mesh sizes), waits, checks
+ * errors and squeezes depth-cut gaps, and returns the graph's device time.
+ * Outputs as mgbvh_build_blas_batch_device, mgbvh_instance_boxes_device and
+ * mgbvh_build_aabbs_device.  max_depth <= 64; the TLAS uses the plain method;
+ * a learned BLAS method uses the context's model as it was at creation. */
+typedef struct mgbvh_scene_plan mgbvh_scene_plan;
+int mgbvh_scene_plan_create(mgbvh_ctx* ctx, const float* d_positions, uint32_t vstride_bytes,
+                            uint64_t nverts, const uint32_t* d_tris,
+                            const mgbvh_mesh_desc* meshes, uint32_t nmeshes,
+                            const mgbvh_settings* blas_settings, mgbvh_node32* d_blas_nodes,
+                            uint32_t* d_blas_lens, uint32_t* d_blas_perm, uint32_t* d_tris_out,
+                            const float* d_object_to_world, const uint32_t* d_instance_blas,
+                            uint32_t ninstances, const mgbvh_settings* tlas_settings,
+                            mgbvh_aabb* d_boxes, mgbvh_node32* d_tlas_nodes,
+                            uint32_t* d_tlas_lens, uint32_t* d_tlas_perm,
+                            mgbvh_scene_plan** out);
+int mgbvh_scene_plan_launch(mgbvh_scene_plan* plan, float* out_ms,
+                            uint32_t* out_blas_node_counts, uint64_t* out_tlas_nodes);
+int mgbvh_scene_plan_destroy(mgbvh_scene_plan* plan);
+
+/* Persistent grid sizes of the level kernels (blocks per compute unit, 1..64),
+ * the knobs the autotuner (marigold_amd/autotune.py) sets; results never
+ * depend on them. */
+typedef struct mgbvh_tuning {
+  uint32_t bin_blocks_per_cu;
+  uint32_t partition_blocks_per_cu;
+  uint32_t sweep_blocks_per_cu;
+  uint32_t small_blocks_per_cu;
+  uint32_t compute_units;  /* read-only (get) */
+  uint32_t reserved[3];
+} mgbvh_tuning;
+int mgbvh_ctx_set_tuning(mgbvh_ctx* ctx, const mgbvh_tuning* tuning);
+int mgbvh_ctx_get_tuning(mgbvh_ctx* ctx, mgbvh_tuning* tuning);
+
 /* ---- timing helpers: HIP events recorded on the context stream ---- */
 int mgbvh_event_record(mgbvh_ctx* ctx, int slot);           /* slot 0..59 */
 int mgbvh_event_elapsed_ms(mgbvh_ctx* ctx, int a, int b, float* ms);

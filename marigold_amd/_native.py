@@ -45,6 +45,15 @@ MESH_DTYPE = np.dtype([("tri_offset", "<u4"), ("tri_count", "<u4"), ("vertex_off
 AABB_DTYPE = np.dtype([("min", "<f4", (3,)), ("max", "<f4", (3,))])
 
 
+class Tuning(C.Structure):
+    _fields_ = [("bin_blocks_per_cu", C.c_uint32), ("partition_blocks_per_cu", C.c_uint32),
+                ("sweep_blocks_per_cu", C.c_uint32), ("small_blocks_per_cu", C.c_uint32),
+                ("compute_units", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_[:5]}
+
+
 class Camera(C.Structure):
     _fields_ = [("pos", C.c_float * 3), ("fwd", C.c_float * 3), ("right", C.c_float * 3),
                 ("up", C.c_float * 3), ("fov_y", C.c_float), ("aspect", C.c_float)]
@@ -181,6 +190,13 @@ _EXTRA: dict = {
                                   C.POINTER(C.c_float), C.POINTER(C.c_float)], _I),
     "mgbvh_model_infer_device": ([_P, _P, _P, _U32, _P], _I),
     "mgbvh_ctx_set_model": ([_P, _P], _I),
+    "mgbvh_scene_plan_create": ([_P, _P, _U32, _U64, _P, _P, _U32, C.POINTER(Settings), _P, _P,
+                                 _P, _P, _P, _P, _U32, C.POINTER(Settings), _P, _P, _P, _P,
+                                 C.POINTER(_P)], _I),
+    "mgbvh_scene_plan_launch": ([_P, C.POINTER(C.c_float), _P, C.POINTER(C.c_uint64)], _I),
+    "mgbvh_scene_plan_destroy": ([_P], _I),
+    "mgbvh_ctx_set_tuning": ([_P, _P], _I),
+    "mgbvh_ctx_get_tuning": ([_P, _P], _I),
     "mgbvh_tree_sah_device": ([_P, _P, _P, _P, _P, _U32, _P], _I),
     "mgbvh_build_blas_batch_device": ([_P, _P, _U32, _U64, _P, _P, _U32, _P, C.POINTER(Settings),
                                        _P, _P, _P, _P, _P, _P, _P, C.POINTER(Stats)], _I),
@@ -324,6 +340,36 @@ class Model:
             self.handle = None
 
 
+class ScenePlan:
+    """A whole scene build recorded as one HIP graph (mgbvh_scene_plan_*)."""
+
+    def __init__(self, ctx: "Context", pos, vstride, nverts, tris, meshes, blas_settings, blas_nodes,
+                 blas_lens, blas_perm, tris_out, o2w, inst_blas, ninst, tlas_settings, boxes,
+                 tlas_nodes, tlas_lens, tlas_perm):
+        self.ctx = ctx
+        self.meshes = np.ascontiguousarray(meshes, dtype=MESH_DTYPE)
+        self._keep = (blas_settings, tlas_settings)
+        h = C.c_void_p()
+        check(lib().mgbvh_scene_plan_create(
+            ctx.handle, _dv(pos), vstride, nverts, _dv(tris), ptr(self.meshes), len(self.meshes),
+            C.byref(blas_settings), _dv(blas_nodes), _dv(blas_lens), _dv(blas_perm), _dv(tris_out),
+            _dv(o2w), _dv(inst_blas), ninst, C.byref(tlas_settings) if tlas_settings else None,
+            _dv(boxes), _dv(tlas_nodes), _dv(tlas_lens), _dv(tlas_perm), C.byref(h)))
+        self.handle = h
+
+    def launch(self):
+        """Replays the graph; returns (device ms, BLAS node counts, TLAS node count)."""
+        ms, nt = C.c_float(), C.c_uint64()
+        counts = np.zeros(max(len(self.meshes), 1), dtype=np.uint32)
+        check(lib().mgbvh_scene_plan_launch(self.handle, C.byref(ms), ptr(counts), C.byref(nt)))
+        return ms.value, counts[:len(self.meshes)], nt.value
+
+    def close(self):
+        if self.handle is not None:
+            lib().mgbvh_scene_plan_destroy(self.handle)
+            self.handle = None
+
+
 def _dv(b):
     """Device pointer of a DeviceBuffer (None -> NULL)."""
     return None if b is None else b.ptr
@@ -360,6 +406,20 @@ class Context:
 
     KERNEL_KINDS = ["setup", "root", "bin", "sweep", "partition", "small", "compact", "permute",
                     "trace", "learn"]
+
+    def set_tuning(self, **kw):
+        t = self.get_tuning_struct()
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        check(lib().mgbvh_ctx_set_tuning(self.handle, C.byref(t)))
+
+    def get_tuning_struct(self) -> "Tuning":
+        t = Tuning()
+        check(lib().mgbvh_ctx_get_tuning(self.handle, C.byref(t)))
+        return t
+
+    def get_tuning(self) -> dict:
+        return self.get_tuning_struct().as_dict()
 
     def set_model(self, model):
         check(lib().mgbvh_ctx_set_model(self.handle, None if model is None else model.handle))

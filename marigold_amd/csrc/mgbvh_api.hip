@@ -80,6 +80,15 @@ static void free_buf(DevBuf& b) {
   b.cap = 0;
 }
 
+void scratch_free(Scratch& sc) {
+  DevBuf* bufs[] = {&sc.recs[0], &sc.recs[1], &sc.L[0],     &sc.L[1],     &sc.hist,  &sc.blk,
+                    &sc.cn[0],   &sc.cn[1],   &sc.ck[0],    &sc.ck[1],    &sc.small, &sc.ctr,
+                    &sc.gaps,    &sc.lens,    &sc.mrows,    &sc.macc,     &sc.mnb,   &sc.lrows[0],
+                    &sc.lrows[1], &sc.lv,     &sc.lfeat,    &sc.llogit,   &sc.gtab,  &sc.tmpn,
+                    &sc.tmpl,    &sc.tmps};
+  for (DevBuf* b : bufs) free_buf(*b);
+}
+
 static int check_settings(const mgbvh_settings* s, const mgbvh_ctx* c) {
   if (!s) { set_error("settings is NULL"); return MGBVH_ERR_INVALID_ARG; }
   if (s->min_leaf_objects != 1 || s->max_leaf_objects != 1) {
@@ -263,6 +272,7 @@ int mgbvh_ctx_create(int device, mgbvh_ctx** out) {
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
   mgbvh_ctx* c = new mgbvh_ctx();
   c->device = device;
+  c->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate"); }
   for (int i = 0; i < 64; ++i) {
@@ -271,6 +281,7 @@ int mgbvh_ctx_create(int device, mgbvh_ctx** out) {
   }
   e = hipHostMalloc((void**)&c->h_ctr, sizeof(Counters), hipHostMallocDefault);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipHostMalloc"); }
+  c->sc.h_ctr = c->h_ctr;
   *out = c;
   return MGBVH_OK;
 }
@@ -279,10 +290,8 @@ int mgbvh_ctx_destroy(mgbvh_ctx* c) {
   if (!c) return MGBVH_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  DevBuf* bufs[] = {&c->recs[0], &c->recs[1], &c->L[0], &c->L[1], &c->hist, &c->blk,
-                    &c->cn[0], &c->cn[1], &c->ck[0], &c->ck[1], &c->small, &c->ctr,
-                    &c->gaps, &c->gtab, &c->tmpn, &c->tmpl, &c->tmps, &c->lens,
-                    &c->io_pos, &c->io_tris, &c->io_nodes, &c->io_lens, &c->io_perm,
+  scratch_free(c->sc);
+  DevBuf* bufs[] = {&c->io_pos, &c->io_tris, &c->io_nodes, &c->io_lens, &c->io_perm,
                     &c->io_tris_out, &c->io_splits, &c->io_boxes};
   for (DevBuf* b : bufs) free_buf(*b);
   for (int i = 0; i < 64; ++i)

@@ -181,9 +181,15 @@ __global__ __launch_bounds__(256) void k_setup_aabbs(const float* __restrict__ b
 __global__ void k_root(const MeshRow* __restrict__ rows, const uint32_t* __restrict__ macc,
                        Counters* ctr, LNode* L, uint32_t* cnode, uint32_t* ck, SNode* small,
                        uint32_t n_small_roots, uint32_t learn_min, uint32_t* lrows,
-                       uint32_t* hist) {
+                       uint32_t* hist, uint32_t* lv0, uint32_t n_large, uint32_t n_chunks,
+                       uint32_t n_counted) {
   const uint32_t m = blockIdx.x;
-  if (m == 0 && threadIdx.x == 0) ctr->n_small = n_small_roots;
+  if (m == 0 && threadIdx.x == 0) {
+    ctr->n_small = n_small_roots;
+    lv0[LV_NL] = n_large;
+    lv0[LV_NC] = n_chunks;
+    lv0[LV_NCOUNTED] = n_counted;
+  }
   const MeshRow mr = rows[m];
   if (mr.root_kind == 0) return;
   const uint32_t* acc = macc + 12 * (size_t)m;
@@ -235,14 +241,11 @@ __global__ void k_root(const MeshRow* __restrict__ rows, const uint32_t* __restr
 // Learned method: pass 1 counts all three axes of a counted node (no bin
 // boxes); pass 2 adds the bin boxes of its predicted axis (window) or of all
 // axes (fallback).  Plain nodes are binned completely in pass 1.
-__global__ __launch_bounds__(256) void k_bin(const LNode* __restrict__ L,
-                                             const uint32_t* __restrict__ cnode,
-                                             const uint32_t* __restrict__ ck,
-                                             const Rec* __restrict__ recs,
-                                             uint32_t* __restrict__ hist,
-                                             uint32_t* __restrict__ blk, int pass) {
+__device__ inline void bin_chunk(uint32_t c, const LNode* __restrict__ L,
+                                 const uint32_t* __restrict__ cnode,
+                                 const uint32_t* __restrict__ ck, const Rec* __restrict__ recs,
+                                 uint32_t* __restrict__ hist, uint32_t* __restrict__ blk, int pass) {
   __shared__ uint32_t h[HIST_WORDS];
-  const uint32_t c = blockIdx.x;
   const uint32_t slot = cnode[c];
   const uint32_t k = ck[c];
   const LNode& nd = L[slot];
@@ -301,6 +304,22 @@ __global__ __launch_bounds__(256) void k_bin(const LNode* __restrict__ L,
   }
 }
 
+// Persistent over the level's chunk queue (count in device memory): each block
+// takes chunks blockIdx.x, blockIdx.x + gridDim.x, ...
+__global__ __launch_bounds__(256) void k_bin(const LNode* __restrict__ L,
+                                             const uint32_t* __restrict__ cnode,
+                                             const uint32_t* __restrict__ ck,
+                                             const Rec* __restrict__ recs,
+                                             uint32_t* __restrict__ hist,
+                                             uint32_t* __restrict__ blk, int pass,
+                                             const uint32_t* __restrict__ lvc) {
+  const uint32_t nC = lvc[LV_NC];
+  for (uint32_t c = blockIdx.x; c < nC; c += gridDim.x) {
+    bin_chunk(c, L, cnode, ck, recs, hist, blk, pass);
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ sweep
 __device__ inline Box hist_box(const uint32_t* hb) {
   Box b;
@@ -318,7 +337,7 @@ __device__ inline void put_child(bool large, uint32_t s, uint32_t c0, uint32_t s
                                  uint32_t count, const Box& b, uint32_t idx, uint32_t P,
                                  uint32_t depth, uint32_t parity_next, uint32_t nbase,
                                  uint32_t obase, LNode* Ln, SNode* small, uint32_t learn_min,
-                                 Counters* ctr, uint32_t* lrows, bool prebinned) {
+                                 uint32_t* lvn, uint32_t* lrows, bool prebinned) {
   if (large) {
     LNode r = {};
     r.start = start; r.count = count; r.idx = idx; r.P = P; r.depth = depth;
@@ -328,7 +347,7 @@ __device__ inline void put_child(bool large, uint32_t s, uint32_t c0, uint32_t s
     r.nbase = nbase; r.obase = obase;
     r.flags = prebinned ? LF_PREBINNED : 0u;
     r.mode = (!prebinned && learn_min && count >= learn_min) ? LM_COUNTED : LM_PLAIN;
-    if (r.mode == LM_COUNTED) lrows[atomicAdd(&ctr->n_learned_next, 1u)] = s;
+    if (r.mode == LM_COUNTED) lrows[atomicAdd(&lvn[LV_NCOUNTED], 1u)] = s;
     Ln[s] = r;
   } else {
     SNode sn = {};
@@ -345,19 +364,17 @@ constexpr uint32_t SW_WAVES = 16;  // nodes per k_sweep block (one wavefront eac
 // the binned histograms, node record, children.  Children are appended to the
 // next level's lists with one atomic per list per block (a same-address atomic
 // costs ~11 ns, serialised, so per-node atomics would dominate).
-__global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t nL,
-                                                const uint32_t* __restrict__ hist,
-                                                const Rec* __restrict__ recs, LNode* Ln,
-                                                uint32_t* cn_next, uint32_t* ck_next,
-                                                SNode* small, Counters* ctr,
-                                                NodeOut out, uint32_t max_depth,
-                                                uint32_t parity_next, uint32_t learn_min,
-                                                uint32_t* lrows_next) {
+__device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32_t nL,
+                                   const uint32_t* __restrict__ hist,
+                                   const Rec* __restrict__ recs, LNode* Ln, uint32_t* cn_next,
+                                   uint32_t* ck_next, SNode* small, Counters* ctr, NodeOut out,
+                                   uint32_t max_depth, uint32_t parity_next, uint32_t learn_min,
+                                   uint32_t* lrows_next, uint32_t* lvn) {
   __shared__ uint32_t s_need[SW_WAVES][4];  // large slots, chunks, small slots, objects
   __shared__ uint32_t s_base[SW_WAVES][3];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t lane = lane_id();
-  const uint32_t slot = blockIdx.x * SW_WAVES + w;
+  const uint32_t slot = group * SW_WAVES + w;
   const bool active = slot < nL;  // wave-uniform; every wave reaches the barriers
   Box lt = box_empty(), gt = box_empty(), nb = box_empty();
   uint32_t nl = 0, ng = 0, axis = 0, value = 0, kind = 0;
@@ -531,7 +548,7 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
     const uint32_t k = threadIdx.x;
     uint32_t tot = 0;
     for (uint32_t q = 0; q < SW_WAVES; ++q) tot += s_need[q][k];
-    uint32_t* counter = k == 0 ? &ctr->n_large_next : (k == 1 ? &ctr->n_chunks_next : &ctr->n_small);
+    uint32_t* counter = k == 0 ? &lvn[LV_NL] : (k == 1 ? &lvn[LV_NC] : &ctr->n_small);
     uint32_t run = tot ? atomicAdd(counter, tot) : 0;
     for (uint32_t q = 0; q < SW_WAVES; ++q) {
       s_base[q][k] = run;
@@ -549,9 +566,9 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
     out.write_split(nd.idx, axis | (kind << 8), value, nl, fbits(cost));
     if (kind == 2) atomicAdd(&ctr->n_median, 1u);
     put_child(lg, sgs, c0g, nd.start + nl, ng, gt, nd.P, nd.P + 2, cd, parity_next, nd.nbase,
-              nd.obase, Ln, small, learn_min, ctr, lrows_next, nd.nchunk == 1);
+              nd.obase, Ln, small, learn_min, lvn, lrows_next, nd.nchunk == 1);
     put_child(ll, sls, c0l, nd.start, nl, lt, nd.P + 1, nd.P + 2 + 2 * (ng - 1), cd, parity_next,
-              nd.nbase, nd.obase, Ln, small, learn_min, ctr, lrows_next, nd.nchunk == 1);
+              nd.nbase, nd.obase, Ln, small, learn_min, lvn, lrows_next, nd.nchunk == 1);
     L[slot].axis = axis;
     L[slot].cut = value;
     L[slot].n_lt = nl;
@@ -563,20 +580,38 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L, uint32_t 
   for (uint32_t k = lane; k < chl; k += 64) { cn_next[c0l + k] = sls; ck_next[c0l + k] = k; }
 }
 
+// Persistent over the level's node queue, SW_WAVES nodes per block and step.
+// lvc: this level's counts; lvn: the next level's, filled by the allocation.
+__global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L,
+                                                const uint32_t* __restrict__ lvc,
+                                                const uint32_t* __restrict__ hist,
+                                                const Rec* __restrict__ recs, LNode* Ln,
+                                                uint32_t* cn_next, uint32_t* ck_next,
+                                                SNode* small, Counters* ctr, NodeOut out,
+                                                uint32_t max_depth, uint32_t parity_next,
+                                                uint32_t learn_min, uint32_t* lrows_next,
+                                                uint32_t* lvn) {
+  const uint32_t nL = lvc[LV_NL];
+  const uint32_t ngroups = (nL + SW_WAVES - 1) / SW_WAVES;
+  for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    sweep_group(g, L, nL, hist, recs, Ln, cn_next, ck_next, small, ctr, out, max_depth,
+                parity_next, learn_min, lrows_next, lvn);
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ partition
 // Stable partition of one chunk: lt objects keep their order at the front of
 // the node range, gt objects after n_lt.  Offsets of earlier chunks come from
 // their per-bin counts.  Children's centroid bounds are accumulated here.
-__global__ __launch_bounds__(256) void k_partition(const LNode* __restrict__ L,
-                                                   const uint32_t* __restrict__ cnode,
-                                                   const uint32_t* __restrict__ ck,
-                                                   const uint32_t* __restrict__ blk,
-                                                   const Rec* __restrict__ src,
-                                                   Rec* __restrict__ dst, LNode* Ln,
-                                                   uint32_t* __restrict__ hist) {
+__device__ inline void partition_chunk(uint32_t c, const LNode* __restrict__ L,
+                                       const uint32_t* __restrict__ cnode,
+                                       const uint32_t* __restrict__ ck,
+                                       const uint32_t* __restrict__ blk,
+                                       const Rec* __restrict__ src, Rec* __restrict__ dst,
+                                       LNode* Ln, uint32_t* __restrict__ hist) {
   __shared__ uint32_t s_red[4];
   __shared__ uint32_t s_lt[4], s_gt[4];
-  const uint32_t c = blockIdx.x;
   const uint32_t slot = cnode[c];
   const uint32_t k = ck[c];
   const LNode& nd = L[slot];
@@ -732,6 +767,21 @@ __global__ __launch_bounds__(256) void k_partition(const LNode* __restrict__ L,
   }
 }
 
+__global__ __launch_bounds__(256) void k_partition(const LNode* __restrict__ L,
+                                                   const uint32_t* __restrict__ cnode,
+                                                   const uint32_t* __restrict__ ck,
+                                                   const uint32_t* __restrict__ blk,
+                                                   const Rec* __restrict__ src,
+                                                   Rec* __restrict__ dst, LNode* Ln,
+                                                   uint32_t* __restrict__ hist,
+                                                   const uint32_t* __restrict__ lvc) {
+  const uint32_t nC = lvc[LV_NC];
+  for (uint32_t c = blockIdx.x; c < nC; c += gridDim.x) {
+    partition_chunk(c, L, cnode, ck, blk, src, dst, Ln, hist);
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ small subtrees
 // One wavefront builds two subtrees of <= 32 objects (lanes 0-31 and 32-63),
 // level by level.  Every node of a level is a segment of consecutive lanes
@@ -815,19 +865,13 @@ __device__ inline void put_perm(uint32_t pos, uint32_t id, uint32_t obase, uint3
   }
 }
 
-__global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small, uint32_t nS,
-                                               const Rec* __restrict__ rec0,
-                                               const Rec* __restrict__ rec1, NodeOut out,
-                                               uint32_t* __restrict__ perm,
-                                               const uint32_t* __restrict__ tris_in,
-                                               uint32_t* __restrict__ tris_out, Counters* ctr,
-                                               uint32_t* gaps, uint32_t max_depth) {
-  __shared__ SmallObj lds_obj[4][64];
-  const uint32_t w = threadIdx.x >> 6;
+__device__ inline void small_pair(uint32_t wave, SmallObj* S, const SNode* __restrict__ small,
+                                  uint32_t nS, const Rec* __restrict__ rec0,
+                                  const Rec* __restrict__ rec1, NodeOut out,
+                                  uint32_t* __restrict__ perm, const uint32_t* __restrict__ tris_in,
+                                  uint32_t* __restrict__ tris_out, Counters* ctr, uint32_t* gaps,
+                                  uint32_t max_depth) {
   const uint32_t lane = lane_id();
-  SmallObj* S = lds_obj[w];
-  const uint32_t wave = blockIdx.x * 4 + w;
-  if (2 * wave >= nS) return;  // only wave-local synchronisation below
   const uint32_t h = lane >> 5, hl = lane & 31, off = 32 * h;
   const uint32_t e = 2 * wave + h;
 
@@ -1035,6 +1079,24 @@ __global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small, 
   }
 }
 
+// Persistent over the small-subtree queue: each wavefront takes pairs of
+// subtrees (only wave-local synchronisation inside).
+__global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small,
+                                               const uint32_t* __restrict__ nS_dev,
+                                               const Rec* __restrict__ rec0,
+                                               const Rec* __restrict__ rec1, NodeOut out,
+                                               uint32_t* __restrict__ perm,
+                                               const uint32_t* __restrict__ tris_in,
+                                               uint32_t* __restrict__ tris_out, Counters* ctr,
+                                               uint32_t* gaps, uint32_t max_depth) {
+  __shared__ SmallObj lds_obj[4][64];
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t nS = *nS_dev;
+  for (uint32_t wave = blockIdx.x * 4 + w; 2 * wave < nS; wave += gridDim.x * 4)
+    small_pair(wave, lds_obj[w], small, nS, rec0, rec1, out, perm, tris_in, tris_out, ctr, gaps,
+               max_depth);
+}
+
 // ------------------------------------------------------------------ compaction
 // Only after max_depth cut subtrees short.  A cut leaf of n objects at pair
 // base P leaves slots [P, P + 2(n - 1)) unused; inside its mesh, slot i moves
@@ -1104,6 +1166,20 @@ namespace mg {
 
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+__global__ void k_fill(uint32_t* __restrict__ p, uint32_t v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Fill with a kernel, not hipMemsetAsync: the same sequence is captured in graphs.
+static int fill_u32(hipStream_t st, void* p, uint32_t v, uint64_t n) {
+  if (!n) return MGBVH_OK;
+  k_fill<<<std::min<uint64_t>(cdiv(n, 256), 2048), 256, 0, st>>>((uint32_t*)p, v, n);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MGBVH_OK : hip_fail(e, "k_fill");
+}
+
 int build_device(mgbvh_ctx* c, const BuildInput& in, uint32_t n, const mgbvh_settings* s,
                  const BuildOutput& o, BuildResult* res) {
   MeshIn m{};
@@ -1119,191 +1195,267 @@ int build_device(mgbvh_ctx* c, const BuildInput& in, uint32_t n, const mgbvh_set
 int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& meshes,
                 const mgbvh_settings* s, const BuildOutput& o, BuildResult* res) {
   *res = BuildResult{};
-  const uint32_t nm = (uint32_t)meshes.size();
-  res->mesh_nodes.assign(nm, 0);
-  res->mesh_nbase.assign(nm, 0);
-  // plan: chunks, node slots and root slots of every mesh (meshes are contiguous)
-  std::vector<MeshRow> rows(nm);
-  uint64_t n64 = 0, slots64 = 0, setup_chunks = 0, lchunks = 0;
-  uint32_t n_large = 0, n_small = 0;
-  for (uint32_t i = 0; i < nm; ++i) {
+  BuildPlan p;
+  int rc = plan_build(c, c->sc, in, meshes, s, o, &p);
+  if (rc) return rc;
+  res->mesh_nodes.assign(p.nm, 0);
+  res->mesh_nbase.assign(p.nm, 0);
+  for (uint32_t i = 0; i < p.nm; ++i) res->mesh_nbase[i] = p.rows[i].nbase;
+  if (p.n == 0) return MGBVH_OK;
+  hipStream_t st = c->stream;
+  MG_HIP(hipEventRecord(c->ev[62], st));
+  uint32_t levels = 0;
+  if ((rc = enqueue_build(c, c->sc, p, 0, true, &levels))) return rc;
+  if ((rc = finish_build(c, c->sc, p, res))) return rc;
+  MG_HIP(hipEventRecord(c->ev[63], st));
+  MG_HIP(hipEventSynchronize(c->ev[63]));
+  float ms = 0.0f;
+  MG_HIP(hipEventElapsedTime(&ms, c->ev[62], c->ev[63]));
+  if ((rc = prof_collect(c))) return rc;
+  res->ms = ms;
+  return MGBVH_OK;
+}
+
+int plan_build(mgbvh_ctx* c, Scratch& sc, const BuildInput& in, const std::vector<MeshIn>& meshes,
+               const mgbvh_settings* s, const BuildOutput& o, BuildPlan* pp) {
+  BuildPlan& p = *pp;
+  p = BuildPlan{};
+  p.in = in;
+  p.o = o;
+  p.s = *s;
+  p.nm = (uint32_t)meshes.size();
+  // chunks, node slots and root slots of every mesh (meshes are contiguous)
+  p.rows.resize(p.nm);
+  uint64_t n64 = 0, slots64 = 0;
+  for (uint32_t i = 0; i < p.nm; ++i) {
     const MeshIn& m = meshes[i];
     if (m.tri_offset != n64) {
       set_error("meshes must be contiguous: tri_offset of each mesh = sum of earlier counts");
       return MGBVH_ERR_INVALID_ARG;
     }
-    MeshRow& r = rows[i];
+    MeshRow& r = p.rows[i];
     r = MeshRow{};
     r.tri_offset = m.tri_offset;
     r.count = m.count;
     r.vertex_offset = m.vertex_offset;
     r.vertex_count = m.vertex_count;
-    r.setup_chunk0 = (uint32_t)setup_chunks;
+    r.setup_chunk0 = (uint32_t)p.setup_chunks;
     r.nbase = (uint32_t)slots64;
     r.has_bounds = m.has_bounds ? 1u : 0u;
     for (int a = 0; a < 3; ++a) { r.bmn[a] = m.bmn[a]; r.bmx[a] = m.bmx[a]; }
-    res->mesh_nbase[i] = r.nbase;
     if (m.count == 0) {
       r.root_kind = 0;  // lib.rs:703-708: empty BVH
     } else if (m.count > MG_SMALL && s->max_depth > 0) {
       r.root_kind = 1;
-      r.root_slot = n_large++;
-      r.lchunk0 = (uint32_t)lchunks;
-      lchunks += cdiv(m.count, CH);
+      r.root_slot = p.n_large++;
+      r.lchunk0 = (uint32_t)p.lchunks;
+      p.lchunks += cdiv(m.count, CH);
     } else {
       r.root_kind = 2;
-      r.root_slot = n_small++;
+      r.root_slot = p.n_small++;
     }
     n64 += m.count;
     slots64 += m.count ? 2ull * m.count - 1 : 0;
-    setup_chunks += cdiv(m.count, CH);
+    p.setup_chunks += cdiv(m.count, CH);
     if (n64 > 0x7fffffffull || slots64 > 0xffffffffull) {
       set_error("more than 2^31-1 objects in one build");
       return MGBVH_ERR_INVALID_ARG;
     }
   }
-  const uint32_t n = (uint32_t)n64;
-  if (n == 0) return MGBVH_OK;
-  hipStream_t st = c->stream;
-  const uint32_t capL = n / (MG_SMALL + 1) + 2;
-  const uint32_t capC = cdiv(n, CH) + capL + 2;
-  const uint64_t nslots = slots64;
+  p.n = (uint32_t)n64;
+  p.nslots = slots64;
+  if (p.n == 0) return MGBVH_OK;
+  p.capL = p.n / (MG_SMALL + 1) + 2;
+  p.capC = cdiv(p.n, CH) + p.capL + 2;
+  // nodes with > 32 objects have depth < max_depth, one level per depth
+  p.max_levels = std::min<uint32_t>(s->max_depth, p.n);
   int rc;
-  if ((rc = ensure(c->recs[0], (size_t)n * sizeof(Rec)))) return rc;
-  if ((rc = ensure(c->recs[1], (size_t)n * sizeof(Rec)))) return rc;
-  for (int p = 0; p < 2; ++p) {
-    if ((rc = ensure(c->L[p], (size_t)capL * sizeof(LNode)))) return rc;
-    if ((rc = ensure(c->cn[p], (size_t)capC * 4))) return rc;
-    if ((rc = ensure(c->ck[p], (size_t)capC * 4))) return rc;
+  if (s->method == MGBVH_METHOD_LEARNED) {
+    if (!c->model) { set_error("the learned method needs a model"); return MGBVH_ERR_UNSUPPORTED; }
+    p.lp.min_objects = s->learned_min_objects
+                           ? std::max<uint32_t>(s->learned_min_objects, MG_SMALL + 1)
+                           : 512u;
+    p.lp.window = s->learned_window ? s->learned_window : 3u;
+    p.lp.guard = s->learned_guard;
+    p.lp.no_network = s->learned_no_network;
+    for (uint32_t i = 0; i < p.nm; ++i)
+      if (p.rows[i].root_kind == 1 && p.rows[i].count >= p.lp.min_objects)
+        p.rows[i].lrow = p.n_counted++;
+    for (int q = 0; q < 2; ++q)
+      if ((rc = ensure(sc.lrows[q], (size_t)p.capL * 4))) return rc;
+    if ((rc = ensure(sc.lfeat, (size_t)p.capL * MLP_IN * 4)) ||
+        (rc = ensure(sc.llogit, (size_t)p.capL * MLP_OUT * 4)))
+      return rc;
   }
-  if ((rc = ensure(c->hist, (size_t)capL * HIST_WORDS * 4))) return rc;
-  if ((rc = ensure(c->blk, (size_t)capC * 96 * 4))) return rc;
-  if ((rc = ensure(c->small, (size_t)n * sizeof(SNode)))) return rc;
-  if ((rc = ensure(c->ctr, sizeof(Counters)))) return rc;
-  if ((rc = ensure(c->gaps, ((size_t)n / 2 + 1) * 8))) return rc;
-  if ((rc = ensure(c->mrows, (size_t)nm * sizeof(MeshRow)))) return rc;
-  if ((rc = ensure(c->macc, (size_t)nm * 12 * 4))) return rc;
-  uint32_t* lens = o.lens;
-  if (!lens) {
-    if ((rc = ensure(c->lens, nslots * 4))) return rc;
-    lens = (uint32_t*)c->lens.p;
+  const uint32_t n = p.n;
+  if ((rc = ensure(sc.recs[0], (size_t)n * sizeof(Rec)))) return rc;
+  if ((rc = ensure(sc.recs[1], (size_t)n * sizeof(Rec)))) return rc;
+  for (int q = 0; q < 2; ++q) {
+    if ((rc = ensure(sc.L[q], (size_t)p.capL * sizeof(LNode)))) return rc;
+    if ((rc = ensure(sc.cn[q], (size_t)p.capC * 4))) return rc;
+    if ((rc = ensure(sc.ck[q], (size_t)p.capC * 4))) return rc;
   }
-  Rec* recs[2] = {(Rec*)c->recs[0].p, (Rec*)c->recs[1].p};
-  LNode* L[2] = {(LNode*)c->L[0].p, (LNode*)c->L[1].p};
-  uint32_t* cn[2] = {(uint32_t*)c->cn[0].p, (uint32_t*)c->cn[1].p};
-  uint32_t* ck[2] = {(uint32_t*)c->ck[0].p, (uint32_t*)c->ck[1].p};
-  Counters* ctr = (Counters*)c->ctr.p;
-  SNode* small = (SNode*)c->small.p;
-  uint32_t* hist = (uint32_t*)c->hist.p;
-  uint32_t* blk = (uint32_t*)c->blk.p;
-  uint32_t* gaps = (uint32_t*)c->gaps.p;
-  MeshRow* drows = (MeshRow*)c->mrows.p;
-  uint32_t* macc = (uint32_t*)c->macc.p;
-  NodeOut out{o.nodes, lens, o.splits};
+  if ((rc = ensure(sc.hist, (size_t)p.capL * HIST_WORDS * 4))) return rc;
+  if ((rc = ensure(sc.blk, (size_t)p.capC * 96 * 4))) return rc;
+  if ((rc = ensure(sc.small, (size_t)n * sizeof(SNode)))) return rc;
+  if ((rc = ensure(sc.ctr, sizeof(Counters)))) return rc;
+  if ((rc = ensure(sc.gaps, ((size_t)n / 2 + 1) * 8))) return rc;
+  if ((rc = ensure(sc.mrows, (size_t)p.nm * sizeof(MeshRow)))) return rc;
+  if ((rc = ensure(sc.macc, (size_t)p.nm * 12 * 4))) return rc;
+  if ((rc = ensure(sc.lv, ((size_t)p.max_levels + 2) * LV_WORDS * 4))) return rc;
+  p.lens = o.lens;
+  if (!p.lens) {
+    if ((rc = ensure(sc.lens, p.nslots * 4))) return rc;
+    p.lens = (uint32_t*)sc.lens.p;
+  }
+  return MGBVH_OK;
+}
+
+int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels,
+                  bool upload_rows, uint32_t* levels_run) {
+  hipStream_t st = c->stream;
+  const BuildInput& in = p.in;
+  const BuildOutput& o = p.o;
+  const mgbvh_settings* s = &p.s;
+  const uint32_t nm = p.nm, n = p.n;
+  const LearnParams& lp = p.lp;
+  int rc;
+  Rec* recs[2] = {(Rec*)sc.recs[0].p, (Rec*)sc.recs[1].p};
+  LNode* L[2] = {(LNode*)sc.L[0].p, (LNode*)sc.L[1].p};
+  uint32_t* cn[2] = {(uint32_t*)sc.cn[0].p, (uint32_t*)sc.cn[1].p};
+  uint32_t* ck[2] = {(uint32_t*)sc.ck[0].p, (uint32_t*)sc.ck[1].p};
+  uint32_t* lrows[2] = {(uint32_t*)sc.lrows[0].p, (uint32_t*)sc.lrows[1].p};
+  Counters* ctr = (Counters*)sc.ctr.p;
+  SNode* small = (SNode*)sc.small.p;
+  uint32_t* hist = (uint32_t*)sc.hist.p;
+  uint32_t* blk = (uint32_t*)sc.blk.p;
+  uint32_t* gaps = (uint32_t*)sc.gaps.p;
+  uint32_t* lv = (uint32_t*)sc.lv.p;
+  MeshRow* drows = (MeshRow*)sc.mrows.p;
+  uint32_t* macc = (uint32_t*)sc.macc.p;
+  NodeOut out{o.nodes, p.lens, o.splits};
   out.samples = o.samples;
   out.labels = o.labels;
   out.max_samples = o.max_samples;
   out.sample_min = o.sample_min;
+  const bool graph = fixed_levels > 0;
+  const uint32_t cus = (uint32_t)c->cus;
+  const Tuning& tu = c->tune;
 
-  LearnParams lp{0, 0, 0.0f, 0};
-  uint32_t n_counted = 0;
-  if (s->method == MGBVH_METHOD_LEARNED) {
-    if (!c->model) { set_error("the learned method needs a model"); return MGBVH_ERR_UNSUPPORTED; }
-    lp.min_objects = s->learned_min_objects ? std::max<uint32_t>(s->learned_min_objects, MG_SMALL + 1)
-                                            : 512u;
-    lp.window = s->learned_window ? s->learned_window : 3u;
-    lp.guard = s->learned_guard;
-    lp.no_network = s->learned_no_network;
-    for (uint32_t i = 0; i < nm; ++i)
-      if (rows[i].root_kind == 1 && rows[i].count >= lp.min_objects) rows[i].lrow = n_counted++;
-    for (int p = 0; p < 2; ++p)
-      if ((rc = ensure(c->lrows[p], (size_t)capL * 4))) return rc;
-  }
-  uint32_t* lrows[2] = {(uint32_t*)c->lrows[0].p, (uint32_t*)c->lrows[1].p};
-  MG_HIP(hipEventRecord(c->ev[62], st));
-  MG_HIP(hipMemcpyAsync(drows, rows.data(), (size_t)nm * sizeof(MeshRow), hipMemcpyHostToDevice,
-                        st));
-  MG_HIP(hipMemsetAsync(ctr, 0, sizeof(Counters), st));
-  MG_HIP(hipMemsetAsync(macc, 0, (size_t)nm * 12 * 4, st));
-  if (o.splits) MG_HIP(hipMemsetAsync(o.splits, 0xff, nslots * sizeof(mgbvh_split_rec), st));
+  if (upload_rows)
+    MG_HIP(hipMemcpyAsync(drows, p.rows.data(), (size_t)nm * sizeof(MeshRow),
+                          hipMemcpyHostToDevice, st));
+  if ((rc = fill_u32(st, ctr, 0, sizeof(Counters) / 4))) return rc;
+  if ((rc = fill_u32(st, macc, 0, (uint64_t)nm * 12))) return rc;
+  if ((rc = fill_u32(st, lv, 0, ((uint64_t)p.max_levels + 2) * LV_WORDS))) return rc;
+  if (o.splits && (rc = fill_u32(st, o.splits, 0xffffffffu, p.nslots * 4))) return rc;
 
   MG_PROF(KK_SETUP, 0);
   if (in.kind == 0)
-    k_setup_tris<<<(uint32_t)setup_chunks, 256, 0, st>>>(in.pos, in.vstride_floats, in.tris, drows,
-                                                         nm, recs[0], macc, ctr);
+    k_setup_tris<<<(uint32_t)p.setup_chunks, 256, 0, st>>>(in.pos, in.vstride_floats, in.tris,
+                                                           drows, nm, recs[0], macc, ctr);
   else
-    k_setup_aabbs<<<(uint32_t)setup_chunks, 256, 0, st>>>(in.boxes, drows, nm, recs[0], macc, ctr);
+    k_setup_aabbs<<<(uint32_t)p.setup_chunks, 256, 0, st>>>(in.boxes, drows, nm, recs[0], macc,
+                                                            ctr);
   MG_HIP(hipGetLastError());
   MG_PROF(KK_SETUP, 1);
-
   MG_PROF(KK_ROOT, 0);
-  k_root<<<nm, 64, 0, st>>>(drows, macc, ctr, L[0], cn[0], ck[0], small, n_small, lp.min_objects,
-                            lrows[0], hist);
+  k_root<<<nm, 64, 0, st>>>(drows, macc, ctr, L[0], cn[0], ck[0], small, p.n_small,
+                            lp.min_objects, lrows[0], hist, lv, p.n_large,
+                            (uint32_t)p.lchunks, p.n_counted);
   MG_PROF(KK_ROOT, 1);
   MG_HIP(hipGetLastError());
 
-  uint32_t nL = n_large, nC = (uint32_t)lchunks;
+  // level-by-level: exact grids from the counts read back after each level;
+  // fixed levels (graph capture): persistent grids sized for the capacities
+  uint32_t nL = p.n_large, nC = (uint32_t)p.lchunks, n_counted = p.n_counted;
   int cur = 0;
-  uint32_t levels = 0;
-  uint64_t large_nodes = 0;
-  while (nL > 0) {
-    large_nodes += nL;
-    MG_HIP(hipMemsetAsync(ctr, 0, 8, st));  // n_large_next, n_chunks_next
-    MG_HIP(hipMemsetAsync(&ctr->n_learned_next, 0, 4, st));
+  uint32_t level = 0;
+  for (;; ++level) {
+    if (graph) {
+      if (level >= fixed_levels) break;
+      nL = p.capL;
+      nC = p.capC;
+      n_counted = lp.min_objects ? p.capL : 0;
+    } else {
+      if (nL == 0) break;
+      if (level > p.max_levels + 1) { set_error("level loop did not terminate"); return MGBVH_ERR_INTERNAL; }
+    }
+    const uint32_t* lvc = lv + (size_t)level * LV_WORDS;
+    uint32_t* lvn = lv + (size_t)(level + 1) * LV_WORDS;
+    const uint32_t g_bin = std::max<uint32_t>(1, std::min<uint32_t>(nC, cus * tu.bin_bpc));
+    const uint32_t g_part = std::max<uint32_t>(1, std::min<uint32_t>(nC, cus * tu.part_bpc));
+    const uint32_t g_sweep =
+        std::max<uint32_t>(1, std::min<uint32_t>(cdiv(nL, SW_WAVES), cus * tu.sweep_bpc));
     MG_PROF(KK_BIN, 0);
-    k_bin<<<nC, 256, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk, 1);
+    k_bin<<<g_bin, 256, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk, 1, lvc);
     MG_PROF(KK_BIN, 1);
     if (n_counted) {
       MG_PROF(KK_LEARN, 0);
-      if ((rc = learned_decide(c, L[cur], lrows[cur], n_counted, hist, ctr, lp))) return rc;
+      if ((rc = learned_decide(c, sc, L[cur], lrows[cur], lvc + LV_NCOUNTED, n_counted, hist,
+                               ctr, lp)))
+        return rc;
       MG_PROF(KK_LEARN, 1);
       MG_PROF(KK_BIN, 0);
-      k_bin<<<nC, 256, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk, 2);
+      k_bin<<<g_bin, 256, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk, 2, lvc);
       MG_PROF(KK_BIN, 1);
     }
     MG_PROF(KK_SWEEP, 0);
-    k_sweep<<<cdiv(nL, SW_WAVES), 1024, 0, st>>>(L[cur], nL, hist, recs[cur], L[cur ^ 1],
-                                                 cn[cur ^ 1], ck[cur ^ 1], small, ctr, out,
-                                                 s->max_depth, (uint32_t)(cur ^ 1),
-                                                 lp.min_objects, lrows[cur ^ 1]);
+    k_sweep<<<g_sweep, 1024, 0, st>>>(L[cur], lvc, hist, recs[cur], L[cur ^ 1], cn[cur ^ 1],
+                                      ck[cur ^ 1], small, ctr, out, s->max_depth,
+                                      (uint32_t)(cur ^ 1), lp.min_objects, lrows[cur ^ 1], lvn);
     MG_PROF(KK_SWEEP, 1);
     MG_PROF(KK_PARTITION, 0);
-    k_partition<<<nC, 256, 0, st>>>(L[cur], cn[cur], ck[cur], blk, recs[cur], recs[cur ^ 1],
-                                    L[cur ^ 1], hist);
+    k_partition<<<g_part, 256, 0, st>>>(L[cur], cn[cur], ck[cur], blk, recs[cur], recs[cur ^ 1],
+                                        L[cur ^ 1], hist, lvc);
     MG_PROF(KK_PARTITION, 1);
     MG_HIP(hipGetLastError());
-    MG_HIP(hipMemcpyAsync(c->h_ctr, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
-    MG_HIP(hipStreamSynchronize(st));
-    if (c->h_ctr->err) break;
-    nL = c->h_ctr->n_large_next;
-    nC = c->h_ctr->n_chunks_next;
-    n_counted = c->h_ctr->n_learned_next;
+    if (!graph) {
+      MG_HIP(hipMemcpyAsync(&sc.h_ctr->pad[0], lvn, LV_WORDS * 4, hipMemcpyDeviceToHost, st));
+      MG_HIP(hipMemcpyAsync(&sc.h_ctr->err, &ctr->err, 4, hipMemcpyDeviceToHost, st));
+      MG_HIP(hipStreamSynchronize(st));
+      if (sc.h_ctr->err) { ++level; break; }
+      nL = sc.h_ctr->pad[LV_NL];
+      nC = sc.h_ctr->pad[LV_NC];
+      n_counted = sc.h_ctr->pad[LV_NCOUNTED];
+    }
     cur ^= 1;
-    ++levels;
-    if (levels > 4096) { set_error("level loop did not terminate"); return MGBVH_ERR_INTERNAL; }
   }
-  MG_HIP(hipMemcpyAsync(c->h_ctr, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
+  if (levels_run) *levels_run = level;
+  // subtrees of <= 32 objects (and depth-cut leaves), persistent over their queue
+  const uint32_t g_small = std::max<uint32_t>(1, std::min<uint32_t>(cdiv(n, 8), cus * tu.small_bpc));
+  MG_PROF(KK_SMALL, 0);
+  k_small<<<g_small, 256, 0, st>>>(small, &ctr->n_small, recs[0], recs[1], out, o.perm,
+                                   in.kind == 0 ? in.tris : nullptr,
+                                   in.kind == 0 ? o.tris_out : nullptr, ctr, gaps, s->max_depth);
+  MG_PROF(KK_SMALL, 1);
+  MG_HIP(hipGetLastError());
+  return MGBVH_OK;
+}
+
+int finish_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, BuildResult* res) {
+  hipStream_t st = c->stream;
+  const BuildOutput& o = p.o;
+  const uint32_t nm = p.nm;
+  const uint64_t nslots = p.nslots;
+  Counters* ctr = (Counters*)sc.ctr.p;
+  uint32_t* gaps = (uint32_t*)sc.gaps.p;
+  uint32_t* lens = p.lens;
+  int rc;
+  MG_HIP(hipMemcpyAsync(sc.h_ctr, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
   MG_HIP(hipStreamSynchronize(st));
-  if (c->h_ctr->err & ERR_INDEX) { set_error("triangle index out of range"); return MGBVH_ERR_INDEX; }
-  if (c->h_ctr->err & ERR_NONFINITE) {
+  const Counters hc = *sc.h_ctr;
+  if (hc.err & ERR_INDEX) { set_error("triangle index out of range"); return MGBVH_ERR_INDEX; }
+  if (hc.err & ERR_NONFINITE) {
     set_error("non-finite or out-of-range (|x| >= 2^60) input coordinate");
     return MGBVH_ERR_NONFINITE;
   }
-  const uint32_t nS = c->h_ctr->n_small;
-  if (nS) {
-    MG_PROF(KK_SMALL, 0);
-    k_small<<<cdiv(nS, 8), 256, 0, st>>>(small, nS, recs[0], recs[1], out, o.perm,
-                                         in.kind == 0 ? in.tris : nullptr,
-                                         in.kind == 0 ? o.tris_out : nullptr, ctr, gaps,
-                                         s->max_depth);
-    MG_PROF(KK_SMALL, 1);
-    MG_HIP(hipGetLastError());
+  res->mesh_nodes.assign(nm, 0);
+  res->mesh_nbase.assign(nm, 0);
+  for (uint32_t i = 0; i < nm; ++i) {
+    res->mesh_nodes[i] = p.rows[i].count ? 2 * p.rows[i].count - 1 : 0;
+    res->mesh_nbase[i] = p.rows[i].nbase;
   }
-  MG_HIP(hipMemcpyAsync(c->h_ctr, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
-  MG_HIP(hipStreamSynchronize(st));
-  const Counters hc = *c->h_ctr;
-  for (uint32_t i = 0; i < nm; ++i)
-    res->mesh_nodes[i] = meshes[i].count ? 2 * meshes[i].count - 1 : 0;
   uint64_t removed = 0;
   if (hc.n_gaps) {
     std::vector<uint32_t> g(2 * (size_t)hc.n_gaps);
@@ -1319,51 +1471,58 @@ int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& m
       tab[i] = gp[i].first;
       tab[hc.n_gaps + i] = gp[i].second;
       tab[2 * hc.n_gaps + i] = (uint32_t)cum;
-      while (mi + 1 < nm && rows[mi + 1].nbase <= gp[i].first) ++mi;
+      while (mi + 1 < nm && p.rows[mi + 1].nbase <= gp[i].first) ++mi;
       res->mesh_nodes[mi] -= gp[i].second;
     }
     removed = cum;
     std::vector<uint32_t> mnb(nm);
-    for (uint32_t i = 0; i < nm; ++i) mnb[i] = rows[i].nbase;
-    if ((rc = ensure(c->gtab, tab.size() * 4))) return rc;
-    if ((rc = ensure(c->mnb, (size_t)nm * 4))) return rc;
-    if ((rc = ensure(c->tmpn, nslots * sizeof(mgbvh_node32)))) return rc;
-    if ((rc = ensure(c->tmpl, nslots * 4))) return rc;
-    if (o.splits && (rc = ensure(c->tmps, nslots * sizeof(mgbvh_split_rec)))) return rc;
-    MG_HIP(hipMemcpyAsync(c->gtab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, st));
-    MG_HIP(hipMemcpyAsync(c->mnb.p, mnb.data(), mnb.size() * 4, hipMemcpyHostToDevice, st));
-    const uint32_t* gt = (const uint32_t*)c->gtab.p;
+    for (uint32_t i = 0; i < nm; ++i) mnb[i] = p.rows[i].nbase;
+    if ((rc = ensure(sc.gtab, tab.size() * 4))) return rc;
+    if ((rc = ensure(sc.mnb, (size_t)nm * 4))) return rc;
+    if ((rc = ensure(sc.tmpn, nslots * sizeof(mgbvh_node32)))) return rc;
+    if ((rc = ensure(sc.tmpl, nslots * 4))) return rc;
+    if (o.splits && (rc = ensure(sc.tmps, nslots * sizeof(mgbvh_split_rec)))) return rc;
+    MG_HIP(hipMemcpyAsync(sc.gtab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, st));
+    MG_HIP(hipMemcpyAsync(sc.mnb.p, mnb.data(), mnb.size() * 4, hipMemcpyHostToDevice, st));
+    const uint32_t* gt = (const uint32_t*)sc.gtab.p;
     MG_PROF(KK_COMPACT, 0);
     k_compact<<<cdiv(nslots, 256), 256, 0, st>>>(
-        o.nodes, lens, o.splits, (mgbvh_node32*)c->tmpn.p, (uint32_t*)c->tmpl.p,
-        o.splits ? (mgbvh_split_rec*)c->tmps.p : nullptr, (uint32_t)nslots, gt, gt + hc.n_gaps,
-        gt + 2 * hc.n_gaps, hc.n_gaps, (const uint32_t*)c->mnb.p, nm);
+        o.nodes, lens, o.splits, (mgbvh_node32*)sc.tmpn.p, (uint32_t*)sc.tmpl.p,
+        o.splits ? (mgbvh_split_rec*)sc.tmps.p : nullptr, (uint32_t)nslots, gt, gt + hc.n_gaps,
+        gt + 2 * hc.n_gaps, hc.n_gaps, (const uint32_t*)sc.mnb.p, nm);
     MG_PROF(KK_COMPACT, 1);
     MG_HIP(hipGetLastError());
     // every mesh keeps its first slot; its compacted nodes fill the front of its range
-    MG_HIP(hipMemcpyAsync(o.nodes, c->tmpn.p, nslots * sizeof(mgbvh_node32),
+    MG_HIP(hipMemcpyAsync(o.nodes, sc.tmpn.p, nslots * sizeof(mgbvh_node32),
                           hipMemcpyDeviceToDevice, st));
-    MG_HIP(hipMemcpyAsync(lens, c->tmpl.p, nslots * 4, hipMemcpyDeviceToDevice, st));
+    MG_HIP(hipMemcpyAsync(lens, sc.tmpl.p, nslots * 4, hipMemcpyDeviceToDevice, st));
     if (o.splits)
-      MG_HIP(hipMemcpyAsync(o.splits, c->tmps.p, nslots * sizeof(mgbvh_split_rec),
+      MG_HIP(hipMemcpyAsync(o.splits, sc.tmps.p, nslots * sizeof(mgbvh_split_rec),
                             hipMemcpyDeviceToDevice, st));
+    MG_HIP(hipStreamSynchronize(st));
   }
-  MG_HIP(hipEventRecord(c->ev[63], st));
-  MG_HIP(hipEventSynchronize(c->ev[63]));
-  float ms = 0.0f;
-  MG_HIP(hipEventElapsedTime(&ms, c->ev[62], c->ev[63]));
-  if ((rc = prof_collect(c))) return rc;
-  res->nnodes = nslots - removed;
-  res->height = hc.height;
+  // level counts: nodes with > 32 objects, and whether a fixed level count sufficed
+  std::vector<uint32_t> lvh(((size_t)p.max_levels + 2) * LV_WORDS);
+  MG_HIP(hipMemcpy(lvh.data(), sc.lv.p, lvh.size() * 4, hipMemcpyDeviceToHost));
+  uint64_t large_nodes = 0;
+  uint32_t levels = 0;
+  for (uint32_t l = 0; l <= p.max_levels; ++l) {
+    large_nodes += lvh[(size_t)l * LV_WORDS + LV_NL];
+    if (lvh[(size_t)l * LV_WORDS + LV_NL]) levels = l + 1;
+  }
+  if (lvh[((size_t)p.max_levels + 1) * LV_WORDS + LV_NL]) {
+    set_error("builder produced nodes beyond max_depth levels");
+    return MGBVH_ERR_INTERNAL;
+  }
   res->n_levels = levels;
+  res->large_nodes = large_nodes;
+  res->nnodes = nslots - removed;
   res->n_median = hc.n_median;
   res->n_cutoff = hc.n_cutoff;
-  res->large_nodes = large_nodes;
   res->large_visits = hc.sum_large;
   res->n_samples = hc.n_samples;
   res->n_window = hc.n_window;
   res->n_guard = hc.n_fallback;
-  res->ms = ms;
   return MGBVH_OK;
 }
 

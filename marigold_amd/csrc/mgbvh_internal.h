@@ -88,16 +88,16 @@ struct MeshRow {
 };
 
 struct Counters {
-  uint32_t n_large_next, n_chunks_next, n_small, height;
-  uint32_t err, n_gaps, n_median, n_cutoff;
-  uint32_t root_acc[12];  // bounds (~min, max) and centroid bounds (~min, max)
+  uint32_t n_small, err, n_gaps, n_median;
+  uint32_t n_cutoff, n_samples, n_window, n_fallback;
   unsigned long long sum_large;  // objects of all nodes split on the level path
-  uint32_t n_samples;            // split samples written (may exceed the capacity)
-  uint32_t n_learned_next;       // counted (learned) nodes created for the next level
-  uint32_t pad_l;
-  uint32_t n_window, n_fallback; // learned nodes swept in a window / by the plain sweep
-  uint32_t pad[5];
+  uint32_t pad[14];
 };
+
+// Per-level work counts in device memory (LV_WORDS words per level): the
+// level kernels read their own level's counts and the sweep fills the next
+// level's, so a whole build can be enqueued without reading counts back.
+constexpr uint32_t LV_NL = 0, LV_NC = 1, LV_NCOUNTED = 2, LV_WORDS = 4;
 
 // Split proposer network: 102 features -> 128 -> 128 (ReLU) -> 96 logits
 // (axis * 32 + cut).  Features of a node with > 32 objects: log1p(count) /
@@ -211,8 +211,25 @@ struct BuildResult {
 enum KernelKind { KK_SETUP = 0, KK_ROOT, KK_BIN, KK_SWEEP, KK_PARTITION, KK_SMALL, KK_COMPACT,
                   KK_PERMUTE, KK_TRACE, KK_LEARN, KK_COUNT = 16 };
 
+namespace mg {
+// Device scratch of one build: the context keeps one for its calls, a graph
+// plan owns its own (a captured graph must keep its addresses).
+struct Scratch {
+  DevBuf recs[2], L[2], hist, blk, cn[2], ck[2], small, ctr, gaps, lens, mrows, macc, mnb;
+  DevBuf lrows[2], lv, lfeat, llogit, gtab, tmpn, tmpl, tmps;
+  Counters* h_ctr = nullptr;  // pinned
+};
+void scratch_free(Scratch& sc);
+
+// Persistent grid sizes of the level kernels, in blocks per compute unit.
+struct Tuning {
+  uint32_t bin_bpc = 8, part_bpc = 8, sweep_bpc = 2, small_bpc = 16;
+};
+}  // namespace mg
+
 struct mgbvh_ctx {
   int device = 0;
+  int cus = 256;                          // compute units of the device
   uint32_t prof_mask = 0;                 // bit k: time kernels of kind k
   std::vector<hipEvent_t> prof_pool;      // event pairs
   std::vector<int> prof_kind;             // kind of each recorded pair
@@ -220,12 +237,11 @@ struct mgbvh_ctx {
   uint64_t prof_n[KK_COUNT] = {};
   hipStream_t stream = nullptr;
   hipEvent_t ev[64] = {};
-  DevBuf recs[2], L[2], hist, blk, cn[2], ck[2], small, ctr, gaps, gtab, tmpn, tmpl, tmps, lens;
-  DevBuf mrows, macc, mnb;
+  mg::Scratch sc;                 // scratch of this context's builds
   DevBuf io_pos, io_tris, io_nodes, io_lens, io_perm, io_tris_out, io_splits, io_boxes;
-  mg::Counters* h_ctr = nullptr;  // pinned
+  mg::Counters* h_ctr = nullptr;  // pinned, the same as sc.h_ctr
   mgbvh_model* model = nullptr;   // network of the learned build method (not owned)
-  DevBuf lfeat, llogit, lrows[2]; // learned method: features, logits, slot of each row
+  mg::Tuning tune;
 };
 
 namespace mg {
@@ -238,12 +254,41 @@ int build_device(mgbvh_ctx* c, const BuildInput& in, uint32_t n, const mgbvh_set
 // would give it alone, in node slots [nbase, nbase + 2n - 1).
 int build_multi(mgbvh_ctx* c, const BuildInput& in, const std::vector<MeshIn>& meshes,
                 const mgbvh_settings* s, const BuildOutput& o, BuildResult* res);
+
+// A build split into planning (sizes, slots, buffers), enqueueing (every
+// kernel, either level by level with count read-backs or with a fixed level
+// count for graph capture) and finishing (errors, depth-cut compaction).
+struct BuildPlan {
+  BuildInput in{};
+  BuildOutput o{};
+  mgbvh_settings s{};
+  std::vector<MeshRow> rows;
+  uint32_t nm = 0, n = 0, n_large = 0, n_small = 0, n_counted = 0;
+  uint64_t lchunks = 0, setup_chunks = 0, nslots = 0;
+  uint32_t capL = 0, capC = 0, max_levels = 0;
+  LearnParams lp{0, 0, 0.0f, 0};
+  uint32_t* lens = nullptr;  // o.lens or scratch
+};
+int plan_build(mgbvh_ctx* c, Scratch& sc, const BuildInput& in, const std::vector<MeshIn>& meshes,
+               const mgbvh_settings* s, const BuildOutput& o, BuildPlan* p);
+// fixed_levels = 0: level by level with read-backs; else exactly that many levels.
+int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels,
+                  bool upload_rows, uint32_t* levels_run);
+int finish_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, BuildResult* res);
+__global__ void k_fill(uint32_t* __restrict__ p, uint32_t v, uint64_t n);
+__global__ void k_instance_boxes(const mgbvh_node32* __restrict__ nodes,
+                                 const mgbvh_blas_desc* __restrict__ blas, uint32_t nblas,
+                                 const float* __restrict__ o2w,
+                                 const uint32_t* __restrict__ inst_blas, uint32_t ninst,
+                                 mgbvh_aabb* __restrict__ out, uint32_t* err);
 // Event pair around one kernel launch (no-op unless kind is in ctx->prof_mask).
 int prof_begin(mgbvh_ctx* c, int kind);
 int prof_end(mgbvh_ctx* c, int kind);
 int prof_collect(mgbvh_ctx* c);  // after a synchronize: accumulate and recycle
 // Learned method, one level: features of the counted nodes, network
 // inference, window and guard decision (LNode::mode -> window or fallback).
-int learned_decide(mgbvh_ctx* c, LNode* L, const uint32_t* rows, uint32_t n_counted,
-                   const uint32_t* hist, Counters* ctr, const LearnParams& lp);
+// n_dev: the row count in device memory; cap: its bound (grid sizes).
+int learned_decide(mgbvh_ctx* c, Scratch& sc, LNode* L, const uint32_t* rows,
+                   const uint32_t* n_dev, uint32_t cap, const uint32_t* hist, Counters* ctr,
+                   const LearnParams& lp);
 }  // namespace mg

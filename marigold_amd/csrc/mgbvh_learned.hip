@@ -193,18 +193,12 @@ __device__ inline void mlp_layer(const uint16_t* in_lds, const uint16_t* __restr
 
 // Logits of n feature rows (MLP_IN floats each, row stride `stride` floats).
 // One wavefront per 16 rows; activations staged in LDS as BF16.
-__global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ feat, int stride,
-                                                   int n, const uint16_t* __restrict__ wbf,
-                                                   const float* __restrict__ params,
-                                                   float* __restrict__ logits) {
-  __shared__ __align__(16) uint16_t bufA[4][16 * MLP_INP];
-  __shared__ __align__(16) uint16_t bufB[4][16 * MLP_INP];
-  const int w = threadIdx.x >> 6, l = lane_id();
-  const int row0 = (blockIdx.x * 4 + w) * 16;
-  if (row0 >= n) return;  // wave-uniform; only wave-local synchronisation follows
+__device__ inline void mlp_tile(int row0, int n, uint16_t* A, uint16_t* Bf,
+                                const float* __restrict__ feat, int stride,
+                                const uint16_t* __restrict__ wbf, const float* __restrict__ params,
+                                float* __restrict__ logits) {
+  const int l = lane_id();
   const int nrows = min(16, n - row0);
-  uint16_t* A = bufA[w];
-  uint16_t* Bf = bufB[w];
   for (int e = l; e < 16 * MLP_INP; e += 64) {
     const int i = e / MLP_INP, k = e % MLP_INP;
     const float x = (i < nrows && k < MLP_IN) ? feat[(size_t)(row0 + i) * stride + k] : 0.0f;
@@ -225,18 +219,33 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ fea
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   mlp_layer<MLP_OUT, true>(A, W3, params + OFF_B3, nullptr, logits, row0, nrows);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();  // the next tile overwrites A
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Logits of n feature rows (MLP_IN floats each, row stride `stride` floats);
+// n from n_dev when given.  One wavefront per 16 rows, persistent over tiles.
+__global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ feat, int stride,
+                                                   int n_host, const uint32_t* __restrict__ n_dev,
+                                                   const uint16_t* __restrict__ wbf,
+                                                   const float* __restrict__ params,
+                                                   float* __restrict__ logits) {
+  __shared__ __align__(16) uint16_t bufA[4][16 * MLP_INP];
+  __shared__ __align__(16) uint16_t bufB[4][16 * MLP_INP];
+  const int w = threadIdx.x >> 6;
+  const int n = n_dev ? (int)*n_dev : n_host;
+  for (int row0 = (blockIdx.x * 4 + w) * 16; row0 < n; row0 += gridDim.x * 64)
+    mlp_tile(row0, n, bufA[w], bufB[w], feat, stride, wbf, params, logits);
 }
 
 // ------------------------------------------------------------ learned build step
 // One wavefront per network row (a counted node, listed when it was created):
 // its features.
-__global__ __launch_bounds__(256) void k_feat(const LNode* __restrict__ L,
-                                              const uint32_t* __restrict__ rows, uint32_t n,
-                                              const uint32_t* __restrict__ hist,
-                                              float* __restrict__ feat) {
-  const uint32_t row = blockIdx.x * 4 + (threadIdx.x >> 6);
+__device__ inline void feat_row(uint32_t row, const LNode* __restrict__ L,
+                                const uint32_t* __restrict__ rows,
+                                const uint32_t* __restrict__ hist, float* __restrict__ feat) {
   const uint32_t lane = lane_id();
-  if (row >= n) return;
   const uint32_t slot = rows[row];
   const LNode& nd = L[slot];
   const uint32_t* h = hist + (size_t)slot * HIST_WORDS;
@@ -250,6 +259,17 @@ __global__ __launch_bounds__(256) void k_feat(const LNode* __restrict__ L,
   }
 }
 
+// Persistent over the level's network rows (count in device memory).
+__global__ __launch_bounds__(256) void k_feat(const LNode* __restrict__ L,
+                                              const uint32_t* __restrict__ rows,
+                                              const uint32_t* __restrict__ n_dev,
+                                              const uint32_t* __restrict__ hist,
+                                              float* __restrict__ feat) {
+  const uint32_t n = *n_dev;
+  for (uint32_t row = blockIdx.x * 4 + (threadIdx.x >> 6); row < n; row += gridDim.x * 4)
+    feat_row(row, L, rows, hist, feat);
+}
+
 // One wavefront per network row: the best valid (axis, cut) by the logits
 // (valid = both sides non-empty and a non-zero centroid extent, exactly the
 // binned sweep's rule, so a window always holds a valid cut), the window of
@@ -257,14 +277,12 @@ __global__ __launch_bounds__(256) void k_feat(const LNode* __restrict__ L,
 // object of a side inside the node box clipped at the cut plane; a window
 // whose best proxy exceeds (1 + guard) x the best proxy over all valid cuts
 // sends the node to the plain sweep.
-__global__ __launch_bounds__(256) void k_decide(LNode* __restrict__ L,
-                                                const uint32_t* __restrict__ list, uint32_t n,
-                                                const uint32_t* __restrict__ hist,
-                                                const float* __restrict__ logits, uint32_t win,
-                                                float guard, Counters* ctr) {
-  const uint32_t row = blockIdx.x * 4 + (threadIdx.x >> 6);
+__device__ inline void decide_row(uint32_t row, LNode* __restrict__ L,
+                                  const uint32_t* __restrict__ list,
+                                  const uint32_t* __restrict__ hist,
+                                  const float* __restrict__ logits, uint32_t win, float guard,
+                                  Counters* ctr) {
   const uint32_t lane = lane_id();
-  if (row >= n) return;
   const uint32_t slot = list[row];
   const LNode& nd = L[slot];
   const uint32_t* h = hist + (size_t)slot * HIST_WORDS;
@@ -340,6 +358,17 @@ __global__ __launch_bounds__(256) void k_decide(LNode* __restrict__ L,
     L[slot].mode = mode;
     atomicAdd(lmode_kind(mode) == LM_WINDOW ? &ctr->n_window : &ctr->n_fallback, 1u);
   }
+}
+
+__global__ __launch_bounds__(256) void k_decide(LNode* __restrict__ L,
+                                                const uint32_t* __restrict__ list,
+                                                const uint32_t* __restrict__ n_dev,
+                                                const uint32_t* __restrict__ hist,
+                                                const float* __restrict__ logits, uint32_t win,
+                                                float guard, Counters* ctr) {
+  const uint32_t n = *n_dev;
+  for (uint32_t row = blockIdx.x * 4 + (threadIdx.x >> 6); row < n; row += gridDim.x * 4)
+    decide_row(row, L, list, hist, logits, win, guard, ctr);
 }
 
 // ------------------------------------------------------------ host helpers
@@ -421,33 +450,38 @@ int model_bf16(mgbvh_ctx* c, mgbvh_model* m) {
   return MGBVH_OK;
 }
 
-int model_infer(mgbvh_ctx* c, mgbvh_model* m, const float* feat, int stride, int n, float* logits) {
+// n rows (or, with n_dev, at most n rows, the count read on the device).
+int model_infer(mgbvh_ctx* c, mgbvh_model* m, const float* feat, int stride, int n,
+                const uint32_t* n_dev, float* logits) {
   if (n <= 0) return MGBVH_OK;
   int rc = model_bf16(c, m);
   if (rc) return rc;
-  k_mlp_infer<<<cdivu(n, 64), 256, 0, c->stream>>>(feat, stride, n, (const uint16_t*)m->wbf.p,
-                                                     (const float*)m->params.p, logits);
+  const unsigned grid = std::max(1u, std::min<unsigned>(cdivu(n, 64), (unsigned)c->cus * 4));
+  k_mlp_infer<<<grid, 256, 0, c->stream>>>(feat, stride, n, n_dev, (const uint16_t*)m->wbf.p,
+                                           (const float*)m->params.p, logits);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MGBVH_OK : hip_fail(e, "k_mlp_infer");
 }
 
-int learned_decide(mgbvh_ctx* c, LNode* L, const uint32_t* list, uint32_t n_counted,
-                   const uint32_t* hist, Counters* ctr, const LearnParams& lp) {
+int learned_decide(mgbvh_ctx* c, Scratch& sc, LNode* L, const uint32_t* list,
+                   const uint32_t* n_dev, uint32_t cap, const uint32_t* hist, Counters* ctr,
+                   const LearnParams& lp) {
   int rc;
-  if ((rc = ensure(c->lfeat, (size_t)n_counted * MLP_IN * 4)) ||
-      (rc = ensure(c->llogit, (size_t)n_counted * MLP_OUT * 4)))
+  if ((rc = ensure(sc.lfeat, (size_t)cap * MLP_IN * 4)) ||
+      (rc = ensure(sc.llogit, (size_t)cap * MLP_OUT * 4)))
     return rc;
   hipStream_t st = c->stream;
-  float* feat = (float*)c->lfeat.p;
-  float* logits = (float*)c->llogit.p;
-  k_feat<<<cdivu(n_counted, 4), 256, 0, st>>>(L, list, n_counted, hist, feat);
+  float* feat = (float*)sc.lfeat.p;
+  float* logits = (float*)sc.llogit.p;
+  const unsigned grid = std::max(1u, std::min<unsigned>(cdivu(cap, 4), (unsigned)c->cus * 8));
+  k_feat<<<grid, 256, 0, st>>>(L, list, n_dev, hist, feat);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "k_feat");
-  if (!lp.no_network && (rc = model_infer(c, c->model, feat, MLP_IN, (int)n_counted, logits)))
+  if (!lp.no_network &&
+      (rc = model_infer(c, c->model, feat, MLP_IN, (int)cap, n_dev, logits)))
     return rc;
-  k_decide<<<cdivu(n_counted, 4), 256, 0, st>>>(L, list, n_counted, hist,
-                                                 lp.no_network ? nullptr : logits, lp.window,
-                                                 lp.guard, ctr);
+  k_decide<<<grid, 256, 0, st>>>(L, list, n_dev, hist, lp.no_network ? nullptr : logits,
+                                 lp.window, lp.guard, ctr);
   e = hipGetLastError();
   return e == hipSuccess ? MGBVH_OK : hip_fail(e, "k_decide");
 }
@@ -610,7 +644,7 @@ int mgbvh_model_infer_device(mgbvh_ctx* c, mgbvh_model* m, const float* d_featur
   }
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-  int rc = model_infer(c, m, d_features, MLP_IN, (int)n, d_logits);
+  int rc = model_infer(c, m, d_features, MLP_IN, (int)n, nullptr, d_logits);
   if (rc) return rc;
   e = hipStreamSynchronize(c->stream);
   return e == hipSuccess ? MGBVH_OK : hip_fail(e, "k_mlp_infer");

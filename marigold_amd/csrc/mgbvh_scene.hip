@@ -1,5 +1,7 @@
 // Scene-level pieces around the builder: instance world boxes for the TLAS.
 #include <algorithm>
+#include <cstring>
+#include <string>
 #include <vector>
 
 #include "mgbvh_common.h"
@@ -123,18 +125,18 @@ extern "C" int mgbvh_tree_sah_device(mgbvh_ctx* c, const mgbvh_node32* d_nodes,
   }
   if (nslots > 0xffffffffull) { set_error("too many node slots"); return MGBVH_ERR_INVALID_ARG; }
   int rc;
-  if ((rc = ensure(c->gtab, tab.size() * 4)) || (rc = ensure(c->tmps, (size_t)nblas * 8))) return rc;
+  if ((rc = ensure(c->sc.gtab, tab.size() * 4)) || (rc = ensure(c->sc.tmps, (size_t)nblas * 8))) return rc;
   hipStream_t st = c->stream;
-  e = hipMemcpyAsync(c->gtab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemsetAsync(c->tmps.p, 0, (size_t)nblas * 8, st);
+  e = hipMemcpyAsync(c->sc.gtab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemsetAsync(c->sc.tmps.p, 0, (size_t)nblas * 8, st);
   if (e != hipSuccess) return hip_fail(e, "tree sah setup");
-  const uint32_t* t = (const uint32_t*)c->gtab.p;
+  const uint32_t* t = (const uint32_t*)c->sc.gtab.p;
   if (nslots)
     k_tree_sah<<<(uint32_t)((nslots + 255) / 256), 256, 0, st>>>(
-        d_nodes, d_lens, t, t + nblas, nblas, (uint32_t)nslots, (double*)c->tmps.p);
+        d_nodes, d_lens, t, t + nblas, nblas, (uint32_t)nslots, (double*)c->sc.tmps.p);
   e = hipGetLastError();
   if (e == hipSuccess)
-    e = hipMemcpyAsync(out_sah, c->tmps.p, (size_t)nblas * 8, hipMemcpyDeviceToHost, st);
+    e = hipMemcpyAsync(out_sah, c->sc.tmps.p, (size_t)nblas * 8, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   return e == hipSuccess ? MGBVH_OK : hip_fail(e, "k_tree_sah");
 }
@@ -152,9 +154,9 @@ extern "C" int mgbvh_instance_boxes_device(mgbvh_ctx* c, const mgbvh_node32* d_b
   }
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-  int rc = ensure(c->ctr, sizeof(Counters));
+  int rc = ensure(c->sc.ctr, sizeof(Counters));
   if (rc) return rc;
-  uint32_t* flag = reinterpret_cast<uint32_t*>(c->ctr.p);
+  uint32_t* flag = reinterpret_cast<uint32_t*>(c->sc.ctr.p);
   e = hipMemsetAsync(flag, 0, 4, c->stream);
   if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
   k_instance_boxes<<<(ninst + 255) / 256, 256, 0, c->stream>>>(
@@ -169,3 +171,241 @@ extern "C" int mgbvh_instance_boxes_device(mgbvh_ctx* c, const mgbvh_node32* d_b
   }
   return MGBVH_OK;
 }
+
+// ---------------------------------------------------------------- scene plan
+// The whole build of a scene -- every BLAS level, the instance boxes and the
+// TLAS -- recorded once as a HIP graph over buffers the plan owns, replayed for
+// each rebuild (new vertex positions, same mesh sizes).  The level kernels are
+// persistent over device-side work queues and run a fixed number of levels
+// (max_depth: a node with > 32 objects has depth < max_depth), so nothing in
+// the graph depends on counts read back by the host.
+struct mgbvh_scene_plan {
+  mgbvh_ctx* c = nullptr;
+  mg::Scratch bsc, tsc;
+  mg::BuildPlan bp, tp;
+  DevBuf blas_desc, flag;
+  const float* o2w = nullptr;
+  const uint32_t* inst_blas = nullptr;
+  uint32_t nblas = 0, ninst = 0;
+  mgbvh_aabb* boxes = nullptr;
+  mgbvh_node32* blas_nodes = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::vector<uint32_t> blas_counts;
+  uint64_t tlas_nodes = 0;
+};
+
+namespace mg {
+static void plan_release(mgbvh_scene_plan* pl) {
+  if (!pl) return;
+  if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+  if (pl->graph) (void)hipGraphDestroy(pl->graph);
+  if (pl->e0) (void)hipEventDestroy(pl->e0);
+  if (pl->e1) (void)hipEventDestroy(pl->e1);
+  scratch_free(pl->bsc);
+  scratch_free(pl->tsc);
+  for (DevBuf* b : {&pl->blas_desc, &pl->flag})
+    if (b->p) (void)hipFree(b->p);
+  if (pl->bsc.h_ctr) (void)hipHostFree(pl->bsc.h_ctr);
+  if (pl->tsc.h_ctr) (void)hipHostFree(pl->tsc.h_ctr);
+  delete pl;
+}
+
+int model_bf16(mgbvh_ctx* c, mgbvh_model* m);
+}  // namespace mg
+
+extern "C" {
+
+int mgbvh_scene_plan_create(mgbvh_ctx* c, const float* d_pos, uint32_t vstride_bytes,
+                            uint64_t nverts, const uint32_t* d_tris,
+                            const mgbvh_mesh_desc* meshes, uint32_t nmeshes,
+                            const mgbvh_settings* blas_settings, mgbvh_node32* d_blas_nodes,
+                            uint32_t* d_blas_lens, uint32_t* d_blas_perm, uint32_t* d_tris_out,
+                            const float* d_object_to_world, const uint32_t* d_instance_blas,
+                            uint32_t ninstances, const mgbvh_settings* tlas_settings,
+                            mgbvh_aabb* d_boxes, mgbvh_node32* d_tlas_nodes,
+                            uint32_t* d_tlas_lens, uint32_t* d_tlas_perm,
+                            mgbvh_scene_plan** out) {
+  if (!c || !out || !blas_settings || (nmeshes && !meshes)) {
+    set_error("null argument");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  if (blas_settings->has_bounds || blas_settings->profiling_info || blas_settings->max_depth > 64 ||
+      (ninstances && (!tlas_settings || tlas_settings->has_bounds || tlas_settings->profiling_info ||
+                      tlas_settings->max_depth > 64 ||
+                      tlas_settings->method != MGBVH_METHOD_PLAIN))) {
+    set_error("scene plans take union bounds, no profiling JSON, max_depth <= 64 and a plain TLAS");
+    return MGBVH_ERR_UNSUPPORTED;
+  }
+  if (blas_settings->min_leaf_objects != 1 || blas_settings->max_leaf_objects != 1 ||
+      (blas_settings->method == MGBVH_METHOD_LEARNED && !c->model) ||
+      blas_settings->method > MGBVH_METHOD_LEARNED) {
+    set_error("unsupported BLAS settings (leaf sizes, method or missing model)");
+    return MGBVH_ERR_UNSUPPORTED;
+  }
+  if (vstride_bytes < 12 || vstride_bytes % 4) {
+    set_error("vstride_bytes must be a multiple of 4 and >= 12");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  std::vector<MeshIn> ms(nmeshes);
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < nmeshes; ++i) {
+    const mgbvh_mesh_desc& d = meshes[i];
+    if (d.tri_offset != total || (d.tri_count && (uint64_t)d.vertex_offset + d.vertex_count > nverts)) {
+      set_error("mesh " + std::to_string(i) + ": not contiguous or vertices beyond nverts");
+      return MGBVH_ERR_INVALID_ARG;
+    }
+    ms[i] = MeshIn{d.tri_offset, d.tri_count, d.vertex_offset, d.vertex_count, 0, {0, 0, 0},
+                   {0, 0, 0}};
+    total += d.tri_count;
+  }
+  if (total == 0 || !d_pos || !d_tris || !d_blas_nodes || !d_blas_perm ||
+      (ninstances && (!d_object_to_world || !d_instance_blas || !d_boxes || !d_tlas_nodes ||
+                      !d_tlas_perm))) {
+    set_error("empty scene or null device buffer");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  mgbvh_scene_plan* pl = new mgbvh_scene_plan();
+  pl->c = c;
+  pl->nblas = nmeshes;
+  pl->ninst = ninstances;
+  pl->o2w = d_object_to_world;
+  pl->inst_blas = d_instance_blas;
+  pl->boxes = d_boxes;
+  pl->blas_nodes = d_blas_nodes;
+  int rc = MGBVH_OK;
+  auto fail = [&](int r) { plan_release(pl); return r; };
+  if ((e = hipHostMalloc((void**)&pl->bsc.h_ctr, sizeof(Counters), hipHostMallocDefault)) ||
+      (e = hipHostMalloc((void**)&pl->tsc.h_ctr, sizeof(Counters), hipHostMallocDefault)) ||
+      (e = hipEventCreate(&pl->e0)) || (e = hipEventCreate(&pl->e1)))
+    return fail(hip_fail(e, "plan setup"));
+  BuildInput bin{0, d_pos, vstride_bytes / 4, nverts, d_tris, nullptr};
+  BuildOutput bout{d_blas_nodes, d_blas_lens, d_blas_perm, d_tris_out, nullptr};
+  if ((rc = plan_build(c, pl->bsc, bin, ms, blas_settings, bout, &pl->bp))) return fail(rc);
+  std::vector<mgbvh_blas_desc> desc(nmeshes);
+  for (uint32_t i = 0; i < nmeshes; ++i)
+    desc[i] = mgbvh_blas_desc{pl->bp.rows[i].nbase, meshes[i].tri_offset, meshes[i].vertex_offset, 0};
+  if ((rc = ensure(pl->blas_desc, (size_t)std::max(nmeshes, 1u) * sizeof(mgbvh_blas_desc))) ||
+      (rc = ensure(pl->flag, 256)))
+    return fail(rc);
+  if (ninstances) {
+    BuildInput tin{1, nullptr, 0, 0, nullptr, (const float*)d_boxes};
+    BuildOutput tout{d_tlas_nodes, d_tlas_lens, d_tlas_perm, nullptr, nullptr};
+    std::vector<MeshIn> one{MeshIn{0, ninstances, 0, 0, 0, {0, 0, 0}, {0, 0, 0}}};
+    if ((rc = plan_build(c, pl->tsc, tin, one, tlas_settings, tout, &pl->tp))) return fail(rc);
+  }
+  // constant inputs of the graph, outside it
+  hipStream_t st = c->stream;
+  if ((e = hipMemcpyAsync(pl->blas_desc.p, desc.data(), desc.size() * sizeof(mgbvh_blas_desc),
+                          hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(pl->bsc.mrows.p, pl->bp.rows.data(), pl->bp.rows.size() * sizeof(MeshRow),
+                          hipMemcpyHostToDevice, st)) ||
+      (ninstances &&
+       (e = hipMemcpyAsync(pl->tsc.mrows.p, pl->tp.rows.data(), pl->tp.rows.size() * sizeof(MeshRow),
+                           hipMemcpyHostToDevice, st))))
+    return fail(hip_fail(e, "plan upload"));
+  if (blas_settings->method == MGBVH_METHOD_LEARNED && !blas_settings->learned_no_network &&
+      (rc = model_bf16(c, c->model)))
+    return fail(rc);
+  if ((e = hipStreamSynchronize(st))) return fail(hip_fail(e, "plan upload"));
+  // record
+  const uint32_t prof = c->prof_mask;
+  c->prof_mask = 0;
+  e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) { c->prof_mask = prof; return fail(hip_fail(e, "hipStreamBeginCapture")); }
+  uint32_t lv_run = 0;
+  rc = enqueue_build(c, pl->bsc, pl->bp, pl->bp.max_levels ? pl->bp.max_levels : 1, false, &lv_run);
+  if (!rc && ninstances) {
+    k_fill<<<1, 64, 0, st>>>((uint32_t*)pl->flag.p, 0u, 1);
+    k_instance_boxes<<<(ninstances + 255) / 256, 256, 0, st>>>(
+        d_blas_nodes, (const mgbvh_blas_desc*)pl->blas_desc.p, nmeshes, d_object_to_world,
+        d_instance_blas, ninstances, d_boxes, (uint32_t*)pl->flag.p);
+    e = hipGetLastError();
+    if (e != hipSuccess) rc = hip_fail(e, "k_instance_boxes");
+    if (!rc)
+      rc = enqueue_build(c, pl->tsc, pl->tp, pl->tp.max_levels ? pl->tp.max_levels : 1, false,
+                         &lv_run);
+  }
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(st, &g);
+  c->prof_mask = prof;
+  pl->graph = g;
+  if (rc) return fail(rc);
+  if (e != hipSuccess) return fail(hip_fail(e, "hipStreamEndCapture"));
+  e = hipGraphInstantiate(&pl->exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) return fail(hip_fail(e, "hipGraphInstantiate"));
+  *out = pl;
+  return MGBVH_OK;
+}
+
+int mgbvh_scene_plan_launch(mgbvh_scene_plan* pl, float* out_ms, uint32_t* out_blas_node_counts,
+                            uint64_t* out_tlas_nodes) {
+  if (!pl) { set_error("plan is NULL"); return MGBVH_ERR_INVALID_ARG; }
+  mgbvh_ctx* c = pl->c;
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  hipStream_t st = c->stream;
+  if ((e = hipEventRecord(pl->e0, st)) || (e = hipGraphLaunch(pl->exec, st)) ||
+      (e = hipEventRecord(pl->e1, st)) || (e = hipEventSynchronize(pl->e1)))
+    return hip_fail(e, "hipGraphLaunch");
+  float ms = 0.0f;
+  if ((e = hipEventElapsedTime(&ms, pl->e0, pl->e1))) return hip_fail(e, "hipEventElapsedTime");
+  BuildResult r;
+  int rc = finish_build(c, pl->bsc, pl->bp, &r);
+  if (rc) return rc;
+  if (out_blas_node_counts)
+    for (uint32_t i = 0; i < pl->nblas; ++i) out_blas_node_counts[i] = r.mesh_nodes[i];
+  if (pl->ninst) {
+    uint32_t flag = 0;
+    if ((e = hipMemcpy(&flag, pl->flag.p, 4, hipMemcpyDeviceToHost)))
+      return hip_fail(e, "hipMemcpy(flag)");
+    if (flag) { set_error("an instance names a BLAS >= nmeshes"); return MGBVH_ERR_INDEX; }
+    BuildResult rt;
+    if ((rc = finish_build(c, pl->tsc, pl->tp, &rt))) return rc;
+    if (out_tlas_nodes) *out_tlas_nodes = rt.nnodes;
+  } else if (out_tlas_nodes) {
+    *out_tlas_nodes = 0;
+  }
+  if (out_ms) *out_ms = ms;
+  return MGBVH_OK;
+}
+
+int mgbvh_scene_plan_destroy(mgbvh_scene_plan* pl) {
+  if (!pl) return MGBVH_OK;
+  (void)hipSetDevice(pl->c->device);
+  (void)hipStreamSynchronize(pl->c->stream);
+  plan_release(pl);
+  return MGBVH_OK;
+}
+
+int mgbvh_ctx_set_tuning(mgbvh_ctx* c, const mgbvh_tuning* t) {
+  if (!c || !t) { set_error("null argument"); return MGBVH_ERR_INVALID_ARG; }
+  auto ok = [](uint32_t v) { return v >= 1 && v <= 64; };
+  if (!ok(t->bin_blocks_per_cu) || !ok(t->partition_blocks_per_cu) ||
+      !ok(t->sweep_blocks_per_cu) || !ok(t->small_blocks_per_cu)) {
+    set_error("blocks per compute unit must be in [1, 64]");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  c->tune.bin_bpc = t->bin_blocks_per_cu;
+  c->tune.part_bpc = t->partition_blocks_per_cu;
+  c->tune.sweep_bpc = t->sweep_blocks_per_cu;
+  c->tune.small_bpc = t->small_blocks_per_cu;
+  return MGBVH_OK;
+}
+
+int mgbvh_ctx_get_tuning(mgbvh_ctx* c, mgbvh_tuning* t) {
+  if (!c || !t) { set_error("null argument"); return MGBVH_ERR_INVALID_ARG; }
+  memset(t, 0, sizeof *t);
+  t->bin_blocks_per_cu = c->tune.bin_bpc;
+  t->partition_blocks_per_cu = c->tune.part_bpc;
+  t->sweep_blocks_per_cu = c->tune.sweep_bpc;
+  t->small_blocks_per_cu = c->tune.small_bpc;
+  t->compute_units = (uint32_t)c->cus;
+  return MGBVH_OK;
+}
+
+}  // extern "C"

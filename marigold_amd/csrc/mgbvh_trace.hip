@@ -380,9 +380,9 @@ int trace_begin(mgbvh_ctx* c, const void* rays, const void* hits, uint64_t nrays
   }
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-  int rc = ensure(c->ctr, sizeof(Counters));
+  int rc = ensure(c->sc.ctr, sizeof(Counters));
   if (rc) return rc;
-  *flag = reinterpret_cast<uint32_t*>(c->ctr.p);
+  *flag = reinterpret_cast<uint32_t*>(c->sc.ctr.p);
   e = hipMemsetAsync(*flag, 0, 4, c->stream);
   if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
   return MGBVH_OK;

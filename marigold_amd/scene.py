@@ -118,8 +118,23 @@ class GpuScene:
                                                  d["tlas_nodes"], d["tlas_lens"], d["tlas_perm"])
 
     def build(self):
+        if getattr(self, "plan", None) is not None:
+            ms, self.blas_node_counts, self.n_tlas_nodes = self.plan.launch()
+            self.last_graph_ms = ms
+            return
         self.build_blas()
         self.build_tlas()
+
+    def record_graph(self):
+        """Records the whole scene build (batched BLAS levels, instance boxes,
+        TLAS) as one HIP graph; later `build()` calls replay it."""
+        assert self.mode == "batched"
+        d = self.d
+        self.plan = N.ScenePlan(self.ctx, d["pos"], 12, self.positions.shape[0], d["tris"],
+                                self.mesh_desc, self.blas_settings, d["blas_nodes"],
+                                d["blas_lens"], d["blas_perm"], d["tris_perm"], d["o2w"],
+                                d["inst_blas"], self.n_instances, self.tlas_settings, d["boxes"],
+                                d["tlas_nodes"], d["tlas_lens"], d["tlas_perm"])
 
     def blas_sah(self) -> np.ndarray:
         """SAH cost of every BLAS (reference constants, relative to its root)."""
@@ -157,6 +172,9 @@ class GpuScene:
                             self.n_meshes, d["pos"], 12, d["tris_perm"], seed, rays_out)
 
     def free(self):
+        if getattr(self, "plan", None) is not None:
+            self.plan.close()
+            self.plan = None
         for b in self.d.values():
             b.free()
         self.d = {}

@@ -183,3 +183,60 @@ def test_scene_plain_and_batched_identical(ctx):
     assert np.array_equal(tl, O.encode(ref["nodes"]))
     a.free()
     b.free()
+
+
+def _scene_arrays(sc, n):
+    return {name: sc.download(name, dt, cnt(sc)) for name, dt, cnt in (
+        ("blas_nodes", np.uint32, lambda s: 8 * s.n_slots), ("blas_lens", np.uint32, lambda s: s.n_slots),
+        ("blas_perm", np.uint32, lambda s: s.n_tris), ("tris_perm", np.uint32, lambda s: 3 * s.n_tris),
+        ("boxes", np.float32, lambda s: 6 * s.n_instances),
+        ("tlas_nodes", np.uint32, lambda s: 8 * (2 * s.n_instances - 1)),
+        ("tlas_perm", np.uint32, lambda s: s.n_instances))}
+
+
+def test_scene_graph_replays_equal_level_by_level(ctx):
+    # the whole scene build recorded as one HIP graph (persistent level kernels,
+    # fixed level count) gives the level-by-level batched build's arrays, also
+    # on a replay after the vertices moved
+    print("[case] scene graph plan vs level-by-level", flush=True)
+    from marigold_amd.scene import GpuScene
+    sc = scenes.batched_scene(n_meshes=40, n_instances=200, lo=20, hi=30_000, seed=12)
+    meshes = list(zip(sc["positions"], sc["tris"]))
+    ref = GpuScene(ctx, meshes, sc["transforms"], sc["instance_mesh"], mode="batched")
+    want = _scene_arrays(ref, None)
+    g = GpuScene(ctx, meshes, sc["transforms"], sc["instance_mesh"], mode="batched", build=False)
+    g.record_graph()
+    for rep in range(2):
+        g.build()
+        got = _scene_arrays(g, None)
+        for k in want:
+            assert np.array_equal(got[k].view(np.uint32), want[k].view(np.uint32)), (rep, k)
+        assert np.array_equal(g.blas_node_counts, ref.blas_node_counts)
+    # new vertex positions in the same buffers: the replay equals a fresh build
+    moved = [(p * np.float32(1.5) + np.float32(0.25), t) for p, t in meshes]
+    ref2 = GpuScene(ctx, moved, sc["transforms"], sc["instance_mesh"], mode="batched")
+    g.d["pos"].upload(np.ascontiguousarray(np.concatenate([p for p, _ in moved])))
+    g.build()
+    got, want2 = _scene_arrays(g, None), _scene_arrays(ref2, None)
+    for k in want2:
+        assert np.array_equal(got[k].view(np.uint32), want2[k].view(np.uint32)), ("moved", k)
+    for s_ in (ref, g, ref2):
+        s_.free()
+
+
+@pytest.mark.parametrize("tuning", [dict(bin_blocks_per_cu=1, partition_blocks_per_cu=1,
+                                         sweep_blocks_per_cu=1, small_blocks_per_cu=1),
+                                    dict(bin_blocks_per_cu=32, partition_blocks_per_cu=16,
+                                         sweep_blocks_per_cu=4, small_blocks_per_cu=64)])
+def test_tuning_does_not_change_results(ctx, tuning):
+    print(f"[case] tuning {tuning}", flush=True)
+    meshes = _mixed_meshes()
+    base, desc = _batch(ctx, meshes)
+    old = ctx.get_tuning()
+    ctx.set_tuning(**tuning)
+    try:
+        res, _ = _batch(ctx, meshes)
+    finally:
+        ctx.set_tuning(**{k: v for k, v in old.items() if k != "compute_units"})
+    for k in ("nodes", "lens", "splits", "perm", "tris"):
+        assert np.array_equal(np.asarray(res[k]).view(np.uint8), np.asarray(base[k]).view(np.uint8)), k
