@@ -237,6 +237,7 @@ def heightfield_16m_rays(ctx, learned: dict):
 def scene_main(args):
     from marigold_amd import _native as N
     from marigold_amd import scenes
+    from marigold_amd.autotune import autotune
     from marigold_amd.learned import DEFAULT_LEARNED, train_split_model
     from marigold_amd.scene import GpuScene, look_at_camera
 
@@ -251,8 +252,14 @@ def scene_main(args):
     model, train_info = train_split_model(ctx, epochs=(4, 2) if args.quick_train else (30, 15))
     ctx.set_model(model)
     log(f"split proposer trained on the GPU: {train_info}")
-    learned = dict(DEFAULT_LEARNED)
     scene_args = (ctx, meshes, sc["transforms"], sc["instance_mesh"])
+    tuned = None
+    learned = dict(DEFAULT_LEARNED)
+    if not args.no_autotune:
+        tuned = autotune(*scene_args, use_cache=False, log=log)
+        if tuned["learned"]:
+            learned = dict(tuned["learned"])
+        log(f"autotuned: {tuned['tuning']} learned {learned}")
 
     # primary rays into the scene for the hit comparison of the two trees
     w, h = 1024, 1024
@@ -263,6 +270,7 @@ def scene_main(args):
     # ---- learned + batched (the headline)
     lea = GpuScene(*scene_args, mode="batched", build=False, method=N.METHOD_LEARNED,
                    learned=learned)
+    # level by level, with every kernel class timed: which class dominates
     ctx.set_profiling(N.Context.KERNEL_KINDS)
     for _ in range(max(args.warmup, 1)):
         lea.build()
@@ -272,7 +280,7 @@ def scene_main(args):
     _, _, st = ctx.build_blas_batch(d["pos"], 12, nv, d["tris"], lea.mesh_desc, lea.blas_settings,
                                     d["blas_nodes"], d["blas_lens"], d["blas_perm"], d["tris_perm"])
     terms = terms_from_stats(st)
-    log(f"warm-up done; dominant kernel class {dominant}")
+    # level by level, timed, with event pairs around the dominant class only
     ctx.set_profiling([dominant])
     ctx.synchronize()
     ctx.record(0)
@@ -280,21 +288,34 @@ def scene_main(args):
         lea.build()
     ctx.record(1)
     ctx.synchronize()
-    ms_step = ctx.elapsed_ms(0, 1) / args.steps
+    ms_level = ctx.elapsed_ms(0, 1) / args.steps
     kt = ctx.kernel_times(reset=True)[dominant]
     ctx.set_profiling([])
-    dom_ms = kt[0] / args.steps
-    dom_bytes = kernel_bytes(dominant, n, nv, 12, terms)
+    dom_ms = kt[0] / kt[1] if kt[1] else 0.0  # per launch
+    dom_launches = kt[1] / args.steps
+    dom_bytes = kernel_bytes(dominant, n, nv, 12, terms) / max(dom_launches, 1)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     build_bytes = sum(kernel_bytes(k, n, nv, 12, terms) for k in ("setup", "bin", "partition",
                                                                    "sweep", "small"))
+    # the whole scene build as one graph replay: the timed region of `value`
+    lea.record_graph()
+    lea.build()
+    ctx.synchronize()
+    ctx.record(0)
+    for _ in range(args.steps):
+        lea.build()
+    ctx.record(1)
+    ctx.synchronize()
+    ms_step = ctx.elapsed_ms(0, 1) / args.steps
     sah_learned = lea.scene_sah()
     hits_learned = _hits_of(ctx, lea.trace, d_rays, w * h)
-    log(f"learned+batched {ms_step:.2f} ms/scene, SAH {sah_learned:.4f}")
+    log(f"learned+batched {ms_step:.2f} ms/scene (graph), {ms_level:.2f} ms level by level, "
+        f"SAH {sah_learned:.4f}")
     lea.free()
 
     # ---- batched, plain split rule (bit-identical to the per-mesh plain builds)
     bat = GpuScene(*scene_args, mode="batched", build=False)
+    bat.record_graph()
     bat.build()
     ctx.synchronize()
     ctx.record(2)
@@ -306,7 +327,7 @@ def scene_main(args):
     sah_plain = bat.scene_sah()
     hits_plain = _hits_of(ctx, bat.trace, d_rays, w * h)
     bat_sum = int(bat.download("blas_nodes", np.uint32, 8 * bat.n_slots).astype(np.uint64).sum())
-    log(f"batched plain {ms_bat:.2f} ms/scene, SAH {sah_plain:.4f}")
+    log(f"batched plain {ms_bat:.2f} ms/scene (graph), SAH {sah_plain:.4f}")
     bat.free()
 
     # ---- plain: one device build call per mesh (the reference's API shape)
@@ -353,10 +374,13 @@ def scene_main(args):
         "dtype": "f32",
         "data": "synthetic (seeded)",
         "config": {"workload": "2,000-mesh scene (log-uniform 100..200,000 tris, 50/50 soup / "
-                               "heightfield) + TLAS over 10,000 instances; whole scene build",
+                               "heightfield) + TLAS over 10,000 instances; whole scene build "
+                               "replayed as one HIP graph",
                    "tris": n, "verts": nv, "meshes": len(meshes),
                    "instances": len(sc["instance_mesh"]), "max_depth": 32,
-                   "method": "learned + batched", "learned": learned},
+                   "method": "learned + batched", "learned": learned,
+                   "tuning": ctx.get_tuning()},
+        "level_by_level_ms": ms_level,
         "plain_per_mesh": None if ms_plain is None else
         {"value": n / (ms_plain * 1e-3) / 1e6, "ms_per_step": ms_plain, "steps": args.plain_steps,
          "node_arrays_equal_batched_plain": same_nodes},
@@ -365,13 +389,16 @@ def scene_main(args):
         "sah": {"plain": sah_plain, "learned": sah_learned, "ratio": sah_learned / sah_plain},
         "learned_stats": {"window_splits": st["n_window_splits"],
                           "guard_fallbacks": st["n_guard_fallbacks"], "training": train_info},
+        "autotune": None if tuned is None else {k: v for k, v in tuned.items() if k != "trials"},
         "scene_hits_identical": {"rays": w * h, "identical": same_hits(hits_plain, hits_learned),
                                  "hits": int((hits_plain["prim"] != N.MISS).sum())},
         "heightfield_16m_rays": full,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
                      "kernel": dominant, "kernel_ms_per_launch": dom_ms,
-                     "alg_bytes_per_launch": dom_bytes},
+                     "launches_per_build": dom_launches, "alg_bytes_per_launch": dom_bytes,
+                     "timed": "level-by-level builds of the same scene, HIP event pairs around "
+                              "each launch of the kernel class on the build stream"},
         "build_alg_gbs": build_bytes / (ms_step * 1e-3) / 1e9,
         "kernel_ms_per_build": {k: v[0] / max(args.warmup, 1) for k, v in prof.items()},
         "work": terms,
@@ -398,6 +425,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-rays", action="store_true", help="skip the 16M-ray hit comparison")
+    ap.add_argument("--no-autotune", action="store_true", help="default tuning and learned settings")
     ap.add_argument("--quick-train", action="store_true",
                     help="6 training epochs instead of 45 (profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -421,8 +421,11 @@ class Context:
     def get_tuning(self) -> dict:
         return self.get_tuning_struct().as_dict()
 
+    model = None  # the network attached for the learned method
+
     def set_model(self, model):
         check(lib().mgbvh_ctx_set_model(self.handle, None if model is None else model.handle))
+        self.model = model
 
     def set_profiling(self, kinds):
         mask = 0

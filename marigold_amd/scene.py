@@ -129,6 +129,8 @@ class GpuScene:
         """Records the whole scene build (batched BLAS levels, instance boxes,
         TLAS) as one HIP graph; later `build()` calls replay it."""
         assert self.mode == "batched"
+        if getattr(self, "plan", None) is not None:
+            self.plan.close()
         d = self.d
         self.plan = N.ScenePlan(self.ctx, d["pos"], 12, self.positions.shape[0], d["tris"],
                                 self.mesh_desc, self.blas_settings, d["blas_nodes"],
