@@ -414,7 +414,9 @@ typedef struct mgbvh_tuning {
   uint32_t sweep_blocks_per_cu;
   uint32_t small_blocks_per_cu;
   uint32_t compute_units;  /* read-only (get) */
-  uint32_t reserved[3];
+  uint32_t partition_waves; /* wavefronts per partition block: 1 or 4 (0 = keep) */
+  uint32_t bin_waves;       /* wavefronts per binning block: 1..4 (0 = keep) */
+  uint32_t reserved[1];
 } mgbvh_tuning;
 int mgbvh_ctx_set_tuning(mgbvh_ctx* ctx, const mgbvh_tuning* tuning);
 int mgbvh_ctx_get_tuning(mgbvh_ctx* ctx, mgbvh_tuning* tuning);

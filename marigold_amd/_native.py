@@ -48,10 +48,11 @@ AABB_DTYPE = np.dtype([("min", "<f4", (3,)), ("max", "<f4", (3,))])
 class Tuning(C.Structure):
     _fields_ = [("bin_blocks_per_cu", C.c_uint32), ("partition_blocks_per_cu", C.c_uint32),
                 ("sweep_blocks_per_cu", C.c_uint32), ("small_blocks_per_cu", C.c_uint32),
-                ("compute_units", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("compute_units", C.c_uint32), ("partition_waves", C.c_uint32),
+                ("bin_waves", C.c_uint32), ("reserved", C.c_uint32 * 1)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_[:5]}
+        return {k: getattr(self, k) for k, _ in self._fields_[:7]}
 
 
 class Camera(C.Structure):

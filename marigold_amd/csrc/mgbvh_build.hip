@@ -604,14 +604,15 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L,
 // Stable partition of one chunk: lt objects keep their order at the front of
 // the node range, gt objects after n_lt.  Offsets of earlier chunks come from
 // their per-bin counts.  Children's centroid bounds are accumulated here.
+template <int NW>
 __device__ inline void partition_chunk(uint32_t c, const LNode* __restrict__ L,
                                        const uint32_t* __restrict__ cnode,
                                        const uint32_t* __restrict__ ck,
                                        const uint32_t* __restrict__ blk,
                                        const Rec* __restrict__ src, Rec* __restrict__ dst,
                                        LNode* Ln, uint32_t* __restrict__ hist) {
-  __shared__ uint32_t s_red[4];
-  __shared__ uint32_t s_lt[4], s_gt[4];
+  __shared__ uint32_t s_red[NW];
+  __shared__ uint32_t s_lt[NW], s_gt[NW];
   const uint32_t slot = cnode[c];
   const uint32_t k = ck[c];
   const LNode& nd = L[slot];
@@ -632,7 +633,9 @@ __device__ inline void partition_chunk(uint32_t c, const LNode* __restrict__ L,
     part = wave_sum_u32(part);
     if (lane == 0) s_red[w] = part;
     __syncthreads();
-    lt_before = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+    lt_before = 0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) lt_before += s_red[q];
   } else {
     lt_before = min(k * CH, n_lt);
   }
@@ -660,7 +663,8 @@ __device__ inline void partition_chunk(uint32_t c, const LNode* __restrict__ L,
     if (lane == 0) { s_lt[w] = (uint32_t)__popcll(ml); s_gt[w] = (uint32_t)__popcll(mg); }
     __syncthreads();
     uint32_t pl = 0, pg = 0, tl = 0, tg = 0;
-    for (uint32_t q = 0; q < 4; ++q) {
+#pragma unroll
+    for (uint32_t q = 0; q < (uint32_t)NW; ++q) {
       if (q < w) { pl += s_lt[q]; pg += s_gt[q]; }
       tl += s_lt[q];
       tg += s_gt[q];
@@ -688,7 +692,7 @@ __device__ inline void partition_chunk(uint32_t c, const LNode* __restrict__ L,
     __syncthreads();
   }
   // children centroid bounds
-  __shared__ uint32_t red[4][12];
+  __shared__ uint32_t red[NW][12];
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
     acc_lt[j] = wave_max_u32(acc_lt[j]);
@@ -703,8 +707,9 @@ __device__ inline void partition_chunk(uint32_t c, const LNode* __restrict__ L,
   if (threadIdx.x < 12) {
     const int which = threadIdx.x < 6 ? 0 : 1;  // 0 = gt child, 1 = lt child
     const int32_t cs = nd.child_slot[which];
-    uint32_t v = max(max(red[0][threadIdx.x], red[1][threadIdx.x]),
-                     max(red[2][threadIdx.x], red[3][threadIdx.x]));
+    uint32_t v = 0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) v = max(v, red[q][threadIdx.x]);
     if (cs >= 0) {
       if (single) Ln[cs].cacc[threadIdx.x % 6] = v;  // this block saw all of the child
       else if (v) atomicMax(&Ln[cs].cacc[threadIdx.x % 6], v);
@@ -735,7 +740,7 @@ __device__ inline void partition_chunk(uint32_t c, const LNode* __restrict__ L,
     for (int a = 0; a < 3; ++a) {
       uint32_t mnw = 0, mxw = 0;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < NW; ++q) {
         mnw = max(mnw, red[q][6 * which + a]);
         mxw = max(mxw, red[q][6 * which + 3 + a]);
       }
@@ -767,17 +772,20 @@ __device__ inline void partition_chunk(uint32_t c, const LNode* __restrict__ L,
   }
 }
 
-__global__ __launch_bounds__(256) void k_partition(const LNode* __restrict__ L,
-                                                   const uint32_t* __restrict__ cnode,
-                                                   const uint32_t* __restrict__ ck,
-                                                   const uint32_t* __restrict__ blk,
-                                                   const Rec* __restrict__ src,
-                                                   Rec* __restrict__ dst, LNode* Ln,
-                                                   uint32_t* __restrict__ hist,
-                                                   const uint32_t* __restrict__ lvc) {
+// NW wavefronts per block: 1 suits the many small nodes of the lower levels
+// (no block barriers between waves), 4 the large chunks (the tuning picks).
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void k_partition(const LNode* __restrict__ L,
+                                                       const uint32_t* __restrict__ cnode,
+                                                       const uint32_t* __restrict__ ck,
+                                                       const uint32_t* __restrict__ blk,
+                                                       const Rec* __restrict__ src,
+                                                       Rec* __restrict__ dst, LNode* Ln,
+                                                       uint32_t* __restrict__ hist,
+                                                       const uint32_t* __restrict__ lvc) {
   const uint32_t nC = lvc[LV_NC];
   for (uint32_t c = blockIdx.x; c < nC; c += gridDim.x) {
-    partition_chunk(c, L, cnode, ck, blk, src, dst, Ln, hist);
+    partition_chunk<NW>(c, L, cnode, ck, blk, src, dst, Ln, hist);
     __syncthreads();
   }
 }
@@ -1388,7 +1396,8 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
     const uint32_t g_sweep =
         std::max<uint32_t>(1, std::min<uint32_t>(cdiv(nL, SW_WAVES), cus * tu.sweep_bpc));
     MG_PROF(KK_BIN, 0);
-    k_bin<<<g_bin, 256, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk, 1, lvc);
+    k_bin<<<g_bin, 64 * tu.bin_waves, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk, 1,
+                                                lvc);
     MG_PROF(KK_BIN, 1);
     if (n_counted) {
       MG_PROF(KK_LEARN, 0);
@@ -1397,7 +1406,8 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
         return rc;
       MG_PROF(KK_LEARN, 1);
       MG_PROF(KK_BIN, 0);
-      k_bin<<<g_bin, 256, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk, 2, lvc);
+      k_bin<<<g_bin, 64 * tu.bin_waves, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk,
+                                                  2, lvc);
       MG_PROF(KK_BIN, 1);
     }
     MG_PROF(KK_SWEEP, 0);
@@ -1406,8 +1416,12 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
                                       (uint32_t)(cur ^ 1), lp.min_objects, lrows[cur ^ 1], lvn);
     MG_PROF(KK_SWEEP, 1);
     MG_PROF(KK_PARTITION, 0);
-    k_partition<<<g_part, 256, 0, st>>>(L[cur], cn[cur], ck[cur], blk, recs[cur], recs[cur ^ 1],
-                                        L[cur ^ 1], hist, lvc);
+    if (tu.part_waves == 1)
+      k_partition<1><<<g_part, 64, 0, st>>>(L[cur], cn[cur], ck[cur], blk, recs[cur],
+                                            recs[cur ^ 1], L[cur ^ 1], hist, lvc);
+    else
+      k_partition<4><<<g_part, 256, 0, st>>>(L[cur], cn[cur], ck[cur], blk, recs[cur],
+                                             recs[cur ^ 1], L[cur ^ 1], hist, lvc);
     MG_PROF(KK_PARTITION, 1);
     MG_HIP(hipGetLastError());
     if (!graph) {

@@ -223,7 +223,8 @@ void scratch_free(Scratch& sc);
 
 // Persistent grid sizes of the level kernels, in blocks per compute unit.
 struct Tuning {
-  uint32_t bin_bpc = 8, part_bpc = 8, sweep_bpc = 2, small_bpc = 16;
+  uint32_t bin_bpc = 32, part_bpc = 32, sweep_bpc = 4, small_bpc = 16;
+  uint32_t part_waves = 1, bin_waves = 4;  // wavefronts per block
 };
 }  // namespace mg
 

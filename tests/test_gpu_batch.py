@@ -225,9 +225,11 @@ def test_scene_graph_replays_equal_level_by_level(ctx):
 
 
 @pytest.mark.parametrize("tuning", [dict(bin_blocks_per_cu=1, partition_blocks_per_cu=1,
-                                         sweep_blocks_per_cu=1, small_blocks_per_cu=1),
+                                         sweep_blocks_per_cu=1, small_blocks_per_cu=1,
+                                         partition_waves=4, bin_waves=1),
                                     dict(bin_blocks_per_cu=32, partition_blocks_per_cu=16,
-                                         sweep_blocks_per_cu=4, small_blocks_per_cu=64)])
+                                         sweep_blocks_per_cu=4, small_blocks_per_cu=64,
+                                         partition_waves=1, bin_waves=2)])
 def test_tuning_does_not_change_results(ctx, tuning):
     print(f"[case] tuning {tuning}", flush=True)
     meshes = _mixed_meshes()
