@@ -797,58 +797,119 @@ __global__ __launch_bounds__(NW * 64) void k_partition(const LNode* __restrict__
 // once: segmented scans give centroid bounds and the first-minimum split, each
 // lane evaluates the 3 candidates (axis a, threshold r) of its list position
 // (adaptive sweep, lib.rs:391-442), and a stable in-segment partition moves the
-// objects for the next level.
+// objects for the next level.  A triangle's three vertex indices travel with
+// it from the first load, so leaves store the permuted triangle list without
+// another global read.
 struct __align__(16) SmallObj {
   float4 a;  // c0 c1 c2 mn0
   float4 b;  // mn1 mn2 mx0 mx1
-  float4 d;  // mx2 id - -
+  float4 d;  // mx2 id tri0 tri1
+  float4 e;  // tri2 - - -
 };
 
-__device__ inline uint32_t seg_max_u32(uint32_t v, uint32_t lane, uint32_t ls, uint32_t last) {
-#pragma unroll
-  for (uint32_t d = 1; d < 64; d <<= 1) {
-    const uint32_t o = __shfl_up(v, d, 64);
-    if (lane >= ls + d) v = max(v, o);
-  }
-  return __shfl(v, (int)last, 64);
+// Segmented inclusive scans inside 32-lane halves (a segment [ls, last] never
+// crosses lane 31/32) with DPP row shifts: row_shr 1, 2, 4, 8 inside each
+// 16-lane row, then row_bcast:15 carries lane 15 (47) into row 1 (3) for
+// segments that started in the row before.  Lanes outside the segment see the
+// identity `id`.
+template <int CTRL, int ROWS>
+__device__ inline uint32_t dpp_mov(uint32_t id, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, CTRL, ROWS, 0xF, false);
 }
 
+template <class Op>
+__device__ inline uint32_t seg_scan32(uint32_t v, uint32_t id, uint32_t lane, uint32_t ls, Op op) {
+  const uint32_t rs = lane & ~15u, s = ls > rs ? ls : rs;
+  uint32_t t;
+  t = dpp_mov<0x111, 0xF>(id, v); if (lane >= s + 1) v = op(v, t);
+  t = dpp_mov<0x112, 0xF>(id, v); if (lane >= s + 2) v = op(v, t);
+  t = dpp_mov<0x114, 0xF>(id, v); if (lane >= s + 4) v = op(v, t);
+  t = dpp_mov<0x118, 0xF>(id, v); if (lane >= s + 8) v = op(v, t);
+  t = dpp_mov<0x142, 0xA>(id, v); if (ls < rs) v = op(v, t);
+  return v;
+}
+
+// Minimum of u64 keys over the segment, returned on every lane of it.
 __device__ inline uint64_t seg_min_u64(uint64_t v, uint32_t lane, uint32_t ls, uint32_t last) {
-#pragma unroll
-  for (uint32_t d = 1; d < 64; d <<= 1) {
-    const uint64_t o = __shfl_up(v, d, 64);
-    if (lane >= ls + d) v = o < v ? o : v;
-  }
+  const uint32_t rs = lane & ~15u, s = ls > rs ? ls : rs;
+  uint64_t t;
+#define MG_SEG_MIN_STEP(C, R, COND)                                                   \
+  t = ((uint64_t)dpp_mov<C, R>(0xFFFFFFFFu, (uint32_t)(v >> 32)) << 32) |             \
+      dpp_mov<C, R>(0xFFFFFFFFu, (uint32_t)v);                                        \
+  if (COND) v = t < v ? t : v;
+  MG_SEG_MIN_STEP(0x111, 0xF, lane >= s + 1)
+  MG_SEG_MIN_STEP(0x112, 0xF, lane >= s + 2)
+  MG_SEG_MIN_STEP(0x114, 0xF, lane >= s + 4)
+  MG_SEG_MIN_STEP(0x118, 0xF, lane >= s + 8)
+  MG_SEG_MIN_STEP(0x142, 0xA, ls < rs)
+#undef MG_SEG_MIN_STEP
   return __shfl(v, (int)last, 64);
 }
 
-// Union of the boxes of lanes [ls, last] that have `take`.
+// Union of the boxes of lanes [ls, last] that have `take`, on every lane of
+// the segment.  Box coordinates are finite and carry no -0 (object boxes are
+// canonicalised at setup, centroids of such boxes cannot be -0), so the fold
+// order does not change the result and fminf/fmaxf equal the x86 min/max.
 __device__ inline Box seg_box(bool take, const float* mn, const float* mx, uint32_t lane,
                               uint32_t ls, uint32_t last) {
+  const auto fmn = [](uint32_t a, uint32_t b) { return fbits(fminf(bitsf(a), bitsf(b))); };
+  const auto fmx = [](uint32_t a, uint32_t b) { return fbits(fmaxf(bitsf(a), bitsf(b))); };
+  uint32_t lo[3], hi[3];
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    lo[x] = seg_scan32(take ? fbits(mn[x]) : 0x7f800000u, 0x7f800000u, lane, ls, fmn);
+    hi[x] = seg_scan32(take ? fbits(mx[x]) : 0xff800000u, 0xff800000u, lane, ls, fmx);
+  }
   Box r;
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
-    r.mn[x] = acc_min_dec(seg_max_u32(take ? acc_min_enc(mn[x]) : 0u, lane, ls, last));
-    r.mx[x] = acc_max_dec(seg_max_u32(take ? acc_max_enc(mx[x]) : 0u, lane, ls, last));
+    r.mn[x] = bitsf((uint32_t)__shfl((int)lo[x], (int)last, 64));
+    r.mx[x] = bitsf((uint32_t)__shfl((int)hi[x], (int)last, 64));
   }
   return r;
 }
 
+// Wave-wide maximum of a value in [0, 63] from six ballots.
+__device__ inline uint32_t wave_max6(uint32_t v) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int b = 5; b >= 0; --b) {
+    const uint32_t t = m | (1u << b);
+    if (__ballot(v >= t)) m = t;
+  }
+  return m;
+}
+
 struct SmallLane {
   float c[3], mn[3], mx[3];
-  uint32_t id;
+  uint32_t id, tri[3];
 };
 
 __device__ inline void obj_put(SmallObj* S, uint32_t pos, const SmallLane& o) {
   S[pos].a = make_float4(o.c[0], o.c[1], o.c[2], o.mn[0]);
   S[pos].b = make_float4(o.mn[1], o.mn[2], o.mx[0], o.mx[1]);
-  S[pos].d = make_float4(o.mx[2], __uint_as_float(o.id), 0.0f, 0.0f);
+  S[pos].d = make_float4(o.mx[2], __uint_as_float(o.id), __uint_as_float(o.tri[0]),
+                         __uint_as_float(o.tri[1]));
+  S[pos].e = make_float4(__uint_as_float(o.tri[2]), 0.0f, 0.0f, 0.0f);
 }
 __device__ inline void obj_get(const SmallObj* S, uint32_t pos, SmallLane& o) {
-  const float4 a = S[pos].a, b = S[pos].b, d = S[pos].d;
+  const float4 a = S[pos].a, b = S[pos].b, d = S[pos].d, e = S[pos].e;
   o.c[0] = a.x; o.c[1] = a.y; o.c[2] = a.z; o.mn[0] = a.w;
   o.mn[1] = b.x; o.mn[2] = b.y; o.mx[0] = b.z; o.mx[1] = b.w;
   o.mx[2] = d.x; o.id = __float_as_uint(d.y);
+  o.tri[0] = __float_as_uint(d.z); o.tri[1] = __float_as_uint(d.w); o.tri[2] = __float_as_uint(e.x);
+}
+// The centroid and box of the object at `pos` (what the candidate loop reads).
+struct SmallGeo {
+  float4 a, b;
+  float mx2;
+};
+__device__ inline SmallGeo geo_get(const SmallObj* S, uint32_t pos) {
+  SmallGeo g;
+  g.a = S[pos].a;
+  g.b = S[pos].b;
+  g.mx2 = S[pos].d.x;
+  return g;
 }
 
 // Moves each lane's object to `newpos` (a permutation of the lanes).
@@ -872,6 +933,15 @@ __device__ inline void put_perm(uint32_t pos, uint32_t id, uint32_t obase, uint3
     tris_out[3 * (size_t)pos + 2] = c;
   }
 }
+__device__ inline void put_leaf(uint32_t pos, const SmallLane& o, uint32_t obase, uint32_t* perm,
+                                uint32_t* tris_out) {
+  perm[pos] = o.id - obase;
+  if (tris_out) {
+    tris_out[3 * (size_t)pos] = o.tri[0];
+    tris_out[3 * (size_t)pos + 1] = o.tri[1];
+    tris_out[3 * (size_t)pos + 2] = o.tri[2];
+  }
+}
 
 __device__ inline void small_pair(uint32_t wave, SmallObj* S, const SNode* __restrict__ small,
                                   uint32_t nS, const Rec* __restrict__ rec0,
@@ -886,7 +956,7 @@ __device__ inline void small_pair(uint32_t wave, SmallObj* S, const SNode* __res
   bool has = false;
   SmallLane o;
 #pragma unroll
-  for (int x = 0; x < 3; ++x) { o.c[x] = 0.0f; o.mn[x] = 0.0f; o.mx[x] = 0.0f; }
+  for (int x = 0; x < 3; ++x) { o.c[x] = 0.0f; o.mn[x] = 0.0f; o.mx[x] = 0.0f; o.tri[x] = 0; }
   o.id = 0;
   uint32_t ls = lane, lc = 1, idx = 0, P = 0, depth = 0, base = 0, nbase = 0, obase = 0;
   Box nb = box_empty();
@@ -910,6 +980,11 @@ __device__ inline void small_pair(uint32_t wave, SmallObj* S, const SNode* __res
     } else if (hl < sn.count) {
       Box b;
       rec_load(&src[sn.start + hl], b, o.id);
+      if (tris_out) {
+        o.tri[0] = tris_in[3 * (size_t)o.id];
+        o.tri[1] = tris_in[3 * (size_t)o.id + 1];
+        o.tri[2] = tris_in[3 * (size_t)o.id + 2];
+      }
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
         o.mn[x] = b.mn[x];
@@ -939,7 +1014,7 @@ __device__ inline void small_pair(uint32_t wave, SmallObj* S, const SNode* __res
           atomicAdd(&ctr->n_cutoff, 1u);
         }
       }
-      put_perm(base + lane - off, o.id, obase, perm, tris_in, tris_out);
+      put_leaf(base + lane - off, o, obase, perm, tris_out);
       has = false;
     }
     const bool act = has;
@@ -947,32 +1022,37 @@ __device__ inline void small_pair(uint32_t wave, SmallObj* S, const SNode* __res
     const uint32_t last = ls + lc - 1;
     const uint32_t r = lane - ls;
     // centroid bounds of each node (lib.rs:499-509)
-    Box cb = seg_box(act, o.c, o.c, lane, ls, last);
     const float sa_p = box_sa(nb);
-    float th[3];
+    float th[3] = {o.c[0], o.c[1], o.c[2]};
+    if (__ballot(act && lc >= 8)) {
+      const Box cb = seg_box(act, o.c, o.c, lane, ls, last);
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-      th[a] = lc < 8 ? o.c[a] : adaptive_threshold(cb.mn[a], cb.mx[a], lc, r);
+      for (int a = 0; a < 3; ++a)
+        if (lc >= 8) th[a] = adaptive_threshold(cb.mn[a], cb.mx[a], lc, r);
+    }
     // the three candidates (a, r) of this lane over the node's objects
     Box lt[3], gt[3];
     uint32_t nl[3], ng[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) { lt[a] = box_empty(); gt[a] = box_empty(); nl[a] = 0; ng[a] = 0; }
-    const uint32_t maxlc = wave_max_u32(act ? lc : 0u);
+    const uint32_t maxlc = wave_max6(act ? lc : 0u);
+    // the next object's LDS reads are issued before the current one is folded in
+    SmallGeo g = geo_get(S, act ? ls : lane);
     for (uint32_t k = 0; k < maxlc; ++k) {
       const bool v = act && k < lc;
-      SmallLane q;
-      obj_get(S, v ? ls + k : lane, q);
+      const SmallGeo gn = geo_get(S, act && k + 1 < lc ? ls + k + 1 : lane);
       if (v) {
+        const float qc[3] = {g.a.x, g.a.y, g.a.z};
         Box qb;
-#pragma unroll
-        for (int x = 0; x < 3; ++x) { qb.mn[x] = q.mn[x]; qb.mx[x] = q.mx[x]; }
+        qb.mn[0] = g.a.w; qb.mn[1] = g.b.x; qb.mn[2] = g.b.y;
+        qb.mx[0] = g.b.z; qb.mx[1] = g.b.w; qb.mx[2] = g.mx2;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          if (q.c[a] < th[a]) { box_grow(lt[a], qb); ++nl[a]; }
+          if (qc[a] < th[a]) { box_grow(lt[a], qb); ++nl[a]; }
           else { box_grow(gt[a], qb); ++ng[a]; }
         }
       }
+      g = gn;
     }
     uint64_t key = ~0ull;
     int ba = 0;
@@ -1051,6 +1131,13 @@ __device__ inline void small_pair(uint32_t wave, SmallObj* S, const SNode* __res
       }
     }
     if (act && !med) cost = split_cost(sa_p, box_sa(clt), box_sa(cgt), n_lt, lc - n_lt);
+    // a split always leaves both children non-empty; anything else is a bug and
+    // stops this subtree rather than writing outside its slots
+    const bool bad = act && (n_lt == 0 || n_lt >= lc);
+    if (__ballot(bad)) {
+      if (bad) atomicOr(&ctr->err, ERR_INTERNAL);
+      break;
+    }
 
     // stable partition inside each node (partition_in_place, lib.rs:598-600)
     {
@@ -1464,6 +1551,7 @@ int finish_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, BuildResult* res) {
     set_error("non-finite or out-of-range (|x| >= 2^60) input coordinate");
     return MGBVH_ERR_NONFINITE;
   }
+  if (hc.err & ERR_INTERNAL) { set_error("small-subtree split count out of range"); return MGBVH_ERR_INTERNAL; }
   res->mesh_nodes.assign(nm, 0);
   res->mesh_nbase.assign(nm, 0);
   for (uint32_t i = 0; i < nm; ++i) {

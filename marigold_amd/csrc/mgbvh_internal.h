@@ -16,6 +16,7 @@ constexpr uint32_t HIST_WORDS = 3 * 32 * BIN_WORDS;  // one node: 3 axes x 32 bi
 
 constexpr uint32_t ERR_INDEX = 1;
 constexpr uint32_t ERR_NONFINITE = 2;
+constexpr uint32_t ERR_INTERNAL = 4;   // a device-side invariant failed (split count out of range)
 
 // One object during the build: its box and original index (32 bytes, two dwordx4).
 struct __align__(16) Rec {
