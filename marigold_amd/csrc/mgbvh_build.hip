@@ -53,6 +53,182 @@ __device__ inline float rl(float v, uint32_t k) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)k));
 }
 
+// min/max of coordinates that are finite and never -0 (object boxes are
+// canonicalised at setup; centroids and unions of such boxes, and the empty
+// box's infinities, keep that): the IEEE minimum/maximum then give the bits of
+// the x86 min/max, in one instruction without operand canonicalisation.
+__device__ inline float min_nz(float a, float b) { return __builtin_elementwise_minimum(a, b); }
+__device__ inline float max_nz(float a, float b) { return __builtin_elementwise_maximum(a, b); }
+__device__ inline void box_grow_nz(Box& acc, const Box& o) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    acc.mn[a] = min_nz(acc.mn[a], o.mn[a]);
+    acc.mx[a] = max_nz(acc.mx[a], o.mx[a]);
+  }
+}
+
+// Segmented inclusive scans inside 32-lane halves (a segment [ls, last] never
+// crosses lane 31/32) with DPP row shifts: row_shr 1, 2, 4, 8 inside each
+// 16-lane row, then row_bcast:15 carries lane 15 (47) into row 1 (3) for
+// segments that started in the row before.  Lanes outside the segment see the
+// identity `id`.
+template <int CTRL, int ROWS>
+__device__ inline uint32_t dpp_mov(uint32_t id, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, CTRL, ROWS, 0xF, false);
+}
+
+template <class Op>
+__device__ inline uint32_t seg_scan32(uint32_t v, uint32_t id, uint32_t lane, uint32_t ls, Op op) {
+  const uint32_t rs = lane & ~15u, s = ls > rs ? ls : rs;
+  uint32_t t;
+  t = dpp_mov<0x111, 0xF>(id, v); if (lane >= s + 1) v = op(v, t);
+  t = dpp_mov<0x112, 0xF>(id, v); if (lane >= s + 2) v = op(v, t);
+  t = dpp_mov<0x114, 0xF>(id, v); if (lane >= s + 4) v = op(v, t);
+  t = dpp_mov<0x118, 0xF>(id, v); if (lane >= s + 8) v = op(v, t);
+  t = dpp_mov<0x142, 0xA>(id, v); if (ls < rs) v = op(v, t);
+  return v;
+}
+
+// Suffix (inclusive toward higher lanes) scan over each whole 32-lane half:
+// row_shl 1, 2, 4, 8 inside each 16-lane row, then rows 0 and 2 fold in the
+// suffix of row 1 (3), read from lane 16 (48).
+template <class Op>
+__device__ inline uint32_t half_suffix32(uint32_t v, uint32_t id, uint32_t lane, Op op) {
+  v = op(v, dpp_mov<0x101, 0xF>(id, v));
+  v = op(v, dpp_mov<0x102, 0xF>(id, v));
+  v = op(v, dpp_mov<0x104, 0xF>(id, v));
+  v = op(v, dpp_mov<0x108, 0xF>(id, v));
+  const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+  const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+  if ((lane & 16u) == 0) v = op(v, lane < 32 ? r1 : r3);
+  return v;
+}
+
+// Value of lane - 1 (DPP wave_shr:1); lane 0 gets `id`.
+__device__ inline uint32_t from_prev_lane(uint32_t v, uint32_t id) { return dpp_mov<0x138, 0xF>(id, v); }
+
+// Inclusive segmented minimum scan of u64 keys inside 32-lane halves.
+__device__ inline uint64_t seg_scan_min_u64(uint64_t v, uint32_t lane, uint32_t ls) {
+  const uint32_t rs = lane & ~15u, s = ls > rs ? ls : rs;
+  uint64_t t;
+#define MG_SEG_MIN_STEP(C, R, COND)                                                   \
+  t = ((uint64_t)dpp_mov<C, R>(0xFFFFFFFFu, (uint32_t)(v >> 32)) << 32) |             \
+      dpp_mov<C, R>(0xFFFFFFFFu, (uint32_t)v);                                        \
+  if (COND) v = t < v ? t : v;
+  MG_SEG_MIN_STEP(0x111, 0xF, lane >= s + 1)
+  MG_SEG_MIN_STEP(0x112, 0xF, lane >= s + 2)
+  MG_SEG_MIN_STEP(0x114, 0xF, lane >= s + 4)
+  MG_SEG_MIN_STEP(0x118, 0xF, lane >= s + 8)
+  MG_SEG_MIN_STEP(0x142, 0xA, ls < rs)
+#undef MG_SEG_MIN_STEP
+  return v;
+}
+
+// Minimum of u64 keys over the segment, returned on every lane of it.
+__device__ inline uint64_t seg_min_u64(uint64_t v, uint32_t lane, uint32_t ls, uint32_t last) {
+  return __shfl(seg_scan_min_u64(v, lane, ls), (int)last, 64);
+}
+
+// Minimum of u64 keys over the wavefront (the two half minima sit in lanes 31, 63).
+__device__ inline uint64_t wave_min_u64_dpp(uint64_t v, uint32_t lane) {
+  v = seg_scan_min_u64(v, lane, lane & 32u);
+  const uint64_t a = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(v >> 32), 31) << 32) |
+                     (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 31);
+  const uint64_t b = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(v >> 32), 63) << 32) |
+                     (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+  return a < b ? a : b;
+}
+
+// Union of the boxes of lanes [ls, last] that have `take`, on every lane of
+// the segment.  Box coordinates are finite and carry no -0 (object boxes are
+// canonicalised at setup, centroids of such boxes cannot be -0), so the fold
+// order does not change the result.
+__device__ inline Box seg_box(bool take, const float* mn, const float* mx, uint32_t lane,
+                              uint32_t ls, uint32_t last) {
+  // order-preserving keys: both folds become u32 max with identity 0
+  const auto umax = [](uint32_t a, uint32_t b) { return a > b ? a : b; };
+  uint32_t lo[3], hi[3];
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    lo[x] = seg_scan32(take ? acc_min_enc(mn[x]) : 0u, 0u, lane, ls, umax);
+    hi[x] = seg_scan32(take ? acc_max_enc(mx[x]) : 0u, 0u, lane, ls, umax);
+  }
+  Box r;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    r.mn[x] = acc_min_dec((uint32_t)__shfl((int)lo[x], (int)last, 64));
+    r.mx[x] = acc_max_dec((uint32_t)__shfl((int)hi[x], (int)last, 64));
+  }
+  return r;
+}
+
+// Wave-wide maximum of a value in [0, 63] from six ballots.
+__device__ inline uint32_t wave_max6(uint32_t v) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int b = 5; b >= 0; --b) {
+    const uint32_t t = m | (1u << b);
+    if (__ballot(v >= t)) m = t;
+  }
+  return m;
+}
+
+// Maximum over the whole wavefront (every lane active): row prefix maxima
+// with row_shr, then row_bcast:15 / row_bcast:31 carry them to lane 63.
+__device__ inline uint32_t wave_max_u32_dpp(uint32_t v) {
+  const auto umax = [](uint32_t a, uint32_t b) { return a > b ? a : b; };
+  v = umax(v, dpp_mov<0x111, 0xF>(0u, v));
+  v = umax(v, dpp_mov<0x112, 0xF>(0u, v));
+  v = umax(v, dpp_mov<0x114, 0xF>(0u, v));
+  v = umax(v, dpp_mov<0x118, 0xF>(0u, v));
+  v = umax(v, dpp_mov<0x142, 0xA>(0u, v));
+  v = umax(v, dpp_mov<0x143, 0xC>(0u, v));
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Adds each valid lane's object to bin `bi` of axis `a` of an LDS histogram
+// row (count and/or the box words e: ~okey(min) x3, okey(max) x3).  LDS
+// serialises atomics on one address, and spatially coherent chunks put most
+// of a wavefront into one bin (a heightfield row: all 64 lanes), so up to
+// three bins shared by >= 8 lanes are merged first: the wave maximum of their
+// box words and their popcount go in with one set of atomics.  Counts and
+// max folds are order independent: the row is the same as with per-lane
+// atomics.  Call with every lane of the wavefront active.
+__device__ inline void hist_add(uint32_t* h, uint32_t a, uint32_t bi, bool valid,
+                                const uint32_t (&e)[6], bool do_count, bool do_boxes) {
+  const uint32_t lane = lane_id();
+  uint64_t rem = __ballot(valid);
+#pragma unroll 1
+  for (int round = 0; round < 3 && rem; ++round) {
+    const uint32_t lead = (uint32_t)__ffsll((unsigned long long)rem) - 1;
+    const uint32_t B = (uint32_t)__builtin_amdgcn_readlane((int)bi, (int)lead);
+    const bool in = ((rem >> lane) & 1ull) && bi == B;
+    const uint64_t m = __ballot(in);
+    const uint32_t k = (uint32_t)__popcll(m);
+    if (k < 8) break;
+    uint32_t* hb = &h[(a * MG_NBINS + B) * BIN_WORDS];
+    if (do_boxes) {
+      uint32_t r[6];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) r[j] = wave_max_u32_dpp(in ? e[j] : 0u);
+      if (lane == lead) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) atomicMax(&hb[1 + j], r[j]);
+      }
+    }
+    if (do_count && lane == lead) atomicAdd(&hb[0], k);
+    rem &= ~m;
+  }
+  if ((rem >> lane) & 1ull) {
+    uint32_t* hb = &h[(a * MG_NBINS + bi) * BIN_WORDS];
+    if (do_count) atomicAdd(&hb[0], 1u);
+    if (do_boxes) {
+#pragma unroll
+      for (int j = 0; j < 6; ++j) atomicMax(&hb[1 + j], e[j]);
+    }
+  }
+}
+
 __device__ inline bool coord_ok(float x) { return fabsf(x) < 1.152921504606846976e18f; }  // 2^60
 
 // Block-wide max of 12 accumulator words, then one atomicMax each.
@@ -264,25 +440,24 @@ __device__ inline void bin_chunk(uint32_t c, const LNode* __restrict__ L,
   __syncthreads();
   const uint32_t lo = nd.start + k * CH;
   const uint32_t hi = min(lo + CH, nd.start + nd.count);
-  for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+  // whole-block steps (every lane active for the wavefront merges in hist_add)
+  for (uint32_t base = lo; base < hi; base += blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    const bool valid = i < hi;
     Box b;
-    uint32_t id;
-    rec_load(&recs[i], b, id);
+    uint32_t id = 0;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) { b.mn[x] = 0.0f; b.mx[x] = 0.0f; }
+    if (valid) rec_load(&recs[i], b, id);
+    uint32_t e[6];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) { e[j] = acc_min_enc(b.mn[j]); e[3 + j] = acc_max_enc(b.mx[j]); }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const bool boxes = pass == 1 ? lk == LM_PLAIN : (lk == LM_FALLBACK || a == (int)la);
-      if (pass == 2 && !boxes) continue;
-      float cc = center(b.mn[a], b.mx[a]);
-      uint32_t bi = bin_index(cc, cmin[a], cext[a]);
-      uint32_t* hb = &h[(a * MG_NBINS + bi) * BIN_WORDS];
-      if (pass == 1) atomicAdd(&hb[0], 1u);
-      if (boxes) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          atomicMax(&hb[1 + j], acc_min_enc(b.mn[j]));
-          atomicMax(&hb[4 + j], acc_max_enc(b.mx[j]));
-        }
-      }
+      if (pass == 2 && !boxes) continue;  // block-uniform
+      const uint32_t bi = valid ? bin_index(center(b.mn[a], b.mx[a]), cmin[a], cext[a]) : 0u;
+      hist_add(h, (uint32_t)a, bi, valid, e, pass == 1, boxes);
     }
   }
   __syncthreads();
@@ -408,37 +583,38 @@ __device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32
     const bool qact = q < 3 * MG_NBINS;
     const uint32_t b = q % MG_NBINS;
     const int a = qact ? (int)(q / MG_NBINS) : 0;
+    // bin words: count, ~okey(min) x3, okey(max) x3 (0 = empty), so every box
+    // fold is a u32 max with identity 0; each 32-lane half holds one axis' bins
+    // and is scanned forward (prefix) and backward (suffix) with DPP
     uint32_t cnt = 0;
-    Box bb = box_empty();
+    uint32_t e[6] = {0u, 0u, 0u, 0u, 0u, 0u};
     if (qact) {
       const uint32_t* hb = &h[q * BIN_WORDS];
       cnt = hb[0];
-      if (cnt) bb = hist_box(hb);
-    }
-    uint32_t pc = cnt, sc = cnt;
-    Box pb = bb, sb = bb;
+      if (cnt) {
 #pragma unroll
-    for (uint32_t d = 1; d < MG_NBINS; d <<= 1) {
-      const uint32_t oc = __shfl_up(pc, d, MG_NBINS);
-      const uint32_t uc = __shfl_down(sc, d, MG_NBINS);
-      Box ob, ub;
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        ob.mn[x] = __shfl_up(pb.mn[x], d, MG_NBINS);
-        ob.mx[x] = __shfl_up(pb.mx[x], d, MG_NBINS);
-        ub.mn[x] = __shfl_down(sb.mn[x], d, MG_NBINS);
-        ub.mx[x] = __shfl_down(sb.mx[x], d, MG_NBINS);
+        for (int j = 0; j < 6; ++j) e[j] = hb[1 + j];
       }
-      if (b >= d) { pc += oc; box_grow(pb, ob); }
-      if (b + d < MG_NBINS) { sc += uc; box_grow(sb, ub); }
     }
-    // lt of cut b is the prefix ending at bin b - 1
-    const uint32_t cnl = __shfl_up(pc, 1, MG_NBINS);
-    Box clt;
+    const uint32_t hs = lane & 32u;
+    const auto add = [](uint32_t x, uint32_t y) { return x + y; };
+    const auto umax = [](uint32_t x, uint32_t y) { return x > y ? x : y; };
+    const uint32_t pc = seg_scan32(cnt, 0u, lane, hs, add);
+    const uint32_t sc = half_suffix32(cnt, 0u, lane, add);
+    // lt of cut b is the prefix ending at bin b - 1 (lane - 1; b >= 1 below)
+    const uint32_t cnl = from_prev_lane(pc, 0u);
+    Box clt, sb;
 #pragma unroll
-    for (int x = 0; x < 3; ++x) {
-      clt.mn[x] = __shfl_up(pb.mn[x], 1, MG_NBINS);
-      clt.mx[x] = __shfl_up(pb.mx[x], 1, MG_NBINS);
+    for (int j = 0; j < 6; ++j) {
+      const uint32_t pe = from_prev_lane(seg_scan32(e[j], 0u, lane, hs, umax), 0u);
+      const uint32_t se = half_suffix32(e[j], 0u, lane, umax);
+      if (j < 3) {
+        clt.mn[j] = acc_min_dec(pe);
+        sb.mn[j] = acc_min_dec(se);
+      } else {
+        clt.mx[j - 3] = acc_max_dec(pe);
+        sb.mx[j - 3] = acc_max_dec(se);
+      }
     }
     const uint32_t cng = sc;
     plt[pass] = clt;
@@ -453,7 +629,7 @@ __device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32
       best = key < best ? key : best;
     }
   }
-  best = wave_min_u64(best);
+  best = wave_min_u64_dpp(best, lane);
 
   if (best != ~0ull) {
     const uint32_t q = (uint32_t)best;
@@ -462,17 +638,17 @@ __device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32
     kind = 0;
     // the winning candidate's boxes and counts, from the lane that evaluated it
     // (prefix / suffix unions of exact min/max: the same values as summing bins)
-    const int src = (int)(q & 63u);
+    const uint32_t src = __builtin_amdgcn_readfirstlane(q & 63u);
     const bool p1 = q >= 64;  // wave-uniform
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-      lt.mn[x] = __shfl(p1 ? plt[1].mn[x] : plt[0].mn[x], src, 64);
-      lt.mx[x] = __shfl(p1 ? plt[1].mx[x] : plt[0].mx[x], src, 64);
-      gt.mn[x] = __shfl(p1 ? pgt[1].mn[x] : pgt[0].mn[x], src, 64);
-      gt.mx[x] = __shfl(p1 ? pgt[1].mx[x] : pgt[0].mx[x], src, 64);
+      lt.mn[x] = rl(p1 ? plt[1].mn[x] : plt[0].mn[x], src);
+      lt.mx[x] = rl(p1 ? plt[1].mx[x] : plt[0].mx[x], src);
+      gt.mn[x] = rl(p1 ? pgt[1].mn[x] : pgt[0].mn[x], src);
+      gt.mx[x] = rl(p1 ? pgt[1].mx[x] : pgt[0].mx[x], src);
     }
-    nl = __shfl(p1 ? pnl[1] : pnl[0], src, 64);
-    ng = __shfl(p1 ? png[1] : png[0], src, 64);
+    nl = (uint32_t)__builtin_amdgcn_readlane((int)(p1 ? pnl[1] : pnl[0]), (int)src);
+    ng = (uint32_t)__builtin_amdgcn_readlane((int)(p1 ? png[1] : png[0]), (int)src);
     cost = split_cost(sa_p, box_sa(lt), box_sa(gt), nl, ng);
   } else {
     // Median fallback (lib.rs:444-491, 527-531).  With > 32 objects it is only
@@ -749,20 +925,21 @@ __device__ inline void partition_chunk(uint32_t c, const LNode* __restrict__ L,
     }
     for (uint32_t t = threadIdx.x; t < HIST_WORDS; t += blockDim.x) h[t] = 0;
     __syncthreads();  // also orders this block's record stores before the reads below
-    for (uint32_t i = cstart + threadIdx.x; i < cstart + ccount; i += blockDim.x) {
+    for (uint32_t base = cstart; base < cstart + ccount; base += blockDim.x) {
+      const uint32_t i = base + threadIdx.x;
+      const bool valid = i < cstart + ccount;
       Box b;
-      uint32_t id;
-      rec_load(&dst[i], b, id);
+      uint32_t id = 0;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) { b.mn[x] = 0.0f; b.mx[x] = 0.0f; }
+      if (valid) rec_load(&dst[i], b, id);
+      uint32_t e[6];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) { e[j] = acc_min_enc(b.mn[j]); e[3 + j] = acc_max_enc(b.mx[j]); }
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const uint32_t bi = bin_index(center(b.mn[a], b.mx[a]), cmin3[a], cext3[a]);
-        uint32_t* hb = &h[(a * MG_NBINS + bi) * BIN_WORDS];
-        atomicAdd(&hb[0], 1u);
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          atomicMax(&hb[1 + j], acc_min_enc(b.mn[j]));
-          atomicMax(&hb[4 + j], acc_max_enc(b.mx[j]));
-        }
+        const uint32_t bi = valid ? bin_index(center(b.mn[a], b.mx[a]), cmin3[a], cext3[a]) : 0u;
+        hist_add(h, (uint32_t)a, bi, valid, e, true, true);
       }
     }
     __syncthreads();
@@ -800,120 +977,50 @@ __global__ __launch_bounds__(NW * 64) void k_partition(const LNode* __restrict__
 // objects for the next level.  A triangle's three vertex indices travel with
 // it from the first load, so leaves store the permuted triangle list without
 // another global read.
-struct __align__(16) SmallObj {
-  float4 a;  // c0 c1 c2 mn0
-  float4 b;  // mn1 mn2 mx0 mx1
-  float4 d;  // mx2 id tri0 tri1
-  float4 e;  // tri2 - - -
+// One wavefront's objects in LDS, one array per field group, so lanes that
+// touch different positions hit different banks.  Positions 64-95 are never
+// written: they only keep the candidate loop's reads (position <= 63 plus up
+// to 32) inside the arrays.
+struct SmallObjs {
+  float4 a[96];     // c0 c1 c2 mn0
+  float4 b[96];     // mn1 mn2 mx0 mx1
+  float4 d[96];     // mx2 id tri0 tri1
+  uint32_t e[64];   // tri2
 };
-
-// Segmented inclusive scans inside 32-lane halves (a segment [ls, last] never
-// crosses lane 31/32) with DPP row shifts: row_shr 1, 2, 4, 8 inside each
-// 16-lane row, then row_bcast:15 carries lane 15 (47) into row 1 (3) for
-// segments that started in the row before.  Lanes outside the segment see the
-// identity `id`.
-template <int CTRL, int ROWS>
-__device__ inline uint32_t dpp_mov(uint32_t id, uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, CTRL, ROWS, 0xF, false);
-}
-
-template <class Op>
-__device__ inline uint32_t seg_scan32(uint32_t v, uint32_t id, uint32_t lane, uint32_t ls, Op op) {
-  const uint32_t rs = lane & ~15u, s = ls > rs ? ls : rs;
-  uint32_t t;
-  t = dpp_mov<0x111, 0xF>(id, v); if (lane >= s + 1) v = op(v, t);
-  t = dpp_mov<0x112, 0xF>(id, v); if (lane >= s + 2) v = op(v, t);
-  t = dpp_mov<0x114, 0xF>(id, v); if (lane >= s + 4) v = op(v, t);
-  t = dpp_mov<0x118, 0xF>(id, v); if (lane >= s + 8) v = op(v, t);
-  t = dpp_mov<0x142, 0xA>(id, v); if (ls < rs) v = op(v, t);
-  return v;
-}
-
-// Minimum of u64 keys over the segment, returned on every lane of it.
-__device__ inline uint64_t seg_min_u64(uint64_t v, uint32_t lane, uint32_t ls, uint32_t last) {
-  const uint32_t rs = lane & ~15u, s = ls > rs ? ls : rs;
-  uint64_t t;
-#define MG_SEG_MIN_STEP(C, R, COND)                                                   \
-  t = ((uint64_t)dpp_mov<C, R>(0xFFFFFFFFu, (uint32_t)(v >> 32)) << 32) |             \
-      dpp_mov<C, R>(0xFFFFFFFFu, (uint32_t)v);                                        \
-  if (COND) v = t < v ? t : v;
-  MG_SEG_MIN_STEP(0x111, 0xF, lane >= s + 1)
-  MG_SEG_MIN_STEP(0x112, 0xF, lane >= s + 2)
-  MG_SEG_MIN_STEP(0x114, 0xF, lane >= s + 4)
-  MG_SEG_MIN_STEP(0x118, 0xF, lane >= s + 8)
-  MG_SEG_MIN_STEP(0x142, 0xA, ls < rs)
-#undef MG_SEG_MIN_STEP
-  return __shfl(v, (int)last, 64);
-}
-
-// Union of the boxes of lanes [ls, last] that have `take`, on every lane of
-// the segment.  Box coordinates are finite and carry no -0 (object boxes are
-// canonicalised at setup, centroids of such boxes cannot be -0), so the fold
-// order does not change the result and fminf/fmaxf equal the x86 min/max.
-__device__ inline Box seg_box(bool take, const float* mn, const float* mx, uint32_t lane,
-                              uint32_t ls, uint32_t last) {
-  const auto fmn = [](uint32_t a, uint32_t b) { return fbits(fminf(bitsf(a), bitsf(b))); };
-  const auto fmx = [](uint32_t a, uint32_t b) { return fbits(fmaxf(bitsf(a), bitsf(b))); };
-  uint32_t lo[3], hi[3];
-#pragma unroll
-  for (int x = 0; x < 3; ++x) {
-    lo[x] = seg_scan32(take ? fbits(mn[x]) : 0x7f800000u, 0x7f800000u, lane, ls, fmn);
-    hi[x] = seg_scan32(take ? fbits(mx[x]) : 0xff800000u, 0xff800000u, lane, ls, fmx);
-  }
-  Box r;
-#pragma unroll
-  for (int x = 0; x < 3; ++x) {
-    r.mn[x] = bitsf((uint32_t)__shfl((int)lo[x], (int)last, 64));
-    r.mx[x] = bitsf((uint32_t)__shfl((int)hi[x], (int)last, 64));
-  }
-  return r;
-}
-
-// Wave-wide maximum of a value in [0, 63] from six ballots.
-__device__ inline uint32_t wave_max6(uint32_t v) {
-  uint32_t m = 0;
-#pragma unroll
-  for (int b = 5; b >= 0; --b) {
-    const uint32_t t = m | (1u << b);
-    if (__ballot(v >= t)) m = t;
-  }
-  return m;
-}
 
 struct SmallLane {
   float c[3], mn[3], mx[3];
   uint32_t id, tri[3];
 };
 
-__device__ inline void obj_put(SmallObj* S, uint32_t pos, const SmallLane& o) {
-  S[pos].a = make_float4(o.c[0], o.c[1], o.c[2], o.mn[0]);
-  S[pos].b = make_float4(o.mn[1], o.mn[2], o.mx[0], o.mx[1]);
-  S[pos].d = make_float4(o.mx[2], __uint_as_float(o.id), __uint_as_float(o.tri[0]),
-                         __uint_as_float(o.tri[1]));
-  S[pos].e = make_float4(__uint_as_float(o.tri[2]), 0.0f, 0.0f, 0.0f);
+__device__ inline void obj_put(SmallObjs* S, uint32_t pos, const SmallLane& o) {
+  S->a[pos] = make_float4(o.c[0], o.c[1], o.c[2], o.mn[0]);
+  S->b[pos] = make_float4(o.mn[1], o.mn[2], o.mx[0], o.mx[1]);
+  S->d[pos] = make_float4(o.mx[2], __uint_as_float(o.id), __uint_as_float(o.tri[0]),
+                          __uint_as_float(o.tri[1]));
+  S->e[pos] = o.tri[2];
 }
-__device__ inline void obj_get(const SmallObj* S, uint32_t pos, SmallLane& o) {
-  const float4 a = S[pos].a, b = S[pos].b, d = S[pos].d, e = S[pos].e;
+__device__ inline void obj_get(const SmallObjs* S, uint32_t pos, SmallLane& o) {
+  const float4 a = S->a[pos], b = S->b[pos], d = S->d[pos];
   o.c[0] = a.x; o.c[1] = a.y; o.c[2] = a.z; o.mn[0] = a.w;
   o.mn[1] = b.x; o.mn[2] = b.y; o.mx[0] = b.z; o.mx[1] = b.w;
   o.mx[2] = d.x; o.id = __float_as_uint(d.y);
-  o.tri[0] = __float_as_uint(d.z); o.tri[1] = __float_as_uint(d.w); o.tri[2] = __float_as_uint(e.x);
+  o.tri[0] = __float_as_uint(d.z); o.tri[1] = __float_as_uint(d.w); o.tri[2] = S->e[pos];
 }
 // The centroid and box of the object at `pos` (what the candidate loop reads).
 struct SmallGeo {
   float4 a, b;
   float mx2;
 };
-__device__ inline SmallGeo geo_get(const SmallObj* S, uint32_t pos) {
+__device__ inline SmallGeo geo_get(const SmallObjs* S, uint32_t pos) {
   SmallGeo g;
-  g.a = S[pos].a;
-  g.b = S[pos].b;
-  g.mx2 = S[pos].d.x;
+  g.a = S->a[pos];
+  g.b = S->b[pos];
+  g.mx2 = S->d[pos].x;
   return g;
 }
-
 // Moves each lane's object to `newpos` (a permutation of the lanes).
-__device__ inline void lane_move(SmallObj* S, uint32_t lane, uint32_t newpos, SmallLane& o) {
+__device__ inline void lane_move(SmallObjs* S, uint32_t lane, uint32_t newpos, SmallLane& o) {
   obj_put(S, newpos, o);
   wave_sync();
   obj_get(S, lane, o);
@@ -943,7 +1050,7 @@ __device__ inline void put_leaf(uint32_t pos, const SmallLane& o, uint32_t obase
   }
 }
 
-__device__ inline void small_pair(uint32_t wave, SmallObj* S, const SNode* __restrict__ small,
+__device__ inline void small_pair(uint32_t wave, SmallObjs* S, const SNode* __restrict__ small,
                                   uint32_t nS, const Rec* __restrict__ rec0,
                                   const Rec* __restrict__ rec1, NodeOut out,
                                   uint32_t* __restrict__ perm, const uint32_t* __restrict__ tris_in,
@@ -1036,24 +1143,30 @@ __device__ inline void small_pair(uint32_t wave, SmallObj* S, const SNode* __res
 #pragma unroll
     for (int a = 0; a < 3; ++a) { lt[a] = box_empty(); gt[a] = box_empty(); nl[a] = 0; ng[a] = 0; }
     const uint32_t maxlc = wave_max6(act ? lc : 0u);
-    // the next object's LDS reads are issued before the current one is folded in
-    SmallGeo g = geo_get(S, act ? ls : lane);
-    for (uint32_t k = 0; k < maxlc; ++k) {
-      const bool v = act && k < lc;
-      const SmallGeo gn = geo_get(S, act && k + 1 < lc ? ls + k + 1 : lane);
-      if (v) {
-        const float qc[3] = {g.a.x, g.a.y, g.a.z};
-        Box qb;
-        qb.mn[0] = g.a.w; qb.mn[1] = g.b.x; qb.mn[2] = g.b.y;
-        qb.mx[0] = g.b.z; qb.mx[1] = g.b.w; qb.mx[2] = g.mx2;
+    const auto fold = [&](const SmallGeo& g) {
+      const float qc[3] = {g.a.x, g.a.y, g.a.z};
+      Box qb;
+      qb.mn[0] = g.a.w; qb.mn[1] = g.b.x; qb.mn[2] = g.b.y;
+      qb.mx[0] = g.b.z; qb.mx[1] = g.b.w; qb.mx[2] = g.mx2;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          if (qc[a] < th[a]) { box_grow(lt[a], qb); ++nl[a]; }
-          else { box_grow(gt[a], qb); ++ng[a]; }
-        }
+      for (int a = 0; a < 3; ++a) {
+        if (qc[a] < th[a]) { box_grow_nz(lt[a], qb); ++nl[a]; }
+        else box_grow_nz(gt[a], qb);
       }
-      g = gn;
+    };
+    // two objects in flight: the next one's LDS reads are issued before the
+    // current one is folded in; p0 + k <= 63 + 32 stays inside the arrays
+    const uint32_t p0 = act ? ls : lane;
+    SmallGeo g0 = geo_get(S, p0);
+    for (uint32_t k = 0; k < maxlc; k += 2) {
+      const SmallGeo g1 = geo_get(S, p0 + k + 1);
+      if (act && k < lc) fold(g0);
+      if (k + 1 >= maxlc) break;
+      g0 = geo_get(S, p0 + k + 2);
+      if (act && k + 1 < lc) fold(g1);
     }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ng[a] = act ? lc - nl[a] : 0u;
     uint64_t key = ~0ull;
     int ba = 0;
 #pragma unroll
@@ -1184,11 +1297,11 @@ __global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small,
                                                const uint32_t* __restrict__ tris_in,
                                                uint32_t* __restrict__ tris_out, Counters* ctr,
                                                uint32_t* gaps, uint32_t max_depth) {
-  __shared__ SmallObj lds_obj[4][64];
+  __shared__ SmallObjs lds_obj[4];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t nS = *nS_dev;
   for (uint32_t wave = blockIdx.x * 4 + w; 2 * wave < nS; wave += gridDim.x * 4)
-    small_pair(wave, lds_obj[w], small, nS, rec0, rec1, out, perm, tris_in, tris_out, ctr, gaps,
+    small_pair(wave, &lds_obj[w], small, nS, rec0, rec1, out, perm, tris_in, tris_out, ctr, gaps,
                max_depth);
 }
 
