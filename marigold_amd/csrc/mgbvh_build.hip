@@ -142,22 +142,23 @@ __device__ inline uint64_t wave_min_u64_dpp(uint64_t v, uint32_t lane) {
 // Union of the boxes of lanes [ls, last] that have `take`, on every lane of
 // the segment.  Box coordinates are finite and carry no -0 (object boxes are
 // canonicalised at setup, centroids of such boxes cannot be -0), so the fold
-// order does not change the result.
+// order does not change the result and fminf/fmaxf equal the x86 min/max.
 __device__ inline Box seg_box(bool take, const float* mn, const float* mx, uint32_t lane,
                               uint32_t ls, uint32_t last) {
   // order-preserving keys: both folds become u32 max with identity 0
-  const auto umax = [](uint32_t a, uint32_t b) { return a > b ? a : b; };
+  const auto fmn = [](uint32_t a, uint32_t b) { return fbits(fminf(bitsf(a), bitsf(b))); };
+  const auto fmx = [](uint32_t a, uint32_t b) { return fbits(fmaxf(bitsf(a), bitsf(b))); };
   uint32_t lo[3], hi[3];
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
-    lo[x] = seg_scan32(take ? acc_min_enc(mn[x]) : 0u, 0u, lane, ls, umax);
-    hi[x] = seg_scan32(take ? acc_max_enc(mx[x]) : 0u, 0u, lane, ls, umax);
+    lo[x] = seg_scan32(take ? fbits(mn[x]) : 0x7f800000u, 0x7f800000u, lane, ls, fmn);
+    hi[x] = seg_scan32(take ? fbits(mx[x]) : 0xff800000u, 0xff800000u, lane, ls, fmx);
   }
   Box r;
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
-    r.mn[x] = acc_min_dec((uint32_t)__shfl((int)lo[x], (int)last, 64));
-    r.mx[x] = acc_max_dec((uint32_t)__shfl((int)hi[x], (int)last, 64));
+    r.mn[x] = bitsf((uint32_t)__shfl((int)lo[x], (int)last, 64));
+    r.mx[x] = bitsf((uint32_t)__shfl((int)hi[x], (int)last, 64));
   }
   return r;
 }
@@ -938,8 +939,14 @@ __device__ inline void partition_chunk(uint32_t c, const LNode* __restrict__ L,
       for (int j = 0; j < 3; ++j) { e[j] = acc_min_enc(b.mn[j]); e[3 + j] = acc_max_enc(b.mx[j]); }
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const uint32_t bi = valid ? bin_index(center(b.mn[a], b.mx[a]), cmin3[a], cext3[a]) : 0u;
-        hist_add(h, (uint32_t)a, bi, valid, e, true, true);
+        // children of deep nodes spread over the bins: per-lane atomics
+        if (valid) {
+          const uint32_t bi = bin_index(center(b.mn[a], b.mx[a]), cmin3[a], cext3[a]);
+          uint32_t* hb = &h[(a * MG_NBINS + bi) * BIN_WORDS];
+          atomicAdd(&hb[0], 1u);
+#pragma unroll
+          for (int j = 0; j < 6; ++j) atomicMax(&hb[1 + j], e[j]);
+        }
       }
     }
     __syncthreads();
