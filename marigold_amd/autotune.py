@@ -31,9 +31,10 @@ GRID_CANDIDATES = [
          small_blocks_per_cu=32, partition_waves=1, bin_waves=2),
 ]
 LEARNED_CANDIDATES = [
-    dict(learned_min_objects=512, learned_window=3, learned_guard=0.02),
     dict(learned_min_objects=1024, learned_window=3, learned_guard=0.02),
     dict(learned_min_objects=2048, learned_window=2, learned_guard=0.02),
+    dict(learned_min_objects=8192, learned_window=2, learned_guard=0.02),
+    dict(learned_min_objects=32768, learned_window=2, learned_guard=0.02),
 ]
 
 

@@ -546,7 +546,8 @@ __device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32
                                    uint32_t* ck_next, SNode* small, Counters* ctr, NodeOut out,
                                    uint32_t max_depth, uint32_t parity_next, uint32_t learn_min,
                                    uint32_t* lrows_next, uint32_t* lvn) {
-  __shared__ uint32_t s_need[SW_WAVES][4];  // large slots, chunks, small slots, objects
+  // large slots, chunks, small slots, objects, window nodes, fallback nodes
+  __shared__ uint32_t s_need[SW_WAVES][6];
   __shared__ uint32_t s_base[SW_WAVES][3];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t lane = lane_id();
@@ -714,12 +715,19 @@ __device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32
     s_need[w][1] = chg + chl;
     s_need[w][2] = active ? 2 - ((uint32_t)lg + (uint32_t)ll) : 0;
     s_need[w][3] = active ? nd.count : 0;
+    s_need[w][4] = active && lmode_kind(nd.mode) == LM_WINDOW;
+    s_need[w][5] = active && lmode_kind(nd.mode) == LM_FALLBACK;
   }
   __syncthreads();
   if (threadIdx.x == 3) {  // work counter: objects of the nodes split on this level
     unsigned long long tot = 0;
     for (uint32_t q = 0; q < SW_WAVES; ++q) tot += s_need[q][3];
     if (tot) atomicAdd(&ctr->sum_large, tot);
+  }
+  if (threadIdx.x == 4 || threadIdx.x == 5) {  // learned method: window / fallback nodes
+    uint32_t tot = 0;
+    for (uint32_t q = 0; q < SW_WAVES; ++q) tot += s_need[q][threadIdx.x];
+    if (tot) atomicAdd(threadIdx.x == 4 ? &ctr->n_window : &ctr->n_fallback, tot);
   }
   if (threadIdx.x < 3) {
     const uint32_t k = threadIdx.x;

@@ -152,11 +152,18 @@ __global__ void k_to_bf16(const float* __restrict__ params, uint16_t* __restrict
 typedef short v4i16 __attribute__((ext_vector_type(4)));
 typedef float v4f32 __attribute__((ext_vector_type(4)));
 
-// One dense layer for the 16 rows of a wave: out[16][N] = act(in[16][128] * W^T + b).
+// LDS row stride of weights and activations: 128 BF16 values + 8 of padding
+// (68 words), so the 16 rows one MFMA operand read touches fall in different
+// banks.
+constexpr int LDS_ROW = MLP_INP + 8;
+constexpr int W_ROWS = 2 * MLP_H + MLP_OUT;  // W1, W2, W3 rows of 128 inputs each
+
+// One dense layer for the 16 rows of a wave: out[16][N] = act(in[16][128] * W^T + b),
+// weights and activations in LDS (row stride LDS_ROW).
 // MFMA 16x16x16 bf16 lane layout: A[i = l % 16][k = 4 (l / 16) + j],
 // B[k = 4 (l / 16) + j][n = l % 16], D[i = 4 (l / 16) + j][n = l % 16].
 template <int N, bool LAST>
-__device__ inline void mlp_layer(const uint16_t* in_lds, const uint16_t* __restrict__ W,
+__device__ inline void mlp_layer(const uint16_t* in_lds, const uint16_t* W,
                                  const float* __restrict__ bias, uint16_t* out_lds,
                                  float* __restrict__ out_logits, int row0, int nrows) {
   const int l = lane_id();
@@ -169,8 +176,8 @@ __device__ inline void mlp_layer(const uint16_t* in_lds, const uint16_t* __restr
     for (int kc = 0; kc < MLP_INP / 16; ++kc) {
       const int k = 16 * kc + 4 * q;
       v4i16 a, b;
-      const uint2 av = *reinterpret_cast<const uint2*>(&in_lds[r * MLP_INP + k]);
-      const uint2 bv = *reinterpret_cast<const uint2*>(&W[(size_t)n * MLP_INP + k]);
+      const uint2 av = *reinterpret_cast<const uint2*>(&in_lds[r * LDS_ROW + k]);
+      const uint2 bv = *reinterpret_cast<const uint2*>(&W[n * LDS_ROW + k]);
       a[0] = (short)(av.x & 0xffff); a[1] = (short)(av.x >> 16);
       a[2] = (short)(av.y & 0xffff); a[3] = (short)(av.y >> 16);
       b[0] = (short)(bv.x & 0xffff); b[1] = (short)(bv.x >> 16);
@@ -185,59 +192,71 @@ __device__ inline void mlp_layer(const uint16_t* in_lds, const uint16_t* __restr
       if (LAST) {
         if (i < nrows) out_logits[(size_t)(row0 + i) * MLP_OUT + n] = z;
       } else {
-        out_lds[i * MLP_INP + n] = f2bf(fmaxf(z, 0.0f));
+        out_lds[i * LDS_ROW + n] = f2bf(fmaxf(z, 0.0f));
       }
     }
   }
 }
 
-// Logits of n feature rows (MLP_IN floats each, row stride `stride` floats).
-// One wavefront per 16 rows; activations staged in LDS as BF16.
-__device__ inline void mlp_tile(int row0, int n, uint16_t* A, uint16_t* Bf,
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Logits of the (up to) 16 feature rows from row0 (MLP_IN floats each, row
+// stride `stride` floats).  One wavefront; activations staged in LDS as BF16.
+__device__ inline void mlp_tile(int row0, int n, uint16_t* A, uint16_t* Bf, const uint16_t* Wl,
                                 const float* __restrict__ feat, int stride,
-                                const uint16_t* __restrict__ wbf, const float* __restrict__ params,
-                                float* __restrict__ logits) {
+                                const float* __restrict__ params, float* __restrict__ logits) {
   const int l = lane_id();
   const int nrows = min(16, n - row0);
   for (int e = l; e < 16 * MLP_INP; e += 64) {
     const int i = e / MLP_INP, k = e % MLP_INP;
     const float x = (i < nrows && k < MLP_IN) ? feat[(size_t)(row0 + i) * stride + k] : 0.0f;
-    A[e] = f2bf(x);
+    A[i * LDS_ROW + k] = f2bf(x);
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const uint16_t* W1 = wbf;
-  const uint16_t* W2 = wbf + MLP_H * MLP_INP;
-  const uint16_t* W3 = W2 + MLP_H * MLP_H;
+  wave_lds_sync();
+  const uint16_t* W1 = Wl;
+  const uint16_t* W2 = Wl + MLP_H * LDS_ROW;
+  const uint16_t* W3 = W2 + MLP_H * LDS_ROW;
   mlp_layer<MLP_H, false>(A, W1, params + OFF_B1, Bf, nullptr, row0, nrows);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  wave_lds_sync();
   mlp_layer<MLP_H, false>(Bf, W2, params + OFF_B2, A, nullptr, row0, nrows);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  wave_lds_sync();
   mlp_layer<MLP_OUT, true>(A, W3, params + OFF_B3, nullptr, logits, row0, nrows);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();  // the next tile overwrites A
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  wave_lds_sync();  // the next tile overwrites A
 }
 
 // Logits of n feature rows (MLP_IN floats each, row stride `stride` floats);
-// n from n_dev when given.  One wavefront per 16 rows, persistent over tiles.
+// n from n_dev when given.  Each block first copies the BF16 weights into LDS
+// (one read of 88 KB, then every MFMA operand comes from LDS); then one
+// wavefront per 16 rows, persistent over tiles.
 __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ feat, int stride,
                                                    int n_host, const uint32_t* __restrict__ n_dev,
                                                    const uint16_t* __restrict__ wbf,
                                                    const float* __restrict__ params,
                                                    float* __restrict__ logits) {
-  __shared__ __align__(16) uint16_t bufA[4][16 * MLP_INP];
-  __shared__ __align__(16) uint16_t bufB[4][16 * MLP_INP];
+  extern __shared__ __align__(16) uint16_t lds[];
+  uint16_t* Wl = lds;                                   // W_ROWS x LDS_ROW
+  uint16_t* act = lds + W_ROWS * LDS_ROW;               // 4 waves x 2 x 16 x LDS_ROW
   const int w = threadIdx.x >> 6;
   const int n = n_dev ? (int)*n_dev : n_host;
+  if (blockIdx.x * 64 >= (unsigned)n) return;          // block-uniform: no tile for this block
+  // 16-byte pieces: 16 per 128-value row
+  for (int e = threadIdx.x; e < W_ROWS * 16; e += blockDim.x) {
+    const int row = e >> 4, c = e & 15;
+    *reinterpret_cast<uint4*>(&Wl[row * LDS_ROW + 8 * c]) =
+        *reinterpret_cast<const uint4*>(&wbf[(size_t)row * MLP_INP + 8 * c]);
+  }
+  __syncthreads();
+  uint16_t* A = act + (size_t)w * 2 * 16 * LDS_ROW;
+  uint16_t* Bf = A + 16 * LDS_ROW;
   for (int row0 = (blockIdx.x * 4 + w) * 16; row0 < n; row0 += gridDim.x * 64)
-    mlp_tile(row0, n, bufA[w], bufB[w], feat, stride, wbf, params, logits);
+    mlp_tile(row0, n, A, Bf, Wl, feat, stride, params, logits);
 }
+
+constexpr size_t MLP_LDS_BYTES = (size_t)(W_ROWS + 4 * 2 * 16) * LDS_ROW * 2;
 
 // ------------------------------------------------------------ learned build step
 // One wavefront per network row (a counted node, listed when it was created):
@@ -354,10 +373,8 @@ __device__ inline void decide_row(uint32_t row, LNode* __restrict__ L,
     const float pall = okey_inv((uint32_t)all), pwin = okey_inv((uint32_t)inw);
     if (inw != ~0ull && pwin <= pall * (1.0f + guard)) mode = lmode_pack(LM_WINDOW, pa, lo, hi);
   }
-  if (lane == 0) {
-    L[slot].mode = mode;
-    atomicAdd(lmode_kind(mode) == LM_WINDOW ? &ctr->n_window : &ctr->n_fallback, 1u);
-  }
+  // (window / fallback nodes are counted by the sweep, one atomic per block)
+  if (lane == 0) L[slot].mode = mode;
 }
 
 __global__ __launch_bounds__(256) void k_decide(LNode* __restrict__ L,
@@ -456,9 +473,15 @@ int model_infer(mgbvh_ctx* c, mgbvh_model* m, const float* feat, int stride, int
   if (n <= 0) return MGBVH_OK;
   int rc = model_bf16(c, m);
   if (rc) return rc;
-  const unsigned grid = std::max(1u, std::min<unsigned>(cdivu(n, 64), (unsigned)c->cus * 4));
-  k_mlp_infer<<<grid, 256, 0, c->stream>>>(feat, stride, n, n_dev, (const uint16_t*)m->wbf.p,
-                                           (const float*)m->params.p, logits);
+  // one block per compute unit at most (the weights fill most of its LDS)
+  hipError_t ea = hipFuncSetAttribute((const void*)k_mlp_infer,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)MLP_LDS_BYTES);
+  if (ea != hipSuccess) return hip_fail(ea, "hipFuncSetAttribute(k_mlp_infer)");
+  const unsigned grid = std::max(1u, std::min<unsigned>(cdivu(n, 64), (unsigned)c->cus));
+  k_mlp_infer<<<grid, 256, MLP_LDS_BYTES, c->stream>>>(feat, stride, n, n_dev,
+                                                       (const uint16_t*)m->wbf.p,
+                                                       (const float*)m->params.p, logits);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MGBVH_OK : hip_fail(e, "k_mlp_infer");
 }
