@@ -280,17 +280,21 @@ def scene_main(args):
     _, _, st = ctx.build_blas_batch(d["pos"], 12, nv, d["tris"], lea.mesh_desc, lea.blas_settings,
                                     d["blas_nodes"], d["blas_lens"], d["blas_perm"], d["tris_perm"])
     terms = terms_from_stats(st)
-    # level by level, timed, with event pairs around the dominant class only
-    ctx.set_profiling([dominant])
+    # level by level, timed, with event pairs around the dominant class's
+    # launches of the BLAS build (the TLAS build's launches are not counted)
+    ctx.set_profiling([])
     ctx.synchronize()
+    ctx.kernel_times(reset=True)
     ctx.record(0)
     for _ in range(args.steps):
-        lea.build()
+        ctx.set_profiling([dominant])
+        lea.build_blas()
+        ctx.set_profiling([])
+        lea.build_tlas()
     ctx.record(1)
     ctx.synchronize()
     ms_level = ctx.elapsed_ms(0, 1) / args.steps
     kt = ctx.kernel_times(reset=True)[dominant]
-    ctx.set_profiling([])
     dom_ms = kt[0] / kt[1] if kt[1] else 0.0  # per launch
     dom_launches = kt[1] / args.steps
     dom_bytes = kernel_bytes(dominant, n, nv, 12, terms) / max(dom_launches, 1)
@@ -298,24 +302,29 @@ def scene_main(args):
     build_bytes = sum(kernel_bytes(k, n, nv, 12, terms) for k in ("setup", "bin", "partition",
                                                                    "sweep", "small"))
     # the whole scene build as one graph replay: the timed region of `value`
-    lea.record_graph()
-    lea.build()
-    ctx.synchronize()
-    ctx.record(0)
-    for _ in range(args.steps):
+    # (--no-graph: level-by-level builds instead, for profiler runs)
+    if not args.no_graph:
+        lea.record_graph()
         lea.build()
-    ctx.record(1)
-    ctx.synchronize()
-    ms_step = ctx.elapsed_ms(0, 1) / args.steps
+        ctx.synchronize()
+        ctx.record(0)
+        for _ in range(args.steps):
+            lea.build()
+        ctx.record(1)
+        ctx.synchronize()
+        ms_step = ctx.elapsed_ms(0, 1) / args.steps
+    else:
+        ms_step = ms_level
     sah_learned = lea.scene_sah()
     hits_learned = _hits_of(ctx, lea.trace, d_rays, w * h)
-    log(f"learned+batched {ms_step:.2f} ms/scene (graph), {ms_level:.2f} ms level by level, "
-        f"SAH {sah_learned:.4f}")
+    log(f"learned+batched {ms_step:.2f} ms/scene ({'level by level' if args.no_graph else 'graph'}), "
+        f"{ms_level:.2f} ms level by level, SAH {sah_learned:.4f}")
     lea.free()
 
     # ---- batched, plain split rule (bit-identical to the per-mesh plain builds)
     bat = GpuScene(*scene_args, mode="batched", build=False)
-    bat.record_graph()
+    if not args.no_graph:
+        bat.record_graph()
     bat.build()
     ctx.synchronize()
     ctx.record(2)
@@ -327,7 +336,8 @@ def scene_main(args):
     sah_plain = bat.scene_sah()
     hits_plain = _hits_of(ctx, bat.trace, d_rays, w * h)
     bat_sum = int(bat.download("blas_nodes", np.uint32, 8 * bat.n_slots).astype(np.uint64).sum())
-    log(f"batched plain {ms_bat:.2f} ms/scene (graph), SAH {sah_plain:.4f}")
+    log(f"batched plain {ms_bat:.2f} ms/scene ({'level by level' if args.no_graph else 'graph'}), "
+        f"SAH {sah_plain:.4f}")
     bat.free()
 
     # ---- plain: one device build call per mesh (the reference's API shape)
@@ -376,6 +386,7 @@ def scene_main(args):
         "config": {"workload": "2,000-mesh scene (log-uniform 100..200,000 tris, 50/50 soup / "
                                "heightfield) + TLAS over 10,000 instances; whole scene build "
                                "replayed as one HIP graph",
+                   "timed": "level-by-level builds" if args.no_graph else "graph replays",
                    "tris": n, "verts": nv, "meshes": len(meshes),
                    "instances": len(sc["instance_mesh"]), "max_depth": 32,
                    "method": "learned + batched", "learned": learned,
@@ -398,7 +409,8 @@ def scene_main(args):
                      "kernel": dominant, "kernel_ms_per_launch": dom_ms,
                      "launches_per_build": dom_launches, "alg_bytes_per_launch": dom_bytes,
                      "timed": "level-by-level builds of the same scene, HIP event pairs around "
-                              "each launch of the kernel class on the build stream"},
+                              "each launch of the kernel class in the BLAS build, on the build "
+                              "stream"},
         "build_alg_gbs": build_bytes / (ms_step * 1e-3) / 1e9,
         "kernel_ms_per_build": {k: v[0] / max(args.warmup, 1) for k, v in prof.items()},
         "work": terms,
@@ -426,6 +438,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-rays", action="store_true", help="skip the 16M-ray hit comparison")
     ap.add_argument("--no-autotune", action="store_true", help="default tuning and learned settings")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time level-by-level builds instead of graph replays (profiler runs)")
     ap.add_argument("--quick-train", action="store_true",
                     help="6 training epochs instead of 45 (profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",

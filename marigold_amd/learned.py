@@ -16,7 +16,7 @@ from . import scenes
 
 # learned-method settings used by the scene builds (per-node-size dispatch,
 # window half-width, guard); `autotune` may replace them
-DEFAULT_LEARNED = {"learned_min_objects": 1024, "learned_window": 3, "learned_guard": 0.02}
+DEFAULT_LEARNED = {"learned_min_objects": 32768, "learned_window": 2, "learned_guard": 0.02}
 
 
 def training_meshes():
