@@ -534,43 +534,28 @@ __device__ inline void put_child(bool large, uint32_t s, uint32_t c0, uint32_t s
   }
 }
 
-constexpr uint32_t SW_WAVES = 16;  // nodes per k_sweep block (one wavefront each)
+// The split of a node with > 32 objects (select_split, lib.rs:493-534).
+struct SplitDec {
+  Box lt, gt;
+  uint32_t nl, ng, axis, value, kind;  // kind 0 binned cut, 2 median
+  float cost;
+  uint32_t pad[2];
+};
+static_assert(sizeof(SplitDec) == 80, "SplitDec is 80 bytes");
+static_assert(sizeof(SplitDec) <= HIST_WORDS * 4, "a SplitDec fits a histogram row");
 
-// One wavefront per node with > 32 objects: select_split (lib.rs:493-534) over
-// the binned histograms, node record, children.  Children are appended to the
-// next level's lists with one atomic per list per block (a same-address atomic
-// costs ~11 ns, serialised, so per-node atomics would dominate).
-__device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32_t nL,
-                                   const uint32_t* __restrict__ hist,
-                                   const Rec* __restrict__ recs, LNode* Ln, uint32_t* cn_next,
-                                   uint32_t* ck_next, SNode* small, Counters* ctr, NodeOut out,
-                                   uint32_t max_depth, uint32_t parity_next, uint32_t learn_min,
-                                   uint32_t* lrows_next, uint32_t* lvn) {
-  // large slots, chunks, small slots, objects, window nodes, fallback nodes
-  __shared__ uint32_t s_need[SW_WAVES][6];
-  __shared__ uint32_t s_base[SW_WAVES][3];
-  const uint32_t w = threadIdx.x >> 6;
+// One wavefront: the split of a node from its binned histogram h (global
+// memory or LDS), node box nb, centroid bounds (cmin, cext), its objects
+// [start, start + count) in recs (median fallback only) and its learned-method
+// state lm (a window node only considers its window's cuts).  Every lane
+// returns the same decision.
+__device__ inline SplitDec split_from_hist(const uint32_t* h, const Box& nb,
+                                           const float (&cmin)[3], const float (&cext)[3],
+                                           uint32_t start, uint32_t count,
+                                           const Rec* __restrict__ recs, uint32_t lm) {
   const uint32_t lane = lane_id();
-  const uint32_t slot = group * SW_WAVES + w;
-  const bool active = slot < nL;  // wave-uniform; every wave reaches the barriers
-  Box lt = box_empty(), gt = box_empty(), nb = box_empty();
-  uint32_t nl = 0, ng = 0, axis = 0, value = 0, kind = 0;
-  float cost = 0.0f;
-  LNode nd = {};
-  if (active) {
-  const uint32_t* h = hist + (size_t)slot * HIST_WORDS;  // each lane reads its own bins
-  nd = L[slot];
-  float cmin[3], cext[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    nb.mn[a] = nd.bmn[a];
-    nb.mx[a] = nd.bmx[a];
-    cmin[a] = acc_min_dec(nd.cacc[a]);
-    cext[a] = __fsub_rn(acc_max_dec(nd.cacc[3 + a]), cmin[a]);
-  }
+  SplitDec d;
   const float sa_p = box_sa(nb);
-  // learned method: a window node only considers its window's cuts
-  const uint32_t lm = nd.mode;
   const bool win = lmode_kind(lm) == LM_WINDOW;
   const int wa = (int)lmode_axis(lm);
   const uint32_t wlo = lmode_lo(lm), whi = lmode_hi(lm);
@@ -635,35 +620,34 @@ __device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32
 
   if (best != ~0ull) {
     const uint32_t q = (uint32_t)best;
-    axis = q / MG_NBINS;
-    value = q % MG_NBINS;
-    kind = 0;
+    d.axis = q / MG_NBINS;
+    d.value = q % MG_NBINS;
+    d.kind = 0;
     // the winning candidate's boxes and counts, from the lane that evaluated it
     // (prefix / suffix unions of exact min/max: the same values as summing bins)
     const uint32_t src = __builtin_amdgcn_readfirstlane(q & 63u);
     const bool p1 = q >= 64;  // wave-uniform
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-      lt.mn[x] = rl(p1 ? plt[1].mn[x] : plt[0].mn[x], src);
-      lt.mx[x] = rl(p1 ? plt[1].mx[x] : plt[0].mx[x], src);
-      gt.mn[x] = rl(p1 ? pgt[1].mn[x] : pgt[0].mn[x], src);
-      gt.mx[x] = rl(p1 ? pgt[1].mx[x] : pgt[0].mx[x], src);
+      d.lt.mn[x] = rl(p1 ? plt[1].mn[x] : plt[0].mn[x], src);
+      d.lt.mx[x] = rl(p1 ? plt[1].mx[x] : plt[0].mx[x], src);
+      d.gt.mn[x] = rl(p1 ? pgt[1].mn[x] : pgt[0].mn[x], src);
+      d.gt.mx[x] = rl(p1 ? pgt[1].mx[x] : pgt[0].mx[x], src);
     }
-    nl = (uint32_t)__builtin_amdgcn_readlane((int)(p1 ? pnl[1] : pnl[0]), (int)src);
-    ng = (uint32_t)__builtin_amdgcn_readlane((int)(p1 ? png[1] : png[0]), (int)src);
-    cost = split_cost(sa_p, box_sa(lt), box_sa(gt), nl, ng);
+    d.nl = (uint32_t)__builtin_amdgcn_readlane((int)(p1 ? pnl[1] : pnl[0]), (int)src);
+    d.ng = (uint32_t)__builtin_amdgcn_readlane((int)(p1 ? png[1] : png[0]), (int)src);
   } else {
     // Median fallback (lib.rs:444-491, 527-531).  With > 32 objects it is only
     // reached when every centroid coincides, so each axis' select keeps the
     // list order and axis 0 wins: lt = first n/2 objects.
-    const uint32_t m = nd.count / 2;
+    const uint32_t m = count / 2;
     uint32_t acc[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[k] = 0;
-    for (uint32_t j = lane; j < nd.count; j += 64) {
+    for (uint32_t j = lane; j < count; j += 64) {
       Box b;
       uint32_t id;
-      rec_load(&recs[nd.start + j], b, id);
+      rec_load(&recs[start + j], b, id);
       const int o = j < m ? 0 : 6;
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
@@ -675,21 +659,74 @@ __device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32
     for (int k = 0; k < 12; ++k) acc[k] = wave_max_u32(acc[k]);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      lt.mn[a] = acc_min_dec(acc[a]);
-      lt.mx[a] = acc_max_dec(acc[3 + a]);
-      gt.mn[a] = acc_min_dec(acc[6 + a]);
-      gt.mx[a] = acc_max_dec(acc[9 + a]);
+      d.lt.mn[a] = acc_min_dec(acc[a]);
+      d.lt.mx[a] = acc_max_dec(acc[3 + a]);
+      d.gt.mn[a] = acc_min_dec(acc[6 + a]);
+      d.gt.mx[a] = acc_max_dec(acc[9 + a]);
     }
-    nl = m;
-    ng = nd.count - m;
-    axis = 0;
-    value = m;
-    kind = 2;
-    cost = split_cost(sa_p, box_sa(lt), box_sa(gt), nl, ng);
+    d.nl = m;
+    d.ng = count - m;
+    d.axis = 0;
+    d.value = m;
+    d.kind = 2;
   }
+  d.cost = split_cost(sa_p, box_sa(d.lt), box_sa(d.gt), d.nl, d.ng);
+  d.pad[0] = d.pad[1] = 0;
+  return d;
+}
+
+constexpr uint32_t SW_WAVES = 16;  // nodes per k_sweep block (one wavefront each)
+
+// One wavefront per node with > 32 objects: select_split (lib.rs:493-534) over
+// the binned histograms, node record, children.  Children are appended to the
+// next level's lists with one atomic per list per block (a same-address atomic
+// costs ~11 ns, serialised, so per-node atomics would dominate).
+__device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32_t nL,
+                                   const uint32_t* __restrict__ hist,
+                                   const Rec* __restrict__ recs, LNode* Ln, uint32_t* cn_next,
+                                   uint32_t* ck_next, SNode* small, Counters* ctr, NodeOut out,
+                                   uint32_t max_depth, uint32_t parity_next, uint32_t learn_min,
+                                   uint32_t* lrows_next, uint32_t* lvn) {
+  // large slots, chunks, small slots, objects, window nodes, fallback nodes
+  __shared__ uint32_t s_need[SW_WAVES][6];
+  __shared__ uint32_t s_base[SW_WAVES][3];
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t lane = lane_id();
+  const uint32_t slot = group * SW_WAVES + w;
+  const bool active = slot < nL;  // wave-uniform; every wave reaches the barriers
+  Box lt = box_empty(), gt = box_empty(), nb = box_empty();
+  uint32_t nl = 0, ng = 0, axis = 0, value = 0, kind = 0;
+  float cost = 0.0f;
+  LNode nd = {};
+  if (active) {
+  const uint32_t* h = hist + (size_t)slot * HIST_WORDS;  // each lane reads its own bins
+  nd = L[slot];
+  float cmin[3], cext[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    nb.mn[a] = nd.bmn[a];
+    nb.mx[a] = nd.bmx[a];
+    cmin[a] = acc_min_dec(nd.cacc[a]);
+    cext[a] = __fsub_rn(acc_max_dec(nd.cacc[3 + a]), cmin[a]);
+  }
+  const bool decided = nd.flags & LF_DECIDED;
+  SplitDec d;
+  if (decided)  // taken by the parent's partition block (same function, same histogram)
+    d = *reinterpret_cast<const SplitDec*>(h);
+  else
+    d = split_from_hist(h, nb, cmin, cext, nd.start, nd.count, recs, nd.mode);
+  lt = d.lt;
+  gt = d.gt;
+  nl = d.nl;
+  ng = d.ng;
+  axis = d.axis;
+  value = d.value;
+  kind = d.kind;
+  cost = d.cost;
 
   // training sample of the split proposer: this node's features and choice
-  if (out.samples && kind == 0 && nd.count >= out.sample_min) {
+  // (nodes are not decided early while samples are collected)
+  if (out.samples && !decided && kind == 0 && nd.count >= out.sample_min) {
     uint32_t si = 0;
     if (lane == 0) si = atomicAdd(&ctr->n_samples, 1u);
     si = __shfl(si, 0, 64);
@@ -795,7 +832,7 @@ __device__ inline void partition_chunk(uint32_t c, const LNode* __restrict__ L,
                                        const uint32_t* __restrict__ ck,
                                        const uint32_t* __restrict__ blk,
                                        const Rec* __restrict__ src, Rec* __restrict__ dst,
-                                       LNode* Ln, uint32_t* __restrict__ hist) {
+                                       LNode* Ln, uint32_t* __restrict__ hist, uint32_t fuse) {
   __shared__ uint32_t s_red[NW];
   __shared__ uint32_t s_lt[NW], s_gt[NW];
   const uint32_t slot = cnode[c];
@@ -959,7 +996,22 @@ __device__ inline void partition_chunk(uint32_t c, const LNode* __restrict__ L,
     }
     __syncthreads();
     uint32_t* gh = hist + (size_t)cs * HIST_WORDS;
-    for (uint32_t t = threadIdx.x; t < HIST_WORDS; t += blockDim.x) gh[t] = h[t];
+    if (fuse) {
+      // the child's split straight from the histogram in LDS: its decision, not
+      // the 2688-byte row, goes to memory, and the next level's sweep skips it
+      if (w == 0) {
+        Box cb;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { cb.mn[a] = Ln[cs].bmn[a]; cb.mx[a] = Ln[cs].bmx[a]; }
+        const SplitDec d = split_from_hist(h, cb, cmin3, cext3, cstart, ccount, dst, LM_PLAIN);
+        if (lane == 0) {
+          *reinterpret_cast<SplitDec*>(gh) = d;
+          Ln[cs].flags = LF_PREBINNED | LF_DECIDED;
+        }
+      }
+    } else {
+      for (uint32_t t = threadIdx.x; t < HIST_WORDS; t += blockDim.x) gh[t] = h[t];
+    }
     __syncthreads();
   }
 }
@@ -974,10 +1026,11 @@ __global__ __launch_bounds__(NW * 64) void k_partition(const LNode* __restrict__
                                                        const Rec* __restrict__ src,
                                                        Rec* __restrict__ dst, LNode* Ln,
                                                        uint32_t* __restrict__ hist,
-                                                       const uint32_t* __restrict__ lvc) {
+                                                       const uint32_t* __restrict__ lvc,
+                                                       uint32_t fuse) {
   const uint32_t nC = lvc[LV_NC];
   for (uint32_t c = blockIdx.x; c < nC; c += gridDim.x) {
-    partition_chunk<NW>(c, L, cnode, ck, blk, src, dst, Ln, hist);
+    partition_chunk<NW>(c, L, cnode, ck, blk, src, dst, Ln, hist, fuse);
     __syncthreads();
   }
 }
@@ -1631,12 +1684,15 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
                                       (uint32_t)(cur ^ 1), lp.min_objects, lrows[cur ^ 1], lvn);
     MG_PROF(KK_SWEEP, 1);
     MG_PROF(KK_PARTITION, 0);
+    // children's splits are taken inside the partition unless samples are
+    // collected (those need the histogram rows)
+    const uint32_t fuse = out.samples ? 0u : 1u;
     if (tu.part_waves == 1)
       k_partition<1><<<g_part, 64, 0, st>>>(L[cur], cn[cur], ck[cur], blk, recs[cur],
-                                            recs[cur ^ 1], L[cur ^ 1], hist, lvc);
+                                            recs[cur ^ 1], L[cur ^ 1], hist, lvc, fuse);
     else
       k_partition<4><<<g_part, 256, 0, st>>>(L[cur], cn[cur], ck[cur], blk, recs[cur],
-                                             recs[cur ^ 1], L[cur ^ 1], hist, lvc);
+                                             recs[cur ^ 1], L[cur ^ 1], hist, lvc, fuse);
     MG_PROF(KK_PARTITION, 1);
     MG_HIP(hipGetLastError());
     if (!graph) {

@@ -45,6 +45,10 @@ struct LNode {
 // partition block (which knows the node's exact centroid bounds first), so
 // its own binning pass is skipped.
 constexpr uint32_t LF_PREBINNED = 1;
+// ... and, outside sample collection, that block also takes the node's split
+// decision from the histogram it holds in LDS and stores the decision (a
+// SplitDec) at the start of the node's histogram row instead of the row.
+constexpr uint32_t LF_DECIDED = 2;
 
 // Learned-method state of a node with > 32 objects (LNode::mode):
 // bits 0-1 kind (0 plain, 1 counted, 2 window, 3 fallback), bits 2-3 predicted
