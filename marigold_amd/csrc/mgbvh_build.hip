@@ -67,78 +67,6 @@ __device__ inline void box_grow_nz(Box& acc, const Box& o) {
   }
 }
 
-// Segmented inclusive scans inside 32-lane halves (a segment [ls, last] never
-// crosses lane 31/32) with DPP row shifts: row_shr 1, 2, 4, 8 inside each
-// 16-lane row, then row_bcast:15 carries lane 15 (47) into row 1 (3) for
-// segments that started in the row before.  Lanes outside the segment see the
-// identity `id`.
-template <int CTRL, int ROWS>
-__device__ inline uint32_t dpp_mov(uint32_t id, uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, CTRL, ROWS, 0xF, false);
-}
-
-template <class Op>
-__device__ inline uint32_t seg_scan32(uint32_t v, uint32_t id, uint32_t lane, uint32_t ls, Op op) {
-  const uint32_t rs = lane & ~15u, s = ls > rs ? ls : rs;
-  uint32_t t;
-  t = dpp_mov<0x111, 0xF>(id, v); if (lane >= s + 1) v = op(v, t);
-  t = dpp_mov<0x112, 0xF>(id, v); if (lane >= s + 2) v = op(v, t);
-  t = dpp_mov<0x114, 0xF>(id, v); if (lane >= s + 4) v = op(v, t);
-  t = dpp_mov<0x118, 0xF>(id, v); if (lane >= s + 8) v = op(v, t);
-  t = dpp_mov<0x142, 0xA>(id, v); if (ls < rs) v = op(v, t);
-  return v;
-}
-
-// Suffix (inclusive toward higher lanes) scan over each whole 32-lane half:
-// row_shl 1, 2, 4, 8 inside each 16-lane row, then rows 0 and 2 fold in the
-// suffix of row 1 (3), read from lane 16 (48).
-template <class Op>
-__device__ inline uint32_t half_suffix32(uint32_t v, uint32_t id, uint32_t lane, Op op) {
-  v = op(v, dpp_mov<0x101, 0xF>(id, v));
-  v = op(v, dpp_mov<0x102, 0xF>(id, v));
-  v = op(v, dpp_mov<0x104, 0xF>(id, v));
-  v = op(v, dpp_mov<0x108, 0xF>(id, v));
-  const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
-  const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
-  if ((lane & 16u) == 0) v = op(v, lane < 32 ? r1 : r3);
-  return v;
-}
-
-// Value of lane - 1 (DPP wave_shr:1); lane 0 gets `id`.
-__device__ inline uint32_t from_prev_lane(uint32_t v, uint32_t id) { return dpp_mov<0x138, 0xF>(id, v); }
-
-// Inclusive segmented minimum scan of u64 keys inside 32-lane halves.
-__device__ inline uint64_t seg_scan_min_u64(uint64_t v, uint32_t lane, uint32_t ls) {
-  const uint32_t rs = lane & ~15u, s = ls > rs ? ls : rs;
-  uint64_t t;
-#define MG_SEG_MIN_STEP(C, R, COND)                                                   \
-  t = ((uint64_t)dpp_mov<C, R>(0xFFFFFFFFu, (uint32_t)(v >> 32)) << 32) |             \
-      dpp_mov<C, R>(0xFFFFFFFFu, (uint32_t)v);                                        \
-  if (COND) v = t < v ? t : v;
-  MG_SEG_MIN_STEP(0x111, 0xF, lane >= s + 1)
-  MG_SEG_MIN_STEP(0x112, 0xF, lane >= s + 2)
-  MG_SEG_MIN_STEP(0x114, 0xF, lane >= s + 4)
-  MG_SEG_MIN_STEP(0x118, 0xF, lane >= s + 8)
-  MG_SEG_MIN_STEP(0x142, 0xA, ls < rs)
-#undef MG_SEG_MIN_STEP
-  return v;
-}
-
-// Minimum of u64 keys over the segment, returned on every lane of it.
-__device__ inline uint64_t seg_min_u64(uint64_t v, uint32_t lane, uint32_t ls, uint32_t last) {
-  return __shfl(seg_scan_min_u64(v, lane, ls), (int)last, 64);
-}
-
-// Minimum of u64 keys over the wavefront (the two half minima sit in lanes 31, 63).
-__device__ inline uint64_t wave_min_u64_dpp(uint64_t v, uint32_t lane) {
-  v = seg_scan_min_u64(v, lane, lane & 32u);
-  const uint64_t a = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(v >> 32), 31) << 32) |
-                     (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 31);
-  const uint64_t b = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(v >> 32), 63) << 32) |
-                     (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
-  return a < b ? a : b;
-}
-
 // Union of the boxes of lanes [ls, last] that have `take`, on every lane of
 // the segment.  Box coordinates are finite and carry no -0 (object boxes are
 // canonicalised at setup, centroids of such boxes cannot be -0), so the fold
@@ -172,19 +100,6 @@ __device__ inline uint32_t wave_max6(uint32_t v) {
     if (__ballot(v >= t)) m = t;
   }
   return m;
-}
-
-// Maximum over the whole wavefront (every lane active): row prefix maxima
-// with row_shr, then row_bcast:15 / row_bcast:31 carry them to lane 63.
-__device__ inline uint32_t wave_max_u32_dpp(uint32_t v) {
-  const auto umax = [](uint32_t a, uint32_t b) { return a > b ? a : b; };
-  v = umax(v, dpp_mov<0x111, 0xF>(0u, v));
-  v = umax(v, dpp_mov<0x112, 0xF>(0u, v));
-  v = umax(v, dpp_mov<0x114, 0xF>(0u, v));
-  v = umax(v, dpp_mov<0x118, 0xF>(0u, v));
-  v = umax(v, dpp_mov<0x142, 0xA>(0u, v));
-  v = umax(v, dpp_mov<0x143, 0xC>(0u, v));
-  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 // Adds each valid lane's object to bin `bi` of axis `a` of an LDS histogram
@@ -237,7 +152,7 @@ __device__ inline void block_acc12_flush(uint32_t (&acc)[12], uint32_t* dst) {
   __shared__ uint32_t red[4][12];
   const uint32_t w = threadIdx.x >> 6;
 #pragma unroll
-  for (int k = 0; k < 12; ++k) acc[k] = wave_max_u32(acc[k]);
+  for (int k = 0; k < 12; ++k) acc[k] = wave_max_u32_dpp(acc[k]);
   if (lane_id() == 0) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) red[w][k] = acc[k];
@@ -544,11 +459,37 @@ struct SplitDec {
 static_assert(sizeof(SplitDec) == 80, "SplitDec is 80 bytes");
 static_assert(sizeof(SplitDec) <= HIST_WORDS * 4, "a SplitDec fits a histogram row");
 
+// Inclusive scans inside each 16-lane DPP row, toward higher lanes (prefix)
+// or lower lanes (suffix); lanes past the row's edge contribute the identity,
+// so no segment test is needed.
+template <class Op>
+__device__ inline uint32_t row_prefix(uint32_t v, uint32_t id, Op op) {
+  v = op(v, dpp_mov<0x111, 0xF>(id, v));
+  v = op(v, dpp_mov<0x112, 0xF>(id, v));
+  v = op(v, dpp_mov<0x114, 0xF>(id, v));
+  v = op(v, dpp_mov<0x118, 0xF>(id, v));
+  return v;
+}
+template <class Op>
+__device__ inline uint32_t row_suffix(uint32_t v, uint32_t id, Op op) {
+  v = op(v, dpp_mov<0x101, 0xF>(id, v));
+  v = op(v, dpp_mov<0x102, 0xF>(id, v));
+  v = op(v, dpp_mov<0x104, 0xF>(id, v));
+  v = op(v, dpp_mov<0x108, 0xF>(id, v));
+  return v;
+}
+
 // One wavefront: the split of a node from its binned histogram h (global
 // memory or LDS), node box nb, centroid bounds (cmin, cext), its objects
 // [start, start + count) in recs (median fallback only) and its learned-method
 // state lm (a window node only considers its window's cuts).  Every lane
 // returns the same decision.
+//
+// Layout: DPP row a (lanes 16a .. 16a + 15) holds axis a, lane j of the row
+// bins 2j and 2j + 1, and evaluates the cuts b = 2j and 2j + 1 (lt = bins
+// [0, b), gt = bins [b, 32)).  Bin words are a count and ~okey(min) x3,
+// okey(max) x3 (0 = empty), so every fold is a sum or a u32 maximum with
+// identity 0 and each scan step is one DPP-fused instruction.
 __device__ inline SplitDec split_from_hist(const uint32_t* h, const Box& nb,
                                            const float (&cmin)[3], const float (&cext)[3],
                                            uint32_t start, uint32_t count,
@@ -557,65 +498,63 @@ __device__ inline SplitDec split_from_hist(const uint32_t* h, const Box& nb,
   SplitDec d;
   const float sa_p = box_sa(nb);
   const bool win = lmode_kind(lm) == LM_WINDOW;
-  const int wa = (int)lmode_axis(lm);
+  const uint32_t wa = lmode_axis(lm);
   const uint32_t wlo = lmode_lo(lm), whi = lmode_hi(lm);
-  // Candidates (axis a, cut i), lane = a * 32 + bin: lt = bins [0, i), gt = bins
-  // [i, 32) from 32-lane inclusive prefix / suffix scans (exact min/max unions).
-  uint64_t best = ~0ull;
-  Box plt[2], pgt[2];       // each pass's candidate boxes (lt = prefix, gt = suffix)
-  uint32_t pnl[2], png[2];
+  const uint32_t row = lane >> 4, j = lane & 15u;
+  const bool aact = row < 3;
+  const uint32_t a = aact ? row : 0u;
+  uint32_t c0 = 0, c1 = 0;
+  uint32_t e0[6] = {0u, 0u, 0u, 0u, 0u, 0u}, e1[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+  if (aact) {
+    const uint32_t* hb = &h[(a * MG_NBINS + 2 * j) * BIN_WORDS];
+    c0 = hb[0];
+    c1 = hb[BIN_WORDS];
+    if (c0) {
 #pragma unroll
-  for (uint32_t pass = 0; pass < 2; ++pass) {
-    const uint32_t q = lane + 64 * pass;
-    const bool qact = q < 3 * MG_NBINS;
-    const uint32_t b = q % MG_NBINS;
-    const int a = qact ? (int)(q / MG_NBINS) : 0;
-    // bin words: count, ~okey(min) x3, okey(max) x3 (0 = empty), so every box
-    // fold is a u32 max with identity 0; each 32-lane half holds one axis' bins
-    // and is scanned forward (prefix) and backward (suffix) with DPP
-    uint32_t cnt = 0;
-    uint32_t e[6] = {0u, 0u, 0u, 0u, 0u, 0u};
-    if (qact) {
-      const uint32_t* hb = &h[q * BIN_WORDS];
-      cnt = hb[0];
-      if (cnt) {
-#pragma unroll
-        for (int j = 0; j < 6; ++j) e[j] = hb[1 + j];
-      }
+      for (int k = 0; k < 6; ++k) e0[k] = hb[1 + k];
     }
-    const uint32_t hs = lane & 32u;
-    const auto add = [](uint32_t x, uint32_t y) { return x + y; };
-    const auto umax = [](uint32_t x, uint32_t y) { return x > y ? x : y; };
-    const uint32_t pc = seg_scan32(cnt, 0u, lane, hs, add);
-    const uint32_t sc = half_suffix32(cnt, 0u, lane, add);
-    // lt of cut b is the prefix ending at bin b - 1 (lane - 1; b >= 1 below)
-    const uint32_t cnl = from_prev_lane(pc, 0u);
-    Box clt, sb;
+    if (c1) {
 #pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      const uint32_t pe = from_prev_lane(seg_scan32(e[j], 0u, lane, hs, umax), 0u);
-      const uint32_t se = half_suffix32(e[j], 0u, lane, umax);
-      if (j < 3) {
-        clt.mn[j] = acc_min_dec(pe);
-        sb.mn[j] = acc_min_dec(se);
-      } else {
-        clt.mx[j - 3] = acc_max_dec(pe);
-        sb.mx[j - 3] = acc_max_dec(se);
-      }
-    }
-    const uint32_t cng = sc;
-    plt[pass] = clt;
-    pgt[pass] = sb;
-    pnl[pass] = cnl;
-    png[pass] = cng;
-    // lib.rs:339-341 (zero extent refused), 379-381 (MIN objects per side)
-    const bool inwin = !win || (a == wa && b >= wlo && b <= whi);
-    if (qact && inwin && b >= 1 && cext[a] != 0.0f && cnl >= 1 && cng >= 1) {
-      const float ccost = split_cost(sa_p, box_sa(clt), box_sa(sb), cnl, cng);
-      const uint64_t key = ((uint64_t)tc_ukey(ccost) << 32) | q;
-      best = key < best ? key : best;
+      for (int k = 0; k < 6; ++k) e1[k] = hb[BIN_WORDS + 1 + k];
     }
   }
+  const auto add = [](uint32_t x, uint32_t y) { return x + y; };
+  const auto umax = [](uint32_t x, uint32_t y) { return x > y ? x : y; };
+  // counts: E = bins below the lane's pair, S = bins above it
+  const uint32_t pc = c0 + c1;
+  const uint32_t Ec = dpp_mov<0x111, 0xF>(0u, row_prefix(pc, 0u, add));
+  const uint32_t Sc = dpp_mov<0x101, 0xF>(0u, row_suffix(pc, 0u, add));
+  const uint32_t nl0 = Ec, ng0 = pc + Sc;         // cut 2j
+  const uint32_t nl1 = Ec + c0, ng1 = c1 + Sc;    // cut 2j + 1
+  Box lt0, gt0, lt1, gt1;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const uint32_t pe = umax(e0[k], e1[k]);
+    const uint32_t E = dpp_mov<0x111, 0xF>(0u, row_prefix(pe, 0u, umax));
+    const uint32_t S = dpp_mov<0x101, 0xF>(0u, row_suffix(pe, 0u, umax));
+    const uint32_t l0 = E, g0 = umax(pe, S), l1 = umax(E, e0[k]), g1 = umax(e1[k], S);
+    if (k < 3) {
+      lt0.mn[k] = acc_min_dec(l0); gt0.mn[k] = acc_min_dec(g0);
+      lt1.mn[k] = acc_min_dec(l1); gt1.mn[k] = acc_min_dec(g1);
+    } else {
+      lt0.mx[k - 3] = acc_max_dec(l0); gt0.mx[k - 3] = acc_max_dec(g0);
+      lt1.mx[k - 3] = acc_max_dec(l1); gt1.mx[k - 3] = acc_max_dec(g1);
+    }
+  }
+  // candidate key (cost, axis * 32 + cut): the first minimum under total_cmp,
+  // axes 0, 1, 2, cuts ascending (lib.rs:373-388, 515-524)
+  uint64_t best = ~0ull;
+  const auto consider = [&](uint32_t b, uint32_t nl, uint32_t ng, const Box& l, const Box& g) {
+    // lib.rs:339-341 (zero extent refused), 379-381 (MIN objects per side)
+    const bool inwin = !win || (a == wa && b >= wlo && b <= whi);
+    if (aact && inwin && b >= 1 && cext[a] != 0.0f && nl >= 1 && ng >= 1) {
+      const float c = split_cost(sa_p, box_sa(l), box_sa(g), nl, ng);
+      const uint64_t key = ((uint64_t)tc_ukey(c) << 32) | (a * MG_NBINS + b);
+      best = key < best ? key : best;
+    }
+  };
+  consider(2 * j, nl0, ng0, lt0, gt0);
+  consider(2 * j + 1, nl1, ng1, lt1, gt1);
   best = wave_min_u64_dpp(best, lane);
 
   if (best != ~0ull) {
@@ -625,17 +564,17 @@ __device__ inline SplitDec split_from_hist(const uint32_t* h, const Box& nb,
     d.kind = 0;
     // the winning candidate's boxes and counts, from the lane that evaluated it
     // (prefix / suffix unions of exact min/max: the same values as summing bins)
-    const uint32_t src = __builtin_amdgcn_readfirstlane(q & 63u);
-    const bool p1 = q >= 64;  // wave-uniform
+    const uint32_t src = __builtin_amdgcn_readfirstlane(16u * d.axis + (d.value >> 1));
+    const bool odd = d.value & 1u;  // wave-uniform
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-      d.lt.mn[x] = rl(p1 ? plt[1].mn[x] : plt[0].mn[x], src);
-      d.lt.mx[x] = rl(p1 ? plt[1].mx[x] : plt[0].mx[x], src);
-      d.gt.mn[x] = rl(p1 ? pgt[1].mn[x] : pgt[0].mn[x], src);
-      d.gt.mx[x] = rl(p1 ? pgt[1].mx[x] : pgt[0].mx[x], src);
+      d.lt.mn[x] = rl(odd ? lt1.mn[x] : lt0.mn[x], src);
+      d.lt.mx[x] = rl(odd ? lt1.mx[x] : lt0.mx[x], src);
+      d.gt.mn[x] = rl(odd ? gt1.mn[x] : gt0.mn[x], src);
+      d.gt.mx[x] = rl(odd ? gt1.mx[x] : gt0.mx[x], src);
     }
-    d.nl = (uint32_t)__builtin_amdgcn_readlane((int)(p1 ? pnl[1] : pnl[0]), (int)src);
-    d.ng = (uint32_t)__builtin_amdgcn_readlane((int)(p1 ? png[1] : png[0]), (int)src);
+    d.nl = (uint32_t)__builtin_amdgcn_readlane((int)(odd ? nl1 : nl0), (int)src);
+    d.ng = (uint32_t)__builtin_amdgcn_readlane((int)(odd ? ng1 : ng0), (int)src);
   } else {
     // Median fallback (lib.rs:444-491, 527-531).  With > 32 objects it is only
     // reached when every centroid coincides, so each axis' select keeps the
@@ -656,7 +595,7 @@ __device__ inline SplitDec split_from_hist(const uint32_t* h, const Box& nb,
       }
     }
 #pragma unroll
-    for (int k = 0; k < 12; ++k) acc[k] = wave_max_u32(acc[k]);
+    for (int k = 0; k < 12; ++k) acc[k] = wave_max_u32_dpp(acc[k]);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       d.lt.mn[a] = acc_min_dec(acc[a]);
@@ -852,7 +791,7 @@ __device__ inline void partition_chunk(uint32_t c, const LNode* __restrict__ L,
       uint32_t kk = t / cut, b = t % cut;
       part += blk[(size_t)(nd.chunk0 + kk) * 96 + axis * MG_NBINS + b];
     }
-    part = wave_sum_u32(part);
+    part = wave_sum_u32_dpp(part);
     if (lane == 0) s_red[w] = part;
     __syncthreads();
     lt_before = 0;
@@ -917,8 +856,8 @@ __device__ inline void partition_chunk(uint32_t c, const LNode* __restrict__ L,
   __shared__ uint32_t red[NW][12];
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
-    acc_lt[j] = wave_max_u32(acc_lt[j]);
-    acc_gt[j] = wave_max_u32(acc_gt[j]);
+    acc_lt[j] = wave_max_u32_dpp(acc_lt[j]);
+    acc_gt[j] = wave_max_u32_dpp(acc_gt[j]);
   }
   if (lane == 0) {
 #pragma unroll

@@ -147,6 +147,103 @@ __device__ inline uint32_t wave_sum_u32(uint32_t v) {
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
+// ---- DPP cross-lane primitives (CDNA data-parallel primitives: no LDS round trip)
+// Segmented inclusive scans inside 32-lane halves (a segment [ls, last] never
+// crosses lane 31/32) with DPP row shifts: row_shr 1, 2, 4, 8 inside each
+// 16-lane row, then row_bcast:15 carries lane 15 (47) into row 1 (3) for
+// segments that started in the row before.  Lanes outside the segment see the
+// identity `id`.
+template <int CTRL, int ROWS>
+__device__ inline uint32_t dpp_mov(uint32_t id, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, CTRL, ROWS, 0xF, false);
+}
+
+template <class Op>
+__device__ inline uint32_t seg_scan32(uint32_t v, uint32_t id, uint32_t lane, uint32_t ls, Op op) {
+  const uint32_t rs = lane & ~15u, s = ls > rs ? ls : rs;
+  uint32_t t;
+  t = dpp_mov<0x111, 0xF>(id, v); if (lane >= s + 1) v = op(v, t);
+  t = dpp_mov<0x112, 0xF>(id, v); if (lane >= s + 2) v = op(v, t);
+  t = dpp_mov<0x114, 0xF>(id, v); if (lane >= s + 4) v = op(v, t);
+  t = dpp_mov<0x118, 0xF>(id, v); if (lane >= s + 8) v = op(v, t);
+  t = dpp_mov<0x142, 0xA>(id, v); if (ls < rs) v = op(v, t);
+  return v;
+}
+
+// Suffix (inclusive toward higher lanes) scan over each whole 32-lane half:
+// row_shl 1, 2, 4, 8 inside each 16-lane row, then rows 0 and 2 fold in the
+// suffix of row 1 (3), read from lane 16 (48).
+template <class Op>
+__device__ inline uint32_t half_suffix32(uint32_t v, uint32_t id, uint32_t lane, Op op) {
+  v = op(v, dpp_mov<0x101, 0xF>(id, v));
+  v = op(v, dpp_mov<0x102, 0xF>(id, v));
+  v = op(v, dpp_mov<0x104, 0xF>(id, v));
+  v = op(v, dpp_mov<0x108, 0xF>(id, v));
+  const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+  const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+  if ((lane & 16u) == 0) v = op(v, lane < 32 ? r1 : r3);
+  return v;
+}
+
+// Value of lane - 1 (DPP wave_shr:1); lane 0 gets `id`.
+__device__ inline uint32_t from_prev_lane(uint32_t v, uint32_t id) { return dpp_mov<0x138, 0xF>(id, v); }
+
+// Inclusive segmented minimum scan of u64 keys inside 32-lane halves.
+__device__ inline uint64_t seg_scan_min_u64(uint64_t v, uint32_t lane, uint32_t ls) {
+  const uint32_t rs = lane & ~15u, s = ls > rs ? ls : rs;
+  uint64_t t;
+#define MG_SEG_MIN_STEP(C, R, COND)                                                   \
+  t = ((uint64_t)dpp_mov<C, R>(0xFFFFFFFFu, (uint32_t)(v >> 32)) << 32) |             \
+      dpp_mov<C, R>(0xFFFFFFFFu, (uint32_t)v);                                        \
+  if (COND) v = t < v ? t : v;
+  MG_SEG_MIN_STEP(0x111, 0xF, lane >= s + 1)
+  MG_SEG_MIN_STEP(0x112, 0xF, lane >= s + 2)
+  MG_SEG_MIN_STEP(0x114, 0xF, lane >= s + 4)
+  MG_SEG_MIN_STEP(0x118, 0xF, lane >= s + 8)
+  MG_SEG_MIN_STEP(0x142, 0xA, ls < rs)
+#undef MG_SEG_MIN_STEP
+  return v;
+}
+
+// Minimum of u64 keys over the segment, returned on every lane of it.
+__device__ inline uint64_t seg_min_u64(uint64_t v, uint32_t lane, uint32_t ls, uint32_t last) {
+  return __shfl(seg_scan_min_u64(v, lane, ls), (int)last, 64);
+}
+
+// Minimum of u64 keys over the wavefront (the two half minima sit in lanes 31, 63).
+__device__ inline uint64_t wave_min_u64_dpp(uint64_t v, uint32_t lane) {
+  v = seg_scan_min_u64(v, lane, lane & 32u);
+  const uint64_t a = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(v >> 32), 31) << 32) |
+                     (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 31);
+  const uint64_t b = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(v >> 32), 63) << 32) |
+                     (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+  return a < b ? a : b;
+}
+
+// Maximum over the whole wavefront (every lane active): row prefix maxima
+// with row_shr, then row_bcast:15 / row_bcast:31 carry them to lane 63.
+__device__ inline uint32_t wave_max_u32_dpp(uint32_t v) {
+  const auto umax = [](uint32_t a, uint32_t b) { return a > b ? a : b; };
+  v = umax(v, dpp_mov<0x111, 0xF>(0u, v));
+  v = umax(v, dpp_mov<0x112, 0xF>(0u, v));
+  v = umax(v, dpp_mov<0x114, 0xF>(0u, v));
+  v = umax(v, dpp_mov<0x118, 0xF>(0u, v));
+  v = umax(v, dpp_mov<0x142, 0xA>(0u, v));
+  v = umax(v, dpp_mov<0x143, 0xC>(0u, v));
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Sum over the whole wavefront (every lane active), as wave_max_u32_dpp.
+__device__ inline uint32_t wave_sum_u32_dpp(uint32_t v) {
+  v += dpp_mov<0x111, 0xF>(0u, v);
+  v += dpp_mov<0x112, 0xF>(0u, v);
+  v += dpp_mov<0x114, 0xF>(0u, v);
+  v += dpp_mov<0x118, 0xF>(0u, v);
+  v += dpp_mov<0x142, 0xA>(0u, v);
+  v += dpp_mov<0x143, 0xC>(0u, v);
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 __device__ inline uint32_t lanemask_lt_popc(uint64_t mask) {
   uint32_t lo = (uint32_t)mask, hi = (uint32_t)(mask >> 32);
   return __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));

@@ -351,7 +351,7 @@ __device__ inline void decide_row(uint32_t row, LNode* __restrict__ L,
     }
     proxy[pass] = pr;
   }
-  bestk = wave_min_u64(bestk);
+  bestk = wave_min_u64_dpp(bestk, lane);
   uint32_t mode = LM_FALLBACK;
   if (bestk != ~0ull) {
     const uint32_t qb = (uint32_t)bestk;
@@ -368,8 +368,8 @@ __device__ inline void decide_row(uint32_t row, LNode* __restrict__ L,
       const uint32_t a = q / MG_NBINS, b = q % MG_NBINS;
       if (a == pa && b >= lo && b <= hi) inw = k < inw ? k : inw;
     }
-    all = wave_min_u64(all);
-    inw = wave_min_u64(inw);
+    all = wave_min_u64_dpp(all, lane);
+    inw = wave_min_u64_dpp(inw, lane);
     const float pall = okey_inv((uint32_t)all), pwin = okey_inv((uint32_t)inw);
     if (inw != ~0ull && pwin <= pall * (1.0f + guard)) mode = lmode_pack(LM_WINDOW, pa, lo, hi);
   }
