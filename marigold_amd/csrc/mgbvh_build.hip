@@ -614,134 +614,155 @@ __device__ inline SplitDec split_from_hist(const uint32_t* h, const Box& nb,
   return d;
 }
 
-constexpr uint32_t SW_WAVES = 16;  // nodes per k_sweep block (one wavefront each)
+constexpr uint32_t SW_WAVES = 16;  // wavefronts per k_sweep block
+constexpr uint32_t SW_NPW = 4;     // nodes per wavefront and step on crowded levels
+constexpr uint32_t SW_NODES = SW_WAVES * SW_NPW;
+// Levels with fewer nodes than this give each wavefront one node (their nodes
+// are large and mostly swept from histograms: parallelism first).
+constexpr uint32_t SW_CROWDED = 65536;
 
 // One wavefront per node with > 32 objects: select_split (lib.rs:493-534) over
-// the binned histograms, node record, children.  Children are appended to the
-// next level's lists with one atomic per list per block (a same-address atomic
-// costs ~11 ns, serialised, so per-node atomics would dominate).
-__device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32_t nL,
+// the binned histograms (or the decision its parent's partition block already
+// took), node record, children.  A block takes SW_WAVES * npw nodes (npw per
+// wavefront) and appends their children to the next level's lists with one
+// atomic per list: a same-address atomic costs ~11 ns, serialised across the
+// whole device, so fewer, larger allocations keep the lower levels (hundreds of
+// thousands of nodes) from waiting on the counters.  Levels with fewer nodes
+// use one node per wavefront (npw = 1).
+__device__ inline void sweep_group(uint32_t group, uint32_t npw, LNode* __restrict__ L, uint32_t nL,
                                    const uint32_t* __restrict__ hist,
                                    const Rec* __restrict__ recs, LNode* Ln, uint32_t* cn_next,
                                    uint32_t* ck_next, SNode* small, Counters* ctr, NodeOut out,
                                    uint32_t max_depth, uint32_t parity_next, uint32_t learn_min,
                                    uint32_t* lrows_next, uint32_t* lvn) {
   // large slots, chunks, small slots, objects, window nodes, fallback nodes
-  __shared__ uint32_t s_need[SW_WAVES][6];
-  __shared__ uint32_t s_base[SW_WAVES][3];
+  __shared__ uint32_t s_need[SW_NODES][6];
+  __shared__ uint32_t s_base[SW_NODES][3];
+  __shared__ SplitDec s_dec[SW_NODES];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t lane = lane_id();
-  const uint32_t slot = group * SW_WAVES + w;
-  const bool active = slot < nL;  // wave-uniform; every wave reaches the barriers
-  Box lt = box_empty(), gt = box_empty(), nb = box_empty();
-  uint32_t nl = 0, ng = 0, axis = 0, value = 0, kind = 0;
-  float cost = 0.0f;
-  LNode nd = {};
-  if (active) {
-  const uint32_t* h = hist + (size_t)slot * HIST_WORDS;  // each lane reads its own bins
-  nd = L[slot];
-  float cmin[3], cext[3];
+  const uint32_t gnodes = SW_WAVES * npw;
+  // phase 1: the split of each of the wavefront's nodes
+  for (uint32_t t = 0; t < npw; ++t) {
+    const uint32_t e = w * npw + t;
+    const uint32_t slot = group * gnodes + e;
+    const bool active = slot < nL;  // wave-uniform
+    uint32_t need[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+    if (active) {
+      const uint32_t* h = hist + (size_t)slot * HIST_WORDS;  // each lane reads its own bins
+      const LNode nd = L[slot];
+      Box nb;
+      float cmin[3], cext[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    nb.mn[a] = nd.bmn[a];
-    nb.mx[a] = nd.bmx[a];
-    cmin[a] = acc_min_dec(nd.cacc[a]);
-    cext[a] = __fsub_rn(acc_max_dec(nd.cacc[3 + a]), cmin[a]);
-  }
-  const bool decided = nd.flags & LF_DECIDED;
-  SplitDec d;
-  if (decided)  // taken by the parent's partition block (same function, same histogram)
-    d = *reinterpret_cast<const SplitDec*>(h);
-  else
-    d = split_from_hist(h, nb, cmin, cext, nd.start, nd.count, recs, nd.mode);
-  lt = d.lt;
-  gt = d.gt;
-  nl = d.nl;
-  ng = d.ng;
-  axis = d.axis;
-  value = d.value;
-  kind = d.kind;
-  cost = d.cost;
-
-  // training sample of the split proposer: this node's features and choice
-  // (nodes are not decided early while samples are collected)
-  if (out.samples && !decided && kind == 0 && nd.count >= out.sample_min) {
-    uint32_t si = 0;
-    if (lane == 0) si = atomicAdd(&ctr->n_samples, 1u);
-    si = __shfl(si, 0, 64);
-    if (si < out.max_samples) {
-      float* f = out.samples + (size_t)si * MLP_IN;
-      const float inv = 1.0f / log1pf((float)nd.count);
-      for (uint32_t q = lane; q < 3 * MG_NBINS; q += 64) f[q] = log1pf((float)h[q * BIN_WORDS]) * inv;
-      if (lane == 0) {
-        node_extent_features(nb.mn, nb.mx, cext, f + 3 * MG_NBINS);
-        out.labels[si] = axis * MG_NBINS + value;
+      for (int a = 0; a < 3; ++a) {
+        nb.mn[a] = nd.bmn[a];
+        nb.mx[a] = nd.bmx[a];
+        cmin[a] = acc_min_dec(nd.cacc[a]);
+        cext[a] = __fsub_rn(acc_max_dec(nd.cacc[3 + a]), cmin[a]);
       }
+      const bool decided = nd.flags & LF_DECIDED;
+      SplitDec d;
+      if (decided)  // taken by the parent's partition block (same function, same histogram)
+        d = *reinterpret_cast<const SplitDec*>(h);
+      else
+        d = split_from_hist(h, nb, cmin, cext, nd.start, nd.count, recs, nd.mode);
+      // training sample of the split proposer: this node's features and choice
+      // (nodes are not decided early while samples are collected)
+      if (out.samples && !decided && d.kind == 0 && nd.count >= out.sample_min) {
+        uint32_t si = 0;
+        if (lane == 0) si = atomicAdd(&ctr->n_samples, 1u);
+        si = __shfl(si, 0, 64);
+        if (si < out.max_samples) {
+          float* f = out.samples + (size_t)si * MLP_IN;
+          const float inv = 1.0f / log1pf((float)nd.count);
+          for (uint32_t q = lane; q < 3 * MG_NBINS; q += 64) f[q] = log1pf((float)h[q * BIN_WORDS]) * inv;
+          if (lane == 0) {
+            node_extent_features(nb.mn, nb.mx, cext, f + 3 * MG_NBINS);
+            out.labels[si] = d.axis * MG_NBINS + d.value;
+          }
+        }
+      }
+      // children (lib.rs:648-663): gt at P, lt at P + 1
+      const uint32_t cd = nd.depth + 1;
+      const bool lg = d.ng > MG_SMALL && cd < max_depth;
+      const bool ll = d.nl > MG_SMALL && cd < max_depth;
+      need[0] = (uint32_t)lg + (uint32_t)ll;
+      need[1] = (lg ? (d.ng + CH - 1) / CH : 0) + (ll ? (d.nl + CH - 1) / CH : 0);
+      need[2] = 2 - need[0];
+      need[3] = nd.count;
+      need[4] = lmode_kind(nd.mode) == LM_WINDOW;
+      need[5] = lmode_kind(nd.mode) == LM_FALLBACK;
+      if (lane == 0) s_dec[e] = d;
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) s_need[e][k] = need[k];
     }
   }
-  }  // active
-
-  // children (lib.rs:648-663): gt at P, lt at P + 1
-  const uint32_t cd = nd.depth + 1;
-  const bool lg = active && ng > MG_SMALL && cd < max_depth;
-  const bool ll = active && nl > MG_SMALL && cd < max_depth;
-  const uint32_t chg = lg ? (ng + CH - 1) / CH : 0, chl = ll ? (nl + CH - 1) / CH : 0;
-  if (lane == 0) {
-    s_need[w][0] = (uint32_t)lg + (uint32_t)ll;
-    s_need[w][1] = chg + chl;
-    s_need[w][2] = active ? 2 - ((uint32_t)lg + (uint32_t)ll) : 0;
-    s_need[w][3] = active ? nd.count : 0;
-    s_need[w][4] = active && lmode_kind(nd.mode) == LM_WINDOW;
-    s_need[w][5] = active && lmode_kind(nd.mode) == LM_FALLBACK;
-  }
   __syncthreads();
+  // phase 2: one allocation per list for the block's nodes
   if (threadIdx.x == 3) {  // work counter: objects of the nodes split on this level
     unsigned long long tot = 0;
-    for (uint32_t q = 0; q < SW_WAVES; ++q) tot += s_need[q][3];
+    for (uint32_t q = 0; q < gnodes; ++q) tot += s_need[q][3];
     if (tot) atomicAdd(&ctr->sum_large, tot);
   }
   if (threadIdx.x == 4 || threadIdx.x == 5) {  // learned method: window / fallback nodes
     uint32_t tot = 0;
-    for (uint32_t q = 0; q < SW_WAVES; ++q) tot += s_need[q][threadIdx.x];
+    for (uint32_t q = 0; q < gnodes; ++q) tot += s_need[q][threadIdx.x];
     if (tot) atomicAdd(threadIdx.x == 4 ? &ctr->n_window : &ctr->n_fallback, tot);
   }
   if (threadIdx.x < 3) {
     const uint32_t k = threadIdx.x;
     uint32_t tot = 0;
-    for (uint32_t q = 0; q < SW_WAVES; ++q) tot += s_need[q][k];
+    for (uint32_t q = 0; q < gnodes; ++q) tot += s_need[q][k];
     uint32_t* counter = k == 0 ? &lvn[LV_NL] : (k == 1 ? &lvn[LV_NC] : &ctr->n_small);
     uint32_t run = tot ? atomicAdd(counter, tot) : 0;
-    for (uint32_t q = 0; q < SW_WAVES; ++q) {
+    for (uint32_t q = 0; q < gnodes; ++q) {
       s_base[q][k] = run;
       run += s_need[q][k];
     }
   }
   __syncthreads();
-  if (!active) return;
-  const uint32_t bl = s_base[w][0], bc = s_base[w][1], bs = s_base[w][2];
-  const uint32_t sgs = lg ? bl : bs;                          // gt child slot
-  const uint32_t sls = ll ? bl + (uint32_t)lg : bs + (uint32_t)!lg;  // lt child slot
-  const uint32_t c0g = bc, c0l = bc + chg;
-  if (lane == 0) {
-    out.write_node(nd.idx, nb, nd.start - nd.obase, nd.P - nd.nbase, nd.count);
-    out.write_split(nd.idx, axis | (kind << 8), value, nl, fbits(cost));
-    if (kind == 2) atomicAdd(&ctr->n_median, 1u);
-    put_child(lg, sgs, c0g, nd.start + nl, ng, gt, nd.P, nd.P + 2, cd, parity_next, nd.nbase,
-              nd.obase, Ln, small, learn_min, lvn, lrows_next, nd.nchunk == 1);
-    put_child(ll, sls, c0l, nd.start, nl, lt, nd.P + 1, nd.P + 2 + 2 * (ng - 1), cd, parity_next,
-              nd.nbase, nd.obase, Ln, small, learn_min, lvn, lrows_next, nd.nchunk == 1);
-    L[slot].axis = axis;
-    L[slot].cut = value;
-    L[slot].n_lt = nl;
-    L[slot].kind = kind;
-    L[slot].child_slot[0] = lg ? (int32_t)sgs : -1;
-    L[slot].child_slot[1] = ll ? (int32_t)sls : -1;
+  // phase 3: node records and children
+  for (uint32_t t = 0; t < npw; ++t) {
+    const uint32_t e = w * npw + t;
+    const uint32_t slot = group * gnodes + e;
+    if (slot >= nL) break;  // wave-uniform; later nodes of this wavefront are past the list too
+    const LNode nd = L[slot];
+    const SplitDec d = s_dec[e];
+    Box nb;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { nb.mn[a] = nd.bmn[a]; nb.mx[a] = nd.bmx[a]; }
+    const uint32_t cd = nd.depth + 1;
+    const bool lg = d.ng > MG_SMALL && cd < max_depth;
+    const bool ll = d.nl > MG_SMALL && cd < max_depth;
+    const uint32_t chg = lg ? (d.ng + CH - 1) / CH : 0, chl = ll ? (d.nl + CH - 1) / CH : 0;
+    const uint32_t bl = s_base[e][0], bc = s_base[e][1], bs = s_base[e][2];
+    const uint32_t sgs = lg ? bl : bs;                                // gt child slot
+    const uint32_t sls = ll ? bl + (uint32_t)lg : bs + (uint32_t)!lg;  // lt child slot
+    const uint32_t c0g = bc, c0l = bc + chg;
+    if (lane == 0) {
+      out.write_node(nd.idx, nb, nd.start - nd.obase, nd.P - nd.nbase, nd.count);
+      out.write_split(nd.idx, d.axis | (d.kind << 8), d.value, d.nl, fbits(d.cost));
+      if (d.kind == 2) atomicAdd(&ctr->n_median, 1u);
+      put_child(lg, sgs, c0g, nd.start + d.nl, d.ng, d.gt, nd.P, nd.P + 2, cd, parity_next,
+                nd.nbase, nd.obase, Ln, small, learn_min, lvn, lrows_next, nd.nchunk == 1);
+      put_child(ll, sls, c0l, nd.start, d.nl, d.lt, nd.P + 1, nd.P + 2 + 2 * (d.ng - 1), cd,
+                parity_next, nd.nbase, nd.obase, Ln, small, learn_min, lvn, lrows_next,
+                nd.nchunk == 1);
+      L[slot].axis = d.axis;
+      L[slot].cut = d.value;
+      L[slot].n_lt = d.nl;
+      L[slot].kind = d.kind;
+      L[slot].child_slot[0] = lg ? (int32_t)sgs : -1;
+      L[slot].child_slot[1] = ll ? (int32_t)sls : -1;
+    }
+    for (uint32_t k = lane; k < chg; k += 64) { cn_next[c0g + k] = sgs; ck_next[c0g + k] = k; }
+    for (uint32_t k = lane; k < chl; k += 64) { cn_next[c0l + k] = sls; ck_next[c0l + k] = k; }
   }
-  for (uint32_t k = lane; k < chg; k += 64) { cn_next[c0g + k] = sgs; ck_next[c0g + k] = k; }
-  for (uint32_t k = lane; k < chl; k += 64) { cn_next[c0l + k] = sls; ck_next[c0l + k] = k; }
 }
 
-// Persistent over the level's node queue, SW_WAVES nodes per block and step.
+// Persistent over the level's node queue, SW_WAVES * npw nodes per block and step.
 // lvc: this level's counts; lvn: the next level's, filled by the allocation.
 __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L,
                                                 const uint32_t* __restrict__ lvc,
@@ -753,9 +774,10 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L,
                                                 uint32_t learn_min, uint32_t* lrows_next,
                                                 uint32_t* lvn) {
   const uint32_t nL = lvc[LV_NL];
-  const uint32_t ngroups = (nL + SW_WAVES - 1) / SW_WAVES;
+  const uint32_t npw = nL >= SW_CROWDED ? SW_NPW : 1u;
+  const uint32_t ngroups = (nL + SW_WAVES * npw - 1) / (SW_WAVES * npw);
   for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
-    sweep_group(g, L, nL, hist, recs, Ln, cn_next, ck_next, small, ctr, out, max_depth,
+    sweep_group(g, npw, L, nL, hist, recs, Ln, cn_next, ck_next, small, ctr, out, max_depth,
                 parity_next, learn_min, lrows_next, lvn);
     __syncthreads();
   }
@@ -1250,7 +1272,8 @@ __device__ inline void small_pair(uint32_t wave, SmallObjs* S, const SNode* __re
         pred = r < m;
       }
     }
-    if (act && !med) cost = split_cost(sa_p, box_sa(clt), box_sa(cgt), n_lt, lc - n_lt);
+    // the winner's cost, as its lane computed it, from the key
+    if (act && !med) cost = tc_ukey_inv((uint32_t)(skey >> 32));
     // a split always leaves both children non-empty; anything else is a bug and
     // stops this subtree rather than writing outside its slots
     const bool bad = act && (n_lt == 0 || n_lt >= lc);

@@ -46,6 +46,11 @@ __device__ inline uint32_t tc_ukey(float f) {
   i ^= (int32_t)(((uint32_t)(i >> 31)) >> 1);
   return (uint32_t)i ^ 0x80000000u;
 }
+// The float (bits) a tc_ukey came from.
+__device__ inline float tc_ukey_inv(uint32_t k) {
+  const uint32_t i = k ^ 0x80000000u;
+  return __uint_as_float((i & 0x80000000u) ? i ^ 0x7fffffffu : i);
+}
 
 // x86-64 SSE produces the default NaN 0xFFC00000 for invalid operations; all
 // NaN costs are that value (inputs are finite).
