@@ -337,7 +337,13 @@ __device__ inline void bin_chunk(uint32_t c, const LNode* __restrict__ L,
                                  const uint32_t* __restrict__ cnode,
                                  const uint32_t* __restrict__ ck, const Rec* __restrict__ recs,
                                  uint32_t* __restrict__ hist, uint32_t* __restrict__ blk, int pass) {
-  __shared__ uint32_t h[HIST_WORDS];
+  // BIN_COPIES copies of the row, lane l adding to copy l % BIN_COPIES: lanes of
+  // one wavefront that share a bin mostly hit different addresses (LDS
+  // serialises same-address atomics).  The odd stride puts the copies of a
+  // word in different banks.  The flush merges the copies (sums and max folds
+  // are order independent).
+  constexpr uint32_t BIN_COPIES = 4, COPY_STRIDE = HIST_WORDS + 1;
+  __shared__ uint32_t h[BIN_COPIES * COPY_STRIDE];
   const uint32_t slot = cnode[c];
   const uint32_t k = ck[c];
   const LNode& nd = L[slot];
@@ -345,7 +351,8 @@ __device__ inline void bin_chunk(uint32_t c, const LNode* __restrict__ L,
   if (nd.flags & LF_PREBINNED) return;       // block-uniform: binned by the parent's partition
   if (pass == 2 && lk < LM_WINDOW) return;  // block-uniform: nothing left to bin
   const bool single = nd.nchunk == 1;
-  for (uint32_t t = threadIdx.x; t < HIST_WORDS; t += blockDim.x) h[t] = 0;
+  for (uint32_t t = threadIdx.x; t < BIN_COPIES * COPY_STRIDE; t += blockDim.x) h[t] = 0;
+  uint32_t* hl = h + (lane_id() % BIN_COPIES) * COPY_STRIDE;
   float cmin[3], cext[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -373,14 +380,19 @@ __device__ inline void bin_chunk(uint32_t c, const LNode* __restrict__ L,
       const bool boxes = pass == 1 ? lk == LM_PLAIN : (lk == LM_FALLBACK || a == (int)la);
       if (pass == 2 && !boxes) continue;  // block-uniform
       const uint32_t bi = valid ? bin_index(center(b.mn[a], b.mx[a]), cmin[a], cext[a]) : 0u;
-      hist_add(h, (uint32_t)a, bi, valid, e, pass == 1, boxes);
+      hist_add(hl, (uint32_t)a, bi, valid, e, pass == 1, boxes);
     }
   }
   __syncthreads();
   uint32_t* gh = hist + (size_t)slot * HIST_WORDS;
   for (uint32_t t = threadIdx.x; t < HIST_WORDS; t += blockDim.x) {
-    uint32_t v = h[t];
     uint32_t f = t % BIN_WORDS;
+    uint32_t v = h[t];
+#pragma unroll
+    for (uint32_t q = 1; q < BIN_COPIES; ++q) {
+      const uint32_t u = h[q * COPY_STRIDE + t];
+      v = f == 0 ? v + u : max(v, u);
+    }
     if (f == 0) {
       if (pass == 1) {
         blk[(size_t)c * 96 + t / BIN_WORDS] = v;
