@@ -404,6 +404,12 @@ int mgbvh_scene_plan_create(mgbvh_ctx* ctx, const float* d_positions, uint32_t v
 int mgbvh_scene_plan_launch(mgbvh_scene_plan* plan, float* out_ms,
                             uint32_t* out_blas_node_counts, uint64_t* out_tlas_nodes);
 int mgbvh_scene_plan_destroy(mgbvh_scene_plan* plan);
+/* Levels the plan's graph runs for the BLASes and the TLAS (sized from a
+ * level-by-level build at creation, plus a margin of 2), and how many launches
+ * so far found a tree deeper than that and were rebuilt level by level (same
+ * results, slower launch). */
+int mgbvh_scene_plan_info(const mgbvh_scene_plan* plan, uint32_t* blas_levels,
+                          uint32_t* tlas_levels, uint32_t* launches_rebuilt);
 
 /* Persistent grid sizes of the level kernels (blocks per compute unit, 1..64),
  * the knobs the autotuner (marigold_amd/autotune.py) sets; results never
@@ -416,7 +422,8 @@ typedef struct mgbvh_tuning {
   uint32_t compute_units;  /* read-only (get) */
   uint32_t partition_waves; /* wavefronts per partition block: 1 or 4 (0 = keep) */
   uint32_t bin_waves;       /* wavefronts per binning block: 1..4 (0 = keep) */
-  uint32_t reserved[1];
+  uint32_t sweep_pack_min_nodes; /* levels with at least this many nodes give each sweep
+                                    wavefront 4 nodes (0 = keep; 1 = every level) */
 } mgbvh_tuning;
 int mgbvh_ctx_set_tuning(mgbvh_ctx* ctx, const mgbvh_tuning* tuning);
 int mgbvh_ctx_get_tuning(mgbvh_ctx* ctx, mgbvh_tuning* tuning);

@@ -49,10 +49,10 @@ class Tuning(C.Structure):
     _fields_ = [("bin_blocks_per_cu", C.c_uint32), ("partition_blocks_per_cu", C.c_uint32),
                 ("sweep_blocks_per_cu", C.c_uint32), ("small_blocks_per_cu", C.c_uint32),
                 ("compute_units", C.c_uint32), ("partition_waves", C.c_uint32),
-                ("bin_waves", C.c_uint32), ("reserved", C.c_uint32 * 1)]
+                ("bin_waves", C.c_uint32), ("sweep_pack_min_nodes", C.c_uint32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_[:7]}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Camera(C.Structure):
@@ -196,6 +196,8 @@ _EXTRA: dict = {
                                  C.POINTER(_P)], _I),
     "mgbvh_scene_plan_launch": ([_P, C.POINTER(C.c_float), _P, C.POINTER(C.c_uint64)], _I),
     "mgbvh_scene_plan_destroy": ([_P], _I),
+    "mgbvh_scene_plan_info": ([_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                               C.POINTER(C.c_uint32)], _I),
     "mgbvh_ctx_set_tuning": ([_P, _P], _I),
     "mgbvh_ctx_get_tuning": ([_P, _P], _I),
     "mgbvh_tree_sah_device": ([_P, _P, _P, _P, _P, _U32, _P], _I),
@@ -364,6 +366,12 @@ class ScenePlan:
         counts = np.zeros(max(len(self.meshes), 1), dtype=np.uint32)
         check(lib().mgbvh_scene_plan_launch(self.handle, C.byref(ms), ptr(counts), C.byref(nt)))
         return ms.value, counts[:len(self.meshes)], nt.value
+
+    def info(self) -> dict:
+        """Levels the graph runs (BLAS, TLAS) and launches rebuilt level by level."""
+        b, t, r = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().mgbvh_scene_plan_info(self.handle, C.byref(b), C.byref(t), C.byref(r)))
+        return {"blas_levels": b.value, "tlas_levels": t.value, "launches_rebuilt": r.value}
 
     def close(self):
         if self.handle is not None:

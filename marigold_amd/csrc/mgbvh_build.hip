@@ -629,9 +629,6 @@ __device__ inline SplitDec split_from_hist(const uint32_t* h, const Box& nb,
 constexpr uint32_t SW_WAVES = 16;  // wavefronts per k_sweep block
 constexpr uint32_t SW_NPW = 4;     // nodes per wavefront and step on crowded levels
 constexpr uint32_t SW_NODES = SW_WAVES * SW_NPW;
-// Levels with fewer nodes than this give each wavefront one node (their nodes
-// are large and mostly swept from histograms: parallelism first).
-constexpr uint32_t SW_CROWDED = 65536;
 
 // One wavefront per node with > 32 objects: select_split (lib.rs:493-534) over
 // the binned histograms (or the decision its parent's partition block already
@@ -784,9 +781,11 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L,
                                                 SNode* small, Counters* ctr, NodeOut out,
                                                 uint32_t max_depth, uint32_t parity_next,
                                                 uint32_t learn_min, uint32_t* lrows_next,
-                                                uint32_t* lvn) {
+                                                uint32_t* lvn, uint32_t pack_min) {
+  // levels with fewer nodes than pack_min give each wavefront one node (their
+  // nodes are large and mostly swept from histograms: parallelism first)
   const uint32_t nL = lvc[LV_NL];
-  const uint32_t npw = nL >= SW_CROWDED ? SW_NPW : 1u;
+  const uint32_t npw = nL >= pack_min ? SW_NPW : 1u;
   const uint32_t ngroups = (nL + SW_WAVES * npw - 1) / (SW_WAVES * npw);
   for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
     sweep_group(g, npw, L, nL, hist, recs, Ln, cn_next, ck_next, small, ctr, out, max_depth,
@@ -1655,7 +1654,8 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
     MG_PROF(KK_SWEEP, 0);
     k_sweep<<<g_sweep, 1024, 0, st>>>(L[cur], lvc, hist, recs[cur], L[cur ^ 1], cn[cur ^ 1],
                                       ck[cur ^ 1], small, ctr, out, s->max_depth,
-                                      (uint32_t)(cur ^ 1), lp.min_objects, lrows[cur ^ 1], lvn);
+                                      (uint32_t)(cur ^ 1), lp.min_objects, lrows[cur ^ 1], lvn,
+                                      tu.sweep_pack);
     MG_PROF(KK_SWEEP, 1);
     MG_PROF(KK_PARTITION, 0);
     // children's splits are taken inside the partition unless samples are
@@ -1710,6 +1710,12 @@ int finish_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, BuildResult* res) {
     return MGBVH_ERR_NONFINITE;
   }
   if (hc.err & ERR_INTERNAL) { set_error("small-subtree split count out of range"); return MGBVH_ERR_INTERNAL; }
+  if (p.fixed_levels) {  // a replayed graph: did its levels reach every node?
+    uint32_t waiting = 0;
+    MG_HIP(hipMemcpy(&waiting, (uint32_t*)sc.lv.p + (size_t)p.fixed_levels * LV_WORDS + LV_NL, 4,
+                     hipMemcpyDeviceToHost));
+    if (waiting) return MG_LEVELS_SHORT;
+  }
   res->mesh_nodes.assign(nm, 0);
   res->mesh_nbase.assign(nm, 0);
   for (uint32_t i = 0; i < nm; ++i) {

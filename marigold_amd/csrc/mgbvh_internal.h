@@ -230,6 +230,7 @@ void scratch_free(Scratch& sc);
 struct Tuning {
   uint32_t bin_bpc = 32, part_bpc = 32, sweep_bpc = 4, small_bpc = 16;
   uint32_t part_waves = 1, bin_waves = 4;  // wavefronts per block
+  uint32_t sweep_pack = 65536;             // sweep: 4 nodes per wavefront from this many nodes
 };
 }  // namespace mg
 
@@ -274,7 +275,11 @@ struct BuildPlan {
   uint32_t capL = 0, capC = 0, max_levels = 0;
   LearnParams lp{0, 0, 0.0f, 0};
   uint32_t* lens = nullptr;  // o.lens or scratch
+  // levels a recorded graph runs (0: not a graph); finish_build reports
+  // MG_LEVELS_SHORT when nodes were left waiting at that level
+  uint32_t fixed_levels = 0;
 };
+constexpr int MG_LEVELS_SHORT = 1000;  // internal: a graph ran too few levels
 int plan_build(mgbvh_ctx* c, Scratch& sc, const BuildInput& in, const std::vector<MeshIn>& meshes,
                const mgbvh_settings* s, const BuildOutput& o, BuildPlan* p);
 // fixed_levels = 0: level by level with read-backs; else exactly that many levels.

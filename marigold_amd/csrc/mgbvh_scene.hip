@@ -194,9 +194,35 @@ struct mgbvh_scene_plan {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   std::vector<uint32_t> blas_counts;
   uint64_t tlas_nodes = 0;
+  uint32_t n_redo = 0;  // launches rebuilt level by level (trees deeper than recorded)
 };
 
 namespace mg {
+// The plan's scene build level by level (read-backs after every level): sizes
+// the recorded graph, and redoes a replay whose trees needed more levels.
+static int plan_levels(mgbvh_scene_plan* pl, BuildResult* rb, BuildResult* rt) {
+  mgbvh_ctx* c = pl->c;
+  hipStream_t st = c->stream;
+  const uint32_t fb = pl->bp.fixed_levels, ft = pl->tp.fixed_levels;
+  pl->bp.fixed_levels = pl->tp.fixed_levels = 0;
+  uint32_t lv_run = 0;
+  int rc = enqueue_build(c, pl->bsc, pl->bp, 0, false, &lv_run);
+  if (!rc) rc = finish_build(c, pl->bsc, pl->bp, rb);
+  if (!rc && pl->ninst) {
+    k_fill<<<1, 64, 0, st>>>((uint32_t*)pl->flag.p, 0u, 1);
+    k_instance_boxes<<<(pl->ninst + 255) / 256, 256, 0, st>>>(
+        pl->blas_nodes, (const mgbvh_blas_desc*)pl->blas_desc.p, pl->nblas, pl->o2w,
+        pl->inst_blas, pl->ninst, pl->boxes, (uint32_t*)pl->flag.p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = hip_fail(e, "k_instance_boxes");
+    if (!rc) rc = enqueue_build(c, pl->tsc, pl->tp, 0, false, &lv_run);
+    if (!rc) rc = finish_build(c, pl->tsc, pl->tp, rt);
+  }
+  pl->bp.fixed_levels = fb;
+  pl->tp.fixed_levels = ft;
+  return rc;
+}
+
 static void plan_release(mgbvh_scene_plan* pl) {
   if (!pl) return;
   if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
@@ -312,13 +338,23 @@ int mgbvh_scene_plan_create(mgbvh_ctx* c, const float* d_pos, uint32_t vstride_b
       (rc = model_bf16(c, c->model)))
     return fail(rc);
   if ((e = hipStreamSynchronize(st))) return fail(hip_fail(e, "plan upload"));
+  // one build level by level gives each tree's level count; the graph runs
+  // that many levels plus a margin (launch redoes a replay that needed more)
+  {
+    BuildResult rb, rt;
+    if ((rc = plan_levels(pl, &rb, &rt))) return fail(rc);
+    constexpr uint32_t kMargin = 2;
+    pl->bp.fixed_levels = std::max<uint32_t>(1, std::min(pl->bp.max_levels, rb.n_levels + kMargin));
+    if (ninstances)
+      pl->tp.fixed_levels = std::max<uint32_t>(1, std::min(pl->tp.max_levels, rt.n_levels + kMargin));
+  }
   // record
   const uint32_t prof = c->prof_mask;
   c->prof_mask = 0;
   e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) { c->prof_mask = prof; return fail(hip_fail(e, "hipStreamBeginCapture")); }
   uint32_t lv_run = 0;
-  rc = enqueue_build(c, pl->bsc, pl->bp, pl->bp.max_levels ? pl->bp.max_levels : 1, false, &lv_run);
+  rc = enqueue_build(c, pl->bsc, pl->bp, pl->bp.fixed_levels, false, &lv_run);
   if (!rc && ninstances) {
     k_fill<<<1, 64, 0, st>>>((uint32_t*)pl->flag.p, 0u, 1);
     k_instance_boxes<<<(ninstances + 255) / 256, 256, 0, st>>>(
@@ -327,8 +363,7 @@ int mgbvh_scene_plan_create(mgbvh_ctx* c, const float* d_pos, uint32_t vstride_b
     e = hipGetLastError();
     if (e != hipSuccess) rc = hip_fail(e, "k_instance_boxes");
     if (!rc)
-      rc = enqueue_build(c, pl->tsc, pl->tp, pl->tp.max_levels ? pl->tp.max_levels : 1, false,
-                         &lv_run);
+      rc = enqueue_build(c, pl->tsc, pl->tp, pl->tp.fixed_levels, false, &lv_run);
   }
   hipGraph_t g = nullptr;
   e = hipStreamEndCapture(st, &g);
@@ -354,9 +389,21 @@ int mgbvh_scene_plan_launch(mgbvh_scene_plan* pl, float* out_ms, uint32_t* out_b
     return hip_fail(e, "hipGraphLaunch");
   float ms = 0.0f;
   if ((e = hipEventElapsedTime(&ms, pl->e0, pl->e1))) return hip_fail(e, "hipEventElapsedTime");
-  BuildResult r;
+  BuildResult r, rt;
   int rc = finish_build(c, pl->bsc, pl->bp, &r);
-  if (rc) return rc;
+  bool redo = rc == MG_LEVELS_SHORT;
+  if (rc && !redo) return rc;
+  if (!redo && pl->ninst) {
+    rc = finish_build(c, pl->tsc, pl->tp, &rt);
+    redo = rc == MG_LEVELS_SHORT;
+    if (rc && !redo) return rc;
+  }
+  if (redo) {
+    // the new geometry needs deeper trees than the graph records: build this
+    // scene level by level instead (same results; slower for this launch)
+    if ((rc = plan_levels(pl, &r, &rt))) return rc;
+    pl->n_redo++;
+  }
   if (out_blas_node_counts)
     for (uint32_t i = 0; i < pl->nblas; ++i) out_blas_node_counts[i] = r.mesh_nodes[i];
   if (pl->ninst) {
@@ -364,13 +411,20 @@ int mgbvh_scene_plan_launch(mgbvh_scene_plan* pl, float* out_ms, uint32_t* out_b
     if ((e = hipMemcpy(&flag, pl->flag.p, 4, hipMemcpyDeviceToHost)))
       return hip_fail(e, "hipMemcpy(flag)");
     if (flag) { set_error("an instance names a BLAS >= nmeshes"); return MGBVH_ERR_INDEX; }
-    BuildResult rt;
-    if ((rc = finish_build(c, pl->tsc, pl->tp, &rt))) return rc;
     if (out_tlas_nodes) *out_tlas_nodes = rt.nnodes;
   } else if (out_tlas_nodes) {
     *out_tlas_nodes = 0;
   }
   if (out_ms) *out_ms = ms;
+  return MGBVH_OK;
+}
+
+int mgbvh_scene_plan_info(const mgbvh_scene_plan* pl, uint32_t* blas_levels,
+                          uint32_t* tlas_levels, uint32_t* launches_rebuilt) {
+  if (!pl) { set_error("plan is NULL"); return MGBVH_ERR_INVALID_ARG; }
+  if (blas_levels) *blas_levels = pl->bp.fixed_levels;
+  if (tlas_levels) *tlas_levels = pl->tp.fixed_levels;
+  if (launches_rebuilt) *launches_rebuilt = pl->n_redo;
   return MGBVH_OK;
 }
 
@@ -398,6 +452,7 @@ int mgbvh_ctx_set_tuning(mgbvh_ctx* c, const mgbvh_tuning* t) {
   c->tune.small_bpc = t->small_blocks_per_cu;
   if (t->partition_waves) c->tune.part_waves = t->partition_waves;
   if (t->bin_waves) c->tune.bin_waves = t->bin_waves;
+  if (t->sweep_pack_min_nodes) c->tune.sweep_pack = t->sweep_pack_min_nodes;
   return MGBVH_OK;
 }
 
@@ -411,6 +466,7 @@ int mgbvh_ctx_get_tuning(mgbvh_ctx* c, mgbvh_tuning* t) {
   t->compute_units = (uint32_t)c->cus;
   t->partition_waves = c->tune.part_waves;
   t->bin_waves = c->tune.bin_waves;
+  t->sweep_pack_min_nodes = c->tune.sweep_pack;
   return MGBVH_OK;
 }
 

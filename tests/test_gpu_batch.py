@@ -220,16 +220,34 @@ def test_scene_graph_replays_equal_level_by_level(ctx):
     got, want2 = _scene_arrays(g, None), _scene_arrays(ref2, None)
     for k in want2:
         assert np.array_equal(got[k].view(np.uint32), want2[k].view(np.uint32)), ("moved", k)
-    for s_ in (ref, g, ref2):
+    assert g.plan.info()["launches_rebuilt"] == 0
+    # outliers at 4, 16, ..., 4^24 on the largest mesh: its tree needs about 8
+    # more levels than the graph records, so the launch rebuilds level by level
+    big = int(np.argmax([t.shape[0] for _, t in meshes]))
+    deep = [(p.copy(), t) for p, t in meshes]
+    k = np.arange(24)
+    q = deep[big][0]
+    q[k * 7 % q.shape[0], 0] = np.float32(4.0) ** (k + 1).astype(np.float32)
+    ref3 = GpuScene(ctx, deep, sc["transforms"], sc["instance_mesh"], mode="batched")
+    g.d["pos"].upload(np.ascontiguousarray(np.concatenate([p for p, _ in deep])))
+    g.build()
+    assert g.plan.info()["launches_rebuilt"] == 1, g.plan.info()
+    got, want3 = _scene_arrays(g, None), _scene_arrays(ref3, None)
+    for k_ in want3:
+        assert np.array_equal(got[k_].view(np.uint32), want3[k_].view(np.uint32)), ("deep", k_)
+    assert np.array_equal(g.blas_node_counts, ref3.blas_node_counts)
+    for s_ in (ref, g, ref2, ref3):
         s_.free()
 
 
 @pytest.mark.parametrize("tuning", [dict(bin_blocks_per_cu=1, partition_blocks_per_cu=1,
                                          sweep_blocks_per_cu=1, small_blocks_per_cu=1,
-                                         partition_waves=4, bin_waves=1),
+                                         partition_waves=4, bin_waves=1,
+                                         sweep_pack_min_nodes=1),
                                     dict(bin_blocks_per_cu=32, partition_blocks_per_cu=16,
                                          sweep_blocks_per_cu=4, small_blocks_per_cu=64,
-                                         partition_waves=1, bin_waves=2)])
+                                         partition_waves=1, bin_waves=2,
+                                         sweep_pack_min_nodes=1 << 30)])
 def test_tuning_does_not_change_results(ctx, tuning):
     print(f"[case] tuning {tuning}", flush=True)
     meshes = _mixed_meshes()
