@@ -560,7 +560,7 @@ __device__ inline SplitDec split_from_hist(const uint32_t* h, const Box& nb,
     // lib.rs:339-341 (zero extent refused), 379-381 (MIN objects per side)
     const bool inwin = !win || (a == wa && b >= wlo && b <= whi);
     if (aact && inwin && b >= 1 && cext[a] != 0.0f && nl >= 1 && ng >= 1) {
-      const float c = split_cost(sa_p, box_sa(l), box_sa(g), nl, ng);
+      const float c = split_cost(sa_p, box_sa_ne(l), box_sa_ne(g), nl, ng);  // nl, ng >= 1
       const uint64_t key = ((uint64_t)tc_ukey(c) << 32) | (a * MG_NBINS + b);
       best = key < best ? key : best;
     }
@@ -1212,7 +1212,8 @@ __device__ inline void small_pair(uint32_t wave, SmallObjs* S, const SNode* __re
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       if (act && nl[a] >= 1 && ng[a] >= 1) {
-        const float cst = split_cost(sa_p, box_sa(lt[a]), box_sa(gt[a]), nl[a], ng[a]);
+        // both sides hold objects: their boxes are not empty
+        const float cst = split_cost(sa_p, box_sa_ne(lt[a]), box_sa_ne(gt[a]), nl[a], ng[a]);
         const uint64_t k2 = ((uint64_t)tc_ukey(cst) << 32) | (uint32_t)(a * 32 + r);
         if (k2 < key) { key = k2; ba = a; }
       }

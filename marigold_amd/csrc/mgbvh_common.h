@@ -93,6 +93,17 @@ __device__ inline float box_sa(const Box& b) {
   return __fmul_rn(2.0f, s);
 }
 
+// box_sa of a box known to hold at least one object (never empty): the same
+// value without the emptiness test.
+__device__ inline float box_sa_ne(const Box& b) {
+  const float w = __fsub_rn(b.mx[0], b.mn[0]);
+  const float h = __fsub_rn(b.mx[1], b.mn[1]);
+  const float d = __fsub_rn(b.mx[2], b.mn[2]);
+  float s = __fadd_rn(__fmul_rn(w, h), __fmul_rn(w, d));
+  s = __fadd_rn(s, __fmul_rn(h, d));
+  return __fmul_rn(2.0f, s);
+}
+
 // (1 + ((SA_lt / SA_p) * 8) * n_lt) + ((SA_gt / SA_p) * 8) * n_gt, lib.rs:278-288.
 __device__ inline float split_cost(float sa_p, float sa_lt, float sa_gt, uint32_t n_lt,
                                    uint32_t n_gt) {
