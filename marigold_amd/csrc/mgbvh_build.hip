@@ -416,6 +416,9 @@ __global__ __launch_bounds__(256) void k_bin(const LNode* __restrict__ L,
                                              uint32_t* __restrict__ hist,
                                              uint32_t* __restrict__ blk, int pass,
                                              const uint32_t* __restrict__ lvc) {
+  // pass 2 only bins the boxes of learned-method nodes: none on this level
+  // (every level of a recorded graph launches it) means nothing to do
+  if (pass == 2 && lvc[LV_NCOUNTED] == 0) return;
   const uint32_t nC = lvc[LV_NC];
   for (uint32_t c = blockIdx.x; c < nC; c += gridDim.x) {
     bin_chunk(c, L, cnode, ck, recs, hist, blk, pass);
