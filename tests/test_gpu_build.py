@@ -38,6 +38,23 @@ def _check_tris(ctx, pos, tris, max_depth=32, bounds=None, tag=""):
     return ref, stats
 
 
+def test_centroids_on_bin_boundaries(ctx):
+    # point triangles whose x centroids sit on k/32 of the root's centroid range
+    # and one ulp either side: the multiply-by-reciprocal bin index must hand
+    # these to the exact division (root: several chunks; children: one chunk)
+    rng = np.random.default_rng(21)
+    base = np.arange(33, dtype=np.float32) / np.float32(32)
+    xs = np.concatenate([base, np.nextafter(base, np.float32(-1)), np.nextafter(base, np.float32(2))])
+    xs = np.clip(xs, np.float32(0), np.float32(1)).astype(np.float32)
+    n = 5000
+    x = xs[rng.integers(0, xs.size, n)]
+    x[:2] = [0.0, 1.0]
+    pts = np.stack([x, rng.random(n, dtype=np.float32), rng.random(n, dtype=np.float32)], axis=1)
+    pos = np.repeat(pts, 3, axis=0).astype(np.float32)
+    tris = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    _check_tris(ctx, pos, tris, tag="bin-boundaries")
+
+
 def test_soup_65536_bit_exact(ctx):
     pos, tris = scenes.soup(65536, seed=1)
     ref, st = _check_tris(ctx, pos, tris, tag="soup65536")
