@@ -28,18 +28,24 @@
 namespace mg {
 
 // --------------------------------------------------------------- records
+typedef float f4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef float f3_a4 __attribute__((ext_vector_type(3), aligned(4)));
 __device__ inline void rec_load(const Rec* r, Box& b, uint32_t& id) {
-  const float4* p = reinterpret_cast<const float4*>(r);
-  float4 a = p[0];
-  float4 c = p[1];
+  const float* p = reinterpret_cast<const float*>(r);
+  const f4_a4 a = *reinterpret_cast<const f4_a4*>(p);
+  const f3_a4 c = *reinterpret_cast<const f3_a4*>(p + 4);
   b.mn[0] = a.x; b.mn[1] = a.y; b.mn[2] = a.z;
   b.mx[0] = a.w; b.mx[1] = c.x; b.mx[2] = c.y;
   id = __float_as_uint(c.z);
 }
 __device__ inline void rec_store(Rec* r, const Box& b, uint32_t id) {
-  float4* p = reinterpret_cast<float4*>(r);
-  p[0] = make_float4(b.mn[0], b.mn[1], b.mn[2], b.mx[0]);
-  p[1] = make_float4(b.mx[1], b.mx[2], __uint_as_float(id), 0.0f);
+  float* p = reinterpret_cast<float*>(r);
+  f4_a4 a;
+  a.x = b.mn[0]; a.y = b.mn[1]; a.z = b.mn[2]; a.w = b.mx[0];
+  f3_a4 c;
+  c.x = b.mx[1]; c.y = b.mx[2]; c.z = __uint_as_float(id);
+  *reinterpret_cast<f4_a4*>(p) = a;
+  *reinterpret_cast<f3_a4*>(p + 4) = c;
 }
 
 __device__ inline void wave_sync() {

@@ -18,13 +18,14 @@ constexpr uint32_t ERR_INDEX = 1;
 constexpr uint32_t ERR_NONFINITE = 2;
 constexpr uint32_t ERR_INTERNAL = 4;   // a device-side invariant failed (split count out of range)
 
-// One object during the build: its box and original index (32 bytes, two dwordx4).
-struct __align__(16) Rec {
+// One object during the build: its box and original index, 28 bytes, packed
+// (4-byte aligned; read and written as a 16-byte and a 12-byte access).
+struct Rec {
   float mn[3];
   float mx[3];
   uint32_t id;
-  uint32_t pad;
 };
+static_assert(sizeof(Rec) == 28, "Rec is 28 bytes");
 
 // A node with > 32 objects at the current breadth-first level.  Object
 // positions, idx and P are absolute (over all meshes of a batched build);
