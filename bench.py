@@ -33,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+REC_BYTES = 28.0  # the builder's object record: box (6 floats) + object index
 
 
 def tree_terms(nodes: np.ndarray, lens: np.ndarray):
@@ -61,15 +62,15 @@ def kernel_bytes(kind: str, n: int, nverts: int, vstride: int, t: dict) -> float
     """Algorithmic HBM bytes per build of one kernel class (each byte the
     algorithm must move once; scratch re-reads by the implementation excluded)."""
     if kind == "setup":
-        return 12.0 * n + 12.0 * nverts + 32.0 * n          # indices, positions, object records
+        return 12.0 * n + 12.0 * nverts + REC_BYTES * n      # indices, positions, object records
     if kind == "bin":
-        return 32.0 * t["s_large"]                            # one record read per object per level
+        return REC_BYTES * t["s_large"]                       # one record read per object per level
     if kind == "partition":
-        return 64.0 * t["s_large"]                            # record read + write
+        return 2 * REC_BYTES * t["s_large"]                   # record read + write
     if kind == "sweep":
         return t["n_large"] * (3 * 32 * 7 * 4 + 32.0)        # histogram read + node write
     if kind == "small":  # records, node + len writes, perm write, permuted triangle read + write
-        return 32.0 * n + 36.0 * t["n_small_nodes"] + 4.0 * n + 24.0 * n
+        return REC_BYTES * n + 36.0 * t["n_small_nodes"] + 4.0 * n + 24.0 * n
     return 0.0
 
 
