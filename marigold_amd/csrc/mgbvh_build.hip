@@ -287,6 +287,7 @@ __global__ void k_root(const MeshRow* __restrict__ rows, const uint32_t* __restr
     lv0[LV_NL] = n_large;
     lv0[LV_NC] = n_chunks;
     lv0[LV_NCOUNTED] = n_counted;
+    lv0[LV_NBIN] = n_chunks;  // the roots are binned by k_bin
   }
   const MeshRow mr = rows[m];
   if (mr.root_kind == 0) return;
@@ -422,8 +423,10 @@ __global__ __launch_bounds__(256) void k_bin(const LNode* __restrict__ L,
                                              uint32_t* __restrict__ hist,
                                              uint32_t* __restrict__ blk, int pass,
                                              const uint32_t* __restrict__ lvc) {
-  // pass 2 only bins the boxes of learned-method nodes: none on this level
-  // (every level of a recorded graph launches it) means nothing to do
+  // Every level of a recorded graph launches both passes.  No chunk to bin
+  // (the lower levels' nodes were all binned by their parents' partition
+  // blocks) or, for pass 2, no learned-method node means nothing to do.
+  if (lvc[LV_NBIN] == 0) return;
   if (pass == 2 && lvc[LV_NCOUNTED] == 0) return;
   const uint32_t nC = lvc[LV_NC];
   for (uint32_t c = blockIdx.x; c < nC; c += gridDim.x) {
@@ -653,8 +656,9 @@ __device__ inline void sweep_group(uint32_t group, uint32_t npw, LNode* __restri
                                    uint32_t* ck_next, SNode* small, Counters* ctr, NodeOut out,
                                    uint32_t max_depth, uint32_t parity_next, uint32_t learn_min,
                                    uint32_t* lrows_next, uint32_t* lvn) {
-  // large slots, chunks, small slots, objects, window nodes, fallback nodes
-  __shared__ uint32_t s_need[SW_NODES][6];
+  // large slots, chunks, small slots, objects, window nodes, fallback nodes,
+  // chunks left to k_bin
+  __shared__ uint32_t s_need[SW_NODES][7];
   __shared__ uint32_t s_base[SW_NODES][3];
   __shared__ SplitDec s_dec[SW_NODES];
   const uint32_t w = threadIdx.x >> 6;
@@ -665,7 +669,7 @@ __device__ inline void sweep_group(uint32_t group, uint32_t npw, LNode* __restri
     const uint32_t e = w * npw + t;
     const uint32_t slot = group * gnodes + e;
     const bool active = slot < nL;  // wave-uniform
-    uint32_t need[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+    uint32_t need[7] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
     if (active) {
       const uint32_t* h = hist + (size_t)slot * HIST_WORDS;  // each lane reads its own bins
       const LNode nd = L[slot];
@@ -710,11 +714,12 @@ __device__ inline void sweep_group(uint32_t group, uint32_t npw, LNode* __restri
       need[3] = nd.count;
       need[4] = lmode_kind(nd.mode) == LM_WINDOW;
       need[5] = lmode_kind(nd.mode) == LM_FALLBACK;
+      need[6] = nd.nchunk > 1 ? need[1] : 0u;  // children of a one-block node are binned there
       if (lane == 0) s_dec[e] = d;
     }
     if (lane == 0) {
 #pragma unroll
-      for (int k = 0; k < 6; ++k) s_need[e][k] = need[k];
+      for (int k = 0; k < 7; ++k) s_need[e][k] = need[k];
     }
   }
   __syncthreads();
@@ -724,10 +729,12 @@ __device__ inline void sweep_group(uint32_t group, uint32_t npw, LNode* __restri
     for (uint32_t q = 0; q < gnodes; ++q) tot += s_need[q][3];
     if (tot) atomicAdd(&ctr->sum_large, tot);
   }
-  if (threadIdx.x == 4 || threadIdx.x == 5) {  // learned method: window / fallback nodes
+  if (threadIdx.x >= 4 && threadIdx.x < 7) {  // window / fallback nodes, chunks to bin
     uint32_t tot = 0;
     for (uint32_t q = 0; q < gnodes; ++q) tot += s_need[q][threadIdx.x];
-    if (tot) atomicAdd(threadIdx.x == 4 ? &ctr->n_window : &ctr->n_fallback, tot);
+    uint32_t* dst = threadIdx.x == 4 ? &ctr->n_window
+                    : (threadIdx.x == 5 ? &ctr->n_fallback : &lvn[LV_NBIN]);
+    if (tot) atomicAdd(dst, tot);
   }
   if (threadIdx.x < 3) {
     const uint32_t k = threadIdx.x;
@@ -1627,7 +1634,7 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
 
   // level-by-level: exact grids from the counts read back after each level;
   // fixed levels (graph capture): persistent grids sized for the capacities
-  uint32_t nL = p.n_large, nC = (uint32_t)p.lchunks, n_counted = p.n_counted;
+  uint32_t nL = p.n_large, nC = (uint32_t)p.lchunks, n_counted = p.n_counted, nBin = nC;
   int cur = 0;
   uint32_t level = 0;
   for (;; ++level) {
@@ -1636,6 +1643,7 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
       nL = p.capL;
       nC = p.capC;
       n_counted = lp.min_objects ? p.capL : 0;
+      nBin = nC;
     } else {
       if (nL == 0) break;
       if (level > p.max_levels + 1) { set_error("level loop did not terminate"); return MGBVH_ERR_INTERNAL; }
@@ -1646,10 +1654,12 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
     const uint32_t g_part = std::max<uint32_t>(1, std::min<uint32_t>(nC, cus * tu.part_bpc));
     const uint32_t g_sweep =
         std::max<uint32_t>(1, std::min<uint32_t>(cdiv(nL, SW_WAVES), cus * tu.sweep_bpc));
-    MG_PROF(KK_BIN, 0);
-    k_bin<<<g_bin, 64 * tu.bin_waves, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk, 1,
-                                                lvc);
-    MG_PROF(KK_BIN, 1);
+    if (nBin) {  // level by level: skipped once every node was binned by its parent
+      MG_PROF(KK_BIN, 0);
+      k_bin<<<g_bin, 64 * tu.bin_waves, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk,
+                                                  1, lvc);
+      MG_PROF(KK_BIN, 1);
+    }
     if (n_counted) {
       MG_PROF(KK_LEARN, 0);
       if ((rc = learned_decide(c, sc, L[cur], lrows[cur], lvc + LV_NCOUNTED, n_counted, hist,
@@ -1687,6 +1697,7 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
       nL = sc.h_ctr->pad[LV_NL];
       nC = sc.h_ctr->pad[LV_NC];
       n_counted = sc.h_ctr->pad[LV_NCOUNTED];
+      nBin = sc.h_ctr->pad[LV_NBIN];
     }
     cur ^= 1;
   }

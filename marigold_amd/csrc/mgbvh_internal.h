@@ -103,7 +103,9 @@ struct Counters {
 // Per-level work counts in device memory (LV_WORDS words per level): the
 // level kernels read their own level's counts and the sweep fills the next
 // level's, so a whole build can be enqueued without reading counts back.
-constexpr uint32_t LV_NL = 0, LV_NC = 1, LV_NCOUNTED = 2, LV_WORDS = 4;
+// LV_NBIN: chunks of nodes that k_bin bins (a node whose parent's partition
+// fitted one block was binned there); 0 lets k_bin return at once.
+constexpr uint32_t LV_NL = 0, LV_NC = 1, LV_NCOUNTED = 2, LV_NBIN = 3, LV_WORDS = 4;
 
 // Split proposer network: 102 features -> 128 -> 128 (ReLU) -> 96 logits
 // (axis * 32 + cut).  Features of a node with > 32 objects: log1p(count) /
