@@ -422,8 +422,9 @@ typedef struct mgbvh_tuning {
   uint32_t compute_units;  /* read-only (get) */
   uint32_t partition_waves; /* wavefronts per partition block: 1 or 4 (0 = keep) */
   uint32_t bin_waves;       /* wavefronts per binning block: 1..4 (0 = keep) */
-  uint32_t sweep_pack_min_nodes; /* levels with at least this many nodes give each sweep
-                                    wavefront 4 nodes (0 = keep; 1 = every level) */
+  uint32_t sweep_pack_min_nodes; /* levels with at least this many nodes are swept one node
+                                    per lane, the others one node per wavefront
+                                    (0 = keep; 1 = every level) */
 } mgbvh_tuning;
 int mgbvh_ctx_set_tuning(mgbvh_ctx* ctx, const mgbvh_tuning* tuning);
 int mgbvh_ctx_get_tuning(mgbvh_ctx* ctx, mgbvh_tuning* tuning);

@@ -639,174 +639,296 @@ __device__ inline SplitDec split_from_hist(const uint32_t* h, const Box& nb,
 }
 
 constexpr uint32_t SW_WAVES = 16;  // wavefronts per k_sweep block
-constexpr uint32_t SW_NPW = 4;     // nodes per wavefront and step on crowded levels
-constexpr uint32_t SW_NODES = SW_WAVES * SW_NPW;
 
-// One wavefront per node with > 32 objects: select_split (lib.rs:493-534) over
-// the binned histograms (or the decision its parent's partition block already
-// took), node record, children.  A block takes SW_WAVES * npw nodes (npw per
-// wavefront) and appends their children to the next level's lists with one
-// atomic per list: a same-address atomic costs ~11 ns, serialised across the
-// whole device, so fewer, larger allocations keep the lower levels (hundreds of
-// thousands of nodes) from waiting on the counters.  Levels with fewer nodes
-// use one node per wavefront (npw = 1).
-__device__ inline void sweep_group(uint32_t group, uint32_t npw, LNode* __restrict__ L, uint32_t nL,
+// Training sample of the split proposer: a node's bin counts, its extents and
+// the split it got (whole wavefront; nodes are not decided early while
+// samples are collected).
+__device__ inline void put_sample(const NodeOut& out, Counters* ctr, const uint32_t* h,
+                                  const LNode& nd, const Box& nb, const float (&cext)[3],
+                                  const SplitDec& d) {
+  if (!out.samples || d.kind != 0 || nd.count < out.sample_min) return;  // wave-uniform
+  const uint32_t lane = lane_id();
+  uint32_t si = 0;
+  if (lane == 0) si = atomicAdd(&ctr->n_samples, 1u);
+  si = __shfl(si, 0, 64);
+  if (si < out.max_samples) {
+    float* f = out.samples + (size_t)si * MLP_IN;
+    const float inv = 1.0f / log1pf((float)nd.count);
+    for (uint32_t q = lane; q < 3 * MG_NBINS; q += 64) f[q] = log1pf((float)h[q * BIN_WORDS]) * inv;
+    if (lane == 0) {
+      node_extent_features(nb.mn, nb.mx, cext, f + 3 * MG_NBINS);
+      out.labels[si] = d.axis * MG_NBINS + d.value;
+    }
+  }
+}
+
+// select_split (lib.rs:493-534) of a node from its histogram row h, by the
+// whole wavefront (every lane returns the decision).
+__device__ inline SplitDec sweep_split(const LNode& nd, const uint32_t* h,
+                                       const Rec* __restrict__ recs, const NodeOut& out,
+                                       Counters* ctr) {
+  Box nb;
+  float cmin[3], cext[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    nb.mn[a] = nd.bmn[a];
+    nb.mx[a] = nd.bmx[a];
+    cmin[a] = acc_min_dec(nd.cacc[a]);
+    cext[a] = __fsub_rn(acc_max_dec(nd.cacc[3 + a]), cmin[a]);
+  }
+  const SplitDec d = split_from_hist(h, nb, cmin, cext, nd.start, nd.count, recs, nd.mode);
+  put_sample(out, ctr, h, nd, nb, cext, d);
+  return d;
+}
+
+// What a split node asks of the next level: large slots, chunks, small slots,
+// objects (work counter), window nodes, fallback nodes, chunks left to k_bin.
+// Children (lib.rs:648-663): one with > 32 objects below max_depth is split
+// on the next level, any other goes to the small-subtree list.
+constexpr int SW_NEEDS = 7;
+__device__ inline void sweep_need(const LNode& nd, const SplitDec& d, uint32_t max_depth,
+                                  uint32_t (&need)[SW_NEEDS]) {
+  const uint32_t cd = nd.depth + 1;
+  const bool lg = d.ng > MG_SMALL && cd < max_depth;
+  const bool ll = d.nl > MG_SMALL && cd < max_depth;
+  need[0] = (uint32_t)lg + (uint32_t)ll;
+  need[1] = (lg ? (d.ng + CH - 1) / CH : 0) + (ll ? (d.nl + CH - 1) / CH : 0);
+  need[2] = 2 - need[0];
+  need[3] = nd.count;
+  need[4] = lmode_kind(nd.mode) == LM_WINDOW;
+  need[5] = lmode_kind(nd.mode) == LM_FALLBACK;
+  need[6] = nd.nchunk > 1 ? need[1] : 0u;  // children of a one-block node are binned there
+}
+
+// Adds a block's total of need k to the next level's list or counter; for
+// the three lists the base of the block's range comes back in s_gbase[k].
+__device__ inline void sweep_alloc(uint32_t k, uint32_t tot, uint32_t* s_gbase, Counters* ctr,
+                                   uint32_t* lvn) {
+  if (k < 3) {
+    uint32_t* counter = k == 0 ? &lvn[LV_NL] : (k == 1 ? &lvn[LV_NC] : &ctr->n_small);
+    s_gbase[k] = tot ? atomicAdd(counter, tot) : 0u;
+  } else if (tot) {
+    if (k == 3) atomicAdd(&ctr->sum_large, (unsigned long long)tot);
+    else atomicAdd(k == 4 ? &ctr->n_window : (k == 5 ? &ctr->n_fallback : &lvn[LV_NBIN]), tot);
+  }
+}
+
+// Where a split node's children go, from its bases in the next level's
+// large (bl), chunk (bc) and small (bs) lists.
+struct SweepSlots {
+  bool lg, ll;
+  uint32_t sgs, sls, c0g, c0l, chg, chl;
+};
+__device__ inline SweepSlots sweep_slots(const LNode& nd, const SplitDec& d, uint32_t bl,
+                                         uint32_t bc, uint32_t bs, uint32_t max_depth) {
+  SweepSlots r;
+  const uint32_t cd = nd.depth + 1;
+  r.lg = d.ng > MG_SMALL && cd < max_depth;
+  r.ll = d.nl > MG_SMALL && cd < max_depth;
+  r.chg = r.lg ? (d.ng + CH - 1) / CH : 0;
+  r.chl = r.ll ? (d.nl + CH - 1) / CH : 0;
+  r.sgs = r.lg ? bl : bs;                                    // gt child slot
+  r.sls = r.ll ? bl + (uint32_t)r.lg : bs + (uint32_t)!r.lg;  // lt child slot
+  r.c0g = bc;
+  r.c0l = bc + r.chg;
+  return r;
+}
+
+// Node record, split record, children and the node's split fields (read by
+// the partition).  One lane; the chunk lists are written by the caller.
+__device__ inline void sweep_emit(LNode* __restrict__ L, uint32_t slot, const LNode& nd,
+                                  const SplitDec& d, const SweepSlots& r, LNode* Ln,
+                                  SNode* small, Counters* ctr, const NodeOut& out,
+                                  uint32_t parity_next, uint32_t learn_min, uint32_t* lrows_next,
+                                  uint32_t* lvn) {
+  Box nb;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) { nb.mn[a] = nd.bmn[a]; nb.mx[a] = nd.bmx[a]; }
+  const uint32_t cd = nd.depth + 1;
+  out.write_node(nd.idx, nb, nd.start - nd.obase, nd.P - nd.nbase, nd.count);
+  out.write_split(nd.idx, d.axis | (d.kind << 8), d.value, d.nl, fbits(d.cost));
+  if (d.kind == 2) atomicAdd(&ctr->n_median, 1u);
+  // gt child at P, lt child at P + 1
+  put_child(r.lg, r.sgs, r.c0g, nd.start + d.nl, d.ng, d.gt, nd.P, nd.P + 2, cd, parity_next,
+            nd.nbase, nd.obase, Ln, small, learn_min, lvn, lrows_next, nd.nchunk == 1);
+  put_child(r.ll, r.sls, r.c0l, nd.start, d.nl, d.lt, nd.P + 1, nd.P + 2 + 2 * (d.ng - 1), cd,
+            parity_next, nd.nbase, nd.obase, Ln, small, learn_min, lvn, lrows_next,
+            nd.nchunk == 1);
+  L[slot].axis = d.axis;
+  L[slot].cut = d.value;
+  L[slot].n_lt = d.nl;
+  L[slot].kind = d.kind;
+  L[slot].child_slot[0] = r.lg ? (int32_t)r.sgs : -1;
+  L[slot].child_slot[1] = r.ll ? (int32_t)r.sls : -1;
+}
+
+// Levels with few, mostly large nodes: one wavefront per node.  The split
+// comes from the node's histogram (whole wavefront) or is the decision its
+// parent's partition block already took.  A block appends the children of
+// its SW_WAVES nodes with one atomic per list: a same-address atomic costs
+// ~11 ns, serialised across the whole device.
+__device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32_t nL,
                                    const uint32_t* __restrict__ hist,
                                    const Rec* __restrict__ recs, LNode* Ln, uint32_t* cn_next,
                                    uint32_t* ck_next, SNode* small, Counters* ctr, NodeOut out,
                                    uint32_t max_depth, uint32_t parity_next, uint32_t learn_min,
                                    uint32_t* lrows_next, uint32_t* lvn) {
-  // large slots, chunks, small slots, objects, window nodes, fallback nodes,
-  // chunks left to k_bin
-  __shared__ uint32_t s_need[SW_NODES][7];
-  __shared__ uint32_t s_base[SW_NODES][3];
-  __shared__ SplitDec s_dec[SW_NODES];
+  __shared__ uint32_t s_need[SW_WAVES][SW_NEEDS];
+  __shared__ uint32_t s_base[SW_WAVES][3];
+  __shared__ uint32_t s_gbase[3];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t lane = lane_id();
-  const uint32_t gnodes = SW_WAVES * npw;
-  // phase 1: the split of each of the wavefront's nodes
-  for (uint32_t t = 0; t < npw; ++t) {
-    const uint32_t e = w * npw + t;
-    const uint32_t slot = group * gnodes + e;
-    const bool active = slot < nL;  // wave-uniform
-    uint32_t need[7] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
-    if (active) {
-      const uint32_t* h = hist + (size_t)slot * HIST_WORDS;  // each lane reads its own bins
-      const LNode nd = L[slot];
-      Box nb;
-      float cmin[3], cext[3];
+  const uint32_t slot = group * SW_WAVES + w;
+  const bool active = slot < nL;  // wave-uniform
+  uint32_t need[SW_NEEDS] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  LNode nd = {};
+  SplitDec d = {};
+  if (active) {
+    nd = L[slot];
+    const uint32_t* h = hist + (size_t)slot * HIST_WORDS;
+    if (nd.flags & LF_DECIDED)  // taken by the parent's partition block (same function, same histogram)
+      d = *reinterpret_cast<const SplitDec*>(h);
+    else
+      d = sweep_split(nd, h, recs, out, ctr);
+    sweep_need(nd, d, max_depth, need);
+  }
+  if (lane == 0) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        nb.mn[a] = nd.bmn[a];
-        nb.mx[a] = nd.bmx[a];
-        cmin[a] = acc_min_dec(nd.cacc[a]);
-        cext[a] = __fsub_rn(acc_max_dec(nd.cacc[3 + a]), cmin[a]);
-      }
-      const bool decided = nd.flags & LF_DECIDED;
-      SplitDec d;
-      if (decided)  // taken by the parent's partition block (same function, same histogram)
-        d = *reinterpret_cast<const SplitDec*>(h);
-      else
-        d = split_from_hist(h, nb, cmin, cext, nd.start, nd.count, recs, nd.mode);
-      // training sample of the split proposer: this node's features and choice
-      // (nodes are not decided early while samples are collected)
-      if (out.samples && !decided && d.kind == 0 && nd.count >= out.sample_min) {
-        uint32_t si = 0;
-        if (lane == 0) si = atomicAdd(&ctr->n_samples, 1u);
-        si = __shfl(si, 0, 64);
-        if (si < out.max_samples) {
-          float* f = out.samples + (size_t)si * MLP_IN;
-          const float inv = 1.0f / log1pf((float)nd.count);
-          for (uint32_t q = lane; q < 3 * MG_NBINS; q += 64) f[q] = log1pf((float)h[q * BIN_WORDS]) * inv;
-          if (lane == 0) {
-            node_extent_features(nb.mn, nb.mx, cext, f + 3 * MG_NBINS);
-            out.labels[si] = d.axis * MG_NBINS + d.value;
-          }
-        }
-      }
-      // children (lib.rs:648-663): gt at P, lt at P + 1
-      const uint32_t cd = nd.depth + 1;
-      const bool lg = d.ng > MG_SMALL && cd < max_depth;
-      const bool ll = d.nl > MG_SMALL && cd < max_depth;
-      need[0] = (uint32_t)lg + (uint32_t)ll;
-      need[1] = (lg ? (d.ng + CH - 1) / CH : 0) + (ll ? (d.nl + CH - 1) / CH : 0);
-      need[2] = 2 - need[0];
-      need[3] = nd.count;
-      need[4] = lmode_kind(nd.mode) == LM_WINDOW;
-      need[5] = lmode_kind(nd.mode) == LM_FALLBACK;
-      need[6] = nd.nchunk > 1 ? need[1] : 0u;  // children of a one-block node are binned there
-      if (lane == 0) s_dec[e] = d;
-    }
-    if (lane == 0) {
-#pragma unroll
-      for (int k = 0; k < 7; ++k) s_need[e][k] = need[k];
-    }
+    for (int k = 0; k < SW_NEEDS; ++k) s_need[w][k] = need[k];
   }
   __syncthreads();
-  // phase 2: one allocation per list for the block's nodes
-  if (threadIdx.x == 3) {  // work counter: objects of the nodes split on this level
-    unsigned long long tot = 0;
-    for (uint32_t q = 0; q < gnodes; ++q) tot += s_need[q][3];
-    if (tot) atomicAdd(&ctr->sum_large, tot);
-  }
-  if (threadIdx.x >= 4 && threadIdx.x < 7) {  // window / fallback nodes, chunks to bin
-    uint32_t tot = 0;
-    for (uint32_t q = 0; q < gnodes; ++q) tot += s_need[q][threadIdx.x];
-    uint32_t* dst = threadIdx.x == 4 ? &ctr->n_window
-                    : (threadIdx.x == 5 ? &ctr->n_fallback : &lvn[LV_NBIN]);
-    if (tot) atomicAdd(dst, tot);
-  }
-  if (threadIdx.x < 3) {
+  if (threadIdx.x < SW_NEEDS) {
     const uint32_t k = threadIdx.x;
     uint32_t tot = 0;
-    for (uint32_t q = 0; q < gnodes; ++q) tot += s_need[q][k];
-    uint32_t* counter = k == 0 ? &lvn[LV_NL] : (k == 1 ? &lvn[LV_NC] : &ctr->n_small);
-    uint32_t run = tot ? atomicAdd(counter, tot) : 0;
-    for (uint32_t q = 0; q < gnodes; ++q) {
-      s_base[q][k] = run;
-      run += s_need[q][k];
+    for (uint32_t q = 0; q < SW_WAVES; ++q) tot += s_need[q][k];
+    sweep_alloc(k, tot, s_gbase, ctr, lvn);
+    if (k < 3) {
+      uint32_t run = s_gbase[k];
+      for (uint32_t q = 0; q < SW_WAVES; ++q) {
+        s_base[q][k] = run;
+        run += s_need[q][k];
+      }
     }
   }
   __syncthreads();
-  // phase 3: node records and children
-  for (uint32_t t = 0; t < npw; ++t) {
-    const uint32_t e = w * npw + t;
-    const uint32_t slot = group * gnodes + e;
-    if (slot >= nL) break;  // wave-uniform; later nodes of this wavefront are past the list too
-    const LNode nd = L[slot];
-    const SplitDec d = s_dec[e];
-    Box nb;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) { nb.mn[a] = nd.bmn[a]; nb.mx[a] = nd.bmx[a]; }
-    const uint32_t cd = nd.depth + 1;
-    const bool lg = d.ng > MG_SMALL && cd < max_depth;
-    const bool ll = d.nl > MG_SMALL && cd < max_depth;
-    const uint32_t chg = lg ? (d.ng + CH - 1) / CH : 0, chl = ll ? (d.nl + CH - 1) / CH : 0;
-    const uint32_t bl = s_base[e][0], bc = s_base[e][1], bs = s_base[e][2];
-    const uint32_t sgs = lg ? bl : bs;                                // gt child slot
-    const uint32_t sls = ll ? bl + (uint32_t)lg : bs + (uint32_t)!lg;  // lt child slot
-    const uint32_t c0g = bc, c0l = bc + chg;
-    if (lane == 0) {
-      out.write_node(nd.idx, nb, nd.start - nd.obase, nd.P - nd.nbase, nd.count);
-      out.write_split(nd.idx, d.axis | (d.kind << 8), d.value, d.nl, fbits(d.cost));
-      if (d.kind == 2) atomicAdd(&ctr->n_median, 1u);
-      put_child(lg, sgs, c0g, nd.start + d.nl, d.ng, d.gt, nd.P, nd.P + 2, cd, parity_next,
-                nd.nbase, nd.obase, Ln, small, learn_min, lvn, lrows_next, nd.nchunk == 1);
-      put_child(ll, sls, c0l, nd.start, d.nl, d.lt, nd.P + 1, nd.P + 2 + 2 * (d.ng - 1), cd,
-                parity_next, nd.nbase, nd.obase, Ln, small, learn_min, lvn, lrows_next,
-                nd.nchunk == 1);
-      L[slot].axis = d.axis;
-      L[slot].cut = d.value;
-      L[slot].n_lt = d.nl;
-      L[slot].kind = d.kind;
-      L[slot].child_slot[0] = lg ? (int32_t)sgs : -1;
-      L[slot].child_slot[1] = ll ? (int32_t)sls : -1;
-    }
-    for (uint32_t k = lane; k < chg; k += 64) { cn_next[c0g + k] = sgs; ck_next[c0g + k] = k; }
-    for (uint32_t k = lane; k < chl; k += 64) { cn_next[c0l + k] = sls; ck_next[c0l + k] = k; }
-  }
+  if (!active) return;
+  const SweepSlots r = sweep_slots(nd, d, s_base[w][0], s_base[w][1], s_base[w][2], max_depth);
+  if (lane == 0)
+    sweep_emit(L, slot, nd, d, r, Ln, small, ctr, out, parity_next, learn_min, lrows_next, lvn);
+  for (uint32_t k = lane; k < r.chg; k += 64) { cn_next[r.c0g + k] = r.sgs; ck_next[r.c0g + k] = k; }
+  for (uint32_t k = lane; k < r.chl; k += 64) { cn_next[r.c0l + k] = r.sls; ck_next[r.c0l + k] = k; }
 }
 
-// Persistent over the level's node queue, SW_WAVES * npw nodes per block and step.
-// lvc: this level's counts; lvn: the next level's, filled by the allocation.
+// Crowded levels (the lower levels: hundreds of thousands of small nodes,
+// nearly all decided by their parents' partition blocks): one node per lane,
+// SW_WAVES * 64 nodes per block.  A decided node needs no cross-lane work, so
+// each lane loads its own node and decision and writes its own records, and
+// the loads of 64 nodes are in flight together instead of one node's after
+// another's.  The block's undecided nodes are listed in LDS and split by its
+// wavefronts in turn (whole wavefront per node); each decision goes to the
+// start of the node's histogram row (not read again) for its lane.  Child
+// slots: exclusive DPP prefix sums inside each wavefront, wavefront totals
+// through LDS, one atomic per list per block.
+__device__ inline void sweep_lanes(uint32_t group, LNode* __restrict__ L, uint32_t nL,
+                                   uint32_t* __restrict__ hist, const Rec* __restrict__ recs,
+                                   LNode* Ln, uint32_t* cn_next, uint32_t* ck_next, SNode* small,
+                                   Counters* ctr, NodeOut out, uint32_t max_depth,
+                                   uint32_t parity_next, uint32_t learn_min,
+                                   uint32_t* lrows_next, uint32_t* lvn) {
+  __shared__ uint32_t s_wtot[SW_WAVES][SW_NEEDS];
+  __shared__ uint32_t s_gbase[3];
+  __shared__ uint32_t s_und[SW_WAVES * 64];
+  __shared__ uint32_t s_nund;
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t lane = lane_id();
+  const uint32_t slot = group * SW_WAVES * 64 + threadIdx.x;
+  const bool active = slot < nL;
+  if (threadIdx.x == 0) s_nund = 0;
+  LNode nd = {};
+  bool decided = false;
+  if (active) {
+    nd = L[slot];
+    decided = (nd.flags & LF_DECIDED) != 0;
+  }
+  __syncthreads();
+  const uint64_t um = __ballot(active && !decided);
+  if (um) {  // wave-uniform
+    uint32_t at = 0;
+    if (lane == 0) at = atomicAdd(&s_nund, (uint32_t)__popcll(um));
+    at = (uint32_t)__builtin_amdgcn_readfirstlane((int)at);
+    if (active && !decided) s_und[at + lanemask_lt_popc(um)] = slot;
+  }
+  __syncthreads();
+  const uint32_t nund = s_nund;
+  for (uint32_t i = w; i < nund; i += SW_WAVES) {
+    const uint32_t s2 = s_und[i];
+    uint32_t* h2 = hist + (size_t)s2 * HIST_WORDS;
+    const SplitDec d2 = sweep_split(L[s2], h2, recs, out, ctr);
+    if (lane == 0) *reinterpret_cast<SplitDec*>(h2) = d2;
+  }
+  if (nund) __syncthreads();  // block-uniform: the decisions are visible to their lanes
+  SplitDec d = {};
+  if (active) d = *reinterpret_cast<const SplitDec*>(hist + (size_t)slot * HIST_WORDS);
+  uint32_t need[SW_NEEDS] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  if (active) sweep_need(nd, d, max_depth, need);
+  uint32_t pre[3];
+#pragma unroll
+  for (int k = 0; k < SW_NEEDS; ++k) {
+    const uint32_t inc = wave_scan_u32_dpp(need[k]);
+    if (k < 3) pre[k] = inc - need[k];
+    const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+    if (lane == 0) s_wtot[w][k] = tot;
+  }
+  __syncthreads();
+  if (threadIdx.x < SW_NEEDS) {
+    const uint32_t k = threadIdx.x;
+    uint32_t tot = 0;
+    for (uint32_t q = 0; q < SW_WAVES; ++q) tot += s_wtot[q][k];
+    sweep_alloc(k, tot, s_gbase, ctr, lvn);
+  }
+  __syncthreads();
+  if (!active) return;
+  uint32_t base[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    uint32_t b = s_gbase[k] + pre[k];
+    for (uint32_t q = 0; q < w; ++q) b += s_wtot[q][k];
+    base[k] = b;
+  }
+  const SweepSlots r = sweep_slots(nd, d, base[0], base[1], base[2], max_depth);
+  sweep_emit(L, slot, nd, d, r, Ln, small, ctr, out, parity_next, learn_min, lrows_next, lvn);
+  for (uint32_t k = 0; k < r.chg; ++k) { cn_next[r.c0g + k] = r.sgs; ck_next[r.c0g + k] = k; }
+  for (uint32_t k = 0; k < r.chl; ++k) { cn_next[r.c0l + k] = r.sls; ck_next[r.c0l + k] = k; }
+}
+
+// Persistent over the level's node queue.  lvc: this level's counts; lvn: the
+// next level's, filled by the allocation.  Levels with at least lanes_min
+// nodes take one node per lane, the others one node per wavefront (their
+// nodes are large and mostly swept from histograms: parallelism first).
 __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L,
                                                 const uint32_t* __restrict__ lvc,
-                                                const uint32_t* __restrict__ hist,
+                                                uint32_t* __restrict__ hist,
                                                 const Rec* __restrict__ recs, LNode* Ln,
                                                 uint32_t* cn_next, uint32_t* ck_next,
                                                 SNode* small, Counters* ctr, NodeOut out,
                                                 uint32_t max_depth, uint32_t parity_next,
                                                 uint32_t learn_min, uint32_t* lrows_next,
-                                                uint32_t* lvn, uint32_t pack_min) {
-  // levels with fewer nodes than pack_min give each wavefront one node (their
-  // nodes are large and mostly swept from histograms: parallelism first)
+                                                uint32_t* lvn, uint32_t lanes_min) {
   const uint32_t nL = lvc[LV_NL];
-  const uint32_t npw = nL >= pack_min ? SW_NPW : 1u;
-  const uint32_t ngroups = (nL + SW_WAVES * npw - 1) / (SW_WAVES * npw);
-  for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
-    sweep_group(g, npw, L, nL, hist, recs, Ln, cn_next, ck_next, small, ctr, out, max_depth,
-                parity_next, learn_min, lrows_next, lvn);
-    __syncthreads();
+  if (nL >= lanes_min) {
+    const uint32_t ngroups = (nL + SW_WAVES * 64 - 1) / (SW_WAVES * 64);
+    for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+      sweep_lanes(g, L, nL, hist, recs, Ln, cn_next, ck_next, small, ctr, out, max_depth,
+                  parity_next, learn_min, lrows_next, lvn);
+      __syncthreads();
+    }
+  } else {
+    const uint32_t ngroups = (nL + SW_WAVES - 1) / SW_WAVES;
+    for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+      sweep_group(g, L, nL, hist, recs, Ln, cn_next, ck_next, small, ctr, out, max_depth,
+                  parity_next, learn_min, lrows_next, lvn);
+      __syncthreads();
+    }
   }
 }
 

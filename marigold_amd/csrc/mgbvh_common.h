@@ -249,15 +249,22 @@ __device__ inline uint32_t wave_max_u32_dpp(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
-// Sum over the whole wavefront (every lane active), as wave_max_u32_dpp.
-__device__ inline uint32_t wave_sum_u32_dpp(uint32_t v) {
+// Inclusive prefix sum over the whole wavefront (every lane active): row
+// prefixes with row_shr, then row_bcast:15 adds rows 0 / 2 into rows 1 / 3 and
+// row_bcast:31 adds rows 0-1 into rows 2-3.
+__device__ inline uint32_t wave_scan_u32_dpp(uint32_t v) {
   v += dpp_mov<0x111, 0xF>(0u, v);
   v += dpp_mov<0x112, 0xF>(0u, v);
   v += dpp_mov<0x114, 0xF>(0u, v);
   v += dpp_mov<0x118, 0xF>(0u, v);
   v += dpp_mov<0x142, 0xA>(0u, v);
   v += dpp_mov<0x143, 0xC>(0u, v);
-  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+  return v;
+}
+
+// Sum over the whole wavefront (every lane active).
+__device__ inline uint32_t wave_sum_u32_dpp(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_u32_dpp(v), 63);
 }
 
 __device__ inline uint32_t lanemask_lt_popc(uint64_t mask) {

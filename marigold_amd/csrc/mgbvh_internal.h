@@ -233,7 +233,7 @@ void scratch_free(Scratch& sc);
 struct Tuning {
   uint32_t bin_bpc = 32, part_bpc = 32, sweep_bpc = 4, small_bpc = 16;
   uint32_t part_waves = 1, bin_waves = 4;  // wavefronts per block
-  uint32_t sweep_pack = 65536;             // sweep: 4 nodes per wavefront from this many nodes
+  uint32_t sweep_pack = 65536;             // sweep: one node per lane from this many nodes
 };
 }  // namespace mg
 
