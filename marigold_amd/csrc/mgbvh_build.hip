@@ -599,6 +599,7 @@ __device__ inline SplitDec split_from_hist(const uint32_t* h, const Box& nb,
     }
     d.nl = (uint32_t)__builtin_amdgcn_readlane((int)(odd ? nl1 : nl0), (int)src);
     d.ng = (uint32_t)__builtin_amdgcn_readlane((int)(odd ? ng1 : ng0), (int)src);
+    d.cost = tc_ukey_inv((uint32_t)(best >> 32));  // the winner's cost, as its lane computed it
   } else {
     // Median fallback (lib.rs:444-491, 527-531).  With > 32 objects it is only
     // reached when every centroid coincides, so each axis' select keeps the
@@ -632,8 +633,8 @@ __device__ inline SplitDec split_from_hist(const uint32_t* h, const Box& nb,
     d.axis = 0;
     d.value = m;
     d.kind = 2;
+    d.cost = split_cost(sa_p, box_sa(d.lt), box_sa(d.gt), d.nl, d.ng);
   }
-  d.cost = split_cost(sa_p, box_sa(d.lt), box_sa(d.gt), d.nl, d.ng);
   d.pad[0] = d.pad[1] = 0;
   return d;
 }
