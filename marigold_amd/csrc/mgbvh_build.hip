@@ -1469,8 +1469,10 @@ __device__ inline void small_pair(uint32_t wave, SmallObjs* S, const SNode* __re
 }
 
 // Persistent over the small-subtree queue: each wavefront takes pairs of
-// subtrees (only wave-local synchronisation inside).
-__global__ __launch_bounds__(256) void k_small(const SNode* __restrict__ small,
+// subtrees (only wave-local synchronisation inside).  Five wavefronts per SIMD
+// (at most 96 VGPRs; the compiler keeps ~60 bytes per lane of pair set-up
+// state in scratch, outside the candidate loop) beat four at 115 VGPRs.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_small(const SNode* __restrict__ small,
                                                const uint32_t* __restrict__ nS_dev,
                                                const Rec* __restrict__ rec0,
                                                const Rec* __restrict__ rec1, NodeOut out,
