@@ -21,14 +21,14 @@ import numpy as np
 from . import _native as N
 
 GRID_CANDIDATES = [
-    dict(bin_blocks_per_cu=32, partition_blocks_per_cu=32, sweep_blocks_per_cu=4,
+    dict(bin_blocks_per_cu=128, partition_blocks_per_cu=192, sweep_blocks_per_cu=4,
+         small_blocks_per_cu=64, partition_waves=1, bin_waves=4),
+    dict(bin_blocks_per_cu=64, partition_blocks_per_cu=128, sweep_blocks_per_cu=4,
+         small_blocks_per_cu=32, partition_waves=1, bin_waves=4),
+    dict(bin_blocks_per_cu=128, partition_blocks_per_cu=256, sweep_blocks_per_cu=4,
+         small_blocks_per_cu=128, partition_waves=1, bin_waves=4),
+    dict(bin_blocks_per_cu=32, partition_blocks_per_cu=64, sweep_blocks_per_cu=4,
          small_blocks_per_cu=16, partition_waves=1, bin_waves=4),
-    dict(bin_blocks_per_cu=8, partition_blocks_per_cu=64, sweep_blocks_per_cu=4,
-         small_blocks_per_cu=16, partition_waves=1, bin_waves=4),
-    dict(bin_blocks_per_cu=16, partition_blocks_per_cu=16, sweep_blocks_per_cu=4,
-         small_blocks_per_cu=16, partition_waves=4, bin_waves=4),
-    dict(bin_blocks_per_cu=32, partition_blocks_per_cu=32, sweep_blocks_per_cu=2,
-         small_blocks_per_cu=32, partition_waves=1, bin_waves=2),
 ]
 LEARNED_CANDIDATES = [
     dict(learned_min_objects=1024, learned_window=3, learned_guard=0.02),

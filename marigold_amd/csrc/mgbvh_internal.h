@@ -229,9 +229,13 @@ struct Scratch {
 };
 void scratch_free(Scratch& sc);
 
-// Persistent grid sizes of the level kernels, in blocks per compute unit.
+// Grid sizes of the level kernels, in blocks per compute unit.  The kernels
+// loop over their work lists, but grids well past the resident block count
+// measured faster for k_bin, k_partition and k_small: blocks that finish
+// early free their slots for others, which balances uneven chunks and
+// subtrees better than a fixed share per resident block.
 struct Tuning {
-  uint32_t bin_bpc = 32, part_bpc = 32, sweep_bpc = 4, small_bpc = 16;
+  uint32_t bin_bpc = 128, part_bpc = 192, sweep_bpc = 4, small_bpc = 64;
   uint32_t part_waves = 1, bin_waves = 4;  // wavefronts per block
   uint32_t sweep_pack = 65536;             // sweep: one node per lane from this many nodes
 };

@@ -438,12 +438,12 @@ int mgbvh_scene_plan_destroy(mgbvh_scene_plan* pl) {
 
 int mgbvh_ctx_set_tuning(mgbvh_ctx* c, const mgbvh_tuning* t) {
   if (!c || !t) { set_error("null argument"); return MGBVH_ERR_INVALID_ARG; }
-  auto ok = [](uint32_t v) { return v >= 1 && v <= 64; };
+  auto ok = [](uint32_t v) { return v >= 1 && v <= 256; };
   auto waves = [](uint32_t v) { return v == 0 || v == 1 || v == 4; };
   if (!ok(t->bin_blocks_per_cu) || !ok(t->partition_blocks_per_cu) ||
       !ok(t->sweep_blocks_per_cu) || !ok(t->small_blocks_per_cu) || !waves(t->partition_waves) ||
       !(t->bin_waves <= 4)) {
-    set_error("blocks per compute unit must be in [1, 64]; partition_waves 1 or 4, bin_waves <= 4");
+    set_error("blocks per compute unit must be in [1, 256]; partition_waves 1 or 4, bin_waves <= 4");
     return MGBVH_ERR_INVALID_ARG;
   }
   c->tune.bin_bpc = t->bin_blocks_per_cu;

@@ -244,8 +244,8 @@ def test_scene_graph_replays_equal_level_by_level(ctx):
                                          sweep_blocks_per_cu=1, small_blocks_per_cu=1,
                                          partition_waves=4, bin_waves=1,
                                          sweep_pack_min_nodes=1),
-                                    dict(bin_blocks_per_cu=32, partition_blocks_per_cu=16,
-                                         sweep_blocks_per_cu=4, small_blocks_per_cu=64,
+                                    dict(bin_blocks_per_cu=32, partition_blocks_per_cu=256,
+                                         sweep_blocks_per_cu=4, small_blocks_per_cu=256,
                                          partition_waves=1, bin_waves=2,
                                          sweep_pack_min_nodes=1 << 30)])
 def test_tuning_does_not_change_results(ctx, tuning):
