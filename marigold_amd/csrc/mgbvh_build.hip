@@ -956,7 +956,9 @@ __device__ inline void partition_chunk(uint32_t c, const LNode* __restrict__ L,
   const uint32_t hi = min(lo + CH, nd.start + nd.count);
 
   uint32_t lt_before;
-  if (kind == 0) {
+  if (kind == 0 && k == 0) {  // block-uniform: the node's first chunk
+    lt_before = 0;
+  } else if (kind == 0) {
     uint32_t part = 0;
     const uint32_t tot = k * cut;
     for (uint32_t t = threadIdx.x; t < tot; t += blockDim.x) {
