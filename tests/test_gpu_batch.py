@@ -260,3 +260,20 @@ def test_tuning_does_not_change_results(ctx, tuning):
         ctx.set_tuning(**{k: v for k, v in old.items() if k != "compute_units"})
     for k in ("nodes", "lens", "splits", "perm", "tris"):
         assert np.array_equal(np.asarray(res[k]).view(np.uint8), np.asarray(base[k]).view(np.uint8)), k
+
+
+def test_tuning_limits(ctx):
+    """Grid sizes 1..256 blocks per CU, 1 or 4 partition wavefronts; anything
+    else is refused and leaves the tuning as it was."""
+    print("[case] tuning limits", flush=True)
+    old = ctx.get_tuning()
+    try:
+        ctx.set_tuning(bin_blocks_per_cu=256, small_blocks_per_cu=1)
+        assert ctx.get_tuning()["bin_blocks_per_cu"] == 256
+        for bad in (dict(bin_blocks_per_cu=257), dict(partition_blocks_per_cu=0),
+                    dict(partition_waves=2), dict(bin_waves=5)):
+            with pytest.raises(N.MgbvhError):
+                ctx.set_tuning(**bad)
+        assert ctx.get_tuning()["bin_blocks_per_cu"] == 256
+    finally:
+        ctx.set_tuning(**{k: v for k, v in old.items() if k != "compute_units"})

@@ -61,6 +61,35 @@ def test_samples_root_features_and_label(ctx, samples):
     assert np.all((samples["f"] >= 0) & (samples["f"] <= 1))
 
 
+def test_samples_same_with_one_node_per_lane(ctx, samples):
+    """The sweep's one-node-per-lane mode (crowded levels) splits undecided
+    nodes by whole wavefronts too and writes the same samples."""
+    print("[case] split samples with the sweep one node per lane on every level", flush=True)
+    pos, tris = samples["pos"], samples["tris"]
+    n = tris.shape[0]
+    old = ctx.get_tuning()
+    d_pos, d_tris = ctx.to_device(pos), ctx.to_device(tris)
+    cap = 8192
+    d_f, d_l = ctx.alloc(cap * 102 * 4), ctx.alloc(cap * 4)
+    try:
+        ctx.set_tuning(sweep_pack_min_nodes=1)
+        cnt = ctx.collect_samples(d_pos, 12, pos.shape[0], d_tris, n, N.default_settings(), 33,
+                                  d_f, d_l, cap)
+        f = d_f.download(np.float32, cnt * 102).reshape(cnt, 102)
+        lab = d_l.download(np.uint32, cnt)
+    finally:
+        ctx.set_tuning(**{k: v for k, v in old.items() if k != "compute_units"})
+        for b in (d_pos, d_tris, d_f, d_l):
+            b.free()
+    assert cnt == samples["count"]
+
+    def rows(f, lab):  # samples in a canonical order (their slots follow atomics)
+        r = np.concatenate([lab[:, None], f.view(np.uint32)], axis=1)
+        return r[np.lexsort(r.T[::-1])]
+
+    assert np.array_equal(rows(f, lab), rows(samples["f"], samples["l"]))
+
+
 def test_gradients_match_oracle(ctx, samples):
     print("[case] training gradients vs oracle", flush=True)
     m = N.Model(ctx, seed=7)
