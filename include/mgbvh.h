@@ -411,9 +411,9 @@ int mgbvh_scene_plan_destroy(mgbvh_scene_plan* plan);
 int mgbvh_scene_plan_info(const mgbvh_scene_plan* plan, uint32_t* blas_levels,
                           uint32_t* tlas_levels, uint32_t* launches_rebuilt);
 
-/* Persistent grid sizes of the level kernels (blocks per compute unit, 1..256),
- * the knobs the autotuner (marigold_amd/autotune.py) sets; results never
- * depend on them. */
+/* Grid sizes of the level kernels (blocks per compute unit, 1..256; the
+ * kernels loop over their work lists, so any size is correct), the knobs the
+ * autotuner (marigold_amd/autotune.py) sets; results never depend on them. */
 typedef struct mgbvh_tuning {
   uint32_t bin_blocks_per_cu;
   uint32_t partition_blocks_per_cu;

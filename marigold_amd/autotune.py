@@ -1,7 +1,7 @@
 """Autotuner of the batched scene build.
 
 Picks, by timing candidates on the scene itself, the settings that do not
-change the plain result (persistent grid sizes of the level kernels, in blocks
+change the plain result (grid sizes of the level kernels, in blocks
 per compute unit) and the per-node-size dispatch of the learned method (the
 smallest node that takes the learned path, the window half-width), keeping
 only learned candidates whose scene SAH stays within `sah_tolerance` of the
@@ -79,7 +79,7 @@ def autotune(ctx: N.Context, meshes, transforms, instance_mesh, reps: int = 3,
             probe.free()
             return dict(cached, cached=True)
     trials = []
-    # 1. persistent grid sizes, plain split rule (the result does not depend on them)
+    # 1. grid sizes, plain split rule (the result does not depend on them)
     best_grid, best_ms = None, float("inf")
     for cand in GRID_CANDIDATES:
         ctx.set_tuning(**cand)

@@ -2,17 +2,20 @@
 // reproduces marigold's recursive builder (bvh/src/lib.rs:536-665) bit for bit.
 //
 // Pipeline for one object list of n boxes:
-//   k_setup_*     object AABBs (mesh_interface/src/lib.rs:19-30) into 32-byte
+//   k_setup_*     object AABBs (mesh_interface/src/lib.rs:19-30) into 28-byte
 //                 records, root bounds and root centroid bounds (lib.rs:220-228, 499-509)
 //   level loop    for every node with > 32 objects (binned sweep, lib.rs:331-389):
 //     k_bin       per chunk of 2048 objects: LDS-staged 3-axis x 32-bin histograms
 //                 (count + bin box), merged into a per-node histogram
-//     k_sweep     one wavefront per node: 3 x 31 prefix-SAH candidates, first
-//                 minimum under f32::total_cmp (lib.rs:373-388, 515-524), median
-//                 fallback (lib.rs:527-531), node record, child allocation
+//     k_sweep     3 x 31 prefix-SAH candidates per node, first minimum under
+//                 f32::total_cmp (lib.rs:373-388, 515-524), median fallback
+//                 (lib.rs:527-531), node record, child allocation (one wavefront
+//                 per node, or one lane per node on crowded levels)
 //     k_partition stable coalesced partition by bin (select_nth_unstable at
-//                 lib.rs:601-607 restated as a stable split), children's centroid bounds
-//   k_small       one wavefront per subtree of <= 32 objects: adaptive sweep
+//                 lib.rs:601-607 restated as a stable split), children's centroid
+//                 bounds; a node that fits one block also bins its children and
+//                 takes their splits
+//   k_small       one wavefront per two subtrees of <= 32 objects: adaptive sweep
 //                 (lib.rs:391-442) down to the leaves, entirely in LDS
 //   k_compact     only when max_depth cut a subtree short: squeeze the slots
 //                 reserved for the missing nodes
@@ -1760,7 +1763,7 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
   MG_HIP(hipGetLastError());
 
   // level-by-level: exact grids from the counts read back after each level;
-  // fixed levels (graph capture): persistent grids sized for the capacities
+  // fixed levels (graph capture): grids sized for the capacities
   uint32_t nL = p.n_large, nC = (uint32_t)p.lchunks, n_counted = p.n_counted, nBin = nC;
   int cur = 0;
   uint32_t level = 0;
