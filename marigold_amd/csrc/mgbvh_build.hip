@@ -114,23 +114,26 @@ __device__ inline uint32_t wave_max6(uint32_t v) {
 // Adds each valid lane's object to bin `bi` of axis `a` of an LDS histogram
 // row (count and/or the box words e: ~okey(min) x3, okey(max) x3).  LDS
 // serialises atomics on one address, and spatially coherent chunks put most
-// of a wavefront into one bin (a heightfield row: all 64 lanes), so up to
-// three bins shared by >= 8 lanes are merged first: the wave maximum of their
-// box words and their popcount go in with one set of atomics.  Counts and
-// max folds are order independent: the row is the same as with per-lane
-// atomics.  Call with every lane of the wavefront active.
+// of a wavefront into one bin (a heightfield row: all 64 lanes), so a bin
+// shared by at least half the wavefront (at most two such bins) is merged
+// first: the wave maximum of its box words and its popcount go in with one set
+// of atomics.  Smaller groups go per lane: with four histogram copies their
+// address conflicts cost less than a merge's DPP reductions (measured: a
+// threshold of 8 lanes was 0.3 ms per scene build slower).  Counts and max
+// folds are order independent: the row is the same as with per-lane atomics.
+// Call with every lane of the wavefront active.
 __device__ inline void hist_add(uint32_t* h, uint32_t a, uint32_t bi, bool valid,
                                 const uint32_t (&e)[6], bool do_count, bool do_boxes) {
   const uint32_t lane = lane_id();
   uint64_t rem = __ballot(valid);
 #pragma unroll 1
-  for (int round = 0; round < 3 && rem; ++round) {
+  for (int round = 0; round < 2 && rem; ++round) {
     const uint32_t lead = (uint32_t)__ffsll((unsigned long long)rem) - 1;
     const uint32_t B = (uint32_t)__builtin_amdgcn_readlane((int)bi, (int)lead);
     const bool in = ((rem >> lane) & 1ull) && bi == B;
     const uint64_t m = __ballot(in);
     const uint32_t k = (uint32_t)__popcll(m);
-    if (k < 8) break;
+    if (k < 32) break;
     uint32_t* hb = &h[(a * MG_NBINS + B) * BIN_WORDS];
     if (do_boxes) {
       uint32_t r[6];
