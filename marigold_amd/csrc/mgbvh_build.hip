@@ -442,15 +442,6 @@ __global__ __launch_bounds__(256) void k_bin(const LNode* __restrict__ L,
 }
 
 // ------------------------------------------------------------------ sweep
-__device__ inline Box hist_box(const uint32_t* hb) {
-  Box b;
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    b.mn[j] = acc_min_dec(hb[1 + j]);
-    b.mx[j] = acc_max_dec(hb[4 + j]);
-  }
-  return b;
-}
 
 // Child record of a split: a node with > 32 objects for the next level (slot s,
 // chunks [c0, c0 + nch)), or a subtree root of <= 32 objects / a depth-cut leaf.

@@ -139,28 +139,6 @@ __device__ inline float adaptive_threshold(float cmin, float cmax, uint32_t step
   return __fadd_rn(cmin, __fmul_rn(step, fi));
 }
 
-__device__ inline uint64_t wave_min_u64(uint64_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    uint64_t o = __shfl_xor(v, off, 64);
-    v = o < v ? o : v;
-  }
-  return v;
-}
-__device__ inline uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    uint32_t o = __shfl_xor(v, off, 64);
-    v = o > v ? o : v;
-  }
-  return v;
-}
-__device__ inline uint32_t wave_sum_u32(uint32_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
 // ---- DPP cross-lane primitives (CDNA data-parallel primitives: no LDS round trip)
@@ -185,24 +163,6 @@ __device__ inline uint32_t seg_scan32(uint32_t v, uint32_t id, uint32_t lane, ui
   t = dpp_mov<0x142, 0xA>(id, v); if (ls < rs) v = op(v, t);
   return v;
 }
-
-// Suffix (inclusive toward higher lanes) scan over each whole 32-lane half:
-// row_shl 1, 2, 4, 8 inside each 16-lane row, then rows 0 and 2 fold in the
-// suffix of row 1 (3), read from lane 16 (48).
-template <class Op>
-__device__ inline uint32_t half_suffix32(uint32_t v, uint32_t id, uint32_t lane, Op op) {
-  v = op(v, dpp_mov<0x101, 0xF>(id, v));
-  v = op(v, dpp_mov<0x102, 0xF>(id, v));
-  v = op(v, dpp_mov<0x104, 0xF>(id, v));
-  v = op(v, dpp_mov<0x108, 0xF>(id, v));
-  const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
-  const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
-  if ((lane & 16u) == 0) v = op(v, lane < 32 ? r1 : r3);
-  return v;
-}
-
-// Value of lane - 1 (DPP wave_shr:1); lane 0 gets `id`.
-__device__ inline uint32_t from_prev_lane(uint32_t v, uint32_t id) { return dpp_mov<0x138, 0xF>(id, v); }
 
 // Inclusive segmented minimum scan of u64 keys inside 32-lane halves.
 __device__ inline uint64_t seg_scan_min_u64(uint64_t v, uint32_t lane, uint32_t ls) {
