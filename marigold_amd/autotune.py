@@ -35,6 +35,7 @@ LEARNED_CANDIDATES = [
     dict(learned_min_objects=2048, learned_window=2, learned_guard=0.02),
     dict(learned_min_objects=8192, learned_window=2, learned_guard=0.02),
     dict(learned_min_objects=32768, learned_window=2, learned_guard=0.02),
+    dict(learned_min_objects=131072, learned_window=2, learned_guard=0.02),
 ]
 
 
