@@ -41,7 +41,7 @@ int ensure(DevBuf& b, size_t bytes) {
   return MGBVH_OK;
 }
 
-int prof_begin(mgbvh_ctx* c, int kind) {
+int prof_begin(mgbvh_ctx* c, int kind, hipStream_t st) {
   const size_t k = c->prof_kind.size();
   while (c->prof_pool.size() < 2 * (k + 1)) {
     hipEvent_t e;
@@ -50,14 +50,14 @@ int prof_begin(mgbvh_ctx* c, int kind) {
     c->prof_pool.push_back(e);
   }
   c->prof_kind.push_back(kind);
-  hipError_t r = hipEventRecord(c->prof_pool[2 * k], c->stream);
+  hipError_t r = hipEventRecord(c->prof_pool[2 * k], st ? st : c->stream);
   return r == hipSuccess ? MGBVH_OK : hip_fail(r, "hipEventRecord");
 }
 
-int prof_end(mgbvh_ctx* c, int kind) {
+int prof_end(mgbvh_ctx* c, int kind, hipStream_t st) {
   (void)kind;
   const size_t k = c->prof_kind.size() - 1;
-  hipError_t r = hipEventRecord(c->prof_pool[2 * k + 1], c->stream);
+  hipError_t r = hipEventRecord(c->prof_pool[2 * k + 1], st ? st : c->stream);
   return r == hipSuccess ? MGBVH_OK : hip_fail(r, "hipEventRecord");
 }
 
@@ -275,12 +275,19 @@ int mgbvh_ctx_create(int device, mgbvh_ctx** out) {
   c->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate"); }
+  // the second stream has the default priority too (mixed priorities are avoided)
+  if ((e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming)) != hipSuccess) {
+    (void)mgbvh_ctx_destroy(c);
+    return hip_fail(e, "hipStreamCreate/hipEventCreate");
+  }
   for (int i = 0; i < 64; ++i) {
     e = hipEventCreate(&c->ev[i]);
-    if (e != hipSuccess) { delete c; return hip_fail(e, "hipEventCreate"); }
+    if (e != hipSuccess) { (void)mgbvh_ctx_destroy(c); return hip_fail(e, "hipEventCreate"); }
   }
   e = hipHostMalloc((void**)&c->h_ctr, sizeof(Counters), hipHostMallocDefault);
-  if (e != hipSuccess) { delete c; return hip_fail(e, "hipHostMalloc"); }
+  if (e != hipSuccess) { (void)mgbvh_ctx_destroy(c); return hip_fail(e, "hipHostMalloc"); }
   c->sc.h_ctr = c->h_ctr;
   *out = c;
   return MGBVH_OK;
@@ -289,7 +296,8 @@ int mgbvh_ctx_create(int device, mgbvh_ctx** out) {
 int mgbvh_ctx_destroy(mgbvh_ctx* c) {
   if (!c) return MGBVH_OK;
   (void)hipSetDevice(c->device);
-  (void)hipStreamSynchronize(c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   scratch_free(c->sc);
   DevBuf* bufs[] = {&c->io_pos, &c->io_tris, &c->io_nodes, &c->io_lens, &c->io_perm,
                     &c->io_tris_out, &c->io_splits, &c->io_boxes};
@@ -298,6 +306,9 @@ int mgbvh_ctx_destroy(mgbvh_ctx* c) {
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
   if (c->h_ctr) (void)hipHostFree(c->h_ctr);
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+  if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return MGBVH_OK;

@@ -1546,12 +1546,13 @@ namespace mg {
     if (e_ != hipSuccess) return hip_fail(e_, #call);  \
   } while (0)
 
-#define MG_PROF(kind, end)                                                   \
-  do {                                                                       \
-    if (c->prof_mask & (1u << (kind))) {                                     \
-      int r_ = (end) ? prof_end(c, (kind)) : prof_begin(c, (kind));         \
-      if (r_) return r_;                                                     \
-    }                                                                        \
+// kernel timing events on the enclosing function's stream `st`
+#define MG_PROF(kind, end)                                                       \
+  do {                                                                           \
+    if (c->prof_mask & (1u << (kind))) {                                         \
+      int r_ = (end) ? prof_end(c, (kind), st) : prof_begin(c, (kind), st);    \
+      if (r_) return r_;                                                         \
+    }                                                                            \
   } while (0)
 
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -1702,8 +1703,9 @@ int plan_build(mgbvh_ctx* c, Scratch& sc, const BuildInput& in, const std::vecto
 }
 
 int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels,
-                  bool upload_rows, uint32_t* levels_run) {
-  hipStream_t st = c->stream;
+                  bool upload_rows, uint32_t* levels_run, hipStream_t stream,
+                  const std::function<int()>* after_setup) {
+  hipStream_t st = stream ? stream : c->stream;
   const BuildInput& in = p.in;
   const BuildOutput& o = p.o;
   const mgbvh_settings* s = &p.s;
@@ -1755,6 +1757,8 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
                             (uint32_t)p.lchunks, p.n_counted);
   MG_PROF(KK_ROOT, 1);
   MG_HIP(hipGetLastError());
+  // the object boxes and the per-mesh root bounds are in place
+  if (after_setup && (rc = (*after_setup)())) return rc;
 
   // level-by-level: exact grids from the counts read back after each level;
   // fixed levels (graph capture): grids sized for the capacities

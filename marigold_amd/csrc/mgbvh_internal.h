@@ -166,6 +166,7 @@ __device__ inline void NodeOut::write_node(uint32_t idx, const Box& b, uint32_t 
 
 // ---------------------------------------------------------------- host side
 #include <hip/hip_runtime.h>
+#include <functional>
 
 struct DevBuf {
   void* p = nullptr;
@@ -250,6 +251,8 @@ struct mgbvh_ctx {
   double prof_ms[KK_COUNT] = {};
   uint64_t prof_n[KK_COUNT] = {};
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // a scene plan's TLAS branch, recorded beside the BLAS levels
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   hipEvent_t ev[64] = {};
   mg::Scratch sc;                 // scratch of this context's builds
   DevBuf io_pos, io_tris, io_nodes, io_lens, io_perm, io_tris_out, io_splits, io_boxes;
@@ -291,7 +294,9 @@ int plan_build(mgbvh_ctx* c, Scratch& sc, const BuildInput& in, const std::vecto
                const mgbvh_settings* s, const BuildOutput& o, BuildPlan* p);
 // fixed_levels = 0: level by level with read-backs; else exactly that many levels.
 int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels,
-                  bool upload_rows, uint32_t* levels_run);
+                  bool upload_rows, uint32_t* levels_run,
+                  hipStream_t stream = nullptr,  // nullptr: c->stream
+                  const std::function<int()>* after_setup = nullptr);
 int finish_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, BuildResult* res);
 __global__ void k_fill(uint32_t* __restrict__ p, uint32_t v, uint64_t n);
 __global__ void k_instance_boxes(const mgbvh_node32* __restrict__ nodes,
@@ -300,8 +305,8 @@ __global__ void k_instance_boxes(const mgbvh_node32* __restrict__ nodes,
                                  const uint32_t* __restrict__ inst_blas, uint32_t ninst,
                                  mgbvh_aabb* __restrict__ out, uint32_t* err);
 // Event pair around one kernel launch (no-op unless kind is in ctx->prof_mask).
-int prof_begin(mgbvh_ctx* c, int kind);
-int prof_end(mgbvh_ctx* c, int kind);
+int prof_begin(mgbvh_ctx* c, int kind, hipStream_t st = nullptr);  // nullptr: c->stream
+int prof_end(mgbvh_ctx* c, int kind, hipStream_t st = nullptr);
 int prof_collect(mgbvh_ctx* c);  // after a synchronize: accumulate and recycle
 // Learned method, one level: features of the counted nodes, network
 // inference, window and guard decision (LNode::mode -> window or fallback).

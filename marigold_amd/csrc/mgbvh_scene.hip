@@ -14,6 +14,37 @@ namespace mg {
 // summed left to right, w = 1), then the component-wise min and max.  The mesh
 // box is the BLAS root's bounds (= mesh.bounds when the BLAS was built with
 // them).  object_to_world: 3 rows x 4 floats per instance.
+__device__ inline mgbvh_aabb instance_box(const float (&bmin)[3], const float (&bmax)[3],
+                                          const float* __restrict__ m) {
+  float lo[3], hi[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = __uint_as_float(0x7f800000u);
+    hi[a] = __uint_as_float(0xff800000u);
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const float x = (k & 4) ? bmax[0] : bmin[0];
+    const float y = (k & 2) ? bmax[1] : bmin[1];
+    const float z = (k & 1) ? bmax[2] : bmin[2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float* row = m + 4 * a;
+      float w = __fmul_rn(row[0], x);
+      w = __fadd_rn(w, __fmul_rn(row[1], y));
+      w = __fadd_rn(w, __fmul_rn(row[2], z));
+      w = __fadd_rn(w, row[3]);
+      lo[a] = fmin_x86(lo[a], w);
+      hi[a] = fmax_x86(hi[a], w);
+    }
+  }
+  mgbvh_aabb r;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) { r.min[a] = lo[a]; r.max[a] = hi[a]; }
+  return r;
+}
+
+// The mesh box from each BLAS's root node.
 __global__ __launch_bounds__(256) void k_instance_boxes(const mgbvh_node32* __restrict__ nodes,
                                                         const mgbvh_blas_desc* __restrict__ blas,
                                                         uint32_t nblas,
@@ -29,33 +60,37 @@ __global__ __launch_bounds__(256) void k_instance_boxes(const mgbvh_node32* __re
     return;
   }
   const mgbvh_node32 root = nodes[blas[b].node_offset];
-  const float* m = o2w + 12 * (size_t)i;
-  float lo[3], hi[3];
+  float bmin[3], bmax[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) { bmin[a] = root.min[a]; bmax[a] = root.max[a]; }
+  out[i] = instance_box(bmin, bmax, o2w + 12 * (size_t)i);
+}
+
+// The mesh box from the batched build's per-mesh root bounds (the union of
+// the mesh's object boxes, the root node's box of a non-empty mesh built
+// without given bounds), available as soon as the objects are set up.
+__global__ __launch_bounds__(256) void k_instance_boxes_acc(const uint32_t* __restrict__ macc,
+                                                            uint32_t nblas,
+                                                            const float* __restrict__ o2w,
+                                                            const uint32_t* __restrict__ inst_blas,
+                                                            uint32_t ninst,
+                                                            mgbvh_aabb* __restrict__ out,
+                                                            uint32_t* err) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ninst) return;
+  const uint32_t b = inst_blas[i];
+  if (b >= nblas) {
+    atomicOr(err, 1u);
+    return;
+  }
+  const uint32_t* acc = macc + 12 * (size_t)b;
+  float bmin[3], bmax[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    lo[a] = __uint_as_float(0x7f800000u);
-    hi[a] = __uint_as_float(0xff800000u);
+    bmin[a] = acc_min_dec(acc[a]);
+    bmax[a] = acc_max_dec(acc[3 + a]);
   }
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const float x = (k & 4) ? root.max[0] : root.min[0];
-    const float y = (k & 2) ? root.max[1] : root.min[1];
-    const float z = (k & 1) ? root.max[2] : root.min[2];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const float* row = m + 4 * a;
-      float w = __fmul_rn(row[0], x);
-      w = __fadd_rn(w, __fmul_rn(row[1], y));
-      w = __fadd_rn(w, __fmul_rn(row[2], z));
-      w = __fadd_rn(w, row[3]);
-      lo[a] = fmin_x86(lo[a], w);
-      hi[a] = fmax_x86(hi[a], w);
-    }
-  }
-  mgbvh_aabb r;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) { r.min[a] = lo[a]; r.max[a] = hi[a]; }
-  out[i] = r;
+  out[i] = instance_box(bmin, bmax, o2w + 12 * (size_t)i);
 }
 
 // SAH cost terms of every node slot: mesh m owns slots [off[m], off[m] + cnt[m]).
@@ -353,9 +388,37 @@ int mgbvh_scene_plan_create(mgbvh_ctx* c, const float* d_pos, uint32_t vstride_b
   c->prof_mask = 0;
   e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) { c->prof_mask = prof; return fail(hip_fail(e, "hipStreamBeginCapture")); }
-  uint32_t lv_run = 0;
-  rc = enqueue_build(c, pl->bsc, pl->bp, pl->bp.fixed_levels, false, &lv_run);
-  if (!rc && ninstances) {
+  uint32_t lv_run = 0, lv_run_t = 0;
+  // The TLAS needs only each mesh's box, known once the BLAS objects are set
+  // up: with every mesh non-empty its branch (instance boxes from the per-mesh
+  // root bounds, then the TLAS levels) is recorded on the second stream beside
+  // the BLAS levels, whose stream waits for it at the end.  An empty mesh's
+  // instance box is read from the node array, so such a scene is recorded in
+  // sequence.
+  bool all_nonempty = true;
+  for (uint32_t i = 0; i < nmeshes; ++i) all_nonempty = all_nonempty && meshes[i].tri_count > 0;
+  const bool branch = ninstances && all_nonempty;
+  hipStream_t st2 = c->stream2;
+  const std::function<int()> tlas_branch = [&]() -> int {
+    hipError_t e2;
+    if ((e2 = hipEventRecord(c->fork_ev, st)) != hipSuccess ||
+        (e2 = hipStreamWaitEvent(st2, c->fork_ev, 0)) != hipSuccess)
+      return hip_fail(e2, "TLAS branch fork");
+    k_fill<<<1, 64, 0, st2>>>((uint32_t*)pl->flag.p, 0u, 1);
+    k_instance_boxes_acc<<<(ninstances + 255) / 256, 256, 0, st2>>>(
+        (const uint32_t*)pl->bsc.macc.p, nmeshes, d_object_to_world, d_instance_blas, ninstances,
+        d_boxes, (uint32_t*)pl->flag.p);
+    if ((e2 = hipGetLastError()) != hipSuccess) return hip_fail(e2, "k_instance_boxes_acc");
+    const int r2 = enqueue_build(c, pl->tsc, pl->tp, pl->tp.fixed_levels, false, &lv_run_t, st2);
+    if (r2) return r2;
+    if ((e2 = hipEventRecord(c->join_ev, st2)) != hipSuccess) return hip_fail(e2, "TLAS branch join");
+    return MGBVH_OK;
+  };
+  rc = enqueue_build(c, pl->bsc, pl->bp, pl->bp.fixed_levels, false, &lv_run, st,
+                     branch ? &tlas_branch : nullptr);
+  if (!rc && branch) {
+    if ((e = hipStreamWaitEvent(st, c->join_ev, 0)) != hipSuccess) rc = hip_fail(e, "TLAS branch join");
+  } else if (!rc && ninstances) {
     k_fill<<<1, 64, 0, st>>>((uint32_t*)pl->flag.p, 0u, 1);
     k_instance_boxes<<<(ninstances + 255) / 256, 256, 0, st>>>(
         d_blas_nodes, (const mgbvh_blas_desc*)pl->blas_desc.p, nmeshes, d_object_to_world,
@@ -363,7 +426,7 @@ int mgbvh_scene_plan_create(mgbvh_ctx* c, const float* d_pos, uint32_t vstride_b
     e = hipGetLastError();
     if (e != hipSuccess) rc = hip_fail(e, "k_instance_boxes");
     if (!rc)
-      rc = enqueue_build(c, pl->tsc, pl->tp, pl->tp.fixed_levels, false, &lv_run);
+      rc = enqueue_build(c, pl->tsc, pl->tp, pl->tp.fixed_levels, false, &lv_run_t);
   }
   hipGraph_t g = nullptr;
   e = hipStreamEndCapture(st, &g);
