@@ -117,6 +117,15 @@ def test_batch_depth_cut(ctx, max_depth):
     _check_against_single(ctx, meshes, res, desc, max_depth=max_depth, tag=f"cut{max_depth}")
     if max_depth < 6:
         assert res["stats"]["n_cutoff_leaves"] > 0
+    # the sweep's one-node-per-lane mode (crowded levels) cuts the same leaves
+    old = ctx.get_tuning()
+    try:
+        ctx.set_tuning(sweep_pack_min_nodes=1)
+        lanes, _ = _batch(ctx, meshes, max_depth=max_depth)
+    finally:
+        ctx.set_tuning(**{k: v for k, v in old.items() if k != "compute_units"})
+    for k in ("nodes", "lens", "splits", "perm", "tris"):
+        assert np.array_equal(np.asarray(lanes[k]).view(np.uint8), np.asarray(res[k]).view(np.uint8)), k
 
 
 def test_batch_mesh_bounds(ctx):
