@@ -1228,7 +1228,9 @@ __device__ inline void put_leaf(uint32_t pos, const SmallLane& o, uint32_t obase
   }
 }
 
-__device__ inline void small_pair(uint32_t wave, SmallObjs* S, const SNode* __restrict__ small,
+// Subtrees ea (lanes 0-31) and eb (lanes 32-63); an index >= nS is none.
+__device__ inline void small_pair(uint32_t ea, uint32_t eb, SmallObjs* S,
+                                  const SNode* __restrict__ small,
                                   uint32_t nS, const Rec* __restrict__ rec0,
                                   const Rec* __restrict__ rec1, NodeOut out,
                                   uint32_t* __restrict__ perm, const uint32_t* __restrict__ tris_in,
@@ -1236,7 +1238,7 @@ __device__ inline void small_pair(uint32_t wave, SmallObjs* S, const SNode* __re
                                   uint32_t max_depth) {
   const uint32_t lane = lane_id();
   const uint32_t h = lane >> 5, hl = lane & 31, off = 32 * h;
-  const uint32_t e = 2 * wave + h;
+  const uint32_t e = h ? eb : ea;
 
   bool has = false;
   SmallLane o;
@@ -1467,8 +1469,12 @@ __device__ inline void small_pair(uint32_t wave, SmallObjs* S, const SNode* __re
   }
 }
 
-// Persistent over the small-subtree queue: each wavefront takes pairs of
-// subtrees (only wave-local synchronisation inside).  Five wavefronts per SIMD
+// Persistent over the small-subtree queue in windows of SMALL_WIN subtrees: a
+// wavefront ranks its window's subtrees by size and builds them in pairs of
+// neighbours in that order, so the two halves of a wavefront mostly hold
+// subtrees of similar size and depth (a level costs as much as its largest
+// node, and a pair as many levels as its deeper subtree).  Only wave-local
+// synchronisation inside.  Five wavefronts per SIMD
 // (at most 96 VGPRs; the compiler keeps ~60 bytes per lane of pair set-up
 // state in scratch, outside the candidate loop) beat four at 115 VGPRs.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_small(const SNode* __restrict__ small,
@@ -1479,12 +1485,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                                                const uint32_t* __restrict__ tris_in,
                                                uint32_t* __restrict__ tris_out, Counters* ctr,
                                                uint32_t* gaps, uint32_t max_depth) {
+  constexpr uint32_t SMALL_WIN = 8;  // 16 and 32 measured slower: coarser work items
   __shared__ SmallObjs lds_obj[4];
+  __shared__ uint32_t lds_order[4][SMALL_WIN];
   const uint32_t w = threadIdx.x >> 6;
+  const uint32_t lane = lane_id();
   const uint32_t nS = *nS_dev;
-  for (uint32_t wave = blockIdx.x * 4 + w; 2 * wave < nS; wave += gridDim.x * 4)
-    small_pair(wave, &lds_obj[w], small, nS, rec0, rec1, out, perm, tris_in, tris_out, ctr, gaps,
-               max_depth);
+  for (uint32_t win = blockIdx.x * 4 + w; win * SMALL_WIN < nS; win += gridDim.x * 4) {
+    const uint32_t base = win * SMALL_WIN;
+    const uint32_t cnt = min(SMALL_WIN, nS - base);
+    // key: size (forced leaves above 32 as 33), then position: all distinct
+    uint32_t key = ~0u;
+    if (lane < cnt) key = (min(small[base + lane].count, 33u) << 3) | lane;
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < cnt; ++j)
+      rank += (uint32_t)__builtin_amdgcn_readlane((int)key, (int)j) < key;
+    if (lane < cnt) lds_order[w][rank] = base + lane;
+    wave_sync();
+    for (uint32_t q = 0; q < cnt; q += 2) {
+      const uint32_t ea = lds_order[w][q];
+      const uint32_t eb = q + 1 < cnt ? lds_order[w][q + 1] : nS;
+      small_pair(ea, eb, &lds_obj[w], small, nS, rec0, rec1, out, perm, tris_in, tris_out, ctr,
+                 gaps, max_depth);
+    }
+    wave_sync();  // the next window's order overwrites this one
+  }
 }
 
 // ------------------------------------------------------------------ compaction
