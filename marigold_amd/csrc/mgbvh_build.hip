@@ -1469,12 +1469,14 @@ __device__ inline void small_pair(uint32_t ea, uint32_t eb, SmallObjs* S,
   }
 }
 
-// Persistent over the small-subtree queue in windows of SMALL_WIN subtrees: a
-// wavefront ranks its window's subtrees by size and builds them in pairs of
-// neighbours in that order, so the two halves of a wavefront mostly hold
-// subtrees of similar size and depth (a level costs as much as its largest
-// node, and a pair as many levels as its deeper subtree).  Only wave-local
-// synchronisation inside.  Five wavefronts per SIMD
+// Persistent over the small-subtree queue in windows of `win` (2..SMALL_WIN)
+// subtrees: a wavefront ranks its window's subtrees by size and builds them in
+// pairs of neighbours in that order, so the two halves of a wavefront mostly
+// hold subtrees of similar size and depth (a level costs as much as its
+// largest node, and a pair as many levels as its deeper subtree).  Small builds
+// take windows of 2 (plain pairs): there a window's pairs, built one after
+// another, would lengthen the launch.  Only wave-local synchronisation inside.
+constexpr uint32_t SMALL_WIN = 8;  // windows of 16 and 32 measured slower: coarser work items  Five wavefronts per SIMD
 // (at most 96 VGPRs; the compiler keeps ~60 bytes per lane of pair set-up
 // state in scratch, outside the candidate loop) beat four at 115 VGPRs.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_small(const SNode* __restrict__ small,
@@ -1484,16 +1486,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                                                uint32_t* __restrict__ perm,
                                                const uint32_t* __restrict__ tris_in,
                                                uint32_t* __restrict__ tris_out, Counters* ctr,
-                                               uint32_t* gaps, uint32_t max_depth) {
-  constexpr uint32_t SMALL_WIN = 8;  // 16 and 32 measured slower: coarser work items
+                                               uint32_t* gaps, uint32_t max_depth,
+                                               uint32_t win) {
   __shared__ SmallObjs lds_obj[4];
   __shared__ uint32_t lds_order[4][SMALL_WIN];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t lane = lane_id();
   const uint32_t nS = *nS_dev;
-  for (uint32_t win = blockIdx.x * 4 + w; win * SMALL_WIN < nS; win += gridDim.x * 4) {
-    const uint32_t base = win * SMALL_WIN;
-    const uint32_t cnt = min(SMALL_WIN, nS - base);
+  for (uint32_t wi = blockIdx.x * 4 + w; wi * win < nS; wi += gridDim.x * 4) {
+    const uint32_t base = wi * win;
+    const uint32_t cnt = min(win, nS - base);
     // key: size (forced leaves above 32 as 33), then position: all distinct
     uint32_t key = ~0u;
     if (lane < cnt) key = (min(small[base + lane].count, 33u) << 3) | lane;
@@ -1862,9 +1864,14 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
   // subtrees of <= 32 objects (and depth-cut leaves), persistent over their queue
   const uint32_t g_small = std::max<uint32_t>(1, std::min<uint32_t>(cdiv(n, 8), cus * tu.small_bpc));
   MG_PROF(KK_SMALL, 0);
+  // size-ranked windows where even the fewest possible subtrees (n / 32) fill
+  // every wavefront of the grid with a window; plain pairs otherwise
+  const uint32_t win =
+      (uint64_t)n / (32u * SMALL_WIN) >= 4ull * g_small ? SMALL_WIN : 2u;
   k_small<<<g_small, 256, 0, st>>>(small, &ctr->n_small, recs[0], recs[1], out, o.perm,
                                    in.kind == 0 ? in.tris : nullptr,
-                                   in.kind == 0 ? o.tris_out : nullptr, ctr, gaps, s->max_depth);
+                                   in.kind == 0 ? o.tris_out : nullptr, ctr, gaps, s->max_depth,
+                                   win);
   MG_PROF(KK_SMALL, 1);
   MG_HIP(hipGetLastError());
   return MGBVH_OK;

@@ -286,3 +286,20 @@ def test_tuning_limits(ctx):
         assert ctx.get_tuning()["bin_blocks_per_cu"] == 256
     finally:
         ctx.set_tuning(**{k: v for k, v in old.items() if k != "compute_units"})
+
+
+def test_small_windows_same_trees(ctx):
+    """k_small ranks windows of subtrees by size only when even the fewest
+    possible subtrees fill its grid; a one-block-per-CU grid makes a 320 k
+    triangle batch take that path, and the trees equal the plain pairs'."""
+    print("[case] size-ranked subtree windows vs plain pairs", flush=True)
+    meshes = [scenes.soup(200000, seed=901), scenes.heightfield(300, 200, seed=902)]
+    base, _ = _batch(ctx, meshes)
+    old = ctx.get_tuning()
+    try:
+        ctx.set_tuning(small_blocks_per_cu=1)
+        win, _ = _batch(ctx, meshes)
+    finally:
+        ctx.set_tuning(**{k: v for k, v in old.items() if k != "compute_units"})
+    for k in ("nodes", "lens", "splits", "perm", "tris"):
+        assert np.array_equal(np.asarray(win[k]).view(np.uint8), np.asarray(base[k]).view(np.uint8)), k
