@@ -261,6 +261,33 @@ int mgbvh_trace_scene_device(mgbvh_ctx* ctx, const mgbvh_node32* d_tlas_nodes,
                              const uint32_t* d_tris_perm, const uint32_t* d_blas_perm,
                              const mgbvh_ray* d_rays, uint64_t nrays, mgbvh_hit* d_hits);
 
+/* The path tracer's own closest-hit traversal, step for step: replaces
+ * Scene.raytraceTlas / raytraceBlas (scene.slang:91-221) with the box test of
+ * scene.slang:296-310, the triangle test of :19-89 and Hit.merge of
+ * intersect.slang:73-82.  Unlike mgbvh_trace_scene_device the result depends
+ * on the tree, as the shader's does: one 64-entry stack shared by both levels
+ * (BVH_STACK_SIZE, scene.slang:5-7; an overflow returns MGBVH_ERR_INTERNAL
+ * where the shader would write past it); children pushed when their box
+ * distance is strictly below the closest distance, nearer on top; no cull on
+ * pop; the BLAS root pushed without a box test; the BLAS pass starts from the
+ * world distance times length(object-space direction) (:165-166) and returns
+ * its distance unconverted (:211-213); an equal distance goes to the hit
+ * tested later.  Every op is IEEE fp32 round-to-nearest without contraction;
+ * min/max of a NaN take the other operand.  One object per leaf (the shader's
+ * node has no length); every instance's BLAS must be non-empty (the shader
+ * reads its root unconditionally).  ray.tmax is ignored (the shader has none).
+ * Per ray: t = Hit.distance (0 for NO_HIT); prim, inst, tri_slot as in
+ * mgbvh_hit, 0xffffffff on a miss; tri_slot 0xfffffffe for a success that
+ * holds no triangle (a merge that took NO_HIT, only when a distance reaches
+ * float.maxValue). */
+int mgbvh_raytrace_tlas_device(mgbvh_ctx* ctx, const mgbvh_node32* d_tlas_nodes,
+                               const uint32_t* d_tlas_perm, const mgbvh_instance* d_instances,
+                               uint32_t ninstances, const mgbvh_blas_desc* d_blas, uint32_t nblas,
+                               const mgbvh_node32* d_blas_nodes, const float* d_positions,
+                               uint32_t vstride_bytes, const uint32_t* d_tris_perm,
+                               const uint32_t* d_blas_perm, const mgbvh_ray* d_rays,
+                               uint64_t nrays, mgbvh_hit* d_hits);
+
 /* Primary rays of a width x height image, row-major from the top-left pixel. */
 int mgbvh_gen_primary_rays_device(mgbvh_ctx* ctx, const mgbvh_camera* cam, uint32_t width,
                                   uint32_t height, mgbvh_ray* d_rays);

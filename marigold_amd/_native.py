@@ -177,6 +177,8 @@ _EXTRA: dict = {
     "mgbvh_trace_closest_device": ([_P, _P, _P, _P, _U32, _P, _P, _P, _U64, _P], _I),
     "mgbvh_trace_scene_device": ([_P, _P, _P, _P, _P, _U32, _P, _U32, _P, _P, _P, _U32, _P, _P,
                                   _P, _U64, _P], _I),
+    "mgbvh_raytrace_tlas_device": ([_P, _P, _P, _P, _U32, _P, _U32, _P, _P, _U32, _P, _P, _P,
+                                    _U64, _P], _I),
     "mgbvh_gen_primary_rays_device": ([_P, C.POINTER(Camera), _U32, _U32, _P], _I),
     "mgbvh_copy_d2d": ([_P, _P, _P, _U64], _I),
     "mgbvh_instance_boxes_device": ([_P, _P, _P, _U32, _P, _P, _U32, _P], _I),
@@ -474,6 +476,14 @@ class Context:
             self.handle, _dv(tlas_nodes), _dv(tlas_lens), _dv(tlas_perm), _dv(instances),
             ninstances, _dv(blas), nblas, _dv(blas_nodes), _dv(blas_lens), _dv(pos), vstride,
             _dv(tris_perm), _dv(blas_perm), _dv(rays), nrays, _dv(hits)))
+
+    def raytrace_tlas(self, tlas_nodes, tlas_perm, instances, ninstances, blas, nblas,
+                      blas_nodes, pos, vstride, tris_perm, blas_perm, rays, nrays, hits):
+        """The path tracer's own traversal (tree dependent, scene.slang:91-221)."""
+        check(lib().mgbvh_raytrace_tlas_device(
+            self.handle, _dv(tlas_nodes), _dv(tlas_perm), _dv(instances), ninstances, _dv(blas),
+            nblas, _dv(blas_nodes), _dv(pos), vstride, _dv(tris_perm), _dv(blas_perm), _dv(rays),
+            nrays, _dv(hits)))
 
     def gen_primary(self, cam: "Camera", width: int, height: int, rays):
         check(lib().mgbvh_gen_primary_rays_device(self.handle, C.byref(cam), width, height,

@@ -58,8 +58,9 @@ __device__ inline float dot3_rn(const float* a, const float* b) {
   return __fadd_rn(__fadd_rn(__fmul_rn(a[0], b[0]), __fmul_rn(a[1], b[1])), __fmul_rn(a[2], b[2]));
 }
 
-// scene.slang:19-89 (distance only); +inf on a miss.
-__device__ inline float ray_tri(const RayD& r, const float* v1, const float* v2, const float* v3) {
+// scene.slang:19-89 (distance only): true and t on a hit.
+__device__ inline bool ray_tri_hit(const RayD& r, const float* v1, const float* v2,
+                                   const float* v3, float& t_out) {
   float e1[3], e2[3], h[3], s[3], q[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) { e1[a] = __fsub_rn(v2[a], v1[a]); e2[a] = __fsub_rn(v3[a], v1[a]); }
@@ -67,20 +68,27 @@ __device__ inline float ray_tri(const RayD& r, const float* v1, const float* v2,
   h[1] = __fsub_rn(__fmul_rn(r.d[2], e2[0]), __fmul_rn(r.d[0], e2[2]));
   h[2] = __fsub_rn(__fmul_rn(r.d[0], e2[1]), __fmul_rn(r.d[1], e2[0]));
   const float det = dot3_rn(h, e1);
-  if (fabsf(det) < 1e-6f) return f_inf();
+  if (fabsf(det) < 1e-6f) return false;
   const float f = __fdiv_rn(1.0f, det);
 #pragma unroll
   for (int a = 0; a < 3; ++a) s[a] = __fsub_rn(r.o[a], v1[a]);
   const float u = __fmul_rn(f, dot3_rn(s, h));
-  if (u < 0.0f || u > 1.0f) return f_inf();
+  if (u < 0.0f || u > 1.0f) return false;
   q[0] = __fsub_rn(__fmul_rn(s[1], e1[2]), __fmul_rn(s[2], e1[1]));
   q[1] = __fsub_rn(__fmul_rn(s[2], e1[0]), __fmul_rn(s[0], e1[2]));
   q[2] = __fsub_rn(__fmul_rn(s[0], e1[1]), __fmul_rn(s[1], e1[0]));
   const float v = __fmul_rn(f, dot3_rn(r.d, q));
-  if (v < 0.0f || __fadd_rn(u, v) > 1.0f) return f_inf();
+  if (v < 0.0f || __fadd_rn(u, v) > 1.0f) return false;
   const float t = __fmul_rn(f, dot3_rn(e2, q));
-  if (t < 1e-6f) return f_inf();
-  return t;
+  if (t < 1e-6f) return false;
+  t_out = t;
+  return true;
+}
+
+// t of a hit, +inf on a miss.
+__device__ inline float ray_tri(const RayD& r, const float* v1, const float* v2, const float* v3) {
+  float t;
+  return ray_tri_hit(r, v1, v2, v3, t) ? t : f_inf();
 }
 
 __device__ inline void load_node(const mgbvh_node32* nodes, uint32_t v, float* bmn, float* bmx,
@@ -260,6 +268,152 @@ __global__ __launch_bounds__(256) void k_trace_scene(
   if (overflow) atomicOr(overflow_flag, overflow);
 }
 
+// ------------------------------------------------ the path tracer's traversal
+// scene.slang:91-221 step for step: one 64-entry stack of node payloads
+// (objectIndex, childNode) shared by both levels (BVH_STACK_SIZE,
+// scene.slang:5-7); children pushed when their box distance is strictly below
+// the closest distance, nearer on top; a popped node is not culled again; the
+// BLAS root is pushed without a box test; the BLAS pass starts from the world
+// distance times length(object-space direction) and its result is merged back
+// without conversion; Hit.merge keeps the earlier hit only when strictly
+// closer.  min/max of a NaN take the other operand (fminf/fmaxf).
+constexpr int SHADER_STACK = 64;
+constexpr uint32_t SHADER_NO_TRI = 0xfffffffeu;
+
+struct SHit {  // Hit.success, Hit.distance and the triangle behind them
+  float t;
+  uint32_t succ, inst, prim, slot;
+};
+
+__device__ inline float f_max_finite() { return __uint_as_float(0x7f7fffffu); }
+
+// rayIntersectBoundingVolume, scene.slang:296-310.
+__device__ inline bool shader_box(const RayD& r, const float* bmn, const float* bmx, float& dist) {
+  float t1[3], t2[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float lo = __fmul_rn(__fsub_rn(bmn[a], r.o[a]), r.inv[a]);
+    const float hi = __fmul_rn(__fsub_rn(bmx[a], r.o[a]), r.inv[a]);
+    t1[a] = fminf(lo, hi);
+    t2[a] = fmaxf(lo, hi);
+  }
+  const float tn = fmaxf(fmaxf(t1[0], t1[1]), t1[2]);
+  const float tf = fminf(fminf(t2[0], t2[1]), t2[2]);
+  const bool front = tn == 0.0f;
+  dist = front ? tf : tn;
+  return front || (tf >= tn && tf > 0.0f);
+}
+
+// Hit.merge, intersect.slang:73-82.
+__device__ inline void shader_merge(SHit& a, const SHit& b) {
+  const float da = a.succ ? a.t : f_max_finite();
+  const float db = b.succ ? b.t : f_max_finite();
+  const uint32_t s = a.succ | b.succ;
+  if (!(da < db)) a = b;
+  a.succ = s;
+}
+
+__device__ inline void shader_push(uint32_t* ps, uint32_t* pc, int& sp, uint32_t start,
+                                   uint32_t child, uint32_t& err) {
+  if (sp >= SHADER_STACK) { err |= 1u; return; }
+  ps[sp] = start;
+  pc[sp] = child;
+  ++sp;
+}
+
+// The child step of both loops (scene.slang:112-134, 180-202).
+__device__ inline void shader_children(const RayD& r, const mgbvh_node32* nodes, uint32_t child,
+                                       const SHit& closest, uint32_t* ps, uint32_t* pc, int& sp,
+                                       uint32_t& err) {
+  float amn[3], amx[3], bmn[3], bmx[3], da, db;
+  uint32_t as, ac, bs, bc;
+  load_node(nodes, child, amn, amx, as, ac);
+  load_node(nodes, child + 1, bmn, bmx, bs, bc);
+  const float cd = closest.succ ? closest.t : f_max_finite();
+  const bool pa = shader_box(r, amn, amx, da) && da < cd;
+  const bool pb = shader_box(r, bmn, bmx, db) && db < cd;
+  if (pa && pb) {
+    const bool a_first = da < db;  // the nearer child is pushed last
+    shader_push(ps, pc, sp, a_first ? bs : as, a_first ? bc : ac, err);
+    shader_push(ps, pc, sp, a_first ? as : bs, a_first ? ac : bc, err);
+  } else if (pa || pb) {
+    shader_push(ps, pc, sp, pa ? as : bs, pa ? ac : bc, err);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_raytrace_tlas(
+    const mgbvh_node32* __restrict__ tnodes, const uint32_t* __restrict__ tperm,
+    const mgbvh_instance* __restrict__ insts, const mgbvh_blas_desc* __restrict__ blas,
+    const mgbvh_node32* __restrict__ bnodes, const float* __restrict__ pos, uint32_t vs,
+    const uint32_t* __restrict__ tris, const uint32_t* __restrict__ bperm,
+    const mgbvh_ray* __restrict__ rays, uint32_t nrays, uint32_t ninst, uint32_t nblas,
+    mgbvh_hit* __restrict__ hits, uint32_t* err_flag) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrays) return;
+  float unused_tmax;
+  const RayD r = load_ray(rays, i, unused_tmax);
+  SHit best = {0.0f, 0u, NONE, NONE, NONE};  // Hit.NO_HIT
+  uint32_t err = 0;
+  uint32_t ps[SHADER_STACK], pc[SHADER_STACK];
+  int sp = 0;
+  float bmn[3], bmx[3], dist;
+  uint32_t start, child;
+  load_node(tnodes, 0, bmn, bmx, start, child);
+  if (shader_box(r, bmn, bmx, dist)) shader_push(ps, pc, sp, start, child, err);
+  while (sp > 0) {
+    --sp;
+    const uint32_t s0 = ps[sp], c0 = pc[sp];
+    if (c0 != 0) {
+      shader_children(r, tnodes, c0, best, ps, pc, sp, err);
+      continue;
+    }
+    // raytraceBlas (scene.slang:143-220) for meshes[s0]
+    const uint32_t id = tperm[s0];
+    if (id >= ninst) { err |= 2u; continue; }
+    const mgbvh_instance& in = insts[id];
+    if (in.blas >= nblas) { err |= 2u; continue; }
+    const mgbvh_blas_desc bd = blas[in.blas];
+    RayD lr;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {  // mul(inverseTransform, float4(v, w)), w = 1 and 0
+      const float* row = in.world_to_object + 4 * k;
+      lr.o[k] = __fadd_rn(dot3_rn(row, r.o), __fmul_rn(row[3], 1.0f));
+      lr.d[k] = __fadd_rn(dot3_rn(row, r.d), __fmul_rn(row[3], 0.0f));
+    }
+    set_inv(lr);
+    SHit loc = best;
+    loc.t = __fmul_rn(best.t, sqrtf(dot3_rn(lr.d, lr.d)));  // correctly rounded
+    const mgbvh_node32* bn = bnodes + bd.node_offset;
+    const int base = sp;
+    load_node(bn, 0, bmn, bmx, start, child);
+    shader_push(ps, pc, sp, start, child, err);
+    while (sp > base) {
+      --sp;
+      const uint32_t s1 = ps[sp], c1 = pc[sp];
+      if (c1 != 0) {
+        shader_children(lr, bn, c1, loc, ps, pc, sp, err);
+      } else {
+        const uint32_t slot = bd.tri_offset + s1;
+        const uint32_t* tr = tris + 3 * (size_t)slot;
+        SHit h = {0.0f, 0u, NONE, NONE, NONE};
+        float t;
+        if (ray_tri_hit(lr, pos + (size_t)(tr[0] + bd.vertex_offset) * vs,
+                        pos + (size_t)(tr[1] + bd.vertex_offset) * vs,
+                        pos + (size_t)(tr[2] + bd.vertex_offset) * vs, t)) {
+          h.t = t; h.succ = 1u; h.inst = id; h.prim = bperm[slot]; h.slot = slot;
+        }
+        shader_merge(loc, h);
+      }
+    }
+    shader_merge(best, loc);
+  }
+  const bool no_tri = best.succ && best.prim == NONE;
+  reinterpret_cast<uint4*>(hits)[i] =
+      make_uint4(__float_as_uint(best.t), best.succ ? best.prim : NONE,
+                 best.succ ? best.inst : NONE, !best.succ ? NONE : no_tri ? SHADER_NO_TRI : best.slot);
+  if (err) atomicOr(err_flag, err);
+}
+
 // ---------------------------------------------------------------- rays
 // PCG hash of random.slang:5-9.
 __device__ inline uint32_t pcg(uint32_t& seed) {
@@ -354,7 +508,8 @@ int launch_done(const char* what) {
 }
 
 // Reads the overflow flag after the stream drains.
-int finish_trace(mgbvh_ctx* c, uint32_t* flag) {
+int finish_trace(mgbvh_ctx* c, uint32_t* flag,
+                 const char* overflow = "traversal stack overflow (tree height above 62)") {
   hipError_t e = hipMemcpyAsync(&c->h_ctr->err, flag, 4, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
@@ -365,7 +520,7 @@ int finish_trace(mgbvh_ctx* c, uint32_t* flag) {
     return MGBVH_ERR_INDEX;
   }
   if (c->h_ctr->err) {
-    set_error("traversal stack overflow (tree height above 62)");
+    set_error(overflow);
     return MGBVH_ERR_INTERNAL;
   }
   return MGBVH_OK;
@@ -454,6 +609,36 @@ int mgbvh_trace_scene_device(mgbvh_ctx* c, const mgbvh_node32* d_tlas_nodes,
   MG_TRACE_PROF(1);
   if ((rc = launch_done("k_trace_scene"))) return rc;
   return finish_trace(c, flag);
+}
+
+int mgbvh_raytrace_tlas_device(mgbvh_ctx* c, const mgbvh_node32* d_tlas_nodes,
+                               const uint32_t* d_tlas_perm, const mgbvh_instance* d_instances,
+                               uint32_t ninstances, const mgbvh_blas_desc* d_blas, uint32_t nblas,
+                               const mgbvh_node32* d_blas_nodes, const float* d_pos,
+                               uint32_t vstride_bytes, const uint32_t* d_tris_perm,
+                               const uint32_t* d_blas_perm, const mgbvh_ray* d_rays,
+                               uint64_t nrays, mgbvh_hit* d_hits) {
+  if (!c) { set_error("ctx is NULL"); return MGBVH_ERR_INVALID_ARG; }
+  if (nrays == 0) return MGBVH_OK;
+  if (ninstances == 0 || nblas == 0) {
+    set_error("a scene needs at least one instance and one BLAS");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  if (!d_tlas_nodes || !d_tlas_perm || !d_instances || !d_blas || !d_blas_nodes || !d_pos ||
+      !d_tris_perm || !d_blas_perm) {
+    set_error("null device buffer");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  uint32_t* flag = nullptr;
+  int rc = trace_begin(c, d_rays, d_hits, nrays, vstride_bytes, &flag);
+  if (rc) return rc;
+  MG_TRACE_PROF(0);
+  k_raytrace_tlas<<<(uint32_t)((nrays + 255) / 256), 256, 0, c->stream>>>(
+      d_tlas_nodes, d_tlas_perm, d_instances, d_blas, d_blas_nodes, d_pos, vstride_bytes / 4,
+      d_tris_perm, d_blas_perm, d_rays, (uint32_t)nrays, ninstances, nblas, d_hits, flag);
+  MG_TRACE_PROF(1);
+  if ((rc = launch_done("k_raytrace_tlas"))) return rc;
+  return finish_trace(c, flag, "the 64-entry traversal stack of the path tracer overflowed");
 }
 
 int mgbvh_gen_primary_rays_device(mgbvh_ctx* c, const mgbvh_camera* cam, uint32_t width,

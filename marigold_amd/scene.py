@@ -168,6 +168,13 @@ class GpuScene:
                              d["blas_lens"], d["pos"], 12, d["tris_perm"], d["blas_perm"],
                              rays_dev, nrays, hits_dev)
 
+    def raytrace(self, rays_dev: N.DeviceBuffer, nrays: int, hits_dev: N.DeviceBuffer):
+        """The path tracer's own traversal over these trees (tree dependent)."""
+        d = self.d
+        self.ctx.raytrace_tlas(d["tlas_nodes"], d["tlas_perm"], d["instances"], self.n_instances,
+                               d["blas"], self.n_meshes, d["blas_nodes"], d["pos"], 12,
+                               d["tris_perm"], d["blas_perm"], rays_dev, nrays, hits_dev)
+
     def bounce(self, rays_in, hits, nrays, seed, rays_out):
         d = self.d
         self.ctx.gen_bounce(rays_in, hits, nrays, d["instances"], self.n_instances, d["blas"],

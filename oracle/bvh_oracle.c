@@ -27,6 +27,7 @@
  */
 #include "bvh_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -559,8 +560,9 @@ double or_tree_sah(const or_node* nodes, uint32_t nn) {
 /* ---------------------------------------------------------------- closest hit */
 
 /* Moller-Trumbore as in scene.slang:19-89 (float3 cross/dot expanded left to
- * right, no FMA).  Returns t, or +inf on a miss. */
-float or_ray_tri(const float* r, const float* v1, const float* v2, const float* v3) {
+ * right, no FMA).  Returns 1 and t on a hit, 0 on a miss. */
+static int or_ray_tri_hit(const float* r, const float* v1, const float* v2, const float* v3,
+                          float* t_out) {
   float e1[3], e2[3], h[3], s[3], q[3];
   for (int a = 0; a < 3; ++a) { e1[a] = v2[a] - v1[a]; e2[a] = v3[a] - v1[a]; }
   const float* o = r;
@@ -570,25 +572,32 @@ float or_ray_tri(const float* r, const float* v1, const float* v2, const float* 
   h[2] = d[0] * e2[1] - d[1] * e2[0];
   float det = h[0] * e1[0] + h[1] * e1[1];
   det = det + h[2] * e1[2];
-  if (fabsf(det) < 1e-6f) return INFINITY;
+  if (fabsf(det) < 1e-6f) return 0;
   float f = 1.0f / det;
   for (int a = 0; a < 3; ++a) s[a] = o[a] - v1[a];
   float du = s[0] * h[0] + s[1] * h[1];
   du = du + s[2] * h[2];
   float u = f * du;
-  if (u < 0.0f || u > 1.0f) return INFINITY;
+  if (u < 0.0f || u > 1.0f) return 0;
   q[0] = s[1] * e1[2] - s[2] * e1[1];
   q[1] = s[2] * e1[0] - s[0] * e1[2];
   q[2] = s[0] * e1[1] - s[1] * e1[0];
   float dv = d[0] * q[0] + d[1] * q[1];
   dv = dv + d[2] * q[2];
   float v = f * dv;
-  if (v < 0.0f || u + v > 1.0f) return INFINITY;
+  if (v < 0.0f || u + v > 1.0f) return 0;
   float dt = e2[0] * q[0] + e2[1] * q[1];
   dt = dt + e2[2] * q[2];
   float t = f * dt;
-  if (t < 1e-6f) return INFINITY;
-  return t;
+  if (t < 1e-6f) return 0;
+  *t_out = t;
+  return 1;
+}
+
+/* t of a hit, +inf on a miss. */
+float or_ray_tri(const float* r, const float* v1, const float* v2, const float* v3) {
+  float t;
+  return or_ray_tri_hit(r, v1, v2, v3, &t) ? t : INFINITY;
 }
 
 void or_closest_hit_brute(const float* pos, uint32_t vs, const uint32_t* tris, uint32_t ntris,
@@ -709,4 +718,153 @@ void or_closest_hit_scene_brute(const float* pos, uint32_t vs, const uint32_t* t
     prim_out[r] = bp;
     inst_out[r] = bi;
   }
+}
+
+/* ------------------------------------------------- the path tracer's traversal */
+
+/* Hit of scene.slang's traversal: the fields the closest-hit decision reads
+ * (Hit.success, Hit.distance, intersect.slang:27-41) plus the triangle that
+ * made it.  NO_HIT has distance 0 and no triangle. */
+typedef struct {
+  float t;
+  uint32_t succ, inst, prim, slot;
+} sh_hit;
+
+#define SH_STACK 64 /* BVH_STACK_SIZE = TLAS_MAX_DEPTH + BLAS_MAX_DEPTH, scene.slang:5-7 */
+#define SH_NONE 0xffffffffu
+
+/* rayIntersectBoundingVolume, scene.slang:296-310.  min/max take the other
+ * operand when one is NaN (IEEE minNum, fminf). */
+static int sh_box(const float* o, const float* inv, const float* bmn, const float* bmx,
+                  float* dist) {
+  float t1[3], t2[3];
+  for (int a = 0; a < 3; ++a) {
+    float lo = (bmn[a] - o[a]) * inv[a];
+    float hi = (bmx[a] - o[a]) * inv[a];
+    t1[a] = fminf(lo, hi);
+    t2[a] = fmaxf(lo, hi);
+  }
+  float tn = fmaxf(fmaxf(t1[0], t1[1]), t1[2]);
+  float tf = fminf(fminf(t2[0], t2[1]), t2[2]);
+  int front = tn == 0.0f;
+  *dist = front ? tf : tn;
+  return front || (tf >= tn && tf > 0.0f);
+}
+
+/* Hit.merge, intersect.slang:73-82: b replaces a unless a is strictly closer
+ * (a missing hit counts as float.maxValue); success is the OR of both. */
+static void sh_merge(sh_hit* a, const sh_hit* b) {
+  float da = a->succ ? a->t : FLT_MAX;
+  float db = b->succ ? b->t : FLT_MAX;
+  uint32_t s = a->succ | b->succ;
+  if (!(da < db)) *a = *b;
+  a->succ = s;
+}
+
+/* Stack.push, stack.slang: the shader writes past a full stack; here the push
+ * is dropped and reported. */
+static void sh_push(uint32_t* ps, uint32_t* pc, int* sp, uint32_t start, uint32_t child,
+                    int* err) {
+  if (*sp >= SH_STACK) { *err |= 1; return; }
+  ps[*sp] = start;
+  pc[*sp] = child;
+  ++*sp;
+}
+
+/* The child step of both loops, scene.slang:112-134 and 180-202. */
+static void sh_children(const uint32_t* nodes8, uint32_t child, const float* o, const float* inv,
+                        const sh_hit* closest, uint32_t* ps, uint32_t* pc, int* sp, int* err) {
+  const uint32_t* na = nodes8 + 8 * (size_t)child;
+  const uint32_t* nb = na + 8;
+  float cd = closest->succ ? closest->t : FLT_MAX, da, db;
+  int ha = sh_box(o, inv, (const float*)na, (const float*)na + 4, &da);
+  int hb = sh_box(o, inv, (const float*)nb, (const float*)nb + 4, &db);
+  int pa = ha && da < cd, pb = hb && db < cd;
+  if (pa && pb) {
+    const uint32_t* first = da < db ? nb : na;
+    const uint32_t* second = da < db ? na : nb;
+    sh_push(ps, pc, sp, first[3], first[7], err);
+    sh_push(ps, pc, sp, second[3], second[7], err);
+  } else if (pa || pb) {
+    const uint32_t* one = pa ? na : nb;
+    sh_push(ps, pc, sp, one[3], one[7], err);
+  }
+}
+
+int or_raytrace_tlas(const uint32_t* tlas8, const uint32_t* tlas_perm, const uint32_t* inst16,
+                     uint32_t ninst, const uint32_t* blas4, uint32_t nblas, const uint32_t* blas8,
+                     const float* pos, uint32_t vs, const uint32_t* tris_perm,
+                     const uint32_t* blas_perm, const float* rays, uint32_t nrays,
+                     uint32_t* hits4) {
+  int err = 0;
+  for (uint32_t r = 0; r < nrays; ++r) {
+    const float* o = rays + 6 * (size_t)r;
+    const float* d = o + 3;
+    float inv[3];
+    for (int a = 0; a < 3; ++a) inv[a] = 1.0f / d[a]; /* Ray.__init, intersect.slang:9 */
+    sh_hit best = {0.0f, 0, SH_NONE, SH_NONE, SH_NONE};
+    uint32_t ps[SH_STACK], pc[SH_STACK];
+    int sp = 0;
+    float dist;
+    if (sh_box(o, inv, (const float*)tlas8, (const float*)tlas8 + 4, &dist))
+      sh_push(ps, pc, &sp, tlas8[3], tlas8[7], &err);
+    while (sp > 0) {
+      --sp;
+      const uint32_t start = ps[sp], child = pc[sp];
+      if (child != 0) {
+        sh_children(tlas8, child, o, inv, &best, ps, pc, &sp, &err);
+        continue;
+      }
+      /* raytraceBlas, scene.slang:143-220, for meshes[start] */
+      const uint32_t id = tlas_perm[start];
+      if (id >= ninst) { err |= 2; continue; }
+      const uint32_t* rec = inst16 + 16 * (size_t)id;
+      if (rec[12] >= nblas) { err |= 2; continue; }
+      const uint32_t* bd = blas4 + 4 * (size_t)rec[12];
+      float m[12], lo[3], ld[3], linv[3];
+      memcpy(m, rec, sizeof m);
+      for (int i = 0; i < 3; ++i) { /* mul(inverseTransform, float4(p, w)), :158-161 */
+        const float* row = m + 4 * i;
+        float x = row[0] * o[0] + row[1] * o[1];
+        x = x + row[2] * o[2];
+        lo[i] = x + row[3] * 1.0f;
+        float y = row[0] * d[0] + row[1] * d[1];
+        y = y + row[2] * d[2];
+        ld[i] = y + row[3] * 0.0f;
+      }
+      for (int a = 0; a < 3; ++a) linv[a] = 1.0f / ld[a];
+      float dd = ld[0] * ld[0] + ld[1] * ld[1];
+      dd = dd + ld[2] * ld[2];
+      sh_hit loc = best; /* :165-166: the world distance scaled by length(localRay.dir) */
+      loc.t = best.t * sqrtf(dd);
+      const uint32_t* bn = blas8 + 8 * (size_t)bd[0];
+      const int base = sp;
+      sh_push(ps, pc, &sp, bn[3], bn[7], &err); /* the root, without a box test (:172) */
+      float lr[6] = {lo[0], lo[1], lo[2], ld[0], ld[1], ld[2]};
+      while (sp > base) {
+        --sp;
+        const uint32_t s2 = ps[sp], c2 = pc[sp];
+        if (c2 != 0) {
+          sh_children(bn, c2, lo, linv, &loc, ps, pc, &sp, &err);
+        } else {
+          const uint32_t slot = bd[1] + s2;
+          const uint32_t* tr = tris_perm + 3 * (size_t)slot;
+          float t;
+          int ok = or_ray_tri_hit(lr, pos + (size_t)(tr[0] + bd[2]) * vs,
+                                  pos + (size_t)(tr[1] + bd[2]) * vs,
+                                  pos + (size_t)(tr[2] + bd[2]) * vs, &t);
+          sh_hit h = {0.0f, 0, SH_NONE, SH_NONE, SH_NONE};
+          if (ok) { h.t = t; h.succ = 1; h.inst = id; h.prim = blas_perm[slot]; h.slot = slot; }
+          sh_merge(&loc, &h);
+        }
+      }
+      sh_merge(&best, &loc); /* :216, no conversion back to world distance (:211-213) */
+    }
+    uint32_t* out = hits4 + 4 * (size_t)r;
+    out[0] = fbits(best.t);
+    out[1] = best.succ ? best.prim : SH_NONE;
+    out[2] = best.succ ? best.inst : SH_NONE;
+    out[3] = !best.succ ? SH_NONE : best.prim == SH_NONE ? 0xfffffffeu : best.slot;
+  }
+  return err;
 }

@@ -97,6 +97,28 @@ void or_closest_hit_scene_brute(const float* pos, uint32_t vstride_floats, const
                                 const float* rays, uint32_t nrays, float* t_out,
                                 uint32_t* prim_out, uint32_t* inst_out);
 
+/* The path tracer's own two-level traversal, step for step
+ * (scene.slang:91-221; box test :296-310; Hit.merge intersect.slang:73-82;
+ * one 64-entry stack shared by both levels, scene.slang:5-7): the BLAS root is
+ * pushed without a box test, a popped node is not culled again, children are
+ * pushed when their box distance is strictly below the closest distance, the
+ * BLAS pass starts from the world distance times length(object-space
+ * direction) and its result is merged back unconverted, and an equal distance
+ * goes to the hit tested later.  Tree dependent by design.
+ * tlas8/blas8: 32-byte nodes as (k, 8) u32 (BLAS child indices relative to
+ * the BLAS's first node); tlas_perm: TLAS start_index -> instance;
+ * inst16: as or_closest_hit_scene_brute; blas4: per BLAS {node offset,
+ * triangle offset, vertex offset, 0}; tris_perm/blas_perm: permuted
+ * triangles and their original indices.  hits4 per ray: {distance bits,
+ * prim, inst, slot}: a miss is {distance, ~0, ~0, ~0}; a success that holds no
+ * triangle (a merge that took NO_HIT) has slot 0xfffffffe.  Returns 0, or
+ * bit 0 = a push on the full stack was dropped, bit 1 = an index out of range. */
+int or_raytrace_tlas(const uint32_t* tlas8, const uint32_t* tlas_perm, const uint32_t* inst16,
+                     uint32_t ninst, const uint32_t* blas4, uint32_t nblas, const uint32_t* blas8,
+                     const float* pos, uint32_t vstride_floats, const uint32_t* tris_perm,
+                     const uint32_t* blas_perm, const float* rays, uint32_t nrays,
+                     uint32_t* hits4);
+
 #ifdef __cplusplus
 }
 #endif

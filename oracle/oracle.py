@@ -72,6 +72,9 @@ def lib():
         L.or_closest_hit_bvh.argtypes = [P, P, P, C.c_uint32, P, P, P, C.c_uint32, P, P]
         L.or_closest_hit_scene_brute.argtypes = [P, C.c_uint32, P, P, P, C.c_uint32, P,
                                                  C.c_uint32, P, P, P]
+        L.or_raytrace_tlas.argtypes = [P, P, P, C.c_uint32, P, C.c_uint32, P, P, C.c_uint32, P,
+                                       P, P, C.c_uint32, P]
+        L.or_raytrace_tlas.restype = C.c_int
         L.or_ray_tri.argtypes = [P, P, P, P]
         L.or_ray_tri.restype = C.c_float
         _lib = L
@@ -195,6 +198,24 @@ def closest_hit_scene_brute(pos, tris, mesh3, inst16, rays):
     lib().or_closest_hit_scene_brute(_p(pos), pos.shape[1], _p(tris), _p(mesh3), _p(inst16),
                                      inst16.shape[0], _p(rays), n, _p(t), _p(prim), _p(inst))
     return t, prim, inst
+
+
+def raytrace_tlas(tlas8, tlas_perm, inst16, blas4, blas8, pos, tris_perm, blas_perm, rays):
+    """The path tracer's own traversal (scene.slang:91-221), tree dependent.
+    tlas8/blas8: (k, 8) u32 nodes; tlas_perm: TLAS start_index -> instance;
+    inst16: (I, 16) u32 instance records; blas4: (M, 4) u32 {node offset,
+    triangle offset, vertex offset, 0}; rays: (N, 6).  Returns ((N, 4) u32
+    {distance bits, prim, inst, slot}, status bits)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.uint32) for a in
+            (tlas8, tlas_perm, inst16, blas4, blas8, tris_perm, blas_perm)]
+    tlas8, tlas_perm, inst16, blas4, blas8, tris_perm, blas_perm = arrs
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    out = np.empty((rays.shape[0], 4), dtype=np.uint32)
+    rc = lib().or_raytrace_tlas(_p(tlas8), _p(tlas_perm), _p(inst16), inst16.shape[0], _p(blas4),
+                                blas4.shape[0], _p(blas8), _p(pos), pos.shape[1], _p(tris_perm),
+                                _p(blas_perm), _p(rays), rays.shape[0], _p(out))
+    return out, int(rc)
 
 
 def instance_boxes(mesh_min, mesh_max, o2w12):
