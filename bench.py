@@ -336,6 +336,24 @@ def scene_main(args):
     ms_bat = ctx.elapsed_ms(2, 3) / args.steps
     sah_plain = bat.scene_sah()
     hits_plain = _hits_of(ctx, bat.trace, d_rays, w * h)
+    # both traversals over the plain trees, 1M primary rays each, event-timed
+    trace_ms = {}
+    d_hits = ctx.alloc(16 * w * h)
+    for name, fn in (("tree_independent", bat.trace), ("shader_exact", bat.raytrace)):
+        fn(d_rays, w * h, d_hits)
+        ctx.synchronize()
+        ctx.record(2)
+        for _ in range(3):
+            fn(d_rays, w * h, d_hits)
+        ctx.record(3)
+        ctx.synchronize()
+        trace_ms[name] = ctx.elapsed_ms(2, 3) / 3
+    hits_shader = d_hits.download(N.HIT_DTYPE, w * h)
+    d_hits.free()
+    shader_same = float(np.mean((hits_shader["prim"] == hits_plain["prim"])
+                                & (hits_shader["inst"] == hits_plain["inst"])))
+    log(f"traversal of {w * h} rays: " + ", ".join(f"{k} {v:.2f} ms" for k, v in trace_ms.items())
+        + f"; shader-exact hits on the same triangle as the tree-independent ones: {shader_same:.5f}")
     bat_sum = int(bat.download("blas_nodes", np.uint32, 8 * bat.n_slots).astype(np.uint64).sum())
     log(f"batched plain {ms_bat:.2f} ms/scene ({'level by level' if args.no_graph else 'graph'}), "
         f"SAH {sah_plain:.4f}")
@@ -405,6 +423,9 @@ def scene_main(args):
         "scene_hits_identical": {"rays": w * h, "identical": same_hits(hits_plain, hits_learned),
                                  "hits": int((hits_plain["prim"] != N.MISS).sum())},
         "heightfield_16m_rays": full,
+        "scene_traversal": {"rays": w * h, "trees": "batched plain",
+                            "mrays_per_s": {k: w * h / (v * 1e-3) / 1e6 for k, v in trace_ms.items()},
+                            "ms": trace_ms, "shader_exact_same_triangle_frac": shader_same},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
                      "kernel": dominant, "kernel_ms_per_launch": dom_ms,

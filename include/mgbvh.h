@@ -307,6 +307,61 @@ int mgbvh_gen_bounce_rays_device(mgbvh_ctx* ctx, const mgbvh_ray* d_rays_in,
                                  const uint32_t* d_tris_perm, uint32_t seed,
                                  mgbvh_ray* d_rays_out);
 
+/* ---- compressed 8-wide BVH ----------------------------------------------------
+ * No counterpart in the reference's code: its README (README.md, "A BVH8
+ * implementation is planned, following Efficient Incoherent Ray Traversal on
+ * GPUs Through Compressed Wide BVHs (Ylitie et al.)") names it as the step
+ * after the binary builder, which keeps one object per leaf for it.  The
+ * collapse is that paper's dynamic program: C(v, i) = cost of v's subtree as
+ * at most i subtrees; a wide leaf of p <= p_max objects costs
+ * area * p * c_prim, a wide internal node area * c_node plus the best split
+ * of its 8 slots; first minimum on ties, a leaf on a leaf/internal tie; area
+ * = 2 * ((dx*dy + dy*dz) + dz*dx) of the binary node's box, all fp32. */
+typedef struct mgbvh_wide8_params {
+  float c_node;      /* default 1.0 */
+  float c_prim;      /* default 0.3 */
+  uint32_t p_max;    /* objects per wide leaf, 1..3 (default 3) */
+  uint32_t reserved;
+} mgbvh_wide8_params;
+
+/* 80-byte node.  Child k's box is origin + q * 2^(exponent - 127) per axis in
+ * fp32 (q = qlo or qhi), conservative for the binary node it came from.
+ * meta[k] = 0: empty slot; internal child (imask bit k): 0x20 | (24 + r), the
+ * child is wide node base_child + r; leaf child: (unary object count << 5) |
+ * offset, its objects are tri_index[base_tri + offset ...].  Layout: root at
+ * 0; a node's internal children contiguous, followed by their own subtrees in
+ * child order; a node's leaf objects contiguous, followed by its children's. */
+typedef struct mgbvh_wide_node {
+  float origin[3];
+  uint8_t exponent[3];
+  uint8_t imask;
+  uint32_t base_child;
+  uint32_t base_tri;
+  uint8_t meta[8];
+  uint8_t qlo[3][8];
+  uint8_t qhi[3][8];
+} mgbvh_wide_node;
+
+/* Collapses a binary tree (device nodes from any build here; d_lens optional,
+ * leaves of more than p_max objects are an error).  d_wide: capacity wide_cap
+ * (objects - 1 is always enough, at least 1); d_tri_index: capacity tri_cap
+ * (the object count), receives the binary leaves' start_index values in wide
+ * leaf order.  out_cost (optional): the root's collapse cost.  params NULL =
+ * defaults.  Synchronises the context stream. */
+int mgbvh_wide8_collapse_device(mgbvh_ctx* ctx, const mgbvh_node32* d_nodes,
+                                const uint32_t* d_lens, uint64_t nnodes,
+                                const mgbvh_wide8_params* params, mgbvh_wide_node* d_wide,
+                                uint64_t wide_cap, uint32_t* d_tri_index, uint64_t tri_cap,
+                                uint64_t* out_nwide, float* out_cost);
+
+/* Closest hit of each ray through a wide tree of one BLAS, with the hit rules
+ * of mgbvh_trace_closest_device (the same hits as the binary tree). */
+int mgbvh_trace_wide8_device(mgbvh_ctx* ctx, const mgbvh_wide_node* d_wide, uint64_t nwide,
+                             const uint32_t* d_tri_index, const float* d_positions,
+                             uint32_t vstride_bytes, const uint32_t* d_tris_perm,
+                             const uint32_t* d_perm, const mgbvh_ray* d_rays, uint64_t nrays,
+                             mgbvh_hit* d_hits);
+
 /* ---- batched BLAS build ------------------------------------------------------
  * Replaces the per-mesh BLAS loop of the scene upload (model.rs:76-95, one
  * BoundingVolumeHierarchy::new per mesh on rayon workers): every mesh of the

@@ -116,6 +116,14 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class Wide8Params(C.Structure):
+    _fields_ = [("c_node", C.c_float), ("c_prim", C.c_float), ("p_max", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+WIDE_NODE_BYTES = 80
+
+
 class MgbvhError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"mgbvh error {code}: {msg}")
@@ -179,6 +187,9 @@ _EXTRA: dict = {
                                   _P, _U64, _P], _I),
     "mgbvh_raytrace_tlas_device": ([_P, _P, _P, _P, _U32, _P, _U32, _P, _P, _U32, _P, _P, _P,
                                     _U64, _P], _I),
+    "mgbvh_wide8_collapse_device": ([_P, _P, _P, _U64, _P, _P, _U64, _P, _U64, C.POINTER(_U64),
+                                     C.POINTER(C.c_float)], _I),
+    "mgbvh_trace_wide8_device": ([_P, _P, _U64, _P, _P, _U32, _P, _P, _P, _U64, _P], _I),
     "mgbvh_gen_primary_rays_device": ([_P, C.POINTER(Camera), _U32, _U32, _P], _I),
     "mgbvh_copy_d2d": ([_P, _P, _P, _U64], _I),
     "mgbvh_instance_boxes_device": ([_P, _P, _P, _U32, _P, _P, _U32, _P], _I),
@@ -484,6 +495,23 @@ class Context:
             self.handle, _dv(tlas_nodes), _dv(tlas_perm), _dv(instances), ninstances, _dv(blas),
             nblas, _dv(blas_nodes), _dv(pos), vstride, _dv(tris_perm), _dv(blas_perm), _dv(rays),
             nrays, _dv(hits)))
+
+    def wide8_collapse(self, nodes, lens, nnodes: int, wide, wide_cap: int, tri_index,
+                       tri_cap: int, c_node: float = 1.0, c_prim: float = 0.3, p_max: int = 3):
+        """Compressed 8-wide tree from device binary nodes.  Returns (nwide, cost)."""
+        prm = Wide8Params(c_node, c_prim, p_max, 0)
+        nw, cost = C.c_uint64(), C.c_float()
+        check(lib().mgbvh_wide8_collapse_device(self.handle, _dv(nodes), _dv(lens), nnodes,
+                                                C.byref(prm), _dv(wide), wide_cap,
+                                                _dv(tri_index), tri_cap, C.byref(nw),
+                                                C.byref(cost)))
+        return nw.value, cost.value
+
+    def trace_wide8(self, wide, nwide: int, tri_index, pos, vstride, tris_perm, perm, rays,
+                    nrays, hits):
+        check(lib().mgbvh_trace_wide8_device(self.handle, _dv(wide), nwide, _dv(tri_index),
+                                             _dv(pos), vstride, _dv(tris_perm), _dv(perm),
+                                             _dv(rays), nrays, _dv(hits)))
 
     def gen_primary(self, cam: "Camera", width: int, height: int, rays):
         check(lib().mgbvh_gen_primary_rays_device(self.handle, C.byref(cam), width, height,

@@ -7,16 +7,17 @@
 // rejected); the closest t wins and equal t goes to the lowest original
 // (instance, triangle) index.  The reference's Hit.merge
 // (intersect.slang:57-82) keeps whichever equal hit was tested later, which
-// depends on the tree.  The slab test never culls a box that can hold the
-// closest hit: far distances are widened by 1 + 2^-20, near distances compared
-// against the closest t widened by 1 + 2^-10, and an axis whose plane distance
-// is 0 * inf (a ray inside a face plane) left unbounded.
+// depends on the tree.  The slab test keeps a box that holds the closest hit
+// even when Moller-Trumbore's rounding puts a grazing hit slightly outside the
+// triangle's box: far distances are widened by 1 + 2^-10, near distances
+// compared against the closest t widened by 1 + 2^-10, and an axis whose plane
+// distance is 0 * inf (a ray inside a face plane) left unbounded.
 #include "mgbvh_common.h"
 #include "mgbvh_internal.h"
 
 namespace mg {
 
-constexpr float TF_WIDEN = 1.00000095367431640625f;  // 1 + 2^-20
+constexpr float TF_WIDEN = 1.0009765625f;  // 1 + 2^-10
 constexpr float TN_SLACK = 1.0009765625f;            // 1 + 2^-10
 constexpr int TRACE_STACK = 64;                      // > tree height + 1
 constexpr uint32_t NONE = 0xffffffffu;

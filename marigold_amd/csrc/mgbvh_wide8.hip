@@ -1,0 +1,616 @@
+// Compressed 8-wide BVH from the binary tree, and closest-hit traversal of it.
+//
+// The wide format and its collapse are those of Ylitie, Karras and Laine,
+// "Efficient Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs"
+// (HPG 2017), which marigold's README names as the step after the binary
+// builder (the binary tree keeps one object per leaf for it).  The collapse
+// is the paper's dynamic program over the binary nodes: C(v, i) is the cost of
+// v's subtree as at most i subtrees (i = 1..7); a wide leaf of p <= p_max
+// objects costs area * p * c_prim, a wide internal node area * c_node plus the
+// best split of 8 slots between the children.  Minima keep the first k on
+// ties, a leaf wins a leaf/internal tie, and every cost is fp32 in one fixed
+// order, so the tree is a pure function of the binary tree.
+//
+// GPU shape: one thread per binary leaf walks to the root; at each node the
+// second child to arrive (an atomic counter per node) computes the node, the
+// first stops.  The layout is then written one wide level per launch from the
+// subtree sizes (W wide nodes, T objects) the walk recorded: a node's internal
+// children sit contiguously at base_child, followed by the children's own
+// subtrees in child order, so every index follows from the sizes and the
+// result does not depend on thread order.
+#include "mgbvh_common.h"
+#include "mgbvh_internal.h"
+
+namespace mg {
+
+constexpr uint32_t W8_NONE = 0xffffffffu;
+constexpr int W8_STACK = 256;  // traversal: at most 7 more entries per wide level
+
+struct W8Rec {   // per binary node
+  float c[7];    // C(v, i), i = 1..7
+  uint8_t dec[8];  // [0]: 1 = leaf at one slot; [i-1], i = 2..7: 0 reduce, k split; [7]: k of 8
+  uint32_t p, w, t;  // objects; wide nodes and objects of the wide subtree rooted here
+};
+static_assert(sizeof(W8Rec) == 48, "W8Rec is 48 bytes");
+
+struct W8Item {  // a wide internal node to write
+  uint32_t bin, idx, region, tri;
+};
+
+__device__ inline float w8_area(const float* mn, const float* mx) {
+  const float dx = __fsub_rn(mx[0], mn[0]), dy = __fsub_rn(mx[1], mn[1]);
+  const float dz = __fsub_rn(mx[2], mn[2]);
+  float a = __fmul_rn(dx, dy);
+  a = __fadd_rn(a, __fmul_rn(dy, dz));
+  a = __fadd_rn(a, __fmul_rn(dz, dx));
+  return __fmul_rn(2.0f, a);
+}
+
+__device__ inline void w8_box(const mgbvh_node32* nodes, uint32_t v, float* mn, float* mx,
+                              uint32_t& start, uint32_t& child) {
+  const uint4* q = reinterpret_cast<const uint4*>(&nodes[v]);
+  const uint4 a = q[0], b = q[1];
+  mn[0] = __uint_as_float(a.x); mn[1] = __uint_as_float(a.y); mn[2] = __uint_as_float(a.z);
+  start = a.w;
+  mx[0] = __uint_as_float(b.x); mx[1] = __uint_as_float(b.y); mx[2] = __uint_as_float(b.z);
+  child = b.w;
+}
+
+__global__ void k_w8_prep(uint32_t nn, uint32_t* __restrict__ parent, uint32_t* __restrict__ cnt) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  cnt[v] = 0;
+  parent[v] = W8_NONE;
+}
+
+// Separate from k_w8_prep: a node's parent entry is written after every entry
+// was reset.  Children out of range are left unlinked (and flagged).
+__global__ void k_w8_link(const mgbvh_node32* __restrict__ nodes, uint32_t nn,
+                          uint32_t* __restrict__ parent, uint32_t* err_flag) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  const uint32_t ch = nodes[v].child_node;
+  if (ch == 0) return;
+  if (ch <= v || ch + 1 >= nn) { atomicOr(err_flag, 16u); return; }
+  parent[ch] = v;
+  parent[ch + 1] = v;
+}
+
+// Slots of budget j at binary node m, left child first (an explicit stack).
+__device__ inline int w8_expand(const mgbvh_node32* nodes, uint32_t nn, const W8Rec* rec,
+                                uint32_t m, uint32_t j, uint32_t* out, int n) {
+  uint32_t sm[16], sj[16];
+  int sp = 0;
+  sm[sp] = m; sj[sp] = j; ++sp;
+  while (sp > 0) {
+    --sp;
+    uint32_t u = sm[sp], b = sj[sp];
+    const uint32_t ch = nodes[u].child_node;
+    if (ch != 0 && ch + 1 >= nn) return 99;
+    while (b > 1 && ch != 0 && rec[u].dec[b - 1] == 0) --b;
+    if (b == 1 || ch == 0) {
+      if (n < 8) out[n] = u;
+      ++n;
+      continue;
+    }
+    const uint32_t k = rec[u].dec[b - 1];
+    if (sp + 2 > 16) return 99;  // cannot happen: a budget of 8 splits at most 7 times
+    sm[sp] = ch + 1; sj[sp] = b - k; ++sp;  // right pushed first: left expands first
+    sm[sp] = ch; sj[sp] = k; ++sp;
+  }
+  return n;
+}
+
+// Slots of a wide internal node built from v.
+__device__ inline int w8_slots(const mgbvh_node32* nodes, uint32_t nn, const W8Rec* rec,
+                               uint32_t v, uint32_t* out) {
+  const uint32_t ch = nodes[v].child_node;
+  if (ch == 0) { out[0] = v; return 1; }
+  if (ch + 1 >= nn) return 99;
+  const uint32_t k = rec[v].dec[7];
+  if (k < 1 || k > 7) return 99;
+  const int n = w8_expand(nodes, nn, rec, ch, k, out, 0);
+  return n > 8 ? n : w8_expand(nodes, nn, rec, ch + 1, 8 - k, out, n);
+}
+
+__device__ inline bool w8_leaf_slot(const mgbvh_node32* nodes, const W8Rec* rec, uint32_t m) {
+  return nodes[m].child_node == 0 || rec[m].dec[0] == 1;
+}
+
+__device__ void w8_node_dp(const mgbvh_node32* nodes, uint32_t nn, W8Rec* rec, uint32_t v,
+                           float c_node,
+                           float c_prim, uint32_t p_max, uint32_t& err) {
+  float mn[3], mx[3];
+  uint32_t start, L;
+  w8_box(nodes, v, mn, mx, start, L);
+  const float area = w8_area(mn, mx);
+  const W8Rec a = rec[L], b = rec[L + 1];
+  W8Rec r;
+  r.p = a.p + b.p;
+  float dist[9];
+  uint32_t arg[9];
+#pragma unroll
+  for (uint32_t i = 2; i <= 8; ++i) {
+    float best = __uint_as_float(0x7f800000u);
+    uint32_t bk = 1;
+#pragma unroll
+    for (uint32_t k = 1; k < i; ++k) {
+      if (k > 7 || i - k > 7) continue;
+      const float s = __fadd_rn(a.c[k - 1], b.c[i - k - 1]);
+      if (s < best) { best = s; bk = k; }
+    }
+    dist[i] = best;
+    arg[i] = bk;
+  }
+  const float cint = __fadd_rn(__fmul_rn(area, c_node), dist[8]);
+  const float cleaf = r.p <= p_max ? __fmul_rn(__fmul_rn(area, (float)r.p), c_prim)
+                                   : __uint_as_float(0x7f800000u);
+  const bool leaf = cleaf <= cint;
+  r.dec[0] = leaf ? 1 : 0;
+  r.c[0] = leaf ? cleaf : cint;
+#pragma unroll
+  for (uint32_t i = 2; i <= 7; ++i) {
+    const bool split = dist[i] < r.c[i - 2];
+    r.c[i - 1] = split ? dist[i] : r.c[i - 2];
+    r.dec[i - 1] = split ? (uint8_t)arg[i] : 0;
+  }
+  r.dec[7] = (uint8_t)arg[8];
+  r.w = 0;
+  r.t = 0;
+  rec[v] = r;  // the decisions first: the slot expansion below reads them
+  uint32_t s[8];
+  const int ns = w8_slots(nodes, nn, rec, v, s);
+  if (ns > 8) { err |= 4u; return; }
+  uint32_t w = 1, t = 0;
+  for (int k = 0; k < ns; ++k) {
+    const W8Rec& q = rec[s[k]];
+    if (w8_leaf_slot(nodes, rec, s[k])) t += q.p;
+    else { w += q.w; t += q.t; }
+  }
+  rec[v].w = w;
+  rec[v].t = t;
+}
+
+__global__ __launch_bounds__(256) void k_w8_dp(const mgbvh_node32* __restrict__ nodes,
+                                               const uint32_t* __restrict__ lens, uint32_t nn,
+                                               const uint32_t* __restrict__ parent,
+                                               uint32_t* __restrict__ cnt, W8Rec* rec,
+                                               float c_node, float c_prim, uint32_t p_max,
+                                               uint32_t* err_flag) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  float mn[3], mx[3];
+  uint32_t start, ch;
+  w8_box(nodes, v, mn, mx, start, ch);
+  if (ch != 0) return;
+  uint32_t err = 0;
+  W8Rec r;
+  r.p = lens ? lens[v] : 1u;
+  if (r.p > p_max) err |= 8u;
+  const float cl = __fmul_rn(__fmul_rn(w8_area(mn, mx), (float)r.p), c_prim);
+#pragma unroll
+  for (int i = 0; i < 7; ++i) r.c[i] = cl;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) r.dec[i] = 0;
+  r.dec[0] = 1;
+  r.w = 0;
+  r.t = r.p;
+  rec[v] = r;
+  for (int guard = 0; guard < 4096; ++guard) {
+    __threadfence();
+    const uint32_t u = parent[v];
+    if (u >= nn) break;  // the root (W8_NONE)
+    if (atomicAdd(&cnt[u], 1u) == 0) break;  // the sibling's subtree is not done yet
+    __threadfence();
+    w8_node_dp(nodes, nn, rec, u, c_node, c_prim, p_max, err);
+    v = u;
+  }
+  if (err) atomicOr(err_flag, err);
+}
+
+// Largest q with p + q * s <= x (from the double floor), smallest with >= x.
+__device__ inline uint32_t w8_qlo(float p, float s, float x) {
+  const double q = floor(((double)x - (double)p) / (double)s);
+  uint32_t qi = q < 0.0 ? 0u : q > 255.0 ? 255u : (uint32_t)q;
+  while (qi > 0 && __fadd_rn(p, __fmul_rn((float)qi, s)) > x) --qi;
+  return qi;
+}
+__device__ inline uint32_t w8_qhi(float p, float s, float x) {
+  const double q = ceil(((double)x - (double)p) / (double)s);
+  uint32_t qi = q < 0.0 ? 0u : q > 255.0 ? 255u : (uint32_t)q;
+  while (qi < 255 && __fadd_rn(p, __fmul_rn((float)qi, s)) < x) ++qi;
+  return qi;
+}
+__device__ inline uint32_t w8_exponent(float lo, float hi) {
+  const double ext = (double)hi - (double)lo;
+  int ex;
+  frexp(ext, &ex);
+  int e = ex - 8;
+  if (e < -126) e = -126;
+  while (e < 127 && ldexp(255.0, e) < ext) ++e;
+  while (e > -126 && ldexp(255.0, e - 1) >= ext) --e;
+  return (uint32_t)(e + 127);
+}
+
+__global__ __launch_bounds__(256) void k_w8_emit(
+    const mgbvh_node32* __restrict__ nodes, const uint32_t* __restrict__ lens,
+    const W8Rec* __restrict__ rec, const W8Item* __restrict__ in, const uint32_t* n_in,
+    W8Item* __restrict__ out, uint32_t* n_out, uint32_t front_cap, uint8_t* __restrict__ wide,
+    uint32_t* __restrict__ tri_index, uint32_t nn, uint32_t wide_cap, uint32_t tri_cap,
+    uint32_t* err_flag) {
+  const uint32_t count = *n_in;
+  for (uint32_t it = blockIdx.x * blockDim.x + threadIdx.x; it < count;
+       it += gridDim.x * blockDim.x) {
+    const W8Item item = in[it];
+    const uint32_t v = item.bin;
+    uint32_t s[8];
+    if (v >= nn) { atomicOr(err_flag, 4u); continue; }
+    const int ns = w8_slots(nodes, nn, rec, v, s);
+    if (ns > 8 || item.idx >= wide_cap) { atomicOr(err_flag, 4u); continue; }
+    float pmn[3], pmx[3];
+    uint32_t st, ch;
+    w8_box(nodes, v, pmn, pmx, st, ch);
+    uint32_t words[20];
+    float sc[3];
+    uint32_t ebytes = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const uint32_t E = w8_exponent(pmn[a], pmx[a]);
+      sc[a] = __uint_as_float(E << 23);
+      ebytes |= E << (8 * a);
+      words[a] = __float_as_uint(pmn[a]);
+    }
+    uint32_t nint = 0, own = 0;
+    for (int k = 0; k < ns; ++k) {
+      if (w8_leaf_slot(nodes, rec, s[k])) own += rec[s[k]].p;
+      else ++nint;
+    }
+    words[4] = item.region;
+    words[5] = item.tri;
+    uint8_t meta[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint8_t q[6][8];
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) q[a][k] = 0;
+    uint32_t ci = 0, toff = 0, sub = item.region + nint, tsub = item.tri + own, imask = 0;
+    for (int k = 0; k < ns; ++k) {
+      const uint32_t m = s[k];
+      float cmn[3], cmx[3];
+      uint32_t cst, cch;
+      w8_box(nodes, m, cmn, cmx, cst, cch);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        q[a][k] = (uint8_t)w8_qlo(pmn[a], sc[a], cmn[a]);
+        q[3 + a][k] = (uint8_t)w8_qhi(pmn[a], sc[a], cmx[a]);
+      }
+      if (w8_leaf_slot(nodes, rec, m)) {
+        const uint32_t np = rec[m].p;
+        meta[k] = (uint8_t)((((1u << np) - 1u) << 5) | toff);
+        uint32_t stk[8];
+        int sp = 0;
+        stk[sp++] = m;
+        while (sp > 0) {  // the leaf's objects, child before child + 1
+          const uint32_t u = stk[--sp];
+          const uint32_t uc = nodes[u].child_node;
+          if (uc == 0) {
+            const uint32_t len = lens ? lens[u] : 1u;
+            const uint32_t us = nodes[u].start_index;
+            for (uint32_t j = 0; j < len; ++j) {
+              const uint32_t at = item.tri + toff++;
+              if (at < tri_cap) tri_index[at] = us + j;
+              else atomicOr(err_flag, 4u);
+            }
+          } else if (sp + 2 <= 8 && uc + 1 < nn) {
+            stk[sp++] = uc + 1;
+            stk[sp++] = uc;
+          } else {
+            atomicOr(err_flag, 4u);
+          }
+        }
+      } else {
+        imask |= 1u << k;
+        meta[k] = (uint8_t)((1u << 5) | (24 + ci));
+        const uint32_t slot = atomicAdd(n_out, 1u);
+        if (slot < front_cap) out[slot] = W8Item{m, item.region + ci, sub, tsub};
+        else atomicOr(err_flag, 4u);
+        sub += rec[m].w - 1;
+        tsub += rec[m].t;
+        ++ci;
+      }
+    }
+    words[3] = ebytes | (imask << 24);
+    words[6] = meta[0] | (meta[1] << 8) | (meta[2] << 16) | ((uint32_t)meta[3] << 24);
+    words[7] = meta[4] | (meta[5] << 8) | (meta[6] << 16) | ((uint32_t)meta[7] << 24);
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+      words[8 + 2 * a] = q[a][0] | (q[a][1] << 8) | (q[a][2] << 16) | ((uint32_t)q[a][3] << 24);
+      words[9 + 2 * a] = q[a][4] | (q[a][5] << 8) | (q[a][6] << 16) | ((uint32_t)q[a][7] << 24);
+    }
+    uint4* dst = reinterpret_cast<uint4*>(wide + 80 * (size_t)item.idx);
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      dst[k] = make_uint4(words[4 * k], words[4 * k + 1], words[4 * k + 2], words[4 * k + 3]);
+  }
+}
+
+// ------------------------------------------------------------------ traversal
+__device__ inline bool w8_slab(const float* o, const float* inv, const float* bmn,
+                               const float* bmx, float best, float& tn_out) {
+  float tn = -__uint_as_float(0x7f800000u), tf = __uint_as_float(0x7f800000u);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float t0 = __fmul_rn(__fsub_rn(bmn[a], o[a]), inv[a]);
+    const float t1 = __fmul_rn(__fsub_rn(bmx[a], o[a]), inv[a]);
+    const bool free_axis = (t0 != t0) | (t1 != t1);
+    tn = fmaxf(tn, free_axis ? -__uint_as_float(0x7f800000u) : fminf(t0, t1));
+    tf = fminf(tf, free_axis ? __uint_as_float(0x7f800000u) : fmaxf(t0, t1));
+  }
+  tf = __fmul_rn(tf, 1.0009765625f);  // the binary traversal's margins
+  tn_out = tn;
+  return tn <= tf && tf >= 0.0f && tn <= __fmul_rn(best, 1.0009765625f);
+}
+
+__device__ inline float w8_dot(const float* a, const float* b) {
+  return __fadd_rn(__fadd_rn(__fmul_rn(a[0], b[0]), __fmul_rn(a[1], b[1])), __fmul_rn(a[2], b[2]));
+}
+
+__device__ inline float w8_tri(const float* o, const float* d, const float* v1, const float* v2,
+                               const float* v3) {
+  float e1[3], e2[3], h[3], s[3], q[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) { e1[a] = __fsub_rn(v2[a], v1[a]); e2[a] = __fsub_rn(v3[a], v1[a]); }
+  h[0] = __fsub_rn(__fmul_rn(d[1], e2[2]), __fmul_rn(d[2], e2[1]));
+  h[1] = __fsub_rn(__fmul_rn(d[2], e2[0]), __fmul_rn(d[0], e2[2]));
+  h[2] = __fsub_rn(__fmul_rn(d[0], e2[1]), __fmul_rn(d[1], e2[0]));
+  const float det = w8_dot(h, e1);
+  const float inf = __uint_as_float(0x7f800000u);
+  if (fabsf(det) < 1e-6f) return inf;
+  const float f = __fdiv_rn(1.0f, det);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) s[a] = __fsub_rn(o[a], v1[a]);
+  const float u = __fmul_rn(f, w8_dot(s, h));
+  if (u < 0.0f || u > 1.0f) return inf;
+  q[0] = __fsub_rn(__fmul_rn(s[1], e1[2]), __fmul_rn(s[2], e1[1]));
+  q[1] = __fsub_rn(__fmul_rn(s[2], e1[0]), __fmul_rn(s[0], e1[2]));
+  q[2] = __fsub_rn(__fmul_rn(s[0], e1[1]), __fmul_rn(s[1], e1[0]));
+  const float vv = __fmul_rn(f, w8_dot(d, q));
+  if (vv < 0.0f || __fadd_rn(u, vv) > 1.0f) return inf;
+  const float t = __fmul_rn(f, w8_dot(e2, q));
+  return t < 1e-6f ? inf : t;
+}
+
+__global__ __launch_bounds__(256) void k_trace_w8(
+    const uint8_t* __restrict__ wide, const uint32_t* __restrict__ tri_index,
+    const float* __restrict__ pos, uint32_t vs, const uint32_t* __restrict__ tris,
+    const uint32_t* __restrict__ perm, const mgbvh_ray* __restrict__ rays, uint32_t nrays,
+    uint32_t nwide, mgbvh_hit* __restrict__ hits, uint32_t* err_flag) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrays) return;
+  const float4* rp = reinterpret_cast<const float4*>(&rays[i]);
+  const float4 ra = rp[0], rb = rp[1];
+  const float o[3] = {ra.x, ra.y, ra.z}, d[3] = {rb.x, rb.y, rb.z};
+  float inv[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) inv[a] = __fdiv_rn(1.0f, d[a]);
+  float best = rb.w;
+  uint32_t bp = W8_NONE, bslot = W8_NONE, err = 0;
+  uint32_t stk[W8_STACK];
+  float stn[W8_STACK];
+  int sp = 0;
+  stk[sp] = 0; stn[sp] = -__uint_as_float(0x7f800000u); ++sp;
+  while (sp > 0) {
+    --sp;
+    if (!(stn[sp] <= __fmul_rn(best, 1.0009765625f))) continue;
+    const uint32_t w = stk[sp];
+    if (w >= nwide) { err |= 2u; continue; }
+    const uint4* np = reinterpret_cast<const uint4*>(wide + 80 * (size_t)w);
+    uint32_t words[20];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const uint4 x = np[k];
+      words[4 * k] = x.x; words[4 * k + 1] = x.y; words[4 * k + 2] = x.z; words[4 * k + 3] = x.w;
+    }
+    const float p[3] = {__uint_as_float(words[0]), __uint_as_float(words[1]),
+                        __uint_as_float(words[2])};
+    const float sc[3] = {__uint_as_float((words[3] & 0xffu) << 23),
+                         __uint_as_float(((words[3] >> 8) & 0xffu) << 23),
+                         __uint_as_float(((words[3] >> 16) & 0xffu) << 23)};
+    const uint32_t imask = words[3] >> 24, bc = words[4], bt = words[5];
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t meta = (words[6 + (k >> 2)] >> (8 * (k & 3))) & 0xffu;
+      if (meta == 0) continue;
+      float mn[3], mx[3], tn;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const uint32_t ql = (words[8 + 2 * a + (k >> 2)] >> (8 * (k & 3))) & 0xffu;
+        const uint32_t qh = (words[14 + 2 * a + (k >> 2)] >> (8 * (k & 3))) & 0xffu;
+        mn[a] = __fadd_rn(p[a], __fmul_rn((float)ql, sc[a]));
+        mx[a] = __fadd_rn(p[a], __fmul_rn((float)qh, sc[a]));
+      }
+      if (!w8_slab(o, inv, mn, mx, best, tn)) continue;
+      if ((imask >> k) & 1u) {
+        if (sp < W8_STACK) { stk[sp] = bc + (meta & 31u) - 24u; stn[sp] = tn; ++sp; }
+        else err |= 1u;
+      } else {
+        const uint32_t cnt = __popc(meta >> 5), off = meta & 31u;
+        for (uint32_t j = 0; j < cnt; ++j) {
+          const uint32_t slot = tri_index[bt + off + j];
+          const uint32_t* tr = tris + 3 * (size_t)slot;
+          const float t = w8_tri(o, d, pos + (size_t)tr[0] * vs, pos + (size_t)tr[1] * vs,
+                                 pos + (size_t)tr[2] * vs);
+          const uint32_t id = perm[slot];
+          if (t < best || (t == best && t != __uint_as_float(0x7f800000u) && id < bp)) {
+            best = t; bp = id; bslot = slot;
+          }
+        }
+      }
+    }
+  }
+  reinterpret_cast<uint4*>(hits)[i] = make_uint4(__float_as_uint(best), bp,
+                                                 bp == W8_NONE ? W8_NONE : 0u, bslot);
+  if (err) atomicOr(err_flag, err);
+}
+
+static float host_area(const mgbvh_node32& n) {
+  const float dx = n.max[0] - n.min[0], dy = n.max[1] - n.min[1], dz = n.max[2] - n.min[2];
+  float a = dx * dy;
+  a = a + dy * dz;
+  a = a + dz * dx;
+  return 2.0f * a;
+}
+
+}  // namespace mg
+
+using namespace mg;
+
+extern "C" {
+
+int mgbvh_wide8_collapse_device(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint32_t* d_lens,
+                                uint64_t nnodes, const mgbvh_wide8_params* params,
+                                mgbvh_wide_node* d_wide, uint64_t wide_cap,
+                                uint32_t* d_tri_index, uint64_t tri_cap, uint64_t* out_nwide,
+                                float* out_cost) {
+  if (!c) { set_error("ctx is NULL"); return MGBVH_ERR_INVALID_ARG; }
+  if (out_nwide) *out_nwide = 0;
+  if (nnodes == 0) return MGBVH_OK;
+  if (!d_nodes || !d_wide || !d_tri_index) {
+    set_error("null device buffer");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  if (nnodes >= 0x7fffffffull) { set_error("more than 2^31-1 nodes"); return MGBVH_ERR_INVALID_ARG; }
+  const mgbvh_wide8_params def = {1.0f, 0.3f, 3u, 0u};
+  const mgbvh_wide8_params& P = params ? *params : def;
+  if (P.p_max < 1 || P.p_max > 3 || !(P.c_node >= 0.0f) || !(P.c_prim >= 0.0f)) {
+    set_error("wide8 parameters: p_max in 1..3 (a leaf's count is 3 unary bits), costs >= 0");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  const uint32_t nn = (uint32_t)nnodes;
+  Scratch& sc = c->sc;
+  int rc;
+  if ((rc = ensure(sc.w8rec, sizeof(W8Rec) * (size_t)nn))) return rc;
+  if ((rc = ensure(sc.w8par, 8 * (size_t)nn))) return rc;  // parents, then counters
+  const uint32_t front_cap = nn / 2 + 1;  // wide internal nodes <= binary internal nodes
+  const size_t front = sizeof(W8Item) * (size_t)front_cap;
+  if ((rc = ensure(sc.w8front[0], front)) || (rc = ensure(sc.w8front[1], front))) return rc;
+  if ((rc = ensure(sc.w8ctr, 16))) return rc;
+  W8Rec* rec = reinterpret_cast<W8Rec*>(sc.w8rec.p);
+  uint32_t* parent = reinterpret_cast<uint32_t*>(sc.w8par.p);
+  uint32_t* cnt = parent + nn;
+  uint32_t* ctr = reinterpret_cast<uint32_t*>(sc.w8ctr.p);  // [0] err, [1..2] frontier counts
+  hipStream_t st = c->stream;
+  const uint32_t grid = (nn + 255) / 256;
+  if ((e = hipMemsetAsync(ctr, 0, 16, st)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+  k_w8_prep<<<grid, 256, 0, st>>>(nn, parent, cnt);
+  k_w8_link<<<grid, 256, 0, st>>>(d_nodes, nn, parent, ctr);
+  k_w8_dp<<<grid, 256, 0, st>>>(d_nodes, d_lens, nn, parent, cnt, rec, P.c_node, P.c_prim,
+                                 P.p_max, ctr);
+  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "k_w8_dp");
+  // the root's record, its children's and the first node: sizes, cost, errors
+  mgbvh_node32 root;
+  W8Rec r0{}, rl{}, rr{};
+  uint32_t err = 0;
+  if ((e = hipMemcpyAsync(&root, d_nodes, sizeof root, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(&r0, rec, sizeof r0, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(&err, ctr, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipStreamSynchronize(st)) != hipSuccess)
+    return hip_fail(e, "wide8 root readback");
+  if (err & 16u) { set_error("a child index is out of range or not after its parent"); return MGBVH_ERR_INDEX; }
+  if (err & 8u) { set_error("a binary leaf holds more than p_max objects"); return MGBVH_ERR_INVALID_ARG; }
+  if (err) { set_error("wide8 collapse: slot expansion out of range"); return MGBVH_ERR_INTERNAL; }
+  const uint32_t L = root.child_node;
+  if (L != 0) {
+    if (L + 1 >= nn) { set_error("root child index out of range"); return MGBVH_ERR_INDEX; }
+    if ((e = hipMemcpy(&rl, rec + L, sizeof rl, hipMemcpyDeviceToHost)) != hipSuccess ||
+        (e = hipMemcpy(&rr, rec + L + 1, sizeof rr, hipMemcpyDeviceToHost)) != hipSuccess)
+      return hip_fail(e, "wide8 readback");
+  }
+  const uint64_t nwide = L == 0 ? 1 : r0.w;
+  const uint64_t nobj = r0.t;
+  if (nwide > wide_cap || nobj > tri_cap) {
+    set_error("wide8 output capacity too small (nodes " + std::to_string(nwide) + ", objects " +
+              std::to_string(nobj) + ")");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  // one launch per wide level, the frontier count read back after each
+  W8Item first{0u, 0u, 1u, 0u};
+  W8Item* fr[2] = {reinterpret_cast<W8Item*>(sc.w8front[0].p),
+                   reinterpret_cast<W8Item*>(sc.w8front[1].p)};
+  uint32_t one = 1;
+  if ((e = hipMemcpyAsync(fr[0], &first, sizeof first, hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(ctr + 1, &one, 4, hipMemcpyHostToDevice, st)) != hipSuccess)
+    return hip_fail(e, "wide8 frontier");
+  uint32_t n_in = 1;
+  const uint32_t egrid = std::max<uint32_t>(1, std::min<uint32_t>((nn / 2 + 255) / 256, 4096));
+  for (int level = 0; n_in > 0 && level < 256; ++level) {
+    uint32_t* cin = ctr + 1 + (level & 1);
+    uint32_t* cout = ctr + 1 + ((level + 1) & 1);
+    if ((e = hipMemsetAsync(cout, 0, 4, st)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+    k_w8_emit<<<std::min(egrid, (n_in + 255) / 256), 256, 0, st>>>(
+        d_nodes, d_lens, rec, fr[level & 1], cin, fr[(level + 1) & 1], cout, front_cap,
+        reinterpret_cast<uint8_t*>(d_wide), d_tri_index, nn,
+        (uint32_t)std::min<uint64_t>(wide_cap, 0xffffffffu),
+        (uint32_t)std::min<uint64_t>(tri_cap, 0xffffffffu), ctr);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "k_w8_emit");
+    if ((e = hipMemcpyAsync(&n_in, cout, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess)
+      return hip_fail(e, "wide8 level");
+  }
+  if ((e = hipMemcpy(&err, ctr, 4, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "err");
+  if (err || n_in != 0) { set_error("wide8 collapse: layout out of range"); return MGBVH_ERR_INTERNAL; }
+  if (out_nwide) *out_nwide = nwide;
+  if (out_cost) {
+    if (L == 0) *out_cost = r0.c[0];
+    else {
+      const uint32_t k = r0.dec[7];
+      *out_cost = host_area(root) * P.c_node + (rl.c[k - 1] + rr.c[8 - k - 1]);
+    }
+  }
+  return MGBVH_OK;
+}
+
+int mgbvh_trace_wide8_device(mgbvh_ctx* c, const mgbvh_wide_node* d_wide, uint64_t nwide,
+                             const uint32_t* d_tri_index, const float* d_pos,
+                             uint32_t vstride_bytes, const uint32_t* d_tris_perm,
+                             const uint32_t* d_perm, const mgbvh_ray* d_rays, uint64_t nrays,
+                             mgbvh_hit* d_hits) {
+  if (!c) { set_error("ctx is NULL"); return MGBVH_ERR_INVALID_ARG; }
+  if (nrays == 0) return MGBVH_OK;
+  if (!d_wide || !d_tri_index || !d_pos || !d_tris_perm || !d_perm || !d_rays || !d_hits ||
+      nwide == 0 || nwide > 0xffffffffull) {
+    set_error("null device buffer or no wide nodes");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  if (vstride_bytes < 12 || vstride_bytes % 4 || nrays > 0xffffffffull) {
+    set_error("bad vstride_bytes or more than 2^32-1 rays");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  int rc = ensure(c->sc.w8ctr, 16);
+  if (rc) return rc;
+  uint32_t* flag = reinterpret_cast<uint32_t*>(c->sc.w8ctr.p);
+  if ((e = hipMemsetAsync(flag, 0, 4, c->stream)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+  if (c->prof_mask & (1u << KK_TRACE)) {
+    if ((rc = prof_begin(c, KK_TRACE))) return rc;
+  }
+  k_trace_w8<<<(uint32_t)((nrays + 255) / 256), 256, 0, c->stream>>>(
+      reinterpret_cast<const uint8_t*>(d_wide), d_tri_index, d_pos, vstride_bytes / 4,
+      d_tris_perm, d_perm, d_rays, (uint32_t)nrays, (uint32_t)nwide, d_hits, flag);
+  if (c->prof_mask & (1u << KK_TRACE)) {
+    if ((rc = prof_end(c, KK_TRACE))) return rc;
+  }
+  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "k_trace_w8");
+  uint32_t err = 0;
+  if ((e = hipMemcpyAsync(&err, flag, 4, hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+    return hip_fail(e, "hipStreamSynchronize");
+  if ((rc = prof_collect(c))) return rc;
+  if (err & 2u) { set_error("a wide child index is past nwide"); return MGBVH_ERR_INDEX; }
+  if (err) { set_error("wide traversal stack overflow"); return MGBVH_ERR_INTERNAL; }
+  return MGBVH_OK;
+}
+
+}  // extern "C"

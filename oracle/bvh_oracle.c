@@ -617,10 +617,10 @@ void or_closest_hit_brute(const float* pos, uint32_t vs, const uint32_t* tris, u
 
 /* Slab test of the traversal (not the reference's box test, which may cull a
  * box holding the closest hit by rounding): the far distance is widened by
- * (1 + 2^-20), an axis whose plane distance is 0 * inf (the ray runs inside a
+ * (1 + 2^-10), an axis whose plane distance is 0 * inf (the ray runs inside a
  * face plane) does not bound t, and the near distance must not exceed the
  * current closest t widened by (1 + 2^-10). */
-#define OR_TF_WIDEN 1.00000095367431640625f /* 1 + 2^-20 */
+#define OR_TF_WIDEN 1.0009765625f /* 1 + 2^-10 */
 #define OR_TN_SLACK 1.0009765625f           /* 1 + 2^-10 */
 
 
@@ -867,4 +867,268 @@ int or_raytrace_tlas(const uint32_t* tlas8, const uint32_t* tlas_perm, const uin
     out[3] = !best.succ ? SH_NONE : best.prim == SH_NONE ? 0xfffffffeu : best.slot;
   }
   return err;
+}
+
+/* ------------------------------------------- compressed 8-wide BVH (collapse) */
+
+/* Surface area of a node box for the collapse cost: 2 * ((dx*dy + dy*dz) + dz*dx). */
+static float w8_area(const float* mn, const float* mx) {
+  float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+  float a = dx * dy;
+  a = a + dy * dz;
+  a = a + dz * dx;
+  return 2.0f * a;
+}
+
+typedef struct {
+  float c[7];       /* C(v, i), i = 1..7 */
+  uint8_t dec[8];   /* [0]: 1 = leaf at one slot; [i-1] (i = 2..7): 0 reduce, k split; [7]: k of dist(8) */
+  uint32_t p, w, t; /* objects, wide internal nodes and triangles of v's wide subtree */
+} w8_rec;
+
+typedef struct {
+  const uint32_t* n8;
+  const uint32_t* lens;
+  w8_rec* r;
+  float c_node, c_prim;
+  uint32_t p_max;
+} w8_ctx;
+
+static inline uint32_t w8_child(const w8_ctx* c, uint32_t v) { return c->n8[8 * (size_t)v + 7]; }
+
+/* The slots of a budget j at binary node m, in order (left child first). */
+static int w8_expand(const w8_ctx* c, uint32_t m, uint32_t j, uint32_t* out, int n) {
+  while (j > 1 && w8_child(c, m) != 0 && c->r[m].dec[j - 1] == 0) --j;
+  if (j == 1 || w8_child(c, m) == 0) { out[n++] = m; return n; }
+  uint32_t k = c->r[m].dec[j - 1], ch = w8_child(c, m);
+  n = w8_expand(c, ch, k, out, n);
+  return w8_expand(c, ch + 1, j - k, out, n);
+}
+
+/* Slots of a wide internal node built from binary node v. */
+static int w8_slots(const w8_ctx* c, uint32_t v, uint32_t* out) {
+  uint32_t ch = w8_child(c, v);
+  if (ch == 0) { out[0] = v; return 1; }
+  uint32_t k = c->r[v].dec[7];
+  int n = w8_expand(c, ch, k, out, 0);
+  return w8_expand(c, ch + 1, 8 - k, out, n);
+}
+
+static inline int w8_is_leaf_slot(const w8_ctx* c, uint32_t m) {
+  return w8_child(c, m) == 0 || c->r[m].dec[0] == 1;
+}
+
+static void w8_dp(w8_ctx* c, uint32_t v) {
+  const uint32_t* nd = c->n8 + 8 * (size_t)v;
+  const float* b = (const float*)nd;
+  float area = w8_area(b, b + 4);
+  w8_rec* r = &c->r[v];
+  memset(r, 0, sizeof *r);
+  if (nd[7] == 0) {
+    r->p = c->lens ? c->lens[v] : 1;
+    float cl = (area * (float)r->p) * c->c_prim;
+    for (int i = 0; i < 7; ++i) r->c[i] = cl;
+    r->dec[0] = 1;
+    r->t = r->p;
+    return;
+  }
+  uint32_t L = nd[7], R = L + 1;
+  w8_dp(c, L);
+  w8_dp(c, R);
+  const w8_rec *a = &c->r[L], *bb = &c->r[R];
+  r->p = a->p + bb->p;
+  float dist[9];
+  uint32_t arg[9];
+  for (uint32_t i = 2; i <= 8; ++i) {
+    float best = INFINITY;
+    uint32_t bk = 1;
+    for (uint32_t k = 1; k < i; ++k) {
+      if (k > 7 || i - k > 7) continue;
+      float s = a->c[k - 1] + bb->c[i - k - 1];
+      if (s < best) { best = s; bk = k; }
+    }
+    dist[i] = best;
+    arg[i] = bk;
+  }
+  float cint = area * c->c_node + dist[8];
+  float cleaf = r->p <= c->p_max ? (area * (float)r->p) * c->c_prim : INFINITY;
+  r->dec[0] = cleaf <= cint ? 1 : 0;
+  r->c[0] = cleaf <= cint ? cleaf : cint;
+  for (uint32_t i = 2; i <= 7; ++i) {
+    if (dist[i] < r->c[i - 2]) { r->c[i - 1] = dist[i]; r->dec[i - 1] = (uint8_t)arg[i]; }
+    else { r->c[i - 1] = r->c[i - 2]; r->dec[i - 1] = 0; }
+  }
+  r->dec[7] = (uint8_t)arg[8];
+  /* wide subtree sizes if v becomes a wide internal node */
+  uint32_t s[8];
+  int ns = w8_slots(c, v, s);
+  r->w = 1;
+  r->t = 0;
+  for (int k = 0; k < ns; ++k) {
+    if (w8_is_leaf_slot(c, s[k])) r->t += c->r[s[k]].p;
+    else { r->w += c->r[s[k]].w; r->t += c->r[s[k]].t; }
+  }
+}
+
+/* Smallest biased exponent E in [1, 254] with 255 * 2^(E-127) >= ext. */
+static uint32_t w8_exponent(float lo, float hi) {
+  double ext = (double)hi - (double)lo;
+  int ex;
+  frexp(ext, &ex);
+  int e = ex - 8;
+  if (e < -126) e = -126;
+  while (e < 127 && ldexp(255.0, e) < ext) ++e;
+  while (e > -126 && ldexp(255.0, e - 1) >= ext) --e;
+  return (uint32_t)(e + 127);
+}
+
+/* Conservative 8-bit plane of value x relative to origin p and scale s. */
+static uint32_t w8_qlo(float p, float s, float x) {
+  double q = floor(((double)x - (double)p) / (double)s);
+  uint32_t qi = q < 0 ? 0 : q > 255 ? 255 : (uint32_t)q;
+  while (qi > 0 && p + (float)qi * s > x) --qi;
+  return qi;
+}
+static uint32_t w8_qhi(float p, float s, float x) {
+  double q = ceil(((double)x - (double)p) / (double)s);
+  uint32_t qi = q < 0 ? 0 : q > 255 ? 255 : (uint32_t)q;
+  while (qi < 255 && p + (float)qi * s < x) ++qi;
+  return qi;
+}
+
+static void w8_emit(const w8_ctx* c, uint32_t v, uint32_t idx, uint32_t region, uint32_t tri,
+                    uint8_t* out, uint32_t* tri_index) {
+  uint8_t* o = out + 80 * (size_t)idx;
+  memset(o, 0, 80);
+  const float* b = (const float*)(c->n8 + 8 * (size_t)v);
+  uint32_t s[8];
+  int ns = w8_slots(c, v, s);
+  float p[3], sc[3];
+  for (int a = 0; a < 3; ++a) {
+    uint32_t E = w8_exponent(b[a], b[4 + a]);
+    p[a] = b[a];
+    sc[a] = bitsf(E << 23);
+    o[12 + a] = (uint8_t)E;
+  }
+  memcpy(o, p, 12);
+  uint32_t nint = 0, own = 0;
+  for (int k = 0; k < ns; ++k) {
+    if (w8_is_leaf_slot(c, s[k])) own += c->r[s[k]].p;
+    else ++nint;
+  }
+  uint32_t base_child = region, base_tri = tri;
+  memcpy(o + 16, &base_child, 4);
+  memcpy(o + 20, &base_tri, 4);
+  uint32_t ci = 0, toff = 0, sub = region + nint, tsub = tri + own;
+  uint8_t imask = 0;
+  for (int k = 0; k < ns; ++k) {
+    uint32_t m = s[k];
+    const float* cb = (const float*)(c->n8 + 8 * (size_t)m);
+    for (int a = 0; a < 3; ++a) {
+      o[32 + 8 * a + k] = (uint8_t)w8_qlo(p[a], sc[a], cb[a]);
+      o[56 + 8 * a + k] = (uint8_t)w8_qhi(p[a], sc[a], cb[4 + a]);
+    }
+    if (w8_is_leaf_slot(c, m)) {
+      uint32_t np = c->r[m].p;
+      o[24 + k] = (uint8_t)((((1u << np) - 1u) << 5) | toff);
+      /* the objects of m's binary subtree in order, child before child + 1 */
+      uint32_t st[64];
+      int sp = 0;
+      st[sp++] = m;
+      while (sp > 0) {
+        uint32_t u = st[--sp];
+        const uint32_t* un = c->n8 + 8 * (size_t)u;
+        if (un[7] == 0) {
+          uint32_t len = c->lens ? c->lens[u] : 1;
+          for (uint32_t j = 0; j < len; ++j) tri_index[tri + toff++] = un[3] + j;
+        } else {
+          st[sp++] = un[7] + 1;
+          st[sp++] = un[7];
+        }
+      }
+    } else {
+      imask |= (uint8_t)(1u << k);
+      o[24 + k] = (uint8_t)((1u << 5) | (24 + ci));
+      w8_emit(c, m, region + ci, sub, tsub, out, tri_index);
+      sub += c->r[m].w - 1;
+      tsub += c->r[m].t;
+      ++ci;
+    }
+  }
+  o[15] = imask;
+}
+
+int or_wide8_collapse(const uint32_t* nodes8, const uint32_t* lens, uint32_t nnodes,
+                      float c_node, float c_prim, uint32_t p_max, uint8_t* out80,
+                      uint32_t* tri_index, uint32_t* out_nwide, float* out_cost) {
+  if (nnodes == 0) { *out_nwide = 0; return 0; }
+  if (p_max < 1 || p_max > 3) return -3; /* a leaf's count is 3 unary bits of meta */
+  w8_ctx c = {nodes8, lens, (w8_rec*)calloc(nnodes, sizeof(w8_rec)), c_node, c_prim, p_max};
+  if (!c.r) return -1;
+  for (uint32_t v = 0; v < nnodes; ++v)
+    if (nodes8[8 * (size_t)v + 7] == 0 && (lens ? lens[v] : 1) > p_max) { free(c.r); return -2; }
+  w8_dp(&c, 0);
+  uint32_t nw = nodes8[7] == 0 ? 1 : c.r[0].w;
+  w8_emit(&c, 0, 0, 1, 0, out80, tri_index);
+  *out_nwide = nw;
+  if (out_cost) {
+    float a = w8_area((const float*)nodes8, (const float*)nodes8 + 4);
+    uint32_t k = c.r[0].dec[7];
+    *out_cost = nodes8[7] == 0 ? c.r[0].c[0]
+                               : a * c_node + (c.r[nodes8[7]].c[k - 1] + c.r[nodes8[7] + 1].c[8 - k - 1]);
+  }
+  free(c.r);
+  return 0;
+}
+
+/* Closest hit through the wide tree: child boxes decoded as p + q * 2^e in
+ * fp32, the conservative slab test of the binary traversal, lowest original
+ * triangle on equal t. */
+void or_trace_wide8(const uint8_t* w80, const uint32_t* tri_index, const float* pos, uint32_t vs,
+                    const uint32_t* tris, const uint32_t* perm, const float* rays, uint32_t nrays,
+                    float* t_out, uint32_t* prim_out) {
+  for (uint32_t r = 0; r < nrays; ++r) {
+    const float* ray = rays + 6 * (size_t)r;
+    float inv[3];
+    for (int a = 0; a < 3; ++a) inv[a] = 1.0f / ray[3 + a];
+    float best = INFINITY;
+    uint32_t bp = 0xffffffffu;
+    uint32_t stack[512];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+      const uint8_t* o = w80 + 80 * (size_t)stack[--sp];
+      float p[3], sc[3];
+      memcpy(p, o, 12);
+      for (int a = 0; a < 3; ++a) sc[a] = bitsf((uint32_t)o[12 + a] << 23);
+      uint32_t bc, bt;
+      memcpy(&bc, o + 16, 4);
+      memcpy(&bt, o + 20, 4);
+      for (int k = 0; k < 8; ++k) {
+        uint8_t meta = o[24 + k];
+        if (meta == 0) continue;
+        float mn[3], mx[3], tn;
+        for (int a = 0; a < 3; ++a) {
+          mn[a] = p[a] + (float)o[32 + 8 * a + k] * sc[a];
+          mx[a] = p[a] + (float)o[56 + 8 * a + k] * sc[a];
+        }
+        if (!slab(ray, inv, mn, mx, best, &tn)) continue;
+        if ((o[15] >> k) & 1) {
+          if (sp < 512) stack[sp++] = bc + (meta & 31u) - 24u;
+        } else {
+          uint32_t cnt = (uint32_t)__builtin_popcount(meta >> 5), off = meta & 31u;
+          for (uint32_t j = 0; j < cnt; ++j) {
+            uint32_t slot = tri_index[bt + off + j];
+            const uint32_t* tr = tris + 3 * (size_t)slot;
+            float t = or_ray_tri(ray, pos + (size_t)tr[0] * vs, pos + (size_t)tr[1] * vs,
+                                 pos + (size_t)tr[2] * vs);
+            uint32_t id = perm[slot];
+            if (t < best || (t == best && t != INFINITY && id < bp)) { best = t; bp = id; }
+          }
+        }
+      }
+    }
+    t_out[r] = best;
+    prim_out[r] = bp;
+  }
 }

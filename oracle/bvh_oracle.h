@@ -119,6 +119,32 @@ int or_raytrace_tlas(const uint32_t* tlas8, const uint32_t* tlas_perm, const uin
                      const uint32_t* blas_perm, const float* rays, uint32_t nrays,
                      uint32_t* hits4);
 
+/* Compressed 8-wide BVH from a binary tree (the wide format of Ylitie,
+ * Karras and Laine, "Efficient Incoherent Ray Traversal on GPUs Through
+ * Compressed Wide BVHs", HPG 2017, which marigold's README names as its next
+ * step).  Collapse by the paper's dynamic program: C(v, i), the cost of v's
+ * subtree as at most i subtrees; a wide leaf holds at most p_max objects
+ * (cost area * count * c_prim), a wide internal node costs area * c_node plus
+ * the best split of 8 slots between the children; first minimum over k, a
+ * leaf on a tie.  Layout: the root at 0; a node's internal children
+ * contiguous at base_child, then each child's own subtree in child order; its
+ * leaves' objects contiguous at base_tri, then each child's objects.  Node of
+ * 80 bytes: p[3] f32, e[3] biased exponents, imask, base_child, base_tri,
+ * meta[8], qlo x/y/z [8], qhi x/y/z [8]; a child plane decodes as
+ * p + q * 2^(e-127) in fp32 and always contains the exact box.  tri_index:
+ * the binary leaves' start_index values in wide-leaf order.  p_max is 1..3 (-3
+ * otherwise).  Returns 0, -2
+ * when a binary leaf holds more than p_max objects. */
+int or_wide8_collapse(const uint32_t* nodes8, const uint32_t* lens, uint32_t nnodes,
+                      float c_node, float c_prim, uint32_t p_max, uint8_t* out80,
+                      uint32_t* tri_index, uint32_t* out_nwide, float* out_cost);
+
+/* Closest hit through such a wide tree (conservative slab test, lowest
+ * original triangle on equal t): equals or_closest_hit_bvh on the binary tree. */
+void or_trace_wide8(const uint8_t* w80, const uint32_t* tri_index, const float* pos,
+                    uint32_t vstride_floats, const uint32_t* tris_perm, const uint32_t* perm,
+                    const float* rays, uint32_t nrays, float* t_out, uint32_t* prim_out);
+
 #ifdef __cplusplus
 }
 #endif

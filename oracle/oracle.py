@@ -75,6 +75,10 @@ def lib():
         L.or_raytrace_tlas.argtypes = [P, P, P, C.c_uint32, P, C.c_uint32, P, P, C.c_uint32, P,
                                        P, P, C.c_uint32, P]
         L.or_raytrace_tlas.restype = C.c_int
+        L.or_wide8_collapse.argtypes = [P, P, C.c_uint32, C.c_float, C.c_float, C.c_uint32, P, P,
+                                        P, P]
+        L.or_wide8_collapse.restype = C.c_int
+        L.or_trace_wide8.argtypes = [P, P, P, C.c_uint32, P, P, P, C.c_uint32, P, P]
         L.or_ray_tri.argtypes = [P, P, P, P]
         L.or_ray_tri.restype = C.c_float
         _lib = L
@@ -216,6 +220,41 @@ def raytrace_tlas(tlas8, tlas_perm, inst16, blas4, blas8, pos, tris_perm, blas_p
                                 blas4.shape[0], _p(blas8), _p(pos), pos.shape[1], _p(tris_perm),
                                 _p(blas_perm), _p(rays), rays.shape[0], _p(out))
     return out, int(rc)
+
+
+WIDE8_DEFAULTS = {"c_node": 1.0, "c_prim": 0.3, "p_max": 3}
+
+
+def wide8_collapse(nodes8, lens=None, c_node=1.0, c_prim=0.3, p_max=3):
+    """Compressed 8-wide tree from binary nodes ((k, 8) u32).  Returns
+    (wide (m, 80) u8, tri_index (objects,) u32, root cost)."""
+    nodes8 = np.ascontiguousarray(nodes8, dtype=np.uint32)
+    lens = None if lens is None else np.ascontiguousarray(lens, dtype=np.uint32)
+    k = nodes8.shape[0]
+    nobj = int(lens[nodes8[:, 7] == 0].sum()) if lens is not None else (k + 1) // 2
+    out = np.zeros((max(k, 1), 80), dtype=np.uint8)
+    tri = np.zeros(max(nobj, 1), dtype=np.uint32)
+    nw = C.c_uint32()
+    cost = C.c_float()
+    rc = lib().or_wide8_collapse(_p(nodes8), _p(lens), k, c_node, c_prim, p_max, _p(out), _p(tri),
+                                 C.byref(nw), C.byref(cost))
+    if rc != 0:
+        raise ValueError(f"wide8 collapse failed: {rc}")
+    return out[:nw.value], tri[:nobj], float(cost.value)
+
+
+def trace_wide8(wide, tri_index, pos, tris_perm, perm, rays):
+    wide = np.ascontiguousarray(wide, dtype=np.uint8)
+    tri_index = np.ascontiguousarray(tri_index, dtype=np.uint32)
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    tris_perm = np.ascontiguousarray(tris_perm, dtype=np.uint32)
+    perm = np.ascontiguousarray(perm, dtype=np.uint32)
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    t = np.empty(rays.shape[0], dtype=np.float32)
+    prim = np.empty(rays.shape[0], dtype=np.uint32)
+    lib().or_trace_wide8(_p(wide), _p(tri_index), _p(pos), pos.shape[1], _p(tris_perm), _p(perm),
+                         _p(rays), rays.shape[0], _p(t), _p(prim))
+    return t, prim
 
 
 def instance_boxes(mesh_min, mesh_max, o2w12):

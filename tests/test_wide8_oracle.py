@@ -1,0 +1,168 @@
+"""CPU pins of the oracle's compressed 8-wide BVH (or_wide8_collapse,
+or_trace_wide8): the collapse cost equals a brute force over every admissible
+wide tree on small trees, the layout and encoding invariants hold, every
+decoded box contains everything below it, and traversal through the wide tree
+returns the brute-force closest hits."""
+import itertools
+from functools import lru_cache
+
+import numpy as np
+import pytest
+
+from marigold_amd import scenes
+from oracle import oracle as O
+
+NONE = 0xFFFFFFFF
+
+
+def _binary(pos, tris, max_depth=32):
+    b = O.build(O.tri_aabbs(pos, tris), max_depth=max_depth)
+    return O.encode(b["nodes"]), b
+
+
+def _brute_cost(n8, c_node, c_prim, p_max):
+    box = n8[:, [0, 1, 2, 4, 5, 6]].view(np.float32).astype(np.float64)
+    child = n8[:, 7]
+
+    def area(v):
+        d = box[v, 3:] - box[v, :3]
+        return 2.0 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0])
+
+    @lru_cache(None)
+    def prims(v):
+        return 1 if child[v] == 0 else prims(int(child[v])) + prims(int(child[v]) + 1)
+
+    @lru_cache(None)
+    def cuts(v):
+        out = [(v,)]
+        if child[v]:
+            for a, b in itertools.product(cuts(int(child[v])), cuts(int(child[v]) + 1)):
+                if len(a) + len(b) <= 8:
+                    out.append(a + b)
+        return tuple(out)
+
+    @lru_cache(None)
+    def slot(m):
+        best = np.inf
+        if prims(m) <= p_max:
+            best = area(m) * prims(m) * c_prim
+        if child[m]:
+            best = min(best, internal(m))
+        return best
+
+    @lru_cache(None)
+    def internal(v):
+        L, R = int(child[v]), int(child[v]) + 1
+        best = np.inf
+        for a, b in itertools.product(cuts(L), cuts(R)):
+            if len(a) + len(b) <= 8:
+                best = min(best, sum(slot(m) for m in a + b))
+        return area(v) * c_node + best
+
+    return slot(0) if child[0] == 0 else internal(0)
+
+
+@pytest.mark.parametrize("n,seed,params", [(2, 1, (1.0, 0.3, 3)), (5, 2, (1.0, 0.3, 3)),
+                                           (9, 3, (1.0, 0.3, 3)), (12, 4, (1.0, 0.3, 3)),
+                                           (10, 5, (1.0, 2.0, 3)), (11, 6, (0.2, 0.3, 1)),
+                                           (13, 7, (1.0, 0.05, 2)), (8, 8, (50.0, 0.3, 1)),
+                                           (16, 9, (50.0, 0.3, 1))])
+def test_collapse_cost_is_the_brute_force_minimum(n, seed, params):
+    pos, tris = scenes.soup(n, seed=seed, extent=0.3)
+    n8, _ = _binary(pos, tris)
+    c_node, c_prim, p_max = params
+    _, _, cost = O.wide8_collapse(n8, c_node=c_node, c_prim=c_prim, p_max=p_max)
+    want = _brute_cost(n8, c_node, c_prim, p_max)
+    assert cost == pytest.approx(want, rel=1e-5)
+
+
+def _decode(w, k):
+    p = w[0:12].view(np.float32)
+    s = (w[12:15].astype(np.uint32) << 23).view(np.float32)
+    lo = np.array([p[a] + np.float32(w[32 + 8 * a + k]) * s[a] for a in range(3)], np.float32)
+    hi = np.array([p[a] + np.float32(w[56 + 8 * a + k]) * s[a] for a in range(3)], np.float32)
+    return lo, hi
+
+
+def _check_structure(wide, tri_index, n_obj, aabb_of_slot, p_max=3):
+    nw = wide.shape[0]
+    assert sorted(tri_index.tolist()) == list(range(n_obj))
+    refs = np.zeros(nw, np.int64)
+    seen = np.zeros(n_obj, np.int64)
+    stack = [(0, [])]
+    while stack:
+        w, boxes = stack.pop()
+        o = wide[w]
+        bc = int(o[16:20].view(np.uint32)[0])
+        bt = int(o[20:24].view(np.uint32)[0])
+        imask = int(o[15])
+        nslots = int((o[24:32] != 0).sum())
+        assert nslots >= (1 if nw == 1 else 2) and np.all(o[24 + nslots:32] == 0)
+        nint = 0
+        for k in range(nslots):
+            meta = int(o[24 + k])
+            box = _decode(o, k)
+            if imask >> k & 1:
+                assert meta >> 5 == 1 and (meta & 31) - 24 == nint
+                c = bc + nint
+                nint += 1
+                assert 0 < c < nw
+                refs[c] += 1
+                stack.append((c, boxes + [box]))
+            else:
+                cnt = bin(meta >> 5).count("1")
+                assert meta >> 5 in (1, 3, 7) and 1 <= cnt <= p_max
+                for j in range(cnt):
+                    slot = int(tri_index[bt + (meta & 31) + j])
+                    seen[slot] += 1
+                    lo, hi = aabb_of_slot(slot)
+                    for blo, bhi in boxes + [box]:  # inside every decoded box above it
+                        assert np.all(blo <= lo) and np.all(hi <= bhi), (w, k, slot)
+        assert imask >> nslots == 0
+    assert refs[0] == 0 and np.all(refs[1:] == 1)
+    assert np.all(seen == 1)
+
+
+@pytest.mark.parametrize("kind", ["soup", "heightfield", "clusters", "one", "three"])
+def test_structure_and_traversal(kind):
+    pos, tris = {"soup": lambda: scenes.soup(3000, seed=11),
+                 "heightfield": lambda: scenes.heightfield(40, 30, seed=12),
+                 "clusters": lambda: scenes.clusters(2500, k=6, seed=13),
+                 "one": lambda: scenes.soup(1, seed=14, extent=0.5),
+                 "three": lambda: scenes.soup(3, seed=15, extent=0.5)}[kind]()
+    n8, b = _binary(pos, tris)
+    wide, tri_index, _ = O.wide8_collapse(n8)
+    tp = tris[b["list"]]
+    aabbs = O.tri_aabbs(pos, tp)
+    _check_structure(wide, tri_index, tris.shape[0], lambda s: (aabbs[s, :3], aabbs[s, 3:]))
+    assert wide.shape[0] <= max(1, tris.shape[0] - 1)
+    rng = np.random.default_rng(16)
+    o = rng.uniform(-0.5, 1.5, (4000, 3))
+    tgt = pos[rng.integers(0, pos.shape[0], 4000)]
+    d = tgt - o
+    d[:200] = [0, -1, 0]  # axis-aligned rays
+    o[:200, 1] = 2.0
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.concatenate([o, d], 1).astype(np.float32)
+    t1, p1 = O.trace_wide8(wide, tri_index, pos, tp, b["list"], r)
+    t0, p0 = O.closest_hit_brute(pos, tris, r)
+    assert (p0 != NONE).sum() > 300
+    assert np.array_equal(t1.view(np.uint32), t0.view(np.uint32)) and np.array_equal(p1, p0)
+
+
+def test_depth_cut_leaves():
+    # leaves of several objects (a depth cut) become wide leaves when they
+    # hold at most three; more than three cannot be encoded
+    pos, tris = scenes.soup(400, seed=17)
+    n8, b = _binary(pos, tris, max_depth=9)
+    lens = b["nodes"]["len"]
+    assert lens[n8[:, 7] == 0].max() == 3
+    wide, tri_index, _ = O.wide8_collapse(n8, lens)
+    tp = tris[b["list"]]
+    aabbs = O.tri_aabbs(pos, tp)
+    _check_structure(wide, tri_index, 400, lambda s: (aabbs[s, :3], aabbs[s, 3:]))
+    n8, b = _binary(pos, tris, max_depth=4)
+    with pytest.raises(ValueError):
+        O.wide8_collapse(n8, b["nodes"]["len"])
+    with pytest.raises(ValueError):
+        O.wide8_collapse(n8, b["nodes"]["len"], p_max=4)
