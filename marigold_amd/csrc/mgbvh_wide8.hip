@@ -11,10 +11,13 @@
 // ties, a leaf wins a leaf/internal tie, and every cost is fp32 in one fixed
 // order, so the tree is a pure function of the binary tree.
 //
-// GPU shape: one thread per binary leaf walks to the root; at each node the
-// second child to arrive (an atomic counter per node) computes the node, the
-// first stops.  The layout is then written one wide level per launch from the
-// subtree sizes (W wide nodes, T objects) the walk recorded: a node's internal
+// GPU shape: node depths by pointer jumping, then one launch per binary depth,
+// deepest first, each node combining its children's records (no fences or
+// atomics between nodes: a per-node device-scope fence, which writes back and
+// invalidates the XCD's L2, made a bottom-up atomic walk 10x slower).  A record
+// carries, per slot budget, the wide nodes and objects below its slots, so no
+// node looks further down than its children.  The layout is then written one
+// wide level per launch from the subtree sizes (W wide nodes, T objects): a node's internal
 // children sit contiguously at base_child, followed by the children's own
 // subtrees in child order, so every index follows from the sizes and the
 // result does not depend on thread order.
@@ -30,8 +33,9 @@ struct W8Rec {   // per binary node
   float c[7];    // C(v, i), i = 1..7
   uint8_t dec[8];  // [0]: 1 = leaf at one slot; [i-1], i = 2..7: 0 reduce, k split; [7]: k of 8
   uint32_t p, w, t;  // objects; wide nodes and objects of the wide subtree rooted here
+  uint32_t ws[7], ts[7];  // wide nodes and objects below the slots of budget i, i = 1..7
 };
-static_assert(sizeof(W8Rec) == 48, "W8Rec is 48 bytes");
+static_assert(sizeof(W8Rec) == 104, "W8Rec is 104 bytes");
 
 struct W8Item {  // a wide internal node to write
   uint32_t bin, idx, region, tri;
@@ -56,10 +60,9 @@ __device__ inline void w8_box(const mgbvh_node32* nodes, uint32_t v, float* mn, 
   child = b.w;
 }
 
-__global__ void k_w8_prep(uint32_t nn, uint32_t* __restrict__ parent, uint32_t* __restrict__ cnt) {
+__global__ void k_w8_prep(uint32_t nn, uint32_t* __restrict__ parent) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nn) return;
-  cnt[v] = 0;
   parent[v] = W8_NONE;
 }
 
@@ -155,55 +158,81 @@ __device__ void w8_node_dp(const mgbvh_node32* nodes, uint32_t nn, W8Rec* rec, u
     r.dec[i - 1] = split ? (uint8_t)arg[i] : 0;
   }
   r.dec[7] = (uint8_t)arg[8];
-  r.w = 0;
-  r.t = 0;
-  rec[v] = r;  // the decisions first: the slot expansion below reads them
-  uint32_t s[8];
-  const int ns = w8_slots(nodes, nn, rec, v, s);
-  if (ns > 8) { err |= 4u; return; }
-  uint32_t w = 1, t = 0;
-  for (int k = 0; k < ns; ++k) {
-    const W8Rec& q = rec[s[k]];
-    if (w8_leaf_slot(nodes, rec, s[k])) t += q.p;
-    else { w += q.w; t += q.t; }
+  // sizes from the children's per-budget sums: no walk below the children
+  const uint32_t k8 = arg[8];
+  r.w = 1 + a.ws[k8 - 1] + b.ws[8 - k8 - 1];
+  r.t = a.ts[k8 - 1] + b.ts[8 - k8 - 1];
+  r.ws[0] = leaf ? 0u : r.w;
+  r.ts[0] = leaf ? r.p : r.t;
+#pragma unroll
+  for (uint32_t i = 2; i <= 7; ++i) {
+    const uint32_t k = r.dec[i - 1];
+    r.ws[i - 1] = k == 0 ? r.ws[i - 2] : a.ws[k - 1] + b.ws[i - k - 1];
+    r.ts[i - 1] = k == 0 ? r.ts[i - 2] : a.ts[k - 1] + b.ts[i - k - 1];
   }
-  rec[v].w = w;
-  rec[v].t = t;
+  (void)nn;
+  (void)err;
+  rec[v] = r;
 }
 
-__global__ __launch_bounds__(256) void k_w8_dp(const mgbvh_node32* __restrict__ nodes,
-                                               const uint32_t* __restrict__ lens, uint32_t nn,
-                                               const uint32_t* __restrict__ parent,
-                                               uint32_t* __restrict__ cnt, W8Rec* rec,
-                                               float c_node, float c_prim, uint32_t p_max,
-                                               uint32_t* err_flag) {
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+// Depth of every node by pointer jumping: after step s each node's ancestor
+// link spans 2^s levels, so 7 steps cover a height of up to 127.
+__global__ void k_w8_depth_init(const uint32_t* __restrict__ parent, uint32_t nn,
+                                uint32_t* __restrict__ anc, uint32_t* __restrict__ dep) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nn) return;
+  const uint32_t p = parent[v];
+  anc[v] = p;
+  dep[v] = p < nn ? 1u : 0u;
+}
+
+__global__ void k_w8_depth_step(const uint32_t* __restrict__ anc_in,
+                                const uint32_t* __restrict__ dep_in, uint32_t nn,
+                                uint32_t* __restrict__ anc_out, uint32_t* __restrict__ dep_out,
+                                uint32_t* maxdep) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  const uint32_t a = anc_in[v];
+  uint32_t d = dep_in[v];
+  if (a < nn) {
+    d += dep_in[a];
+    anc_out[v] = anc_in[a];
+  } else {
+    anc_out[v] = W8_NONE;
+  }
+  dep_out[v] = d;
+  if (maxdep && anc_out[v] == W8_NONE) atomicMax(maxdep, d);
+}
+
+// One binary depth: leaves get their leaf record, internal nodes combine their
+// children's (one depth deeper, written by the previous launch).
+__global__ __launch_bounds__(256) void k_w8_dp_level(const mgbvh_node32* __restrict__ nodes,
+                                                     const uint32_t* __restrict__ lens,
+                                                     uint32_t nn, const uint32_t* __restrict__ dep,
+                                                     uint32_t level, W8Rec* rec, float c_node,
+                                                     float c_prim, uint32_t p_max,
+                                                     uint32_t* err_flag) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn || dep[v] != level) return;
   float mn[3], mx[3];
   uint32_t start, ch;
   w8_box(nodes, v, mn, mx, start, ch);
-  if (ch != 0) return;
   uint32_t err = 0;
-  W8Rec r;
-  r.p = lens ? lens[v] : 1u;
-  if (r.p > p_max) err |= 8u;
-  const float cl = __fmul_rn(__fmul_rn(w8_area(mn, mx), (float)r.p), c_prim);
+  if (ch != 0) {
+    w8_node_dp(nodes, nn, rec, v, c_node, c_prim, p_max, err);
+  } else {
+    W8Rec r;
+    r.p = lens ? lens[v] : 1u;
+    if (r.p > p_max) err |= 8u;
+    const float cl = __fmul_rn(__fmul_rn(w8_area(mn, mx), (float)r.p), c_prim);
 #pragma unroll
-  for (int i = 0; i < 7; ++i) r.c[i] = cl;
+    for (int i = 0; i < 7; ++i) { r.c[i] = cl; r.ws[i] = 0; r.ts[i] = r.p; }
 #pragma unroll
-  for (int i = 0; i < 8; ++i) r.dec[i] = 0;
-  r.dec[0] = 1;
-  r.w = 0;
-  r.t = r.p;
-  rec[v] = r;
-  for (int guard = 0; guard < 4096; ++guard) {
-    __threadfence();
-    const uint32_t u = parent[v];
-    if (u >= nn) break;  // the root (W8_NONE)
-    if (atomicAdd(&cnt[u], 1u) == 0) break;  // the sibling's subtree is not done yet
-    __threadfence();
-    w8_node_dp(nodes, nn, rec, u, c_node, c_prim, p_max, err);
-    v = u;
+    for (int i = 0; i < 8; ++i) r.dec[i] = 0;
+    r.dec[0] = 1;
+    r.w = 0;
+    r.t = r.p;
+    rec[v] = r;
   }
   if (err) atomicOr(err_flag, err);
 }
@@ -491,23 +520,36 @@ int mgbvh_wide8_collapse_device(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const
   Scratch& sc = c->sc;
   int rc;
   if ((rc = ensure(sc.w8rec, sizeof(W8Rec) * (size_t)nn))) return rc;
-  if ((rc = ensure(sc.w8par, 8 * (size_t)nn))) return rc;  // parents, then counters
+  if ((rc = ensure(sc.w8par, 20 * (size_t)nn))) return rc;  // parent, 2 x (ancestor, depth)
   const uint32_t front_cap = nn / 2 + 1;  // wide internal nodes <= binary internal nodes
   const size_t front = sizeof(W8Item) * (size_t)front_cap;
   if ((rc = ensure(sc.w8front[0], front)) || (rc = ensure(sc.w8front[1], front))) return rc;
   if ((rc = ensure(sc.w8ctr, 16))) return rc;
   W8Rec* rec = reinterpret_cast<W8Rec*>(sc.w8rec.p);
   uint32_t* parent = reinterpret_cast<uint32_t*>(sc.w8par.p);
-  uint32_t* cnt = parent + nn;
-  uint32_t* ctr = reinterpret_cast<uint32_t*>(sc.w8ctr.p);  // [0] err, [1..2] frontier counts
+  uint32_t* anc[2] = {parent + nn, parent + 3 * (size_t)nn};
+  uint32_t* dep[2] = {parent + 2 * (size_t)nn, parent + 4 * (size_t)nn};
+  uint32_t* ctr = reinterpret_cast<uint32_t*>(sc.w8ctr.p);  // [0] err, [1..2] frontiers, [3] height
   hipStream_t st = c->stream;
   const uint32_t grid = (nn + 255) / 256;
   if ((e = hipMemsetAsync(ctr, 0, 16, st)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
-  k_w8_prep<<<grid, 256, 0, st>>>(nn, parent, cnt);
+  k_w8_prep<<<grid, 256, 0, st>>>(nn, parent);
   k_w8_link<<<grid, 256, 0, st>>>(d_nodes, nn, parent, ctr);
-  k_w8_dp<<<grid, 256, 0, st>>>(d_nodes, d_lens, nn, parent, cnt, rec, P.c_node, P.c_prim,
-                                 P.p_max, ctr);
-  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "k_w8_dp");
+  k_w8_depth_init<<<grid, 256, 0, st>>>(parent, nn, anc[0], dep[0]);
+  for (int step = 0; step < 7; ++step)
+    k_w8_depth_step<<<grid, 256, 0, st>>>(anc[step & 1], dep[step & 1], nn, anc[(step + 1) & 1],
+                                          dep[(step + 1) & 1], step == 6 ? ctr + 3 : nullptr);
+  uint32_t height = 0;
+  if ((e = hipGetLastError()) != hipSuccess ||
+      (e = hipMemcpyAsync(&height, ctr + 3, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipStreamSynchronize(st)) != hipSuccess)
+    return hip_fail(e, "wide8 depths");
+  if (height > 126) { set_error("binary tree deeper than 126"); return MGBVH_ERR_INVALID_ARG; }
+  const uint32_t* depth = dep[7 & 1];
+  for (int level = (int)height; level >= 0; --level)
+    k_w8_dp_level<<<grid, 256, 0, st>>>(d_nodes, d_lens, nn, depth, (uint32_t)level, rec,
+                                        P.c_node, P.c_prim, P.p_max, ctr);
+  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "k_w8_dp_level");
   // the root's record, its children's and the first node: sizes, cost, errors
   mgbvh_node32 root;
   W8Rec r0{}, rl{}, rr{};
