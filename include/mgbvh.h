@@ -354,6 +354,24 @@ int mgbvh_wide8_collapse_device(mgbvh_ctx* ctx, const mgbvh_node32* d_nodes,
                                 uint64_t wide_cap, uint32_t* d_tri_index, uint64_t tri_cap,
                                 uint64_t* out_nwide, float* out_cost);
 
+/* Every BLAS of a batched build (mgbvh_build_blas_batch_device outputs: BLAS m's
+ * nodes at blas[m].node_offset, node_counts[m] of them, child indices relative
+ * to that offset) collapsed in one call.  Each BLAS gets the tree
+ * mgbvh_wide8_collapse_device gives its nodes alone, at d_wide +
+ * out_wide_first[m] (out_wide_count[m] nodes, 0 for an empty BLAS; base_child
+ * relative to its first wide node), its tri_index at blas[m].tri_offset
+ * (entries relative to the BLAS's permuted triangle list).  Host arrays:
+ * blas, node_counts, out_wide_first, out_wide_count (nblas each).  wide_cap:
+ * total objects is always enough. */
+int mgbvh_wide8_collapse_batch_device(mgbvh_ctx* ctx, const mgbvh_node32* d_nodes,
+                                      const uint32_t* d_lens, uint64_t nnodes,
+                                      const mgbvh_blas_desc* blas, const uint32_t* node_counts,
+                                      uint32_t nblas, const mgbvh_wide8_params* params,
+                                      mgbvh_wide_node* d_wide, uint64_t wide_cap,
+                                      uint32_t* d_tri_index, uint64_t tri_cap,
+                                      uint32_t* out_wide_first, uint32_t* out_wide_count,
+                                      uint64_t* out_nwide);
+
 /* Closest hit of each ray through a wide tree of one BLAS, with the hit rules
  * of mgbvh_trace_closest_device (the same hits as the binary tree). */
 int mgbvh_trace_wide8_device(mgbvh_ctx* ctx, const mgbvh_wide_node* d_wide, uint64_t nwide,

@@ -189,6 +189,8 @@ _EXTRA: dict = {
                                     _U64, _P], _I),
     "mgbvh_wide8_collapse_device": ([_P, _P, _P, _U64, _P, _P, _U64, _P, _U64, C.POINTER(_U64),
                                      C.POINTER(C.c_float)], _I),
+    "mgbvh_wide8_collapse_batch_device": ([_P, _P, _P, _U64, _P, _P, _U32, _P, _P, _U64, _P, _U64,
+                                           _P, _P, C.POINTER(_U64)], _I),
     "mgbvh_trace_wide8_device": ([_P, _P, _U64, _P, _P, _U32, _P, _P, _P, _U64, _P], _I),
     "mgbvh_gen_primary_rays_device": ([_P, C.POINTER(Camera), _U32, _U32, _P], _I),
     "mgbvh_copy_d2d": ([_P, _P, _P, _U64], _I),
@@ -506,6 +508,25 @@ class Context:
                                                 _dv(tri_index), tri_cap, C.byref(nw),
                                                 C.byref(cost)))
         return nw.value, cost.value
+
+    def wide8_collapse_batch(self, nodes, lens, nnodes: int, blas: np.ndarray,
+                             node_counts: np.ndarray, wide, wide_cap: int, tri_index,
+                             tri_cap: int, c_node: float = 1.0, c_prim: float = 0.3,
+                             p_max: int = 3):
+        """Every BLAS of a batched build collapsed at once.  blas: BLAS_DTYPE
+        array, node_counts: u32 per BLAS (host).  Returns (first, count, total)."""
+        prm = Wide8Params(c_node, c_prim, p_max, 0)
+        blas = np.ascontiguousarray(blas)
+        node_counts = np.ascontiguousarray(node_counts, dtype=np.uint32)
+        nb = blas.shape[0]
+        first = np.zeros(nb, np.uint32)
+        count = np.zeros(nb, np.uint32)
+        tot = C.c_uint64()
+        check(lib().mgbvh_wide8_collapse_batch_device(
+            self.handle, _dv(nodes), _dv(lens), nnodes, ptr(blas), ptr(node_counts), nb,
+            C.byref(prm), _dv(wide), wide_cap, _dv(tri_index), tri_cap, ptr(first), ptr(count),
+            C.byref(tot)))
+        return first, count, tot.value
 
     def trace_wide8(self, wide, nwide: int, tri_index, pos, vstride, tris_perm, perm, rays,
                     nrays, hits):

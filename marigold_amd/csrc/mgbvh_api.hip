@@ -86,7 +86,7 @@ void scratch_free(Scratch& sc) {
                     &sc.gaps,    &sc.lens,    &sc.mrows,    &sc.macc,     &sc.mnb,   &sc.lrows[0],
                     &sc.lrows[1], &sc.lv,     &sc.lfeat,    &sc.llogit,   &sc.gtab,  &sc.tmpn,
                     &sc.tmpl,    &sc.tmps,    &sc.w8rec,    &sc.w8par,    &sc.w8front[0],
-                    &sc.w8front[1], &sc.w8ctr};
+                    &sc.w8front[1], &sc.w8ctr,   &sc.w8base};
   for (DevBuf* b : bufs) free_buf(*b);
 }
 

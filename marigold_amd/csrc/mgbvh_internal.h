@@ -226,7 +226,7 @@ namespace mg {
 struct Scratch {
   DevBuf recs[2], L[2], hist, blk, cn[2], ck[2], small, ctr, gaps, lens, mrows, macc, mnb;
   DevBuf lrows[2], lv, lfeat, llogit, gtab, tmpn, tmpl, tmps;
-  DevBuf w8rec, w8par, w8front[2], w8ctr;  // 8-wide collapse
+  DevBuf w8rec, w8par, w8front[2], w8ctr, w8base;  // 8-wide collapse (its own: plans keep pointers)
   Counters* h_ctr = nullptr;  // pinned
 };
 void scratch_free(Scratch& sc);

@@ -8,16 +8,19 @@
 // (instance, triangle) index.  The reference's Hit.merge
 // (intersect.slang:57-82) keeps whichever equal hit was tested later, which
 // depends on the tree.  The slab test keeps a box that holds the closest hit
-// even when Moller-Trumbore's rounding puts a grazing hit slightly outside the
-// triangle's box: far distances are widened by 1 + 2^-10, near distances
-// compared against the closest t widened by 1 + 2^-10, and an axis whose plane
-// distance is 0 * inf (a ray inside a face plane) left unbounded.
+// even when Moller-Trumbore's rounding puts a grazing hit a few ulps outside
+// the triangle's box: each box is padded by 2^-20 x max(|min|, |max|) per axis
+// (on an axis the ray is nearly parallel to, a few ulps in space are far in t),
+// far distances are widened by 1 + 2^-20, near distances compared against the
+// closest t widened by 1 + 2^-10, and an axis whose plane distance is 0 * inf
+// (a ray inside a face plane) left unbounded.
 #include "mgbvh_common.h"
 #include "mgbvh_internal.h"
 
 namespace mg {
 
-constexpr float TF_WIDEN = 1.0009765625f;  // 1 + 2^-10
+constexpr float TF_WIDEN = 1.00000095367431640625f;  // 1 + 2^-20
+constexpr float BOX_PAD = 9.5367431640625e-07f;         // 2^-20 of the box's coordinate size
 constexpr float TN_SLACK = 1.0009765625f;            // 1 + 2^-10
 constexpr int TRACE_STACK = 64;                      // > tree height + 1
 constexpr uint32_t NONE = 0xffffffffu;
@@ -43,8 +46,9 @@ __device__ inline bool slab(const RayD& r, const float* bmn, const float* bmx, f
   float tn = -f_inf(), tf = f_inf();
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float t0 = __fmul_rn(__fsub_rn(bmn[a], r.o[a]), r.inv[a]);
-    const float t1 = __fmul_rn(__fsub_rn(bmx[a], r.o[a]), r.inv[a]);
+    const float pad = __fmul_rn(fmaxf(fabsf(bmn[a]), fabsf(bmx[a])), BOX_PAD);
+    const float t0 = __fmul_rn(__fsub_rn(__fsub_rn(bmn[a], pad), r.o[a]), r.inv[a]);
+    const float t1 = __fmul_rn(__fsub_rn(__fadd_rn(bmx[a], pad), r.o[a]), r.inv[a]);
     // 0 * inf: the ray runs inside a face plane, so this axis does not bound t
     const bool free_axis = (t0 != t0) | (t1 != t1);
     tn = fmaxf(tn, free_axis ? -f_inf() : fminf(t0, t1));

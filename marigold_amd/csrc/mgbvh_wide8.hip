@@ -38,8 +38,18 @@ struct W8Rec {   // per binary node
 static_assert(sizeof(W8Rec) == 104, "W8Rec is 104 bytes");
 
 struct W8Item {  // a wide internal node to write
-  uint32_t bin, idx, region, tri;
+  uint32_t bin;           // binary node (absolute index)
+  uint32_t idx, region;   // wide node and its descendants' first index, relative to wbase
+  uint32_t tri;           // first object slot of its leaves, relative to tbase
+  uint32_t wbase, tbase;  // the tree's first wide node and object slot
 };
+
+// Absolute index of v's first child (0: a leaf).  In a forest of BLASes the
+// child indices are relative to the BLAS's first node, base[v].
+__device__ inline uint32_t w8_child(const mgbvh_node32* nodes, const uint32_t* base, uint32_t v) {
+  const uint32_t ch = nodes[v].child_node;
+  return ch == 0 ? 0u : (base ? base[v] + ch : ch);
+}
 
 __device__ inline float w8_area(const float* mn, const float* mx) {
   const float dx = __fsub_rn(mx[0], mn[0]), dy = __fsub_rn(mx[1], mn[1]);
@@ -68,11 +78,12 @@ __global__ void k_w8_prep(uint32_t nn, uint32_t* __restrict__ parent) {
 
 // Separate from k_w8_prep: a node's parent entry is written after every entry
 // was reset.  Children out of range are left unlinked (and flagged).
-__global__ void k_w8_link(const mgbvh_node32* __restrict__ nodes, uint32_t nn,
+__global__ void k_w8_link(const mgbvh_node32* __restrict__ nodes,
+                          const uint32_t* __restrict__ base, uint32_t nn,
                           uint32_t* __restrict__ parent, uint32_t* err_flag) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nn) return;
-  const uint32_t ch = nodes[v].child_node;
+  const uint32_t ch = w8_child(nodes, base, v);
   if (ch == 0) return;
   if (ch <= v || ch + 1 >= nn) { atomicOr(err_flag, 16u); return; }
   parent[ch] = v;
@@ -80,15 +91,15 @@ __global__ void k_w8_link(const mgbvh_node32* __restrict__ nodes, uint32_t nn,
 }
 
 // Slots of budget j at binary node m, left child first (an explicit stack).
-__device__ inline int w8_expand(const mgbvh_node32* nodes, uint32_t nn, const W8Rec* rec,
-                                uint32_t m, uint32_t j, uint32_t* out, int n) {
+__device__ inline int w8_expand(const mgbvh_node32* nodes, const uint32_t* base, uint32_t nn,
+                                const W8Rec* rec, uint32_t m, uint32_t j, uint32_t* out, int n) {
   uint32_t sm[16], sj[16];
   int sp = 0;
   sm[sp] = m; sj[sp] = j; ++sp;
   while (sp > 0) {
     --sp;
     uint32_t u = sm[sp], b = sj[sp];
-    const uint32_t ch = nodes[u].child_node;
+    const uint32_t ch = w8_child(nodes, base, u);
     if (ch != 0 && ch + 1 >= nn) return 99;
     while (b > 1 && ch != 0 && rec[u].dec[b - 1] == 0) --b;
     if (b == 1 || ch == 0) {
@@ -105,27 +116,27 @@ __device__ inline int w8_expand(const mgbvh_node32* nodes, uint32_t nn, const W8
 }
 
 // Slots of a wide internal node built from v.
-__device__ inline int w8_slots(const mgbvh_node32* nodes, uint32_t nn, const W8Rec* rec,
-                               uint32_t v, uint32_t* out) {
-  const uint32_t ch = nodes[v].child_node;
+__device__ inline int w8_slots(const mgbvh_node32* nodes, const uint32_t* base, uint32_t nn,
+                               const W8Rec* rec, uint32_t v, uint32_t* out) {
+  const uint32_t ch = w8_child(nodes, base, v);
   if (ch == 0) { out[0] = v; return 1; }
   if (ch + 1 >= nn) return 99;
   const uint32_t k = rec[v].dec[7];
   if (k < 1 || k > 7) return 99;
-  const int n = w8_expand(nodes, nn, rec, ch, k, out, 0);
-  return n > 8 ? n : w8_expand(nodes, nn, rec, ch + 1, 8 - k, out, n);
+  const int n = w8_expand(nodes, base, nn, rec, ch, k, out, 0);
+  return n > 8 ? n : w8_expand(nodes, base, nn, rec, ch + 1, 8 - k, out, n);
 }
 
 __device__ inline bool w8_leaf_slot(const mgbvh_node32* nodes, const W8Rec* rec, uint32_t m) {
   return nodes[m].child_node == 0 || rec[m].dec[0] == 1;
 }
 
-__device__ void w8_node_dp(const mgbvh_node32* nodes, uint32_t nn, W8Rec* rec, uint32_t v,
-                           float c_node,
-                           float c_prim, uint32_t p_max, uint32_t& err) {
+__device__ void w8_node_dp(const mgbvh_node32* nodes, const uint32_t* base, W8Rec* rec,
+                           uint32_t v, float c_node, float c_prim, uint32_t p_max) {
   float mn[3], mx[3];
   uint32_t start, L;
   w8_box(nodes, v, mn, mx, start, L);
+  L = w8_child(nodes, base, v);
   const float area = w8_area(mn, mx);
   const W8Rec a = rec[L], b = rec[L + 1];
   W8Rec r;
@@ -170,8 +181,6 @@ __device__ void w8_node_dp(const mgbvh_node32* nodes, uint32_t nn, W8Rec* rec, u
     r.ws[i - 1] = k == 0 ? r.ws[i - 2] : a.ws[k - 1] + b.ws[i - k - 1];
     r.ts[i - 1] = k == 0 ? r.ts[i - 2] : a.ts[k - 1] + b.ts[i - k - 1];
   }
-  (void)nn;
-  (void)err;
   rec[v] = r;
 }
 
@@ -207,6 +216,7 @@ __global__ void k_w8_depth_step(const uint32_t* __restrict__ anc_in,
 // One binary depth: leaves get their leaf record, internal nodes combine their
 // children's (one depth deeper, written by the previous launch).
 __global__ __launch_bounds__(256) void k_w8_dp_level(const mgbvh_node32* __restrict__ nodes,
+                                                     const uint32_t* __restrict__ base,
                                                      const uint32_t* __restrict__ lens,
                                                      uint32_t nn, const uint32_t* __restrict__ dep,
                                                      uint32_t level, W8Rec* rec, float c_node,
@@ -219,7 +229,7 @@ __global__ __launch_bounds__(256) void k_w8_dp_level(const mgbvh_node32* __restr
   w8_box(nodes, v, mn, mx, start, ch);
   uint32_t err = 0;
   if (ch != 0) {
-    w8_node_dp(nodes, nn, rec, v, c_node, c_prim, p_max, err);
+    w8_node_dp(nodes, base, rec, v, c_node, c_prim, p_max);
   } else {
     W8Rec r;
     r.p = lens ? lens[v] : 1u;
@@ -262,7 +272,8 @@ __device__ inline uint32_t w8_exponent(float lo, float hi) {
 }
 
 __global__ __launch_bounds__(256) void k_w8_emit(
-    const mgbvh_node32* __restrict__ nodes, const uint32_t* __restrict__ lens,
+    const mgbvh_node32* __restrict__ nodes, const uint32_t* __restrict__ base,
+    const uint32_t* __restrict__ lens,
     const W8Rec* __restrict__ rec, const W8Item* __restrict__ in, const uint32_t* n_in,
     W8Item* __restrict__ out, uint32_t* n_out, uint32_t front_cap, uint8_t* __restrict__ wide,
     uint32_t* __restrict__ tri_index, uint32_t nn, uint32_t wide_cap, uint32_t tri_cap,
@@ -274,8 +285,9 @@ __global__ __launch_bounds__(256) void k_w8_emit(
     const uint32_t v = item.bin;
     uint32_t s[8];
     if (v >= nn) { atomicOr(err_flag, 4u); continue; }
-    const int ns = w8_slots(nodes, nn, rec, v, s);
-    if (ns > 8 || item.idx >= wide_cap) { atomicOr(err_flag, 4u); continue; }
+    const int ns = w8_slots(nodes, base, nn, rec, v, s);
+    const uint64_t widx = (uint64_t)item.wbase + item.idx;
+    if (ns > 8 || widx >= wide_cap) { atomicOr(err_flag, 4u); continue; }
     float pmn[3], pmx[3];
     uint32_t st, ch;
     w8_box(nodes, v, pmn, pmx, st, ch);
@@ -321,12 +333,12 @@ __global__ __launch_bounds__(256) void k_w8_emit(
         stk[sp++] = m;
         while (sp > 0) {  // the leaf's objects, child before child + 1
           const uint32_t u = stk[--sp];
-          const uint32_t uc = nodes[u].child_node;
+          const uint32_t uc = w8_child(nodes, base, u);
           if (uc == 0) {
             const uint32_t len = lens ? lens[u] : 1u;
             const uint32_t us = nodes[u].start_index;
             for (uint32_t j = 0; j < len; ++j) {
-              const uint32_t at = item.tri + toff++;
+              const uint64_t at = (uint64_t)item.tbase + item.tri + toff++;
               if (at < tri_cap) tri_index[at] = us + j;
               else atomicOr(err_flag, 4u);
             }
@@ -341,7 +353,8 @@ __global__ __launch_bounds__(256) void k_w8_emit(
         imask |= 1u << k;
         meta[k] = (uint8_t)((1u << 5) | (24 + ci));
         const uint32_t slot = atomicAdd(n_out, 1u);
-        if (slot < front_cap) out[slot] = W8Item{m, item.region + ci, sub, tsub};
+        if (slot < front_cap)
+          out[slot] = W8Item{m, item.region + ci, sub, tsub, item.wbase, item.tbase};
         else atomicOr(err_flag, 4u);
         sub += rec[m].w - 1;
         tsub += rec[m].t;
@@ -356,7 +369,7 @@ __global__ __launch_bounds__(256) void k_w8_emit(
       words[8 + 2 * a] = q[a][0] | (q[a][1] << 8) | (q[a][2] << 16) | ((uint32_t)q[a][3] << 24);
       words[9 + 2 * a] = q[a][4] | (q[a][5] << 8) | (q[a][6] << 16) | ((uint32_t)q[a][7] << 24);
     }
-    uint4* dst = reinterpret_cast<uint4*>(wide + 80 * (size_t)item.idx);
+    uint4* dst = reinterpret_cast<uint4*>(wide + 80 * (size_t)widx);
 #pragma unroll
     for (int k = 0; k < 5; ++k)
       dst[k] = make_uint4(words[4 * k], words[4 * k + 1], words[4 * k + 2], words[4 * k + 3]);
@@ -369,13 +382,14 @@ __device__ inline bool w8_slab(const float* o, const float* inv, const float* bm
   float tn = -__uint_as_float(0x7f800000u), tf = __uint_as_float(0x7f800000u);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float t0 = __fmul_rn(__fsub_rn(bmn[a], o[a]), inv[a]);
-    const float t1 = __fmul_rn(__fsub_rn(bmx[a], o[a]), inv[a]);
+    const float pad = __fmul_rn(fmaxf(fabsf(bmn[a]), fabsf(bmx[a])), 9.5367431640625e-07f);
+    const float t0 = __fmul_rn(__fsub_rn(__fsub_rn(bmn[a], pad), o[a]), inv[a]);
+    const float t1 = __fmul_rn(__fsub_rn(__fadd_rn(bmx[a], pad), o[a]), inv[a]);
     const bool free_axis = (t0 != t0) | (t1 != t1);
     tn = fmaxf(tn, free_axis ? -__uint_as_float(0x7f800000u) : fminf(t0, t1));
     tf = fminf(tf, free_axis ? __uint_as_float(0x7f800000u) : fmaxf(t0, t1));
   }
-  tf = __fmul_rn(tf, 1.0009765625f);  // the binary traversal's margins
+  tf = __fmul_rn(tf, 1.00000095367431640625f);  // the binary traversal's margins
   tn_out = tn;
   return tn <= tf && tf >= 0.0f && tn <= __fmul_rn(best, 1.0009765625f);
 }
@@ -489,6 +503,136 @@ static float host_area(const mgbvh_node32& n) {
   return 2.0f * a;
 }
 
+// Every BLAS's first node: base[v] for the nodes of BLAS blockIdx.x.
+__global__ void k_w8_base(const uint32_t* __restrict__ first, const uint32_t* __restrict__ count,
+                          uint32_t* __restrict__ base) {
+  const uint32_t m = blockIdx.x, f = first[m], n = count[m];
+  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) base[f + k] = f;
+}
+
+// Per BLAS root: wide nodes and objects of its tree.
+__global__ void k_w8_roots(const mgbvh_node32* __restrict__ nodes, const W8Rec* __restrict__ rec,
+                           const uint32_t* __restrict__ first, const uint32_t* __restrict__ count,
+                           uint32_t nb, uint32_t* __restrict__ out) {
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nb) return;
+  uint32_t w = 0, t = 0;
+  if (count[m]) {
+    const uint32_t r = first[m];
+    t = rec[r].t;
+    w = nodes[r].child_node == 0 ? 1u : rec[r].w;
+  }
+  out[2 * m] = w;
+  out[2 * m + 1] = t;
+}
+
+struct W8Forest {
+  W8Rec* rec = nullptr;
+  uint32_t* ctr = nullptr;  // [0] err, [1..2] frontier counts, [3] height
+};
+
+// Parents, depths and the dynamic program over every node of the forest.
+static int w8_dp_forest(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint32_t* d_lens,
+                        uint32_t nn, const uint32_t* d_base, const mgbvh_wide8_params& P,
+                        W8Forest& f) {
+  Scratch& sc = c->sc;
+  int rc;
+  if ((rc = ensure(sc.w8rec, sizeof(W8Rec) * (size_t)nn))) return rc;
+  if ((rc = ensure(sc.w8par, 20 * (size_t)nn))) return rc;  // parent, 2 x (ancestor, depth)
+  if ((rc = ensure(sc.w8ctr, 16))) return rc;
+  f.rec = reinterpret_cast<W8Rec*>(sc.w8rec.p);
+  f.ctr = reinterpret_cast<uint32_t*>(sc.w8ctr.p);
+  uint32_t* parent = reinterpret_cast<uint32_t*>(sc.w8par.p);
+  uint32_t* anc[2] = {parent + nn, parent + 3 * (size_t)nn};
+  uint32_t* dep[2] = {parent + 2 * (size_t)nn, parent + 4 * (size_t)nn};
+  hipStream_t st = c->stream;
+  const uint32_t grid = (nn + 255) / 256;
+  hipError_t e;
+  if ((e = hipMemsetAsync(f.ctr, 0, 16, st)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+  k_w8_prep<<<grid, 256, 0, st>>>(nn, parent);
+  k_w8_link<<<grid, 256, 0, st>>>(d_nodes, d_base, nn, parent, f.ctr);
+  k_w8_depth_init<<<grid, 256, 0, st>>>(parent, nn, anc[0], dep[0]);
+  for (int step = 0; step < 7; ++step)
+    k_w8_depth_step<<<grid, 256, 0, st>>>(anc[step & 1], dep[step & 1], nn, anc[(step + 1) & 1],
+                                          dep[(step + 1) & 1], step == 6 ? f.ctr + 3 : nullptr);
+  uint32_t hd[4] = {0, 0, 0, 0};
+  if ((e = hipGetLastError()) != hipSuccess ||
+      (e = hipMemcpyAsync(hd, f.ctr, 16, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipStreamSynchronize(st)) != hipSuccess)
+    return hip_fail(e, "wide8 depths");
+  if (hd[0] & 16u) {
+    set_error("a child index is out of range or not after its parent");
+    return MGBVH_ERR_INDEX;
+  }
+  if (hd[3] > 126) { set_error("binary tree deeper than 126"); return MGBVH_ERR_INVALID_ARG; }
+  const uint32_t* depth = dep[7 & 1];
+  for (int level = (int)hd[3]; level >= 0; --level)
+    k_w8_dp_level<<<grid, 256, 0, st>>>(d_nodes, d_base, d_lens, nn, depth, (uint32_t)level,
+                                        f.rec, P.c_node, P.c_prim, P.p_max, f.ctr);
+  uint32_t err = 0;
+  if ((e = hipGetLastError()) != hipSuccess ||
+      (e = hipMemcpyAsync(&err, f.ctr, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipStreamSynchronize(st)) != hipSuccess)
+    return hip_fail(e, "k_w8_dp_level");
+  if (err & 8u) {
+    set_error("a binary leaf holds more than p_max objects");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  if (err) { set_error("wide8 collapse: slot expansion out of range"); return MGBVH_ERR_INTERNAL; }
+  return MGBVH_OK;
+}
+
+// Writes the wide trees whose roots are `roots`, one wide level per launch.
+static int w8_emit_forest(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint32_t* d_lens,
+                          uint32_t nn, const uint32_t* d_base, const W8Forest& f,
+                          const std::vector<W8Item>& roots, mgbvh_wide_node* d_wide,
+                          uint64_t wide_cap, uint32_t* d_tri_index, uint64_t tri_cap) {
+  if (roots.empty()) return MGBVH_OK;
+  Scratch& sc = c->sc;
+  const uint32_t front_cap = nn / 2 + (uint32_t)roots.size() + 1;
+  const size_t front = sizeof(W8Item) * (size_t)front_cap;
+  int rc;
+  if ((rc = ensure(sc.w8front[0], front)) || (rc = ensure(sc.w8front[1], front))) return rc;
+  W8Item* fr[2] = {reinterpret_cast<W8Item*>(sc.w8front[0].p),
+                   reinterpret_cast<W8Item*>(sc.w8front[1].p)};
+  hipStream_t st = c->stream;
+  uint32_t n_in = (uint32_t)roots.size();
+  hipError_t e;
+  if ((e = hipMemcpyAsync(fr[0], roots.data(), sizeof(W8Item) * roots.size(),
+                          hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(f.ctr + 1, &n_in, 4, hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipStreamSynchronize(st)) != hipSuccess)
+    return hip_fail(e, "wide8 frontier");
+  const uint32_t egrid = std::max<uint32_t>(1, std::min<uint32_t>((nn / 2 + 255) / 256, 4096));
+  for (int level = 0; n_in > 0 && level < 256; ++level) {
+    uint32_t* cin = f.ctr + 1 + (level & 1);
+    uint32_t* cout = f.ctr + 1 + ((level + 1) & 1);
+    if ((e = hipMemsetAsync(cout, 0, 4, st)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+    k_w8_emit<<<std::min(egrid, (n_in + 255) / 256), 256, 0, st>>>(
+        d_nodes, d_base, d_lens, f.rec, fr[level & 1], cin, fr[(level + 1) & 1], cout, front_cap,
+        reinterpret_cast<uint8_t*>(d_wide), d_tri_index, nn,
+        (uint32_t)std::min<uint64_t>(wide_cap, 0xffffffffu),
+        (uint32_t)std::min<uint64_t>(tri_cap, 0xffffffffu), f.ctr);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "k_w8_emit");
+    if ((e = hipMemcpyAsync(&n_in, cout, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess)
+      return hip_fail(e, "wide8 level");
+  }
+  uint32_t err = 0;
+  if ((e = hipMemcpy(&err, f.ctr, 4, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "err");
+  if (err || n_in != 0) { set_error("wide8 collapse: layout out of range"); return MGBVH_ERR_INTERNAL; }
+  return MGBVH_OK;
+}
+
+static int w8_params(const mgbvh_wide8_params* params, mgbvh_wide8_params& P) {
+  P = params ? *params : mgbvh_wide8_params{1.0f, 0.3f, 3u, 0u};
+  if (P.p_max < 1 || P.p_max > 3 || !(P.c_node >= 0.0f) || !(P.c_prim >= 0.0f)) {
+    set_error("wide8 parameters: p_max in 1..3 (a leaf's count is 3 unary bits), costs >= 0");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  return MGBVH_OK;
+}
+
 }  // namespace mg
 
 using namespace mg;
@@ -508,65 +652,24 @@ int mgbvh_wide8_collapse_device(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const
     return MGBVH_ERR_INVALID_ARG;
   }
   if (nnodes >= 0x7fffffffull) { set_error("more than 2^31-1 nodes"); return MGBVH_ERR_INVALID_ARG; }
-  const mgbvh_wide8_params def = {1.0f, 0.3f, 3u, 0u};
-  const mgbvh_wide8_params& P = params ? *params : def;
-  if (P.p_max < 1 || P.p_max > 3 || !(P.c_node >= 0.0f) || !(P.c_prim >= 0.0f)) {
-    set_error("wide8 parameters: p_max in 1..3 (a leaf's count is 3 unary bits), costs >= 0");
-    return MGBVH_ERR_INVALID_ARG;
-  }
+  mgbvh_wide8_params P;
+  int rc = w8_params(params, P);
+  if (rc) return rc;
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
   const uint32_t nn = (uint32_t)nnodes;
-  Scratch& sc = c->sc;
-  int rc;
-  if ((rc = ensure(sc.w8rec, sizeof(W8Rec) * (size_t)nn))) return rc;
-  if ((rc = ensure(sc.w8par, 20 * (size_t)nn))) return rc;  // parent, 2 x (ancestor, depth)
-  const uint32_t front_cap = nn / 2 + 1;  // wide internal nodes <= binary internal nodes
-  const size_t front = sizeof(W8Item) * (size_t)front_cap;
-  if ((rc = ensure(sc.w8front[0], front)) || (rc = ensure(sc.w8front[1], front))) return rc;
-  if ((rc = ensure(sc.w8ctr, 16))) return rc;
-  W8Rec* rec = reinterpret_cast<W8Rec*>(sc.w8rec.p);
-  uint32_t* parent = reinterpret_cast<uint32_t*>(sc.w8par.p);
-  uint32_t* anc[2] = {parent + nn, parent + 3 * (size_t)nn};
-  uint32_t* dep[2] = {parent + 2 * (size_t)nn, parent + 4 * (size_t)nn};
-  uint32_t* ctr = reinterpret_cast<uint32_t*>(sc.w8ctr.p);  // [0] err, [1..2] frontiers, [3] height
-  hipStream_t st = c->stream;
-  const uint32_t grid = (nn + 255) / 256;
-  if ((e = hipMemsetAsync(ctr, 0, 16, st)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
-  k_w8_prep<<<grid, 256, 0, st>>>(nn, parent);
-  k_w8_link<<<grid, 256, 0, st>>>(d_nodes, nn, parent, ctr);
-  k_w8_depth_init<<<grid, 256, 0, st>>>(parent, nn, anc[0], dep[0]);
-  for (int step = 0; step < 7; ++step)
-    k_w8_depth_step<<<grid, 256, 0, st>>>(anc[step & 1], dep[step & 1], nn, anc[(step + 1) & 1],
-                                          dep[(step + 1) & 1], step == 6 ? ctr + 3 : nullptr);
-  uint32_t height = 0;
-  if ((e = hipGetLastError()) != hipSuccess ||
-      (e = hipMemcpyAsync(&height, ctr + 3, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
-      (e = hipStreamSynchronize(st)) != hipSuccess)
-    return hip_fail(e, "wide8 depths");
-  if (height > 126) { set_error("binary tree deeper than 126"); return MGBVH_ERR_INVALID_ARG; }
-  const uint32_t* depth = dep[7 & 1];
-  for (int level = (int)height; level >= 0; --level)
-    k_w8_dp_level<<<grid, 256, 0, st>>>(d_nodes, d_lens, nn, depth, (uint32_t)level, rec,
-                                        P.c_node, P.c_prim, P.p_max, ctr);
-  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "k_w8_dp_level");
-  // the root's record, its children's and the first node: sizes, cost, errors
+  W8Forest f;
+  if ((rc = w8_dp_forest(c, d_nodes, d_lens, nn, nullptr, P, f))) return rc;
   mgbvh_node32 root;
   W8Rec r0{}, rl{}, rr{};
-  uint32_t err = 0;
-  if ((e = hipMemcpyAsync(&root, d_nodes, sizeof root, hipMemcpyDeviceToHost, st)) != hipSuccess ||
-      (e = hipMemcpyAsync(&r0, rec, sizeof r0, hipMemcpyDeviceToHost, st)) != hipSuccess ||
-      (e = hipMemcpyAsync(&err, ctr, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
-      (e = hipStreamSynchronize(st)) != hipSuccess)
+  if ((e = hipMemcpy(&root, d_nodes, sizeof root, hipMemcpyDeviceToHost)) != hipSuccess ||
+      (e = hipMemcpy(&r0, f.rec, sizeof r0, hipMemcpyDeviceToHost)) != hipSuccess)
     return hip_fail(e, "wide8 root readback");
-  if (err & 16u) { set_error("a child index is out of range or not after its parent"); return MGBVH_ERR_INDEX; }
-  if (err & 8u) { set_error("a binary leaf holds more than p_max objects"); return MGBVH_ERR_INVALID_ARG; }
-  if (err) { set_error("wide8 collapse: slot expansion out of range"); return MGBVH_ERR_INTERNAL; }
   const uint32_t L = root.child_node;
   if (L != 0) {
     if (L + 1 >= nn) { set_error("root child index out of range"); return MGBVH_ERR_INDEX; }
-    if ((e = hipMemcpy(&rl, rec + L, sizeof rl, hipMemcpyDeviceToHost)) != hipSuccess ||
-        (e = hipMemcpy(&rr, rec + L + 1, sizeof rr, hipMemcpyDeviceToHost)) != hipSuccess)
+    if ((e = hipMemcpy(&rl, f.rec + L, sizeof rl, hipMemcpyDeviceToHost)) != hipSuccess ||
+        (e = hipMemcpy(&rr, f.rec + L + 1, sizeof rr, hipMemcpyDeviceToHost)) != hipSuccess)
       return hip_fail(e, "wide8 readback");
   }
   const uint64_t nwide = L == 0 ? 1 : r0.w;
@@ -576,32 +679,10 @@ int mgbvh_wide8_collapse_device(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const
               std::to_string(nobj) + ")");
     return MGBVH_ERR_INVALID_ARG;
   }
-  // one launch per wide level, the frontier count read back after each
-  W8Item first{0u, 0u, 1u, 0u};
-  W8Item* fr[2] = {reinterpret_cast<W8Item*>(sc.w8front[0].p),
-                   reinterpret_cast<W8Item*>(sc.w8front[1].p)};
-  uint32_t one = 1;
-  if ((e = hipMemcpyAsync(fr[0], &first, sizeof first, hipMemcpyHostToDevice, st)) != hipSuccess ||
-      (e = hipMemcpyAsync(ctr + 1, &one, 4, hipMemcpyHostToDevice, st)) != hipSuccess)
-    return hip_fail(e, "wide8 frontier");
-  uint32_t n_in = 1;
-  const uint32_t egrid = std::max<uint32_t>(1, std::min<uint32_t>((nn / 2 + 255) / 256, 4096));
-  for (int level = 0; n_in > 0 && level < 256; ++level) {
-    uint32_t* cin = ctr + 1 + (level & 1);
-    uint32_t* cout = ctr + 1 + ((level + 1) & 1);
-    if ((e = hipMemsetAsync(cout, 0, 4, st)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
-    k_w8_emit<<<std::min(egrid, (n_in + 255) / 256), 256, 0, st>>>(
-        d_nodes, d_lens, rec, fr[level & 1], cin, fr[(level + 1) & 1], cout, front_cap,
-        reinterpret_cast<uint8_t*>(d_wide), d_tri_index, nn,
-        (uint32_t)std::min<uint64_t>(wide_cap, 0xffffffffu),
-        (uint32_t)std::min<uint64_t>(tri_cap, 0xffffffffu), ctr);
-    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "k_w8_emit");
-    if ((e = hipMemcpyAsync(&n_in, cout, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
-        (e = hipStreamSynchronize(st)) != hipSuccess)
-      return hip_fail(e, "wide8 level");
-  }
-  if ((e = hipMemcpy(&err, ctr, 4, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "err");
-  if (err || n_in != 0) { set_error("wide8 collapse: layout out of range"); return MGBVH_ERR_INTERNAL; }
+  const std::vector<W8Item> roots = {W8Item{0u, 0u, 1u, 0u, 0u, 0u}};
+  if ((rc = w8_emit_forest(c, d_nodes, d_lens, nn, nullptr, f, roots, d_wide, wide_cap,
+                           d_tri_index, tri_cap)))
+    return rc;
   if (out_nwide) *out_nwide = nwide;
   if (out_cost) {
     if (L == 0) *out_cost = r0.c[0];
@@ -610,6 +691,90 @@ int mgbvh_wide8_collapse_device(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const
       *out_cost = host_area(root) * P.c_node + (rl.c[k - 1] + rr.c[8 - k - 1]);
     }
   }
+  return MGBVH_OK;
+}
+
+int mgbvh_wide8_collapse_batch_device(mgbvh_ctx* c, const mgbvh_node32* d_nodes,
+                                      const uint32_t* d_lens, uint64_t nnodes,
+                                      const mgbvh_blas_desc* blas, const uint32_t* node_counts,
+                                      uint32_t nblas, const mgbvh_wide8_params* params,
+                                      mgbvh_wide_node* d_wide, uint64_t wide_cap,
+                                      uint32_t* d_tri_index, uint64_t tri_cap,
+                                      uint32_t* out_wide_first, uint32_t* out_wide_count,
+                                      uint64_t* out_nwide) {
+  if (!c) { set_error("ctx is NULL"); return MGBVH_ERR_INVALID_ARG; }
+  if (out_nwide) *out_nwide = 0;
+  if (nblas == 0 || nnodes == 0) return MGBVH_OK;
+  if (!d_nodes || !d_wide || !d_tri_index || !blas || !node_counts || !out_wide_first ||
+      !out_wide_count) {
+    set_error("null argument");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  if (nnodes >= 0x7fffffffull) { set_error("more than 2^31-1 nodes"); return MGBVH_ERR_INVALID_ARG; }
+  for (uint32_t m = 0; m < nblas; ++m)
+    if ((uint64_t)blas[m].node_offset + node_counts[m] > nnodes) {
+      set_error("BLAS " + std::to_string(m) + " runs past nnodes");
+      return MGBVH_ERR_INVALID_ARG;
+    }
+  mgbvh_wide8_params P;
+  int rc = w8_params(params, P);
+  if (rc) return rc;
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  const uint32_t nn = (uint32_t)nnodes;
+  Scratch& sc = c->sc;
+  // node bases, then 3 x nblas words of BLAS metadata and root sizes
+  if ((rc = ensure(sc.w8base, 4 * (size_t)nn + 16 * (size_t)nblas))) return rc;
+  uint32_t* base = reinterpret_cast<uint32_t*>(sc.w8base.p);
+  uint32_t* d_first = base + nn;
+  uint32_t* d_count = d_first + nblas;
+  uint32_t* d_sizes = d_count + nblas;
+  std::vector<uint32_t> meta(2 * (size_t)nblas);
+  for (uint32_t m = 0; m < nblas; ++m) {
+    meta[m] = blas[m].node_offset;
+    meta[nblas + m] = node_counts[m];
+  }
+  hipStream_t st = c->stream;
+  if ((e = hipMemcpyAsync(d_first, meta.data(), 8 * (size_t)nblas, hipMemcpyHostToDevice, st)) !=
+      hipSuccess)
+    return hip_fail(e, "wide8 BLAS table");
+  // a node outside every BLAS keeps base 0 and an absolute child index
+  if ((e = hipMemsetAsync(base, 0, 4 * (size_t)nn, st)) != hipSuccess)
+    return hip_fail(e, "hipMemsetAsync");
+  k_w8_base<<<nblas, 256, 0, st>>>(d_first, d_count, base);
+  W8Forest f;
+  if ((rc = w8_dp_forest(c, d_nodes, d_lens, nn, base, P, f))) return rc;
+  k_w8_roots<<<(nblas + 255) / 256, 256, 0, st>>>(d_nodes, f.rec, d_first, d_count, nblas,
+                                                   d_sizes);
+  std::vector<uint32_t> sizes(2 * (size_t)nblas);
+  if ((e = hipGetLastError()) != hipSuccess ||
+      (e = hipMemcpyAsync(sizes.data(), d_sizes, 8 * (size_t)nblas, hipMemcpyDeviceToHost, st)) !=
+          hipSuccess ||
+      (e = hipStreamSynchronize(st)) != hipSuccess)
+    return hip_fail(e, "wide8 root sizes");
+  std::vector<W8Item> roots;
+  roots.reserve(nblas);
+  uint64_t wsum = 0;
+  for (uint32_t m = 0; m < nblas; ++m) {
+    const uint32_t w = sizes[2 * m], t = sizes[2 * m + 1];
+    out_wide_first[m] = (uint32_t)wsum;
+    out_wide_count[m] = w;
+    if (w == 0) continue;
+    if ((uint64_t)blas[m].tri_offset + t > tri_cap) {
+      set_error("wide8 tri_index capacity too small for BLAS " + std::to_string(m));
+      return MGBVH_ERR_INVALID_ARG;
+    }
+    roots.push_back(W8Item{blas[m].node_offset, 0u, 1u, 0u, (uint32_t)wsum, blas[m].tri_offset});
+    wsum += w;
+  }
+  if (wsum > wide_cap || wsum > 0xffffffffull) {
+    set_error("wide8 output capacity too small (nodes " + std::to_string(wsum) + ")");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  if ((rc = w8_emit_forest(c, d_nodes, d_lens, nn, base, f, roots, d_wide, wide_cap, d_tri_index,
+                           tri_cap)))
+    return rc;
+  if (out_nwide) *out_nwide = wsum;
   return MGBVH_OK;
 }
 

@@ -616,11 +616,15 @@ void or_closest_hit_brute(const float* pos, uint32_t vs, const uint32_t* tris, u
 }
 
 /* Slab test of the traversal (not the reference's box test, which may cull a
- * box holding the closest hit by rounding): the far distance is widened by
- * (1 + 2^-10), an axis whose plane distance is 0 * inf (the ray runs inside a
+ * box holding the closest hit by rounding): each box is padded by
+ * 2^-20 x max(|min|, |max|) per axis (a hit point off by a few ulps of its
+ * coordinates, which on an axis the ray is nearly parallel to is far off in t),
+ * the far distance is widened by (1 + 2^-20), an axis whose plane distance is
+ * 0 * inf (the ray runs inside a
  * face plane) does not bound t, and the near distance must not exceed the
  * current closest t widened by (1 + 2^-10). */
-#define OR_TF_WIDEN 1.0009765625f /* 1 + 2^-10 */
+#define OR_TF_WIDEN 1.00000095367431640625f /* 1 + 2^-20 */
+#define OR_BOX_PAD 9.5367431640625e-07f    /* 2^-20: box padded by this x max(|min|, |max|) */
 #define OR_TN_SLACK 1.0009765625f           /* 1 + 2^-10 */
 
 
@@ -628,8 +632,9 @@ static int slab(const float* o, const float* inv, const float* bmn, const float*
                 float* tn_out) {
   float tn = -INFINITY, tf = INFINITY;
   for (int a = 0; a < 3; ++a) {
-    float t0 = (bmn[a] - o[a]) * inv[a];
-    float t1 = (bmx[a] - o[a]) * inv[a];
+    float pad = fmaxf(fabsf(bmn[a]), fabsf(bmx[a])) * OR_BOX_PAD;
+    float t0 = ((bmn[a] - pad) - o[a]) * inv[a];
+    float t1 = ((bmx[a] + pad) - o[a]) * inv[a];
     if (t0 != t0 || t1 != t1) continue; /* 0 * inf: the ray runs inside a face plane */
     float lo = t0 < t1 ? t0 : t1, hi = t0 < t1 ? t1 : t0;
     tn = tn > lo ? tn : lo;

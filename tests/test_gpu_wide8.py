@@ -153,3 +153,37 @@ def test_wide_traversal_random_rays(ctx):
     assert np.array_equal(a["prim"], b["prim"])
     for buf in (d_r, d_pos, d_tp, d_perm, d_nodes, d_wide, d_tri, d_h0, d_h1):
         buf.free()
+
+
+def test_batch_collapse_equals_per_blas(ctx):
+    print("[case] wide8 batch collapse of a 7-mesh batched build (one empty, one 1-triangle)",
+          flush=True)
+    from marigold_amd.scene import GpuScene
+    meshes = [scenes.soup(900, seed=40, extent=0.1), scenes.heightfield(24, 20, seed=41),
+              (np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32)),
+              scenes.soup(37, seed=42, extent=0.2), scenes.soup(1, seed=44, extent=0.3),
+              scenes.clusters(3000, k=5, seed=45), scenes.soup(2, seed=46, extent=0.3)]
+    T = np.tile(np.eye(4, dtype=np.float32), (3, 1, 1))
+    sc = GpuScene(ctx, meshes, T, np.array([0, 1, 5], np.uint32))
+    d = sc.d
+    nt = sc.n_tris
+    d_wide, d_tri = ctx.alloc(N.WIDE_NODE_BYTES * nt), ctx.alloc(4 * nt)
+    first, count, total = ctx.wide8_collapse_batch(d["blas_nodes"], None, sc.n_slots, sc.blas,
+                                                   sc.blas_node_counts, d_wide, nt, d_tri, nt)
+    wide = d_wide.download(np.uint8, N.WIDE_NODE_BYTES * total).reshape(total, -1)
+    tri_index = d_tri.download(np.uint32, nt)
+    nodes = sc.download("blas_nodes", np.uint32, 8 * sc.n_slots).reshape(-1, 8)
+    assert count[2] == 0 and total == int(count.sum())
+    for m in range(len(meshes)):
+        n = int(sc.counts[m])
+        if n == 0:
+            continue
+        off = int(sc.node_offsets[m])
+        rw, rt, _ = O.wide8_collapse(nodes[off:off + 2 * n - 1])
+        got = wide[first[m]:first[m] + count[m]]
+        assert got.shape == rw.shape and np.array_equal(got, rw), m
+        to = int(sc.tri_offsets[m])
+        assert np.array_equal(tri_index[to:to + n], rt), m
+    d_wide.free()
+    d_tri.free()
+    sc.free()

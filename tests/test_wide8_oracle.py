@@ -166,3 +166,27 @@ def test_depth_cut_leaves():
         O.wide8_collapse(n8, b["nodes"]["len"])
     with pytest.raises(ValueError):
         O.wide8_collapse(n8, b["nodes"]["len"], p_max=4)
+
+
+@pytest.mark.parametrize("kind", ["clusters", "heightfield"])
+def test_rays_nearly_parallel_to_box_faces(kind):
+    # a direction component near zero turns a few ulps of a hit point into a
+    # large error in t; both traversals must still equal the brute force
+    pos, tris = (scenes.clusters(6000, k=9, seed=28) if kind == "clusters"
+                 else scenes.heightfield(40, 30, seed=12))
+    n8, b = _binary(pos, tris)
+    wide, tri_index, _ = O.wide8_collapse(n8)
+    tp = tris[b["list"]]
+    rng = np.random.default_rng(29)
+    o = rng.uniform(-0.5, 1.5, (20000, 3))
+    d = pos[rng.integers(0, pos.shape[0], 20000)] - o
+    for a in range(3):
+        d[a * 3000:(a + 1) * 3000, a] *= 1e-3
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.concatenate([o, d], 1).astype(np.float32)
+    t0, p0 = O.closest_hit_brute(pos, tris, r)
+    tb, pb = O.closest_hit_bvh(n8, None, pos, tp, b["list"], r)
+    tw, pw = O.trace_wide8(wide, tri_index, pos, tp, b["list"], r)
+    assert (p0 != NONE).sum() > 3000
+    for t, p in ((tb, pb), (tw, pw)):
+        assert np.array_equal(t.view(np.uint32), t0.view(np.uint32)) and np.array_equal(p, p0)
