@@ -11,7 +11,8 @@
 // ties, a leaf wins a leaf/internal tie, and every cost is fp32 in one fixed
 // order, so the tree is a pure function of the binary tree.
 //
-// GPU shape: node depths by pointer jumping, then one launch per binary depth,
+// GPU shape: the node list of every binary depth, built top-down from the
+// roots (each level's nodes append their children), then one launch per depth,
 // deepest first, each node combining its children's records (no fences or
 // atomics between nodes: a per-node device-scope fence, which writes back and
 // invalidates the XCD's L2, made a bottom-up atomic walk 10x slower).  A record
@@ -68,26 +69,6 @@ __device__ inline void w8_box(const mgbvh_node32* nodes, uint32_t v, float* mn, 
   start = a.w;
   mx[0] = __uint_as_float(b.x); mx[1] = __uint_as_float(b.y); mx[2] = __uint_as_float(b.z);
   child = b.w;
-}
-
-__global__ void k_w8_prep(uint32_t nn, uint32_t* __restrict__ parent) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nn) return;
-  parent[v] = W8_NONE;
-}
-
-// Separate from k_w8_prep: a node's parent entry is written after every entry
-// was reset.  Children out of range are left unlinked (and flagged).
-__global__ void k_w8_link(const mgbvh_node32* __restrict__ nodes,
-                          const uint32_t* __restrict__ base, uint32_t nn,
-                          uint32_t* __restrict__ parent, uint32_t* err_flag) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nn) return;
-  const uint32_t ch = w8_child(nodes, base, v);
-  if (ch == 0) return;
-  if (ch <= v || ch + 1 >= nn) { atomicOr(err_flag, 16u); return; }
-  parent[ch] = v;
-  parent[ch + 1] = v;
 }
 
 // Slots of budget j at binary node m, left child first (an explicit stack).
@@ -184,33 +165,38 @@ __device__ void w8_node_dp(const mgbvh_node32* nodes, const uint32_t* base, W8Re
   rec[v] = r;
 }
 
-// Depth of every node by pointer jumping: after step s each node's ancestor
-// link spans 2^s levels, so 7 steps cover a height of up to 127.
-__global__ void k_w8_depth_init(const uint32_t* __restrict__ parent, uint32_t nn,
-                                uint32_t* __restrict__ anc, uint32_t* __restrict__ dep) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nn) return;
-  const uint32_t p = parent[v];
-  anc[v] = p;
-  dep[v] = p < nn ? 1u : 0u;
-}
-
-__global__ void k_w8_depth_step(const uint32_t* __restrict__ anc_in,
-                                const uint32_t* __restrict__ dep_in, uint32_t nn,
-                                uint32_t* __restrict__ anc_out, uint32_t* __restrict__ dep_out,
-                                uint32_t* maxdep) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nn) return;
-  const uint32_t a = anc_in[v];
-  uint32_t d = dep_in[v];
-  if (a < nn) {
-    d += dep_in[a];
-    anc_out[v] = anc_in[a];
-  } else {
-    anc_out[v] = W8_NONE;
+// Level lists top-down: every node of level l appends its two children to the
+// list of level l + 1 (one atomic per block).  A child index must follow
+// its parent and fit the array, so a malformed tree cannot loop or overrun.
+__global__ __launch_bounds__(1024) void k_w8_next_level(const mgbvh_node32* __restrict__ nodes,
+                                                        const uint32_t* __restrict__ base,
+                                                        uint32_t nn, const uint32_t* __restrict__ in,
+                                                        uint32_t n_in, uint32_t* __restrict__ out,
+                                                        uint32_t out_cap, uint32_t* out_count,
+                                                        uint32_t* err_flag) {
+  __shared__ uint32_t wsum[16], wbase[16], bbase;
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = __lane_id();
+  const uint32_t i = blockIdx.x * blockDim.x + tid;
+  uint32_t v = 0, ch = 0;
+  if (i < n_in) {
+    v = in[i];
+    ch = v < nn ? w8_child(nodes, base, v) : 0u;
+    if (ch != 0 && (ch <= v || ch + 1 >= nn)) { atomicOr(err_flag, 16u); ch = 0; }
   }
-  dep_out[v] = d;
-  if (maxdep && anc_out[v] == W8_NONE) atomicMax(maxdep, d);
+  const uint64_t mask = __ballot(ch != 0);
+  if (lane == 0) wsum[wave] = 2u * (uint32_t)__popcll(mask);
+  __syncthreads();
+  if (tid == 0) {  // one atomic per block: same-address atomics serialise in L2
+    uint32_t acc = 0;
+    for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) { wbase[w] = acc; acc += wsum[w]; }
+    bbase = acc ? atomicAdd(out_count, acc) : 0u;
+  }
+  __syncthreads();
+  if (ch != 0) {
+    const uint32_t pos = bbase + wbase[wave] + 2u * (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (pos + 1 < out_cap) { out[pos] = ch; out[pos + 1] = ch + 1; }
+    else atomicOr(err_flag, 16u);
+  }
 }
 
 // One binary depth: leaves get their leaf record, internal nodes combine their
@@ -218,12 +204,14 @@ __global__ void k_w8_depth_step(const uint32_t* __restrict__ anc_in,
 __global__ __launch_bounds__(256) void k_w8_dp_level(const mgbvh_node32* __restrict__ nodes,
                                                      const uint32_t* __restrict__ base,
                                                      const uint32_t* __restrict__ lens,
-                                                     uint32_t nn, const uint32_t* __restrict__ dep,
-                                                     uint32_t level, W8Rec* rec, float c_node,
+                                                     uint32_t nn, const uint32_t* __restrict__ order,
+                                                     uint32_t count, W8Rec* rec, float c_node,
                                                      float c_prim, uint32_t p_max,
                                                      uint32_t* err_flag) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nn || dep[v] != level) return;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t v = order[i];
+  if (v >= nn) return;
   float mn[3], mx[3];
   uint32_t start, ch;
   w8_box(nodes, v, mn, mx, start, ch);
@@ -531,44 +519,52 @@ struct W8Forest {
   uint32_t* ctr = nullptr;  // [0] err, [1..2] frontier counts, [3] height
 };
 
-// Parents, depths and the dynamic program over every node of the forest.
+// Level lists and the dynamic program over every node of the forest whose
+// roots are `roots` (binary node ids).
 static int w8_dp_forest(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint32_t* d_lens,
-                        uint32_t nn, const uint32_t* d_base, const mgbvh_wide8_params& P,
-                        W8Forest& f) {
+                        uint32_t nn, const uint32_t* d_base, const std::vector<uint32_t>& roots,
+                        const mgbvh_wide8_params& P, W8Forest& f) {
   Scratch& sc = c->sc;
   int rc;
   if ((rc = ensure(sc.w8rec, sizeof(W8Rec) * (size_t)nn))) return rc;
-  if ((rc = ensure(sc.w8par, 20 * (size_t)nn))) return rc;  // parent, 2 x (ancestor, depth)
+  if ((rc = ensure(sc.w8par, 4 * (size_t)nn + 8))) return rc;  // node ids, level by level
   if ((rc = ensure(sc.w8ctr, 16))) return rc;
   f.rec = reinterpret_cast<W8Rec*>(sc.w8rec.p);
   f.ctr = reinterpret_cast<uint32_t*>(sc.w8ctr.p);
-  uint32_t* parent = reinterpret_cast<uint32_t*>(sc.w8par.p);
-  uint32_t* anc[2] = {parent + nn, parent + 3 * (size_t)nn};
-  uint32_t* dep[2] = {parent + 2 * (size_t)nn, parent + 4 * (size_t)nn};
+  uint32_t* order = reinterpret_cast<uint32_t*>(sc.w8par.p);
   hipStream_t st = c->stream;
-  const uint32_t grid = (nn + 255) / 256;
   hipError_t e;
-  if ((e = hipMemsetAsync(f.ctr, 0, 16, st)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
-  k_w8_prep<<<grid, 256, 0, st>>>(nn, parent);
-  k_w8_link<<<grid, 256, 0, st>>>(d_nodes, d_base, nn, parent, f.ctr);
-  k_w8_depth_init<<<grid, 256, 0, st>>>(parent, nn, anc[0], dep[0]);
-  for (int step = 0; step < 7; ++step)
-    k_w8_depth_step<<<grid, 256, 0, st>>>(anc[step & 1], dep[step & 1], nn, anc[(step + 1) & 1],
-                                          dep[(step + 1) & 1], step == 6 ? f.ctr + 3 : nullptr);
-  uint32_t hd[4] = {0, 0, 0, 0};
-  if ((e = hipGetLastError()) != hipSuccess ||
-      (e = hipMemcpyAsync(hd, f.ctr, 16, hipMemcpyDeviceToHost, st)) != hipSuccess ||
-      (e = hipStreamSynchronize(st)) != hipSuccess)
-    return hip_fail(e, "wide8 depths");
-  if (hd[0] & 16u) {
-    set_error("a child index is out of range or not after its parent");
-    return MGBVH_ERR_INDEX;
+  if (roots.size() > nn) { set_error("more roots than nodes"); return MGBVH_ERR_INVALID_ARG; }
+  if ((e = hipMemsetAsync(f.ctr, 0, 16, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(order, roots.data(), 4 * roots.size(), hipMemcpyHostToDevice, st)) !=
+          hipSuccess)
+    return hip_fail(e, "wide8 roots");
+  std::vector<uint32_t> start = {0}, count = {(uint32_t)roots.size()};
+  for (;;) {
+    const uint32_t s0 = start.back(), n0 = count.back();
+    const uint32_t s1 = s0 + n0;
+    if (s1 > nn) { set_error("more nodes reached than the array holds"); return MGBVH_ERR_INDEX; }
+    if (start.size() > 127) { set_error("binary tree deeper than 126"); return MGBVH_ERR_INVALID_ARG; }
+    if ((e = hipMemsetAsync(f.ctr + 3, 0, 4, st)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+    k_w8_next_level<<<(n0 + 1023) / 1024, 1024, 0, st>>>(d_nodes, d_base, nn, order + s0, n0,
+                                                       order + s1, nn - s1, f.ctr + 3, f.ctr);
+    uint32_t hd[4];
+    if ((e = hipGetLastError()) != hipSuccess ||
+        (e = hipMemcpyAsync(hd, f.ctr, 16, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess)
+      return hip_fail(e, "wide8 levels");
+    if (hd[0] & 16u) {
+      set_error("a child index is out of range, not after its parent, or shared");
+      return MGBVH_ERR_INDEX;
+    }
+    if (hd[3] == 0) break;
+    start.push_back(s1);
+    count.push_back(hd[3]);
   }
-  if (hd[3] > 126) { set_error("binary tree deeper than 126"); return MGBVH_ERR_INVALID_ARG; }
-  const uint32_t* depth = dep[7 & 1];
-  for (int level = (int)hd[3]; level >= 0; --level)
-    k_w8_dp_level<<<grid, 256, 0, st>>>(d_nodes, d_base, d_lens, nn, depth, (uint32_t)level,
-                                        f.rec, P.c_node, P.c_prim, P.p_max, f.ctr);
+  for (int level = (int)start.size() - 1; level >= 0; --level)
+    k_w8_dp_level<<<(count[level] + 255) / 256, 256, 0, st>>>(
+        d_nodes, d_base, d_lens, nn, order + start[level], count[level], f.rec, P.c_node,
+        P.c_prim, P.p_max, f.ctr);
   uint32_t err = 0;
   if ((e = hipGetLastError()) != hipSuccess ||
       (e = hipMemcpyAsync(&err, f.ctr, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
@@ -659,7 +655,7 @@ int mgbvh_wide8_collapse_device(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
   const uint32_t nn = (uint32_t)nnodes;
   W8Forest f;
-  if ((rc = w8_dp_forest(c, d_nodes, d_lens, nn, nullptr, P, f))) return rc;
+  if ((rc = w8_dp_forest(c, d_nodes, d_lens, nn, nullptr, {0u}, P, f))) return rc;
   mgbvh_node32 root;
   W8Rec r0{}, rl{}, rr{};
   if ((e = hipMemcpy(&root, d_nodes, sizeof root, hipMemcpyDeviceToHost)) != hipSuccess ||
@@ -742,8 +738,11 @@ int mgbvh_wide8_collapse_batch_device(mgbvh_ctx* c, const mgbvh_node32* d_nodes,
   if ((e = hipMemsetAsync(base, 0, 4 * (size_t)nn, st)) != hipSuccess)
     return hip_fail(e, "hipMemsetAsync");
   k_w8_base<<<nblas, 256, 0, st>>>(d_first, d_count, base);
+  std::vector<uint32_t> broots;
+  for (uint32_t m = 0; m < nblas; ++m)
+    if (node_counts[m]) broots.push_back(blas[m].node_offset);
   W8Forest f;
-  if ((rc = w8_dp_forest(c, d_nodes, d_lens, nn, base, P, f))) return rc;
+  if ((rc = w8_dp_forest(c, d_nodes, d_lens, nn, base, broots, P, f))) return rc;
   k_w8_roots<<<(nblas + 255) / 256, 256, 0, st>>>(d_nodes, f.rec, d_first, d_count, nblas,
                                                    d_sizes);
   std::vector<uint32_t> sizes(2 * (size_t)nblas);
