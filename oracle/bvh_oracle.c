@@ -32,6 +32,17 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* OR_PLANT selects a deliberately wrong variant of one formula.  Only the
+ * tests build such variants (into separate libraries) to show that their
+ * checks fail on a plausible mistake; the oracle proper is built with 0.
+ *   1: adaptive sweep with S = 32 thresholds for every n (lib.rs:430)
+ *   2: adaptive thresholds without the + 0.5 (lib.rs:434)
+ *   3: tree SAH leaf term without len
+ *   4: tree SAH internal term weighted by OBJECT_COST instead of DEPTH_COST */
+#ifndef OR_PLANT
+#define OR_PLANT 0
+#endif
+
 #define DEPTH_COST 1.0f  /* lib.rs:217 */
 #define OBJECT_COST 8.0f /* lib.rs:218 */
 
@@ -267,9 +278,10 @@ static int adaptive_sweep(ctx_t* c, const or_bv* parent, const or_bv* cb, const 
     }
   } else {
     uint32_t steps = n < accurate ? accurate : (n > 32 ? 32 : n);
+    if (OR_PLANT == 1) steps = 32;
     float step = bv_extent(cb, axis) / (float)steps;
     for (uint32_t i = 0; i < steps; ++i) {
-      float fi = (float)i + 0.5f;
+      float fi = (float)i + (OR_PLANT == 2 ? 0.0f : 0.5f);
       float off = step * fi;
       float t = cb->mn[axis] + off;
       split_t s;
@@ -551,8 +563,8 @@ double or_tree_sah(const or_node* nodes, uint32_t nn) {
   double acc = 0.0;
   for (uint32_t i = 0; i < nn; ++i) {
     double sa = (double)bv_sa(&nodes[i].b) / sr;
-    if (nodes[i].child != 0) acc += sa * 1.0;
-    else acc += sa * 8.0 * (double)nodes[i].len;
+    if (nodes[i].child != 0) acc += sa * (OR_PLANT == 4 ? 8.0 : 1.0);
+    else acc += sa * 8.0 * (OR_PLANT == 3 ? 1.0 : (double)nodes[i].len);
   }
   return acc;
 }

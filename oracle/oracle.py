@@ -36,24 +36,40 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-def build_lib(force: bool = False) -> str:
-    """Compiles the oracle with gcc (-O3, no FMA contraction, no fast-math)."""
-    if force or not os.path.exists(SO) or any(os.path.getmtime(s) > os.path.getmtime(SO) for s in SRC):
+def build_lib(force: bool = False, plant: int = 0) -> str:
+    """Compiles the oracle with gcc (-O3, no FMA contraction, no fast-math).
+    plant != 0 builds one of the deliberately wrong variants (OR_PLANT in
+    bvh_oracle.c) into its own library, for tests that show a check fails."""
+    so = SO if not plant else os.path.join(HERE, f"liboracle_plant{plant}.so")
+    if force or not os.path.exists(so) or any(os.path.getmtime(s) > os.path.getmtime(so) for s in SRC):
         cmd = ["gcc", "-O3", "-fPIC", "-shared", "-std=c11", "-ffp-contract=off", "-fno-fast-math",
-               "-pthread", "-o", SO] + SRC + ["-lm"]
+               "-pthread", f"-DOR_PLANT={int(plant)}", "-o", so] + SRC + ["-lm"]
         subprocess.run(cmd, check=True)
-    return SO
+    return so
 
 
 _lib = None
+_planted = {}
+
+
+def planted_lib(plant: int):
+    """A deliberately wrong variant of the oracle (see OR_PLANT in bvh_oracle.c)."""
+    if plant not in _planted:
+        _planted[plant] = _bind(C.CDLL(build_lib(plant=plant)))
+    return _planted[plant]
 
 
 def lib():
     global _lib
     if _lib is None:
         build_lib()
-        L = C.CDLL(SO)
-        P = C.c_void_p
+        _lib = _bind(C.CDLL(SO))
+    return _lib
+
+
+def _bind(L):
+    P = C.c_void_p
+    if True:
         L.or_tri_aabbs.argtypes = [P, C.c_uint32, P, C.c_uint64, P]
         L.or_build.argtypes = [P, C.c_uint32, C.POINTER(Settings), P, P, P, C.POINTER(Stats)]
         L.or_build.restype = C.c_int
@@ -81,8 +97,7 @@ def lib():
         L.or_trace_wide8.argtypes = [P, P, P, C.c_uint32, P, P, P, C.c_uint32, P, P]
         L.or_ray_tri.argtypes = [P, P, P, P]
         L.or_ray_tri.restype = C.c_float
-        _lib = L
-    return _lib
+    return L
 
 
 def _p(a):
@@ -98,7 +113,7 @@ def tri_aabbs(pos: np.ndarray, tris: np.ndarray) -> np.ndarray:
 
 
 def build(aabbs: np.ndarray, max_depth: int = 32, bounds=None, min_leaf: int = 1, max_leaf: int = 1,
-          list_in: np.ndarray | None = None):
+          list_in: np.ndarray | None = None, L=None):
     """Runs BoundingVolumeHierarchy::new over objects with the given boxes.
     Returns dict(nodes, list, recs, stats)."""
     aabbs = np.ascontiguousarray(aabbs, dtype=np.float32)
@@ -114,7 +129,7 @@ def build(aabbs: np.ndarray, max_depth: int = 32, bounds=None, min_leaf: int = 1
     nodes = np.zeros(cap, dtype=NODE_DTYPE)
     recs = np.zeros(cap, dtype=REC_DTYPE)
     st = Stats()
-    rc = lib().or_build(_p(aabbs), n, C.byref(s), _p(lst), _p(nodes), _p(recs), C.byref(st))
+    rc = (L or lib()).or_build(_p(aabbs), n, C.byref(s), _p(lst), _p(nodes), _p(recs), C.byref(st))
     if rc != 0:
         raise RuntimeError(f"oracle build failed: {rc}")
     k = st.nnodes
@@ -129,9 +144,9 @@ def encode(nodes: np.ndarray) -> np.ndarray:
     return out
 
 
-def tree_sah(nodes: np.ndarray) -> float:
+def tree_sah(nodes: np.ndarray, L=None) -> float:
     nodes = np.ascontiguousarray(nodes)
-    return float(lib().or_tree_sah(_p(nodes), nodes.shape[0]))
+    return float((L or lib()).or_tree_sah(_p(nodes), nodes.shape[0]))
 
 
 def surface_area(b6) -> float:
