@@ -33,6 +33,7 @@ namespace mg {
 // --------------------------------------------------------------- records
 typedef float f4_a4 __attribute__((ext_vector_type(4), aligned(4)));
 typedef float f3_a4 __attribute__((ext_vector_type(3), aligned(4)));
+typedef uint32_t u3_a4 __attribute__((ext_vector_type(3), aligned(4)));
 __device__ inline void rec_load(const Rec* r, Box& b, uint32_t& id) {
   const float* p = reinterpret_cast<const float*>(r);
   const f4_a4 a = *reinterpret_cast<const f4_a4*>(p);
@@ -1152,57 +1153,114 @@ __global__ __launch_bounds__(NW * 64) void k_partition(const LNode* __restrict__
 // once: segmented scans give centroid bounds and the first-minimum split, each
 // lane evaluates the 3 candidates (axis a, threshold r) of its list position
 // (adaptive sweep, lib.rs:391-442), and a stable in-segment partition moves the
-// objects for the next level.  A triangle's three vertex indices travel with
-// it from the first load, so leaves store the permuted triangle list without
-// another global read.
+// objects for the next level.
+//
+// Per level the fixed work is kept small, since a subtree pair takes about
+// five levels and the candidate loop is the only part that grows with the
+// node size:
+//   * a node's record is written when its parent splits (everything in it is
+//     known then: box, start, count, pair base), so the lane that found the
+//     split writes both children's records as 64 contiguous bytes; only the
+//     subtree root is written by the root's first lane;
+//   * the other lanes of a node get from the winner only the child count, the
+//     threshold and the two children's surface areas (the next level's
+//     parent SA, the same bits box_sa gives), not the boxes;
+//   * object data lives in LDS only (each lane keeps its centroid in
+//     registers); the final order is written once, after the last level, by
+//     all lanes at consecutive positions.
 // One wavefront's objects in LDS, one array per field group, so lanes that
-// touch different positions hit different banks.  Positions 64-95 are never
-// written: they only keep the candidate loop's reads (position <= 63 plus up
-// to 32) inside the arrays.
+// touch different positions hit different banks, followed by the staged node
+// records of its two subtrees.  The candidate loop reads positions up to
+// 63 + 32 without a bound test: reads past an array's 64 entries land in the
+// next array of the same struct and are never folded in (they belong to no
+// node), so the loop addresses the arrays through byte offsets from the
+// struct's base.
 struct SmallObjs {
-  float4 a[96];     // c0 c1 c2 mn0
-  float4 b[96];     // mn1 mn2 mx0 mx1
-  float4 d[96];     // mx2 id tri0 tri1
-  uint32_t e[64];   // tri2
+  float4 a[64];      // c0 c1 c2 mn0
+  float4 b[64];      // mn1 mn2 mx0 mx1
+  float m2[64];      // mx2
+  uint32_t id[64];   // lane the object was loaded by (its index and triangle stay there)
+  // per half: records of the subtree's descendants (slot - pair base of the
+  // root, at most 62) and of the root itself (entry 63); written to memory
+  // once, after the subtree's last level
+  uint4 rec[2][64][2];
+  uint32_t len[2][64];
+  // per half: constants of its subtree, read by the lanes that write records,
+  // and counts kept in LDS so that the level loop issues no memory operation
+  uint32_t hdr[2][16];
+  uint32_t gap[2][16][2];  // depth-cut leaves: pair base, unused slots (at most 16)
+  // the subtree pair's output by final position: index inside the mesh, triangle
+  uint32_t operm[64];
+  uint32_t otri[64][3];
 };
+// SmallObjs::hdr fields
+constexpr uint32_t H_BASE = 0, H_NBASE = 1, H_OBASE = 2, H_P0 = 3, H_ROOT = 4, H_NREC = 5,
+                   H_OWN = 6, H_NMED = 7, H_ERR = 8, H_NGAP = 9;
+constexpr uint32_t SO_A = 0, SO_B = 64 * 16, SO_M2 = 128 * 16;
+static_assert(offsetof(SmallObjs, b) == SO_B && offsetof(SmallObjs, m2) == SO_M2,
+              "SmallObjs field offsets");
+static_assert(SO_M2 + 96 * 4 <= sizeof(SmallObjs), "loop reads stay inside SmallObjs");
+constexpr uint32_t ROOT_REC = 63;
 
-struct SmallLane {
-  float c[3], mn[3], mx[3];
-  uint32_t id, tri[3];
-};
-
-__device__ inline void obj_put(SmallObjs* S, uint32_t pos, const SmallLane& o) {
-  S->a[pos] = make_float4(o.c[0], o.c[1], o.c[2], o.mn[0]);
-  S->b[pos] = make_float4(o.mn[1], o.mn[2], o.mx[0], o.mx[1]);
-  S->d[pos] = make_float4(o.mx[2], __uint_as_float(o.id), __uint_as_float(o.tri[0]),
-                          __uint_as_float(o.tri[1]));
-  S->e[pos] = o.tri[2];
+__device__ inline void obj_put(SmallObjs* S, uint32_t pos, const Box& bx, const float (&c)[3],
+                               uint32_t id) {
+  S->a[pos] = make_float4(c[0], c[1], c[2], bx.mn[0]);
+  S->b[pos] = make_float4(bx.mn[1], bx.mn[2], bx.mx[0], bx.mx[1]);
+  S->m2[pos] = bx.mx[2];
+  S->id[pos] = id;
 }
-__device__ inline void obj_get(const SmallObjs* S, uint32_t pos, SmallLane& o) {
-  const float4 a = S->a[pos], b = S->b[pos], d = S->d[pos];
-  o.c[0] = a.x; o.c[1] = a.y; o.c[2] = a.z; o.mn[0] = a.w;
-  o.mn[1] = b.x; o.mn[2] = b.y; o.mx[0] = b.z; o.mx[1] = b.w;
-  o.mx[2] = d.x; o.id = __float_as_uint(d.y);
-  o.tri[0] = __float_as_uint(d.z); o.tri[1] = __float_as_uint(d.w); o.tri[2] = S->e[pos];
+// The box of the object at `pos`.
+__device__ inline Box obj_box(const SmallObjs* S, uint32_t pos) {
+  const float4 a = S->a[pos], b = S->b[pos];
+  Box r;
+  r.mn[0] = a.w; r.mn[1] = b.x; r.mn[2] = b.y;
+  r.mx[0] = b.z; r.mx[1] = b.w; r.mx[2] = S->m2[pos];
+  return r;
 }
-// The centroid and box of the object at `pos` (what the candidate loop reads).
+// The centroid and box of the object at `pos` <= 95 (what the candidate loop
+// reads; positions past 63 give values that are never used).
 struct SmallGeo {
   float4 a, b;
   float mx2;
 };
 __device__ inline SmallGeo geo_get(const SmallObjs* S, uint32_t pos) {
+  const char* base = reinterpret_cast<const char*>(S);
   SmallGeo g;
-  g.a = S->a[pos];
-  g.b = S->b[pos];
-  g.mx2 = S->d[pos].x;
+  g.a = *reinterpret_cast<const float4*>(base + SO_A + 16 * pos);
+  g.b = *reinterpret_cast<const float4*>(base + SO_B + 16 * pos);
+  g.mx2 = *reinterpret_cast<const float*>(base + SO_M2 + 4 * pos);
   return g;
 }
-// Moves each lane's object to `newpos` (a permutation of the lanes).
-__device__ inline void lane_move(SmallObjs* S, uint32_t lane, uint32_t newpos, SmallLane& o) {
-  obj_put(S, newpos, o);
+// Moves each lane's object to `newpos` (a permutation of the lanes) and
+// returns the centroid of the object that arrives at the lane's position.
+__device__ inline void lane_move(SmallObjs* S, uint32_t lane, uint32_t newpos, float (&c)[3]) {
+  const float4 a = S->a[lane], b = S->b[lane];
+  const float m2 = S->m2[lane];
+  const uint32_t id = S->id[lane];
   wave_sync();
-  obj_get(S, lane, o);
+  S->a[newpos] = a;
+  S->b[newpos] = b;
+  S->m2[newpos] = m2;
+  S->id[newpos] = id;
   wave_sync();
+  const float4 n = S->a[lane];
+  c[0] = n.x; c[1] = n.y; c[2] = n.z;
+}
+// A node record into the half's LDS staging area.
+__device__ inline void stage_node(SmallObjs* S, uint32_t h, uint32_t e, const Box& b,
+                                  uint32_t start, uint32_t child, uint32_t len) {
+  S->rec[h][e][0] = make_uint4(fbits(b.mn[0]), fbits(b.mn[1]), fbits(b.mn[2]), start);
+  S->rec[h][e][1] = make_uint4(fbits(b.mx[0]), fbits(b.mx[1]), fbits(b.mx[2]), child);
+  S->len[h][e] = len;
+}
+
+// A depth-cut leaf of n > 1 objects at pair base P leaves 2(n - 1) slots
+// unused; k_compact squeezes them out afterwards.
+__device__ inline void put_gap(Counters* ctr, uint32_t* gaps, uint32_t P, uint32_t n) {
+  const uint32_t g = atomicAdd(&ctr->n_gaps, 1u);
+  gaps[2 * g] = P;
+  gaps[2 * g + 1] = 2 * (n - 1);
+  atomicAdd(&ctr->n_cutoff, 1u);
 }
 
 // Writes the final object order: perm (index inside the mesh) and, for
@@ -1218,18 +1276,95 @@ __device__ inline void put_perm(uint32_t pos, uint32_t id, uint32_t obase, uint3
     tris_out[3 * (size_t)pos + 2] = c;
   }
 }
-__device__ inline void put_leaf(uint32_t pos, const SmallLane& o, uint32_t obase, uint32_t* perm,
-                                uint32_t* tris_out) {
-  perm[pos] = o.id - obase;
-  if (tris_out) {
-    tris_out[3 * (size_t)pos] = o.tri[0];
-    tris_out[3 * (size_t)pos + 1] = o.tri[1];
-    tris_out[3 * (size_t)pos + 2] = o.tri[2];
+
+// A depth-cut leaf of the half's subtree, listed in LDS (a subtree of <= 32
+// objects has at most 16 leaves of two or more objects).
+__device__ inline void stage_gap(SmallObjs* S, uint32_t h, uint32_t P, uint32_t n) {
+  const uint32_t k = atomicAdd(&S->hdr[h][H_NGAP], 1u);
+  S->gap[h][k][0] = P;
+  S->gap[h][k][1] = 2 * (n - 1);
+}
+
+// The record of a node created by a split at depth d - 1, staged at entry e:
+// a leaf when it holds one object or d == max_depth (lib.rs:548-558), else an
+// internal node whose children will sit at pair base P.
+__device__ inline void put_child(SmallObjs* S, uint32_t h, uint32_t e, const Box& b,
+                                 uint32_t start, uint32_t n, uint32_t P, uint32_t nbase,
+                                 uint32_t d, uint32_t max_depth) {
+  const bool inner = n > 1 && d < max_depth;
+  stage_node(S, h, e, b, start, inner ? P - nbase : 0u, n);
+  if (n > 1 && d >= max_depth) stage_gap(S, h, P, n);
+}
+
+// Writes out the previous subtree pair staged in LDS (records, lengths, split
+// records, depth-cut gaps, counts, permutation and permuted triangles) and
+// marks the staging area empty.  Called with every lane of the wavefront.
+__device__ inline void small_flush(SmallObjs* S, uint4* __restrict__ sp, const NodeOut& out,
+                                   uint32_t* __restrict__ perm, uint32_t* __restrict__ tris_out,
+                                   Counters* ctr, uint32_t* gaps) {
+  const uint32_t lane = lane_id();
+  const uint32_t h = lane >> 5, hl = lane & 31;
+  const uint32_t* hd = S->hdr[h];
+  const uint32_t own = hd[H_OWN];
+  if (own) {
+    const uint32_t P0 = hd[H_P0], root = hd[H_ROOT], nrec = hd[H_NREC];
+    // descendants fill slots [P0, P0 + nrec): 16-byte stores at consecutive
+    // addresses; the root its own slot
+    uint4* dst = reinterpret_cast<uint4*>(out.nodes + P0);
+    const uint4* src = &S->rec[h][0][0];
+    for (uint32_t j = hl; j < 2 * nrec; j += 32) dst[j] = src[j];
+    if (hl < 2) reinterpret_cast<uint4*>(out.nodes + root)[hl] = S->rec[h][ROOT_REC][hl];
+    if (out.lens) {
+      for (uint32_t j = hl; j < nrec; j += 32) out.lens[P0 + j] = S->len[h][j];
+      if (hl == 0) out.lens[root] = S->len[h][ROOT_REC];
+    }
+    if (sp) {  // split records of the internal nodes (entries without one hold ~0)
+      const uint4* ss = sp + 64 * h;
+      for (uint32_t j = hl; j < nrec; j += 32) {
+        const uint4 v = ss[j];
+        if (v.x != ~0u) reinterpret_cast<uint4*>(out.splits)[P0 + j] = v;
+      }
+      if (hl == 0 && ss[ROOT_REC].x != ~0u)
+        reinterpret_cast<uint4*>(out.splits)[root] = ss[ROOT_REC];
+    }
+    const uint32_t ng = hd[H_NGAP];
+    if (ng) {
+      uint32_t g = 0;
+      if (hl == 0) {
+        g = atomicAdd(&ctr->n_gaps, ng);
+        atomicAdd(&ctr->n_cutoff, ng);
+      }
+      g = (uint32_t)__shfl((int)g, (int)(lane & 32u), 64);
+      if (hl < ng) {
+        gaps[2 * (g + hl)] = S->gap[h][hl][0];
+        gaps[2 * (g + hl) + 1] = S->gap[h][hl][1];
+      }
+    }
+    if (hl == 0 && hd[H_NMED]) atomicAdd(&ctr->n_median, hd[H_NMED]);
+    if (hl == 0 && hd[H_ERR]) atomicOr(&ctr->err, ERR_INTERNAL);
+    if (hl < own) {
+      const uint32_t pos = hd[H_BASE] + hl;
+      perm[pos] = S->operm[lane];
+      if (tris_out) {
+        tris_out[3 * (size_t)pos] = S->otri[lane][0];
+        tris_out[3 * (size_t)pos + 1] = S->otri[lane][1];
+        tris_out[3 * (size_t)pos + 2] = S->otri[lane][2];
+      }
+    }
   }
+  wave_sync();
+  if (hl == 0) S->hdr[h][H_OWN] = 0;
+  wave_sync();
 }
 
 // Subtrees ea (lanes 0-31) and eb (lanes 32-63); an index >= nS is none.
-__device__ inline void small_pair(uint32_t ea, uint32_t eb, SmallObjs* S,
+// Order of memory operations: this pair's loads first, then the previous
+// pair's stores (small_flush), then the level loop, which issues none.  Vector
+// memory operations of a wavefront retire in order, so a wait for a load also
+// waits for every store issued before it; stores issued right before the level
+// loop have long retired when the next pair waits for its loads.  sp: split
+// record staging of the wavefront (null when no split records are written).
+__device__ inline void small_pair(uint32_t ea, uint32_t eb, SmallObjs* S, uint4* __restrict__ sp,
                                   const SNode* __restrict__ small,
                                   uint32_t nS, const Rec* __restrict__ rec0,
                                   const Rec* __restrict__ rec1, NodeOut out,
@@ -1240,88 +1375,124 @@ __device__ inline void small_pair(uint32_t ea, uint32_t eb, SmallObjs* S,
   const uint32_t h = lane >> 5, hl = lane & 31, off = 32 * h;
   const uint32_t e = h ? eb : ea;
 
+  // has: the lane's object sits in a node that is still to be split
   bool has = false;
-  SmallLane o;
-#pragma unroll
-  for (int x = 0; x < 3; ++x) { o.c[x] = 0.0f; o.mn[x] = 0.0f; o.mx[x] = 0.0f; o.tri[x] = 0; }
-  o.id = 0;
-  uint32_t ls = lane, lc = 1, idx = 0, P = 0, depth = 0, base = 0, nbase = 0, obase = 0;
-  Box nb = box_empty();
-  if (e < nS) {
-    const SNode sn = small[e];
+  float c[3] = {0.0f, 0.0f, 0.0f};
+  uint32_t ls = lane, lc = 1, idx = 0, P = 0, depth = 0;
+  float sa_p = 0.0f;
+  // the lane's object: original index and triangle, loaded with its record so
+  // the gather's latency overlaps the record load; written out from this lane
+  uint32_t oid = 0, tri0 = 0, tri1 = 0, tri2 = 0;
+  // the half's subtree constants (written to LDS after the previous flush)
+  uint32_t hv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  bool stage_root = false, forced = false;
+  Box nb;
+  {
+    // Straight-line loads with clamped addresses (every lane loads something
+    // valid), so that no branch makes the compiler wait for the triangle
+    // gather before the level loop: the gather is first used at the end.
+    const SNode sn = small[e < nS ? e : nS - 1];
+    const bool mine = e < nS && sn.count <= MG_SMALL && hl < sn.count;
     const Rec* src = sn.parity ? rec1 : rec0;
-    if (sn.count > MG_SMALL) {
-      // leaf forced by max_depth (lib.rs:556) holding more than 32 objects
-      if (hl == 0) {
-        Box b;
+    const uint32_t j = min(hl, sn.count - 1);
+    Box b;
+    rec_load(&src[sn.start + j], b, oid);
+    // boxes (TLAS) have no triangles: read the record again instead
+    const uint32_t* tsrc = tris_out ? tris_in + 3 * (size_t)oid
+                                    : reinterpret_cast<const uint32_t*>(&src[sn.start + j]);
+    const u3_a4 t = *reinterpret_cast<const u3_a4*>(tsrc);  // one 12-byte load
+    tri0 = t.x;
+    tri1 = t.y;
+    tri2 = t.z;
+    // everything but that load has arrived (s_waitcnt vmcnt(1), other counters
+    // untouched): the record and node fields used below need no further wait,
+    // and the gather stays in flight across the flush and the level loop
+    __builtin_amdgcn_s_waitcnt(0x0F71);
+    forced = e < nS && sn.count > MG_SMALL;  // written after the level loop
 #pragma unroll
-        for (int x = 0; x < 3; ++x) { b.mn[x] = sn.bmn[x]; b.mx[x] = sn.bmx[x]; }
-        out.write_node(sn.idx, b, sn.start - sn.obase, 0, sn.count);
-        const uint32_t g = atomicAdd(&ctr->n_gaps, 1u);
-        gaps[2 * g] = sn.P;
-        gaps[2 * g + 1] = 2 * (sn.count - 1);
-        atomicAdd(&ctr->n_cutoff, 1u);
-      }
-      for (uint32_t j = hl; j < sn.count; j += 32)
-        put_perm(sn.start + j, src[sn.start + j].id, sn.obase, perm, tris_in, tris_out);
-    } else if (hl < sn.count) {
-      Box b;
-      rec_load(&src[sn.start + hl], b, o.id);
-      if (tris_out) {
-        o.tri[0] = tris_in[3 * (size_t)o.id];
-        o.tri[1] = tris_in[3 * (size_t)o.id + 1];
-        o.tri[2] = tris_in[3 * (size_t)o.id + 2];
-      }
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        o.mn[x] = b.mn[x];
-        o.mx[x] = b.mx[x];
-        o.c[x] = center(b.mn[x], b.mx[x]);
-        nb.mn[x] = sn.bmn[x];
-        nb.mx[x] = sn.bmx[x];
-      }
-      has = true;
-      ls = off; lc = sn.count; idx = sn.idx; P = sn.P; depth = sn.depth; base = sn.start;
-      nbase = sn.nbase; obase = sn.obase;
+    for (int x = 0; x < 3; ++x) { nb.mn[x] = sn.bmn[x]; nb.mx[x] = sn.bmx[x]; }
+    if (e < nS && sn.count <= MG_SMALL) {
+      stage_root = true;
+      hv[H_BASE] = sn.start;
+      hv[H_NBASE] = sn.nbase;
+      hv[H_OBASE] = sn.obase;
+      hv[H_P0] = sn.P;
+      hv[H_ROOT] = sn.idx;
+      // descendant records (none below a depth-cut root)
+      hv[H_NREC] = sn.count > 1 && sn.depth < max_depth ? 2 * (sn.count - 1) : 0u;
+      hv[H_OWN] = sn.count;
+      hv[7] = sn.depth;
     }
+    if (mine) {
+      ls = off; lc = sn.count; idx = sn.idx; P = sn.P; depth = sn.depth;
+      has = lc > 1 && depth < max_depth;
+      sa_p = box_sa(nb);
+    } else {  // lanes without an object store zeros (never read as objects)
+#pragma unroll
+      for (int x = 0; x < 3; ++x) { b.mn[x] = 0.0f; b.mx[x] = 0.0f; }
+    }
+#pragma unroll
+    for (int x = 0; x < 3; ++x) c[x] = center(b.mn[x], b.mx[x]);
+    obj_put(S, lane, b, c, lane);
   }
-  obj_put(S, lane, o);
+  wave_sync();
+  small_flush(S, sp, out, perm, tris_out, ctr, gaps);
+  if (hl == 0 && stage_root) {  // the subtree root's own record and constants
+    const uint32_t count = hv[H_OWN], P0 = hv[H_P0], rdepth = hv[7];
+    const bool inner = count > 1 && rdepth < max_depth;
+    uint32_t* hd = S->hdr[h];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) hd[k] = hv[k];
+    hd[H_NMED] = 0;
+    hd[H_ERR] = 0;
+    hd[H_NGAP] = 0;
+    stage_node(S, h, ROOT_REC, nb, hv[H_BASE] - hv[H_OBASE], inner ? P0 - hv[H_NBASE] : 0u,
+               count);
+    if (count > 1 && rdepth >= max_depth) stage_gap(S, h, P0, count);
+  }
+  if (sp) {
+    for (uint32_t j = hl; j < 64; j += 32) sp[64 * h + j] = make_uint4(~0u, ~0u, ~0u, ~0u);
+  }
   wave_sync();
 
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  while (__ballot(has)) {
-    // leaves: one object, or depth == max_depth (lib.rs:548-558)
-    if (has && (lc == 1 || depth == max_depth)) {
-      if (lane == ls) {
-        out.write_node(idx, nb, base + ls - off - obase, 0, lc);
-        if (lc > 1) {
-          const uint32_t g = atomicAdd(&ctr->n_gaps, 1u);
-          gaps[2 * g] = P;
-          gaps[2 * g + 1] = 2 * (lc - 1);
-          atomicAdd(&ctr->n_cutoff, 1u);
-        }
-      }
-      put_leaf(base + lane - off, o, obase, perm, tris_out);
-      has = false;
-    }
+  for (;;) {
     const bool act = has;
     if (!__ballot(act)) break;
     const uint32_t last = ls + lc - 1;
     const uint32_t r = lane - ls;
-    // centroid bounds of each node (lib.rs:499-509)
-    const float sa_p = box_sa(nb);
-    float th[3] = {o.c[0], o.c[1], o.c[2]};
+    // thresholds: the centroids in list order below 8 objects, else S = len
+    // uniform steps over the centroid bounds (lib.rs:417-441)
+    float th[3] = {c[0], c[1], c[2]};
     if (__ballot(act && lc >= 8)) {
-      const Box cb = seg_box(act, o.c, o.c, lane, ls, last);
+      const Box cb = seg_box(act, c, c, lane, ls, last);  // lib.rs:499-509
 #pragma unroll
       for (int a = 0; a < 3; ++a)
         if (lc >= 8) th[a] = adaptive_threshold(cb.mn[a], cb.mx[a], lc, r);
     }
-    // the three candidates (a, r) of this lane over the node's objects
+    // the three candidates (a, r) of this lane over the node's objects; the
+    // first object initialises both sides
     Box lt[3], gt[3];
-    uint32_t nl[3], ng[3];
+    uint32_t nl[3];
+    const uint32_t p0 = act ? ls : lane;
+    {
+      const SmallGeo g = geo_get(S, p0);
+      const float qc[3] = {g.a.x, g.a.y, g.a.z};
+      const float qmn[3] = {g.a.w, g.b.x, g.b.y}, qmx[3] = {g.b.z, g.b.w, g.mx2};
+      const float pinf = __uint_as_float(0x7f800000u), ninf = __uint_as_float(0xff800000u);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) { lt[a] = box_empty(); gt[a] = box_empty(); nl[a] = 0; ng[a] = 0; }
+      for (int a = 0; a < 3; ++a) {
+        const bool in = qc[a] < th[a];
+        nl[a] = in ? 1u : 0u;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          lt[a].mn[x] = in ? qmn[x] : pinf;
+          lt[a].mx[x] = in ? qmx[x] : ninf;
+          gt[a].mn[x] = in ? pinf : qmn[x];
+          gt[a].mx[x] = in ? ninf : qmx[x];
+        }
+      }
+    }
     const uint32_t maxlc = wave_max6(act ? lc : 0u);
     const auto fold = [&](const SmallGeo& g) {
       const float qc[3] = {g.a.x, g.a.y, g.a.z};
@@ -1336,58 +1507,53 @@ __device__ inline void small_pair(uint32_t ea, uint32_t eb, SmallObjs* S,
     };
     // two objects in flight: the next one's LDS reads are issued before the
     // current one is folded in; p0 + k <= 63 + 32 stays inside the arrays
-    const uint32_t p0 = act ? ls : lane;
-    SmallGeo g0 = geo_get(S, p0);
-    for (uint32_t k = 0; k < maxlc; k += 2) {
-      const SmallGeo g1 = geo_get(S, p0 + k + 1);
-      if (act && k < lc) fold(g0);
+    SmallGeo g1 = geo_get(S, p0 + 1);
+    for (uint32_t k = 1; k < maxlc; k += 2) {
+      const SmallGeo g0 = geo_get(S, p0 + k + 1);
+      if (act && k < lc) fold(g1);
       if (k + 1 >= maxlc) break;
-      g0 = geo_get(S, p0 + k + 2);
-      if (act && k + 1 < lc) fold(g1);
+      g1 = geo_get(S, p0 + k + 2);
+      if (act && k + 1 < lc) fold(g0);
     }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) ng[a] = act ? lc - nl[a] : 0u;
+    // costs (lib.rs:278-288) and the lane's best candidate
     uint64_t key = ~0ull;
     int ba = 0;
+    float bsl = 0.0f, bsg = 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      if (act && nl[a] >= 1 && ng[a] >= 1) {
+      const uint32_t ng = lc - nl[a];
+      if (act && nl[a] >= 1 && ng >= 1) {
         // both sides hold objects: their boxes are not empty
-        const float cst = split_cost(sa_p, box_sa_ne(lt[a]), box_sa_ne(gt[a]), nl[a], ng[a]);
+        const float sl = box_sa_ne(lt[a]), sg = box_sa_ne(gt[a]);
+        const float cst = split_cost(sa_p, sl, sg, nl[a], ng);
         const uint64_t k2 = ((uint64_t)tc_ukey(cst) << 32) | (uint32_t)(a * 32 + r);
-        if (k2 < key) { key = k2; ba = a; }
+        if (k2 < key) { key = k2; ba = a; bsl = sl; bsg = sg; }
       }
     }
     // first minimum over (cost, axis, candidate) of the node (lib.rs:515-524)
     const uint64_t skey = seg_min_u64(act ? key : ~0ull, lane, ls, last);
     const bool med = act && skey == ~0ull;
-    // the winner lane's candidate
-    Box mylt, mygt;
-#pragma unroll
-    for (int x = 0; x < 3; ++x) {
-      mylt.mn[x] = ba == 0 ? lt[0].mn[x] : (ba == 1 ? lt[1].mn[x] : lt[2].mn[x]);
-      mylt.mx[x] = ba == 0 ? lt[0].mx[x] : (ba == 1 ? lt[1].mx[x] : lt[2].mx[x]);
-      mygt.mn[x] = ba == 0 ? gt[0].mn[x] : (ba == 1 ? gt[1].mn[x] : gt[2].mn[x]);
-      mygt.mx[x] = ba == 0 ? gt[0].mx[x] : (ba == 1 ? gt[1].mx[x] : gt[2].mx[x]);
-    }
-    const uint32_t mynl = ba == 0 ? nl[0] : (ba == 1 ? nl[1] : nl[2]);
-    const float myth = ba == 0 ? th[0] : (ba == 1 ? th[1] : th[2]);
     const uint32_t q32 = (uint32_t)skey;
     const uint32_t wl = (act && !med) ? ls + (q32 & 31u) : lane;
     uint32_t axis = (act && !med) ? (q32 >> 5) : 0;
-    Box clt, cgt;
-#pragma unroll
-    for (int x = 0; x < 3; ++x) {
-      clt.mn[x] = __shfl(mylt.mn[x], (int)wl, 64);
-      clt.mx[x] = __shfl(mylt.mx[x], (int)wl, 64);
-      cgt.mn[x] = __shfl(mygt.mn[x], (int)wl, 64);
-      cgt.mx[x] = __shfl(mygt.mx[x], (int)wl, 64);
-    }
+    const uint32_t mynl = ba == 0 ? nl[0] : (ba == 1 ? nl[1] : nl[2]);
+    const float myth = ba == 0 ? th[0] : (ba == 1 ? th[1] : th[2]);
     uint32_t n_lt = __shfl(mynl, (int)wl, 64);
     const float tw = __shfl(myth, (int)wl, 64);
+    float sa_lt = __shfl(bsl, (int)wl, 64), sa_gt = __shfl(bsg, (int)wl, 64);
     uint32_t kind = 1, value = fbits(tw);
-    float cost = 0.0f;
-    bool pred = act && (axis == 0 ? o.c[0] : (axis == 1 ? o.c[1] : o.c[2])) < tw;
+    float cost = tc_ukey_inv((uint32_t)(skey >> 32));
+    bool pred = act && (axis == 0 ? c[0] : (axis == 1 ? c[1] : c[2])) < tw;
+    // the lane that writes the children's records, and their boxes
+    bool writer = act && !med && lane == wl;
+    Box wlt, wgt;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      wlt.mn[x] = ba == 0 ? lt[0].mn[x] : (ba == 1 ? lt[1].mn[x] : lt[2].mn[x]);
+      wlt.mx[x] = ba == 0 ? lt[0].mx[x] : (ba == 1 ? lt[1].mx[x] : lt[2].mx[x]);
+      wgt.mn[x] = ba == 0 ? gt[0].mn[x] : (ba == 1 ? gt[1].mn[x] : gt[2].mn[x]);
+      wgt.mx[x] = ba == 0 ? gt[0].mx[x] : (ba == 1 ? gt[1].mx[x] : gt[2].mx[x]);
+    }
 
     if (__ballot(med)) {
       // median fallback on axes 0,1,2, each reordering the list (lib.rs:444-491,
@@ -1399,7 +1565,7 @@ __device__ inline void small_pair(uint32_t ea, uint32_t eb, SmallObjs* S,
       Box mlt = box_empty(), mgt = box_empty();
       float mcost = 0.0f;
       for (uint32_t a = 0; a < 3; ++a) {
-        const float ca = a == 0 ? o.c[0] : (a == 1 ? o.c[1] : o.c[2]);
+        const float ca = a == 0 ? c[0] : (a == 1 ? c[1] : c[2]);
         const uint32_t kme = tc_ukey(ca);
         uint32_t rank = 0;
         for (uint32_t k = 0; k < maxlc; ++k) {
@@ -1411,26 +1577,29 @@ __device__ inline void small_pair(uint32_t ea, uint32_t eb, SmallObjs* S,
         const uint32_t np = !med ? lane
                                  : (sel ? ls + (uint32_t)__popcll(sm & segm & below)
                                         : ls + m + (uint32_t)__popcll(~sm & segm & below));
-        lane_move(S, lane, np, o);
+        lane_move(S, lane, np, c);
+        const Box ob = obj_box(S, lane);
         const bool inl = med && r < m;
-        const Box l2 = seg_box(inl, o.mn, o.mx, lane, ls, last);
-        const Box g2 = seg_box(med && !inl, o.mn, o.mx, lane, ls, last);
+        const Box l2 = seg_box(inl, ob.mn, ob.mx, lane, ls, last);
+        const Box g2 = seg_box(med && !inl, ob.mn, ob.mx, lane, ls, last);
         const float c2 = split_cost(sa_p, box_sa(l2), box_sa(g2), m, lc - m);
         const uint32_t k2 = tc_ukey(c2);
         if (!have || k2 < bk) { have = true; bk = k2; maxis = a; mlt = l2; mgt = g2; mcost = c2; }
       }
       if (med) {
-        axis = maxis; kind = 2; value = m; n_lt = m; clt = mlt; cgt = mgt; cost = mcost;
+        axis = maxis; kind = 2; value = m; n_lt = m; cost = mcost;
+        wlt = mlt; wgt = mgt;
+        sa_lt = box_sa(mlt);
+        sa_gt = box_sa(mgt);
         pred = r < m;
+        writer = lane == ls;
       }
     }
-    // the winner's cost, as its lane computed it, from the key
-    if (act && !med) cost = tc_ukey_inv((uint32_t)(skey >> 32));
     // a split always leaves both children non-empty; anything else is a bug and
     // stops this subtree rather than writing outside its slots
     const bool bad = act && (n_lt == 0 || n_lt >= lc);
     if (__ballot(bad)) {
-      if (bad) atomicOr(&ctr->err, ERR_INTERNAL);
+      if (bad) atomicOr(&S->hdr[h][H_ERR], 1u);
       break;
     }
 
@@ -1442,30 +1611,67 @@ __device__ inline void small_pair(uint32_t ea, uint32_t eb, SmallObjs* S,
       if (act && !med)
         np = pred ? ls + (uint32_t)__popcll(pm & segm & below)
                   : ls + n_lt + (uint32_t)__popcll(~pm & segm & below);
-      if (__ballot(act && !med)) lane_move(S, lane, np, o);
+      if (__ballot(act && !med)) lane_move(S, lane, np, c);
+    }
+    const uint32_t ngt = lc - n_lt;
+    if (writer) {
+      // children (lib.rs:648-663): gt at P, lt at P + 1, 64 contiguous bytes
+      const uint32_t* hd = S->hdr[h];
+      const uint32_t P0 = hd[H_P0], nbase = hd[H_NBASE];
+      const uint32_t start = hd[H_BASE] + (ls - off) - hd[H_OBASE];
+      put_child(S, h, P - P0, wgt, start + n_lt, ngt, P + 2, nbase, depth + 1, max_depth);
+      put_child(S, h, P + 1 - P0, wlt, start, n_lt, P + 2 + 2 * (ngt - 1), nbase, depth + 1,
+                max_depth);
+      if (sp)
+        sp[64 * h + (idx == hd[H_ROOT] ? ROOT_REC : idx - P0)] =
+            make_uint4(axis | (kind << 8), value, n_lt, fbits(cost));
+      if (kind == 2) atomicAdd(&S->hdr[h][H_NMED], 1u);
     }
     if (act) {
-      if (lane == ls) {
-        out.write_node(idx, nb, base + ls - off - obase, P - nbase, lc);
-        out.write_split(idx, axis | (kind << 8), value, n_lt, fbits(cost));
-        if (kind == 2) atomicAdd(&ctr->n_median, 1u);
-      }
-      // children (lib.rs:648-663): gt at P, lt at P + 1
-      const uint32_t ng2 = lc - n_lt;
       if (lane < ls + n_lt) {
         lc = n_lt;
         idx = P + 1;
-        P = P + 2 + 2 * (ng2 - 1);
-        nb = clt;
+        P = P + 2 + 2 * (ngt - 1);
+        sa_p = sa_lt;
       } else {
         ls = ls + n_lt;
-        lc = ng2;
+        lc = ngt;
         idx = P;
         P = P + 2;
-        nb = cgt;
+        sa_p = sa_gt;
       }
       ++depth;
+      has = lc > 1 && depth < max_depth;
     }
+  }
+  // every object's final position is the lane it ended in; the lane that
+  // loaded it learns that position through LDS and stages its output there
+  uint32_t* fin = reinterpret_cast<uint32_t*>(S->m2);  // object data is no longer needed
+  const uint32_t from = S->id[lane];
+  wave_sync();
+  fin[from] = lane;
+  wave_sync();
+  if (hv[H_OWN] && hl < hv[H_OWN]) {
+    const uint32_t f = fin[lane];
+    S->operm[f] = oid - hv[H_OBASE];
+    S->otri[f][0] = tri0;
+    S->otri[f][1] = tri1;
+    S->otri[f][2] = tri2;
+  }
+  wave_sync();  // the next pair's flush reads the staged output
+  if (forced) {
+    // leaf forced by max_depth (lib.rs:556) holding more than 32 objects
+    const SNode sn = small[e];
+    const Rec* src = sn.parity ? rec1 : rec0;
+    if (hl == 0) {
+      Box fb;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) { fb.mn[x] = sn.bmn[x]; fb.mx[x] = sn.bmx[x]; }
+      out.write_node(sn.idx, fb, sn.start - sn.obase, 0, sn.count);
+      put_gap(ctr, gaps, sn.P, sn.count);
+    }
+    for (uint32_t k = hl; k < sn.count; k += 32)
+      put_perm(sn.start + k, src[sn.start + k].id, sn.obase, perm, tris_in, tris_out);
   }
 }
 
@@ -1476,10 +1682,13 @@ __device__ inline void small_pair(uint32_t ea, uint32_t eb, SmallObjs* S,
 // largest node, and a pair as many levels as its deeper subtree).  Small builds
 // take windows of 2 (plain pairs): there a window's pairs, built one after
 // another, would lengthen the launch.  Only wave-local synchronisation inside.
-constexpr uint32_t SMALL_WIN = 8;  // windows of 16 and 32 measured slower: coarser work items  Five wavefronts per SIMD
-// (at most 96 VGPRs; the compiler keeps ~60 bytes per lane of pair set-up
-// state in scratch, outside the candidate loop) beat four at 115 VGPRs.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_small(const SNode* __restrict__ small,
+// Four wavefronts per SIMD: the pair state and the candidate loop fit in 128
+// VGPRs without scratch (at five, spills put vector memory operations, which
+// retire in order behind the stores, into the loop and the flush).
+// SPLITS: split records are staged too (more LDS, fewer resident blocks).
+constexpr uint32_t SMALL_WIN = 8;  // windows of 16 and 32 measured slower: coarser work items
+template <bool SPLITS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_small(const SNode* __restrict__ small,
                                                const uint32_t* __restrict__ nS_dev,
                                                const Rec* __restrict__ rec0,
                                                const Rec* __restrict__ rec1, NodeOut out,
@@ -1490,9 +1699,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                                                uint32_t win) {
   __shared__ SmallObjs lds_obj[4];
   __shared__ uint32_t lds_order[4][SMALL_WIN];
+  __shared__ uint4 lds_split[SPLITS ? 4 * 128 : 1];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t lane = lane_id();
   const uint32_t nS = *nS_dev;
+  SmallObjs* S = &lds_obj[w];
+  uint4* sp = SPLITS ? &lds_split[128 * w] : nullptr;
+  if ((lane & 31) == 0) S->hdr[lane >> 5][H_OWN] = 0;  // nothing staged yet
+  wave_sync();
   for (uint32_t wi = blockIdx.x * 4 + w; wi * win < nS; wi += gridDim.x * 4) {
     const uint32_t base = wi * win;
     const uint32_t cnt = min(win, nS - base);
@@ -1507,11 +1721,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     for (uint32_t q = 0; q < cnt; q += 2) {
       const uint32_t ea = lds_order[w][q];
       const uint32_t eb = q + 1 < cnt ? lds_order[w][q + 1] : nS;
-      small_pair(ea, eb, &lds_obj[w], small, nS, rec0, rec1, out, perm, tris_in, tris_out, ctr,
-                 gaps, max_depth);
+      small_pair(ea, eb, S, sp, small, nS, rec0, rec1, out, perm, tris_in, tris_out, ctr, gaps,
+                 max_depth);
     }
     wave_sync();  // the next window's order overwrites this one
   }
+  small_flush(S, sp, out, perm, tris_out, ctr, gaps);
 }
 
 // ------------------------------------------------------------------ compaction
@@ -1868,7 +2083,8 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
   // every wavefront of the grid with a window; plain pairs otherwise
   const uint32_t win =
       (uint64_t)n / (32u * SMALL_WIN) >= 4ull * g_small ? SMALL_WIN : 2u;
-  k_small<<<g_small, 256, 0, st>>>(small, &ctr->n_small, recs[0], recs[1], out, o.perm,
+  (out.splits ? k_small<true> : k_small<false>)<<<g_small, 256, 0, st>>>(
+      small, &ctr->n_small, recs[0], recs[1], out, o.perm,
                                    in.kind == 0 ? in.tris : nullptr,
                                    in.kind == 0 ? o.tris_out : nullptr, ctr, gaps, s->max_depth,
                                    win);
