@@ -167,6 +167,22 @@ int mgbvh_build_tris_device(mgbvh_ctx* ctx, const float* d_positions, uint32_t v
                             mgbvh_split_rec* d_splits, uint64_t* out_nnodes,
                             mgbvh_stats* out_stats);
 
+/* Allocation-free variant of mgbvh_build_tris_device.  It has no counterpart
+ * in the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * scratch buffer of the build lives in one static device array per GPU
+ * (about 24 MB, held until the process ends), so the call allocates no device
+ * or pinned memory and creates no HIP object.  At most 65536 triangles
+ * (MGBVH_ERR_INVALID_ARG beyond), plain method only, no profiling JSON and no
+ * statistics.  All calls on one GPU share that array: calls that overlap on
+ * one GPU race.  Same arguments and outputs as mgbvh_build_tris_device. */
+int mgbvh_build_tris_device_noalloc(mgbvh_ctx* ctx, const float* d_positions,
+                                    uint32_t vstride_bytes, uint64_t nverts,
+                                    const uint32_t* d_tris, uint64_t ntris,
+                                    const mgbvh_settings* settings, mgbvh_node32* d_nodes,
+                                    uint32_t* d_lens, uint32_t* d_perm, uint32_t* d_tris_out,
+                                    mgbvh_split_rec* d_splits, uint64_t* out_nnodes);
+
 int mgbvh_build_aabbs_device(mgbvh_ctx* ctx, const mgbvh_aabb* d_aabbs, uint64_t n,
                              const mgbvh_settings* settings, mgbvh_node32* d_nodes,
                              uint32_t* d_lens, uint32_t* d_perm, mgbvh_split_rec* d_splits,

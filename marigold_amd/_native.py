@@ -194,6 +194,8 @@ _EXTRA: dict = {
     "mgbvh_trace_wide8_device": ([_P, _P, _U64, _P, _P, _U32, _P, _P, _P, _U64, _P], _I),
     "mgbvh_gen_primary_rays_device": ([_P, C.POINTER(Camera), _U32, _U32, _P], _I),
     "mgbvh_copy_d2d": ([_P, _P, _P, _U64], _I),
+    "mgbvh_build_tris_device_noalloc": ([_P, _P, _U32, _U64, _P, _U64, C.POINTER(Settings), _P, _P,
+                                         _P, _P, _P, C.POINTER(_U64)], _I),
     "mgbvh_instance_boxes_device": ([_P, _P, _P, _U32, _P, _P, _U32, _P], _I),
     "mgbvh_model_create": ([_P, _U64, C.POINTER(_P)], _I),
     "mgbvh_model_destroy": ([_P], _I),

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmgbvh.so")
 
 SOURCES = ["mgbvh_build.hip", "mgbvh_api.hip", "mgbvh_trace.hip", "mgbvh_scene.hip", "mgbvh_learned.hip",
-           "mgbvh_wide8.hip"]
+           "mgbvh_wide8.hip", "mgbvh_noalloc.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

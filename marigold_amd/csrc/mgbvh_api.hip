@@ -28,6 +28,10 @@ int hip_fail(hipError_t e, const char* what) {
 int ensure(DevBuf& b, size_t bytes) {
   if (bytes == 0) bytes = 16;
   if (b.cap >= bytes) return MGBVH_OK;
+  if (b.fixed) {
+    set_error("fixed scratch region too small (internal sizing error)");
+    return MGBVH_ERR_INTERNAL;
+  }
   if (b.p) {
     hipError_t e = hipFree(b.p);
     if (e != hipSuccess) return hip_fail(e, "hipFree");
@@ -75,7 +79,7 @@ int prof_collect(mgbvh_ctx* c) {
 }
 
 static void free_buf(DevBuf& b) {
-  if (b.p) (void)hipFree(b.p);
+  if (b.p && !b.fixed) (void)hipFree(b.p);
   b.p = nullptr;
   b.cap = 0;
 }
@@ -290,6 +294,8 @@ int mgbvh_ctx_create(int device, mgbvh_ctx** out) {
   e = hipHostMalloc((void**)&c->h_ctr, sizeof(Counters), hipHostMallocDefault);
   if (e != hipSuccess) { (void)mgbvh_ctx_destroy(c); return hip_fail(e, "hipHostMalloc"); }
   c->sc.h_ctr = c->h_ctr;
+  int rc = noalloc_init(device);  // once per device, with the device current
+  if (rc) { (void)mgbvh_ctx_destroy(c); return rc; }
   *out = c;
   return MGBVH_OK;
 }

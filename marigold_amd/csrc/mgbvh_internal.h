@@ -171,6 +171,7 @@ __device__ inline void NodeOut::write_node(uint32_t idx, const Box& b, uint32_t 
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  bool fixed = false;  // a view into caller-provided memory: never grown or freed
 };
 
 namespace mg {
@@ -266,6 +267,9 @@ namespace mg {
 void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 int ensure(DevBuf& b, size_t bytes);
+// Address of this device's allocation-free build scratch (set up by
+// mgbvh_ctx_create with the device current); 0 when unavailable.
+int noalloc_init(int device);
 int build_device(mgbvh_ctx* c, const BuildInput& in, uint32_t n, const mgbvh_settings* s,
                  const BuildOutput& o, BuildResult* res);
 // Many object lists (meshes) built together: each gets the tree build_device
