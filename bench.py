@@ -74,6 +74,31 @@ def kernel_bytes(kind: str, n: int, nverts: int, vstride: int, t: dict) -> float
     return 0.0
 
 
+def survey_bytes(n: int, nverts: int, s_large: int, n_small_objects: int) -> float:
+    """Algorithmic bytes of a batched BLAS build by the path's byte model:
+    per BLAS 12N (indices) + 12V (positions) + 24N (object
+    box write) + 32 per object of every internal node with > 32 objects +
+    28 per object of every <= 32-object subtree root + 32 (2N - 1) (node
+    write) + 24N (permuted triangle read + write), summed over the meshes."""
+    return (12.0 * n + 12.0 * nverts + 24.0 * n + 32.0 * s_large + 28.0 * n_small_objects
+            + 32.0 * (2 * n) + 24.0 * n)
+
+
+def measured_valu(kernel: str, path: str = os.path.join(ROOT, "profiles",
+                                                        "small_subtree_sq_counters.md")):
+    """VALU issue of `kernel` from the committed SQ counter summary, or None:
+    {"insts_per_launch", "issue_frac"} (wave64 VALU instructions x 4 cycles over
+    the SIMD cycles of the launch)."""
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) == 4 and f[0] == "valu_per_launch" and f[1] == kernel:
+                return {"insts_per_launch": float(f[2]), "issue_frac": float(f[3])}
+    except OSError:
+        pass
+    return None
+
+
 def measured_traffic(kernel: str, path: str = os.path.join(ROOT, "profiles",
                                                            "scene_build_traffic.md")):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
@@ -124,13 +149,32 @@ def cpu_baseline(pos, tris, budget_s: float = 12.0):
                       f"of marigold bvh (median {med:.3f} s; AABB setup excluded)"}
 
 
+def host_cores() -> dict:
+    """What this host shows: logical CPUs, the ones this process may run on,
+    and the CPU share given to one GPU's jobs (OMP_NUM_THREADS)."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    share = os.environ.get("OMP_NUM_THREADS")
+    return {"cpu_count": os.cpu_count(), "affinity": affinity,
+            "omp_num_threads": int(share) if share and share.isdigit() else None}
+
+
+def cpu_pool_threads() -> int:
+    """Threads of the per-mesh pool: the CPU share of this GPU's jobs
+    (OMP_NUM_THREADS when set), else the cores this process may use."""
+    hc = host_cores()
+    return hc["omp_num_threads"] or hc["affinity"] or hc["cpu_count"] or 1
+
+
 def cpu_baseline_scene(meshes, budget_s: float = 15.0, threads: int = 0):
     """Per-mesh oracle builds on a thread pool (one mesh per task, as rayon's
-    par_iter over meshes in model.rs:76-95), over a random sample of meshes
-    until the budget is spent."""
+    par_iter over meshes in model.rs:76-95, with the rayon pool's one worker
+    per core), over a random sample of meshes until the budget is spent."""
     from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle as O  # CPU baseline only
-    threads = threads or min(16, os.cpu_count() or 1)
+    threads = threads or cpu_pool_threads()
     rng = np.random.default_rng(0)
     order = rng.permutation(len(meshes))
     done_tris = 0
@@ -163,7 +207,8 @@ def cpu_baseline_scene(meshes, budget_s: float = 15.0, threads: int = 0):
     return {"value": done_tris / wall / 1e6, "unit": "Mtri/s", "cores": threads, "kind": "port",
             "sample": f"{n_done} randomly chosen meshes of the scene ({done_tris} tris) built by the C "
                       f"restatement of marigold bvh, one mesh per task on {threads} threads, "
-                      f"{wall:.1f} s wall; AABB setup included, TLAS excluded"}
+                      f"{wall:.1f} s wall; AABB setup included, TLAS excluded",
+            "host": host_cores()}
 
 
 def _hits_of(ctx, sc_or_trace, rays_dev, n):
@@ -300,8 +345,9 @@ def scene_main(args):
     dom_launches = kt[1] / args.steps
     dom_bytes = kernel_bytes(dominant, n, nv, 12, terms) / max(dom_launches, 1)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
-    build_bytes = sum(kernel_bytes(k, n, nv, 12, terms) for k in ("setup", "bin", "partition",
-                                                                   "sweep", "small"))
+    impl_bytes = sum(kernel_bytes(k, n, nv, 12, terms) for k in ("setup", "bin", "partition",
+                                                                  "sweep", "small"))
+    build_bytes = survey_bytes(n, nv, terms["s_large"], n)
     # the whole scene build as one graph replay: the timed region of `value`
     # (--no-graph: level-by-level builds instead, for profiler runs)
     if not args.no_graph:
@@ -366,7 +412,7 @@ def scene_main(args):
                                 & (hits_shader["inst"] == hits_plain["inst"])))
     log(f"traversal of {w * h} rays: " + ", ".join(f"{k} {v:.2f} ms" for k, v in trace_ms.items())
         + f"; shader-exact hits on the same triangle as the tree-independent ones: {shader_same:.5f}")
-    bat_sum = int(bat.download("blas_nodes", np.uint32, 8 * bat.n_slots).astype(np.uint64).sum())
+    bat_nodes = bat.download("blas_nodes", np.uint32, 8 * bat.n_slots)
     log(f"batched plain {ms_bat:.2f} ms/scene ({'level by level' if args.no_graph else 'graph'}), "
         f"SAH {sah_plain:.4f}")
     bat.free()
@@ -383,23 +429,26 @@ def scene_main(args):
         ctx.record(3)
         ctx.synchronize()
         ms_plain = ctx.elapsed_ms(2, 3) / args.plain_steps
-        same_nodes = bat_sum == int(pla.download("blas_nodes", np.uint32, 8 * pla.n_slots)
-                                    .astype(np.uint64).sum())
+        same_nodes = bool(np.array_equal(bat_nodes,
+                                         pla.download("blas_nodes", np.uint32, 8 * pla.n_slots)))
         log(f"plain per mesh {ms_plain:.2f} ms/scene")
         pla.free()
     d_rays.free()
+    del bat_nodes
 
     full = None
     if not args.no_rays:
         log("16M-ray hit comparison on the heightfield")
         full = heightfield_16m_rays(ctx, learned)
     if args.calibrate_copy:
-        # a known-size device copy for scaling the FETCH_SIZE / WRITE_SIZE counters
-        nbytes = 1 << 30
-        a, b = ctx.alloc(nbytes), ctx.alloc(nbytes)
-        N.check(N.lib().mgbvh_copy_d2d(ctx.handle, b.ptr, a.ptr, nbytes))
-        a.free()
-        b.free()
+        # known-size device copies (256 MiB, then 1 GiB) for scaling the
+        # FETCH_SIZE / WRITE_SIZE counters
+        for nbytes in (1 << 28, 1 << 30):
+            a, b = ctx.alloc(nbytes), ctx.alloc(nbytes)
+            N.check(N.lib().mgbvh_copy_d2d(ctx.handle, b.ptr, a.ptr, nbytes))
+            ctx.synchronize()
+            a.free()
+            b.free()
     model.close()
     out = {
         "metric": "BVH build Mtri/s, plain vs learned+batched, at SAH <=1.03x plain; 1 MI355X",
@@ -444,11 +493,17 @@ def scene_main(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
                      "kernel": dominant, "kernel_ms_per_launch": dom_ms,
+                     "issue": measured_valu(dominant),
                      "launches_per_build": dom_launches, "alg_bytes_per_launch": dom_bytes,
                      "timed": "level-by-level builds of the same scene, HIP event pairs around "
                               "each launch of the kernel class in the BLAS build, on the build "
                               "stream"},
         "build_alg_gbs": build_bytes / (ms_step * 1e-3) / 1e9,
+        "build_alg_bytes": build_bytes,
+        "build_alg_model": "12N + 12V + 24N + 32 s_large + 28 N + 64N + 24N over the BLASes "
+                           "(s_large: objects of the > 32-object internal nodes)",
+        "build_impl_gbs": impl_bytes / (ms_step * 1e-3) / 1e9,
+        "build_impl_bytes": impl_bytes,
         "kernel_ms_per_build": {k: v[0] / max(args.warmup, 1) for k, v in prof.items()},
         "work": terms,
     }
@@ -456,6 +511,10 @@ def scene_main(args):
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline_scene(meshes)
+        # RAYON_NUM_THREADS=1, the reference's own measurement setting (README.md:62-66)
+        one = cpu_baseline_scene(meshes, budget_s=10.0, threads=1)
+        out["cpu_baseline"]["one_thread"] = {"value": one["value"], "unit": one["unit"],
+                                             "sample": one["sample"]}
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

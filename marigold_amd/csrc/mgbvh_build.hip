@@ -1911,6 +1911,7 @@ int plan_build(mgbvh_ctx* c, Scratch& sc, const BuildInput& in, const std::vecto
     p.lp.window = s->learned_window ? s->learned_window : 3u;
     p.lp.guard = s->learned_guard;
     p.lp.no_network = s->learned_no_network;
+    p.lp.model = c->model;
     for (uint32_t i = 0; i < p.nm; ++i)
       if (p.rows[i].root_kind == 1 && p.rows[i].count >= p.lp.min_objects)
         p.rows[i].lrow = p.n_counted++;

@@ -70,6 +70,9 @@ struct LearnParams {
   uint32_t window;
   float guard;
   uint32_t no_network;   // windows centred on the best proxy cut (ablation)
+  // the network: the context's model of the call, or a scene plan's own copy
+  // of the model as it was when the plan was created
+  mgbvh_model* model = nullptr;
 };
 
 // A subtree root with <= 32 objects, or a leaf forced by max_depth.
@@ -316,6 +319,8 @@ int prof_collect(mgbvh_ctx* c);  // after a synchronize: accumulate and recycle
 // Learned method, one level: features of the counted nodes, network
 // inference, window and guard decision (LNode::mode -> window or fallback).
 // n_dev: the row count in device memory; cap: its bound (grid sizes).
+// A copy of the model's inference state for a scene plan (mgbvh_model_destroy frees it).
+int model_snapshot(mgbvh_ctx* c, mgbvh_model* src, mgbvh_model** out);
 int learned_decide(mgbvh_ctx* c, Scratch& sc, LNode* L, const uint32_t* rows,
                    const uint32_t* n_dev, uint32_t cap, const uint32_t* hist, Counters* ctr,
                    const LearnParams& lp);

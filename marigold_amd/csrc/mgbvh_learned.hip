@@ -228,16 +228,20 @@ __device__ inline void mlp_tile(int row0, int n, uint16_t* A, uint16_t* Bf, cons
   wave_lds_sync();  // the next tile overwrites A
 }
 
+constexpr size_t MLP_LDS_BYTES = (size_t)(W_ROWS + 4 * 2 * 16) * LDS_ROW * 2;
+
 // Logits of n feature rows (MLP_IN floats each, row stride `stride` floats);
 // n from n_dev when given.  Each block first copies the BF16 weights into LDS
 // (one read of 88 KB, then every MFMA operand comes from LDS); then one
-// wavefront per 16 rows, persistent over tiles.
+// wavefront per 16 rows, persistent over tiles.  The 130,560 bytes of LDS are
+// a static allocation (gfx950 gives a workgroup up to 160 KB), so launches
+// need no function attribute and no dynamic LDS size, in graphs too.
 __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ feat, int stride,
                                                    int n_host, const uint32_t* __restrict__ n_dev,
                                                    const uint16_t* __restrict__ wbf,
                                                    const float* __restrict__ params,
                                                    float* __restrict__ logits) {
-  extern __shared__ __align__(16) uint16_t lds[];
+  __shared__ __align__(16) uint16_t lds[MLP_LDS_BYTES / 2];
   uint16_t* Wl = lds;                                   // W_ROWS x LDS_ROW
   uint16_t* act = lds + W_ROWS * LDS_ROW;               // 4 waves x 2 x 16 x LDS_ROW
   const int w = threadIdx.x >> 6;
@@ -255,8 +259,6 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ fea
   for (int row0 = (blockIdx.x * 4 + w) * 16; row0 < n; row0 += gridDim.x * 64)
     mlp_tile(row0, n, A, Bf, Wl, feat, stride, params, logits);
 }
-
-constexpr size_t MLP_LDS_BYTES = (size_t)(W_ROWS + 4 * 2 * 16) * LDS_ROW * 2;
 
 // ------------------------------------------------------------ learned build step
 // One wavefront per network row (a counted node, listed when it was created):
@@ -474,16 +476,42 @@ int model_infer(mgbvh_ctx* c, mgbvh_model* m, const float* feat, int stride, int
   int rc = model_bf16(c, m);
   if (rc) return rc;
   // one block per compute unit at most (the weights fill most of its LDS)
-  hipError_t ea = hipFuncSetAttribute((const void*)k_mlp_infer,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)MLP_LDS_BYTES);
-  if (ea != hipSuccess) return hip_fail(ea, "hipFuncSetAttribute(k_mlp_infer)");
   const unsigned grid = std::max(1u, std::min<unsigned>(cdivu(n, 64), (unsigned)c->cus));
-  k_mlp_infer<<<grid, 256, MLP_LDS_BYTES, c->stream>>>(feat, stride, n, n_dev,
+  k_mlp_infer<<<grid, 256, 0, c->stream>>>(feat, stride, n, n_dev,
                                                        (const uint16_t*)m->wbf.p,
                                                        (const float*)m->params.p, logits);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MGBVH_OK : hip_fail(e, "k_mlp_infer");
+}
+
+// A copy of a model's inference state (FP32 parameters, whose biases the
+// inference reads, and the BF16 weights), owned by whoever keeps it: a scene
+// plan uses the model as it was when the plan was recorded, whatever happens
+// to the context's model afterwards.
+int model_snapshot(mgbvh_ctx* c, mgbvh_model* src, mgbvh_model** out) {
+  *out = nullptr;
+  int rc = model_bf16(c, src);
+  if (rc) return rc;
+  mgbvh_model* m = new mgbvh_model();
+  m->device = src->device;
+  if ((rc = ensure(m->params, (size_t)MLP_NPARAM * 4)) ||
+      (rc = ensure(m->wbf, (size_t)NW_BF16 * 2))) {
+    (void)mgbvh_model_destroy(m);
+    return rc;
+  }
+  hipError_t e = hipMemcpyAsync(m->params.p, src->params.p, (size_t)MLP_NPARAM * 4,
+                                hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(m->wbf.p, src->wbf.p, (size_t)NW_BF16 * 2, hipMemcpyDeviceToDevice,
+                       c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    (void)mgbvh_model_destroy(m);
+    return hip_fail(e, "model snapshot");
+  }
+  m->bf_valid = true;
+  *out = m;
+  return MGBVH_OK;
 }
 
 int learned_decide(mgbvh_ctx* c, Scratch& sc, LNode* L, const uint32_t* list,
@@ -500,9 +528,10 @@ int learned_decide(mgbvh_ctx* c, Scratch& sc, LNode* L, const uint32_t* list,
   k_feat<<<grid, 256, 0, st>>>(L, list, n_dev, hist, feat);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "k_feat");
-  if (!lp.no_network &&
-      (rc = model_infer(c, c->model, feat, MLP_IN, (int)cap, n_dev, logits)))
-    return rc;
+  if (!lp.no_network) {
+    if (!lp.model) { set_error("the learned method needs a model"); return MGBVH_ERR_UNSUPPORTED; }
+    if ((rc = model_infer(c, lp.model, feat, MLP_IN, (int)cap, n_dev, logits))) return rc;
+  }
   k_decide<<<grid, 256, 0, st>>>(L, list, n_dev, hist, lp.no_network ? nullptr : logits,
                                  lp.window, lp.guard, ctr);
   e = hipGetLastError();

@@ -230,6 +230,7 @@ struct mgbvh_scene_plan {
   std::vector<uint32_t> blas_counts;
   uint64_t tlas_nodes = 0;
   uint32_t n_redo = 0;  // launches rebuilt level by level (trees deeper than recorded)
+  mgbvh_model* model = nullptr;  // learned method: the model as it was at creation
 };
 
 namespace mg {
@@ -270,10 +271,10 @@ static void plan_release(mgbvh_scene_plan* pl) {
     if (b->p) (void)hipFree(b->p);
   if (pl->bsc.h_ctr) (void)hipHostFree(pl->bsc.h_ctr);
   if (pl->tsc.h_ctr) (void)hipHostFree(pl->tsc.h_ctr);
+  if (pl->model) (void)mgbvh_model_destroy(pl->model);
   delete pl;
 }
 
-int model_bf16(mgbvh_ctx* c, mgbvh_model* m);
 }  // namespace mg
 
 extern "C" {
@@ -369,9 +370,13 @@ int mgbvh_scene_plan_create(mgbvh_ctx* c, const float* d_pos, uint32_t vstride_b
        (e = hipMemcpyAsync(pl->tsc.mrows.p, pl->tp.rows.data(), pl->tp.rows.size() * sizeof(MeshRow),
                            hipMemcpyHostToDevice, st))))
     return fail(hip_fail(e, "plan upload"));
-  if (blas_settings->method == MGBVH_METHOD_LEARNED && !blas_settings->learned_no_network &&
-      (rc = model_bf16(c, c->model)))
-    return fail(rc);
+  // the captured kernels and a level-by-level redo both use the plan's own
+  // copy of the model, so later training, set_params, set_model or destroy of
+  // the context's model change nothing in this plan
+  if (blas_settings->method == MGBVH_METHOD_LEARNED && !blas_settings->learned_no_network) {
+    if ((rc = model_snapshot(c, c->model, &pl->model))) return fail(rc);
+    pl->bp.lp.model = pl->model;
+  }
   if ((e = hipStreamSynchronize(st))) return fail(hip_fail(e, "plan upload"));
   // one build level by level gives each tree's level count; the graph runs
   // that many levels plus a margin (launch redoes a replay that needed more)

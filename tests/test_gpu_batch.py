@@ -303,3 +303,59 @@ def test_small_windows_same_trees(ctx):
         ctx.set_tuning(**{k: v for k, v in old.items() if k != "compute_units"})
     for k in ("nodes", "lens", "splits", "perm", "tris"):
         assert np.array_equal(np.asarray(win[k]).view(np.uint8), np.asarray(base[k]).view(np.uint8)), k
+
+
+def test_learned_plan_keeps_its_model(ctx):
+    """A learned scene plan builds with the model as it was when the plan was
+    recorded: after set_params, and after set_model(None), its replays (and a
+    replay that falls back to a level-by-level rebuild) give the trees of the
+    original parameters."""
+    print("[case] learned scene plan keeps its model", flush=True)
+    from marigold_amd.scene import GpuScene
+    sc = scenes.batched_scene(n_meshes=30, n_instances=100, lo=200, hi=20_000, seed=14)
+    meshes = list(zip(sc["positions"], sc["tris"]))
+    learned = dict(learned_min_objects=64, learned_window=1, learned_guard=10.0)
+    model = N.Model(ctx, seed=5)
+    p0 = model.params()
+    ctx.set_model(model)
+    args = (ctx, meshes, sc["transforms"], sc["instance_mesh"])
+    kw = dict(mode="batched", method=N.METHOD_LEARNED, learned=learned)
+    ref = GpuScene(*args, **kw)
+    want = _scene_arrays(ref, None)
+    # geometry whose trees need more levels than the plan records (redo path)
+    big = int(np.argmax([t.shape[0] for _, t in meshes]))
+    deep = [(p.copy(), t) for p, t in meshes]
+    k = np.arange(24)
+    q = deep[big][0]
+    q[k * 7 % q.shape[0], 0] = np.float32(4.0) ** (k + 1).astype(np.float32)
+    ref_deep = GpuScene(ctx, deep, sc["transforms"], sc["instance_mesh"], **kw)
+    want_deep = _scene_arrays(ref_deep, None)
+    g = GpuScene(*args, build=False, **kw)
+    g.record_graph()
+    g.build()
+    got = _scene_arrays(g, None)
+    for key in want:
+        assert np.array_equal(got[key].view(np.uint32), want[key].view(np.uint32)), ("first", key)
+    # other parameters change the context's trees ...
+    rng = np.random.default_rng(3)
+    model.set_params((p0 + rng.normal(0.0, 1.0, p0.shape)).astype(np.float32))
+    other = GpuScene(*args, **kw)
+    changed = _scene_arrays(other, None)
+    assert not np.array_equal(changed["blas_nodes"].view(np.uint32),
+                              want["blas_nodes"].view(np.uint32))
+    # ... but not the plan's
+    g.build()
+    got = _scene_arrays(g, None)
+    for key in want:
+        assert np.array_equal(got[key].view(np.uint32), want[key].view(np.uint32)), ("params", key)
+    # no model on the context: replays and a level-by-level redo still build
+    ctx.set_model(None)
+    g.d["pos"].upload(np.ascontiguousarray(np.concatenate([p for p, _ in deep])))
+    g.build()
+    assert g.plan.info()["launches_rebuilt"] == 1, g.plan.info()
+    got = _scene_arrays(g, None)
+    for key in want_deep:
+        assert np.array_equal(got[key].view(np.uint32), want_deep[key].view(np.uint32)), ("deep", key)
+    for s_ in (ref, ref_deep, g, other):
+        s_.free()
+    model.close()
