@@ -87,8 +87,8 @@ def survey_bytes(n: int, nverts: int, s_large: int, n_small_objects: int) -> flo
 def measured_valu(kernel: str, path: str = os.path.join(ROOT, "profiles",
                                                         "small_subtree_sq_counters.md")):
     """VALU issue of `kernel` from the committed SQ counter summary, or None:
-    {"insts_per_launch", "issue_frac"} (wave64 VALU instructions x 4 cycles over
-    the SIMD cycles of the launch)."""
+    {"insts_per_launch", "issue_frac"} (wave64 VALU instructions x 2 SIMD cycles
+    each, over the SIMD cycles of the launch)."""
     try:
         for line in open(path):
             f = line.split()
