@@ -300,10 +300,13 @@ class DeviceBuffer:
         assert a.nbytes <= self.nbytes
         check(lib().mgbvh_copy_h2d(self.ctx.handle, self.ptr, ptr(a), a.nbytes))
 
-    def download(self, dtype, count) -> np.ndarray:
+    def download(self, dtype, count, offset: int = 0) -> np.ndarray:
+        """`count` elements of `dtype` starting at element `offset`."""
         out = np.empty(count, dtype=dtype)
-        assert out.nbytes <= self.nbytes
-        check(lib().mgbvh_copy_d2h(self.ctx.handle, ptr(out), self.ptr, out.nbytes))
+        start = int(offset) * out.itemsize
+        assert start + out.nbytes <= self.nbytes
+        check(lib().mgbvh_copy_d2h(self.ctx.handle, ptr(out), C.c_void_p(self.ptr.value + start),
+                                   out.nbytes))
         return out
 
     def free(self):
