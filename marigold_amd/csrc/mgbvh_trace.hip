@@ -8,19 +8,25 @@
 // (instance, triangle) index.  The reference's Hit.merge
 // (intersect.slang:57-82) keeps whichever equal hit was tested later, which
 // depends on the tree.  The slab test keeps a box that holds the closest hit
-// even when Moller-Trumbore's rounding puts a grazing hit a few ulps outside
-// the triangle's box: each box is padded by 2^-20 x max(|min|, |max|) per axis
-// (on an axis the ray is nearly parallel to, a few ulps in space are far in t),
-// far distances are widened by 1 + 2^-20, near distances compared against the
-// closest t widened by 1 + 2^-10, and an axis whose plane distance is 0 * inf
-// (a ray inside a face plane) left unbounded.
+// even when Moller-Trumbore's rounding puts a hit outside the triangle's box.
+// That error grows with |o - v1| (the hit is computed from s = o - v1), with
+// 1/cos of the angle between the ray and the triangle's plane, and, through
+// the edges, with the vertices' magnitude; so each box is padded per axis by
+// max(2^-20 x max(|min|, |max|), 2^-16 x D), D the largest |min - o| or
+// |max - o| over the three axes.  2^-16 holds on a stress set of far origins
+// (distance 30 to 1e7 from a unit scene) with rays nearly parallel to a face,
+// where 2^-17 already culls a few closest hits.  Far distances are widened by
+// 1 + 2^-20, near distances compared against the closest t widened by
+// 1 + 2^-10, and an axis whose plane distance is 0 * inf (a ray inside a face
+// plane) left unbounded.
 #include "mgbvh_common.h"
 #include "mgbvh_internal.h"
 
 namespace mg {
 
 constexpr float TF_WIDEN = 1.00000095367431640625f;  // 1 + 2^-20
-constexpr float BOX_PAD = 9.5367431640625e-07f;         // 2^-20 of the box's coordinate size
+constexpr float BOX_PAD = 9.5367431640625e-07f;      // 2^-20 of the box's coordinate size
+constexpr float DIST_PAD = 1.52587890625e-05f;       // 2^-16 of the origin-to-box distance
 constexpr float TN_SLACK = 1.0009765625f;            // 1 + 2^-10
 constexpr int TRACE_STACK = 64;                      // > tree height + 1
 constexpr uint32_t NONE = 0xffffffffu;
@@ -44,11 +50,19 @@ __device__ inline void set_inv(RayD& r) {
 __device__ inline bool slab(const RayD& r, const float* bmn, const float* bmx, float best,
                             float& tn_out) {
   float tn = -f_inf(), tf = f_inf();
+  float lo[3], hi[3], dist = 0.0f;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float pad = __fmul_rn(fmaxf(fabsf(bmn[a]), fabsf(bmx[a])), BOX_PAD);
-    const float t0 = __fmul_rn(__fsub_rn(__fsub_rn(bmn[a], pad), r.o[a]), r.inv[a]);
-    const float t1 = __fmul_rn(__fsub_rn(__fadd_rn(bmx[a], pad), r.o[a]), r.inv[a]);
+    lo[a] = __fsub_rn(bmn[a], r.o[a]);
+    hi[a] = __fsub_rn(bmx[a], r.o[a]);
+    dist = fmaxf(dist, fmaxf(fabsf(lo[a]), fabsf(hi[a])));
+  }
+  const float dp = __fmul_rn(dist, DIST_PAD);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float pad = fmaxf(__fmul_rn(fmaxf(fabsf(bmn[a]), fabsf(bmx[a])), BOX_PAD), dp);
+    const float t0 = __fmul_rn(__fsub_rn(lo[a], pad), r.inv[a]);
+    const float t1 = __fmul_rn(__fadd_rn(hi[a], pad), r.inv[a]);
     // 0 * inf: the ray runs inside a face plane, so this axis does not bound t
     const bool free_axis = (t0 != t0) | (t1 != t1);
     tn = fmaxf(tn, free_axis ? -f_inf() : fminf(t0, t1));

@@ -367,12 +367,22 @@ __global__ __launch_bounds__(256) void k_w8_emit(
 // ------------------------------------------------------------------ traversal
 __device__ inline bool w8_slab(const float* o, const float* inv, const float* bmn,
                                const float* bmx, float best, float& tn_out) {
+  // the binary traversal's pad: max(2^-20 x the box's coordinate size,
+  // 2^-16 x the farthest box coordinate from the origin)
   float tn = -__uint_as_float(0x7f800000u), tf = __uint_as_float(0x7f800000u);
+  float lo[3], hi[3], dist = 0.0f;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float pad = __fmul_rn(fmaxf(fabsf(bmn[a]), fabsf(bmx[a])), 9.5367431640625e-07f);
-    const float t0 = __fmul_rn(__fsub_rn(__fsub_rn(bmn[a], pad), o[a]), inv[a]);
-    const float t1 = __fmul_rn(__fsub_rn(__fadd_rn(bmx[a], pad), o[a]), inv[a]);
+    lo[a] = __fsub_rn(bmn[a], o[a]);
+    hi[a] = __fsub_rn(bmx[a], o[a]);
+    dist = fmaxf(dist, fmaxf(fabsf(lo[a]), fabsf(hi[a])));
+  }
+  const float dp = __fmul_rn(dist, 1.52587890625e-05f);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float pad = fmaxf(__fmul_rn(fmaxf(fabsf(bmn[a]), fabsf(bmx[a])), 9.5367431640625e-07f), dp);
+    const float t0 = __fmul_rn(__fsub_rn(lo[a], pad), inv[a]);
+    const float t1 = __fmul_rn(__fadd_rn(hi[a], pad), inv[a]);
     const bool free_axis = (t0 != t0) | (t1 != t1);
     tn = fmaxf(tn, free_axis ? -__uint_as_float(0x7f800000u) : fminf(t0, t1));
     tf = fminf(tf, free_axis ? __uint_as_float(0x7f800000u) : fmaxf(t0, t1));

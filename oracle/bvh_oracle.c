@@ -38,7 +38,8 @@
  *   1: adaptive sweep with S = 32 thresholds for every n (lib.rs:430)
  *   2: adaptive thresholds without the + 0.5 (lib.rs:434)
  *   3: tree SAH leaf term without len
- *   4: tree SAH internal term weighted by OBJECT_COST instead of DEPTH_COST */
+ *   4: tree SAH internal term weighted by OBJECT_COST instead of DEPTH_COST
+ *   5: traversal box pad without the origin-distance term */
 #ifndef OR_PLANT
 #define OR_PLANT 0
 #endif
@@ -628,25 +629,39 @@ void or_closest_hit_brute(const float* pos, uint32_t vs, const uint32_t* tris, u
 }
 
 /* Slab test of the traversal (not the reference's box test, which may cull a
- * box holding the closest hit by rounding): each box is padded by
- * 2^-20 x max(|min|, |max|) per axis (a hit point off by a few ulps of its
- * coordinates, which on an axis the ray is nearly parallel to is far off in t),
- * the far distance is widened by (1 + 2^-20), an axis whose plane distance is
- * 0 * inf (the ray runs inside a
- * face plane) does not bound t, and the near distance must not exceed the
- * current closest t widened by (1 + 2^-10). */
+ * box holding the closest hit by rounding).  Moller-Trumbore works on
+ * s = o - v1, so the hit it reports can lie off the float ray by a rounding
+ * error that grows with |o - v1| (and with 1/cos of the angle between the ray
+ * and the triangle's plane) and, through the edges v2 - v1 and v3 - v1, with
+ * the vertices' own magnitude.  Each box is therefore padded per axis by
+ *   max(2^-20 x max(|min|, |max|), 2^-16 x D),
+ * D the largest |min - o| or |max - o| over the three axes (the farthest a
+ * vertex inside the box can be from the origin, per coordinate), the far
+ * distance is widened by (1 + 2^-20), an axis whose plane distance is
+ * 0 * inf (the ray runs inside a face plane) does not bound t, and the near
+ * distance must not exceed the current closest t widened by (1 + 2^-10). */
 #define OR_TF_WIDEN 1.00000095367431640625f /* 1 + 2^-20 */
-#define OR_BOX_PAD 9.5367431640625e-07f    /* 2^-20: box padded by this x max(|min|, |max|) */
+#define OR_BOX_PAD 9.5367431640625e-07f    /* 2^-20 */
+#ifndef OR_DIST_PAD
+#define OR_DIST_PAD 1.52587890625e-05f     /* 2^-16 */
+#endif
 #define OR_TN_SLACK 1.0009765625f           /* 1 + 2^-10 */
 
 
 static int slab(const float* o, const float* inv, const float* bmn, const float* bmx, float best,
                 float* tn_out) {
   float tn = -INFINITY, tf = INFINITY;
+  float lo_d[3], hi_d[3], dist = 0.0f;
   for (int a = 0; a < 3; ++a) {
-    float pad = fmaxf(fabsf(bmn[a]), fabsf(bmx[a])) * OR_BOX_PAD;
-    float t0 = ((bmn[a] - pad) - o[a]) * inv[a];
-    float t1 = ((bmx[a] + pad) - o[a]) * inv[a];
+    lo_d[a] = bmn[a] - o[a];
+    hi_d[a] = bmx[a] - o[a];
+    dist = fmaxf(dist, fmaxf(fabsf(lo_d[a]), fabsf(hi_d[a])));
+  }
+  const float dp = OR_PLANT == 5 ? 0.0f : dist * OR_DIST_PAD;
+  for (int a = 0; a < 3; ++a) {
+    float pad = fmaxf(fmaxf(fabsf(bmn[a]), fabsf(bmx[a])) * OR_BOX_PAD, dp);
+    float t0 = (lo_d[a] - pad) * inv[a];
+    float t1 = (hi_d[a] + pad) * inv[a];
     if (t0 != t0 || t1 != t1) continue; /* 0 * inf: the ray runs inside a face plane */
     float lo = t0 < t1 ? t0 : t1, hi = t0 < t1 ? t1 : t0;
     tn = tn > lo ? tn : lo;

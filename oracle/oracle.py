@@ -185,8 +185,9 @@ def closest_hit_brute(pos, tris, rays):
     return t, prim
 
 
-def closest_hit_bvh(nodes8, lens, pos, tris_perm, perm, rays):
-    """Closest hit through a built tree: nodes8 = (k, 8) u32 node records."""
+def closest_hit_bvh(nodes8, lens, pos, tris_perm, perm, rays, L=None):
+    """Closest hit through a built tree: nodes8 = (k, 8) u32 node records.
+    L: a planted variant's library (planted_lib) instead of the oracle's."""
     nodes8 = np.ascontiguousarray(nodes8, dtype=np.uint32)
     lens = None if lens is None else np.ascontiguousarray(lens, dtype=np.uint32)
     pos = np.ascontiguousarray(pos, dtype=np.float32)
@@ -195,8 +196,8 @@ def closest_hit_bvh(nodes8, lens, pos, tris_perm, perm, rays):
     rays = np.ascontiguousarray(rays, dtype=np.float32)
     t = np.empty(rays.shape[0], dtype=np.float32)
     prim = np.empty(rays.shape[0], dtype=np.uint32)
-    lib().or_closest_hit_bvh(_p(nodes8), _p(lens), _p(pos), pos.shape[1], _p(tris_perm), _p(perm),
-                             _p(rays), rays.shape[0], _p(t), _p(prim))
+    (L or lib()).or_closest_hit_bvh(_p(nodes8), _p(lens), _p(pos), pos.shape[1], _p(tris_perm),
+                                    _p(perm), _p(rays), rays.shape[0], _p(t), _p(prim))
     return t, prim
 
 
@@ -258,7 +259,7 @@ def wide8_collapse(nodes8, lens=None, c_node=1.0, c_prim=0.3, p_max=3):
     return out[:nw.value], tri[:nobj], float(cost.value)
 
 
-def trace_wide8(wide, tri_index, pos, tris_perm, perm, rays):
+def trace_wide8(wide, tri_index, pos, tris_perm, perm, rays, L=None):
     wide = np.ascontiguousarray(wide, dtype=np.uint8)
     tri_index = np.ascontiguousarray(tri_index, dtype=np.uint32)
     pos = np.ascontiguousarray(pos, dtype=np.float32)
@@ -267,8 +268,8 @@ def trace_wide8(wide, tri_index, pos, tris_perm, perm, rays):
     rays = np.ascontiguousarray(rays, dtype=np.float32)
     t = np.empty(rays.shape[0], dtype=np.float32)
     prim = np.empty(rays.shape[0], dtype=np.uint32)
-    lib().or_trace_wide8(_p(wide), _p(tri_index), _p(pos), pos.shape[1], _p(tris_perm), _p(perm),
-                         _p(rays), rays.shape[0], _p(t), _p(prim))
+    (L or lib()).or_trace_wide8(_p(wide), _p(tri_index), _p(pos), pos.shape[1], _p(tris_perm),
+                                _p(perm), _p(rays), rays.shape[0], _p(t), _p(prim))
     return t, prim
 
 

@@ -190,3 +190,67 @@ def test_rays_nearly_parallel_to_box_faces(kind):
     assert (p0 != NONE).sum() > 3000
     for t, p in ((tb, pb), (tw, pw)):
         assert np.array_equal(t.view(np.uint32), t0.view(np.uint32)) and np.array_equal(p, p0)
+
+
+def _far_rays(pos, scale, seed=37, m=20000):
+    # origins on a sphere of radius `scale` around a unit scene at the world
+    # origin, directions aimed at vertices (jittered by 1e-3); in each third of
+    # the first 6000 rays one direction component is shrunk by 1e-3, so those
+    # rays run nearly parallel to a box face (and, on the heightfield, graze it)
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal((m, 3))
+    o = (u / np.linalg.norm(u, axis=1, keepdims=True)) * scale
+    target = pos[rng.integers(0, pos.shape[0], m)].astype(np.float64)
+    target += rng.uniform(-1e-3, 1e-3, (m, 3))
+    d = target - o
+    for a in range(3):
+        d[a * 2000:(a + 1) * 2000, a] *= 1e-3
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([o, d], 1).astype(np.float32)
+
+
+def _far_scene(kind):
+    pos, tris = (scenes.clusters(4000, k=7, seed=31) if kind == "clusters"
+                 else scenes.heightfield(30, 20, seed=13))
+    n8, b = _binary(pos, tris)
+    wide, tri_index, _ = O.wide8_collapse(n8)
+    return pos, tris, n8, b, wide, tri_index
+
+
+def _mismatches(t, p, t0, p0):
+    return np.nonzero((t.view(np.uint32) != t0.view(np.uint32)) | (p != p0))[0]
+
+
+@pytest.mark.parametrize("scale", [1e2, 1e4, 1e5])
+@pytest.mark.parametrize("kind", ["clusters", "heightfield"])
+def test_far_origins_nearly_parallel(kind, scale):
+    # Moller-Trumbore's rounding grows with |o - v|, not with the box's
+    # coordinates: traced from far away, both traversals must still return
+    # the brute-force closest hit of every ray
+    pos, tris, n8, b, wide, tri_index = _far_scene(kind)
+    tp = tris[b["list"]]
+    r = _far_rays(pos, scale)
+    t0, p0 = O.closest_hit_brute(pos, tris, r)
+    tb, pb = O.closest_hit_bvh(n8, None, pos, tp, b["list"], r)
+    tw, pw = O.trace_wide8(wide, tri_index, pos, tp, b["list"], r)
+    assert (p0 != NONE).sum() > 2000
+    for t, p in ((tb, pb), (tw, pw)):
+        bad = _mismatches(t, p, t0, p0)
+        assert bad.size == 0, (bad.size, bad[:5])
+
+
+def test_far_origins_need_the_distance_pad():
+    # the same rays with the box pad scaled by the box's coordinates alone
+    # (planted variant 5): some closest hits are culled, so the test above
+    # does exercise the origin-distance term
+    L5 = O.planted_lib(5)
+    bad = 0
+    for kind, scale in (("clusters", 1e5), ("heightfield", 1e4), ("heightfield", 1e5)):
+        pos, tris, n8, b, wide, tri_index = _far_scene(kind)
+        tp = tris[b["list"]]
+        r = _far_rays(pos, scale)
+        t0, p0 = O.closest_hit_brute(pos, tris, r)
+        tb, pb = O.closest_hit_bvh(n8, None, pos, tp, b["list"], r, L=L5)
+        tw, pw = O.trace_wide8(wide, tri_index, pos, tp, b["list"], r, L=L5)
+        bad += _mismatches(tb, pb, t0, p0).size + _mismatches(tw, pw, t0, p0).size
+    assert bad > 0
