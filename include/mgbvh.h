@@ -334,8 +334,8 @@ int mgbvh_gen_bounce_rays_device(mgbvh_ctx* ctx, const mgbvh_ray* d_rays_in,
  * of its 8 slots; first minimum on ties, a leaf on a leaf/internal tie; area
  * = 2 * ((dx*dy + dy*dz) + dz*dx) of the binary node's box, all fp32. */
 typedef struct mgbvh_wide8_params {
-  float c_node;      /* default 1.0 */
-  float c_prim;      /* default 0.3 */
+  float c_node;      /* default 1.0; finite and >= 0 (else MGBVH_ERR_INVALID_ARG) */
+  float c_prim;      /* default 0.3; finite and >= 0 */
   uint32_t p_max;    /* objects per wide leaf, 1..3 (default 3) */
   uint32_t reserved;
 } mgbvh_wide8_params;

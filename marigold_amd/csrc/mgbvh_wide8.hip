@@ -140,7 +140,8 @@ __device__ void w8_node_dp(const mgbvh_node32* nodes, const uint32_t* base, W8Re
   const float cint = __fadd_rn(__fmul_rn(area, c_node), dist[8]);
   const float cleaf = r.p <= p_max ? __fmul_rn(__fmul_rn(area, (float)r.p), c_prim)
                                    : __uint_as_float(0x7f800000u);
-  const bool leaf = cleaf <= cint;
+  // p <= p_max is part of the rule: when the sums overflow, cint is +inf too
+  const bool leaf = r.p <= p_max && cleaf <= cint;
   r.dec[0] = leaf ? 1 : 0;
   r.c[0] = leaf ? cleaf : cint;
 #pragma unroll
@@ -632,8 +633,10 @@ static int w8_emit_forest(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint3
 
 static int w8_params(const mgbvh_wide8_params* params, mgbvh_wide8_params& P) {
   P = params ? *params : mgbvh_wide8_params{1.0f, 0.3f, 3u, 0u};
-  if (P.p_max < 1 || P.p_max > 3 || !(P.c_node >= 0.0f) || !(P.c_prim >= 0.0f)) {
-    set_error("wide8 parameters: p_max in 1..3 (a leaf's count is 3 unary bits), costs >= 0");
+  if (P.p_max < 1 || P.p_max > 3 || !(P.c_node >= 0.0f && P.c_node < INFINITY) ||
+      !(P.c_prim >= 0.0f && P.c_prim < INFINITY)) {
+    set_error("wide8 parameters: p_max in 1..3 (a leaf's count is 3 unary bits), costs finite "
+              "and >= 0");
     return MGBVH_ERR_INVALID_ARG;
   }
   return MGBVH_OK;
@@ -710,21 +713,30 @@ int mgbvh_wide8_collapse_batch_device(mgbvh_ctx* c, const mgbvh_node32* d_nodes,
                                       uint64_t* out_nwide) {
   if (!c) { set_error("ctx is NULL"); return MGBVH_ERR_INVALID_ARG; }
   if (out_nwide) *out_nwide = 0;
-  if (nblas == 0 || nnodes == 0) return MGBVH_OK;
-  if (!d_nodes || !d_wide || !d_tri_index || !blas || !node_counts || !out_wide_first ||
-      !out_wide_count) {
+  if (nblas == 0) return MGBVH_OK;
+  if (!blas || !node_counts || !out_wide_first || !out_wide_count) {
     set_error("null argument");
     return MGBVH_ERR_INVALID_ARG;
   }
-  if (nnodes >= 0x7fffffffull) { set_error("more than 2^31-1 nodes"); return MGBVH_ERR_INVALID_ARG; }
-  for (uint32_t m = 0; m < nblas; ++m)
+  bool any = false;
+  for (uint32_t m = 0; m < nblas; ++m) {
+    out_wide_first[m] = 0;
+    out_wide_count[m] = 0;
+    any |= node_counts[m] != 0;
     if ((uint64_t)blas[m].node_offset + node_counts[m] > nnodes) {
       set_error("BLAS " + std::to_string(m) + " runs past nnodes");
       return MGBVH_ERR_INVALID_ARG;
     }
+  }
   mgbvh_wide8_params P;
   int rc = w8_params(params, P);
   if (rc) return rc;
+  if (!any) return MGBVH_OK;  // every BLAS empty: nothing to collapse
+  if (!d_nodes || !d_wide || !d_tri_index) {
+    set_error("null argument");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  if (nnodes >= 0x7fffffffull) { set_error("more than 2^31-1 nodes"); return MGBVH_ERR_INVALID_ARG; }
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
   const uint32_t nn = (uint32_t)nnodes;

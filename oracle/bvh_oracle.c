@@ -984,8 +984,10 @@ static void w8_dp(w8_ctx* c, uint32_t v) {
   }
   float cint = area * c->c_node + dist[8];
   float cleaf = r->p <= c->p_max ? (area * (float)r->p) * c->c_prim : INFINITY;
-  r->dec[0] = cleaf <= cint ? 1 : 0;
-  r->c[0] = cleaf <= cint ? cleaf : cint;
+  /* p <= p_max is part of the rule: when the sums overflow, cint is +inf too */
+  const int leaf = r->p <= c->p_max && cleaf <= cint;
+  r->dec[0] = leaf ? 1 : 0;
+  r->c[0] = leaf ? cleaf : cint;
   for (uint32_t i = 2; i <= 7; ++i) {
     if (dist[i] < r->c[i - 2]) { r->c[i - 1] = dist[i]; r->dec[i - 1] = (uint8_t)arg[i]; }
     else { r->c[i - 1] = r->c[i - 2]; r->dec[i - 1] = 0; }
@@ -1095,6 +1097,7 @@ int or_wide8_collapse(const uint32_t* nodes8, const uint32_t* lens, uint32_t nno
                       uint32_t* tri_index, uint32_t* out_nwide, float* out_cost) {
   if (nnodes == 0) { *out_nwide = 0; return 0; }
   if (p_max < 1 || p_max > 3) return -3; /* a leaf's count is 3 unary bits of meta */
+  if (!(c_node >= 0.0f && c_node < INFINITY && c_prim >= 0.0f && c_prim < INFINITY)) return -4;
   w8_ctx c = {nodes8, lens, (w8_rec*)calloc(nnodes, sizeof(w8_rec)), c_node, c_prim, p_max};
   if (!c.r) return -1;
   for (uint32_t v = 0; v < nnodes; ++v)

@@ -60,10 +60,15 @@ def test_collapse_matches_oracle(ctx, kind):
 
 @pytest.mark.parametrize("params", [dict(c_node=1.0, c_prim=2.0, p_max=3),
                                     dict(c_node=0.2, c_prim=0.3, p_max=1),
-                                    dict(c_node=50.0, c_prim=0.3, p_max=2)])
+                                    dict(c_node=50.0, c_prim=0.3, p_max=2),
+                                    dict(c_node=3e38, c_prim=0.3, p_max=3)])
 def test_collapse_parameters(ctx, params):
+    # c_node = 3e38 overflows the cost sums to +inf on the scene scaled by 4:
+    # subtrees of more than p_max objects must still become internal nodes
     print(f"[case] wide8 collapse parameters {params}", flush=True)
     pos, tris = scenes.clusters(4000, k=5, seed=27)
+    if params["c_node"] > 1e30:
+        pos = pos * np.float32(4.0)
     got, ref, _, _ = _collapse_both(ctx, pos, tris, **params)
     _assert_same(str(params), got, ref)
 
@@ -83,6 +88,10 @@ def test_collapse_depth_cut_and_errors(ctx):
     assert e.value.code == N.ERR_INVALID_ARG
     with pytest.raises(N.MgbvhError):
         ctx.wide8_collapse(d_n, None, nodes.shape[0], d_w, 400, d_t, 400, p_max=4)
+    for bad in (dict(c_node=np.inf), dict(c_prim=np.inf), dict(c_node=np.nan)):
+        with pytest.raises(N.MgbvhError) as e:
+            ctx.wide8_collapse(d_n, d_l, nodes.shape[0], d_w, 400, d_t, 400, **bad)
+        assert e.value.code == N.ERR_INVALID_ARG
     with pytest.raises(N.MgbvhError):  # capacity below the node count
         t = tris.copy()
         full, _, _, _, _ = ctx.build_tris(pos, t, N.default_settings())
@@ -174,6 +183,10 @@ def test_batch_collapse_equals_per_blas(ctx):
     tri_index = d_tri.download(np.uint32, nt)
     nodes = sc.download("blas_nodes", np.uint32, 8 * sc.n_slots).reshape(-1, 8)
     assert count[2] == 0 and total == int(count.sum())
+    # a batch whose only BLAS is empty: no launch, zero counts
+    f2, c2, t2 = ctx.wide8_collapse_batch(d["blas_nodes"], None, sc.n_slots, sc.blas[2:3],
+                                          sc.blas_node_counts[2:3], d_wide, nt, d_tri, nt)
+    assert t2 == 0 and f2.tolist() == [0] and c2.tolist() == [0]
     for m in range(len(meshes)):
         n = int(sc.counts[m])
         if n == 0:

@@ -76,6 +76,29 @@ def test_collapse_cost_is_the_brute_force_minimum(n, seed, params):
     assert cost == pytest.approx(want, rel=1e-5)
 
 
+def test_collapse_when_cost_sums_overflow():
+    # c_node = 3e38: a wide internal node's cost overflows to +inf wherever its
+    # box area exceeds ~1.13, and a subtree of more than p_max objects has no
+    # leaf cost (+inf).  Such a subtree must still become an internal node: a
+    # "leaf" of more than p_max objects cannot be encoded (3 unary count bits)
+    pos, tris = scenes.soup(300, seed=23, extent=0.3)
+    pos = pos * np.float32(4.0)
+    n8, b = _binary(pos, tris)
+    wide, tri_index, cost = O.wide8_collapse(n8, c_node=3e38)
+    assert cost == np.inf  # the overflowing branch was taken
+    tp = tris[b["list"]]
+    aabbs = O.tri_aabbs(pos, tp)
+    _check_structure(wide, tri_index, 300, lambda s: (aabbs[s, :3], aabbs[s, 3:]))
+    # a finite c_node of 1e30 does not overflow: the brute-force minimum holds
+    pos, tris = scenes.soup(12, seed=24, extent=0.3)
+    n8, _ = _binary(pos, tris)
+    _, _, cost = O.wide8_collapse(n8, c_node=1e30)
+    assert cost == pytest.approx(_brute_cost(n8, 1e30, 0.3, 3), rel=1e-5)
+    for bad in (dict(c_node=np.inf), dict(c_prim=np.inf), dict(c_node=np.nan), dict(c_prim=-1.0)):
+        with pytest.raises(ValueError):
+            O.wide8_collapse(n8, **bad)
+
+
 def _decode(w, k):
     p = w[0:12].view(np.float32)
     s = (w[12:15].astype(np.uint32) << 23).view(np.float32)
