@@ -246,7 +246,7 @@ typedef struct mgbvh_instance {
   uint32_t reserved[3];
 } mgbvh_instance;
 
-/* Pinhole camera of the primary rays (screen.slang:48-72 without jitter). */
+/* Pinhole camera of mgbvh_gen_primary_rays_device (position, orthonormal basis). */
 typedef struct mgbvh_camera {
   float pos[3];
   float fwd[3];
@@ -304,7 +304,23 @@ int mgbvh_raytrace_tlas_device(mgbvh_ctx* ctx, const mgbvh_node32* d_tlas_nodes,
                                const uint32_t* d_blas_perm, const mgbvh_ray* d_rays,
                                uint64_t nrays, mgbvh_hit* d_hits);
 
-/* Primary rays of a width x height image, row-major from the top-left pixel. */
+/* Primary rays of a width x height image, row-major from the top-left pixel,
+ * as the path tracer makes them (pathtrace.slang:46-66, screen.slang:48-72):
+ * texcoord = (pixel + jitter) / (width, height); clip = (2 tx - 1, 1 - 2 ty, 1);
+ * world = (M clip).xyz / (M clip).w; dir = normalize(world - position);
+ * org = position.  inv_view_proj: M, the inverse view-projection matrix, 16
+ * floats column-major (glam's Mat4 layout, camera.rs:355-362).  The jitter is
+ * the shader's: two draws of the PCG stream seeded with frame_count, minus 0.5,
+ * when accumulate is non-zero (shouldAccumulate), else 0.  Each operation is
+ * rounded once in the order written (no FMA, correctly rounded division and
+ * square root), so the rays are reproducible bit for bit. */
+int mgbvh_gen_view_rays_device(mgbvh_ctx* ctx, const float* inv_view_proj, const float* position,
+                               uint32_t width, uint32_t height, uint32_t frame_count,
+                               int accumulate, mgbvh_ray* d_rays);
+
+/* The same rays for a pinhole camera: M from glam's look_at_rh and
+ * perspective_rh (depth 0..1, near 0.1, far 100) of the camera, computed in
+ * double, and a jitter of +0.5, so every ray passes through its pixel's centre. */
 int mgbvh_gen_primary_rays_device(mgbvh_ctx* ctx, const mgbvh_camera* cam, uint32_t width,
                                   uint32_t height, mgbvh_ray* d_rays);
 

@@ -193,6 +193,7 @@ _EXTRA: dict = {
                                            _P, _P, C.POINTER(_U64)], _I),
     "mgbvh_trace_wide8_device": ([_P, _P, _U64, _P, _P, _U32, _P, _P, _P, _U64, _P], _I),
     "mgbvh_gen_primary_rays_device": ([_P, C.POINTER(Camera), _U32, _U32, _P], _I),
+    "mgbvh_gen_view_rays_device": ([_P, _P, _P, _U32, _U32, _U32, C.c_int, _P], _I),
     "mgbvh_copy_d2d": ([_P, _P, _P, _U64], _I),
     "mgbvh_build_tris_device_noalloc": ([_P, _P, _U32, _U64, _P, _U64, C.POINTER(Settings), _P, _P,
                                          _P, _P, _P, C.POINTER(_U64)], _I),
@@ -542,6 +543,15 @@ class Context:
     def gen_primary(self, cam: "Camera", width: int, height: int, rays):
         check(lib().mgbvh_gen_primary_rays_device(self.handle, C.byref(cam), width, height,
                                                   _dv(rays)))
+
+    def gen_view(self, inv_view_proj, position, width: int, height: int, rays,
+                 frame_count: int = 0, accumulate: bool = False):
+        """The path tracer's primary rays from a column-major inverse
+        view-projection matrix (16 floats) and the camera position."""
+        m = np.ascontiguousarray(np.asarray(inv_view_proj, np.float32).reshape(16))
+        p = np.ascontiguousarray(np.asarray(position, np.float32).reshape(3))
+        check(lib().mgbvh_gen_view_rays_device(self.handle, ptr(m), ptr(p), width, height,
+                                               frame_count, 1 if accumulate else 0, _dv(rays)))
 
     def gen_bounce(self, rays_in, hits, nrays, instances, ninstances, blas, nblas, pos, vstride,
                    tris_perm, seed, rays_out):

@@ -19,6 +19,9 @@
 // 1 + 2^-20, near distances compared against the closest t widened by
 // 1 + 2^-10, and an axis whose plane distance is 0 * inf (a ray inside a face
 // plane) left unbounded.
+#include <cmath>
+#include <cstring>
+
 #include "mgbvh_common.h"
 #include "mgbvh_internal.h"
 
@@ -454,19 +457,43 @@ __device__ inline void store_ray(mgbvh_ray* rays, size_t i, const float* o, cons
   p[1] = make_float4(d[0], d[1], d[2], f_inf());
 }
 
-// Pinhole primary ray through the centre of pixel (x, y); the same direction
-// as screen.slang:48-72 viewDir up to rounding.
-__global__ void k_gen_primary(mgbvh_camera cam, uint32_t w, uint32_t h, mgbvh_ray* rays) {
+// The path tracer's primary ray (pathtrace.slang:46-66, screen.slang:48-72):
+// texcoord = (pixel + jitter) / (width, height), clip = (2 tx - 1, 1 - 2 ty, 1)
+// (screen depth 1), world = (M clip).xyz / (M clip).w with M the inverse
+// view-projection matrix (column-major, as glam stores it), direction =
+// normalize(world - position).  Every operation is rounded once, in the
+// order written (sums left to right, no FMA, correctly rounded division and
+// square root), so the rays are reproducible bit for bit.
+struct ViewParams {
+  float m[16];
+  float pos[3];
+  float jx, jy;
+};
+
+__global__ void k_gen_view(ViewParams vp, uint32_t w, uint32_t h, mgbvh_ray* rays) {
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t y = blockIdx.y;
   if (x >= w || y >= h) return;
-  const float th = tanf(0.5f * cam.fov_y);
-  const float u = (2.0f * ((float)x + 0.5f) / (float)w - 1.0f) * th * cam.aspect;
-  const float v = (1.0f - 2.0f * ((float)y + 0.5f) / (float)h) * th;
+  const float tx = __fdiv_rn(__fadd_rn((float)x, vp.jx), (float)w);
+  const float ty = __fdiv_rn(__fadd_rn((float)y, vp.jy), (float)h);
+  const float cl[3] = {__fsub_rn(__fmul_rn(tx, 2.0f), 1.0f), __fsub_rn(1.0f, __fmul_rn(ty, 2.0f)),
+                       1.0f};
+  float t[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    t[i] = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(vp.m[i], cl[0]), __fmul_rn(vp.m[4 + i], cl[1])),
+                               __fmul_rn(vp.m[8 + i], cl[2])),
+                     vp.m[12 + i]);
   float d[3];
-  for (int a = 0; a < 3; ++a) d[a] = cam.fwd[a] + u * cam.right[a] + v * cam.up[a];
-  normalize3(d);
-  store_ray(rays, (size_t)y * w + x, cam.pos, d);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) d[a] = __fsub_rn(__fdiv_rn(t[a], t[3]), vp.pos[a]);
+  // sqrtf, not __fsqrt_rn: without OCML_BASIC_ROUNDED_OPERATIONS the latter is
+  // the native (approximate) square root; sqrtf is correctly rounded
+  const float l = sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(d[0], d[0]), __fmul_rn(d[1], d[1])),
+                                  __fmul_rn(d[2], d[2])));
+#pragma unroll
+  for (int a = 0; a < 3; ++a) d[a] = __fdiv_rn(d[a], l);
+  store_ray(rays, (size_t)y * w + x, vp.pos, d);
 }
 
 // random.slang:29-41 getUnitVector / getCosineVector; seeded as pathtrace.slang:49.
@@ -519,6 +546,56 @@ __global__ void k_gen_bounce(const mgbvh_ray* __restrict__ rin, const mgbvh_hit*
     for (int a = 0; a < 3; ++a) { o[a] = q.org[a]; d[a] = uvec[a]; }
   }
   store_ray(rout, i, o, d);
+}
+
+// Host copy of the PCG stream (random.slang:5-27).
+uint32_t pcg_host(uint32_t& seed) {
+  const uint32_t state = seed * 747796405u + 2891336453u;
+  const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  seed = (word >> 22u) ^ word;
+  return seed;
+}
+float pcg_float(uint32_t& s) { return (float)pcg_host(s) / (float)0xffffffffu; }
+
+// inverse(perspective_rh(fov_y, aspect, 0.1, 100) * look_at_rh) of a pinhole
+// camera given by its position and orthonormal basis, column-major floats.
+void pinhole_inverse_view_proj(const mgbvh_camera& cam, float* out16) {
+  const double n = 0.1, f = 100.0;
+  const double th = std::tan(0.5 * (double)cam.fov_y);
+  // inverse projection (view-space point of a clip point, then / w)
+  double ip[4][4] = {{th * cam.aspect, 0, 0, 0}, {0, th, 0, 0}, {0, 0, 0, -1},
+                     {0, 0, (n - f) / (f * n), 1.0 / n}};
+  // inverse view: view space (x right, y up, -z forward) to world
+  double iv[4][4] = {};
+  for (int r = 0; r < 3; ++r) {
+    iv[r][0] = cam.right[r];
+    iv[r][1] = cam.up[r];
+    iv[r][2] = -(double)cam.fwd[r];
+    iv[r][3] = cam.pos[r];
+  }
+  iv[3][3] = 1.0;
+  for (int r = 0; r < 4; ++r)
+    for (int k = 0; k < 4; ++k) {
+      double acc = 0.0;
+      for (int j = 0; j < 4; ++j) acc += iv[r][j] * ip[j][k];
+      out16[4 * k + r] = (float)acc;  // column k, row r
+    }
+}
+
+int gen_view(mgbvh_ctx* c, const ViewParams& vp, uint32_t width, uint32_t height,
+             mgbvh_ray* d_rays) {
+  if (!width || !height) return MGBVH_OK;
+  if ((uint64_t)width * height > 0xffffffffull || height > 65535) {
+    set_error("image larger than 2^32-1 pixels or 65535 rows");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  k_gen_view<<<dim3((width + 255) / 256, height), 256, 0, c->stream>>>(vp, width, height, d_rays);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "k_gen_view");
+  e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? MGBVH_OK : hip_fail(e, "k_gen_view");
 }
 
 int launch_done(const char* what) {
@@ -660,22 +737,38 @@ int mgbvh_raytrace_tlas_device(mgbvh_ctx* c, const mgbvh_node32* d_tlas_nodes,
   return finish_trace(c, flag, "the 64-entry traversal stack of the path tracer overflowed");
 }
 
+int mgbvh_gen_view_rays_device(mgbvh_ctx* c, const float* inv_view_proj, const float* position,
+                               uint32_t width, uint32_t height, uint32_t frame_count,
+                               int accumulate, mgbvh_ray* d_rays) {
+  if (!c || !inv_view_proj || !position || !d_rays) {
+    set_error("null argument");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  ViewParams vp;
+  std::memcpy(vp.m, inv_view_proj, sizeof vp.m);
+  std::memcpy(vp.pos, position, sizeof vp.pos);
+  // pathtrace.slang:50-61: the frame's jitter, two draws of Random(frameCount)
+  // minus 0.5, applied only while the image accumulates
+  uint32_t s = frame_count;
+  const float j0 = pcg_float(s) - 0.5f;
+  const float j1 = pcg_float(s) - 0.5f;
+  vp.jx = accumulate ? j0 : 0.0f;
+  vp.jy = accumulate ? j1 : 0.0f;
+  return gen_view(c, vp, width, height, d_rays);
+}
+
 int mgbvh_gen_primary_rays_device(mgbvh_ctx* c, const mgbvh_camera* cam, uint32_t width,
                                   uint32_t height, mgbvh_ray* d_rays) {
   if (!c || !cam || !d_rays) { set_error("null argument"); return MGBVH_ERR_INVALID_ARG; }
-  if (!width || !height) return MGBVH_OK;
-  if ((uint64_t)width * height > 0xffffffffull || height > 65535) {
-    set_error("image larger than 2^32-1 pixels or 65535 rows");
-    return MGBVH_ERR_INVALID_ARG;
-  }
-  hipError_t e = hipSetDevice(c->device);
-  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-  k_gen_primary<<<dim3((width + 255) / 256, height), 256, 0, c->stream>>>(*cam, width, height,
-                                                                        d_rays);
-  int rc = launch_done("k_gen_primary");
-  if (rc) return rc;
-  e = hipStreamSynchronize(c->stream);
-  return e == hipSuccess ? MGBVH_OK : hip_fail(e, "k_gen_primary");
+  // the pinhole camera as glam's look_at_rh and perspective_rh (depth 0..1,
+  // near 0.1, far 100) would give it, inverted in double; a jitter of +0.5
+  // puts every ray through its pixel's centre
+  ViewParams vp;
+  pinhole_inverse_view_proj(*cam, vp.m);
+  std::memcpy(vp.pos, cam->pos, sizeof vp.pos);
+  vp.jx = 0.5f;
+  vp.jy = 0.5f;
+  return gen_view(c, vp, width, height, d_rays);
 }
 
 int mgbvh_gen_bounce_rays_device(mgbvh_ctx* c, const mgbvh_ray* d_rays_in,

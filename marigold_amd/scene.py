@@ -189,6 +189,29 @@ class GpuScene:
         self.d = {}
 
 
+def inverse_view_proj(pos, target, fov_y: float, aspect: float, near: float = 0.1,
+                      far: float = 100.0, up=(0.0, 1.0, 0.0)) -> np.ndarray:
+    """inverse(perspective_rh(fov_y, aspect, near, far) * look_at_rh(pos,
+    target, up)) as glam builds them (depth 0..1; camera.rs:105-110, 355-362),
+    computed in float64 and returned as 16 float32 in glam's column-major order."""
+    eye = np.asarray(pos, np.float64)
+    f = np.asarray(target, np.float64) - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, np.asarray(up, np.float64))
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    view = np.eye(4)
+    view[0, :3], view[1, :3], view[2, :3] = s, u, -f
+    view[:3, 3] = [-s @ eye, -u @ eye, f @ eye]
+    h = 1.0 / np.tan(0.5 * fov_y)
+    r = far / (near - far)
+    proj = np.zeros((4, 4))
+    proj[0, 0], proj[1, 1] = h / aspect, h
+    proj[2, 2], proj[2, 3], proj[3, 2] = r, r * near, -1.0
+    inv = np.linalg.inv(proj @ view)
+    return np.ascontiguousarray(inv.T.reshape(16).astype(np.float32))  # column-major
+
+
 def look_at_camera(pos, target, fov_y: float, aspect: float, up=(0.0, 1.0, 0.0)) -> N.Camera:
     pos = np.asarray(pos, np.float64)
     f = np.asarray(target, np.float64) - pos

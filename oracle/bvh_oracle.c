@@ -39,7 +39,8 @@
  *   2: adaptive thresholds without the + 0.5 (lib.rs:434)
  *   3: tree SAH leaf term without len
  *   4: tree SAH internal term weighted by OBJECT_COST instead of DEPTH_COST
- *   5: traversal box pad without the origin-distance term */
+ *   5: traversal box pad without the origin-distance term
+ *   6, 7: ray generation variants (rays_oracle.c) */
 #ifndef OR_PLANT
 #define OR_PLANT 0
 #endif

@@ -14,7 +14,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SO = os.path.join(HERE, "liboracle.so")
-SRC = [os.path.join(HERE, "bvh_oracle.c")]
+SRC = [os.path.join(HERE, "bvh_oracle.c"), os.path.join(HERE, "rays_oracle.c")]
 
 NODE_DTYPE = np.dtype([("min", "<f4", (3,)), ("max", "<f4", (3,)), ("start", "<u4"), ("len", "<u4"),
                        ("child", "<u4")])
@@ -97,6 +97,13 @@ def _bind(L):
         L.or_trace_wide8.argtypes = [P, P, P, C.c_uint32, P, P, P, C.c_uint32, P, P]
         L.or_ray_tri.argtypes = [P, P, P, P]
         L.or_ray_tri.restype = C.c_float
+        L.or_pcg.argtypes = [C.POINTER(C.c_uint32)]
+        L.or_pcg.restype = C.c_uint32
+        L.or_pcg_float.argtypes = [C.POINTER(C.c_uint32)]
+        L.or_pcg_float.restype = C.c_float
+        L.or_frame_jitter.argtypes = [C.c_uint32, C.c_int, P]
+        L.or_gen_view_rays.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, P]
+        L.or_gen_bounce.argtypes = [P, P, C.c_uint32, P, P, P, C.c_uint32, P, C.c_uint32, P]
     return L
 
 
@@ -295,4 +302,49 @@ def instance_boxes(mesh_min, mesh_max, o2w12):
             w = w + m[:, a, 3]
             out[:, a] = np.minimum(out[:, a], w)
             out[:, 3 + a] = np.maximum(out[:, 3 + a], w)
+    return out
+
+
+# ---------------------------------------------------------------- rays (rays_oracle.c)
+def pcg_stream(seed: int, count: int, L=None):
+    """count successive getUint() values of Random(seed) (random.slang:5-22)."""
+    st = C.c_uint32(seed)
+    return np.array([(L or lib()).or_pcg(C.byref(st)) for _ in range(count)], dtype=np.uint32)
+
+
+def pcg_floats(seed: int, count: int, L=None):
+    st = C.c_uint32(seed)
+    return np.array([(L or lib()).or_pcg_float(C.byref(st)) for _ in range(count)],
+                    dtype=np.float32)
+
+
+def frame_jitter(frame_count: int, accumulate: bool, L=None):
+    j = np.zeros(2, np.float32)
+    (L or lib()).or_frame_jitter(frame_count, 1 if accumulate else 0, _p(j))
+    return j
+
+
+def gen_view_rays(inv_view_proj, position, width: int, height: int, frame_count: int = 0,
+                  accumulate: bool = False, L=None):
+    """(width * height, 8) float32 rays: org[3], 0, dir[3], +inf."""
+    m = np.ascontiguousarray(np.asarray(inv_view_proj, np.float32).reshape(16))
+    p = np.ascontiguousarray(np.asarray(position, np.float32).reshape(3))
+    j = frame_jitter(frame_count, accumulate, L)
+    out = np.empty((width * height, 8), np.float32)
+    (L or lib()).or_gen_view_rays(_p(m), _p(p), width, height, float(j[0]), float(j[1]), _p(out))
+    return out
+
+
+def gen_bounce(rays8, hits4, pos, tris_perm, seed: int, inst16=None, blas_voff=None, L=None):
+    """One bounce per ray: (n, 8) float32 rays from (n, 8) rays and (n, 4) u32 hits."""
+    rays8 = np.ascontiguousarray(rays8, dtype=np.float32)
+    hits4 = np.ascontiguousarray(hits4, dtype=np.uint32)
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    tris_perm = np.ascontiguousarray(tris_perm, dtype=np.uint32)
+    inst16 = None if inst16 is None else np.ascontiguousarray(inst16, dtype=np.float32)
+    blas_voff = None if blas_voff is None else np.ascontiguousarray(blas_voff, dtype=np.uint32)
+    n = rays8.shape[0]
+    out = np.empty((n, 8), np.float32)
+    (L or lib()).or_gen_bounce(_p(rays8), _p(hits4), n, _p(inst16), _p(blas_voff), _p(pos),
+                               pos.shape[1], _p(tris_perm), seed, _p(out))
     return out
