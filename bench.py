@@ -172,7 +172,7 @@ def cpu_baseline_scene(meshes, budget_s: float = 15.0, threads: int = 0):
     """Per-mesh oracle builds on a thread pool (one mesh per task, as rayon's
     par_iter over meshes in model.rs:76-95, with the rayon pool's one worker
     per core), over a random sample of meshes until the budget is spent."""
-    from concurrent.futures import ThreadPoolExecutor
+    from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
     from oracle import oracle as O  # CPU baseline only
     threads = threads or cpu_pool_threads()
     rng = np.random.default_rng(0)
@@ -195,14 +195,15 @@ def cpu_baseline_scene(meshes, budget_s: float = 15.0, threads: int = 0):
             if i is not None:
                 pending.add(ex.submit(one, i))
         while pending:
-            f = next(iter(pending))
-            done_tris += f.result()
-            n_done += 1
-            pending.remove(f)
-            if time.perf_counter() < deadline:
-                i = next(it, None)
-                if i is not None:
-                    pending.add(ex.submit(one, i))
+            # every finished task frees its thread for the next mesh at once
+            done, pending = wait(pending, return_when=FIRST_COMPLETED)
+            for f in done:
+                done_tris += f.result()
+                n_done += 1
+                if time.perf_counter() < deadline:
+                    i = next(it, None)
+                    if i is not None:
+                        pending.add(ex.submit(one, i))
     wall = time.perf_counter() - t0
     return {"value": done_tris / wall / 1e6, "unit": "Mtri/s", "cores": threads, "kind": "port",
             "sample": f"{n_done} randomly chosen meshes of the scene ({done_tris} tris) built by the C "
