@@ -212,6 +212,54 @@ def cpu_baseline_scene(meshes, budget_s: float = 15.0, threads: int = 0):
             "host": host_cores()}
 
 
+def single_mesh(ctx, K: int = 10):
+    """The 1024x512-cell heightfield (1,048,576 triangles) built alone with the
+    plain method: level by level (the host reads each level's counts) and as a
+    one-mesh scene plan (one HIP graph, no instances), with both trees compared."""
+    from marigold_amd import _native as N
+    from marigold_amd import scenes
+    from marigold_amd.scene import GpuScene
+    pos, tris = scenes.heightfield(1024, 512, seed=2)
+    n, nv = tris.shape[0], pos.shape[0]
+    d_pos, d_tris = ctx.to_device(pos), ctx.to_device(tris)
+    dn, dl = ctx.alloc(32 * (2 * n - 1)), ctx.alloc(4 * (2 * n - 1))
+    dp, dt = ctx.alloc(4 * n), ctx.alloc(12 * n)
+    st = N.default_settings()
+    for _ in range(3):
+        nn = ctx.build_tris_device(d_pos, 12, nv, d_tris, n, st, dn, dl, dp, dt)
+    ctx.synchronize()
+    terms = tree_terms(dn.download(N.NODE_DTYPE, nn), dl.download(np.uint32, nn))
+    t0 = time.perf_counter()
+    for _ in range(K):
+        ctx.build_tris_device(d_pos, 12, nv, d_tris, n, st, dn, dl, dp, dt)
+    ctx.synchronize()
+    lbl_ms = (time.perf_counter() - t0) / K * 1e3
+    ref = dn.download(np.uint32, 8 * (2 * n - 1))
+    g = GpuScene(ctx, [(pos, tris)], np.zeros((0, 4, 4), np.float32), np.zeros(0, np.uint32),
+                 build=False)
+    g.record_graph()
+    for _ in range(3):
+        g.build()
+    dev = []
+    t0 = time.perf_counter()
+    for _ in range(K):
+        ms, _, _ = g.plan.launch()
+        dev.append(ms)
+    wall_ms = (time.perf_counter() - t0) / K * 1e3
+    same = bool(np.array_equal(g.download("blas_nodes", np.uint32, 8 * (2 * n - 1)), ref))
+    g.free()
+    for b in (d_pos, d_tris, dn, dl, dp, dt):
+        b.free()
+    dev_ms = float(np.median(dev))
+    model = survey_bytes(n, nv, terms["s_large"], terms["s_small"])
+    return {"workload": "1024x512-cell heightfield, 1,048,576 tris, plain method",
+            "graph_device_ms": dev_ms, "graph_wall_ms": wall_ms, "level_by_level_wall_ms": lbl_ms,
+            "mtri_per_s_graph": n / (dev_ms * 1e-3) / 1e6,
+            "byte_model_gb": model / 1e9,
+            "byte_model_frac_of_hbm": model / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "trees_equal": same}
+
+
 def _hits_of(ctx, sc_or_trace, rays_dev, n):
     h = ctx.alloc(16 * n)
     sc_or_trace(rays_dev, n, h)
@@ -441,6 +489,10 @@ def scene_main(args):
     if not args.no_rays:
         log("16M-ray hit comparison on the heightfield")
         full = heightfield_16m_rays(ctx, learned)
+    log("single mesh: the 1M-triangle heightfield alone, level by level and as a graph")
+    single = single_mesh(ctx)
+    log(f"single mesh: graph {single['graph_device_ms']:.3f} ms device, "
+        f"{single['graph_wall_ms']:.3f} ms wall; level by level {single['level_by_level_wall_ms']:.3f} ms")
     if args.calibrate_copy:
         # known-size device copies (256 MiB, then 1 GiB) for scaling the
         # FETCH_SIZE / WRITE_SIZE counters
@@ -485,6 +537,7 @@ def scene_main(args):
         "scene_hits_identical": {"rays": w * h, "identical": same_hits(hits_plain, hits_learned),
                                  "hits": int((hits_plain["prim"] != N.MISS).sum())},
         "heightfield_16m_rays": full,
+        "single_mesh": single,
         "wide8_collapse": {"blas": len(meshes), "wide_nodes": n_wide, "ms": wide_ms[-1],
                            "first_call_ms": wide_ms[0], "timed": "host wall time of one "
                            "mgbvh_wide8_collapse_batch_device call (it synchronises)"},
