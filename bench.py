@@ -480,7 +480,15 @@ def scene_main(args):
         ms_plain = ctx.elapsed_ms(2, 3) / args.plain_steps
         same_nodes = bool(np.array_equal(bat_nodes,
                                          pla.download("blas_nodes", np.uint32, 8 * pla.n_slots)))
-        log(f"plain per mesh {ms_plain:.2f} ms/scene")
+        # the same per-mesh builds once more with every kernel timed: how much of
+        # the wall time is device work (the rest is launches and the host
+        # reading each level's counts, per mesh)
+        ctx.set_profiling(N.Context.KERNEL_KINDS)
+        ctx.kernel_times(reset=True)
+        pla.build()
+        plain_kernel_ms = sum(v[0] for v in ctx.kernel_times(reset=True).values())
+        ctx.set_profiling([])
+        log(f"plain per mesh {ms_plain:.2f} ms/scene, of which kernels {plain_kernel_ms:.2f} ms")
         pla.free()
     d_rays.free()
     del bat_nodes
@@ -527,6 +535,8 @@ def scene_main(args):
         "level_by_level_ms": ms_level,
         "plain_per_mesh": None if ms_plain is None else
         {"value": n / (ms_plain * 1e-3) / 1e6, "ms_per_step": ms_plain, "steps": args.plain_steps,
+         "kernel_ms_per_step": plain_kernel_ms,
+         "speedup_of_value_over_kernel_time": plain_kernel_ms / ms_step,
          "node_arrays_equal_batched_plain": same_nodes},
         "batched_plain": {"value": n / (ms_bat * 1e-3) / 1e6, "ms_per_step": ms_bat},
         "speedup_vs_plain": None if ms_plain is None else ms_plain / ms_step,
