@@ -558,6 +558,9 @@ typedef struct mgbvh_tuning {
   uint32_t sweep_pack_min_nodes; /* levels with at least this many nodes are swept one node
                                     per lane, the others one node per wavefront
                                     (0 = keep; 1 = every level) */
+  uint32_t medium_max_objects;   /* nodes of 33..this many objects build their whole subtree
+                                    in one wavefront instead of level by level: 33..2048;
+                                    1 = off; 0 = keep */
 } mgbvh_tuning;
 int mgbvh_ctx_set_tuning(mgbvh_ctx* ctx, const mgbvh_tuning* tuning);
 int mgbvh_ctx_get_tuning(mgbvh_ctx* ctx, mgbvh_tuning* tuning);

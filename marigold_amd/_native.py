@@ -49,7 +49,8 @@ class Tuning(C.Structure):
     _fields_ = [("bin_blocks_per_cu", C.c_uint32), ("partition_blocks_per_cu", C.c_uint32),
                 ("sweep_blocks_per_cu", C.c_uint32), ("small_blocks_per_cu", C.c_uint32),
                 ("compute_units", C.c_uint32), ("partition_waves", C.c_uint32),
-                ("bin_waves", C.c_uint32), ("sweep_pack_min_nodes", C.c_uint32)]
+                ("bin_waves", C.c_uint32), ("sweep_pack_min_nodes", C.c_uint32),
+                ("medium_max_objects", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -435,7 +436,7 @@ class Context:
         return float(ms.value)
 
     KERNEL_KINDS = ["setup", "root", "bin", "sweep", "partition", "small", "compact", "permute",
-                    "trace", "learn"]
+                    "trace", "learn", "medium"]
 
     def set_tuning(self, **kw):
         t = self.get_tuning_struct()

@@ -1,8 +1,9 @@
 """Autotuner of the batched scene build.
 
 Picks, by timing candidates on the scene itself, the settings that do not
-change the plain result (grid sizes of the level kernels, in blocks
-per compute unit) and the per-node-size dispatch of the learned method (the
+change the plain result (grid sizes of the level kernels, in blocks per
+compute unit, and the largest node whose subtree one wavefront builds) and
+the per-node-size dispatch of the learned method (the
 smallest node that takes the learned path, the window half-width), keeping
 only learned candidates whose scene SAH stays within `sah_tolerance` of the
 plain trees'.  Timings are graph replays (`GpuScene.record_graph`), medians of
@@ -30,6 +31,8 @@ GRID_CANDIDATES = [
     dict(bin_blocks_per_cu=32, partition_blocks_per_cu=64, sweep_blocks_per_cu=4,
          small_blocks_per_cu=16, partition_waves=1, bin_waves=4),
 ]
+# nodes of 33..this many objects built one wavefront per subtree (1 = off)
+MEDIUM_CANDIDATES = [512, 1024, 2048]
 LEARNED_CANDIDATES = [
     dict(learned_min_objects=1024, learned_window=3, learned_guard=0.02),
     dict(learned_min_objects=2048, learned_window=2, learned_guard=0.02),
@@ -83,12 +86,24 @@ def autotune(ctx: N.Context, meshes, transforms, instance_mesh, reps: int = 3,
     # 1. grid sizes, plain split rule (the result does not depend on them)
     best_grid, best_ms = None, float("inf")
     for cand in GRID_CANDIDATES:
+        cand = dict(cand, medium_max_objects=1)
         ctx.set_tuning(**cand)
         probe.record_graph()  # grid sizes are baked into the graph
         ms = _time_graph(probe, reps)
         trials.append({"tuning": cand, "ms": ms})
         if log:
             log(f"autotune grid {cand}: {ms:.2f} ms")
+        if ms < best_ms:
+            best_grid, best_ms = cand, ms
+    # 2. the largest node built one wavefront per subtree (results unchanged)
+    for med in MEDIUM_CANDIDATES:
+        cand = dict(best_grid, medium_max_objects=med)
+        ctx.set_tuning(**cand)
+        probe.record_graph()
+        ms = _time_graph(probe, reps)
+        trials.append({"tuning": cand, "ms": ms})
+        if log:
+            log(f"autotune medium_max_objects {med}: {ms:.2f} ms")
         if ms < best_ms:
             best_grid, best_ms = cand, ms
     ctx.set_tuning(**best_grid)
@@ -98,7 +113,7 @@ def autotune(ctx: N.Context, meshes, transforms, instance_mesh, reps: int = 3,
     probe.free()
     result = {"tuning": best_grid, "learned": None, "ms": best_ms, "plain_sah": plain_sah,
               "learned_sah": None}
-    # 2. learned dispatch, if a model is attached
+    # 3. learned dispatch, if a model is attached
     if ctx_has_model(ctx):
         best_l, best_lms = None, float("inf")
         for cand in LEARNED_CANDIDATES:

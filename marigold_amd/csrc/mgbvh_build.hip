@@ -446,12 +446,13 @@ __global__ __launch_bounds__(256) void k_bin(const LNode* __restrict__ L,
 
 // Child record of a split: a node with > 32 objects for the next level (slot s,
 // chunks [c0, c0 + nch)), or a subtree root of <= 32 objects / a depth-cut leaf.
-__device__ inline void put_child(bool large, uint32_t s, uint32_t c0, uint32_t start,
+__device__ inline void put_child(uint32_t kind, uint32_t s, uint32_t c0, uint32_t start,
                                  uint32_t count, const Box& b, uint32_t idx, uint32_t P,
                                  uint32_t depth, uint32_t parity_next, uint32_t nbase,
-                                 uint32_t obase, LNode* Ln, SNode* small, uint32_t learn_min,
-                                 uint32_t* lvn, uint32_t* lrows, bool prebinned) {
-  if (large) {
+                                 uint32_t obase, LNode* Ln, SNode* small, SNode* med,
+                                 uint32_t learn_min, uint32_t* lvn, uint32_t* lrows,
+                                 bool prebinned) {
+  if (kind == 1) {
     LNode r = {};
     r.start = start; r.count = count; r.idx = idx; r.P = P; r.depth = depth;
     r.chunk0 = c0; r.nchunk = (count + CH - 1) / CH;
@@ -467,8 +468,18 @@ __device__ inline void put_child(bool large, uint32_t s, uint32_t c0, uint32_t s
     sn.start = start; sn.count = count; sn.idx = idx; sn.P = P; sn.depth = depth;
     sn.parity = parity_next; sn.nbase = nbase; sn.obase = obase;
     for (int a = 0; a < 3; ++a) { sn.bmn[a] = b.mn[a]; sn.bmx[a] = b.mx[a]; }
-    small[s] = sn;
+    (kind == 2 ? med : small)[s] = sn;
   }
+}
+
+// Where a child of a split goes (lib.rs:648-663): 0 the small-subtree list
+// (<= 32 objects, or a depth-cut leaf), 2 the medium list (33..med_max
+// objects, not on the learned path), 1 the next level.
+__device__ inline uint32_t child_route(uint32_t count, uint32_t cd, uint32_t max_depth,
+                                       uint32_t med_max, uint32_t learn_min) {
+  if (count <= MG_SMALL || cd >= max_depth) return 0;
+  if (count <= med_max && (learn_min == 0 || count < learn_min)) return 2;
+  return 1;
 }
 
 // The split of a node with > 32 objects (select_split, lib.rs:493-534).
@@ -681,53 +692,62 @@ __device__ inline SplitDec sweep_split(const LNode& nd, const uint32_t* h,
 }
 
 // What a split node asks of the next level: large slots, chunks, small slots,
-// objects (work counter), window nodes, fallback nodes, chunks left to k_bin.
-// Children (lib.rs:648-663): one with > 32 objects below max_depth is split
-// on the next level, any other goes to the small-subtree list.
-constexpr int SW_NEEDS = 7;
-__device__ inline void sweep_need(const LNode& nd, const SplitDec& d, uint32_t max_depth,
+// medium slots (the four lists), objects (work counter), window nodes,
+// fallback nodes, chunks left to k_bin.  Children go where child_route says.
+constexpr int SW_NEEDS = 8, SW_LISTS = 4;
+struct SweepCfg {
+  uint32_t max_depth, med_max, learn_min;
+};
+__device__ inline void sweep_need(const LNode& nd, const SplitDec& d, const SweepCfg& cf,
                                   uint32_t (&need)[SW_NEEDS]) {
   const uint32_t cd = nd.depth + 1;
-  const bool lg = d.ng > MG_SMALL && cd < max_depth;
-  const bool ll = d.nl > MG_SMALL && cd < max_depth;
-  need[0] = (uint32_t)lg + (uint32_t)ll;
-  need[1] = (lg ? (d.ng + CH - 1) / CH : 0) + (ll ? (d.nl + CH - 1) / CH : 0);
-  need[2] = 2 - need[0];
-  need[3] = nd.count;
-  need[4] = lmode_kind(nd.mode) == LM_WINDOW;
-  need[5] = lmode_kind(nd.mode) == LM_FALLBACK;
-  need[6] = nd.nchunk > 1 ? need[1] : 0u;  // children of a one-block node are binned there
+  const uint32_t rg = child_route(d.ng, cd, cf.max_depth, cf.med_max, cf.learn_min);
+  const uint32_t rl = child_route(d.nl, cd, cf.max_depth, cf.med_max, cf.learn_min);
+  need[0] = (uint32_t)(rg == 1) + (uint32_t)(rl == 1);
+  need[1] = (rg == 1 ? (d.ng + CH - 1) / CH : 0) + (rl == 1 ? (d.nl + CH - 1) / CH : 0);
+  need[2] = (uint32_t)(rg == 0) + (uint32_t)(rl == 0);
+  need[3] = (uint32_t)(rg == 2) + (uint32_t)(rl == 2);
+  need[4] = nd.count;
+  need[5] = lmode_kind(nd.mode) == LM_WINDOW;
+  need[6] = lmode_kind(nd.mode) == LM_FALLBACK;
+  need[7] = nd.nchunk > 1 ? need[1] : 0u;  // children of a one-block node are binned there
 }
 
 // Adds a block's total of need k to the next level's list or counter; for
-// the three lists the base of the block's range comes back in s_gbase[k].
+// the four lists the base of the block's range comes back in s_gbase[k].
 __device__ inline void sweep_alloc(uint32_t k, uint32_t tot, uint32_t* s_gbase, Counters* ctr,
                                    uint32_t* lvn) {
-  if (k < 3) {
-    uint32_t* counter = k == 0 ? &lvn[LV_NL] : (k == 1 ? &lvn[LV_NC] : &ctr->n_small);
+  if (k < SW_LISTS) {
+    uint32_t* counter = k == 0 ? &lvn[LV_NL]
+                        : k == 1 ? &lvn[LV_NC]
+                        : k == 2 ? &ctr->n_small
+                                 : &ctr->n_med;
     s_gbase[k] = tot ? atomicAdd(counter, tot) : 0u;
   } else if (tot) {
-    if (k == 3) atomicAdd(&ctr->sum_large, (unsigned long long)tot);
-    else atomicAdd(k == 4 ? &ctr->n_window : (k == 5 ? &ctr->n_fallback : &lvn[LV_NBIN]), tot);
+    if (k == 4) atomicAdd(&ctr->sum_large, (unsigned long long)tot);
+    else atomicAdd(k == 5 ? &ctr->n_window : (k == 6 ? &ctr->n_fallback : &lvn[LV_NBIN]), tot);
   }
 }
 
 // Where a split node's children go, from its bases in the next level's
-// large (bl), chunk (bc) and small (bs) lists.
+// large (bl), chunk (bc), small (bs) and medium (bm) lists.
 struct SweepSlots {
-  bool lg, ll;
+  uint32_t rg, rl;  // child_route of the gt and lt child
   uint32_t sgs, sls, c0g, c0l, chg, chl;
 };
 __device__ inline SweepSlots sweep_slots(const LNode& nd, const SplitDec& d, uint32_t bl,
-                                         uint32_t bc, uint32_t bs, uint32_t max_depth) {
+                                         uint32_t bc, uint32_t bs, uint32_t bm,
+                                         const SweepCfg& cf) {
   SweepSlots r;
   const uint32_t cd = nd.depth + 1;
-  r.lg = d.ng > MG_SMALL && cd < max_depth;
-  r.ll = d.nl > MG_SMALL && cd < max_depth;
-  r.chg = r.lg ? (d.ng + CH - 1) / CH : 0;
-  r.chl = r.ll ? (d.nl + CH - 1) / CH : 0;
-  r.sgs = r.lg ? bl : bs;                                    // gt child slot
-  r.sls = r.ll ? bl + (uint32_t)r.lg : bs + (uint32_t)!r.lg;  // lt child slot
+  r.rg = child_route(d.ng, cd, cf.max_depth, cf.med_max, cf.learn_min);
+  r.rl = child_route(d.nl, cd, cf.max_depth, cf.med_max, cf.learn_min);
+  r.chg = r.rg == 1 ? (d.ng + CH - 1) / CH : 0;
+  r.chl = r.rl == 1 ? (d.nl + CH - 1) / CH : 0;
+  // gt child slot first in its list, then the lt child's (after the gt child
+  // when both go to the same list)
+  r.sgs = r.rg == 1 ? bl : (r.rg == 2 ? bm : bs);
+  r.sls = (r.rl == 1 ? bl : (r.rl == 2 ? bm : bs)) + (uint32_t)(r.rl == r.rg);
   r.c0g = bc;
   r.c0l = bc + r.chg;
   return r;
@@ -737,7 +757,7 @@ __device__ inline SweepSlots sweep_slots(const LNode& nd, const SplitDec& d, uin
 // the partition).  One lane; the chunk lists are written by the caller.
 __device__ inline void sweep_emit(LNode* __restrict__ L, uint32_t slot, const LNode& nd,
                                   const SplitDec& d, const SweepSlots& r, LNode* Ln,
-                                  SNode* small, Counters* ctr, const NodeOut& out,
+                                  SNode* small, SNode* med, Counters* ctr, const NodeOut& out,
                                   uint32_t parity_next, uint32_t learn_min, uint32_t* lrows_next,
                                   uint32_t* lvn) {
   Box nb;
@@ -748,17 +768,18 @@ __device__ inline void sweep_emit(LNode* __restrict__ L, uint32_t slot, const LN
   out.write_split(nd.idx, d.axis | (d.kind << 8), d.value, d.nl, fbits(d.cost));
   if (d.kind == 2) atomicAdd(&ctr->n_median, 1u);
   // gt child at P, lt child at P + 1
-  put_child(r.lg, r.sgs, r.c0g, nd.start + d.nl, d.ng, d.gt, nd.P, nd.P + 2, cd, parity_next,
-            nd.nbase, nd.obase, Ln, small, learn_min, lvn, lrows_next, nd.nchunk == 1);
-  put_child(r.ll, r.sls, r.c0l, nd.start, d.nl, d.lt, nd.P + 1, nd.P + 2 + 2 * (d.ng - 1), cd,
-            parity_next, nd.nbase, nd.obase, Ln, small, learn_min, lvn, lrows_next,
+  put_child(r.rg, r.sgs, r.c0g, nd.start + d.nl, d.ng, d.gt, nd.P, nd.P + 2, cd, parity_next,
+            nd.nbase, nd.obase, Ln, small, med, learn_min, lvn, lrows_next, nd.nchunk == 1);
+  put_child(r.rl, r.sls, r.c0l, nd.start, d.nl, d.lt, nd.P + 1, nd.P + 2 + 2 * (d.ng - 1), cd,
+            parity_next, nd.nbase, nd.obase, Ln, small, med, learn_min, lvn, lrows_next,
             nd.nchunk == 1);
   L[slot].axis = d.axis;
   L[slot].cut = d.value;
   L[slot].n_lt = d.nl;
   L[slot].kind = d.kind;
-  L[slot].child_slot[0] = r.lg ? (int32_t)r.sgs : -1;
-  L[slot].child_slot[1] = r.ll ? (int32_t)r.sls : -1;
+  // only next-level children are prebinned by the partition (medium ones bin themselves)
+  L[slot].child_slot[0] = r.rg == 1 ? (int32_t)r.sgs : -1;
+  L[slot].child_slot[1] = r.rl == 1 ? (int32_t)r.sls : -1;
 }
 
 // Levels with few, mostly large nodes: one wavefront per node.  The split
@@ -769,17 +790,17 @@ __device__ inline void sweep_emit(LNode* __restrict__ L, uint32_t slot, const LN
 __device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32_t nL,
                                    const uint32_t* __restrict__ hist,
                                    const Rec* __restrict__ recs, LNode* Ln, uint32_t* cn_next,
-                                   uint32_t* ck_next, SNode* small, Counters* ctr, NodeOut out,
-                                   uint32_t max_depth, uint32_t parity_next, uint32_t learn_min,
+                                   uint32_t* ck_next, SNode* small, SNode* med, Counters* ctr,
+                                   NodeOut out, const SweepCfg& cf, uint32_t parity_next,
                                    uint32_t* lrows_next, uint32_t* lvn) {
   __shared__ uint32_t s_need[SW_WAVES][SW_NEEDS];
-  __shared__ uint32_t s_base[SW_WAVES][3];
-  __shared__ uint32_t s_gbase[3];
+  __shared__ uint32_t s_base[SW_WAVES][SW_LISTS];
+  __shared__ uint32_t s_gbase[SW_LISTS];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t lane = lane_id();
   const uint32_t slot = group * SW_WAVES + w;
   const bool active = slot < nL;  // wave-uniform
-  uint32_t need[SW_NEEDS] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  uint32_t need[SW_NEEDS] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
   LNode nd = {};
   SplitDec d = {};
   if (active) {
@@ -789,7 +810,7 @@ __device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32
       d = *reinterpret_cast<const SplitDec*>(h);
     else
       d = sweep_split(nd, h, recs, out, ctr);
-    sweep_need(nd, d, max_depth, need);
+    sweep_need(nd, d, cf, need);
   }
   if (lane == 0) {
 #pragma unroll
@@ -801,7 +822,7 @@ __device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32
     uint32_t tot = 0;
     for (uint32_t q = 0; q < SW_WAVES; ++q) tot += s_need[q][k];
     sweep_alloc(k, tot, s_gbase, ctr, lvn);
-    if (k < 3) {
+    if (k < SW_LISTS) {
       uint32_t run = s_gbase[k];
       for (uint32_t q = 0; q < SW_WAVES; ++q) {
         s_base[q][k] = run;
@@ -811,9 +832,11 @@ __device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32
   }
   __syncthreads();
   if (!active) return;
-  const SweepSlots r = sweep_slots(nd, d, s_base[w][0], s_base[w][1], s_base[w][2], max_depth);
+  const SweepSlots r =
+      sweep_slots(nd, d, s_base[w][0], s_base[w][1], s_base[w][2], s_base[w][3], cf);
   if (lane == 0)
-    sweep_emit(L, slot, nd, d, r, Ln, small, ctr, out, parity_next, learn_min, lrows_next, lvn);
+    sweep_emit(L, slot, nd, d, r, Ln, small, med, ctr, out, parity_next, cf.learn_min,
+               lrows_next, lvn);
   for (uint32_t k = lane; k < r.chg; k += 64) { cn_next[r.c0g + k] = r.sgs; ck_next[r.c0g + k] = k; }
   for (uint32_t k = lane; k < r.chl; k += 64) { cn_next[r.c0l + k] = r.sls; ck_next[r.c0l + k] = k; }
 }
@@ -831,11 +854,10 @@ __device__ inline void sweep_group(uint32_t group, LNode* __restrict__ L, uint32
 __device__ inline void sweep_lanes(uint32_t group, LNode* __restrict__ L, uint32_t nL,
                                    uint32_t* __restrict__ hist, const Rec* __restrict__ recs,
                                    LNode* Ln, uint32_t* cn_next, uint32_t* ck_next, SNode* small,
-                                   Counters* ctr, NodeOut out, uint32_t max_depth,
-                                   uint32_t parity_next, uint32_t learn_min,
-                                   uint32_t* lrows_next, uint32_t* lvn) {
+                                   SNode* med, Counters* ctr, NodeOut out, const SweepCfg& cf,
+                                   uint32_t parity_next, uint32_t* lrows_next, uint32_t* lvn) {
   __shared__ uint32_t s_wtot[SW_WAVES][SW_NEEDS];
-  __shared__ uint32_t s_gbase[3];
+  __shared__ uint32_t s_gbase[SW_LISTS];
   __shared__ uint32_t s_und[SW_WAVES * 64];
   __shared__ uint32_t s_nund;
   const uint32_t w = threadIdx.x >> 6;
@@ -868,13 +890,13 @@ __device__ inline void sweep_lanes(uint32_t group, LNode* __restrict__ L, uint32
   if (nund) __syncthreads();  // block-uniform: the decisions are visible to their lanes
   SplitDec d = {};
   if (active) d = *reinterpret_cast<const SplitDec*>(hist + (size_t)slot * HIST_WORDS);
-  uint32_t need[SW_NEEDS] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
-  if (active) sweep_need(nd, d, max_depth, need);
-  uint32_t pre[3];
+  uint32_t need[SW_NEEDS] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  if (active) sweep_need(nd, d, cf, need);
+  uint32_t pre[SW_LISTS];
 #pragma unroll
   for (int k = 0; k < SW_NEEDS; ++k) {
     const uint32_t inc = wave_scan_u32_dpp(need[k]);
-    if (k < 3) pre[k] = inc - need[k];
+    if (k < SW_LISTS) pre[k] = inc - need[k];
     const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
     if (lane == 0) s_wtot[w][k] = tot;
   }
@@ -887,15 +909,16 @@ __device__ inline void sweep_lanes(uint32_t group, LNode* __restrict__ L, uint32
   }
   __syncthreads();
   if (!active) return;
-  uint32_t base[3];
+  uint32_t base[SW_LISTS];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < SW_LISTS; ++k) {
     uint32_t b = s_gbase[k] + pre[k];
     for (uint32_t q = 0; q < w; ++q) b += s_wtot[q][k];
     base[k] = b;
   }
-  const SweepSlots r = sweep_slots(nd, d, base[0], base[1], base[2], max_depth);
-  sweep_emit(L, slot, nd, d, r, Ln, small, ctr, out, parity_next, learn_min, lrows_next, lvn);
+  const SweepSlots r = sweep_slots(nd, d, base[0], base[1], base[2], base[3], cf);
+  sweep_emit(L, slot, nd, d, r, Ln, small, med, ctr, out, parity_next, cf.learn_min, lrows_next,
+             lvn);
   for (uint32_t k = 0; k < r.chg; ++k) { cn_next[r.c0g + k] = r.sgs; ck_next[r.c0g + k] = k; }
   for (uint32_t k = 0; k < r.chl; ++k) { cn_next[r.c0l + k] = r.sls; ck_next[r.c0l + k] = k; }
 }
@@ -909,23 +932,25 @@ __global__ __launch_bounds__(1024) void k_sweep(LNode* __restrict__ L,
                                                 uint32_t* __restrict__ hist,
                                                 const Rec* __restrict__ recs, LNode* Ln,
                                                 uint32_t* cn_next, uint32_t* ck_next,
-                                                SNode* small, Counters* ctr, NodeOut out,
-                                                uint32_t max_depth, uint32_t parity_next,
-                                                uint32_t learn_min, uint32_t* lrows_next,
+                                                SNode* small, SNode* med, Counters* ctr,
+                                                NodeOut out, uint32_t max_depth,
+                                                uint32_t parity_next, uint32_t learn_min,
+                                                uint32_t med_max, uint32_t* lrows_next,
                                                 uint32_t* lvn, uint32_t lanes_min) {
   const uint32_t nL = lvc[LV_NL];
+  const SweepCfg cf{max_depth, med_max, learn_min};
   if (nL >= lanes_min) {
     const uint32_t ngroups = (nL + SW_WAVES * 64 - 1) / (SW_WAVES * 64);
     for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
-      sweep_lanes(g, L, nL, hist, recs, Ln, cn_next, ck_next, small, ctr, out, max_depth,
-                  parity_next, learn_min, lrows_next, lvn);
+      sweep_lanes(g, L, nL, hist, recs, Ln, cn_next, ck_next, small, med, ctr, out, cf,
+                  parity_next, lrows_next, lvn);
       __syncthreads();
     }
   } else {
     const uint32_t ngroups = (nL + SW_WAVES - 1) / SW_WAVES;
     for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
-      sweep_group(g, L, nL, hist, recs, Ln, cn_next, ck_next, small, ctr, out, max_depth,
-                  parity_next, learn_min, lrows_next, lvn);
+      sweep_group(g, L, nL, hist, recs, Ln, cn_next, ck_next, small, med, ctr, out, cf,
+                  parity_next, lrows_next, lvn);
       __syncthreads();
     }
   }
@@ -1143,6 +1168,248 @@ __global__ __launch_bounds__(NW * 64) void k_partition(const LNode* __restrict__
   for (uint32_t c = blockIdx.x; c < nC; c += gridDim.x) {
     partition_chunk<NW>(c, L, cnode, ck, blk, src, dst, Ln, hist, fuse);
     __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ medium nodes
+// One wavefront per node of 33..med_max objects builds the part of its
+// subtree above the <= 32-object subtrees, depth first, with the level path's
+// own steps: bin the node's records into 3 x 32 bins in LDS from its exact
+// centroid bounds, take the split from that histogram (split_from_hist, the
+// same function the sweep and the partition use), write the node's records,
+// and partition its records stably into the other record buffer at the same
+// positions while the children's centroid bounds accumulate.  A child with
+// > 32 objects below max_depth goes on an LDS stack, any other to the
+// small-subtree list (appended in batches, one atomic per batch).  The node's
+// records are read and written while it is worked on, so they stay in cache,
+// where the level path streams every record of every level through HBM.
+constexpr uint32_t MED_STACK = 68;  // depth first: one net push per level, max_depth <= 64
+constexpr uint32_t MED_SBUF = 32;   // small children per list append
+
+struct MedEntry {
+  uint32_t start, count, idx, P;
+  uint32_t depth, parity;
+  float bmn[3], bmx[3];
+  uint32_t cacc[6];  // centroid bounds: ~okey(min) x3, okey(max) x3
+};
+
+__device__ inline void med_flush(SNode* sbuf, uint32_t nsb, SNode* __restrict__ small,
+                                 uint32_t small_cap, Counters* ctr, uint32_t& err) {
+  const uint32_t lane = lane_id();
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(&ctr->n_small, nsb);
+  base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+  if ((uint64_t)base + nsb > small_cap) { err |= ERR_INTERNAL; return; }
+  // 64-byte entries, 16 words each: the wavefront copies them word by word
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(sbuf);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(small + base);
+  for (uint32_t t = lane; t < nsb * 16; t += 64) dst[t] = src[t];
+}
+
+__global__ __launch_bounds__(64) void k_medium(const SNode* __restrict__ med,
+                                               const uint32_t* __restrict__ n_med_dev,
+                                               Rec* __restrict__ rec0, Rec* __restrict__ rec1,
+                                               SNode* __restrict__ small, uint32_t small_cap,
+                                               Counters* ctr, NodeOut out, uint32_t max_depth,
+                                               uint32_t nslots) {
+  __shared__ uint32_t h[HIST_WORDS];
+  __shared__ MedEntry stk[MED_STACK];
+  __shared__ SNode sbuf[MED_SBUF];
+  const uint32_t lane = lane_id();
+  const uint32_t nmed = *n_med_dev;
+  uint32_t nsb = 0, err = 0;  // nsb wave-uniform; err per lane
+  unsigned long long visits = 0;
+  uint32_t splits = 0, medians = 0;
+  for (uint32_t m = blockIdx.x; m < nmed; m += gridDim.x) {
+    const SNode root = med[m];
+    const uint32_t nbase = root.nbase, obase = root.obase;
+    {  // the root's centroid bounds
+      const Rec* X = root.parity ? rec1 : rec0;
+      uint32_t acc[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+      for (uint32_t j = lane; j < root.count; j += 64) {
+        Box b;
+        uint32_t id;
+        rec_load(&X[root.start + j], b, id);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const float cc = center(b.mn[a], b.mx[a]);
+          acc[a] = max(acc[a], acc_min_enc(cc));
+          acc[3 + a] = max(acc[3 + a], acc_max_enc(cc));
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) acc[k] = wave_max_u32_dpp(acc[k]);
+      if (lane == 0) {
+        MedEntry& e = stk[0];
+        e.start = root.start; e.count = root.count; e.idx = root.idx; e.P = root.P;
+        e.depth = root.depth; e.parity = root.parity;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { e.bmn[a] = root.bmn[a]; e.bmx[a] = root.bmx[a]; }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) e.cacc[k] = acc[k];
+      }
+    }
+    uint32_t sp = 1;
+    __syncthreads();
+    while (sp > 0) {
+      --sp;
+      const MedEntry e = stk[sp];
+      const Rec* X = e.parity ? rec1 : rec0;
+      Rec* Y = e.parity ? rec0 : rec1;
+      Box nb;
+      float cmin[3], cext[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        nb.mn[a] = e.bmn[a];
+        nb.mx[a] = e.bmx[a];
+        cmin[a] = acc_min_dec(e.cacc[a]);
+        cext[a] = __fsub_rn(acc_max_dec(e.cacc[3 + a]), cmin[a]);
+      }
+      // bin (the partition's prebinning, one object per lane)
+      for (uint32_t t = lane; t < HIST_WORDS; t += 64) h[t] = 0;
+      __syncthreads();
+      for (uint32_t j = lane; j < e.count; j += 64) {
+        Box b;
+        uint32_t id;
+        rec_load(&X[e.start + j], b, id);
+        uint32_t en[6];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { en[k] = acc_min_enc(b.mn[k]); en[3 + k] = acc_max_enc(b.mx[k]); }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const uint32_t bi = bin_index(center(b.mn[a], b.mx[a]), cmin[a], cext[a]);
+          uint32_t* hb = &h[(a * MG_NBINS + bi) * BIN_WORDS];
+          atomicAdd(&hb[0], 1u);
+#pragma unroll
+          for (int k = 0; k < 6; ++k) atomicMax(&hb[1 + k], en[k]);
+        }
+      }
+      __syncthreads();
+      const SplitDec d = split_from_hist(h, nb, cmin, cext, e.start, e.count, X, LM_PLAIN);
+      // never happens: a split leaves both sides non-empty, and the node and
+      // its children's slots lie in the mesh's node range
+      if (d.nl == 0 || d.nl >= e.count || e.idx >= nslots ||
+          (uint64_t)e.P + 2 * (uint64_t)e.count - 2 > nslots) {
+        err |= ERR_INTERNAL;
+        __syncthreads();
+        continue;
+      }
+      if (lane == 0) {
+        out.write_node(e.idx, nb, e.start - obase, e.P - nbase, e.count);
+        out.write_split(e.idx, d.axis | (d.kind << 8), d.value, d.nl, fbits(d.cost));
+      }
+      visits += e.count;
+      ++splits;
+      medians += d.kind == 2 ? 1u : 0u;
+      // stable partition X -> Y: lt objects first, then gt (partition_chunk's rule)
+      // the split axis' values by selects (a run-time index would put arrays in scratch)
+      const uint32_t ax = d.axis;
+      const float pcmin = ax == 0 ? cmin[0] : (ax == 1 ? cmin[1] : cmin[2]);
+      const float pcext = ax == 0 ? cext[0] : (ax == 1 ? cext[1] : cext[2]);
+      uint32_t acc_lt[6] = {0u, 0u, 0u, 0u, 0u, 0u}, acc_gt[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+      uint32_t run_lt = 0, run_gt = 0;
+      for (uint32_t base = 0; base < e.count; base += 64) {
+        const uint32_t i = base + lane;
+        const bool valid = i < e.count;
+        Box b;
+        uint32_t id = 0;
+        bool pred = false;
+        if (valid) {
+          rec_load(&X[e.start + i], b, id);
+          const float lo = ax == 0 ? b.mn[0] : (ax == 1 ? b.mn[1] : b.mn[2]);
+          const float hi = ax == 0 ? b.mx[0] : (ax == 1 ? b.mx[1] : b.mx[2]);
+          pred = d.kind == 0 ? bin_index(center(lo, hi), pcmin, pcext) < d.value : i < d.nl;
+        }
+        const uint64_t ml = __ballot(valid && pred);
+        const uint64_t mg = __ballot(valid && !pred);
+        if (valid) {
+          const uint32_t pos = pred ? e.start + run_lt + lanemask_lt_popc(ml)
+                                    : e.start + d.nl + run_gt + lanemask_lt_popc(mg);
+          // the predicate and the histogram count the same objects, so every
+          // position lies in the node's range; a store outside it is refused
+          const bool in_range = pred ? pos < e.start + d.nl : pos < e.start + e.count;
+          if (in_range) rec_store(&Y[pos], b, id);
+          else err |= ERR_INTERNAL;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const float cc = center(b.mn[a], b.mx[a]);
+            const uint32_t emn = acc_min_enc(cc), emx = acc_max_enc(cc);
+            if (pred) {
+              acc_lt[a] = max(acc_lt[a], emn);
+              acc_lt[3 + a] = max(acc_lt[3 + a], emx);
+            } else {
+              acc_gt[a] = max(acc_gt[a], emn);
+              acc_gt[3 + a] = max(acc_gt[3 + a], emx);
+            }
+          }
+        }
+        run_lt += (uint32_t)__popcll(ml);
+        run_gt += (uint32_t)__popcll(mg);
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        acc_lt[k] = wave_max_u32_dpp(acc_lt[k]);
+        acc_gt[k] = wave_max_u32_dpp(acc_gt[k]);
+      }
+      // the stores above come before any lane reads Y as a child's X
+      __syncthreads();
+      // children (lib.rs:648-663): gt at P, lt at P + 1
+      const uint32_t cd = e.depth + 1;
+#pragma unroll
+      for (int which = 0; which < 2; ++which) {
+        const bool gt = which == 0;
+        const uint32_t cstart = gt ? e.start + d.nl : e.start;
+        const uint32_t ccount = gt ? d.ng : d.nl;
+        const uint32_t cidx = gt ? e.P : e.P + 1;
+        const uint32_t cP = gt ? e.P + 2 : e.P + 2 + 2 * (d.ng - 1);
+        Box cb;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          cb.mn[a] = gt ? d.gt.mn[a] : d.lt.mn[a];
+          cb.mx[a] = gt ? d.gt.mx[a] : d.lt.mx[a];
+        }
+        if (ccount > MG_SMALL && cd < max_depth) {
+          if (sp >= MED_STACK) { err |= ERR_INTERNAL; continue; }
+          if (lane == 0) {
+            MedEntry& c = stk[sp];
+            c.start = cstart; c.count = ccount; c.idx = cidx; c.P = cP;
+            c.depth = cd; c.parity = e.parity ^ 1u;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) { c.bmn[a] = cb.mn[a]; c.bmx[a] = cb.mx[a]; }
+#pragma unroll
+            for (int k = 0; k < 6; ++k) c.cacc[k] = gt ? acc_gt[k] : acc_lt[k];
+          }
+          ++sp;
+        } else {
+          if (lane == 0) {
+            SNode& sn = sbuf[nsb];
+            sn = SNode{};
+            sn.start = cstart; sn.count = ccount; sn.idx = cidx; sn.P = cP;
+            sn.depth = cd; sn.parity = e.parity ^ 1u; sn.nbase = nbase; sn.obase = obase;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) { sn.bmn[a] = cb.mn[a]; sn.bmx[a] = cb.mx[a]; }
+          }
+          ++nsb;
+          if (nsb == MED_SBUF) {
+            __syncthreads();
+            med_flush(sbuf, nsb, small, small_cap, ctr, err);
+            nsb = 0;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (nsb) {
+    __syncthreads();
+    med_flush(sbuf, nsb, small, small_cap, ctr, err);
+  }
+  const bool failed = __ballot(err != 0) != 0;  // a lane's refused store counts too
+  if (lane == 0) {
+    if (visits) atomicAdd(&ctr->sum_large, visits);
+    if (splits) atomicAdd(&ctr->n_med_split, splits);
+    if (medians) atomicAdd(&ctr->n_median, medians);
+    if (failed) atomicOr(&ctr->err, ERR_INTERNAL);
   }
 }
 
@@ -1931,7 +2198,7 @@ int plan_build(mgbvh_ctx* c, Scratch& sc, const BuildInput& in, const std::vecto
   }
   if ((rc = ensure(sc.hist, (size_t)p.capL * HIST_WORDS * 4))) return rc;
   if ((rc = ensure(sc.blk, (size_t)p.capC * 96 * 4))) return rc;
-  if ((rc = ensure(sc.small, (size_t)n * sizeof(SNode)))) return rc;
+  if ((rc = ensure(sc.small, (size_t)small_list_cap(n) * sizeof(SNode)))) return rc;
   if ((rc = ensure(sc.ctr, sizeof(Counters)))) return rc;
   if ((rc = ensure(sc.gaps, ((size_t)n / 2 + 1) * 8))) return rc;
   if ((rc = ensure(sc.mrows, (size_t)p.nm * sizeof(MeshRow)))) return rc;
@@ -1962,6 +2229,9 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
   uint32_t* lrows[2] = {(uint32_t*)sc.lrows[0].p, (uint32_t*)sc.lrows[1].p};
   Counters* ctr = (Counters*)sc.ctr.p;
   SNode* small = (SNode*)sc.small.p;
+  SNode* med = small + n;  // the medium list follows the small-subtree list's n entries
+  // medium nodes only outside sample collection (samples come from the level path's sweeps)
+  const uint32_t med_max = o.samples ? 0u : c->tune.med_max;
   uint32_t* hist = (uint32_t*)sc.hist.p;
   uint32_t* blk = (uint32_t*)sc.blk.p;
   uint32_t* gaps = (uint32_t*)sc.gaps.p;
@@ -2044,9 +2314,9 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
     }
     MG_PROF(KK_SWEEP, 0);
     k_sweep<<<g_sweep, 1024, 0, st>>>(L[cur], lvc, hist, recs[cur], L[cur ^ 1], cn[cur ^ 1],
-                                      ck[cur ^ 1], small, ctr, out, s->max_depth,
-                                      (uint32_t)(cur ^ 1), lp.min_objects, lrows[cur ^ 1], lvn,
-                                      tu.sweep_pack);
+                                      ck[cur ^ 1], small, med, ctr, out, s->max_depth,
+                                      (uint32_t)(cur ^ 1), lp.min_objects, med_max,
+                                      lrows[cur ^ 1], lvn, tu.sweep_pack);
     MG_PROF(KK_SWEEP, 1);
     MG_PROF(KK_PARTITION, 0);
     // children's splits are taken inside the partition unless samples are
@@ -2077,6 +2347,16 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
     cur ^= 1;
   }
   if (levels_run) *levels_run = level;
+  // medium nodes: their subtrees above the <= 32-object subtrees, one wavefront each
+  if (med_max) {
+    const uint32_t g_med =
+        std::max<uint32_t>(1, std::min<uint32_t>(n / (MG_SMALL + 1) + 1, cus * tu.med_bpc));
+    MG_PROF(KK_MEDIUM, 0);
+    k_medium<<<g_med, 64, 0, st>>>(med, &ctr->n_med, recs[0], recs[1], small, n, ctr, out,
+                                   s->max_depth, (uint32_t)p.nslots);
+    MG_PROF(KK_MEDIUM, 1);
+    MG_HIP(hipGetLastError());
+  }
   // subtrees of <= 32 objects (and depth-cut leaves), persistent over their queue
   const uint32_t g_small = std::max<uint32_t>(1, std::min<uint32_t>(cdiv(n, 8), cus * tu.small_bpc));
   MG_PROF(KK_SMALL, 0);
@@ -2111,7 +2391,10 @@ int finish_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, BuildResult* res) {
     set_error("non-finite or out-of-range (|x| >= 2^60) input coordinate");
     return MGBVH_ERR_NONFINITE;
   }
-  if (hc.err & ERR_INTERNAL) { set_error("small-subtree split count out of range"); return MGBVH_ERR_INTERNAL; }
+  if (hc.err & ERR_INTERNAL) {
+    set_error("builder invariant violated (a split count, a list capacity or a stack depth)");
+    return MGBVH_ERR_INTERNAL;
+  }
   if (p.fixed_levels) {  // a replayed graph: did its levels reach every node?
     uint32_t waiting = 0;
     MG_HIP(hipMemcpy(&waiting, (uint32_t*)sc.lv.p + (size_t)p.fixed_levels * LV_WORDS + LV_NL, 4,
@@ -2172,7 +2455,7 @@ int finish_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, BuildResult* res) {
   // level counts: nodes with > 32 objects, and whether a fixed level count sufficed
   std::vector<uint32_t> lvh(((size_t)p.max_levels + 2) * LV_WORDS);
   MG_HIP(hipMemcpy(lvh.data(), sc.lv.p, lvh.size() * 4, hipMemcpyDeviceToHost));
-  uint64_t large_nodes = 0;
+  uint64_t large_nodes = hc.n_med_split;  // split inside medium subtrees
   uint32_t levels = 0;
   for (uint32_t l = 0; l <= p.max_levels; ++l) {
     large_nodes += lvh[(size_t)l * LV_WORDS + LV_NL];

@@ -40,7 +40,7 @@ constexpr NoallocSizes noalloc_sizes(uint64_t n) {
   z.cn = capC * 4;
   z.hist = capL * HIST_WORDS * 4;
   z.blk = capC * 96 * 4;
-  z.small = n * sizeof(SNode);
+  z.small = small_list_cap(n) * sizeof(SNode);
   z.ctr = sizeof(Counters);
   z.gaps = (n / 2 + 1) * 8;
   z.mrows = sizeof(MeshRow);

@@ -510,8 +510,10 @@ int mgbvh_ctx_set_tuning(mgbvh_ctx* c, const mgbvh_tuning* t) {
   auto waves = [](uint32_t v) { return v == 0 || v == 1 || v == 4; };
   if (!ok(t->bin_blocks_per_cu) || !ok(t->partition_blocks_per_cu) ||
       !ok(t->sweep_blocks_per_cu) || !ok(t->small_blocks_per_cu) || !waves(t->partition_waves) ||
-      !(t->bin_waves <= 4)) {
-    set_error("blocks per compute unit must be in [1, 256]; partition_waves 1 or 4, bin_waves <= 4");
+      !(t->bin_waves <= 4) ||
+      !(t->medium_max_objects <= 1 || (t->medium_max_objects > MG_SMALL && t->medium_max_objects <= CH))) {
+    set_error("blocks per compute unit must be in [1, 256]; partition_waves 1 or 4, bin_waves <= 4; "
+              "medium_max_objects 0, 1 or 33..2048");
     return MGBVH_ERR_INVALID_ARG;
   }
   c->tune.bin_bpc = t->bin_blocks_per_cu;
@@ -521,6 +523,7 @@ int mgbvh_ctx_set_tuning(mgbvh_ctx* c, const mgbvh_tuning* t) {
   if (t->partition_waves) c->tune.part_waves = t->partition_waves;
   if (t->bin_waves) c->tune.bin_waves = t->bin_waves;
   if (t->sweep_pack_min_nodes) c->tune.sweep_pack = t->sweep_pack_min_nodes;
+  if (t->medium_max_objects) c->tune.med_max = t->medium_max_objects == 1 ? 0u : t->medium_max_objects;
   return MGBVH_OK;
 }
 
@@ -535,6 +538,7 @@ int mgbvh_ctx_get_tuning(mgbvh_ctx* c, mgbvh_tuning* t) {
   t->partition_waves = c->tune.part_waves;
   t->bin_waves = c->tune.bin_waves;
   t->sweep_pack_min_nodes = c->tune.sweep_pack;
+  t->medium_max_objects = c->tune.med_max ? c->tune.med_max : 1u;
   return MGBVH_OK;
 }
 

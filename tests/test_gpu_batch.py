@@ -256,7 +256,10 @@ def test_scene_graph_replays_equal_level_by_level(ctx):
                                     dict(bin_blocks_per_cu=32, partition_blocks_per_cu=256,
                                          sweep_blocks_per_cu=4, small_blocks_per_cu=256,
                                          partition_waves=1, bin_waves=2,
-                                         sweep_pack_min_nodes=1 << 30)])
+                                         sweep_pack_min_nodes=1 << 30),
+                                    dict(medium_max_objects=33),
+                                    dict(medium_max_objects=512),
+                                    dict(medium_max_objects=2048, partition_waves=4)])
 def test_tuning_does_not_change_results(ctx, tuning):
     print(f"[case] tuning {tuning}", flush=True)
     meshes = _mixed_meshes()
@@ -359,3 +362,39 @@ def test_learned_plan_keeps_its_model(ctx):
     for s_ in (ref, ref_deep, g, other):
         s_.free()
     model.close()
+
+
+def test_autotune_tries_every_candidate_and_keeps_results(ctx):
+    # the autotuner times every grid and medium-node candidate on the scene's
+    # graph, applies the fastest, and the trees it builds then are the ones an
+    # untuned build gives (none of these settings changes results)
+    print("[case] autotune on a 30-mesh scene", flush=True)
+    from marigold_amd import autotune as A
+    from marigold_amd.scene import GpuScene
+    sc = scenes.batched_scene(n_meshes=30, n_instances=120, lo=20, hi=20_000, seed=14)
+    meshes = list(zip(sc["positions"], sc["tris"]))
+    old = ctx.get_tuning()
+    model = ctx.model
+    ctx.set_model(None)
+    try:
+        ref = GpuScene(ctx, meshes, sc["transforms"], sc["instance_mesh"], mode="batched")
+        want = _scene_arrays(ref, None)
+        ref.free()
+        res = A.autotune(ctx, meshes, sc["transforms"], sc["instance_mesh"], reps=1,
+                         use_cache=False)
+        assert len(res["trials"]) == len(A.GRID_CANDIDATES) + len(A.MEDIUM_CANDIDATES)
+        assert res["learned"] is None and res["ms"] > 0
+        tuned = ctx.get_tuning()
+        for k, v in res["tuning"].items():
+            assert tuned[k] == v, (k, tuned[k], v)
+        g = GpuScene(ctx, meshes, sc["transforms"], sc["instance_mesh"], mode="batched",
+                     build=False)
+        g.record_graph()
+        g.build()
+        got = _scene_arrays(g, None)
+        g.free()
+        for k in want:
+            assert np.array_equal(got[k].view(np.uint32), want[k].view(np.uint32)), k
+    finally:
+        ctx.set_tuning(**{k: v for k, v in old.items() if k != "compute_units"})
+        ctx.set_model(model)
