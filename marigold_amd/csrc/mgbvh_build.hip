@@ -80,12 +80,12 @@ __device__ inline void box_grow_nz(Box& acc, const Box& o) {
 // Union of the boxes of lanes [ls, last] that have `take`, on every lane of
 // the segment.  Box coordinates are finite and carry no -0 (object boxes are
 // canonicalised at setup, centroids of such boxes cannot be -0), so the fold
-// order does not change the result and fminf/fmaxf equal the x86 min/max.
+// order does not change the result and the IEEE minimum/maximum (one
+// instruction each, no operand canonicalisation) equal the x86 min/max.
 __device__ inline Box seg_box(bool take, const float* mn, const float* mx, uint32_t lane,
                               uint32_t ls, uint32_t last) {
-  // order-preserving keys: both folds become u32 max with identity 0
-  const auto fmn = [](uint32_t a, uint32_t b) { return fbits(fminf(bitsf(a), bitsf(b))); };
-  const auto fmx = [](uint32_t a, uint32_t b) { return fbits(fmaxf(bitsf(a), bitsf(b))); };
+  const auto fmn = [](uint32_t a, uint32_t b) { return fbits(min_nz(bitsf(a), bitsf(b))); };
+  const auto fmx = [](uint32_t a, uint32_t b) { return fbits(max_nz(bitsf(a), bitsf(b))); };
   uint32_t lo[3], hi[3];
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
