@@ -577,10 +577,13 @@ __device__ inline SplitDec split_from_hist(const uint32_t* h, const Box& nb,
   // candidate key (cost, axis * 32 + cut): the first minimum under total_cmp,
   // axes 0, 1, 2, cuts ascending (lib.rs:373-388, 515-524)
   uint64_t best = ~0ull;
+  // the lane's axis extent by selects (a run-time index would put cext in scratch)
+  const float ce0 = cext[0], ce1 = cext[1], ce2 = cext[2];
+  const float ce = a == 0 ? ce0 : (a == 1 ? ce1 : ce2);
   const auto consider = [&](uint32_t b, uint32_t nl, uint32_t ng, const Box& l, const Box& g) {
     // lib.rs:339-341 (zero extent refused), 379-381 (MIN objects per side)
     const bool inwin = !win || (a == wa && b >= wlo && b <= whi);
-    if (aact && inwin && b >= 1 && cext[a] != 0.0f && nl >= 1 && ng >= 1) {
+    if (aact && inwin && b >= 1 && ce != 0.0f && nl >= 1 && ng >= 1) {
       const float c = split_cost(sa_p, box_sa_ne(l), box_sa_ne(g), nl, ng);  // nl, ng >= 1
       const uint64_t key = ((uint64_t)tc_ukey(c) << 32) | (a * MG_NBINS + b);
       best = key < best ? key : best;
@@ -1185,6 +1188,7 @@ __global__ __launch_bounds__(NW * 64) void k_partition(const LNode* __restrict__
 // where the level path streams every record of every level through HBM.
 constexpr uint32_t MED_STACK = 68;  // depth first: one net push per level, max_depth <= 64
 constexpr uint32_t MED_SBUF = 32;   // small children per list append
+constexpr uint32_t MED_REG = 4;     // records per lane loaded together (and kept, <= 256 objects)
 
 struct MedEntry {
   uint32_t start, count, idx, P;
@@ -1265,23 +1269,39 @@ __global__ __launch_bounds__(64) void k_medium(const SNode* __restrict__ med,
         cmin[a] = acc_min_dec(e.cacc[a]);
         cext[a] = __fsub_rn(acc_max_dec(e.cacc[3 + a]), cmin[a]);
       }
-      // bin (the partition's prebinning, one object per lane)
+      // bin (the partition's prebinning).  Each lane issues the loads of up
+      // to MED_REG records at once (object j = g0 + 64 k + lane); a sub-node
+      // of <= 64 * MED_REG objects keeps them in registers for the partition,
+      // so it reads its records from memory once.
       for (uint32_t t = lane; t < HIST_WORDS; t += 64) h[t] = 0;
       __syncthreads();
-      for (uint32_t j = lane; j < e.count; j += 64) {
-        Box b;
-        uint32_t id;
-        rec_load(&X[e.start + j], b, id);
-        uint32_t en[6];
+      const bool cached = e.count <= 64 * MED_REG;
+      Box rb[MED_REG];
+      uint32_t rid[MED_REG];
+      for (uint32_t g0 = 0; g0 < e.count; g0 += 64 * MED_REG) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) { en[k] = acc_min_enc(b.mn[k]); en[3 + k] = acc_max_enc(b.mx[k]); }
+        for (uint32_t k = 0; k < MED_REG; ++k) {
+          const uint32_t j = g0 + 64 * k + lane;
+          if (j < e.count) rec_load(&X[e.start + j], rb[k], rid[k]);
+        }
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          const uint32_t bi = bin_index(center(b.mn[a], b.mx[a]), cmin[a], cext[a]);
-          uint32_t* hb = &h[(a * MG_NBINS + bi) * BIN_WORDS];
-          atomicAdd(&hb[0], 1u);
+        for (uint32_t k = 0; k < MED_REG; ++k) {
+          const uint32_t j = g0 + 64 * k + lane;
+          if (j >= e.count) continue;
+          uint32_t en[6];
 #pragma unroll
-          for (int k = 0; k < 6; ++k) atomicMax(&hb[1 + k], en[k]);
+          for (int q = 0; q < 3; ++q) {
+            en[q] = acc_min_enc(rb[k].mn[q]);
+            en[3 + q] = acc_max_enc(rb[k].mx[q]);
+          }
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const uint32_t bi = bin_index(center(rb[k].mn[a], rb[k].mx[a]), cmin[a], cext[a]);
+            uint32_t* hb = &h[(a * MG_NBINS + bi) * BIN_WORDS];
+            atomicAdd(&hb[0], 1u);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) atomicMax(&hb[1 + q], en[q]);
+          }
         }
       }
       __syncthreads();
@@ -1308,43 +1328,57 @@ __global__ __launch_bounds__(64) void k_medium(const SNode* __restrict__ med,
       const float pcext = ax == 0 ? cext[0] : (ax == 1 ? cext[1] : cext[2]);
       uint32_t acc_lt[6] = {0u, 0u, 0u, 0u, 0u, 0u}, acc_gt[6] = {0u, 0u, 0u, 0u, 0u, 0u};
       uint32_t run_lt = 0, run_gt = 0;
-      for (uint32_t base = 0; base < e.count; base += 64) {
-        const uint32_t i = base + lane;
-        const bool valid = i < e.count;
-        Box b;
-        uint32_t id = 0;
-        bool pred = false;
-        if (valid) {
-          rec_load(&X[e.start + i], b, id);
-          const float lo = ax == 0 ? b.mn[0] : (ax == 1 ? b.mn[1] : b.mn[2]);
-          const float hi = ax == 0 ? b.mx[0] : (ax == 1 ? b.mx[1] : b.mx[2]);
-          pred = d.kind == 0 ? bin_index(center(lo, hi), pcmin, pcext) < d.value : i < d.nl;
-        }
-        const uint64_t ml = __ballot(valid && pred);
-        const uint64_t mg = __ballot(valid && !pred);
-        if (valid) {
-          const uint32_t pos = pred ? e.start + run_lt + lanemask_lt_popc(ml)
-                                    : e.start + d.nl + run_gt + lanemask_lt_popc(mg);
-          // the predicate and the histogram count the same objects, so every
-          // position lies in the node's range; a store outside it is refused
-          const bool in_range = pred ? pos < e.start + d.nl : pos < e.start + e.count;
-          if (in_range) rec_store(&Y[pos], b, id);
-          else err |= ERR_INTERNAL;
+      // objects in list order: group g0, then k, then lane (as they were binned)
+      for (uint32_t g0 = 0; g0 < e.count; g0 += 64 * MED_REG) {
+        if (!cached) {
 #pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            const float cc = center(b.mn[a], b.mx[a]);
-            const uint32_t emn = acc_min_enc(cc), emx = acc_max_enc(cc);
-            if (pred) {
-              acc_lt[a] = max(acc_lt[a], emn);
-              acc_lt[3 + a] = max(acc_lt[3 + a], emx);
-            } else {
-              acc_gt[a] = max(acc_gt[a], emn);
-              acc_gt[3 + a] = max(acc_gt[3 + a], emx);
-            }
+          for (uint32_t k = 0; k < MED_REG; ++k) {
+            const uint32_t j = g0 + 64 * k + lane;
+            if (j < e.count) rec_load(&X[e.start + j], rb[k], rid[k]);
           }
         }
-        run_lt += (uint32_t)__popcll(ml);
-        run_gt += (uint32_t)__popcll(mg);
+#pragma unroll
+        for (uint32_t k = 0; k < MED_REG; ++k) {
+          const uint32_t base = g0 + 64 * k;
+          if (base >= e.count) break;
+          const uint32_t i = base + lane;
+          const bool valid = i < e.count;
+          Box b;
+          uint32_t id = 0;
+          bool pred = false;
+          if (valid) {
+            b = rb[k];
+            id = rid[k];
+            const float lo = ax == 0 ? b.mn[0] : (ax == 1 ? b.mn[1] : b.mn[2]);
+            const float hi = ax == 0 ? b.mx[0] : (ax == 1 ? b.mx[1] : b.mx[2]);
+            pred = d.kind == 0 ? bin_index(center(lo, hi), pcmin, pcext) < d.value : i < d.nl;
+          }
+          const uint64_t ml = __ballot(valid && pred);
+          const uint64_t mg = __ballot(valid && !pred);
+          if (valid) {
+            const uint32_t pos = pred ? e.start + run_lt + lanemask_lt_popc(ml)
+                                      : e.start + d.nl + run_gt + lanemask_lt_popc(mg);
+            // the predicate and the histogram count the same objects, so every
+            // position lies in the node's range; a store outside it is refused
+            const bool in_range = pred ? pos < e.start + d.nl : pos < e.start + e.count;
+            if (in_range) rec_store(&Y[pos], b, id);
+            else err |= ERR_INTERNAL;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              const float cc = center(b.mn[a], b.mx[a]);
+              const uint32_t emn = acc_min_enc(cc), emx = acc_max_enc(cc);
+              if (pred) {
+                acc_lt[a] = max(acc_lt[a], emn);
+                acc_lt[3 + a] = max(acc_lt[3 + a], emx);
+              } else {
+                acc_gt[a] = max(acc_gt[a], emn);
+                acc_gt[3 + a] = max(acc_gt[3 + a], emx);
+              }
+            }
+          }
+          run_lt += (uint32_t)__popcll(ml);
+          run_gt += (uint32_t)__popcll(mg);
+        }
       }
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
