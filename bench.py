@@ -584,6 +584,51 @@ def scene_main(args):
     print(json.dumps(out), flush=True)
 
 
+def scene_plain_plan(args):
+    """--plain-plan: the batched plain-rule scene build recorded as one scene
+    plan (HIP graph) and replayed `steps` times (with --no-graph: level by
+    level, every kernel class event-timed), for profiler runs of the timed
+    region's kernels; no network is trained and no learned build runs."""
+    from marigold_amd import _native as N
+    from marigold_amd import scenes
+    from marigold_amd.scene import GpuScene
+
+    sc = scenes.batched_scene()
+    meshes = list(zip(sc["positions"], sc["tris"]))
+    n = int(sum(t.shape[0] for _, t in meshes))
+    ctx = N.Context(0)
+    if args.medium_max is not None:
+        ctx.set_tuning(medium_max_objects=args.medium_max)
+    g = GpuScene(ctx, meshes, sc["transforms"], sc["instance_mesh"], mode="batched", build=False)
+    per_kernel = None
+    if args.no_graph:
+        ctx.set_profiling(N.Context.KERNEL_KINDS)
+        for _ in range(args.warmup):
+            g.build()
+        ctx.kernel_times(reset=True)
+    else:
+        g.record_graph()
+        for _ in range(args.warmup):
+            g.build()
+    ctx.synchronize()
+    ctx.record(0)
+    for _ in range(args.steps):
+        g.build()
+    ctx.record(1)
+    ctx.synchronize()
+    ms = ctx.elapsed_ms(0, 1) / args.steps
+    if args.no_graph:
+        per_kernel = {k: v[0] / args.steps for k, v in ctx.kernel_times(reset=True).items()}
+        ctx.set_profiling([])
+    print(json.dumps({"workload": "2,000-mesh scene, batched plain split rule",
+                      "timed": "level-by-level builds" if args.no_graph else "scene-plan graph replays",
+                      "tris": n, "steps": args.steps, "ms_per_step": ms,
+                      "mtri_per_s": n / (ms * 1e-3) / 1e6, "tuning": ctx.get_tuning(),
+                      "kernel_ms_per_build": per_kernel}), flush=True)
+    g.free()
+    ctx.close()
+
+
 def log(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -604,11 +649,18 @@ def main():
                     help="6 training epochs instead of 45 (profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with one 1 GiB device copy (counter calibration)")
+    ap.add_argument("--plain-plan", action="store_true",
+                    help="only the batched plain-rule scene plan, replayed (profiler runs of the "
+                         "timed region)")
+    ap.add_argument("--medium-max", type=int, default=None,
+                    help="tuning: medium_max_objects of the --plain-plan build")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: multi-GPU sharding "
               f"of the meshes is a design note only", file=sys.stderr)
         raise SystemExit(2)
+    if args.plain_plan:
+        return scene_plain_plan(args)
     if args.workload == "scene":
         return scene_main(args)
 
