@@ -620,11 +620,24 @@ def scene_plain_plan(args):
     if args.no_graph:
         per_kernel = {k: v[0] / args.steps for k, v in ctx.kernel_times(reset=True).items()}
         ctx.set_profiling([])
+    wide8 = None
+    if args.wide8:
+        # every BLAS collapsed to the compressed 8-wide format in one call, 3 times
+        d_wide, d_tri = ctx.alloc(N.WIDE_NODE_BYTES * n), ctx.alloc(4 * n)
+        ms_w = []
+        for _ in range(3):
+            t_w = time.perf_counter()
+            _, _, n_wide = ctx.wide8_collapse_batch(g.d["blas_nodes"], None, g.n_slots, g.blas,
+                                                    g.blas_node_counts, d_wide, n, d_tri, n)
+            ms_w.append((time.perf_counter() - t_w) * 1e3)
+        d_wide.free()
+        d_tri.free()
+        wide8 = {"wide_nodes": n_wide, "ms_per_call": ms_w}
     print(json.dumps({"workload": "2,000-mesh scene, batched plain split rule",
                       "timed": "level-by-level builds" if args.no_graph else "scene-plan graph replays",
                       "tris": n, "steps": args.steps, "ms_per_step": ms,
                       "mtri_per_s": n / (ms * 1e-3) / 1e6, "tuning": ctx.get_tuning(),
-                      "kernel_ms_per_build": per_kernel}), flush=True)
+                      "kernel_ms_per_build": per_kernel, "wide8_collapse": wide8}), flush=True)
     g.free()
     ctx.close()
 
@@ -654,6 +667,8 @@ def main():
                          "timed region)")
     ap.add_argument("--medium-max", type=int, default=None,
                     help="tuning: medium_max_objects of the --plain-plan build")
+    ap.add_argument("--wide8", action="store_true",
+                    help="--plain-plan: then collapse every BLAS to the 8-wide format 3 times")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: multi-GPU sharding "

@@ -89,8 +89,8 @@ void scratch_free(Scratch& sc) {
                     &sc.cn[0],   &sc.cn[1],   &sc.ck[0],    &sc.ck[1],    &sc.small, &sc.ctr,
                     &sc.gaps,    &sc.lens,    &sc.mrows,    &sc.macc,     &sc.mnb,   &sc.lrows[0],
                     &sc.lrows[1], &sc.lv,     &sc.lfeat,    &sc.llogit,   &sc.gtab,  &sc.tmpn,
-                    &sc.tmpl,    &sc.tmps,    &sc.w8rec,    &sc.w8par,    &sc.w8front[0],
-                    &sc.w8front[1], &sc.w8ctr,   &sc.w8base};
+                    &sc.tmpl,    &sc.tmps,    &sc.w8rec,    &sc.w8par,    &sc.w8e,
+                    &sc.w8fast,  &sc.w8front[0], &sc.w8front[1], &sc.w8ctr, &sc.w8base};
   for (DevBuf* b : bufs) free_buf(*b);
 }
 

@@ -236,7 +236,9 @@ namespace mg {
 struct Scratch {
   DevBuf recs[2], L[2], hist, blk, cn[2], ck[2], small, ctr, gaps, lens, mrows, macc, mnb;
   DevBuf lrows[2], lv, lfeat, llogit, gtab, tmpn, tmpl, tmps;
-  DevBuf w8rec, w8par, w8front[2], w8ctr, w8base;  // 8-wide collapse (its own: plans keep pointers)
+  // 8-wide collapse (its own: plans keep pointers): level-path records and
+  // node lists, layout entries, the subtree-local path's tables, frontiers
+  DevBuf w8rec, w8par, w8e, w8fast, w8front[2], w8ctr, w8base;
   Counters* h_ctr = nullptr;  // pinned
 };
 void scratch_free(Scratch& sc);

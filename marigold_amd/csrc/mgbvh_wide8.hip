@@ -30,26 +30,38 @@ namespace mg {
 constexpr uint32_t W8_NONE = 0xffffffffu;
 constexpr int W8_STACK = 256;  // traversal: at most 7 more entries per wide level
 
-struct W8Rec {   // per binary node
+// Per binary node.  The objects below any set of slots that covers v's
+// subtree are all of its p objects (each object sits in exactly one wide
+// leaf), so only the wide-node counts need a value per budget.
+struct W8Rec {
   float c[7];    // C(v, i), i = 1..7
+  float c8;      // area * c_node + the best split of 8 slots (v as a wide internal node)
   uint8_t dec[8];  // [0]: 1 = leaf at one slot; [i-1], i = 2..7: 0 reduce, k split; [7]: k of 8
-  uint32_t p, w, t;  // objects; wide nodes and objects of the wide subtree rooted here
-  uint32_t ws[7], ts[7];  // wide nodes and objects below the slots of budget i, i = 1..7
+  uint32_t p, w;   // objects; wide nodes of the wide subtree rooted here
+  uint32_t ws[7];  // wide nodes below the slots of budget i, i = 1..7
 };
-static_assert(sizeof(W8Rec) == 104, "W8Rec is 104 bytes");
+static_assert(sizeof(W8Rec) == 76, "W8Rec is 76 bytes");
+
+// What the layout reads of a binary node: decisions, objects, wide nodes.
+struct W8E {
+  uint8_t dec[8];
+  uint32_t p, w;
+};
+static_assert(sizeof(W8E) == 16, "W8E is 16 bytes");
 
 struct W8Item {  // a wide internal node to write
   uint32_t bin;           // binary node (absolute index)
   uint32_t idx, region;   // wide node and its descendants' first index, relative to wbase
   uint32_t tri;           // first object slot of its leaves, relative to tbase
   uint32_t wbase, tbase;  // the tree's first wide node and object slot
+  uint32_t bbase, pad;    // the tree's first binary node (child indices are relative to it)
 };
 
 // Absolute index of v's first child (0: a leaf).  In a forest of BLASes the
-// child indices are relative to the BLAS's first node, base[v].
-__device__ inline uint32_t w8_child(const mgbvh_node32* nodes, const uint32_t* base, uint32_t v) {
+// child indices are relative to the BLAS's first node, bbase.
+__device__ inline uint32_t w8_child(const mgbvh_node32* nodes, uint32_t bbase, uint32_t v) {
   const uint32_t ch = nodes[v].child_node;
-  return ch == 0 ? 0u : (base ? base[v] + ch : ch);
+  return ch == 0 ? 0u : bbase + ch;
 }
 
 __device__ inline float w8_area(const float* mn, const float* mx) {
@@ -71,24 +83,31 @@ __device__ inline void w8_box(const mgbvh_node32* nodes, uint32_t v, float* mn, 
   child = b.w;
 }
 
+__device__ inline float w8_node_area(const mgbvh_node32* nodes, uint32_t v) {
+  float mn[3], mx[3];
+  uint32_t st, ch;
+  w8_box(nodes, v, mn, mx, st, ch);
+  return w8_area(mn, mx);
+}
+
 // Slots of budget j at binary node m, left child first (an explicit stack).
-__device__ inline int w8_expand(const mgbvh_node32* nodes, const uint32_t* base, uint32_t nn,
-                                const W8Rec* rec, uint32_t m, uint32_t j, uint32_t* out, int n) {
+__device__ inline int w8_expand(const mgbvh_node32* nodes, uint32_t bbase, uint32_t nn,
+                                const W8E* E, uint32_t m, uint32_t j, uint32_t* out, int n) {
   uint32_t sm[16], sj[16];
   int sp = 0;
   sm[sp] = m; sj[sp] = j; ++sp;
   while (sp > 0) {
     --sp;
     uint32_t u = sm[sp], b = sj[sp];
-    const uint32_t ch = w8_child(nodes, base, u);
+    const uint32_t ch = w8_child(nodes, bbase, u);
     if (ch != 0 && ch + 1 >= nn) return 99;
-    while (b > 1 && ch != 0 && rec[u].dec[b - 1] == 0) --b;
+    while (b > 1 && ch != 0 && E[u].dec[b - 1] == 0) --b;
     if (b == 1 || ch == 0) {
       if (n < 8) out[n] = u;
       ++n;
       continue;
     }
-    const uint32_t k = rec[u].dec[b - 1];
+    const uint32_t k = E[u].dec[b - 1];
     if (sp + 2 > 16) return 99;  // cannot happen: a budget of 8 splits at most 7 times
     sm[sp] = ch + 1; sj[sp] = b - k; ++sp;  // right pushed first: left expands first
     sm[sp] = ch; sj[sp] = k; ++sp;
@@ -97,31 +116,48 @@ __device__ inline int w8_expand(const mgbvh_node32* nodes, const uint32_t* base,
 }
 
 // Slots of a wide internal node built from v.
-__device__ inline int w8_slots(const mgbvh_node32* nodes, const uint32_t* base, uint32_t nn,
-                               const W8Rec* rec, uint32_t v, uint32_t* out) {
-  const uint32_t ch = w8_child(nodes, base, v);
+__device__ inline int w8_slots(const mgbvh_node32* nodes, uint32_t bbase, uint32_t nn,
+                               const W8E* E, uint32_t v, uint32_t* out) {
+  const uint32_t ch = w8_child(nodes, bbase, v);
   if (ch == 0) { out[0] = v; return 1; }
   if (ch + 1 >= nn) return 99;
-  const uint32_t k = rec[v].dec[7];
+  const uint32_t k = E[v].dec[7];
   if (k < 1 || k > 7) return 99;
-  const int n = w8_expand(nodes, base, nn, rec, ch, k, out, 0);
-  return n > 8 ? n : w8_expand(nodes, base, nn, rec, ch + 1, 8 - k, out, n);
+  const int n = w8_expand(nodes, bbase, nn, E, ch, k, out, 0);
+  return n > 8 ? n : w8_expand(nodes, bbase, nn, E, ch + 1, 8 - k, out, n);
 }
 
-__device__ inline bool w8_leaf_slot(const mgbvh_node32* nodes, const W8Rec* rec, uint32_t m) {
-  return nodes[m].child_node == 0 || rec[m].dec[0] == 1;
+__device__ inline bool w8_leaf_slot(const mgbvh_node32* nodes, const W8E* E, uint32_t m) {
+  return nodes[m].child_node == 0 || E[m].dec[0] == 1;
 }
 
-__device__ void w8_node_dp(const mgbvh_node32* nodes, const uint32_t* base, W8Rec* rec,
-                           uint32_t v, float c_node, float c_prim, uint32_t p_max) {
-  float mn[3], mx[3];
-  uint32_t start, L;
-  w8_box(nodes, v, mn, mx, start, L);
-  L = w8_child(nodes, base, v);
-  const float area = w8_area(mn, mx);
-  const W8Rec a = rec[L], b = rec[L + 1];
+// The record of a binary leaf of p objects.
+__device__ inline W8Rec w8_leaf_rec(float area, uint32_t p, float c_prim) {
   W8Rec r;
-  r.p = a.p + b.p;
+  r.p = p;
+  const float cl = __fmul_rn(__fmul_rn(area, (float)p), c_prim);
+#pragma unroll
+  for (int i = 0; i < 7; ++i) { r.c[i] = cl; r.ws[i] = 0; }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) r.dec[i] = 0;
+  r.dec[0] = 1;
+  r.w = 0;
+  r.c8 = cl;
+  return r;
+}
+
+// The record of an internal node of surface area `area` from its children's
+// records A (first child) and B (second).  Costs and counts are read into
+// registers at fixed indices; the per-budget wide-node counts are read where
+// the decisions pick them, through the pointers (A and B may be global or
+// LDS memory; a local copy indexed at run time would live in scratch).
+__device__ inline W8Rec w8_combine(const W8Rec* A, const W8Rec* B, float area, float c_node,
+                                   float c_prim, uint32_t p_max) {
+  float ac[7], bc[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) { ac[k] = A->c[k]; bc[k] = B->c[k]; }
+  W8Rec r;
+  r.p = A->p + B->p;
   float dist[9];
   uint32_t arg[9];
 #pragma unroll
@@ -131,7 +167,7 @@ __device__ void w8_node_dp(const mgbvh_node32* nodes, const uint32_t* base, W8Re
 #pragma unroll
     for (uint32_t k = 1; k < i; ++k) {
       if (k > 7 || i - k > 7) continue;
-      const float s = __fadd_rn(a.c[k - 1], b.c[i - k - 1]);
+      const float s = __fadd_rn(ac[k - 1], bc[i - k - 1]);
       if (s < best) { best = s; bk = k; }
     }
     dist[i] = best;
@@ -142,6 +178,7 @@ __device__ void w8_node_dp(const mgbvh_node32* nodes, const uint32_t* base, W8Re
                                    : __uint_as_float(0x7f800000u);
   // p <= p_max is part of the rule: when the sums overflow, cint is +inf too
   const bool leaf = r.p <= p_max && cleaf <= cint;
+  r.c8 = cint;
   r.dec[0] = leaf ? 1 : 0;
   r.c[0] = leaf ? cleaf : cint;
 #pragma unroll
@@ -152,18 +189,26 @@ __device__ void w8_node_dp(const mgbvh_node32* nodes, const uint32_t* base, W8Re
   }
   r.dec[7] = (uint8_t)arg[8];
   // sizes from the children's per-budget sums: no walk below the children
-  const uint32_t k8 = arg[8];
-  r.w = 1 + a.ws[k8 - 1] + b.ws[8 - k8 - 1];
-  r.t = a.ts[k8 - 1] + b.ts[8 - k8 - 1];
+  const uint32_t k8 = arg[8];  // 1..7
+  r.w = 1 + A->ws[k8 - 1] + B->ws[8 - k8 - 1];
   r.ws[0] = leaf ? 0u : r.w;
-  r.ts[0] = leaf ? r.p : r.t;
 #pragma unroll
   for (uint32_t i = 2; i <= 7; ++i) {
-    const uint32_t k = r.dec[i - 1];
-    r.ws[i - 1] = k == 0 ? r.ws[i - 2] : a.ws[k - 1] + b.ws[i - k - 1];
-    r.ts[i - 1] = k == 0 ? r.ts[i - 2] : a.ts[k - 1] + b.ts[i - k - 1];
+    const uint32_t k = r.dec[i - 1];  // 0 or 1..i-1
+    const uint32_t kk = k == 0 ? 1u : k;
+    const uint32_t split = A->ws[kk - 1] + B->ws[i - kk - 1];
+    r.ws[i - 1] = k == 0 ? r.ws[i - 2] : split;
   }
-  rec[v] = r;
+  return r;
+}
+
+__device__ inline W8E w8_e(const W8Rec& r) {
+  W8E e;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) e.dec[i] = r.dec[i];
+  e.p = r.p;
+  e.w = r.w;
+  return e;
 }
 
 // Level lists top-down: every node of level l appends its two children to the
@@ -181,7 +226,7 @@ __global__ __launch_bounds__(1024) void k_w8_next_level(const mgbvh_node32* __re
   uint32_t v = 0, ch = 0;
   if (i < n_in) {
     v = in[i];
-    ch = v < nn ? w8_child(nodes, base, v) : 0u;
+    ch = v < nn ? w8_child(nodes, base ? base[v] : 0u, v) : 0u;
     if (ch != 0 && (ch <= v || ch + 1 >= nn)) { atomicOr(err_flag, 16u); ch = 0; }
   }
   const uint64_t mask = __ballot(ch != 0);
@@ -201,39 +246,347 @@ __global__ __launch_bounds__(1024) void k_w8_next_level(const mgbvh_node32* __re
 }
 
 // One binary depth: leaves get their leaf record, internal nodes combine their
-// children's (one depth deeper, written by the previous launch).
+// children's (one depth deeper, written by the previous launch).  Every node's
+// layout entry (decisions, objects, wide nodes) goes to E as well.
 __global__ __launch_bounds__(256) void k_w8_dp_level(const mgbvh_node32* __restrict__ nodes,
                                                      const uint32_t* __restrict__ base,
                                                      const uint32_t* __restrict__ lens,
                                                      uint32_t nn, const uint32_t* __restrict__ order,
-                                                     uint32_t count, W8Rec* rec, float c_node,
-                                                     float c_prim, uint32_t p_max,
+                                                     uint32_t count, W8Rec* rec, W8E* E,
+                                                     float c_node, float c_prim, uint32_t p_max,
                                                      uint32_t* err_flag) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const uint32_t v = order[i];
   if (v >= nn) return;
-  float mn[3], mx[3];
-  uint32_t start, ch;
-  w8_box(nodes, v, mn, mx, start, ch);
-  uint32_t err = 0;
-  if (ch != 0) {
-    w8_node_dp(nodes, base, rec, v, c_node, c_prim, p_max);
+  const float area = w8_node_area(nodes, v);
+  const uint32_t L = w8_child(nodes, base ? base[v] : 0u, v);
+  W8Rec r;
+  if (L != 0) {
+    r = w8_combine(&rec[L], &rec[L + 1], area, c_node, c_prim, p_max);
   } else {
-    W8Rec r;
-    r.p = lens ? lens[v] : 1u;
-    if (r.p > p_max) err |= 8u;
-    const float cl = __fmul_rn(__fmul_rn(w8_area(mn, mx), (float)r.p), c_prim);
-#pragma unroll
-    for (int i = 0; i < 7; ++i) { r.c[i] = cl; r.ws[i] = 0; r.ts[i] = r.p; }
-#pragma unroll
-    for (int i = 0; i < 8; ++i) r.dec[i] = 0;
-    r.dec[0] = 1;
-    r.w = 0;
-    r.t = r.p;
-    rec[v] = r;
+    r = w8_leaf_rec(area, lens ? lens[v] : 1u, c_prim);
+    if (r.p > p_max) atomicOr(err_flag, 8u);
   }
-  if (err) atomicOr(err_flag, err);
+  rec[v] = r;
+  E[v] = w8_e(r);
+}
+
+// ------------------------------------------------------------ subtree-local DP
+// The dynamic program without global level lists, for trees built here: a
+// node's descendants occupy one contiguous index range starting at its first
+// child (children follow their parents, pairs in depth-first order).
+//   phase 0 (k_w8_top, one block per tree): the nodes with more than
+//     W8_SUB objects form the top of the tree; a breadth-first pass over
+//     them lists, level by level, the top nodes and, as subtree roots, their
+//     children with at most W8_SUB objects;
+//   phase 1 (k_w8_sub, one block per subtree root): the subtree's levels and
+//     its DP, bottom up, with every record in LDS; each node's layout entry
+//     and the root's record go to memory;
+//   phase 2 (k_w8_topdp, one block per tree): the top nodes' DP, bottom up.
+// Object counts come from lens or, without lens (one object per leaf), from
+// the first children's start indices (a node's lt child starts where it
+// does, its gt child n_lt objects later).  A tree that breaks these
+// assumptions (a subtree past its range, a count that does not add up, a
+// capacity exceeded) raises a flag and the level path runs instead.
+constexpr uint32_t W8_SUB = 64;                // objects of a subtree built in LDS
+constexpr uint32_t W8_SUBN = 2 * W8_SUB - 1;   // its nodes at most
+constexpr uint32_t W8_MAXLV = 64;              // levels of a top part or a subtree
+constexpr uint32_t W8_TAG = 0x80000000u;       // a top entry's child: a subtree slot
+constexpr uint32_t W8_FAST_FAIL = 32u;         // err flag: the fast path does not apply
+
+struct W8Top {   // a top node: n > W8_SUB objects
+  uint32_t v, n;
+  uint32_t ch[2];  // first and second child: top entry index, or W8_TAG | subtree slot
+};
+struct W8Sub {   // a subtree root: n <= W8_SUB objects, or a leaf
+  uint32_t root, bbase, n, pad;
+};
+struct W8Tree {  // one tree of the forest
+  uint32_t first, count;  // its nodes [first, first + count); child indices relative to bbase
+  uint32_t bbase, toff, tcap;
+  uint32_t ntop, nlev;    // top entries and their levels (phase 0)
+  uint32_t root_src;      // the root's record: top entry 0, or W8_TAG | subtree slot
+};
+
+// Object count of u (first child c of a node of n objects at k = 0, second at k = 1).
+__device__ inline uint32_t w8_count(const mgbvh_node32* nodes, const uint32_t* lens, uint32_t c,
+                                    uint32_t n, int k) {
+  if (lens) return lens[c + k];
+  const uint32_t n_lt = nodes[c].start_index - nodes[c + 1].start_index;
+  return k == 1 ? n_lt : n - n_lt;
+}
+
+__global__ __launch_bounds__(256) void k_w8_top(const mgbvh_node32* __restrict__ nodes,
+                                                const uint32_t* __restrict__ lens, uint32_t nn,
+                                                W8Tree* __restrict__ trees, W8Top* __restrict__ top,
+                                                uint32_t* __restrict__ lvstart,
+                                                W8Sub* __restrict__ sub, uint32_t sub_cap,
+                                                uint32_t* ctr) {
+  __shared__ uint32_t s_top, s_sub, s_base;
+  const uint32_t tid = threadIdx.x;
+  W8Tree& T = trees[blockIdx.x];
+  const uint32_t first = T.first, count = T.count, bb = T.bbase;
+  if (count == 0) return;
+  uint32_t* lv = lvstart + (size_t)blockIdx.x * (W8_MAXLV + 1);
+  const uint32_t root = first;
+  const uint32_t nroot = lens ? lens[root] : (count + 1) / 2;
+  if (nroot <= W8_SUB || nodes[root].child_node == 0) {
+    if (tid == 0) {
+      const uint32_t slot = atomicAdd(&ctr[4], 1u);
+      if (slot < sub_cap) sub[slot] = W8Sub{root, bb, nroot, 0u};
+      else atomicOr(&ctr[0], W8_FAST_FAIL);
+      T.ntop = 0;
+      T.nlev = 0;
+      T.root_src = W8_TAG | slot;
+    }
+    return;
+  }
+  W8Top* E0 = top + T.toff;
+  const uint32_t cap = T.tcap;
+  if (tid == 0) {
+    E0[0] = W8Top{root, nroot, {0u, 0u}};
+    lv[0] = 0;
+    lv[1] = 1;
+    T.root_src = 0;
+  }
+  __syncthreads();
+  uint32_t lo = 0, hi = 1, level = 0;
+  bool fail = false;
+  while (lo < hi && !fail) {
+    if (level + 2 > W8_MAXLV) { fail = true; break; }
+    if (tid == 0) { s_top = hi; s_sub = 0; }
+    __syncthreads();
+    // children: top nodes appended to the next level, subtree roots counted
+    for (uint32_t i = lo + tid; i < hi; i += blockDim.x) {
+      W8Top e = E0[i];
+      const uint32_t ch = nodes[e.v].child_node;
+      const uint32_t c = bb + ch;
+      bool bad = ch == 0 || c <= e.v || c + 1 >= first + count || c + 1 >= nn;
+      uint32_t nc[2] = {0u, 0u};
+      if (!bad) {
+        nc[0] = w8_count(nodes, lens, c, e.n, 0);
+        nc[1] = w8_count(nodes, lens, c, e.n, 1);
+        bad = nc[0] == 0 || nc[1] == 0 || nc[0] >= e.n || nc[1] >= e.n || nc[0] + nc[1] != e.n;
+      }
+      if (bad) { atomicOr(&ctr[0], W8_FAST_FAIL); continue; }
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const uint32_t u = c + k;
+        if (nc[k] > W8_SUB && nodes[u].child_node != 0) {
+          const uint32_t t = atomicAdd(&s_top, 1u);
+          if (t < cap) { E0[t] = W8Top{u, nc[k], {0u, 0u}}; e.ch[k] = t; }
+          else atomicOr(&ctr[0], W8_FAST_FAIL);
+        } else {
+          e.ch[k] = W8_TAG | atomicAdd(&s_sub, 1u);  // level-local, rebased below
+        }
+      }
+      E0[i].ch[0] = e.ch[0];
+      E0[i].ch[1] = e.ch[1];
+    }
+    __syncthreads();
+    if (tid == 0) s_base = s_sub ? atomicAdd(&ctr[4], s_sub) : 0u;
+    __syncthreads();
+    // subtree roots of this level into their slots
+    for (uint32_t i = lo + tid; i < hi; i += blockDim.x) {
+      const W8Top e = E0[i];
+      const uint32_t c = bb + nodes[e.v].child_node;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (!(e.ch[k] & W8_TAG)) continue;
+        const uint32_t slot = s_base + (e.ch[k] & ~W8_TAG);
+        if (slot < sub_cap)
+          sub[slot] = W8Sub{c + k, bb, w8_count(nodes, lens, c, e.n, k), 0u};
+        else atomicOr(&ctr[0], W8_FAST_FAIL);
+        E0[i].ch[k] = W8_TAG | slot;
+      }
+    }
+    const uint32_t nhi = min(s_top, cap);
+    __syncthreads();
+    lo = hi;
+    hi = nhi;
+    ++level;
+    if (tid == 0) lv[level + 1] = hi;
+  }
+  if (tid == 0) {
+    T.ntop = hi;
+    T.nlev = level;
+    if (fail) atomicOr(&ctr[0], W8_FAST_FAIL);
+  }
+}
+
+// Phase 1: one wavefront per subtree root, persistent over the list (four
+// wavefronts per block, each on its own subtree, synchronised per wavefront
+// only).  Local index of a node: 0 for the root, 1 + (u - first child) for a
+// descendant.  The child index and surface area of every node of the
+// subtree's range are loaded into LDS first, in one coalesced pass, so the
+// level pass and the DP wait on no memory round trip per level.
+struct W8SubLds {
+  W8Rec R[W8_SUBN];
+  uint32_t lch[W8_SUBN];  // child_node of each local node (0: leaf)
+  float lar[W8_SUBN];     // its surface area
+  uint32_t lvs[W8_MAXLV + 2];
+  uint32_t n;
+  uint8_t order[W8_SUBN + 1];  // local indices level by level
+};
+static_assert(W8_SUBN < 256, "local indices fit a byte");
+
+// LDS writes of the wavefront's lanes visible to all of them.
+__device__ inline void w8_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(256) void k_w8_sub(const mgbvh_node32* __restrict__ nodes,
+                                                const uint32_t* __restrict__ lens, uint32_t nn,
+                                                const W8Sub* __restrict__ sub, const uint32_t* ctr_in,
+                                                W8Rec* __restrict__ rsub, W8E* __restrict__ E,
+                                                float c_node, float c_prim, uint32_t p_max,
+                                                uint32_t* ctr) {
+  __shared__ W8SubLds lds[4];
+  const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+  W8SubLds& L = lds[wv];
+  const uint32_t nsub = ctr_in[4];
+  for (uint32_t slot = blockIdx.x * 4 + wv; slot < nsub; slot += gridDim.x * 4) {
+    const W8Sub S = sub[slot];
+    const uint32_t r = S.root, bb = S.bbase;
+    const uint32_t rch = nodes[r].child_node;
+    if (rch == 0) {  // a leaf
+      if (lane == 0) {
+        const W8Rec x = w8_leaf_rec(w8_node_area(nodes, r), lens ? lens[r] : 1u, c_prim);
+        if (x.p > p_max) atomicOr(&ctr[0], 8u);
+        rsub[slot] = x;
+        E[r] = w8_e(x);
+      }
+      continue;
+    }
+    const uint32_t c0 = bb + rch;
+    // descendants: at most 2 n - 2 local slots after the root's
+    const uint32_t nloc = min(2u * S.n - 1u, W8_SUBN);
+    if (S.n > W8_SUB || c0 <= r || c0 >= nn) {  // wave-uniform
+      if (lane == 0) atomicOr(&ctr[0], W8_FAST_FAIL);
+      continue;
+    }
+    // u's first child at local index cl is usable: after u, inside the
+    // subtree's range and the array (the same test in both passes)
+    const auto child_ok = [&](uint32_t u, uint32_t ch, uint32_t cl) {
+      return bb + ch > u && cl >= 1 && cl + 1 < nloc && bb + ch + 1 < nn;
+    };
+    // the range's nodes (those past the array are never reached: their
+    // parents' child test fails first)
+    for (uint32_t loc = lane; loc < nloc; loc += 64) {
+      const uint32_t u = loc == 0 ? r : c0 + loc - 1;
+      uint32_t ch = 0;
+      float ar = 0.0f;
+      if (u < nn) {
+        float mn[3], mx[3];
+        uint32_t st;
+        w8_box(nodes, u, mn, mx, st, ch);
+        ar = w8_area(mn, mx);
+      }
+      L.lch[loc] = ch;
+      L.lar[loc] = ar;
+    }
+    if (lane == 0) { L.order[0] = 0; L.lvs[0] = 0; L.lvs[1] = 1; L.n = 1; }
+    w8_wave_sync();
+    // levels, top down
+    uint32_t nlev = 1;
+    bool fail = false;
+    for (;;) {
+      const uint32_t a = L.lvs[nlev - 1], b = L.lvs[nlev];
+      if (a == b) break;
+      for (uint32_t i = a + lane; i < b; i += 64) {
+        const uint32_t loc = L.order[i];
+        const uint32_t u = loc == 0 ? r : c0 + loc - 1;
+        const uint32_t ch = L.lch[loc];
+        if (ch == 0) continue;
+        const uint32_t cl = bb + ch - c0 + 1;  // local index of u's first child
+        if (!child_ok(u, ch, cl)) {
+          atomicOr(&ctr[0], W8_FAST_FAIL);
+          continue;
+        }
+        const uint32_t k = atomicAdd(&L.n, 2u);
+        if (k + 1 < W8_SUBN) { L.order[k] = (uint8_t)cl; L.order[k + 1] = (uint8_t)(cl + 1); }
+        else atomicOr(&ctr[0], W8_FAST_FAIL);
+      }
+      w8_wave_sync();
+      if (lane == 0) L.lvs[nlev + 1] = min(L.n, W8_SUBN);
+      w8_wave_sync();
+      ++nlev;
+      if (nlev + 1 > W8_MAXLV + 1) {
+        fail = true;
+        if (lane == 0) atomicOr(&ctr[0], W8_FAST_FAIL);
+        break;
+      }
+    }
+    // the DP, bottom up
+    for (int l = (int)nlev - 1; l >= 0 && !fail; --l) {
+      for (uint32_t i = L.lvs[l] + lane; i < L.lvs[l + 1]; i += 64) {
+        const uint32_t loc = L.order[i];
+        const uint32_t u = loc == 0 ? r : c0 + loc - 1;
+        const uint32_t ch = L.lch[loc];
+        W8Rec x;
+        if (ch == 0) {
+          x = w8_leaf_rec(L.lar[loc], lens ? lens[u] : 1u, c_prim);
+          if (x.p > p_max) atomicOr(&ctr[0], 8u);
+        } else {
+          const uint32_t cl = bb + ch - c0 + 1;
+          if (!child_ok(u, ch, cl)) continue;  // flagged by the level pass
+          x = w8_combine(&L.R[cl], &L.R[cl + 1], L.lar[loc], c_node, c_prim, p_max);
+        }
+        L.R[loc] = x;
+        E[u] = w8_e(x);
+      }
+      w8_wave_sync();
+    }
+    if (lane == 0) rsub[slot] = L.R[0];
+    w8_wave_sync();  // the wavefront's LDS is rewritten by its next subtree
+  }
+}
+
+// Phase 2: the top nodes' DP, level by level bottom up, one block per tree.
+__global__ __launch_bounds__(256) void k_w8_topdp(const mgbvh_node32* __restrict__ nodes,
+                                                  const W8Tree* __restrict__ trees,
+                                                  const W8Top* __restrict__ top,
+                                                  const uint32_t* __restrict__ lvstart,
+                                                  const W8Rec* __restrict__ rsub,
+                                                  W8Rec* __restrict__ rtop, W8E* __restrict__ E,
+                                                  float c_node, float c_prim, uint32_t p_max) {
+  const W8Tree T = trees[blockIdx.x];
+  if (T.count == 0 || T.ntop == 0) return;
+  const uint32_t* lv = lvstart + (size_t)blockIdx.x * (W8_MAXLV + 1);
+  const W8Top* E0 = top + T.toff;
+  W8Rec* R0 = rtop + T.toff;
+  for (int l = (int)T.nlev - 1; l >= 0; --l) {
+    for (uint32_t i = lv[l] + threadIdx.x; i < lv[l + 1]; i += blockDim.x) {
+      const W8Top e = E0[i];
+      const W8Rec* A = (e.ch[0] & W8_TAG) ? &rsub[e.ch[0] & ~W8_TAG] : &R0[e.ch[0]];
+      const W8Rec* B = (e.ch[1] & W8_TAG) ? &rsub[e.ch[1] & ~W8_TAG] : &R0[e.ch[1]];
+      const W8Rec x = w8_combine(A, B, w8_node_area(nodes, e.v), c_node, c_prim, p_max);
+      R0[i] = x;
+      E[e.v] = w8_e(x);
+    }
+    __syncthreads();  // the next level up reads these records (one block: workgroup scope)
+  }
+}
+
+// Each tree's root record, wherever its DP left it.
+__global__ void k_w8_rootrec(const W8Tree* __restrict__ trees, uint32_t nt,
+                             const W8Rec* __restrict__ rsub, const W8Rec* __restrict__ rtop,
+                             W8Rec* __restrict__ out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt || trees[t].count == 0) return;
+  const uint32_t s = trees[t].root_src;
+  out[t] = (s & W8_TAG) ? rsub[s & ~W8_TAG] : rtop[trees[t].toff];
+}
+
+// The same from the level path's node records.
+__global__ void k_w8_rootrec_nodes(const W8Tree* __restrict__ trees, uint32_t nt,
+                                   const W8Rec* __restrict__ rec, W8Rec* __restrict__ out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt || trees[t].count == 0) return;
+  out[t] = rec[trees[t].first];
 }
 
 // Largest q with p + q * s <= x (from the double floor), smallest with >= x.
@@ -261,22 +614,58 @@ __device__ inline uint32_t w8_exponent(float lo, float hi) {
 }
 
 __global__ __launch_bounds__(256) void k_w8_emit(
-    const mgbvh_node32* __restrict__ nodes, const uint32_t* __restrict__ base,
-    const uint32_t* __restrict__ lens,
-    const W8Rec* __restrict__ rec, const W8Item* __restrict__ in, const uint32_t* n_in,
+    const mgbvh_node32* __restrict__ nodes, const uint32_t* __restrict__ lens,
+    const W8E* __restrict__ E, const W8Item* __restrict__ in, const uint32_t* n_in,
     W8Item* __restrict__ out, uint32_t* n_out, uint32_t front_cap, uint8_t* __restrict__ wide,
     uint32_t* __restrict__ tri_index, uint32_t nn, uint32_t wide_cap, uint32_t tri_cap,
     uint32_t* err_flag) {
+  __shared__ uint32_t wtot[4], bbase;
   const uint32_t count = *n_in;
-  for (uint32_t it = blockIdx.x * blockDim.x + threadIdx.x; it < count;
-       it += gridDim.x * blockDim.x) {
-    const W8Item item = in[it];
-    const uint32_t v = item.bin;
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  // block-uniform trip count: every thread takes part in the frontier
+  // allocation below, one atomic per block and step (same-address atomics
+  // serialise device-wide; one per internal child cost most of the layout)
+  for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < count; b0 += gridDim.x * blockDim.x) {
+    const uint32_t it = b0 + threadIdx.x;
+    bool ok = it < count;
+    W8Item item{};
     uint32_t s[8];
-    if (v >= nn) { atomicOr(err_flag, 4u); continue; }
-    const int ns = w8_slots(nodes, base, nn, rec, v, s);
-    const uint64_t widx = (uint64_t)item.wbase + item.idx;
-    if (ns > 8 || widx >= wide_cap) { atomicOr(err_flag, 4u); continue; }
+    int ns = 0;
+    uint64_t widx = 0;
+    if (ok) {
+      item = in[it];
+      if (item.bin >= nn) { atomicOr(err_flag, 4u); ok = false; }
+    }
+    if (ok) {
+      ns = w8_slots(nodes, item.bbase, nn, E, item.bin, s);
+      widx = (uint64_t)item.wbase + item.idx;
+      if (ns > 8 || widx >= wide_cap) { atomicOr(err_flag, 4u); ok = false; }
+    }
+    uint32_t nint = 0, own = 0;
+    if (ok) {
+      for (int k = 0; k < ns; ++k) {
+        if (w8_leaf_slot(nodes, E, s[k])) own += E[s[k]].p;
+        else ++nint;
+      }
+    }
+    // this node's first frontier slot: block exclusive scan of nint
+    const uint32_t incl = wave_scan_u32_dpp(nint);
+    if (lane == 63) wtot[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t tot = 0;
+      for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) {
+        const uint32_t t = wtot[w];
+        wtot[w] = tot;
+        tot += t;
+      }
+      bbase = tot ? atomicAdd(n_out, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t slot = bbase + wtot[wave] + incl - nint;
+    __syncthreads();  // wtot and bbase are rewritten by the next step
+    if (!ok) continue;
+    const uint32_t v = item.bin;
     float pmn[3], pmx[3];
     uint32_t st, ch;
     w8_box(nodes, v, pmn, pmx, st, ch);
@@ -289,11 +678,6 @@ __global__ __launch_bounds__(256) void k_w8_emit(
       sc[a] = __uint_as_float(E << 23);
       ebytes |= E << (8 * a);
       words[a] = __float_as_uint(pmn[a]);
-    }
-    uint32_t nint = 0, own = 0;
-    for (int k = 0; k < ns; ++k) {
-      if (w8_leaf_slot(nodes, rec, s[k])) own += rec[s[k]].p;
-      else ++nint;
     }
     words[4] = item.region;
     words[5] = item.tri;
@@ -314,15 +698,15 @@ __global__ __launch_bounds__(256) void k_w8_emit(
         q[a][k] = (uint8_t)w8_qlo(pmn[a], sc[a], cmn[a]);
         q[3 + a][k] = (uint8_t)w8_qhi(pmn[a], sc[a], cmx[a]);
       }
-      if (w8_leaf_slot(nodes, rec, m)) {
-        const uint32_t np = rec[m].p;
+      if (w8_leaf_slot(nodes, E, m)) {
+        const uint32_t np = E[m].p;
         meta[k] = (uint8_t)((((1u << np) - 1u) << 5) | toff);
         uint32_t stk[8];
         int sp = 0;
         stk[sp++] = m;
         while (sp > 0) {  // the leaf's objects, child before child + 1
           const uint32_t u = stk[--sp];
-          const uint32_t uc = w8_child(nodes, base, u);
+          const uint32_t uc = w8_child(nodes, item.bbase, u);
           if (uc == 0) {
             const uint32_t len = lens ? lens[u] : 1u;
             const uint32_t us = nodes[u].start_index;
@@ -341,12 +725,13 @@ __global__ __launch_bounds__(256) void k_w8_emit(
       } else {
         imask |= 1u << k;
         meta[k] = (uint8_t)((1u << 5) | (24 + ci));
-        const uint32_t slot = atomicAdd(n_out, 1u);
         if (slot < front_cap)
-          out[slot] = W8Item{m, item.region + ci, sub, tsub, item.wbase, item.tbase};
+          out[slot] = W8Item{m, item.region + ci, sub, tsub, item.wbase, item.tbase,
+                             item.bbase, 0u};
         else atomicOr(err_flag, 4u);
-        sub += rec[m].w - 1;
-        tsub += rec[m].t;
+        ++slot;
+        sub += E[m].w - 1;
+        tsub += E[m].p;
         ++ci;
       }
     }
@@ -494,62 +879,64 @@ __global__ __launch_bounds__(256) void k_trace_w8(
   if (err) atomicOr(err_flag, err);
 }
 
-static float host_area(const mgbvh_node32& n) {
-  const float dx = n.max[0] - n.min[0], dy = n.max[1] - n.min[1], dz = n.max[2] - n.min[2];
-  float a = dx * dy;
-  a = a + dy * dz;
-  a = a + dz * dx;
-  return 2.0f * a;
+// Every tree's first node: base[v] for the nodes of tree blockIdx.x (level path).
+__global__ void k_w8_base(const W8Tree* __restrict__ trees, uint32_t nt, uint32_t* __restrict__ base) {
+  if (blockIdx.x >= nt) return;
+  const uint32_t f = trees[blockIdx.x].first, n = trees[blockIdx.x].count;
+  const uint32_t b = trees[blockIdx.x].bbase;
+  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) base[f + k] = b;
 }
 
-// Every BLAS's first node: base[v] for the nodes of BLAS blockIdx.x.
-__global__ void k_w8_base(const uint32_t* __restrict__ first, const uint32_t* __restrict__ count,
-                          uint32_t* __restrict__ base) {
-  const uint32_t m = blockIdx.x, f = first[m], n = count[m];
-  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) base[f + k] = f;
-}
-
-// Per BLAS root: wide nodes and objects of its tree.
-__global__ void k_w8_roots(const mgbvh_node32* __restrict__ nodes, const W8Rec* __restrict__ rec,
-                           const uint32_t* __restrict__ first, const uint32_t* __restrict__ count,
-                           uint32_t nb, uint32_t* __restrict__ out) {
+// Per tree: wide nodes and objects of its collapsed tree.
+__global__ void k_w8_roots(const mgbvh_node32* __restrict__ nodes, const W8Tree* __restrict__ trees,
+                           const W8Rec* __restrict__ rootrec, uint32_t nt, uint32_t* __restrict__ out) {
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= nb) return;
+  if (m >= nt) return;
   uint32_t w = 0, t = 0;
-  if (count[m]) {
-    const uint32_t r = first[m];
-    t = rec[r].t;
-    w = nodes[r].child_node == 0 ? 1u : rec[r].w;
+  if (trees[m].count) {
+    t = rootrec[m].p;
+    w = nodes[trees[m].first].child_node == 0 ? 1u : rootrec[m].w;
   }
   out[2 * m] = w;
   out[2 * m + 1] = t;
 }
 
 struct W8Forest {
-  W8Rec* rec = nullptr;
-  uint32_t* ctr = nullptr;  // [0] err, [1..2] frontier counts, [3] height
+  W8E* E = nullptr;          // layout entries, per binary node
+  W8Rec* rootrec = nullptr;  // per tree: its root's record
+  uint32_t* ctr = nullptr;   // [0] err, [1..2] frontier counts, [3] level count, [4] subtrees
 };
 
-// Level lists and the dynamic program over every node of the forest whose
-// roots are `roots` (binary node ids).
-static int w8_dp_forest(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint32_t* d_lens,
-                        uint32_t nn, const uint32_t* d_base, const std::vector<uint32_t>& roots,
+static inline size_t w8_align(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// The layout entries and root records of every tree: the subtree-local path
+// (w8_dp_fast) for trees built here, else the level path (w8_dp_levels).
+// d_trees: the device copy of `trees`.
+static int w8_dp_levels(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint32_t* d_lens,
+                        uint32_t nn, const std::vector<W8Tree>& trees, W8Tree* d_trees,
                         const mgbvh_wide8_params& P, W8Forest& f) {
   Scratch& sc = c->sc;
   int rc;
+  const uint32_t nt = (uint32_t)trees.size();
   if ((rc = ensure(sc.w8rec, sizeof(W8Rec) * (size_t)nn))) return rc;
   if ((rc = ensure(sc.w8par, 4 * (size_t)nn + 8))) return rc;  // node ids, level by level
-  if ((rc = ensure(sc.w8ctr, 16))) return rc;
-  f.rec = reinterpret_cast<W8Rec*>(sc.w8rec.p);
-  f.ctr = reinterpret_cast<uint32_t*>(sc.w8ctr.p);
+  if ((rc = ensure(sc.w8base, 4 * (size_t)nn))) return rc;     // each node's tree base
+  W8Rec* rec = reinterpret_cast<W8Rec*>(sc.w8rec.p);
   uint32_t* order = reinterpret_cast<uint32_t*>(sc.w8par.p);
+  uint32_t* base = reinterpret_cast<uint32_t*>(sc.w8base.p);
   hipStream_t st = c->stream;
   hipError_t e;
+  std::vector<uint32_t> roots;
+  for (const W8Tree& t : trees)
+    if (t.count) roots.push_back(t.first);
   if (roots.size() > nn) { set_error("more roots than nodes"); return MGBVH_ERR_INVALID_ARG; }
-  if ((e = hipMemsetAsync(f.ctr, 0, 16, st)) != hipSuccess ||
+  if ((e = hipMemsetAsync(f.ctr, 0, 32, st)) != hipSuccess ||
+      (e = hipMemsetAsync(base, 0, 4 * (size_t)nn, st)) != hipSuccess ||
       (e = hipMemcpyAsync(order, roots.data(), 4 * roots.size(), hipMemcpyHostToDevice, st)) !=
           hipSuccess)
     return hip_fail(e, "wide8 roots");
+  // a node outside every tree keeps base 0 and an absolute child index
+  k_w8_base<<<nt, 256, 0, st>>>(d_trees, nt, base);
   std::vector<uint32_t> start = {0}, count = {(uint32_t)roots.size()};
   for (;;) {
     const uint32_t s0 = start.back(), n0 = count.back();
@@ -557,7 +944,7 @@ static int w8_dp_forest(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint32_
     if (s1 > nn) { set_error("more nodes reached than the array holds"); return MGBVH_ERR_INDEX; }
     if (start.size() > 127) { set_error("binary tree deeper than 126"); return MGBVH_ERR_INVALID_ARG; }
     if ((e = hipMemsetAsync(f.ctr + 3, 0, 4, st)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
-    k_w8_next_level<<<(n0 + 1023) / 1024, 1024, 0, st>>>(d_nodes, d_base, nn, order + s0, n0,
+    k_w8_next_level<<<(n0 + 1023) / 1024, 1024, 0, st>>>(d_nodes, base, nn, order + s0, n0,
                                                        order + s1, nn - s1, f.ctr + 3, f.ctr);
     uint32_t hd[4];
     if ((e = hipGetLastError()) != hipSuccess ||
@@ -574,8 +961,9 @@ static int w8_dp_forest(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint32_
   }
   for (int level = (int)start.size() - 1; level >= 0; --level)
     k_w8_dp_level<<<(count[level] + 255) / 256, 256, 0, st>>>(
-        d_nodes, d_base, d_lens, nn, order + start[level], count[level], f.rec, P.c_node,
+        d_nodes, base, d_lens, nn, order + start[level], count[level], rec, f.E, P.c_node,
         P.c_prim, P.p_max, f.ctr);
+  k_w8_rootrec_nodes<<<(nt + 255) / 256, 256, 0, st>>>(d_trees, nt, rec, f.rootrec);
   uint32_t err = 0;
   if ((e = hipGetLastError()) != hipSuccess ||
       (e = hipMemcpyAsync(&err, f.ctr, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
@@ -589,11 +977,94 @@ static int w8_dp_forest(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint32_
   return MGBVH_OK;
 }
 
+// Returns MGBVH_OK, an error, or 1: the trees do not fit the subtree-local
+// path (the level path then computes the same records).
+static int w8_dp_fast(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint32_t* d_lens,
+                      uint32_t nn, std::vector<W8Tree>& trees, W8Tree* d_trees,
+                      const mgbvh_wide8_params& P, W8Forest& f) {
+  Scratch& sc = c->sc;
+  const uint32_t nt = (uint32_t)trees.size();
+  // top nodes of a tree of n <= (count + 1) / 2 objects: each has more than
+  // W8_SUB objects, so at most 2 n / W8_SUB of them; subtree roots at most
+  // 2 per top node + 1
+  uint64_t ttot = 0;
+  for (W8Tree& t : trees) {
+    t.toff = (uint32_t)ttot;
+    t.tcap = t.count ? t.count / W8_SUB + 2u : 0u;
+    ttot += t.tcap;
+  }
+  const uint64_t sub_cap = 2 * ttot + nt;
+  if (ttot + sub_cap > 0x7fffffffull) return 1;
+  const size_t o_top = 0, o_rtop = o_top + w8_align(sizeof(W8Top) * ttot);
+  const size_t o_sub = o_rtop + w8_align(sizeof(W8Rec) * ttot);
+  const size_t o_rsub = o_sub + w8_align(sizeof(W8Sub) * sub_cap);
+  const size_t o_lv = o_rsub + w8_align(sizeof(W8Rec) * sub_cap);
+  const size_t bytes = o_lv + w8_align(4 * (size_t)nt * (W8_MAXLV + 1));
+  int rc = ensure(sc.w8fast, bytes);
+  if (rc) return rc;
+  char* fb = reinterpret_cast<char*>(sc.w8fast.p);
+  W8Top* top = reinterpret_cast<W8Top*>(fb + o_top);
+  W8Rec* rtop = reinterpret_cast<W8Rec*>(fb + o_rtop);
+  W8Sub* sub = reinterpret_cast<W8Sub*>(fb + o_sub);
+  W8Rec* rsub = reinterpret_cast<W8Rec*>(fb + o_rsub);
+  uint32_t* lv = reinterpret_cast<uint32_t*>(fb + o_lv);
+  hipStream_t st = c->stream;
+  hipError_t e;
+  if ((e = hipMemcpyAsync(d_trees, trees.data(), sizeof(W8Tree) * nt, hipMemcpyHostToDevice,
+                          st)) != hipSuccess ||
+      (e = hipMemsetAsync(f.ctr, 0, 32, st)) != hipSuccess)
+    return hip_fail(e, "wide8 trees");
+  k_w8_top<<<nt, 256, 0, st>>>(d_nodes, d_lens, nn, d_trees, top, lv, sub, (uint32_t)sub_cap,
+                               f.ctr);
+  uint32_t hc[8];
+  if ((e = hipGetLastError()) != hipSuccess ||
+      (e = hipMemcpyAsync(hc, f.ctr, 32, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipStreamSynchronize(st)) != hipSuccess)
+    return hip_fail(e, "k_w8_top");
+  if ((hc[0] & W8_FAST_FAIL) || hc[4] > sub_cap) return 1;
+  const uint32_t nsub = hc[4];
+  const uint32_t g1 =
+      std::max<uint32_t>(1, std::min<uint32_t>((nsub + 3) / 4, (uint32_t)c->cus * 8));
+  k_w8_sub<<<g1, 256, 0, st>>>(d_nodes, d_lens, nn, sub, f.ctr, rsub, f.E, P.c_node, P.c_prim,
+                               P.p_max, f.ctr);
+  k_w8_topdp<<<nt, 256, 0, st>>>(d_nodes, d_trees, top, lv, rsub, rtop, f.E, P.c_node,
+                                 P.c_prim, P.p_max);
+  k_w8_rootrec<<<(nt + 255) / 256, 256, 0, st>>>(d_trees, nt, rsub, rtop, f.rootrec);
+  if ((e = hipGetLastError()) != hipSuccess ||
+      (e = hipMemcpyAsync(hc, f.ctr, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipStreamSynchronize(st)) != hipSuccess)
+    return hip_fail(e, "k_w8_sub");
+  if (hc[0] & W8_FAST_FAIL) return 1;
+  if (hc[0] & 8u) {
+    set_error("a binary leaf holds more than p_max objects");
+    return MGBVH_ERR_INVALID_ARG;
+  }
+  return MGBVH_OK;
+}
+
+static int w8_dp(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint32_t* d_lens, uint32_t nn,
+                 std::vector<W8Tree>& trees, const mgbvh_wide8_params& P, W8Forest& f) {
+  Scratch& sc = c->sc;
+  const uint32_t nt = (uint32_t)trees.size();
+  int rc;
+  if ((rc = ensure(sc.w8e, sizeof(W8E) * (size_t)nn))) return rc;
+  if ((rc = ensure(sc.w8ctr, 32 + w8_align(sizeof(W8Tree) * nt) + sizeof(W8Rec) * (size_t)nt)))
+    return rc;
+  f.E = reinterpret_cast<W8E*>(sc.w8e.p);
+  f.ctr = reinterpret_cast<uint32_t*>(sc.w8ctr.p);
+  W8Tree* d_trees = reinterpret_cast<W8Tree*>(reinterpret_cast<char*>(sc.w8ctr.p) + 32);
+  f.rootrec = reinterpret_cast<W8Rec*>(reinterpret_cast<char*>(d_trees) +
+                                       w8_align(sizeof(W8Tree) * nt));
+  rc = w8_dp_fast(c, d_nodes, d_lens, nn, trees, d_trees, P, f);
+  if (rc != 1) return rc;
+  return w8_dp_levels(c, d_nodes, d_lens, nn, trees, d_trees, P, f);
+}
+
 // Writes the wide trees whose roots are `roots`, one wide level per launch.
 static int w8_emit_forest(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint32_t* d_lens,
-                          uint32_t nn, const uint32_t* d_base, const W8Forest& f,
-                          const std::vector<W8Item>& roots, mgbvh_wide_node* d_wide,
-                          uint64_t wide_cap, uint32_t* d_tri_index, uint64_t tri_cap) {
+                          uint32_t nn, const W8Forest& f, const std::vector<W8Item>& roots,
+                          mgbvh_wide_node* d_wide, uint64_t wide_cap, uint32_t* d_tri_index,
+                          uint64_t tri_cap) {
   if (roots.empty()) return MGBVH_OK;
   Scratch& sc = c->sc;
   const uint32_t front_cap = nn / 2 + (uint32_t)roots.size() + 1;
@@ -605,7 +1076,8 @@ static int w8_emit_forest(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint3
   hipStream_t st = c->stream;
   uint32_t n_in = (uint32_t)roots.size();
   hipError_t e;
-  if ((e = hipMemcpyAsync(fr[0], roots.data(), sizeof(W8Item) * roots.size(),
+  if ((e = hipMemsetAsync(f.ctr, 0, 4, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(fr[0], roots.data(), sizeof(W8Item) * roots.size(),
                           hipMemcpyHostToDevice, st)) != hipSuccess ||
       (e = hipMemcpyAsync(f.ctr + 1, &n_in, 4, hipMemcpyHostToDevice, st)) != hipSuccess ||
       (e = hipStreamSynchronize(st)) != hipSuccess)
@@ -616,7 +1088,7 @@ static int w8_emit_forest(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint3
     uint32_t* cout = f.ctr + 1 + ((level + 1) & 1);
     if ((e = hipMemsetAsync(cout, 0, 4, st)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
     k_w8_emit<<<std::min(egrid, (n_in + 255) / 256), 256, 0, st>>>(
-        d_nodes, d_base, d_lens, f.rec, fr[level & 1], cin, fr[(level + 1) & 1], cout, front_cap,
+        d_nodes, d_lens, f.E, fr[level & 1], cin, fr[(level + 1) & 1], cout, front_cap,
         reinterpret_cast<uint8_t*>(d_wide), d_tri_index, nn,
         (uint32_t)std::min<uint64_t>(wide_cap, 0xffffffffu),
         (uint32_t)std::min<uint64_t>(tri_cap, 0xffffffffu), f.ctr);
@@ -668,38 +1140,31 @@ int mgbvh_wide8_collapse_device(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
   const uint32_t nn = (uint32_t)nnodes;
   W8Forest f;
-  if ((rc = w8_dp_forest(c, d_nodes, d_lens, nn, nullptr, {0u}, P, f))) return rc;
+  std::vector<W8Tree> trees(1);
+  trees[0] = W8Tree{0u, nn, 0u, 0u, 0u, 0u, 0u, 0u};
+  if ((rc = w8_dp(c, d_nodes, d_lens, nn, trees, P, f))) return rc;
   mgbvh_node32 root;
-  W8Rec r0{}, rl{}, rr{};
+  W8Rec r0{};
   if ((e = hipMemcpy(&root, d_nodes, sizeof root, hipMemcpyDeviceToHost)) != hipSuccess ||
-      (e = hipMemcpy(&r0, f.rec, sizeof r0, hipMemcpyDeviceToHost)) != hipSuccess)
+      (e = hipMemcpy(&r0, f.rootrec, sizeof r0, hipMemcpyDeviceToHost)) != hipSuccess)
     return hip_fail(e, "wide8 root readback");
   const uint32_t L = root.child_node;
-  if (L != 0) {
-    if (L + 1 >= nn) { set_error("root child index out of range"); return MGBVH_ERR_INDEX; }
-    if ((e = hipMemcpy(&rl, f.rec + L, sizeof rl, hipMemcpyDeviceToHost)) != hipSuccess ||
-        (e = hipMemcpy(&rr, f.rec + L + 1, sizeof rr, hipMemcpyDeviceToHost)) != hipSuccess)
-      return hip_fail(e, "wide8 readback");
-  }
+  if (L != 0 && L + 1 >= nn) { set_error("root child index out of range"); return MGBVH_ERR_INDEX; }
   const uint64_t nwide = L == 0 ? 1 : r0.w;
-  const uint64_t nobj = r0.t;
+  const uint64_t nobj = r0.p;
   if (nwide > wide_cap || nobj > tri_cap) {
     set_error("wide8 output capacity too small (nodes " + std::to_string(nwide) + ", objects " +
               std::to_string(nobj) + ")");
     return MGBVH_ERR_INVALID_ARG;
   }
-  const std::vector<W8Item> roots = {W8Item{0u, 0u, 1u, 0u, 0u, 0u}};
-  if ((rc = w8_emit_forest(c, d_nodes, d_lens, nn, nullptr, f, roots, d_wide, wide_cap,
-                           d_tri_index, tri_cap)))
+  const std::vector<W8Item> roots = {W8Item{0u, 0u, 1u, 0u, 0u, 0u, 0u, 0u}};
+  if ((rc = w8_emit_forest(c, d_nodes, d_lens, nn, f, roots, d_wide, wide_cap, d_tri_index,
+                           tri_cap)))
     return rc;
   if (out_nwide) *out_nwide = nwide;
-  if (out_cost) {
-    if (L == 0) *out_cost = r0.c[0];
-    else {
-      const uint32_t k = r0.dec[7];
-      *out_cost = host_area(root) * P.c_node + (rl.c[k - 1] + rr.c[8 - k - 1]);
-    }
-  }
+  // a leaf root's cost is its leaf cost; an internal root is a wide internal
+  // node: area * c_node + the best split of its 8 slots
+  if (out_cost) *out_cost = L == 0 ? r0.c[0] : r0.c8;
   return MGBVH_OK;
 }
 
@@ -741,32 +1206,18 @@ int mgbvh_wide8_collapse_batch_device(mgbvh_ctx* c, const mgbvh_node32* d_nodes,
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
   const uint32_t nn = (uint32_t)nnodes;
   Scratch& sc = c->sc;
-  // node bases, then 3 x nblas words of BLAS metadata and root sizes
-  if ((rc = ensure(sc.w8base, 4 * (size_t)nn + 16 * (size_t)nblas))) return rc;
-  uint32_t* base = reinterpret_cast<uint32_t*>(sc.w8base.p);
-  uint32_t* d_first = base + nn;
-  uint32_t* d_count = d_first + nblas;
-  uint32_t* d_sizes = d_count + nblas;
-  std::vector<uint32_t> meta(2 * (size_t)nblas);
-  for (uint32_t m = 0; m < nblas; ++m) {
-    meta[m] = blas[m].node_offset;
-    meta[nblas + m] = node_counts[m];
-  }
-  hipStream_t st = c->stream;
-  if ((e = hipMemcpyAsync(d_first, meta.data(), 8 * (size_t)nblas, hipMemcpyHostToDevice, st)) !=
-      hipSuccess)
-    return hip_fail(e, "wide8 BLAS table");
-  // a node outside every BLAS keeps base 0 and an absolute child index
-  if ((e = hipMemsetAsync(base, 0, 4 * (size_t)nn, st)) != hipSuccess)
-    return hip_fail(e, "hipMemsetAsync");
-  k_w8_base<<<nblas, 256, 0, st>>>(d_first, d_count, base);
-  std::vector<uint32_t> broots;
+  std::vector<W8Tree> trees(nblas);
   for (uint32_t m = 0; m < nblas; ++m)
-    if (node_counts[m]) broots.push_back(blas[m].node_offset);
+    trees[m] = W8Tree{blas[m].node_offset, node_counts[m], blas[m].node_offset, 0u, 0u, 0u, 0u, 0u};
   W8Forest f;
-  if ((rc = w8_dp_forest(c, d_nodes, d_lens, nn, base, broots, P, f))) return rc;
-  k_w8_roots<<<(nblas + 255) / 256, 256, 0, st>>>(d_nodes, f.rec, d_first, d_count, nblas,
-                                                   d_sizes);
+  if ((rc = w8_dp(c, d_nodes, d_lens, nn, trees, P, f))) return rc;
+  // root sizes of every tree (after the tree table in w8ctr)
+  if ((rc = ensure(sc.w8base, 8 * (size_t)nblas))) return rc;
+  uint32_t* d_sizes = reinterpret_cast<uint32_t*>(sc.w8base.p);
+  hipStream_t st = c->stream;
+  const W8Tree* d_trees =
+      reinterpret_cast<const W8Tree*>(reinterpret_cast<const char*>(sc.w8ctr.p) + 32);
+  k_w8_roots<<<(nblas + 255) / 256, 256, 0, st>>>(d_nodes, d_trees, f.rootrec, nblas, d_sizes);
   std::vector<uint32_t> sizes(2 * (size_t)nblas);
   if ((e = hipGetLastError()) != hipSuccess ||
       (e = hipMemcpyAsync(sizes.data(), d_sizes, 8 * (size_t)nblas, hipMemcpyDeviceToHost, st)) !=
@@ -785,14 +1236,15 @@ int mgbvh_wide8_collapse_batch_device(mgbvh_ctx* c, const mgbvh_node32* d_nodes,
       set_error("wide8 tri_index capacity too small for BLAS " + std::to_string(m));
       return MGBVH_ERR_INVALID_ARG;
     }
-    roots.push_back(W8Item{blas[m].node_offset, 0u, 1u, 0u, (uint32_t)wsum, blas[m].tri_offset});
+    roots.push_back(W8Item{blas[m].node_offset, 0u, 1u, 0u, (uint32_t)wsum, blas[m].tri_offset,
+                           blas[m].node_offset, 0u});
     wsum += w;
   }
   if (wsum > wide_cap || wsum > 0xffffffffull) {
     set_error("wide8 output capacity too small (nodes " + std::to_string(wsum) + ")");
     return MGBVH_ERR_INVALID_ARG;
   }
-  if ((rc = w8_emit_forest(c, d_nodes, d_lens, nn, base, f, roots, d_wide, wide_cap, d_tri_index,
+  if ((rc = w8_emit_forest(c, d_nodes, d_lens, nn, f, roots, d_wide, wide_cap, d_tri_index,
                            tri_cap)))
     return rc;
   if (out_nwide) *out_nwide = wsum;

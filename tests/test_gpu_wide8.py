@@ -101,6 +101,50 @@ def test_collapse_depth_cut_and_errors(ctx):
         b.free()
 
 
+def _bfs_renumber(nodes, lens):
+    """The same tree numbered breadth first (child pairs stay adjacent and in
+    order, leaves keep their start indices): children still follow their
+    parents, but a subtree's nodes no longer share one index range."""
+    old_child = nodes["child_node"].astype(np.int64)
+    order, q = [0], 0
+    while q < len(order):
+        v = order[q]
+        q += 1
+        c = old_child[v]
+        if c:
+            order += [c, c + 1]
+    order = np.array(order, np.int64)
+    new_of = np.empty(len(order), np.int64)
+    new_of[order] = np.arange(len(order))
+    out = nodes[order].copy()
+    ch = old_child[order]
+    out["child_node"] = np.where(ch != 0, new_of[np.maximum(ch, 0)], 0).astype(np.uint32)
+    return out, lens[order].copy()
+
+
+def test_collapse_of_other_numberings(ctx):
+    # a tree whose subtrees are not contiguous index ranges takes the level
+    # path of the collapse; its wide nodes are those of the same topology
+    print("[case] wide8 collapse of a breadth-first numbered tree (level path)", flush=True)
+    for pos, tris in (scenes.soup(3000, seed=51), scenes.heightfield(50, 40, seed=52)):
+        t = tris.copy()
+        nodes, lens, _, _, _ = ctx.build_tris(pos, t, N.default_settings())
+        bn, bl = _bfs_renumber(nodes, lens)
+        assert not np.array_equal(bn["child_node"], nodes["child_node"])
+        n = tris.shape[0]
+        d_nodes = ctx.to_device(bn)
+        d_wide, d_tri = ctx.alloc(N.WIDE_NODE_BYTES * n), ctx.alloc(4 * n)
+        nw, cost = ctx.wide8_collapse(d_nodes, None, bn.shape[0], d_wide, n, d_tri, n)
+        wide = d_wide.download(np.uint8, N.WIDE_NODE_BYTES * nw).reshape(nw, N.WIDE_NODE_BYTES)
+        ref = O.wide8_collapse(bn.view(np.uint32).reshape(-1, 8), None)
+        _assert_same("breadth first", (wide, d_tri.download(np.uint32, n), cost), ref)
+        # and the same wide nodes as the builder's numbering gives
+        rw0, rt0, _ = O.wide8_collapse(nodes.view(np.uint32).reshape(-1, 8), None)
+        assert np.array_equal(wide, rw0)
+        for b in (d_nodes, d_wide, d_tri):
+            b.free()
+
+
 def test_heightfield_1m_collapse_and_traversal(ctx):
     print("[case] wide8 on the 1024x512 heightfield: collapse vs oracle, 262k camera rays",
           flush=True)
