@@ -613,6 +613,17 @@ __device__ inline uint32_t w8_exponent(float lo, float hi) {
   return (uint32_t)(e + 127);
 }
 
+// One wide node per group of 8 lanes.  The lanes expand the node's slots
+// together: an expansion of budget b is held by the lane at its first
+// position and owns positions [pos, pos + b); a split of budget b into k and
+// b - k keeps the left part and hands the right part to lane pos + k, so the
+// final slots sit in left-first order across the group after as many rounds
+// as the expansion is deep.  Each lane then quantises its slot's box and lays
+// out its leaf's objects or its internal child's frontier item; prefix sums
+// over the group give each slot what the sequential layout gave it (objects
+// of the leaf slots before it, internal slots before it, and the wide nodes
+// and objects of the internal children before it).  The node's words are
+// assembled in LDS and stored as five 16-byte writes.
 __global__ __launch_bounds__(256) void k_w8_emit(
     const mgbvh_node32* __restrict__ nodes, const uint32_t* __restrict__ lens,
     const W8E* __restrict__ E, const W8Item* __restrict__ in, const uint32_t* n_in,
@@ -620,133 +631,203 @@ __global__ __launch_bounds__(256) void k_w8_emit(
     uint32_t* __restrict__ tri_index, uint32_t nn, uint32_t wide_cap, uint32_t tri_cap,
     uint32_t* err_flag) {
   __shared__ uint32_t wtot[4], bbase;
+  __shared__ __align__(16) uint32_t nw[32][20];  // the node words of each group
+  __shared__ uint32_t box_u[256], box_b[256];    // hand-over of right parts, per lane
   const uint32_t count = *n_in;
-  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, t = threadIdx.x;
+  const uint32_t k = lane & 7u, g = t >> 3, g0 = lane & ~7u;
+  uint32_t* W = nw[g];
+  uint8_t* Wb = reinterpret_cast<uint8_t*>(W);
+  box_b[t] = 0;
+  w8_wave_sync();
   // block-uniform trip count: every thread takes part in the frontier
   // allocation below, one atomic per block and step (same-address atomics
-  // serialise device-wide; one per internal child cost most of the layout)
-  for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < count; b0 += gridDim.x * blockDim.x) {
-    const uint32_t it = b0 + threadIdx.x;
-    bool ok = it < count;
+  // serialise device-wide)
+  for (uint32_t b0 = blockIdx.x * 32; b0 < count; b0 += gridDim.x * 32) {
+    const uint32_t it = b0 + g;
+    bool ok = it < count;  // group-uniform
     W8Item item{};
-    uint32_t s[8];
-    int ns = 0;
-    uint64_t widx = 0;
+    uint32_t u = 0, b = 0;  // this lane's expansion (b == 0: none)
+    bool fin = false, bad = false;
     if (ok) {
       item = in[it];
-      if (item.bin >= nn) { atomicOr(err_flag, 4u); ok = false; }
-    }
-    if (ok) {
-      ns = w8_slots(nodes, item.bbase, nn, E, item.bin, s);
-      widx = (uint64_t)item.wbase + item.idx;
-      if (ns > 8 || widx >= wide_cap) { atomicOr(err_flag, 4u); ok = false; }
-    }
-    uint32_t nint = 0, own = 0;
-    if (ok) {
-      for (int k = 0; k < ns; ++k) {
-        if (w8_leaf_slot(nodes, E, s[k])) own += E[s[k]].p;
-        else ++nint;
+      if (item.bin >= nn) {
+        bad = true;
+      } else {
+        const uint32_t v = item.bin, ch = w8_child(nodes, item.bbase, v);
+        if (ch == 0) {
+          if (k == 0) { u = v; b = 1; fin = true; }
+        } else if (ch + 1 >= nn) {
+          bad = true;
+        } else {
+          const uint32_t kk = E[v].dec[7];
+          if (kk < 1 || kk > 7) bad = true;
+          else if (k == 0) { u = ch; b = kk; }
+          else if (k == kk) { u = ch + 1; b = 8 - kk; }
+        }
       }
     }
-    // this node's first frontier slot: block exclusive scan of nint
-    const uint32_t incl = wave_scan_u32_dpp(nint);
+    for (;;) {  // one round per level of the expansion (wave-uniform exit)
+      const bool act = b != 0 && !fin && !bad;
+      if (__ballot(act) == 0) break;
+      bool send = false;
+      uint32_t su = 0, sb = 0, kk = 0;
+      if (act) {
+        const uint32_t c = w8_child(nodes, item.bbase, u);
+        if (c == 0) {
+          fin = true;
+        } else if (c + 1 >= nn) {
+          bad = true;
+        } else {
+          const uint2 dw = *reinterpret_cast<const uint2*>(E[u].dec);
+          const uint64_t d = ((uint64_t)dw.y << 32) | dw.x;
+          while (b > 1 && ((d >> (8 * (b - 1))) & 0xffu) == 0) --b;
+          if (b == 1) {
+            fin = true;
+          } else {
+            kk = (uint32_t)(d >> (8 * (b - 1))) & 0xffu;
+            if (kk < 1 || kk >= b) {
+              bad = true;
+            } else {
+              send = true;
+              su = c + 1;
+              sb = b - kk;
+              u = c;
+              b = kk;
+            }
+          }
+        }
+      }
+      // k + b <= 8 holds for every expansion, so lane k + kk is in the group
+      // and holds none
+      if (send) { box_u[t + kk] = su; box_b[t + kk] = sb; }
+      w8_wave_sync();
+      const uint32_t rb = box_b[t];
+      if (rb != 0) { u = box_u[t]; b = rb; box_b[t] = 0; }
+      w8_wave_sync();
+    }
+    if ((uint32_t)(__ballot(bad) >> g0) & 0xffu) {
+      if (k == 0) atomicOr(err_flag, 4u);
+      ok = false;
+    }
+    uint64_t widx = 0;
+    if (ok) {
+      widx = (uint64_t)item.wbase + item.idx;
+      if (widx >= wide_cap) { if (k == 0) atomicOr(err_flag, 4u); ok = false; }
+    }
+    const bool has = ok && fin;
+    const uint32_t fmask = (uint32_t)(__ballot(has) >> g0) & 0xffu;
+    const uint32_t sidx = __popc(fmask & ((1u << k) - 1u));  // this lane's slot
+    const bool leaf = has && w8_leaf_slot(nodes, E, u);
+    const bool inner = has && !leaf;
+    const W8E em = has ? E[u] : W8E{};
+    const uint32_t p = leaf ? em.p : 0u;        // objects of a leaf slot
+    const uint32_t wi = inner ? em.w - 1 : 0u;  // wide nodes below an internal child
+    const uint32_t ti = inner ? em.p : 0u;      // its objects
+    const uint32_t ii = inner ? 1u : 0u;
+    // inclusive scans over the group's 8 lanes (lane order is slot order)
+    uint32_t sp = p, sw = wi, st_ = ti, si = ii;
+#pragma unroll
+    for (uint32_t d = 1; d < 8; d <<= 1) {
+      const uint32_t a = __shfl_up(sp, d, 8), bb = __shfl_up(sw, d, 8);
+      const uint32_t c = __shfl_up(st_, d, 8), e = __shfl_up(si, d, 8);
+      if (k >= d) { sp += a; sw += bb; st_ += c; si += e; }
+    }
+    const uint32_t own = __shfl(sp, (int)(g0 + 7), 64);   // objects of all leaf slots
+    const uint32_t nint = __shfl(si, (int)(g0 + 7), 64);  // internal slots
+    const uint32_t toff = sp - p, ci = si - ii;
+    // the group's first frontier slot: block exclusive scan of nint (group leaders)
+    const uint32_t lead = k == 0 ? nint : 0u;
+    const uint32_t incl = wave_scan_u32_dpp(lead);
     if (lane == 63) wtot[wave] = incl;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (t == 0) {
       uint32_t tot = 0;
       for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) {
-        const uint32_t t = wtot[w];
+        const uint32_t x = wtot[w];
         wtot[w] = tot;
-        tot += t;
+        tot += x;
       }
       bbase = tot ? atomicAdd(n_out, tot) : 0u;
     }
     __syncthreads();
-    uint32_t slot = bbase + wtot[wave] + incl - nint;
+    const uint32_t gslot = bbase + wtot[wave] + __shfl(incl - lead, (int)g0, 64);
     __syncthreads();  // wtot and bbase are rewritten by the next step
     if (!ok) continue;
-    const uint32_t v = item.bin;
+    // the node's frame (every lane of the group: the same values)
     float pmn[3], pmx[3];
-    uint32_t st, ch;
-    w8_box(nodes, v, pmn, pmx, st, ch);
-    uint32_t words[20];
+    uint32_t pst, pch;
+    w8_box(nodes, item.bin, pmn, pmx, pst, pch);
     float sc[3];
     uint32_t ebytes = 0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const uint32_t E = w8_exponent(pmn[a], pmx[a]);
-      sc[a] = __uint_as_float(E << 23);
-      ebytes |= E << (8 * a);
-      words[a] = __float_as_uint(pmn[a]);
+      const uint32_t Ex = w8_exponent(pmn[a], pmx[a]);
+      sc[a] = __uint_as_float(Ex << 23);
+      ebytes |= Ex << (8 * a);
     }
-    words[4] = item.region;
-    words[5] = item.tri;
-    uint8_t meta[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    uint8_t q[6][8];
-#pragma unroll
-    for (int a = 0; a < 6; ++a)
-#pragma unroll
-      for (int k = 0; k < 8; ++k) q[a][k] = 0;
-    uint32_t ci = 0, toff = 0, sub = item.region + nint, tsub = item.tri + own, imask = 0;
-    for (int k = 0; k < ns; ++k) {
-      const uint32_t m = s[k];
+    // words 6-19 (meta and planes) start at zero: empty slots stay zero
+    W[6 + k] = 0u;
+    if (k < 6) W[14 + k] = 0u;
+    w8_wave_sync();
+    uint32_t ibit = 0;
+    if (has) {
       float cmn[3], cmx[3];
       uint32_t cst, cch;
-      w8_box(nodes, m, cmn, cmx, cst, cch);
+      w8_box(nodes, u, cmn, cmx, cst, cch);
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        q[a][k] = (uint8_t)w8_qlo(pmn[a], sc[a], cmn[a]);
-        q[3 + a][k] = (uint8_t)w8_qhi(pmn[a], sc[a], cmx[a]);
+        Wb[32 + 8 * a + sidx] = (uint8_t)w8_qlo(pmn[a], sc[a], cmn[a]);
+        Wb[56 + 8 * a + sidx] = (uint8_t)w8_qhi(pmn[a], sc[a], cmx[a]);
       }
-      if (w8_leaf_slot(nodes, E, m)) {
-        const uint32_t np = E[m].p;
-        meta[k] = (uint8_t)((((1u << np) - 1u) << 5) | toff);
-        uint32_t stk[8];
-        int sp = 0;
-        stk[sp++] = m;
-        while (sp > 0) {  // the leaf's objects, child before child + 1
-          const uint32_t u = stk[--sp];
-          const uint32_t uc = w8_child(nodes, item.bbase, u);
-          if (uc == 0) {
-            const uint32_t len = lens ? lens[u] : 1u;
-            const uint32_t us = nodes[u].start_index;
-            for (uint32_t j = 0; j < len; ++j) {
-              const uint64_t at = (uint64_t)item.tbase + item.tri + toff++;
-              if (at < tri_cap) tri_index[at] = us + j;
-              else atomicOr(err_flag, 4u);
-            }
-          } else if (sp + 2 <= 8 && uc + 1 < nn) {
-            stk[sp++] = uc + 1;
-            stk[sp++] = uc;
-          } else {
-            atomicOr(err_flag, 4u);
+    }
+    if (leaf) {
+      Wb[24 + sidx] = (uint8_t)((((1u << p) - 1u) << 5) | toff);
+      // the leaf's objects, child before child + 1: p <= 3 objects lie in at
+      // most 3 binary leaves, at most two levels below the slot
+      uint32_t to = toff;
+      const uint32_t c = w8_child(nodes, item.bbase, u);
+      for (uint32_t i = 0; i < (c == 0 ? 1u : 2u); ++i) {
+        const uint32_t x = c == 0 ? u : c + i;
+        const uint32_t cx = c == 0 ? 0u : w8_child(nodes, item.bbase, x);
+        if (cx != 0 && cx + 1 >= nn) { atomicOr(err_flag, 4u); continue; }
+        for (uint32_t j = 0; j < (cx == 0 ? 1u : 2u); ++j) {
+          const uint32_t y = cx == 0 ? x : cx + j;
+          if (cx != 0 && w8_child(nodes, item.bbase, y) != 0) { atomicOr(err_flag, 4u); continue; }
+          const uint32_t len = lens ? lens[y] : 1u;
+          const uint32_t ys = nodes[y].start_index;
+          for (uint32_t q = 0; q < len; ++q) {
+            const uint64_t at = (uint64_t)item.tbase + item.tri + to++;
+            if (at < tri_cap) tri_index[at] = ys + q;
+            else atomicOr(err_flag, 4u);
           }
         }
-      } else {
-        imask |= 1u << k;
-        meta[k] = (uint8_t)((1u << 5) | (24 + ci));
-        if (slot < front_cap)
-          out[slot] = W8Item{m, item.region + ci, sub, tsub, item.wbase, item.tbase,
-                             item.bbase, 0u};
-        else atomicOr(err_flag, 4u);
-        ++slot;
-        sub += E[m].w - 1;
-        tsub += E[m].p;
-        ++ci;
       }
+    } else if (inner) {
+      Wb[24 + sidx] = (uint8_t)((1u << 5) | (24 + ci));
+      ibit = 1u << sidx;
+      const uint32_t slot = gslot + ci;
+      if (slot < front_cap)
+        out[slot] = W8Item{u, item.region + ci, item.region + nint + (sw - wi),
+                           item.tri + own + (st_ - ti), item.wbase, item.tbase, item.bbase, 0u};
+      else atomicOr(err_flag, 4u);
     }
-    words[3] = ebytes | (imask << 24);
-    words[6] = meta[0] | (meta[1] << 8) | (meta[2] << 16) | ((uint32_t)meta[3] << 24);
-    words[7] = meta[4] | (meta[5] << 8) | (meta[6] << 16) | ((uint32_t)meta[7] << 24);
 #pragma unroll
-    for (int a = 0; a < 6; ++a) {
-      words[8 + 2 * a] = q[a][0] | (q[a][1] << 8) | (q[a][2] << 16) | ((uint32_t)q[a][3] << 24);
-      words[9 + 2 * a] = q[a][4] | (q[a][5] << 8) | (q[a][6] << 16) | ((uint32_t)q[a][7] << 24);
+    for (uint32_t d = 1; d < 8; d <<= 1) ibit |= __shfl_xor(ibit, (int)d, 8);
+    if (k == 0) {
+      W[0] = __float_as_uint(pmn[0]);
+      W[1] = __float_as_uint(pmn[1]);
+      W[2] = __float_as_uint(pmn[2]);
+      W[3] = ebytes | (ibit << 24);
+      W[4] = item.region;
+      W[5] = item.tri;
     }
-    uint4* dst = reinterpret_cast<uint4*>(wide + 80 * (size_t)widx);
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-      dst[k] = make_uint4(words[4 * k], words[4 * k + 1], words[4 * k + 2], words[4 * k + 3]);
+    w8_wave_sync();
+    if (k < 5) {
+      const uint4 x = reinterpret_cast<const uint4*>(W)[k];
+      reinterpret_cast<uint4*>(wide + 80 * (size_t)widx)[k] = x;
+    }
+    w8_wave_sync();  // W is rewritten by the group's next node
   }
 }
 
@@ -1082,12 +1163,12 @@ static int w8_emit_forest(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint3
       (e = hipMemcpyAsync(f.ctr + 1, &n_in, 4, hipMemcpyHostToDevice, st)) != hipSuccess ||
       (e = hipStreamSynchronize(st)) != hipSuccess)
     return hip_fail(e, "wide8 frontier");
-  const uint32_t egrid = std::max<uint32_t>(1, std::min<uint32_t>((nn / 2 + 255) / 256, 4096));
+  const uint32_t egrid = std::max<uint32_t>(1, std::min<uint32_t>((nn / 2 + 31) / 32, 8192));
   for (int level = 0; n_in > 0 && level < 256; ++level) {
     uint32_t* cin = f.ctr + 1 + (level & 1);
     uint32_t* cout = f.ctr + 1 + ((level + 1) & 1);
     if ((e = hipMemsetAsync(cout, 0, 4, st)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
-    k_w8_emit<<<std::min(egrid, (n_in + 255) / 256), 256, 0, st>>>(
+    k_w8_emit<<<std::min(egrid, (n_in + 31) / 32), 256, 0, st>>>(
         d_nodes, d_lens, f.E, fr[level & 1], cin, fr[(level + 1) & 1], cout, front_cap,
         reinterpret_cast<uint8_t*>(d_wide), d_tri_index, nn,
         (uint32_t)std::min<uint64_t>(wide_cap, 0xffffffffu),
