@@ -151,7 +151,8 @@ __device__ inline W8Rec w8_leaf_rec(float area, uint32_t p, float c_prim) {
 // registers at fixed indices; the per-budget wide-node counts are read where
 // the decisions pick them, through the pointers (A and B may be global or
 // LDS memory; a local copy indexed at run time would live in scratch).
-__device__ inline W8Rec w8_combine(const W8Rec* A, const W8Rec* B, float area, float c_node,
+template <class Rec>
+__device__ inline W8Rec w8_combine(const Rec* A, const Rec* B, float area, float c_node,
                                    float c_prim, uint32_t p_max) {
   float ac[7], bc[7];
 #pragma unroll
@@ -280,9 +281,9 @@ __global__ __launch_bounds__(256) void k_w8_dp_level(const mgbvh_node32* __restr
 //     W8_SUB objects form the top of the tree; a breadth-first pass over
 //     them lists, level by level, the top nodes and, as subtree roots, their
 //     children with at most W8_SUB objects;
-//   phase 1 (k_w8_sub, one block per subtree root): the subtree's levels and
-//     its DP, bottom up, with every record in LDS; each node's layout entry
-//     and the root's record go to memory;
+//   phase 1 (k_w8_sub, one wavefront per W8_SUBG subtree roots): the
+//     subtrees' levels and their DP, bottom up, with every record in LDS;
+//     each node's layout entry and the roots' records go to memory;
 //   phase 2 (k_w8_topdp, one block per tree): the top nodes' DP, bottom up.
 // Object counts come from lens or, without lens (one object per leaf), from
 // the first children's start indices (a node's lt child starts where it
@@ -415,21 +416,47 @@ __global__ __launch_bounds__(256) void k_w8_top(const mgbvh_node32* __restrict__
   }
 }
 
-// Phase 1: one wavefront per subtree root, persistent over the list (four
-// wavefronts per block, each on its own subtree, synchronised per wavefront
-// only).  Local index of a node: 0 for the root, 1 + (u - first child) for a
-// descendant.  The child index and surface area of every node of the
-// subtree's range are loaded into LDS first, in one coalesced pass, so the
-// level pass and the DP wait on no memory round trip per level.
-struct W8SubLds {
-  W8Rec R[W8_SUBN];
-  uint32_t lch[W8_SUBN];  // child_node of each local node (0: leaf)
-  float lar[W8_SUBN];     // its surface area
-  uint32_t lvs[W8_MAXLV + 2];
-  uint32_t n;
-  uint8_t order[W8_SUBN + 1];  // local indices level by level
+// Phase 1: W8_SUBG subtree roots per wavefront (one wavefront per block,
+// persistent over the list).  The wavefront runs the levels of its subtrees
+// together, so a level holds the nodes of all of them at that depth.  Local
+// index of a node: g * W8_SUBN for the root of the batch's subtree g, plus
+// 1 + (u - first child) for a descendant.  The surface area and the local
+// index of the first child of every node of the subtrees' ranges are loaded
+// into LDS first, in one coalesced pass, so the level pass and the DP wait on
+// no memory round trip per level.  A node's LDS record keeps only what its
+// parent reads.
+constexpr uint32_t W8_SUBG = 3;
+constexpr uint32_t W8_SUBL = W8_SUBG * W8_SUBN;  // local nodes of a batch at most
+
+// Record of a binary node inside a subtree of at most W8_SUB objects, as its
+// parent reads it: costs per budget, wide-node counts per budget (at most the
+// subtree's 63 internal nodes) and objects (at most 64 leaves of p_max <= 3
+// objects; a larger leaf raises the p_max error).
+struct W8L {
+  float c[7];
+  uint8_t ws[7];
+  uint8_t p;
 };
-static_assert(W8_SUBN < 256, "local indices fit a byte");
+static_assert(sizeof(W8L) == 36, "W8L is 36 bytes");
+
+__device__ inline W8L w8_small(const W8Rec& r) {
+  W8L s;
+#pragma unroll
+  for (int i = 0; i < 7; ++i) { s.c[i] = r.c[i]; s.ws[i] = (uint8_t)r.ws[i]; }
+  s.p = (uint8_t)r.p;
+  return s;
+}
+
+struct W8SubLds {
+  W8L R[W8_SUBL];
+  float lar[W8_SUBL];             // surface area of each local node
+  uint32_t lp[W8_SUBL];           // objects of a leaf
+  uint8_t lcl[W8_SUBL];           // local index of its first child in its subtree (0: a leaf)
+  uint16_t order[W8_SUBL + 1];    // local indices level by level
+  uint32_t lvs[W8_MAXLV + 2];
+  uint32_t root[W8_SUBG], c0[W8_SUBG], nloc[W8_SUBG], slot[W8_SUBG];
+};
+static_assert(W8_SUBN < 256 && W8_SUBL < 65536, "local indices fit");
 
 // LDS writes of the wavefront's lanes visible to all of them.
 __device__ inline void w8_wave_sync() {
@@ -438,46 +465,56 @@ __device__ inline void w8_wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__global__ __launch_bounds__(256) void k_w8_sub(const mgbvh_node32* __restrict__ nodes,
-                                                const uint32_t* __restrict__ lens, uint32_t nn,
-                                                const W8Sub* __restrict__ sub, const uint32_t* ctr_in,
-                                                W8Rec* __restrict__ rsub, W8E* __restrict__ E,
-                                                float c_node, float c_prim, uint32_t p_max,
-                                                uint32_t* ctr) {
-  __shared__ W8SubLds lds[4];
-  const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
-  W8SubLds& L = lds[wv];
+__global__ __launch_bounds__(64) void k_w8_sub(const mgbvh_node32* __restrict__ nodes,
+                                               const uint32_t* __restrict__ lens, uint32_t nn,
+                                               const W8Sub* __restrict__ sub, const uint32_t* ctr_in,
+                                               W8Rec* __restrict__ rsub, W8E* __restrict__ E,
+                                               float c_node, float c_prim, uint32_t p_max,
+                                               uint32_t* ctr) {
+  __shared__ W8SubLds L;
+  const uint32_t lane = lane_id();
   const uint32_t nsub = ctr_in[4];
-  for (uint32_t slot = blockIdx.x * 4 + wv; slot < nsub; slot += gridDim.x * 4) {
-    const W8Sub S = sub[slot];
-    const uint32_t r = S.root, bb = S.bbase;
-    const uint32_t rch = nodes[r].child_node;
-    if (rch == 0) {  // a leaf
-      if (lane == 0) {
-        const W8Rec x = w8_leaf_rec(w8_node_area(nodes, r), lens ? lens[r] : 1u, c_prim);
-        if (x.p > p_max) atomicOr(&ctr[0], 8u);
-        rsub[slot] = x;
-        E[r] = w8_e(x);
+  for (uint32_t s0 = blockIdx.x * W8_SUBG; s0 < nsub; s0 += gridDim.x * W8_SUBG) {
+    // the batch's subtrees: lane g sets up subtree s0 + g
+    if (lane < W8_SUBG) {
+      const uint32_t slot = s0 + lane;
+      uint32_t r = 0, c0 = 0, nloc = 0;
+      if (slot < nsub) {
+        const W8Sub S = sub[slot];
+        r = S.root;
+        const uint32_t rch = r < nn ? nodes[r].child_node : 0u;
+        if (r >= nn) {
+          atomicOr(&ctr[0], W8_FAST_FAIL);
+        } else if (rch == 0) {  // a leaf
+          const W8Rec x = w8_leaf_rec(w8_node_area(nodes, r), lens ? lens[r] : 1u, c_prim);
+          if (x.p > p_max) atomicOr(&ctr[0], 8u);
+          rsub[slot] = x;
+          E[r] = w8_e(x);
+        } else {
+          c0 = S.bbase + rch;
+          if (S.n > W8_SUB || S.n < 2 || c0 <= r || c0 >= nn) atomicOr(&ctr[0], W8_FAST_FAIL);
+          else nloc = min(2u * S.n - 1u, W8_SUBN);  // descendants: at most 2 n - 2 after the root
+        }
       }
-      continue;
+      L.root[lane] = r;
+      L.c0[lane] = c0;
+      L.nloc[lane] = nloc;
+      L.slot[lane] = slot;
     }
-    const uint32_t c0 = bb + rch;
-    // descendants: at most 2 n - 2 local slots after the root's
-    const uint32_t nloc = min(2u * S.n - 1u, W8_SUBN);
-    if (S.n > W8_SUB || c0 <= r || c0 >= nn) {  // wave-uniform
-      if (lane == 0) atomicOr(&ctr[0], W8_FAST_FAIL);
-      continue;
-    }
-    // u's first child at local index cl is usable: after u, inside the
-    // subtree's range and the array (the same test in both passes)
-    const auto child_ok = [&](uint32_t u, uint32_t ch, uint32_t cl) {
-      return bb + ch > u && cl >= 1 && cl + 1 < nloc && bb + ch + 1 < nn;
-    };
-    // the range's nodes (those past the array are never reached: their
-    // parents' child test fails first)
-    for (uint32_t loc = lane; loc < nloc; loc += 64) {
+    w8_wave_sync();
+    // the ranges' nodes, flattened over the subtrees (those past the array
+    // are never reached: their parents' child test fails first)
+    uint32_t tot = 0, po[W8_SUBG];
+#pragma unroll
+    for (uint32_t g = 0; g < W8_SUBG; ++g) { po[g] = tot; tot += L.nloc[g]; }
+    for (uint32_t i = lane; i < tot; i += 64) {
+      uint32_t g = 0;
+#pragma unroll
+      for (uint32_t h = 1; h < W8_SUBG; ++h) g = i >= po[h] ? h : g;
+      const uint32_t loc = i - po[g], bb = sub[s0 + g].bbase;
+      const uint32_t r = L.root[g], c0 = L.c0[g], nloc = L.nloc[g];
       const uint32_t u = loc == 0 ? r : c0 + loc - 1;
-      uint32_t ch = 0;
+      uint32_t ch = 0, cl = 0;
       float ar = 0.0f;
       if (u < nn) {
         float mn[3], mx[3];
@@ -485,33 +522,59 @@ __global__ __launch_bounds__(256) void k_w8_sub(const mgbvh_node32* __restrict__
         w8_box(nodes, u, mn, mx, st, ch);
         ar = w8_area(mn, mx);
       }
-      L.lch[loc] = ch;
-      L.lar[loc] = ar;
+      if (ch != 0) {
+        // the first child is usable: after u, inside the subtree's range and
+        // the array
+        cl = bb + ch - c0 + 1;
+        if (!(bb + ch > u && cl >= 1 && cl + 1 < nloc && bb + ch + 1 < nn)) {
+          atomicOr(&ctr[0], W8_FAST_FAIL);
+          cl = 0;
+        }
+      }
+      const uint32_t loc2 = g * W8_SUBN + loc;
+      L.lcl[loc2] = (uint8_t)cl;
+      L.lar[loc2] = ar;
+      if (ch == 0) {  // a leaf's record now: the DP then reads no memory for leaves
+        const uint32_t np = u < nn ? (lens ? lens[u] : 1u) : 0u;
+        L.lp[loc2] = np;
+        L.R[loc2] = w8_small(w8_leaf_rec(ar, np, c_prim));
+      }
     }
-    if (lane == 0) { L.order[0] = 0; L.lvs[0] = 0; L.lvs[1] = 1; L.n = 1; }
+    // level 0: the roots of the subtrees that are not leaves
+    uint32_t n = 0;  // nodes listed so far (wave-uniform)
+    if (lane == 0) {
+      for (uint32_t g = 0; g < W8_SUBG; ++g)
+        if (L.nloc[g] != 0) L.order[n++] = (uint16_t)(g * W8_SUBN);
+      L.lvs[0] = 0;
+      L.lvs[1] = n;
+    }
+    n = __shfl(n, 0, 64);
     w8_wave_sync();
-    // levels, top down
+    // levels, top down: a level's internal nodes append their children in
+    // lane order (ballot and popcount, no atomics)
     uint32_t nlev = 1;
     bool fail = false;
     for (;;) {
       const uint32_t a = L.lvs[nlev - 1], b = L.lvs[nlev];
       if (a == b) break;
-      for (uint32_t i = a + lane; i < b; i += 64) {
-        const uint32_t loc = L.order[i];
-        const uint32_t u = loc == 0 ? r : c0 + loc - 1;
-        const uint32_t ch = L.lch[loc];
-        if (ch == 0) continue;
-        const uint32_t cl = bb + ch - c0 + 1;  // local index of u's first child
-        if (!child_ok(u, ch, cl)) {
-          atomicOr(&ctr[0], W8_FAST_FAIL);
-          continue;
+      for (uint32_t i0 = a; i0 < b; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        uint32_t cl = 0, loc = 0;
+        if (i < b) {
+          loc = L.order[i];
+          cl = L.lcl[loc];
         }
-        const uint32_t k = atomicAdd(&L.n, 2u);
-        if (k + 1 < W8_SUBN) { L.order[k] = (uint8_t)cl; L.order[k + 1] = (uint8_t)(cl + 1); }
-        else atomicOr(&ctr[0], W8_FAST_FAIL);
+        const uint64_t m = __ballot(cl != 0);
+        const uint32_t k = n + 2u * (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        n += 2u * (uint32_t)__popcll(m);
+        if (cl != 0) {
+          const uint32_t c = loc - loc % W8_SUBN + cl;  // combined local index of the first child
+          if (k + 1 < W8_SUBL) { L.order[k] = (uint16_t)c; L.order[k + 1] = (uint16_t)(c + 1); }
+          else atomicOr(&ctr[0], W8_FAST_FAIL);
+        }
       }
-      w8_wave_sync();
-      if (lane == 0) L.lvs[nlev + 1] = min(L.n, W8_SUBN);
+      n = min(n, W8_SUBL);
+      if (lane == 0) L.lvs[nlev + 1] = n;
       w8_wave_sync();
       ++nlev;
       if (nlev + 1 > W8_MAXLV + 1) {
@@ -524,24 +587,28 @@ __global__ __launch_bounds__(256) void k_w8_sub(const mgbvh_node32* __restrict__
     for (int l = (int)nlev - 1; l >= 0 && !fail; --l) {
       for (uint32_t i = L.lvs[l] + lane; i < L.lvs[l + 1]; i += 64) {
         const uint32_t loc = L.order[i];
-        const uint32_t u = loc == 0 ? r : c0 + loc - 1;
-        const uint32_t ch = L.lch[loc];
-        W8Rec x;
-        if (ch == 0) {
-          x = w8_leaf_rec(L.lar[loc], lens ? lens[u] : 1u, c_prim);
-          if (x.p > p_max) atomicOr(&ctr[0], 8u);
-        } else {
-          const uint32_t cl = bb + ch - c0 + 1;
-          if (!child_ok(u, ch, cl)) continue;  // flagged by the level pass
-          x = w8_combine(&L.R[cl], &L.R[cl + 1], L.lar[loc], c_node, c_prim, p_max);
+        const uint32_t g = loc / W8_SUBN, lo = loc - g * W8_SUBN;
+        const uint32_t u = lo == 0 ? L.root[g] : L.c0[g] + lo - 1;
+        const uint32_t cl = L.lcl[loc];
+        if (cl == 0) {  // a leaf (never a root here): its record is in LDS
+          W8E e;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) e.dec[k] = k == 0 ? 1 : 0;
+          e.p = L.lp[loc];
+          e.w = 0;
+          if (e.p > p_max) atomicOr(&ctr[0], 8u);
+          E[u] = e;
+          continue;
         }
-        L.R[loc] = x;
+        const uint32_t c = g * W8_SUBN + cl;
+        const W8Rec x = w8_combine(&L.R[c], &L.R[c + 1], L.lar[loc], c_node, c_prim, p_max);
+        L.R[loc] = w8_small(x);
         E[u] = w8_e(x);
+        if (lo == 0) rsub[L.slot[g]] = x;
       }
       w8_wave_sync();
     }
-    if (lane == 0) rsub[slot] = L.R[0];
-    w8_wave_sync();  // the wavefront's LDS is rewritten by its next subtree
+    w8_wave_sync();  // the wavefront's LDS is rewritten by its next batch
   }
 }
 
@@ -590,14 +657,19 @@ __global__ void k_w8_rootrec_nodes(const W8Tree* __restrict__ trees, uint32_t nt
 }
 
 // Largest q with p + q * s <= x (from the double floor), smallest with >= x.
-__device__ inline uint32_t w8_qlo(float p, float s, float x) {
-  const double q = floor(((double)x - (double)p) / (double)s);
+// s is a power of two, 2^(E - 127) with E in 1..254, and inv = 1 / s exactly,
+// so the product below is the quotient (x - p) / s bit for bit.
+__device__ inline double w8_inv_scale(uint32_t E) {
+  return __longlong_as_double((long long)(1150u - E) << 52);  // 2^(127 - E)
+}
+__device__ inline uint32_t w8_qlo(float p, float s, double inv, float x) {
+  const double q = floor(((double)x - (double)p) * inv);
   uint32_t qi = q < 0.0 ? 0u : q > 255.0 ? 255u : (uint32_t)q;
   while (qi > 0 && __fadd_rn(p, __fmul_rn((float)qi, s)) > x) --qi;
   return qi;
 }
-__device__ inline uint32_t w8_qhi(float p, float s, float x) {
-  const double q = ceil(((double)x - (double)p) / (double)s);
+__device__ inline uint32_t w8_qhi(float p, float s, double inv, float x) {
+  const double q = ceil(((double)x - (double)p) * inv);
   uint32_t qi = q < 0.0 ? 0u : q > 255.0 ? 255u : (uint32_t)q;
   while (qi < 255 && __fadd_rn(p, __fmul_rn((float)qi, s)) < x) ++qi;
   return qi;
@@ -758,12 +830,19 @@ __global__ __launch_bounds__(256) void k_w8_emit(
     float pmn[3], pmx[3];
     uint32_t pst, pch;
     w8_box(nodes, item.bin, pmn, pmx, pst, pch);
+    // lane k of the group finds the exponent of axis k mod 3, the group shares them
+    const uint32_t ka = k % 3u;
+    const float lo_k = ka == 0 ? pmn[0] : (ka == 1 ? pmn[1] : pmn[2]);
+    const float hi_k = ka == 0 ? pmx[0] : (ka == 1 ? pmx[1] : pmx[2]);
+    const uint32_t Ek = w8_exponent(lo_k, hi_k);
     float sc[3];
+    double inv[3];
     uint32_t ebytes = 0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const uint32_t Ex = w8_exponent(pmn[a], pmx[a]);
+      const uint32_t Ex = __shfl(Ek, (int)(g0 + a), 64);
       sc[a] = __uint_as_float(Ex << 23);
+      inv[a] = w8_inv_scale(Ex);
       ebytes |= Ex << (8 * a);
     }
     // words 6-19 (meta and planes) start at zero: empty slots stay zero
@@ -777,8 +856,8 @@ __global__ __launch_bounds__(256) void k_w8_emit(
       w8_box(nodes, u, cmn, cmx, cst, cch);
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        Wb[32 + 8 * a + sidx] = (uint8_t)w8_qlo(pmn[a], sc[a], cmn[a]);
-        Wb[56 + 8 * a + sidx] = (uint8_t)w8_qhi(pmn[a], sc[a], cmx[a]);
+        Wb[32 + 8 * a + sidx] = (uint8_t)w8_qlo(pmn[a], sc[a], inv[a], cmn[a]);
+        Wb[56 + 8 * a + sidx] = (uint8_t)w8_qhi(pmn[a], sc[a], inv[a], cmx[a]);
       }
     }
     if (leaf) {
@@ -1104,10 +1183,10 @@ static int w8_dp_fast(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint32_t*
     return hip_fail(e, "k_w8_top");
   if ((hc[0] & W8_FAST_FAIL) || hc[4] > sub_cap) return 1;
   const uint32_t nsub = hc[4];
-  const uint32_t g1 =
-      std::max<uint32_t>(1, std::min<uint32_t>((nsub + 3) / 4, (uint32_t)c->cus * 8));
-  k_w8_sub<<<g1, 256, 0, st>>>(d_nodes, d_lens, nn, sub, f.ctr, rsub, f.E, P.c_node, P.c_prim,
-                               P.p_max, f.ctr);
+  const uint32_t g1 = std::max<uint32_t>(
+      1, std::min<uint32_t>((nsub + W8_SUBG - 1) / W8_SUBG, (uint32_t)c->cus * 16));
+  k_w8_sub<<<g1, 64, 0, st>>>(d_nodes, d_lens, nn, sub, f.ctr, rsub, f.E, P.c_node, P.c_prim,
+                              P.p_max, f.ctr);
   k_w8_topdp<<<nt, 256, 0, st>>>(d_nodes, d_trees, top, lv, rsub, rtop, f.E, P.c_node,
                                  P.c_prim, P.p_max);
   k_w8_rootrec<<<(nt + 255) / 256, 256, 0, st>>>(d_trees, nt, rsub, rtop, f.rootrec);
