@@ -78,12 +78,25 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
 }
 
 // out[n] = sum over rows of Y[rows][N] (bias gradients).
-__global__ void k_colsum(const float* __restrict__ Y, int rows, int N, float* __restrict__ out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// Column sums of a rows x N row-major matrix with row stride ld: a 1024-thread
+// block takes 64 columns, its 16 wavefronts every 16th row each (coalesced
+// 256-byte row segments), then one wavefront adds the 16 partial sums in a
+// fixed order, so the result is the same on every run.
+__global__ __launch_bounds__(1024) void k_colsum(const float* __restrict__ Y, int rows, int N,
+                                                 int ld, float* __restrict__ out) {
+  __shared__ float part[16][64];
+  const int col = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int n = blockIdx.x * 64 + col;
   float s = 0.0f;
-  for (int r = 0; r < rows; ++r) s += Y[(size_t)r * N + n];
-  out[n] = s;
+  if (n < N)
+    for (int r = g; r < rows; r += 16) s += Y[(size_t)r * ld + n];
+  part[g][col] = s;
+  __syncthreads();
+  if (g == 0 && n < N) {
+    float t = 0.0f;
+    for (int k = 0; k < 16; ++k) t += part[k][col];
+    out[n] = t;
+  }
 }
 
 // Minibatch rows: X[i][0..127] = features[idx[i]] zero-padded, lab[i].
@@ -440,19 +453,19 @@ static int fwd_bwd(hipStream_t st, mgbvh_model* m, int B, const Acts& a, const u
   // backward: dW3 = dZ3^T H2, db3, dZ2 = (dZ3 W3) * (H2 > 0)
   if ((rc = gemm(st, MLP_OUT, MLP_H, B, a.dZ3, MLP_OUT, 1, a.H2, MLP_INP, 0, G + OFF_W3, MLP_H,
                  nullptr, 0, nullptr, 0))) return rc;
-  k_colsum<<<1, 128, 0, st>>>(a.dZ3, B, MLP_OUT, G + OFF_B3);
+  k_colsum<<<cdivu(MLP_OUT, 64), 1024, 0, st>>>(a.dZ3, B, MLP_OUT, MLP_OUT, G + OFF_B3);
   if ((rc = gemm(st, B, MLP_H, MLP_OUT, a.dZ3, MLP_OUT, 0, P + OFF_W3, MLP_H, 0, a.dA, MLP_INP,
                  nullptr, 0, a.H2, MLP_INP))) return rc;
   // dW2 = dZ2^T H1, db2, dZ1 = (dZ2 W2) * (H1 > 0)
   if ((rc = gemm(st, MLP_H, MLP_H, B, a.dA, MLP_INP, 1, a.H1, MLP_INP, 0, G + OFF_W2, MLP_H,
                  nullptr, 0, nullptr, 0))) return rc;
-  k_colsum<<<1, 128, 0, st>>>(a.dA, B, MLP_H, G + OFF_B2);
+  k_colsum<<<cdivu(MLP_H, 64), 1024, 0, st>>>(a.dA, B, MLP_H, MLP_INP, G + OFF_B2);
   if ((rc = gemm(st, B, MLP_H, MLP_H, a.dA, MLP_INP, 0, P + OFF_W2, MLP_H, 0, a.dB, MLP_INP,
                  nullptr, 0, a.H1, MLP_INP))) return rc;
   // dW1 = dZ1^T X, db1
   if ((rc = gemm(st, MLP_H, MLP_INP, B, a.dB, MLP_INP, 1, a.X, MLP_INP, 0, G + OFF_W1, MLP_INP,
                  nullptr, 0, nullptr, 0))) return rc;
-  k_colsum<<<1, 128, 0, st>>>(a.dB, B, MLP_H, G + OFF_B1);
+  k_colsum<<<cdivu(MLP_H, 64), 1024, 0, st>>>(a.dB, B, MLP_H, MLP_INP, G + OFF_B1);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MGBVH_OK : hip_fail(e, "fwd_bwd");
 }
