@@ -130,10 +130,12 @@ __device__ inline bool better(float t, uint32_t inst, uint32_t prim, const Best&
   return inst < b.inst || (inst == b.inst && prim < b.prim);
 }
 
-// Pushes the children of an internal node that pass the slab test, nearer on top.
+// Pushes the children of an internal node that pass the slab test, nearer on
+// top.  Only node indices are kept: a pop reads the node again and repeats its
+// slab test against the closest hit found since (same ray, same box: the same
+// entry distance), which costs less than a stack of distances in scratch.
 __device__ inline void push_children(const RayD& r, const mgbvh_node32* nodes, uint32_t child,
-                                     float best, uint32_t* stk, float* stn, int& sp,
-                                     uint32_t& overflow) {
+                                     float best, uint32_t* stk, int& sp, uint32_t& overflow) {
   float amn[3], amx[3], cmn[3], cmx[3], ta, tb;
   uint32_t s0, c0;
   load_node(nodes, child, amn, amx, s0, c0);
@@ -146,12 +148,12 @@ __device__ inline void push_children(const RayD& r, const mgbvh_node32* nodes, u
   }
   if (ha && hb) {
     const bool a_near = ta <= tb;
-    stk[sp] = a_near ? child + 1 : child; stn[sp] = a_near ? tb : ta; ++sp;
-    stk[sp] = a_near ? child : child + 1; stn[sp] = a_near ? ta : tb; ++sp;
+    stk[sp++] = a_near ? child + 1 : child;
+    stk[sp++] = a_near ? child : child + 1;
   } else if (ha) {
-    stk[sp] = child; stn[sp] = ta; ++sp;
+    stk[sp++] = child;
   } else if (hb) {
-    stk[sp] = child + 1; stn[sp] = tb; ++sp;
+    stk[sp++] = child + 1;
   }
 }
 
@@ -164,17 +166,14 @@ __device__ inline void trace_blas(const RayD& r, const mgbvh_node32* __restrict_
                                   const uint32_t* __restrict__ perm, uint32_t slot0,
                                   uint32_t inst, Best& best, uint32_t& overflow) {
   uint32_t stk[TRACE_STACK];
-  float stn[TRACE_STACK];
-  int sp = 0;
+  int sp = 1;
   float bmn[3], bmx[3], tn;
   uint32_t start, child;
-  load_node(nodes, 0, bmn, bmx, start, child);
-  if (slab(r, bmn, bmx, best.t, tn)) { stk[0] = 0; stn[0] = tn; sp = 1; }
+  stk[0] = 0;
   while (sp > 0) {
-    --sp;
-    const uint32_t v = stk[sp];
-    if (!(stn[sp] <= __fmul_rn(best.t, TN_SLACK))) continue;
+    const uint32_t v = stk[--sp];
     load_node(nodes, v, bmn, bmx, start, child);
+    if (!slab(r, bmn, bmx, best.t, tn)) continue;
     if (child == 0) {
       const uint32_t len = lens ? lens[v] : 1u;
       for (uint32_t k = start; k < start + len; ++k) {
@@ -191,7 +190,7 @@ __device__ inline void trace_blas(const RayD& r, const mgbvh_node32* __restrict_
         }
       }
     } else {
-      push_children(r, nodes, child, best.t, stk, stn, sp, overflow);
+      push_children(r, nodes, child, best.t, stk, sp, overflow);
     }
   }
 }
@@ -258,17 +257,14 @@ __global__ __launch_bounds__(256) void k_trace_scene(
   b.inst = NONE; b.prim = NONE; b.slot = NONE;
   uint32_t overflow = 0;
   uint32_t stk[TRACE_STACK];
-  float stn[TRACE_STACK];
-  int sp = 0;
+  int sp = 1;
   float bmn[3], bmx[3], tn;
   uint32_t start, child;
-  load_node(tnodes, 0, bmn, bmx, start, child);
-  if (slab(r, bmn, bmx, b.t, tn)) { stk[0] = 0; stn[0] = tn; sp = 1; }
+  stk[0] = 0;
   while (sp > 0) {
-    --sp;
-    const uint32_t v = stk[sp];
-    if (!(stn[sp] <= __fmul_rn(b.t, TN_SLACK))) continue;
+    const uint32_t v = stk[--sp];
     load_node(tnodes, v, bmn, bmx, start, child);
+    if (!slab(r, bmn, bmx, b.t, tn)) continue;
     if (child == 0) {
       const uint32_t len = tlens ? tlens[v] : 1u;
       for (uint32_t k = start; k < start + len; ++k) {
@@ -283,7 +279,7 @@ __global__ __launch_bounds__(256) void k_trace_scene(
                    bd.tri_offset, id, b, overflow);
       }
     } else {
-      push_children(r, tnodes, child, b.t, stk, stn, sp, overflow);
+      push_children(r, tnodes, child, b.t, stk, sp, overflow);
     }
   }
   store_hit(hits, i, b);
