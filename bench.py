@@ -350,6 +350,8 @@ def scene_main(args):
     scene_args = (ctx, meshes, sc["transforms"], sc["instance_mesh"])
     tuned = None
     learned = dict(DEFAULT_LEARNED)
+    if args.no_autotune and args.medium_max is not None:
+        ctx.set_tuning(medium_max_objects=args.medium_max)
     if not args.no_autotune:
         tuned = autotune(*scene_args, use_cache=False, log=log)
         if tuned["learned"]:
@@ -666,7 +668,8 @@ def main():
                     help="only the batched plain-rule scene plan, replayed (profiler runs of the "
                          "timed region)")
     ap.add_argument("--medium-max", type=int, default=None,
-                    help="tuning: medium_max_objects of the --plain-plan build")
+                    help="tuning: medium_max_objects of the --plain-plan build, or of "
+                         "the scene builds with --no-autotune")
     ap.add_argument("--wide8", action="store_true",
                     help="--plain-plan: then collapse every BLAS to the 8-wide format 3 times")
     args = ap.parse_args()
