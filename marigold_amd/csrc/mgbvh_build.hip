@@ -1210,6 +1210,15 @@ __device__ inline void med_flush(SNode* sbuf, uint32_t nsb, SNode* __restrict__ 
   for (uint32_t t = lane; t < nsb * 16; t += 64) dst[t] = src[t];
 }
 
+// A wavefront's next medium node: the first gridDim.x are the blocks' own, so
+// a grid that covers the list takes no atomic at all.
+__device__ inline uint32_t med_ticket(Counters* ctr, uint32_t lane, uint32_t nmed) {
+  if (gridDim.x >= nmed) return nmed;
+  uint32_t t = 0;
+  if (lane == 0) t = gridDim.x + atomicAdd(&ctr->n_med_next, 1u);
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+}
+
 __global__ __launch_bounds__(64) void k_medium(const SNode* __restrict__ med,
                                                const uint32_t* __restrict__ n_med_dev,
                                                Rec* __restrict__ rec0, Rec* __restrict__ rec1,
@@ -1224,7 +1233,11 @@ __global__ __launch_bounds__(64) void k_medium(const SNode* __restrict__ med,
   uint32_t nsb = 0, err = 0;  // nsb wave-uniform; err per lane
   unsigned long long visits = 0;
   uint32_t splits = 0, medians = 0;
-  for (uint32_t m = blockIdx.x; m < nmed; m += gridDim.x) {
+  // medium nodes range from 33 to med_max objects: block b takes node b, and
+  // every later one comes from a queue (one atomic per node, none at the
+  // start, where every wavefront would wait on the same address), so that the
+  // wavefronts finish together instead of each working through a fixed share
+  for (uint32_t m = blockIdx.x; m < nmed; m = med_ticket(ctr, lane, nmed)) {
     const SNode root = med[m];
     const uint32_t nbase = root.nbase, obase = root.obase;
     {  // the root's centroid bounds

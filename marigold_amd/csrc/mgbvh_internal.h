@@ -102,7 +102,8 @@ struct Counters {
   unsigned long long sum_large;  // objects of all nodes split with the binned sweep
   uint32_t n_med;                // medium-node list (one wavefront builds each subtree)
   uint32_t n_med_split;          // nodes with > 32 objects split inside those subtrees
-  uint32_t pad[12];
+  uint32_t n_med_next;           // next medium node to take (k_medium's work queue)
+  uint32_t pad[11];
 };
 
 // The small-subtree list (n entries) is followed in the same buffer by the
