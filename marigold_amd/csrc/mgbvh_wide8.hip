@@ -1234,26 +1234,36 @@ static int w8_emit_forest(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint3
   W8Item* fr[2] = {reinterpret_cast<W8Item*>(sc.w8front[0].p),
                    reinterpret_cast<W8Item*>(sc.w8front[1].p)};
   hipStream_t st = c->stream;
+  // Levels go in batches without reading counts back in between: a level's
+  // grid covers up to 8 internal children per node of the last count read,
+  // and a launch with nothing to do returns at once.
   uint32_t n_in = (uint32_t)roots.size();
   hipError_t e;
   if ((e = hipMemsetAsync(f.ctr, 0, 4, st)) != hipSuccess ||
       (e = hipMemcpyAsync(fr[0], roots.data(), sizeof(W8Item) * roots.size(),
                           hipMemcpyHostToDevice, st)) != hipSuccess ||
-      (e = hipMemcpyAsync(f.ctr + 1, &n_in, 4, hipMemcpyHostToDevice, st)) != hipSuccess ||
-      (e = hipStreamSynchronize(st)) != hipSuccess)
+      (e = hipMemcpyAsync(f.ctr + 1, &n_in, 4, hipMemcpyHostToDevice, st)) != hipSuccess)
     return hip_fail(e, "wide8 frontier");
   const uint32_t egrid = std::max<uint32_t>(1, std::min<uint32_t>((nn / 2 + 31) / 32, 8192));
-  for (int level = 0; n_in > 0 && level < 256; ++level) {
-    uint32_t* cin = f.ctr + 1 + (level & 1);
-    uint32_t* cout = f.ctr + 1 + ((level + 1) & 1);
-    if ((e = hipMemsetAsync(cout, 0, 4, st)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
-    k_w8_emit<<<std::min(egrid, (n_in + 31) / 32), 256, 0, st>>>(
-        d_nodes, d_lens, f.E, fr[level & 1], cin, fr[(level + 1) & 1], cout, front_cap,
-        reinterpret_cast<uint8_t*>(d_wide), d_tri_index, nn,
-        (uint32_t)std::min<uint64_t>(wide_cap, 0xffffffffu),
-        (uint32_t)std::min<uint64_t>(tri_cap, 0xffffffffu), f.ctr);
-    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "k_w8_emit");
-    if ((e = hipMemcpyAsync(&n_in, cout, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+  constexpr int W8_EMIT_BATCH = 4;
+  int level = 0;
+  while (n_in > 0 && level < 256) {
+    uint64_t bound = n_in;  // items of this level at most
+    for (int b = 0; b < W8_EMIT_BATCH && level < 256; ++b, ++level) {
+      uint32_t* cin = f.ctr + 1 + (level & 1);
+      uint32_t* cout = f.ctr + 1 + ((level + 1) & 1);
+      const uint32_t grid = (uint32_t)std::min<uint64_t>(egrid, (bound + 31) / 32);
+      if ((e = hipMemsetAsync(cout, 0, 4, st)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+      k_w8_emit<<<std::max<uint32_t>(grid, 1), 256, 0, st>>>(
+          d_nodes, d_lens, f.E, fr[level & 1], cin, fr[(level + 1) & 1], cout, front_cap,
+          reinterpret_cast<uint8_t*>(d_wide), d_tri_index, nn,
+          (uint32_t)std::min<uint64_t>(wide_cap, 0xffffffffu),
+          (uint32_t)std::min<uint64_t>(tri_cap, 0xffffffffu), f.ctr);
+      if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "k_w8_emit");
+      bound = std::min<uint64_t>(bound * 8, front_cap);
+    }
+    if ((e = hipMemcpyAsync(&n_in, f.ctr + 1 + (level & 1), 4, hipMemcpyDeviceToHost, st)) !=
+            hipSuccess ||
         (e = hipStreamSynchronize(st)) != hipSuccess)
       return hip_fail(e, "wide8 level");
   }
