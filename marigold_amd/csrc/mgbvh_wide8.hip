@@ -696,15 +696,17 @@ __device__ inline uint32_t w8_exponent(float lo, float hi) {
 // of the leaf slots before it, internal slots before it, and the wide nodes
 // and objects of the internal children before it).  The node's words are
 // assembled in LDS and stored as five 16-byte writes.
-__global__ __launch_bounds__(256) void k_w8_emit(
+constexpr uint32_t W8E_THREADS = 256;           // threads of a layout block
+constexpr uint32_t W8E_NODES = W8E_THREADS / 8;  // wide nodes per block and step
+__global__ __launch_bounds__(W8E_THREADS) void k_w8_emit(
     const mgbvh_node32* __restrict__ nodes, const uint32_t* __restrict__ lens,
     const W8E* __restrict__ E, const W8Item* __restrict__ in, const uint32_t* n_in,
     W8Item* __restrict__ out, uint32_t* n_out, uint32_t front_cap, uint8_t* __restrict__ wide,
     uint32_t* __restrict__ tri_index, uint32_t nn, uint32_t wide_cap, uint32_t tri_cap,
     uint32_t* err_flag) {
-  __shared__ uint32_t wtot[4], bbase;
-  __shared__ __align__(16) uint32_t nw[32][20];  // the node words of each group
-  __shared__ uint32_t box_u[256], box_b[256];    // hand-over of right parts, per lane
+  __shared__ uint32_t wtot[W8E_THREADS / 64], bbase;
+  __shared__ __align__(16) uint32_t nw[W8E_NODES][20];  // the node words of each group
+  __shared__ uint32_t box_u[W8E_THREADS], box_b[W8E_THREADS];  // hand-over of right parts, per lane
   const uint32_t count = *n_in;
   const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, t = threadIdx.x;
   const uint32_t k = lane & 7u, g = t >> 3, g0 = lane & ~7u;
@@ -715,7 +717,7 @@ __global__ __launch_bounds__(256) void k_w8_emit(
   // block-uniform trip count: every thread takes part in the frontier
   // allocation below, one atomic per block and step (same-address atomics
   // serialise device-wide)
-  for (uint32_t b0 = blockIdx.x * 32; b0 < count; b0 += gridDim.x * 32) {
+  for (uint32_t b0 = blockIdx.x * W8E_NODES; b0 < count; b0 += gridDim.x * W8E_NODES) {
     const uint32_t it = b0 + g;
     bool ok = it < count;  // group-uniform
     W8Item item{};
@@ -1244,7 +1246,8 @@ static int w8_emit_forest(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint3
                           hipMemcpyHostToDevice, st)) != hipSuccess ||
       (e = hipMemcpyAsync(f.ctr + 1, &n_in, 4, hipMemcpyHostToDevice, st)) != hipSuccess)
     return hip_fail(e, "wide8 frontier");
-  const uint32_t egrid = std::max<uint32_t>(1, std::min<uint32_t>((nn / 2 + 31) / 32, 8192));
+  const uint32_t egrid = std::max<uint32_t>(
+      1, std::min<uint32_t>((nn / 2 + W8E_NODES - 1) / W8E_NODES, 8192 * 32 / W8E_NODES));
   constexpr int W8_EMIT_BATCH = 4;
   int level = 0;
   while (n_in > 0 && level < 256) {
@@ -1252,9 +1255,9 @@ static int w8_emit_forest(mgbvh_ctx* c, const mgbvh_node32* d_nodes, const uint3
     for (int b = 0; b < W8_EMIT_BATCH && level < 256; ++b, ++level) {
       uint32_t* cin = f.ctr + 1 + (level & 1);
       uint32_t* cout = f.ctr + 1 + ((level + 1) & 1);
-      const uint32_t grid = (uint32_t)std::min<uint64_t>(egrid, (bound + 31) / 32);
+      const uint32_t grid = (uint32_t)std::min<uint64_t>(egrid, (bound + W8E_NODES - 1) / W8E_NODES);
       if ((e = hipMemsetAsync(cout, 0, 4, st)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
-      k_w8_emit<<<std::max<uint32_t>(grid, 1), 256, 0, st>>>(
+      k_w8_emit<<<std::max<uint32_t>(grid, 1), W8E_THREADS, 0, st>>>(
           d_nodes, d_lens, f.E, fr[level & 1], cin, fr[(level + 1) & 1], cout, front_cap,
           reinterpret_cast<uint8_t*>(d_wide), d_tri_index, nn,
           (uint32_t)std::min<uint64_t>(wide_cap, 0xffffffffu),
