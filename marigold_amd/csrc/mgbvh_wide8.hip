@@ -420,18 +420,20 @@ __global__ __launch_bounds__(256) void k_w8_top(const mgbvh_node32* __restrict__
 // persistent over the list).  The wavefront runs the levels of its subtrees
 // together, so a level holds the nodes of all of them at that depth.  Local
 // index of a node: g * W8_SUBN for the root of the batch's subtree g, plus
-// 1 + (u - first child) for a descendant.  The surface area and the local
-// index of the first child of every node of the subtrees' ranges are loaded
-// into LDS first, in one coalesced pass, so the level pass and the DP wait on
-// no memory round trip per level.  A node's LDS record keeps only what its
-// parent reads.
-constexpr uint32_t W8_SUBG = 3;
-constexpr uint32_t W8_SUBL = W8_SUBG * W8_SUBN;  // local nodes of a batch at most
+// 1 + (u - first child) for a descendant.  The surface area, the object count
+// and the local index of the first child of every node of the subtrees'
+// ranges are loaded into LDS first, in one coalesced pass, so the level pass
+// and the DP wait on no memory round trip per level.  Only internal nodes
+// keep a record in LDS, and only what a parent reads of it: a leaf's record
+// follows from its area and object count, and the DP forms it on the fly.
+constexpr uint32_t W8_SUBG = 4;
+constexpr uint32_t W8_SUBL = W8_SUBG * W8_SUBN;       // local nodes of a batch at most
+constexpr uint32_t W8_SUBI = W8_SUBG * (W8_SUB - 1);  // internal nodes of a batch at most
 
-// Record of a binary node inside a subtree of at most W8_SUB objects, as its
-// parent reads it: costs per budget, wide-node counts per budget (at most the
-// subtree's 63 internal nodes) and objects (at most 64 leaves of p_max <= 3
-// objects; a larger leaf raises the p_max error).
+// Record of an internal binary node inside a subtree of at most W8_SUB
+// objects, as its parent reads it: costs per budget, wide-node counts per
+// budget (at most the subtree's 63 internal nodes) and objects (at most 64
+// leaves of p_max <= 3 objects; a larger leaf raises the p_max error).
 struct W8L {
   float c[7];
   uint8_t ws[7];
@@ -447,12 +449,75 @@ __device__ inline W8L w8_small(const W8Rec& r) {
   return s;
 }
 
+// w8_combine for children that may be leaves: a leaf child (la / lb) reads as
+// the record w8_leaf_rec gives it (every cost its leaf cost cl, no wide
+// nodes, its p objects), an internal child as its LDS record (A / B; for a
+// leaf child any valid slot, whose values are not used).  Selects, not
+// branches, so lanes with different kinds of children stay together; the
+// result equals w8_combine on the full records bit for bit.
+__device__ inline W8Rec w8_combine_lr(const W8L* A, bool la, float cla, uint32_t pa, const W8L* B,
+                                      bool lb, float clb, uint32_t pb, float area, float c_node,
+                                      float c_prim, uint32_t p_max) {
+  float ac[7], bc[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    ac[k] = la ? cla : A->c[k];
+    bc[k] = lb ? clb : B->c[k];
+  }
+  W8Rec r;
+  r.p = (la ? pa : (uint32_t)A->p) + (lb ? pb : (uint32_t)B->p);
+  float dist[9];
+  uint32_t arg[9];
+#pragma unroll
+  for (uint32_t i = 2; i <= 8; ++i) {
+    float best = __uint_as_float(0x7f800000u);
+    uint32_t bk = 1;
+#pragma unroll
+    for (uint32_t k = 1; k < i; ++k) {
+      if (k > 7 || i - k > 7) continue;
+      const float s = __fadd_rn(ac[k - 1], bc[i - k - 1]);
+      if (s < best) { best = s; bk = k; }
+    }
+    dist[i] = best;
+    arg[i] = bk;
+  }
+  const float cint = __fadd_rn(__fmul_rn(area, c_node), dist[8]);
+  const float cleaf = r.p <= p_max ? __fmul_rn(__fmul_rn(area, (float)r.p), c_prim)
+                                   : __uint_as_float(0x7f800000u);
+  const bool leaf = r.p <= p_max && cleaf <= cint;
+  r.c8 = cint;
+  r.dec[0] = leaf ? 1 : 0;
+  r.c[0] = leaf ? cleaf : cint;
+#pragma unroll
+  for (uint32_t i = 2; i <= 7; ++i) {
+    const bool split = dist[i] < r.c[i - 2];
+    r.c[i - 1] = split ? dist[i] : r.c[i - 2];
+    r.dec[i - 1] = split ? (uint8_t)arg[i] : 0;
+  }
+  r.dec[7] = (uint8_t)arg[8];
+  const uint32_t k8 = arg[8];  // 1..7
+  const uint32_t wa8 = la ? 0u : (uint32_t)A->ws[k8 - 1];
+  const uint32_t wb8 = lb ? 0u : (uint32_t)B->ws[8 - k8 - 1];
+  r.w = 1 + wa8 + wb8;
+  r.ws[0] = leaf ? 0u : r.w;
+#pragma unroll
+  for (uint32_t i = 2; i <= 7; ++i) {
+    const uint32_t k = r.dec[i - 1];  // 0 or 1..i-1
+    const uint32_t kk = k == 0 ? 1u : k;
+    const uint32_t wa = la ? 0u : (uint32_t)A->ws[kk - 1];
+    const uint32_t wb = lb ? 0u : (uint32_t)B->ws[i - kk - 1];
+    r.ws[i - 1] = k == 0 ? r.ws[i - 2] : wa + wb;
+  }
+  return r;
+}
+
 struct W8SubLds {
-  W8L R[W8_SUBL];
+  W8L R[W8_SUBI];                 // records of the internal nodes, by internal ordinal
   float lar[W8_SUBL];             // surface area of each local node
-  uint32_t lp[W8_SUBL];           // objects of a leaf
-  uint8_t lcl[W8_SUBL];           // local index of its first child in its subtree (0: a leaf)
+  uint16_t iord[W8_SUBL];         // internal ordinal of an internal node the level pass reached
   uint16_t order[W8_SUBL + 1];    // local indices level by level
+  uint8_t lcl[W8_SUBL];           // local index of its first child in its subtree (0: a leaf)
+  uint8_t lp[W8_SUBL];            // objects of a leaf, saturated at 255
   uint32_t lvs[W8_MAXLV + 2];
   uint32_t root[W8_SUBG], c0[W8_SUBG], nloc[W8_SUBG], slot[W8_SUBG];
 };
@@ -474,6 +539,7 @@ __global__ __launch_bounds__(64) void k_w8_sub(const mgbvh_node32* __restrict__ 
   __shared__ W8SubLds L;
   const uint32_t lane = lane_id();
   const uint32_t nsub = ctr_in[4];
+  const uint64_t below = (1ull << lane) - 1ull;
   for (uint32_t s0 = blockIdx.x * W8_SUBG; s0 < nsub; s0 += gridDim.x * W8_SUBG) {
     // the batch's subtrees: lane g sets up subtree s0 + g
     if (lane < W8_SUBG) {
@@ -534,24 +600,27 @@ __global__ __launch_bounds__(64) void k_w8_sub(const mgbvh_node32* __restrict__ 
       const uint32_t loc2 = g * W8_SUBN + loc;
       L.lcl[loc2] = (uint8_t)cl;
       L.lar[loc2] = ar;
-      if (ch == 0) {  // a leaf's record now: the DP then reads no memory for leaves
-        const uint32_t np = u < nn ? (lens ? lens[u] : 1u) : 0u;
-        L.lp[loc2] = np;
-        L.R[loc2] = w8_small(w8_leaf_rec(ar, np, c_prim));
-      }
+      const uint32_t np = ch == 0 && u < nn ? (lens ? lens[u] : 1u) : 0u;
+      L.lp[loc2] = (uint8_t)min(np, 255u);
     }
-    // level 0: the roots of the subtrees that are not leaves
-    uint32_t n = 0;  // nodes listed so far (wave-uniform)
+    // level 0: the roots of the subtrees that are not leaves (internal
+    // ordinals 0, 1, ...)
+    uint32_t n = 0, ni = 0;  // nodes listed and internal ordinals given so far (wave-uniform)
     if (lane == 0) {
       for (uint32_t g = 0; g < W8_SUBG; ++g)
-        if (L.nloc[g] != 0) L.order[n++] = (uint16_t)(g * W8_SUBN);
+        if (L.nloc[g] != 0) {
+          L.iord[g * W8_SUBN] = (uint16_t)n;
+          L.order[n++] = (uint16_t)(g * W8_SUBN);
+        }
       L.lvs[0] = 0;
       L.lvs[1] = n;
     }
     n = __shfl(n, 0, 64);
+    ni = n;
     w8_wave_sync();
     // levels, top down: a level's internal nodes append their children in
-    // lane order (ballot and popcount, no atomics)
+    // lane order and give the internal ones their ordinals (ballot and
+    // popcount, no atomics)
     uint32_t nlev = 1;
     bool fail = false;
     for (;;) {
@@ -564,20 +633,25 @@ __global__ __launch_bounds__(64) void k_w8_sub(const mgbvh_node32* __restrict__ 
           loc = L.order[i];
           cl = L.lcl[loc];
         }
-        const uint64_t m = __ballot(cl != 0);
-        const uint32_t k = n + 2u * (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        const uint32_t c = loc - loc % W8_SUBN + cl;  // combined local index of the first child
+        const bool i0c = cl != 0 && L.lcl[c] != 0, i1c = cl != 0 && L.lcl[c + 1] != 0;
+        const uint64_t m = __ballot(cl != 0), m0 = __ballot(i0c), m1 = __ballot(i1c);
+        const uint32_t k = n + 2u * (uint32_t)__popcll(m & below);
+        const uint32_t q = ni + (uint32_t)__popcll(m0 & below) + (uint32_t)__popcll(m1 & below);
         n += 2u * (uint32_t)__popcll(m);
+        ni += (uint32_t)__popcll(m0) + (uint32_t)__popcll(m1);
         if (cl != 0) {
-          const uint32_t c = loc - loc % W8_SUBN + cl;  // combined local index of the first child
           if (k + 1 < W8_SUBL) { L.order[k] = (uint16_t)c; L.order[k + 1] = (uint16_t)(c + 1); }
           else atomicOr(&ctr[0], W8_FAST_FAIL);
+          if (i0c) L.iord[c] = (uint16_t)q;
+          if (i1c) L.iord[c + 1] = (uint16_t)(q + (i0c ? 1u : 0u));
         }
       }
       n = min(n, W8_SUBL);
       if (lane == 0) L.lvs[nlev + 1] = n;
       w8_wave_sync();
       ++nlev;
-      if (nlev + 1 > W8_MAXLV + 1) {
+      if (nlev + 1 > W8_MAXLV + 1 || ni > W8_SUBI) {
         fail = true;
         if (lane == 0) atomicOr(&ctr[0], W8_FAST_FAIL);
         break;
@@ -590,19 +664,25 @@ __global__ __launch_bounds__(64) void k_w8_sub(const mgbvh_node32* __restrict__ 
         const uint32_t g = loc / W8_SUBN, lo = loc - g * W8_SUBN;
         const uint32_t u = lo == 0 ? L.root[g] : L.c0[g] + lo - 1;
         const uint32_t cl = L.lcl[loc];
-        if (cl == 0) {  // a leaf (never a root here): its record is in LDS
+        if (cl == 0) {  // a leaf (never a root here): its record follows from area and count
           W8E e;
 #pragma unroll
           for (int k = 0; k < 8; ++k) e.dec[k] = k == 0 ? 1 : 0;
-          e.p = L.lp[loc];
+          e.p = L.lp[loc];  // exact up to 254, so a leaf over p_max <= 3 is still caught
           e.w = 0;
           if (e.p > p_max) atomicOr(&ctr[0], 8u);
           E[u] = e;
           continue;
         }
         const uint32_t c = g * W8_SUBN + cl;
-        const W8Rec x = w8_combine(&L.R[c], &L.R[c + 1], L.lar[loc], c_node, c_prim, p_max);
-        L.R[loc] = w8_small(x);
+        const bool la = L.lcl[c] == 0, lb = L.lcl[c + 1] == 0;
+        const uint32_t pa = L.lp[c], pb = L.lp[c + 1];
+        const float cla = __fmul_rn(__fmul_rn(L.lar[c], (float)pa), c_prim);
+        const float clb = __fmul_rn(__fmul_rn(L.lar[c + 1], (float)pb), c_prim);
+        const uint32_t ia = la ? 0u : L.iord[c], ib = lb ? 0u : L.iord[c + 1];
+        const W8Rec x = w8_combine_lr(&L.R[ia], la, cla, pa, &L.R[ib], lb, clb, pb, L.lar[loc],
+                                      c_node, c_prim, p_max);
+        L.R[L.iord[loc]] = w8_small(x);
         E[u] = w8_e(x);
         if (lo == 0) rsub[L.slot[g]] = x;
       }
