@@ -803,18 +803,29 @@ __global__ __launch_bounds__(W8E_THREADS) void k_w8_emit(
     W8Item item{};
     uint32_t u = 0, b = 0;  // this lane's expansion (b == 0: none)
     bool fin = false, bad = false;
+    // the 32-byte node and the layout entry of u, as last read (a finished
+    // slot keeps those of its own node, so nothing is read again after the
+    // expansion), and the wide node's own box, read before the expansion
+    uint4 na = make_uint4(0u, 0u, 0u, 0u), nb = na, ee = na, pa = na, pb = na;
     if (ok) {
       item = in[it];
       if (item.bin >= nn) {
         bad = true;
       } else {
-        const uint32_t v = item.bin, ch = w8_child(nodes, item.bbase, v);
+        const uint32_t v = item.bin;
+        pa = reinterpret_cast<const uint4*>(&nodes[v])[0];
+        pb = reinterpret_cast<const uint4*>(&nodes[v])[1];
+        const uint32_t ch = pb.w == 0 ? 0u : item.bbase + pb.w;
         if (ch == 0) {
-          if (k == 0) { u = v; b = 1; fin = true; }
+          if (k == 0) {
+            u = v; b = 1; fin = true;
+            na = pa; nb = pb;
+            ee = *reinterpret_cast<const uint4*>(&E[v]);
+          }
         } else if (ch + 1 >= nn) {
           bad = true;
         } else {
-          const uint32_t kk = E[v].dec[7];
+          const uint32_t kk = E[v].dec[7] & 0xffu;
           if (kk < 1 || kk > 7) bad = true;
           else if (k == 0) { u = ch; b = kk; }
           else if (k == kk) { u = ch + 1; b = 8 - kk; }
@@ -827,14 +838,16 @@ __global__ __launch_bounds__(W8E_THREADS) void k_w8_emit(
       bool send = false;
       uint32_t su = 0, sb = 0, kk = 0;
       if (act) {
-        const uint32_t c = w8_child(nodes, item.bbase, u);
+        na = reinterpret_cast<const uint4*>(&nodes[u])[0];
+        nb = reinterpret_cast<const uint4*>(&nodes[u])[1];
+        ee = *reinterpret_cast<const uint4*>(&E[u]);
+        const uint32_t c = nb.w == 0 ? 0u : item.bbase + nb.w;
         if (c == 0) {
           fin = true;
         } else if (c + 1 >= nn) {
           bad = true;
         } else {
-          const uint2 dw = *reinterpret_cast<const uint2*>(E[u].dec);
-          const uint64_t d = ((uint64_t)dw.y << 32) | dw.x;
+          const uint64_t d = ((uint64_t)ee.y << 32) | ee.x;
           while (b > 1 && ((d >> (8 * (b - 1))) & 0xffu) == 0) --b;
           if (b == 1) {
             fin = true;
@@ -872,12 +885,12 @@ __global__ __launch_bounds__(W8E_THREADS) void k_w8_emit(
     const bool has = ok && fin;
     const uint32_t fmask = (uint32_t)(__ballot(has) >> g0) & 0xffu;
     const uint32_t sidx = __popc(fmask & ((1u << k) - 1u));  // this lane's slot
-    const bool leaf = has && w8_leaf_slot(nodes, E, u);
+    // w8_leaf_slot from the registers: no child, or a leaf at one slot
+    const bool leaf = has && (nb.w == 0 || (ee.x & 0xffu) == 1u);
     const bool inner = has && !leaf;
-    const W8E em = has ? E[u] : W8E{};
-    const uint32_t p = leaf ? em.p : 0u;        // objects of a leaf slot
-    const uint32_t wi = inner ? em.w - 1 : 0u;  // wide nodes below an internal child
-    const uint32_t ti = inner ? em.p : 0u;      // its objects
+    const uint32_t p = leaf ? ee.z : 0u;        // objects of a leaf slot
+    const uint32_t wi = inner ? ee.w - 1 : 0u;  // wide nodes below an internal child
+    const uint32_t ti = inner ? ee.z : 0u;      // its objects
     const uint32_t ii = inner ? 1u : 0u;
     // inclusive scans over the group's 8 lanes (lane order is slot order)
     uint32_t sp = p, sw = wi, st_ = ti, si = ii;
@@ -909,9 +922,8 @@ __global__ __launch_bounds__(W8E_THREADS) void k_w8_emit(
     __syncthreads();  // wtot and bbase are rewritten by the next step
     if (!ok) continue;
     // the node's frame (every lane of the group: the same values)
-    float pmn[3], pmx[3];
-    uint32_t pst, pch;
-    w8_box(nodes, item.bin, pmn, pmx, pst, pch);
+    const float pmn[3] = {__uint_as_float(pa.x), __uint_as_float(pa.y), __uint_as_float(pa.z)};
+    const float pmx[3] = {__uint_as_float(pb.x), __uint_as_float(pb.y), __uint_as_float(pb.z)};
     // lane k of the group finds the exponent of axis k mod 3, the group shares them
     const uint32_t ka = k % 3u;
     const float lo_k = ka == 0 ? pmn[0] : (ka == 1 ? pmn[1] : pmn[2]);
@@ -933,9 +945,8 @@ __global__ __launch_bounds__(W8E_THREADS) void k_w8_emit(
     w8_wave_sync();
     uint32_t ibit = 0;
     if (has) {
-      float cmn[3], cmx[3];
-      uint32_t cst, cch;
-      w8_box(nodes, u, cmn, cmx, cst, cch);
+      const float cmn[3] = {__uint_as_float(na.x), __uint_as_float(na.y), __uint_as_float(na.z)};
+      const float cmx[3] = {__uint_as_float(nb.x), __uint_as_float(nb.y), __uint_as_float(nb.z)};
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         Wb[32 + 8 * a + sidx] = (uint8_t)w8_qlo(pmn[a], sc[a], inv[a], cmn[a]);
@@ -947,7 +958,7 @@ __global__ __launch_bounds__(W8E_THREADS) void k_w8_emit(
       // the leaf's objects, child before child + 1: p <= 3 objects lie in at
       // most 3 binary leaves, at most two levels below the slot
       uint32_t to = toff;
-      const uint32_t c = w8_child(nodes, item.bbase, u);
+      const uint32_t c = nb.w == 0 ? 0u : item.bbase + nb.w;
       for (uint32_t i = 0; i < (c == 0 ? 1u : 2u); ++i) {
         const uint32_t x = c == 0 ? u : c + i;
         const uint32_t cx = c == 0 ? 0u : w8_child(nodes, item.bbase, x);
