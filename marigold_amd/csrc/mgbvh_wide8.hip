@@ -573,19 +573,47 @@ __global__ __launch_bounds__(64) void k_w8_sub(const mgbvh_node32* __restrict__ 
     uint32_t tot = 0, po[W8_SUBG];
 #pragma unroll
     for (uint32_t g = 0; g < W8_SUBG; ++g) { po[g] = tot; tot += L.nloc[g]; }
-    for (uint32_t i = lane; i < tot; i += 64) {
+    // all of the batch's reads are issued before any is used (the kernel's
+    // resident wavefronts are bounded by LDS, not registers)
+    constexpr uint32_t W8_LOADS = (W8_SUBL + 63) / 64;
+    uint4 xa[W8_LOADS], xb[W8_LOADS];
+    uint32_t xl[W8_LOADS];
+#pragma unroll
+    for (uint32_t j = 0; j < W8_LOADS; ++j) {
+      const uint32_t i = lane + 64 * j;
+      xa[j] = xb[j] = make_uint4(0u, 0u, 0u, 0u);
+      xl[j] = 0;
+      if (i < tot) {
+        uint32_t g = 0;
+#pragma unroll
+        for (uint32_t h = 1; h < W8_SUBG; ++h) g = i >= po[h] ? h : g;
+        const uint32_t loc = i - po[g];
+        const uint32_t u = loc == 0 ? L.root[g] : L.c0[g] + loc - 1;
+        if (u < nn) {
+          xa[j] = reinterpret_cast<const uint4*>(&nodes[u])[0];
+          xb[j] = reinterpret_cast<const uint4*>(&nodes[u])[1];
+          xl[j] = lens ? lens[u] : 1u;
+        }
+      }
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < W8_LOADS; ++j) {
+      const uint32_t i = lane + 64 * j;
+      if (i >= tot) continue;
       uint32_t g = 0;
 #pragma unroll
       for (uint32_t h = 1; h < W8_SUBG; ++h) g = i >= po[h] ? h : g;
       const uint32_t loc = i - po[g], bb = sub[s0 + g].bbase;
-      const uint32_t r = L.root[g], c0 = L.c0[g], nloc = L.nloc[g];
-      const uint32_t u = loc == 0 ? r : c0 + loc - 1;
-      uint32_t ch = 0, cl = 0;
+      const uint32_t c0 = L.c0[g], nloc = L.nloc[g];
+      const uint32_t u = loc == 0 ? L.root[g] : c0 + loc - 1;
+      const uint32_t ch = u < nn ? xb[j].w : 0u;
+      uint32_t cl = 0;
       float ar = 0.0f;
       if (u < nn) {
-        float mn[3], mx[3];
-        uint32_t st;
-        w8_box(nodes, u, mn, mx, st, ch);
+        const float mn[3] = {__uint_as_float(xa[j].x), __uint_as_float(xa[j].y),
+                             __uint_as_float(xa[j].z)};
+        const float mx[3] = {__uint_as_float(xb[j].x), __uint_as_float(xb[j].y),
+                             __uint_as_float(xb[j].z)};
         ar = w8_area(mn, mx);
       }
       if (ch != 0) {
@@ -600,7 +628,7 @@ __global__ __launch_bounds__(64) void k_w8_sub(const mgbvh_node32* __restrict__ 
       const uint32_t loc2 = g * W8_SUBN + loc;
       L.lcl[loc2] = (uint8_t)cl;
       L.lar[loc2] = ar;
-      const uint32_t np = ch == 0 && u < nn ? (lens ? lens[u] : 1u) : 0u;
+      const uint32_t np = ch == 0 && u < nn ? xl[j] : 0u;
       L.lp[loc2] = (uint8_t)min(np, 255u);
     }
     // level 0: the roots of the subtrees that are not leaves (internal
