@@ -843,17 +843,18 @@ __global__ __launch_bounds__(W8E_THREADS) void k_w8_emit(
         const uint32_t v = item.bin;
         pa = reinterpret_cast<const uint4*>(&nodes[v])[0];
         pb = reinterpret_cast<const uint4*>(&nodes[v])[1];
+        const uint4 ev = *reinterpret_cast<const uint4*>(&E[v]);  // read beside the node
         const uint32_t ch = pb.w == 0 ? 0u : item.bbase + pb.w;
         if (ch == 0) {
           if (k == 0) {
             u = v; b = 1; fin = true;
             na = pa; nb = pb;
-            ee = *reinterpret_cast<const uint4*>(&E[v]);
+            ee = ev;
           }
         } else if (ch + 1 >= nn) {
           bad = true;
         } else {
-          const uint32_t kk = E[v].dec[7] & 0xffu;
+          const uint32_t kk = ev.y >> 24;  // dec[7]
           if (kk < 1 || kk > 7) bad = true;
           else if (k == 0) { u = ch; b = kk; }
           else if (k == kk) { u = ch + 1; b = 8 - kk; }
