@@ -28,7 +28,7 @@
 namespace mg {
 
 constexpr uint32_t W8_NONE = 0xffffffffu;
-constexpr int W8_STACK = 256;  // traversal: at most 7 more entries per wide level
+constexpr int W8_GSTACK = 64;  // traversal: one node group per wide level at most
 
 // Per binary node.  The objects below any set of slots that covers v's
 // subtree are all of its p objects (each object sits in exactly one wide
@@ -1033,30 +1033,25 @@ __global__ __launch_bounds__(W8E_THREADS) void k_w8_emit(
 }
 
 // ------------------------------------------------------------------ traversal
-__device__ inline bool w8_slab(const float* o, const float* inv, const float* bmn,
-                               const float* bmx, float best, float& tn_out) {
-  // the binary traversal's pad: max(2^-20 x the box's coordinate size,
-  // 2^-16 x the farthest box coordinate from the origin)
+// The binary traversal's slab test and margins (the box padded per axis by
+// max(2^-20 x the box's coordinate size, 2^-16 x its farthest coordinate from
+// the origin), far distances widened by 1 + 2^-20, near ones compared with
+// the closest hit widened by 1 + 2^-10), with the pad given: a wide node's
+// pad from its quantisation frame [p, p + 255 s], which holds every child's
+// box, is at least each child's own pad (a larger box and a farther corner),
+// so the test only keeps more.
+__device__ inline bool w8_slab_pad(const float* o, const float* inv, const float* bmn,
+                                   const float* bmx, const float* pad, float best) {
   float tn = -__uint_as_float(0x7f800000u), tf = __uint_as_float(0x7f800000u);
-  float lo[3], hi[3], dist = 0.0f;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    lo[a] = __fsub_rn(bmn[a], o[a]);
-    hi[a] = __fsub_rn(bmx[a], o[a]);
-    dist = fmaxf(dist, fmaxf(fabsf(lo[a]), fabsf(hi[a])));
-  }
-  const float dp = __fmul_rn(dist, 1.52587890625e-05f);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float pad = fmaxf(__fmul_rn(fmaxf(fabsf(bmn[a]), fabsf(bmx[a])), 9.5367431640625e-07f), dp);
-    const float t0 = __fmul_rn(__fsub_rn(lo[a], pad), inv[a]);
-    const float t1 = __fmul_rn(__fadd_rn(hi[a], pad), inv[a]);
+    const float t0 = __fmul_rn(__fsub_rn(__fsub_rn(bmn[a], o[a]), pad[a]), inv[a]);
+    const float t1 = __fmul_rn(__fadd_rn(__fsub_rn(bmx[a], o[a]), pad[a]), inv[a]);
     const bool free_axis = (t0 != t0) | (t1 != t1);
     tn = fmaxf(tn, free_axis ? -__uint_as_float(0x7f800000u) : fminf(t0, t1));
     tf = fminf(tf, free_axis ? __uint_as_float(0x7f800000u) : fmaxf(t0, t1));
   }
-  tf = __fmul_rn(tf, 1.00000095367431640625f);  // the binary traversal's margins
-  tn_out = tn;
+  tf = __fmul_rn(tf, 1.00000095367431640625f);
   return tn <= tf && tf >= 0.0f && tn <= __fmul_rn(best, 1.0009765625f);
 }
 
@@ -1104,14 +1099,24 @@ __global__ __launch_bounds__(256) void k_trace_w8(
   for (int a = 0; a < 3; ++a) inv[a] = __fdiv_rn(1.0f, d[a]);
   float best = rb.w;
   uint32_t bp = W8_NONE, bslot = W8_NONE, err = 0;
-  uint32_t stk[W8_STACK];
-  float stn[W8_STACK];
+  // node groups (Ylitie et al.): the current group is a first internal child
+  // and a mask of the hit internal children after it (bit ci: child base +
+  // ci), kept in registers; a group goes on the stack only when the
+  // traversal descends while siblings remain, so the stack is no deeper than
+  // the wide tree.  Children are tested against the closest hit when their
+  // parent is opened; the hits do not depend on the visiting order.
+  uint2 stk[W8_GSTACK];
   int sp = 0;
-  stk[sp] = 0; stn[sp] = -__uint_as_float(0x7f800000u); ++sp;
-  while (sp > 0) {
-    --sp;
-    if (!(stn[sp] <= __fmul_rn(best, 1.0009765625f))) continue;
-    const uint32_t w = stk[sp];
+  uint32_t gbase = 0, gmask = 1;  // the root as a group of one
+  for (;;) {
+    if (gmask == 0) {
+      if (sp == 0) break;
+      --sp;
+      gbase = stk[sp].x;
+      gmask = stk[sp].y;
+    }
+    const uint32_t w = gbase + (uint32_t)(__ffs(gmask) - 1);
+    gmask &= gmask - 1;
     if (w >= nwide) { err |= 2u; continue; }
     const uint4* np = reinterpret_cast<const uint4*>(wide + 80 * (size_t)w);
     uint32_t words[20];
@@ -1126,10 +1131,26 @@ __global__ __launch_bounds__(256) void k_trace_w8(
                          __uint_as_float(((words[3] >> 8) & 0xffu) << 23),
                          __uint_as_float(((words[3] >> 16) & 0xffu) << 23)};
     const uint32_t imask = words[3] >> 24, bc = words[4], bt = words[5];
+    // the node's pad, from its frame [p, p + 255 s]
+    float pad[3];
+    {
+      float dist = 0.0f, fr[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        fr[a] = __fadd_rn(p[a], __fmul_rn(255.0f, sc[a]));
+        dist = fmaxf(dist, fmaxf(fabsf(__fsub_rn(p[a], o[a])), fabsf(__fsub_rn(fr[a], o[a]))));
+      }
+      const float dp = __fmul_rn(dist, 1.52587890625e-05f);
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        pad[a] = fmaxf(__fmul_rn(fmaxf(fabsf(p[a]), fabsf(fr[a])), 9.5367431640625e-07f), dp);
+    }
+    uint32_t cmask = 0;  // hit internal children, by internal ordinal
+#pragma unroll
     for (int k = 0; k < 8; ++k) {
       const uint32_t meta = (words[6 + (k >> 2)] >> (8 * (k & 3))) & 0xffu;
       if (meta == 0) continue;
-      float mn[3], mx[3], tn;
+      float mn[3], mx[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         const uint32_t ql = (words[8 + 2 * a + (k >> 2)] >> (8 * (k & 3))) & 0xffu;
@@ -1137,10 +1158,9 @@ __global__ __launch_bounds__(256) void k_trace_w8(
         mn[a] = __fadd_rn(p[a], __fmul_rn((float)ql, sc[a]));
         mx[a] = __fadd_rn(p[a], __fmul_rn((float)qh, sc[a]));
       }
-      if (!w8_slab(o, inv, mn, mx, best, tn)) continue;
+      if (!w8_slab_pad(o, inv, mn, mx, pad, best)) continue;
       if ((imask >> k) & 1u) {
-        if (sp < W8_STACK) { stk[sp] = bc + (meta & 31u) - 24u; stn[sp] = tn; ++sp; }
-        else err |= 1u;
+        cmask |= 1u << (((meta & 31u) - 24u) & 7u);
       } else {
         const uint32_t cnt = __popc(meta >> 5), off = meta & 31u;
         for (uint32_t j = 0; j < cnt; ++j) {
@@ -1154,6 +1174,14 @@ __global__ __launch_bounds__(256) void k_trace_w8(
           }
         }
       }
+    }
+    if (cmask != 0) {  // descend; the siblings left wait on the stack
+      if (gmask != 0) {
+        if (sp < W8_GSTACK) stk[sp++] = make_uint2(gbase, gmask);
+        else err |= 1u;
+      }
+      gbase = bc;
+      gmask = cmask;
     }
   }
   reinterpret_cast<uint4*>(hits)[i] = make_uint4(__float_as_uint(best), bp,
