@@ -1162,15 +1162,23 @@ __global__ __launch_bounds__(256) void k_trace_w8(
       if ((imask >> k) & 1u) {
         cmask |= 1u << (((meta & 31u) - 24u) & 7u);
       } else {
+        // a leaf's (at most 3) objects: every read of a step issued together
         const uint32_t cnt = __popc(meta >> 5), off = meta & 31u;
-        for (uint32_t j = 0; j < cnt; ++j) {
-          const uint32_t slot = tri_index[bt + off + j];
-          const uint32_t* tr = tris + 3 * (size_t)slot;
-          const float t = w8_tri(o, d, pos + (size_t)tr[0] * vs, pos + (size_t)tr[1] * vs,
-                                 pos + (size_t)tr[2] * vs);
-          const uint32_t id = perm[slot];
+        uint32_t sl[3], tv[3][3];
+#pragma unroll
+        for (uint32_t j = 0; j < 3; ++j) sl[j] = j < cnt ? tri_index[bt + off + j] : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < 3; ++j)
+#pragma unroll
+          for (uint32_t q = 0; q < 3; ++q) tv[j][q] = j < cnt ? tris[3 * (size_t)sl[j] + q] : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < 3; ++j) {
+          if (j >= cnt) break;
+          const float t = w8_tri(o, d, pos + (size_t)tv[j][0] * vs, pos + (size_t)tv[j][1] * vs,
+                                 pos + (size_t)tv[j][2] * vs);
+          const uint32_t id = perm[sl[j]];
           if (t < best || (t == best && t != __uint_as_float(0x7f800000u) && id < bp)) {
-            best = t; bp = id; bslot = slot;
+            best = t; bp = id; bslot = sl[j];
           }
         }
       }
