@@ -344,13 +344,15 @@ __global__ void k_root(const MeshRow* __restrict__ rows, const uint32_t* __restr
 // One block per chunk of <= CH objects of one node: 3 x 32 bins of
 // {count, bin box} in LDS, flushed into the node histogram; per-chunk counts
 // are kept for the stable partition.
-// Learned method: pass 1 counts all three axes of a counted node (no bin
-// boxes); pass 2 adds the bin boxes of its predicted axis (window) or of all
-// axes (fallback).  Plain nodes are binned completely in pass 1.
+// Every node is binned completely (counts and bin boxes on all three axes),
+// also a learned-method node: its features read the counts and its window
+// restricts the sweep to cuts of the predicted axis, and binning the boxes of
+// the other axes in the same pass costs less than a second read of the
+// records would.
 __device__ inline void bin_chunk(uint32_t c, const LNode* __restrict__ L,
                                  const uint32_t* __restrict__ cnode,
                                  const uint32_t* __restrict__ ck, const Rec* __restrict__ recs,
-                                 uint32_t* __restrict__ hist, uint32_t* __restrict__ blk, int pass) {
+                                 uint32_t* __restrict__ hist, uint32_t* __restrict__ blk) {
   // BIN_COPIES copies of the row, lane l adding to copy l % BIN_COPIES: lanes of
   // one wavefront that share a bin mostly hit different addresses (LDS
   // serialises same-address atomics).  The odd stride puts the copies of a
@@ -361,9 +363,7 @@ __device__ inline void bin_chunk(uint32_t c, const LNode* __restrict__ L,
   const uint32_t slot = cnode[c];
   const uint32_t k = ck[c];
   const LNode& nd = L[slot];
-  const uint32_t lm = nd.mode, lk = lmode_kind(lm), la = lmode_axis(lm);
-  if (nd.flags & LF_PREBINNED) return;       // block-uniform: binned by the parent's partition
-  if (pass == 2 && lk < LM_WINDOW) return;  // block-uniform: nothing left to bin
+  if (nd.flags & LF_PREBINNED) return;  // block-uniform: binned by the parent's partition
   const bool single = nd.nchunk == 1;
   for (uint32_t t = threadIdx.x; t < BIN_COPIES * COPY_STRIDE; t += blockDim.x) h[t] = 0;
   uint32_t* hl = h + (lane_id() % BIN_COPIES) * COPY_STRIDE;
@@ -391,10 +391,8 @@ __device__ inline void bin_chunk(uint32_t c, const LNode* __restrict__ L,
     for (int j = 0; j < 3; ++j) { e[j] = acc_min_enc(b.mn[j]); e[3 + j] = acc_max_enc(b.mx[j]); }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const bool boxes = pass == 1 ? lk == LM_PLAIN : (lk == LM_FALLBACK || a == (int)la);
-      if (pass == 2 && !boxes) continue;  // block-uniform
       const uint32_t bi = valid ? bin_index(center(b.mn[a], b.mx[a]), cmin[a], cext[a]) : 0u;
-      hist_add(hl, (uint32_t)a, bi, valid, e, pass == 1, boxes);
+      hist_add(hl, (uint32_t)a, bi, valid, e, true, true);
     }
   }
   __syncthreads();
@@ -408,13 +406,11 @@ __device__ inline void bin_chunk(uint32_t c, const LNode* __restrict__ L,
       v = f == 0 ? v + u : max(v, u);
     }
     if (f == 0) {
-      if (pass == 1) {
-        blk[(size_t)c * 96 + t / BIN_WORDS] = v;
-        if (single) gh[t] = v;
-        else if (v) atomicAdd(&gh[t], v);
-      }
+      blk[(size_t)c * 96 + t / BIN_WORDS] = v;
+      if (single) gh[t] = v;
+      else if (v) atomicAdd(&gh[t], v);
     } else if (single) {
-      gh[t] = v;  // pass 2 only adds boxes: the counts written by pass 1 stay
+      gh[t] = v;
     } else if (v) {
       atomicMax(&gh[t], v);
     }
@@ -428,16 +424,15 @@ __global__ __launch_bounds__(256) void k_bin(const LNode* __restrict__ L,
                                              const uint32_t* __restrict__ ck,
                                              const Rec* __restrict__ recs,
                                              uint32_t* __restrict__ hist,
-                                             uint32_t* __restrict__ blk, int pass,
+                                             uint32_t* __restrict__ blk,
                                              const uint32_t* __restrict__ lvc) {
-  // Every level of a recorded graph launches both passes.  No chunk to bin
-  // (the lower levels' nodes were all binned by their parents' partition
-  // blocks) or, for pass 2, no learned-method node means nothing to do.
+  // Every level of a recorded graph launches it.  No chunk to bin (the lower
+  // levels' nodes were all binned by their parents' partition blocks) means
+  // nothing to do.
   if (lvc[LV_NBIN] == 0) return;
-  if (pass == 2 && lvc[LV_NCOUNTED] == 0) return;
   const uint32_t nC = lvc[LV_NC];
   for (uint32_t c = blockIdx.x; c < nC; c += gridDim.x) {
-    bin_chunk(c, L, cnode, ck, recs, hist, blk, pass);
+    bin_chunk(c, L, cnode, ck, recs, hist, blk);
     __syncthreads();
   }
 }
@@ -2345,7 +2340,7 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
     if (nBin) {  // level by level: skipped once every node was binned by its parent
       MG_PROF(KK_BIN, 0);
       k_bin<<<g_bin, 64 * tu.bin_waves, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk,
-                                                  1, lvc);
+                                                  lvc);
       MG_PROF(KK_BIN, 1);
     }
     if (n_counted) {
@@ -2354,10 +2349,6 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
                                ctr, lp)))
         return rc;
       MG_PROF(KK_LEARN, 1);
-      MG_PROF(KK_BIN, 0);
-      k_bin<<<g_bin, 64 * tu.bin_waves, 0, st>>>(L[cur], cn[cur], ck[cur], recs[cur], hist, blk,
-                                                  2, lvc);
-      MG_PROF(KK_BIN, 1);
     }
     MG_PROF(KK_SWEEP, 0);
     k_sweep<<<g_sweep, 1024, 0, st>>>(L[cur], lvc, hist, recs[cur], L[cur ^ 1], cn[cur ^ 1],
