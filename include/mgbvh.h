@@ -553,7 +553,7 @@ typedef struct mgbvh_tuning {
   uint32_t small_blocks_per_cu;
   uint32_t compute_units;  /* read-only (get) */
   uint32_t partition_waves; /* wavefronts per partition block below the first 4 levels
-                              (those always take 4): 1 or 4 (0 = keep) */
+                              (those always take 4): 1 or 4, default 4 (0 = keep) */
   uint32_t bin_waves;       /* wavefronts per binning block: 1..4 (0 = keep) */
   uint32_t sweep_pack_min_nodes; /* levels with at least this many nodes are swept one node
                                     per lane, the others one node per wavefront

@@ -23,13 +23,13 @@ from . import _native as N
 
 GRID_CANDIDATES = [
     dict(bin_blocks_per_cu=128, partition_blocks_per_cu=192, sweep_blocks_per_cu=4,
-         small_blocks_per_cu=64, partition_waves=1, bin_waves=4),
+         small_blocks_per_cu=64, partition_waves=4, bin_waves=4),
     dict(bin_blocks_per_cu=64, partition_blocks_per_cu=128, sweep_blocks_per_cu=4,
-         small_blocks_per_cu=32, partition_waves=1, bin_waves=4),
+         small_blocks_per_cu=32, partition_waves=4, bin_waves=4),
     dict(bin_blocks_per_cu=128, partition_blocks_per_cu=256, sweep_blocks_per_cu=4,
-         small_blocks_per_cu=128, partition_waves=1, bin_waves=4),
+         small_blocks_per_cu=128, partition_waves=4, bin_waves=4),
     dict(bin_blocks_per_cu=32, partition_blocks_per_cu=64, sweep_blocks_per_cu=4,
-         small_blocks_per_cu=16, partition_waves=1, bin_waves=4),
+         small_blocks_per_cu=16, partition_waves=4, bin_waves=4),
 ]
 # nodes of 33..this many objects built one wavefront per subtree (1 = off)
 MEDIUM_CANDIDATES = [512, 1024, 2048]

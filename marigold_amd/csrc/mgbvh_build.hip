@@ -2362,7 +2362,7 @@ int enqueue_build(mgbvh_ctx* c, Scratch& sc, BuildPlan& p, uint32_t fixed_levels
     const uint32_t fuse = out.samples ? 0u : 1u;
     // The first levels hold the largest nodes (whole meshes, their halves and
     // quarters: full 2048-object chunks), which four-wavefront blocks move
-    // faster; the many small nodes below take one-wavefront blocks.
+    // faster; with part_waves 1 the levels below take one-wavefront blocks.
     constexpr uint32_t PART_WIDE_LEVELS = 4;
     if (tu.part_waves == 1 && level >= PART_WIDE_LEVELS)
       k_partition<1><<<g_part, 64, 0, st>>>(L[cur], cn[cur], ck[cur], blk, recs[cur],

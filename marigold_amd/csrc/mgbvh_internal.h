@@ -251,7 +251,11 @@ void scratch_free(Scratch& sc);
 // subtrees better than a fixed share per resident block.
 struct Tuning {
   uint32_t bin_bpc = 128, part_bpc = 192, sweep_bpc = 4, small_bpc = 64;
-  uint32_t part_waves = 1, bin_waves = 4;  // wavefronts per block
+  // wavefronts per block; part_waves 1 gives the partition one-wavefront
+  // blocks below the first four levels (better when the level path went down
+  // to small nodes; with the medium-node kernel taking those, four measured
+  // faster on scenes and single meshes alike)
+  uint32_t part_waves = 4, bin_waves = 4;
   uint32_t sweep_pack = 65536;             // sweep: one node per lane from this many nodes
   // nodes of 33..med_max objects build their whole subtree in one wavefront
   // (k_medium) instead of level by level; 0 = off.  512 measured best for
