@@ -803,16 +803,32 @@ __device__ inline uint32_t w8_exponent(float lo, float hi) {
 // over the group give each slot what the sequential layout gave it (objects
 // of the leaf slots before it, internal slots before it, and the wide nodes
 // and objects of the internal children before it).  The node's words are
-// assembled in LDS and stored as five 16-byte writes.
+// assembled in LDS and stored as five 16-byte writes.  The frontier items of
+// the next level are staged in LDS and reserved in the output list one flush
+// at a time (at least W8E_STAGE - 8 * W8E_NODES + 1 items each, and the
+// block's remainder at its end): same-address atomics serialise device-wide,
+// and the list's order is free (an item carries its own layout offsets).
 constexpr uint32_t W8E_THREADS = 256;           // threads of a layout block
 constexpr uint32_t W8E_NODES = W8E_THREADS / 8;  // wide nodes per block and step
+constexpr uint32_t W8E_STAGE = 512;             // staged frontier items per block
+static_assert(W8E_STAGE >= 8 * W8E_NODES, "a step's items fit after a flush");
+
+// Copies the block's staged items to out[base, base + n) (all threads).
+__device__ inline void w8_flush(const W8Item* stage, uint32_t n, uint32_t base, W8Item* out,
+                                uint32_t front_cap, uint32_t* err_flag) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    if ((uint64_t)base + i < front_cap) out[base + i] = stage[i];
+    else atomicOr(err_flag, 4u);
+  }
+}
 __global__ __launch_bounds__(W8E_THREADS) void k_w8_emit(
     const mgbvh_node32* __restrict__ nodes, const uint32_t* __restrict__ lens,
     const W8E* __restrict__ E, const W8Item* __restrict__ in, const uint32_t* n_in,
     W8Item* __restrict__ out, uint32_t* n_out, uint32_t front_cap, uint8_t* __restrict__ wide,
     uint32_t* __restrict__ tri_index, uint32_t nn, uint32_t wide_cap, uint32_t tri_cap,
     uint32_t* err_flag) {
-  __shared__ uint32_t wtot[W8E_THREADS / 64], bbase;
+  __shared__ uint32_t wtot[W8E_THREADS / 64], bbase, bslot, bflush, bstaged;
+  __shared__ __align__(16) W8Item stage[W8E_STAGE];
   __shared__ __align__(16) uint32_t nw[W8E_NODES][20];  // the node words of each group
   __shared__ uint32_t box_u[W8E_THREADS], box_b[W8E_THREADS];  // hand-over of right parts, per lane
   const uint32_t count = *n_in;
@@ -821,10 +837,10 @@ __global__ __launch_bounds__(W8E_THREADS) void k_w8_emit(
   uint32_t* W = nw[g];
   uint8_t* Wb = reinterpret_cast<uint8_t*>(W);
   box_b[t] = 0;
+  if (t == 0) bstaged = 0;
   w8_wave_sync();
-  // block-uniform trip count: every thread takes part in the frontier
-  // allocation below, one atomic per block and step (same-address atomics
-  // serialise device-wide)
+  // block-uniform trip count: every thread takes part in the staging and the
+  // flushes below
   for (uint32_t b0 = blockIdx.x * W8E_NODES; b0 < count; b0 += gridDim.x * W8E_NODES) {
     const uint32_t it = b0 + g;
     bool ok = it < count;  // group-uniform
@@ -944,11 +960,23 @@ __global__ __launch_bounds__(W8E_THREADS) void k_w8_emit(
         wtot[w] = tot;
         tot += x;
       }
-      bbase = tot ? atomicAdd(n_out, tot) : 0u;
+      uint32_t s0 = bstaged, fl = 0;
+      if (s0 + tot > W8E_STAGE) {  // flush the staged items first
+        fl = s0;
+        bbase = atomicAdd(n_out, s0);
+        s0 = 0;
+      }
+      bflush = fl;
+      bslot = s0;
+      bstaged = s0 + tot;
     }
     __syncthreads();
-    const uint32_t gslot = bbase + wtot[wave] + __shfl(incl - lead, (int)g0, 64);
-    __syncthreads();  // wtot and bbase are rewritten by the next step
+    if (bflush) {  // block-uniform
+      w8_flush(stage, bflush, bbase, out, front_cap, err_flag);
+      __syncthreads();  // the stage is refilled below
+    }
+    const uint32_t gslot = bslot + wtot[wave] + __shfl(incl - lead, (int)g0, 64);
+    __syncthreads();  // wtot, bslot and bflush are rewritten by the next step
     if (!ok) continue;
     // the node's frame (every lane of the group: the same values)
     const float pmn[3] = {__uint_as_float(pa.x), __uint_as_float(pa.y), __uint_as_float(pa.z)};
@@ -985,18 +1013,30 @@ __global__ __launch_bounds__(W8E_THREADS) void k_w8_emit(
     if (leaf) {
       Wb[24 + sidx] = (uint8_t)((((1u << p) - 1u) << 5) | toff);
       // the leaf's objects, child before child + 1: p <= 3 objects lie in at
-      // most 3 binary leaves, at most two levels below the slot
+      // most 3 binary leaves, at most two levels below the slot.  A node's
+      // start index and child are read together with its object count; a slot
+      // that is a binary leaf has its node in registers and its count in p
+      // (a leaf slot holds p <= p_max objects, so p is exact)
       uint32_t to = toff;
       const uint32_t c = nb.w == 0 ? 0u : item.bbase + nb.w;
       for (uint32_t i = 0; i < (c == 0 ? 1u : 2u); ++i) {
         const uint32_t x = c == 0 ? u : c + i;
-        const uint32_t cx = c == 0 ? 0u : w8_child(nodes, item.bbase, x);
+        uint32_t xs = na.w, xc = 0u, xl = p;
+        if (c != 0) {
+          xs = nodes[x].start_index;
+          xc = nodes[x].child_node;
+          xl = lens ? lens[x] : 1u;
+        }
+        const uint32_t cx = xc == 0 ? 0u : item.bbase + xc;
         if (cx != 0 && cx + 1 >= nn) { atomicOr(err_flag, 4u); continue; }
         for (uint32_t j = 0; j < (cx == 0 ? 1u : 2u); ++j) {
-          const uint32_t y = cx == 0 ? x : cx + j;
-          if (cx != 0 && w8_child(nodes, item.bbase, y) != 0) { atomicOr(err_flag, 4u); continue; }
-          const uint32_t len = lens ? lens[y] : 1u;
-          const uint32_t ys = nodes[y].start_index;
+          uint32_t ys = xs, len = xl;
+          if (cx != 0) {
+            const uint32_t y = cx + j;
+            ys = nodes[y].start_index;
+            len = lens ? lens[y] : 1u;
+            if (nodes[y].child_node != 0) { atomicOr(err_flag, 4u); continue; }
+          }
           for (uint32_t q = 0; q < len; ++q) {
             const uint64_t at = (uint64_t)item.tbase + item.tri + to++;
             if (at < tri_cap) tri_index[at] = ys + q;
@@ -1007,11 +1047,9 @@ __global__ __launch_bounds__(W8E_THREADS) void k_w8_emit(
     } else if (inner) {
       Wb[24 + sidx] = (uint8_t)((1u << 5) | (24 + ci));
       ibit = 1u << sidx;
-      const uint32_t slot = gslot + ci;
-      if (slot < front_cap)
-        out[slot] = W8Item{u, item.region + ci, item.region + nint + (sw - wi),
-                           item.tri + own + (st_ - ti), item.wbase, item.tbase, item.bbase, 0u};
-      else atomicOr(err_flag, 4u);
+      stage[gslot + ci] = W8Item{u, item.region + ci, item.region + nint + (sw - wi),
+                                 item.tri + own + (st_ - ti), item.wbase, item.tbase,
+                                 item.bbase, 0u};
     }
 #pragma unroll
     for (uint32_t d = 1; d < 8; d <<= 1) ibit |= __shfl_xor(ibit, (int)d, 8);
@@ -1030,6 +1068,14 @@ __global__ __launch_bounds__(W8E_THREADS) void k_w8_emit(
     }
     w8_wave_sync();  // W is rewritten by the group's next node
   }
+  __syncthreads();  // the last step's staged items
+  if (t == 0) {
+    const uint32_t s0 = bstaged;
+    bflush = s0;
+    bbase = s0 ? atomicAdd(n_out, s0) : 0u;
+  }
+  __syncthreads();
+  w8_flush(stage, bflush, bbase, out, front_cap, err_flag);
 }
 
 // ------------------------------------------------------------------ traversal
