@@ -212,13 +212,18 @@ def cpu_baseline_scene(meshes, budget_s: float = 15.0, threads: int = 0):
             "host": host_cores()}
 
 
-def single_mesh(ctx, K: int = 10):
+def single_mesh(ctx, tuning: dict, K: int = 10):
     """The 1024x512-cell heightfield (1,048,576 triangles) built alone with the
-    plain method: level by level (the host reads each level's counts) and as a
-    one-mesh scene plan (one HIP graph, no instances), with both trees compared."""
+    plain method and the given tuning (the library's defaults: the scene's
+    autotuned settings are chosen for 2,000 meshes at once): level by level
+    (the host reads each level's counts) and as a one-mesh scene plan (one HIP
+    graph, no instances), with both trees compared.  The context's tuning is
+    restored afterwards."""
     from marigold_amd import _native as N
     from marigold_amd import scenes
     from marigold_amd.scene import GpuScene
+    saved = ctx.get_tuning()
+    ctx.set_tuning(**tuning)
     pos, tris = scenes.heightfield(1024, 512, seed=2)
     n, nv = tris.shape[0], pos.shape[0]
     d_pos, d_tris = ctx.to_device(pos), ctx.to_device(tris)
@@ -250,10 +255,11 @@ def single_mesh(ctx, K: int = 10):
     g.free()
     for b in (d_pos, d_tris, dn, dl, dp, dt):
         b.free()
+    ctx.set_tuning(**saved)
     dev_ms = float(np.median(dev))
     model = survey_bytes(n, nv, terms["s_large"], terms["s_small"])
     return {"workload": "1024x512-cell heightfield, 1,048,576 tris, plain method",
-            "graph_device_ms": dev_ms, "graph_wall_ms": wall_ms, "level_by_level_wall_ms": lbl_ms,
+            "tuning": tuning, "graph_device_ms": dev_ms, "graph_wall_ms": wall_ms, "level_by_level_wall_ms": lbl_ms,
             "mtri_per_s_graph": n / (dev_ms * 1e-3) / 1e6,
             "byte_model_gb": model / 1e9,
             "byte_model_frac_of_hbm": model / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -344,6 +350,7 @@ def scene_main(args):
     log(f"scene: {len(meshes)} meshes, {n} tris, {nv} verts, {len(sc['instance_mesh'])} instances "
         f"(generated in {time.perf_counter() - t0:.1f} s)")
     ctx = N.Context(0)
+    lib_tuning = ctx.get_tuning()  # the library's defaults, before any tuning
     model, train_info = train_split_model(ctx, epochs=(4, 2) if args.quick_train else (30, 15))
     ctx.set_model(model)
     log(f"split proposer trained on the GPU: {train_info}")
@@ -500,7 +507,7 @@ def scene_main(args):
         log("16M-ray hit comparison on the heightfield")
         full = heightfield_16m_rays(ctx, learned)
     log("single mesh: the 1M-triangle heightfield alone, level by level and as a graph")
-    single = single_mesh(ctx)
+    single = single_mesh(ctx, lib_tuning)
     log(f"single mesh: graph {single['graph_device_ms']:.3f} ms device, "
         f"{single['graph_wall_ms']:.3f} ms wall; level by level {single['level_by_level_wall_ms']:.3f} ms")
     if args.calibrate_copy:
